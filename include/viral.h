@@ -1,0 +1,161 @@
+/*
+ * viral.h -- C ABI of the MI355X-native viscous hydrodynamics step.
+ *
+ * The reference (rjfries/VIRAL) has no plugin API; its step is the call
+ *   TVDRK2 / TVDRK3 / FirstOrder(GRID HydroGrid, double tau, double ts)
+ * (NEWEST10/time.h:300,338,397, called from NEWEST10/main.cpp:144-152), which
+ * reads and writes a global array-of-structs grid plus the globals il..kr, k0
+ * (NEWEST10/init.h:7,16) and compile-time profile macros
+ * (NEWEST10/maindef.h:36-339).  This header replaces that seam: the grid
+ * becomes a context owning structure-of-arrays device buffers, the profile
+ * macros become fields of viral_desc, and every entry point returns a status
+ * instead of calling exit().
+ *
+ * Cell layout (all host/device buffers): ghosted grid of
+ *   Gx = nx + 2*B, Gy = ny + 2*B, Gz = (lbi ? 1 : nz + 2*B), B = VIRAL_BORDER,
+ * linear index ((i*Gy + j)*Gz + k) -- the reference's HydroGrid[i][j][k]
+ * order (NEWEST10/maindef.h:463), eta fastest.  Interior cells are
+ * i in [B, B+nx), j in [B, B+ny), k in [B, B+nz) (k = 0 when lbi), matching
+ * il..ir, jl..jr, kl..kr of NEWEST10/init.h:570-581.
+ *
+ * All quantities are in the reference's units: fm, 1/fm, 1/fm^4.
+ */
+#ifndef VIRAL_H
+#define VIRAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VIRAL_BORDER 3
+
+/* coordinates (NEWEST10 = Milne, TXYZ5 = Cartesian) */
+enum { VIRAL_MILNE = 0, VIRAL_CARTESIAN = 1 };
+/* flux scheme: -DKT (hydroKT.h) or -DSHAS (hydroshasta.h, implies CON) */
+enum { VIRAL_KT = 0, VIRAL_SHASTA = 1 };
+/* equation of state: conformal profile inlines or s95p.h */
+enum { VIRAL_EOS_CONFORMAL = 0, VIRAL_EOS_S95P = 1 };
+/* shear viscosity law, NEWEST10/maindef.h:76,134,282 (law 1) and :187,229,328 (law 0) */
+enum { VIRAL_ETA_TIMES_S = 0,      /* eta = c * s              */
+       VIRAL_ETA_S_OVER_4PI = 1 }; /* eta = c * (s / (4 pi))   */
+/* relaxation time law, NEWEST10/maindef.h:329 (0), :77,135,283 (1), :188,230 (2) */
+enum { VIRAL_TPI_OVER_EP = 0,      /* tau_pi = (c*eta)/(e+p)   */
+       VIRAL_TPI_OVER_P = 1,       /* tau_pi = c*(eta/p)       */
+       VIRAL_TPI_CONST = 2 };      /* tau_pi = c               */
+
+/* Reference behaviours kept as switchable modes (all on = "as reference"). */
+#define VIRAL_Q_GHOST_EIGEN   (1u << 0) /* q1: eigenvalues only in the interior; first face of each row gets zero
+                                           dissipation (NEWEST10/hydroKT.h:182-184,284,453-455) */
+#define VIRAL_Q_K0_BROADCAST  (1u << 1) /* q2: copy v of slice k0 to all eta slices after each root solve
+                                           (NEWEST10/root.h:426-449) */
+#define VIRAL_Q_DPDE_OF_P     (1u << 2) /* q3: a = DPDE(P) in the CON eigenvalues (NEWEST10/maindef.h:490) */
+#define VIRAL_Q_GUBSER_MIRROR (1u << 3) /* q7: mirror u^1,u^2 into x/y ghosts (NEWEST10/boundary.h:32-37,77-81,
+                                           118-122,160-165); off by default: only the GUBSER profile has it */
+#define VIRAL_Q_DEFAULT (VIRAL_Q_GHOST_EIGEN | VIRAL_Q_K0_BROADCAST | VIRAL_Q_DPDE_OF_P)
+
+/* State fields, each one ghosted array of doubles.  Names follow the
+ * reference's cell members (NEWEST10/maindef.h:399-405). */
+enum {
+  VIRAL_F_T00 = 0, VIRAL_F_T10, VIRAL_F_T20, VIRAL_F_T30,       /* T^{tau mu}                     */
+  VIRAL_F_PI0,                                                  /* pi[0..9] = pi^{tt,tx,ty,te,xx,xy,xe,yy,ye,ee}
+                                                                   (NEWEST10/time.h:194)          */
+  VIRAL_F_BULK = VIRAL_F_PI0 + 10,                              /* PI (bulk)                      */
+  VIRAL_F_EN, VIRAL_F_P, VIRAL_F_TEMP,                          /* En, P, Temp                    */
+  VIRAL_F_VX, VIRAL_F_VY, VIRAL_F_VE,                           /* Vx, Vy, Ve                     */
+  VIRAL_F_U0, VIRAL_F_U1, VIRAL_F_U2, VIRAL_F_U3,               /* u[4]                           */
+  VIRAL_F_PU0, VIRAL_F_PU1, VIRAL_F_PU2, VIRAL_F_PU3,           /* prevu[4]                       */
+  VIRAL_NFIELDS                                                 /* = 29                           */
+};
+
+/* per-cell flag bits written by every root solve */
+#define VIRAL_FLAG_ACAUSAL    (1u << 0) /* v^2 >= 1            (NEWEST10/root.h:396)           */
+#define VIRAL_FLAG_ENEG       (1u << 1) /* e <= 0              (NEWEST10/root.h:396)           */
+#define VIRAL_FLAG_SOLVER     (1u << 2) /* f' == 0 or non-finite f, f' in the Newton step
+                                           (the GSL errors that abort the reference)           */
+#define VIRAL_FLAG_MAXITER    (1u << 3) /* Newton hit max_iter without two equal iterates      */
+#define VIRAL_FLAG_FREEZEOUT  (1u << 4) /* Temp < t_freeze                                     */
+
+/* status codes */
+#define VIRAL_OK            0
+#define VIRAL_ERR_ARG      -1
+#define VIRAL_ERR_HIP      -2
+#define VIRAL_ERR_PHYSICS  -3 /* a cell raised ACAUSAL, ENEG or SOLVER; see viral_first_bad_cell */
+#define VIRAL_ERR_UNSUPPORTED -4
+
+typedef struct viral_desc {
+  int nx, ny, nz;           /* interior cells (XCMA, YCMA, ZCMA)                          */
+  int lbi;                  /* 1: boost-invariant build (-DLBI): nz == 1, no eta ghosts   */
+  int coords;               /* VIRAL_MILNE / VIRAL_CARTESIAN                              */
+  int scheme;               /* VIRAL_KT / VIRAL_SHASTA                                    */
+  int rk;                   /* 1, 2, 3 (FirstOrder, TVDRK2, TVDRK3)                        */
+  int eos;                  /* VIRAL_EOS_*                                                */
+  double dx, dy, deta;      /* XS, YS, ZS                                                 */
+  double dof;               /* DOF of the conformal EOS (42.25, NEWEST10/maindef.h:9)     */
+  int eta_law;  double eta_coef;
+  int tpi_law;  double tpi_coef;
+  unsigned quirks;          /* VIRAL_Q_* bits                                             */
+  int k0;                   /* ghosted eta index broadcast by q2; < 0: (Gz-1)/2 as the
+                               reference (NEWEST10/init.h:641-645)                        */
+  int max_iter;             /* Newton iteration cap (5000, NEWEST10/root.h:346)           */
+  double t_freeze;          /* freeze-out temperature in 1/fm for VIRAL_FLAG_FREEZEOUT    */
+  int fast_math;            /* 0: arithmetic in the reference's operation order, bitwise
+                               equal to the CPU restatement; 1: reciprocal-multiply/FMA
+                               forms, within 1e-10 relative per step                     */
+} viral_desc;
+
+/* Fill *d with the reference's defaults for an nx*ny*nz Milne KT RK2 grid. */
+void viral_desc_default(viral_desc* d, int nx, int ny, int nz);
+
+/* Ghosted sizes of a descriptor. */
+size_t viral_cells(const viral_desc* d);
+void viral_ghost_dims(const viral_desc* d, int* gx, int* gy, int* gz);
+
+typedef struct viral_ctx viral_ctx;
+
+/* Create a context on the current HIP device holding nev events described by
+ * descs[0..nev-1] (events may have different grid sizes; all other fields
+ * must agree).  Replaces init()'s allocation (NEWEST10/init.h:548-552,
+ * NEWEST10/alloc.h:33).  Returns VIRAL_OK or an error. */
+int viral_create(const viral_desc* descs, int nev, viral_ctx** out);
+void viral_destroy(viral_ctx* ctx);
+
+/* Copy the VIRAL_NFIELDS state arrays of event ev between host and device.
+ * fields[f] points to viral_cells(desc_ev) doubles (ghosted, reference order).
+ * Replaces direct HydroGrid access by init (NEWEST10/init.h:653-686) and
+ * output (NEWEST10/write.h:87-103). */
+int viral_upload_state(viral_ctx* ctx, int ev, const double* const* fields);
+int viral_download_state(viral_ctx* ctx, int ev, double* const* fields);
+/* One field only (f in [0, VIRAL_NFIELDS)). */
+int viral_download_field(viral_ctx* ctx, int ev, int f, double* out);
+
+/* Advance every event by one full RK step from tau to tau+dtau, including the
+ * ghost fill after each stage.  Replaces TVDRK2/TVDRK3/FirstOrder
+ * (NEWEST10/time.h:300-478) and the halo + boundary calls inside them
+ * (NEWEST10/pack.h:689, NEWEST10/boundary.h:266).  Returns VIRAL_OK, or
+ * VIRAL_ERR_PHYSICS where the reference would have exited
+ * (NEWEST10/root.h:396-421). */
+int viral_step(viral_ctx* ctx, double tau, double dtau);
+
+/* Per-cell flags (uint8, ghosted layout, OR over the stages of the last step)
+ * and Newton iteration counts of stage s (0-based) of the last step (int32). */
+int viral_flags(viral_ctx* ctx, int ev, uint8_t* out);
+int viral_iterations(viral_ctx* ctx, int ev, int stage, int32_t* out);
+/* First interior cell, in the reference's i,j,k loop order, that raised
+ * ACAUSAL/ENEG/SOLVER in the last step's first failing stage; -1 if none.
+ * Writes the ghosted (i,j,k) and the flags of that cell. */
+int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* stage, unsigned* flags);
+
+/* Device reductions the reference's driver runs every step.
+ * MaxTempGev (NEWEST10/utils.h:263-283): max over interior of GEVFM*T(e). */
+int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out);
+
+/* Error text of the last failing call on ctx (never NULL). */
+const char* viral_last_error(viral_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VIRAL_H */

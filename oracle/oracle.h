@@ -1,0 +1,70 @@
+/*
+ * oracle.h -- CPU restatement of the VIRAL viscous hydro step.
+ *
+ * TEST INFRASTRUCTURE.  This is the parity oracle: only tests/, the smoke
+ * check and bench.py's CPU-baseline leg may load it, and only as the checker
+ * or the timed CPU baseline -- never as the product path.
+ *
+ * It restates NEWEST10 (Milne) / TXYZ5 (Cartesian) one pass at a time, in the
+ * reference's loop structure and operation order, on the structure-of-arrays
+ * ghosted layout of include/viral.h.  Each function cites the reference lines
+ * it follows in oracle.c.
+ */
+#ifndef VIRAL_ORACLE_H
+#define VIRAL_ORACLE_H
+#include <stdint.h>
+#include "../include/viral.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_ctx or_ctx;
+
+or_ctx* or_create(const viral_desc* d);
+void or_destroy(or_ctx* c);
+/* state field f (ghosted, reference order) */
+double* or_field(or_ctx* c, int f);
+size_t or_cells(const or_ctx* c);
+
+/* one full RK step (NEWEST10/time.h:300-478); returns VIRAL_OK or VIRAL_ERR_PHYSICS */
+int or_step(or_ctx* c, double tau, double dtau);
+const uint8_t* or_flags(const or_ctx* c);
+const int32_t* or_iters(const or_ctx* c, int stage);
+int or_first_bad(const or_ctx* c, int* i, int* j, int* k, int* stage, unsigned* flags);
+
+/* unit-level entry points (used to pin the oracle against the reference) */
+void or_calc_source(or_ctx* c, double tau, double ts, double* source /* 15*cells, field-major */);
+void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* result /* 15*cells */);
+/* root solve + velocities (+ k0 broadcast) on the interior at tau; per-cell
+ * iteration counts into iters (may be NULL); returns VIRAL_OK/VIRAL_ERR_PHYSICS */
+int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags);
+void or_boundary(or_ctx* c);
+double or_max_temp_gev(or_ctx* c);
+
+/* EOS and transport, scalar (for tests) */
+double or_eos_p(const viral_desc* d, double e);
+double or_eos_dpde(const viral_desc* d, double e);
+double or_eos_T(const viral_desc* d, double e);
+double or_eos_s(const viral_desc* d, double e, double p, double T);
+
+/* Single-cell Newton solve (NEWEST10/root.h:301-364): returns the root and
+ * writes the iteration count and solver flag bits. */
+double or_newton_cell(const viral_desc* d, double tau, double T00, double T10, double T20, double T30,
+                      double pi0, double pi1, double pi2, double pi3, double PI, double guess,
+                      int32_t* iters, unsigned* flags);
+
+/* WENO kernels of NEWEST10/utils.h:57-160 */
+double or_weno_L(double qm2, double qm1, double qc, double qp1, double qp2);
+double or_weno_R(double qm2, double qm1, double qc, double qp1, double qp2);
+double or_weno_der(double qm2, double qm1, double qc, double qp1, double qp2, double dx);
+
+/* deterministic math (oracle/vmath.h) exported for tests */
+double or_vm_exp(double x);
+double or_vm_log(double x);
+double or_vm_pow(double x, double y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

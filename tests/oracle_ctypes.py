@@ -1,0 +1,134 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: used only by tests, the smoke check and bench.py's
+CPU-baseline leg, as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from viral_amd.desc import ViralDesc, NFIELDS
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: %s (run __graft_entry__.build())" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.or_create.restype = vp
+        L.or_create.argtypes = [C.POINTER(ViralDesc)]
+        L.or_destroy.argtypes = [vp]
+        L.or_field.restype = dp
+        L.or_field.argtypes = [vp, C.c_int]
+        L.or_cells.restype = C.c_size_t
+        L.or_cells.argtypes = [vp]
+        L.or_step.restype = C.c_int
+        L.or_step.argtypes = [vp, C.c_double, C.c_double]
+        L.or_flags.restype = C.POINTER(C.c_uint8)
+        L.or_flags.argtypes = [vp]
+        L.or_iters.restype = ip
+        L.or_iters.argtypes = [vp, C.c_int]
+        L.or_first_bad.restype = C.c_int
+        L.or_first_bad.argtypes = [vp] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_uint)]
+        L.or_calc_source.argtypes = [vp, C.c_double, C.c_double, dp]
+        L.or_hydro_explicit.argtypes = [vp, C.c_double, C.c_double, dp]
+        L.or_root.restype = C.c_int
+        L.or_root.argtypes = [vp, C.c_double, ip, C.POINTER(C.c_uint8)]
+        L.or_boundary.argtypes = [vp]
+        L.or_max_temp_gev.restype = C.c_double
+        L.or_max_temp_gev.argtypes = [vp]
+        for name in ("or_eos_p", "or_eos_dpde", "or_eos_T"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.POINTER(ViralDesc), C.c_double]
+        L.or_eos_s.restype = C.c_double
+        L.or_eos_s.argtypes = [C.POINTER(ViralDesc), C.c_double, C.c_double, C.c_double]
+        L.or_newton_cell.restype = C.c_double
+        L.or_newton_cell.argtypes = [C.POINTER(ViralDesc)] + [C.c_double] * 11 + [ip, C.POINTER(C.c_uint)]
+        for name in ("or_weno_L", "or_weno_R"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double] * 5
+        L.or_weno_der.restype = C.c_double
+        L.or_weno_der.argtypes = [C.c_double] * 6
+        for name in ("or_vm_exp", "or_vm_log"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double]
+        L.or_vm_pow.restype = C.c_double
+        L.or_vm_pow.argtypes = [C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """One event on the CPU oracle; fields are numpy views of its buffers."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self.L = lib()
+        self.h = self.L.or_create(C.byref(desc))
+        self.n = self.L.or_cells(self.h)
+        self.dims = desc.ghost_dims()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.or_destroy(self.h)
+            self.h = None
+
+    def field(self, f):
+        p = self.L.or_field(self.h, f)
+        return np.ctypeslib.as_array(p, shape=(self.n,))
+
+    def set_state(self, state):
+        """state: array (NFIELDS, n)"""
+        for f in range(NFIELDS):
+            self.field(f)[:] = state[f]
+
+    def state(self):
+        return np.stack([self.field(f).copy() for f in range(NFIELDS)])
+
+    def step(self, tau, dtau):
+        return self.L.or_step(self.h, tau, dtau)
+
+    def flags(self):
+        return np.ctypeslib.as_array(self.L.or_flags(self.h), shape=(self.n,)).copy()
+
+    def iters(self, stage):
+        return np.ctypeslib.as_array(self.L.or_iters(self.h, stage), shape=(self.n,)).copy()
+
+    def first_bad(self):
+        i, j, k, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        fl = C.c_uint()
+        r = self.L.or_first_bad(self.h, C.byref(i), C.byref(j), C.byref(k), C.byref(s), C.byref(fl))
+        if r < 0:
+            return None
+        return (i.value, j.value, k.value, s.value, fl.value)
+
+    def calc_source(self, tau, ts):
+        out = np.zeros(15 * self.n)
+        self.L.or_calc_source(self.h, tau, ts, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out.reshape(15, self.n)
+
+    def hydro_explicit(self, tau, taustep):
+        out = np.zeros(15 * self.n)
+        self.L.or_hydro_explicit(self.h, tau, taustep, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out.reshape(15, self.n)
+
+    def root(self, tau):
+        it = np.zeros(self.n, dtype=np.int32)
+        fl = np.zeros(self.n, dtype=np.uint8)
+        bad = self.L.or_root(self.h, tau, it.ctypes.data_as(C.POINTER(C.c_int32)),
+                             fl.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return bad, it, fl
+
+    def boundary(self):
+        self.L.or_boundary(self.h)
+
+    def max_temp_gev(self):
+        return self.L.or_max_temp_gev(self.h)

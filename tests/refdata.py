@@ -1,0 +1,58 @@
+"""Runs oracle/_ref harness binaries (reference code compiled in place) and
+parses their output.  Test infrastructure; the binaries exist only where the
+reference tree was present at build time."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from viral_amd.desc import NFIELDS, make_desc, ETA_S_OVER_4PI, TPI_OVER_P, EOS_S95P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+# compile-time profiles of the harness binaries, as runtime descriptors
+# (NEWEST10/maindef.h:36-87 default, :236-294 GINIT, :298-339 GUBSER)
+PROFILES = {
+    "ref_kt": dict(nx=40, ny=40, nz=101, dx=0.1, dy=0.1, deta=0.1,
+                   eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5),
+    "ref_kt_ginit": dict(nx=30, ny=30, nz=101, dx=0.1, dy=0.1, deta=0.1, eos=EOS_S95P,
+                         eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5),
+    "ref_kt_gubser": dict(nx=100, ny=100, nz=1, lbi=1, dx=0.05, dy=0.05, deta=0.1),
+}
+
+
+def available(name):
+    return os.path.exists(os.path.join(REF_DIR, name))
+
+
+def run(name, seed, tau, ts, mode=0):
+    exe = os.path.join(REF_DIR, name)
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "out.bin")
+        subprocess.run([exe, out, str(seed), repr(tau), repr(ts), str(mode)], check=True,
+                       stdout=subprocess.DEVNULL)
+        raw = open(out, "rb").read()
+    if mode == 1:
+        hdr = np.frombuffer(raw[:8], dtype=np.int32)
+        return np.frombuffer(raw[8:], dtype=np.float64).reshape(hdr[1], 5)
+    hdr = np.frombuffer(raw[:32], dtype=np.int32)
+    _, gx, gy, gz, border, nz, one_rank, lbi = [int(v) for v in hdr]
+    th = np.frombuffer(raw[32:48], dtype=np.float64)
+    n = gx * gy * gz
+    arr = np.frombuffer(raw[48:], dtype=np.float64)
+    blocks = arr.reshape(-1, n)
+    assert blocks.shape[0] == NFIELDS + 15 + 15 + NFIELDS
+    desc = make_desc(**PROFILES[name])
+    if lbi and gz != 1:
+        # maindef.h tests LBI (line 22) before the GUBSER block defines it
+        # (line 299), so that profile allocates ZCM = 7 eta slots while
+        # init.h (kl = 0, kr = 1) and the LBI-guarded step code use slot 0 only.
+        blocks = blocks.reshape(-1, gx, gy, gz)[:, :, :, :1].reshape(-1, gx * gy)
+        gz = 1
+    assert desc.ghost_dims() == (gx, gy, gz), (desc.ghost_dims(), (gx, gy, gz))
+    return dict(desc=desc, tau=th[0], ts=th[1], state=blocks[:NFIELDS].copy(),
+                source=blocks[NFIELDS:NFIELDS + 15].copy(),
+                result=blocks[NFIELDS + 15:NFIELDS + 30].copy(),
+                after_boundary=blocks[NFIELDS + 30:].copy(), one_rank=bool(one_rank))
