@@ -609,6 +609,9 @@ static inline void feps(const viral_desc* d, double eps, double Ttt, double pi1,
   *dfv = 1 - ((M2 / (den * den)) * or_eos_dpde(d, eps));
 }
 
+static int newton_shortcut = 1;
+static long newton_cycles[5];
+
 /* GSL newton semantics (roots/newton.c): x_{n+1} = x_n - f/f', stop when
  * x_{n+1} == x_n (gsl_root_test_delta with epsabs 0, epsrel 1e-30, as called
  * at NEWEST10/root.h:354), error when f' == 0 or f, f' are not finite. */
@@ -622,6 +625,7 @@ double or_newton_cell(const viral_desc* d, double tau, double T00, double T10, d
   feps(d, x, T00, pi0, PI, M2, &fv, &dfv);
   int it = 0;
   unsigned fl = 0;
+  double h1 = x, h2 = x, h3 = x, h4 = x; /* x_{it-1} .. x_{it-4} */
   for (;;) {
     it++;
     if (dfv == 0.0) { fl |= VIRAL_FLAG_SOLVER; break; }
@@ -632,11 +636,31 @@ double or_newton_cell(const viral_desc* d, double tau, double T00, double T10, d
     if (!isfinite(fv) || !isfinite(dfv)) { fl |= VIRAL_FLAG_SOLVER; break; }
     if (x == xp) break;
     if (it >= d->max_iter) { fl |= VIRAL_FLAG_MAXITER; break; }
+    h4 = h3; h3 = h2; h2 = h1; h1 = xp;
+    if (newton_shortcut) {
+      /* The Newton map depends on x alone, so x_it == x_{it-p} means the
+       * iterates repeat with period p until max_iter: jump to x_{max_iter}
+       * = x_{it-p+r}, r = (max_iter - it) mod p.  Same result and count as
+       * running the loop out (pinned by tests/test_oracle_root.py). */
+      int per = (it >= 2 && x == h2) ? 2 : (it >= 3 && x == h3) ? 3 : (it >= 4 && x == h4) ? 4 : 0;
+      if (per) {
+        int r = (d->max_iter - it) % per;
+        const double hist[5] = {x, h1, h2, h3, h4};
+        x = hist[per - r];
+        it = d->max_iter;
+        fl |= VIRAL_FLAG_MAXITER;
+        newton_cycles[per]++;
+        break;
+      }
+    }
   }
   *iters = it;
   *flags = fl;
   return x;
 }
+
+void or_set_newton_shortcut(int on) { newton_shortcut = on; }
+long or_newton_cycle_count(int period) { return (period >= 0 && period < 5) ? newton_cycles[period] : -1; }
 
 /* NEWEST10/root.h:301-450 on the interior */
 int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)

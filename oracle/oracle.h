@@ -54,6 +54,11 @@ double or_newton_cell(const viral_desc* d, double tau, double T00, double T10, d
                       double pi0, double pi1, double pi2, double pi3, double PI, double guess,
                       int32_t* iters, unsigned* flags);
 
+/* 1 (default): detect period-2..4 orbits of the Newton iteration and jump to
+ * the iterate the loop would end on; 0: run the loop to max_iter. */
+void or_set_newton_shortcut(int on);
+long or_newton_cycle_count(int period);
+
 /* WENO kernels of NEWEST10/utils.h:57-160 */
 double or_weno_L(double qm2, double qm1, double qc, double qp1, double qp2);
 double or_weno_R(double qm2, double qm1, double qc, double qp1, double qp2);

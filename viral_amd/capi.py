@@ -1,0 +1,162 @@
+"""ctypes host layer over libviral.so (include/viral.h).
+
+This is the Python side of the drop-in boundary: it owns no physics.  The
+shared library is built in-tree by ``__graft_entry__.build()``; if it is
+missing, loading raises -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .desc import ViralDesc, NFIELDS, OK, ERR_PHYSICS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libviral.so")
+
+# every entry point declared in include/viral.h
+EXPORTS = [
+    "viral_desc_default", "viral_cells", "viral_ghost_dims", "viral_create", "viral_destroy",
+    "viral_upload_state", "viral_download_state", "viral_download_field", "viral_step",
+    "viral_flags", "viral_iterations", "viral_first_bad_cell", "viral_max_temp_gev",
+    "viral_last_error",
+]
+
+_lib = None
+
+
+class ViralError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("viral error %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libviral.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libviral.so not built at %s; run __graft_entry__.build()" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    pdesc = C.POINTER(ViralDesc)
+    L.viral_desc_default.argtypes = [pdesc, C.c_int, C.c_int, C.c_int]
+    L.viral_cells.restype = C.c_size_t
+    L.viral_cells.argtypes = [pdesc]
+    L.viral_ghost_dims.argtypes = [pdesc] + [C.POINTER(C.c_int)] * 3
+    L.viral_create.restype = C.c_int
+    L.viral_create.argtypes = [pdesc, C.c_int, C.POINTER(vp)]
+    L.viral_destroy.argtypes = [vp]
+    L.viral_upload_state.restype = C.c_int
+    L.viral_upload_state.argtypes = [vp, C.c_int, C.POINTER(dp)]
+    L.viral_download_state.restype = C.c_int
+    L.viral_download_state.argtypes = [vp, C.c_int, C.POINTER(dp)]
+    L.viral_download_field.restype = C.c_int
+    L.viral_download_field.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.viral_step.restype = C.c_int
+    L.viral_step.argtypes = [vp, C.c_double, C.c_double]
+    L.viral_flags.restype = C.c_int
+    L.viral_flags.argtypes = [vp, C.c_int, C.POINTER(C.c_uint8)]
+    L.viral_iterations.restype = C.c_int
+    L.viral_iterations.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32)]
+    L.viral_first_bad_cell.restype = C.c_int
+    L.viral_first_bad_cell.argtypes = [vp, C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_uint)]
+    L.viral_max_temp_gev.restype = C.c_int
+    L.viral_max_temp_gev.argtypes = [vp, C.c_int, dp]
+    L.viral_last_error.restype = C.c_char_p
+    L.viral_last_error.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _ptr_array(arrays):
+    arr = (C.POINTER(C.c_double) * len(arrays))()
+    for q, a in enumerate(arrays):
+        arr[q] = a.ctypes.data_as(C.POINTER(C.c_double))
+    return arr
+
+
+class Hydro:
+    """A device context holding one or more events (the reference's GRID).
+
+    Mirrors the reference's step seam: ``step(tau, ts)`` is
+    TVDRK2/TVDRK3/FirstOrder(HydroGrid, tau, ts) of NEWEST10/time.h."""
+
+    def __init__(self, descs):
+        if isinstance(descs, ViralDesc):
+            descs = [descs]
+        self.L = load()
+        self.descs = [d.copy() for d in descs]
+        arr = (ViralDesc * len(descs))(*descs)
+        h = C.c_void_p()
+        r = self.L.viral_create(arr, len(descs), C.byref(h))
+        if r != OK:
+            raise ViralError(r, "viral_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.viral_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, r):
+        if r != OK:
+            raise ViralError(r, self.L.viral_last_error(self.h).decode())
+        return r
+
+    def cells(self, ev=0):
+        return self.descs[ev].cells()
+
+    def upload(self, state, ev=0):
+        """state: (NFIELDS, cells) float64"""
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        assert state.shape == (NFIELDS, self.cells(ev)), state.shape
+        rows = [state[f] for f in range(NFIELDS)]
+        self._chk(self.L.viral_upload_state(self.h, ev, _ptr_array(rows)))
+
+    def download(self, ev=0):
+        out = np.empty((NFIELDS, self.cells(ev)))
+        rows = [out[f] for f in range(NFIELDS)]
+        self._chk(self.L.viral_download_state(self.h, ev, _ptr_array(rows)))
+        return out
+
+    def field(self, f, ev=0):
+        out = np.empty(self.cells(ev))
+        self._chk(self.L.viral_download_field(self.h, ev, f, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def step(self, tau, ts, check=True):
+        """One RK step; returns the status (raises on errors other than
+        VIRAL_ERR_PHYSICS when check is set)."""
+        r = self.L.viral_step(self.h, tau, ts)
+        if check and r not in (OK, ERR_PHYSICS):
+            self._chk(r)
+        return r
+
+    def flags(self, ev=0):
+        out = np.empty(self.cells(ev), dtype=np.uint8)
+        self._chk(self.L.viral_flags(self.h, ev, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def iterations(self, stage, ev=0):
+        out = np.empty(self.cells(ev), dtype=np.int32)
+        self._chk(self.L.viral_iterations(self.h, ev, stage, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def first_bad_cell(self, ev=0):
+        i, j, k, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        fl = C.c_uint()
+        r = self.L.viral_first_bad_cell(self.h, ev, C.byref(i), C.byref(j), C.byref(k), C.byref(s), C.byref(fl))
+        if r == -1:
+            return None
+        self._chk(r)
+        return (i.value, j.value, k.value, s.value, fl.value)
+
+    def max_temp_gev(self, ev=0):
+        v = C.c_double()
+        self._chk(self.L.viral_max_temp_gev(self.h, ev, C.byref(v)))
+        return v.value

@@ -1,0 +1,435 @@
+// physics.h -- per-cell device physics of the VIRAL viscous step (gfx950, FP64).
+//
+// Every function here evaluates its expression in the reference's operation
+// order, compiled with -ffp-contract=off: IEEE add/mul/div/sqrt round the same
+// on gfx950 as on the host, so in exact mode the GPU step reproduces the CPU
+// restatement bit for bit.  Reference lines are cited per function.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace viral {
+
+constexpr double GEVFM = 0.1973;               // NEWEST10/main.cpp:44
+constexpr double PIE = 3.141592653589793;      // NEWEST10/main.cpp:45
+constexpr double WENO_EPS = 1e-6;              // NEWEST10/maindef.h:355
+
+// ---------------------------------------------------------------- exp/log/pow
+// The s95p EOS (NEWEST10/s95p.h) calls exp/pow.  These are built from IEEE
+// operations that round identically on host and device, so host and device
+// agree to the bit: exp by Cody-Waite reduction (ln2 split with an exact
+// hi*k product) and the degree-15 Taylor polynomial of e^r, |r| <= ln2/2;
+// log by frexp reduction to [sqrt(1/2), sqrt(2)) and the atanh series of
+// log m in s = (m-1)/(m+1) to s^25; pow(x,y) = exp(y log x).
+#define VM_LN2_HI 6.93147180369123816490e-01
+#define VM_LN2_LO 1.90821492927058770002e-10
+#define VM_INV_LN2 1.44269504088896338700e+00
+
+__device__ __forceinline__ double dexp(double x)
+{
+  if (x != x) return x;
+  if (x > 709.0) return __builtin_huge_val();
+  if (x < -745.0) return 0.0;
+  double k = rint(x * VM_INV_LN2);
+  double r = (x - k * VM_LN2_HI) - k * VM_LN2_LO;
+  double p = 1.0 / 1307674368000.0;
+  p = p * r + 1.0 / 87178291200.0;
+  p = p * r + 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)k);
+}
+
+__device__ __forceinline__ double dlog(double x)
+{
+  if (x != x || x < 0.0) return __builtin_nan("");
+  if (x == 0.0) return -__builtin_huge_val();
+  if (x == __builtin_huge_val()) return x;
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  double s = (m - 1.0) / (m + 1.0);
+  double z = s * s;
+  double p = 1.0 / 25.0;
+  p = p * z + 1.0 / 23.0;
+  p = p * z + 1.0 / 21.0;
+  p = p * z + 1.0 / 19.0;
+  p = p * z + 1.0 / 17.0;
+  p = p * z + 1.0 / 15.0;
+  p = p * z + 1.0 / 13.0;
+  p = p * z + 1.0 / 11.0;
+  p = p * z + 1.0 / 9.0;
+  p = p * z + 1.0 / 7.0;
+  p = p * z + 1.0 / 5.0;
+  p = p * z + 1.0 / 3.0;
+  double lm = 2.0 * s + 2.0 * s * (z * p);
+  double de = (double)e;
+  return de * VM_LN2_HI + (de * VM_LN2_LO + lm);
+}
+
+__device__ __forceinline__ double dpow(double x, double y) { return dexp(y * dlog(x)); }
+
+// ---------------------------------------------------------------- parameters
+struct Params {
+  int milne, lbi, eos, eta_law, tpi_law, max_iter, rk;
+  unsigned quirks;
+  double dx, dy, deta;
+  double eta_coef, tpi_coef;
+  double factor;    // conformal FACTOR = 3*DOF*pi^2/90 (NEWEST10/maindef.h:10)
+  double t_freeze;
+};
+
+// ---------------------------------------------------------------- EOS
+// s95p closed form, NEWEST10/s95p.h:13-123 (break points and fit coefficients
+// are the reference's; strict comparisons keep its e == E_n fall-through).
+#define S95_E1 0.5028563305441270
+#define S95_E2 1.62
+#define S95_E3 1.86
+#define S95_E4 9.9878355786273545
+#define S95_ES1 0.1270769021427449
+#define S95_ES2 0.446707952467404
+#define S95_ES3 1.9402832534193788
+#define S95_ES4 3.7292474570977285
+
+__device__ __forceinline__ double s95p_p(double e)
+{
+  if (e < 1e-5) return 0.143375 * e;
+  double x = e * GEVFM, r;
+  if (x < S95_E1)
+    r = 0.3299 * (dexp(0.4346 * x) - 1);
+  else if (x > S95_E1 && x < S95_E2)
+    r = 1.024e-7 * dexp(6.041 * x) + 0.007273 + 0.14578 * x;
+  else if (x > S95_E2 && x < S95_E3)
+    r = 0.30195 * dexp(0.31308 * x) - 0.256232;
+  else if (x > S95_E3 && x < S95_E4)
+    r = 0.332 * x - 0.3223 * dpow(x, 0.4585) + 0.1167 * dpow(x, -1.233) - 0.003906 * x * dexp(-0.05697 * x) +
+        0.1436 * x * dexp(-0.9131 * x);
+  else
+    r = 0.3327 * x - 0.3223 * dpow(x, 0.4585) - 0.003906 * x * dexp(-0.05697 * x);
+  return r / GEVFM;
+}
+
+__device__ __forceinline__ double s95p_a(double e)
+{
+  if (e < 1e-5) return 0.143375;
+  double x = e * GEVFM, r;
+  if (x < S95_E1)
+    r = 1 * (0.3299 * (0.4346 * dexp(0.4346 * x)));
+  else if (x > S95_E1 && x < S95_E2)
+    r = 1 * (1.024e-7 * 6.041 * dexp(6.041 * x) + 0.14578);
+  else if (x > S95_E2 && x < S95_E3)
+    r = 1 * (0.30195 * 0.31308 * dexp(0.31308 * x));
+  else if (x > S95_E3 && x < S95_E4)
+    r = 1 * (0.332 - 0.3223 * 0.4585 * dpow(x, (0.4585 - 1)) + 0.1167 * (-1.233) * dpow(x, (-1.233 - 1)) -
+             0.003906 * (dexp(-0.05697 * x) - 0.05697 * x * dexp(-0.05697 * x)) +
+             0.1436 * (dexp(-0.9131 * x) - 0.9131 * x * dexp(-0.9131 * x)));
+  else
+    r = 1 * (0.3327 - 0.3223 * 0.4585 * dpow(x, (0.4585 - 1)) -
+             0.003906 * (dexp(-0.05697 * x) - 0.05697 * x * dexp(-0.05697 * x)));
+  if ((x / GEVFM) < 0.12948 * 1e-14) r = 0.286743;
+  return r;
+}
+
+__device__ __forceinline__ double s95p_s(double e)
+{
+  double x = e * GEVFM, r;
+  if (x < S95_ES1)
+    r = 12.2304 * dpow(x, 1.16849);
+  else if (x > S95_ES1 && x < S95_ES2)
+    r = 11.9279 * dpow(x, 1.15635);
+  else if (x > S95_ES2 && x < S95_ES3)
+    r = 0.0580578 + 11.833 * dpow(x, 1.16187);
+  else if (x > S95_ES3 && x < S95_ES4)
+    r = 18.202 * x - 62.021814 - 4.85479 * dexp(-2.72407e-11 * dpow(x, 4.54886)) +
+        65.1272 * dpow(x, -0.128012) * dexp(-0.00369624 * dpow(x, 1.18735)) - 4.75253 * dpow(x, -1.18423);
+  else
+    r = 18.202 * x - 63.0218 - 4.85479 * dexp(-2.72407e-11 * dpow(x, 4.54886)) +
+        65.1272 * dpow(x, -0.128012) * dexp(-0.00369624 * dpow(x, 1.18735));
+  return dpow(r, 0.75);
+}
+
+__device__ __forceinline__ double s95p_T(double e)
+{
+  double x = e * GEVFM, r;
+  if (x < 0.5143939846236409)
+    r = 0.203054 * dpow(x, 0.30679);
+  else
+    r = (x + GEVFM * s95p_p(e)) / s95p_s(e);
+  return r / GEVFM;
+}
+
+// EOS(e), DPDE(e), FT(e), FS(e,P,T): NEWEST10/maindef.h:68-72 (conformal,
+// with x^(1/4) as sqrt(sqrt(x))) and :274-278 (s95p)
+__device__ __forceinline__ double eos_p(const Params& p, double e) { return p.eos ? s95p_p(e) : e / 3.0; }
+__device__ __forceinline__ double eos_dpde(const Params& p, double e) { return p.eos ? s95p_a(e) : 1.0 / 3.0; }
+__device__ __forceinline__ double eos_T(const Params& p, double e)
+{
+  return p.eos ? s95p_T(e) : sqrt(sqrt(e / p.factor));
+}
+__device__ __forceinline__ double eos_s(const Params& p, double e, double P, double T)
+{
+  return p.eos ? s95p_s(e) : (e + P) * (1.0 / T);
+}
+// Feta / Ftaupi laws, NEWEST10/maindef.h:76-77,187-188,282-283,328-329
+__device__ __forceinline__ double eta_of(const Params& p, double s)
+{
+  return p.eta_law == 1 ? p.eta_coef * (s / (4.0 * PIE)) : p.eta_coef * s;
+}
+__device__ __forceinline__ double tpi_of(const Params& p, double eta, double P, double e)
+{
+  if (p.tpi_law == 1) return p.tpi_coef * (eta / P);
+  if (p.tpi_law == 2) return p.tpi_coef;
+  return p.tpi_coef * eta / (e + P);
+}
+
+// ---------------------------------------------------------------- WENO5-JS
+// NEWEST10/utils.h:102-160: L is the value at the right face of the centre
+// cell, R at its left face; d = (0.3, 0.6, 0.1), eps = 1e-6, p = 2.
+__device__ __forceinline__ void weno_beta(double qm2, double qm1, double qc, double qp1, double qp2, double& b0,
+                                          double& b1, double& b2)
+{
+  double a0 = qc - 2 * qp1 + qp2, c0 = 3 * qc - 4 * qp1 + qp2;
+  double a1 = qm1 - 2 * qc + qp1, c1 = qm1 - qp1;
+  double a2 = qm2 - 2 * qm1 + qc, c2 = qm2 - 4 * qm1 + 3 * qc;
+  b0 = (13. / 12.) * (a0 * a0) + 0.25 * (c0 * c0);
+  b1 = (13. / 12.) * (a1 * a1) + 0.25 * (c1 * c1);
+  b2 = (13. / 12.) * (a2 * a2) + 0.25 * (c2 * c2);
+}
+
+__device__ __forceinline__ double weno_L(double qm2, double qm1, double qc, double qp1, double qp2)
+{
+  double b0, b1, b2;
+  weno_beta(qm2, qm1, qc, qp1, qp2, b0, b1, b2);
+  double q0 = (2 * qc + 5 * qp1 - 1 * qp2) / 6.0;
+  double q1 = (-1 * qm1 + 5 * qc + 2 * qp1) / 6.0;
+  double q2 = (2 * qm2 - 7 * qm1 + 11 * qc) / 6.0;
+  double t0 = WENO_EPS + b0, t1 = WENO_EPS + b1, t2 = WENO_EPS + b2;
+  double al0 = 0.3 / (t0 * t0), al1 = 0.6 / (t1 * t1), al2 = 0.1 / (t2 * t2);
+  double sum = 0;
+  sum += al0;
+  sum += al1;
+  sum += al2;
+  double left = 0;
+  left += (al0 / sum) * q0;
+  left += (al1 / sum) * q1;
+  left += (al2 / sum) * q2;
+  return left;
+}
+
+__device__ __forceinline__ double weno_R(double qm2, double qm1, double qc, double qp1, double qp2)
+{
+  double b0, b1, b2;
+  weno_beta(qm2, qm1, qc, qp1, qp2, b0, b1, b2);
+  double q0 = (11 * qc - 7 * qp1 + 2 * qp2) / 6.0;
+  double q1 = (2 * qm1 + 5 * qc - 1 * qp1) / 6.0;
+  double q2 = (-1 * qm2 + 5 * qm1 + 2 * qc) / 6.0;
+  double t0 = WENO_EPS + b0, t1 = WENO_EPS + b1, t2 = WENO_EPS + b2;
+  double al0 = 0.1 / (t0 * t0), al1 = 0.6 / (t1 * t1), al2 = 0.3 / (t2 * t2);
+  double sum = 0;
+  sum += al0;
+  sum += al1;
+  sum += al2;
+  double right = 0;
+  right += (al0 / sum) * q0;
+  right += (al1 / sum) * q1;
+  right += (al2 / sum) * q2;
+  return right;
+}
+
+// NEWEST10/utils.h:57-98
+__device__ __forceinline__ double weno_der(double qm2, double qm1, double qc, double qp1, double qp2, double dx)
+{
+  double left = weno_L(qm2, qm1, qc, qp1, qp2);
+  double right = weno_R(qm2, qm1, qc, qp1, qp2);
+  return (left - right) / dx;
+}
+
+// ---------------------------------------------------------------- shear source
+// Israel-Stewart relaxation of pi^{mu nu} in conservative form (the physics of
+// NEWEST10/Source.h:448-516):
+//   u.d pi = -(pi - 2 eta sigma)/tau_pi - 4/3 pi theta - (u^m pi^{na} + u^n pi^{ma}) u.d u_a,
+//   sigma^{mn} = Delta^{mn}_{ab} d^a u^b (double projector, du[a][b] = d_b u^a
+//   taken as independent), S = (u.d pi - Christoffel part)/u^0 + pi d_i(u^i/u^0).
+// Operation order matches the CPU restatement exactly.
+__device__ __forceinline__ int pidx(int m, int n)
+{
+  // {{0,1,2,3},{1,4,5,6},{2,5,7,8},{3,6,8,9}}
+  const int a = m < n ? m : n, b = m < n ? n : m;
+  return a == 0 ? b : a == 1 ? 3 + b : a == 2 ? 5 + b : 9;
+}
+
+__device__ __forceinline__ void shear_source(int milne, double tau, const double u[4], const double pi10[10],
+                                             const double du[4][4], double SH, double tpi, double out[10])
+{
+  double g[4] = {1.0, -1.0, -1.0, milne ? -(tau * tau) : -1.0};
+  double gi[4] = {1.0, -1.0, -1.0, milne ? -(1.0 / (tau * tau)) : -1.0};
+  double pi[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int n = 0; n < 4; n++) pi[m][n] = pi10[pidx(m, n)];
+  double ul[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
+  double Dc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
+  double itau = milne ? 1.0 / tau : 0.0;
+  if (milne) {
+    Dc[0][3] = Dc[0][3] + tau * u[3];
+    Dc[3][0] = Dc[3][0] + itau * u[3];
+    Dc[3][3] = Dc[3][3] + itau * u[0];
+  }
+  double theta = Dc[0][0] + Dc[1][1] + Dc[2][2] + Dc[3][3];
+  double G[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
+  double Pm[4][4], Pu[4][4], Pl[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      Pm[m][a] = (m == a ? 1.0 : 0.0) - u[m] * ul[a];
+      Pu[m][a] = (m == a ? gi[m] : 0.0) - u[m] * u[a];
+      Pl[m][a] = (m == a ? g[m] : 0.0) - ul[m] * ul[a];
+    }
+  double tr = 0;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) tr += Pl[a][b] * G[a][b];
+  double M[4][4], Am[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      double s = 0;
+#pragma unroll
+      for (int a = 0; a < 4; a++) s += Pm[m][a] * G[a][b];
+      M[m][b] = s;
+    }
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      double s = 0;
+#pragma unroll
+      for (int b = 0; b < 4; b++) s += M[m][b] * Pm[n][b];
+      Am[m][n] = s;
+    }
+  double Dul[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double s = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) s += u[b] * Dc[a][b];
+    Dul[a] = g[a] * s;
+  }
+  double iu0 = 1.0 / u[0];
+  double divv = 0;
+#pragma unroll
+  for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int n = m; n < 4; n++) {
+      double sigma = 0.5 * (Am[m][n] + Am[n][m]) - (1.0 / 3.0) * Pu[m][n] * tr;
+      double pu = 0;
+#pragma unroll
+      for (int a = 0; a < 4; a++) pu += (u[m] * pi[n][a] + u[n] * pi[m][a]) * Dul[a];
+      double rhs = -(pi[m][n] - 2.0 * SH * sigma) / tpi - (4.0 / 3.0) * pi[m][n] * theta - pu;
+      double chr = 0;
+      if (milne) {
+        double cm = 0, cn = 0;
+        if (m == 0) cm = u[3] * tau * pi[3][n];
+        if (m == 3) cm = u[0] * itau * pi[3][n] + u[3] * itau * pi[0][n];
+        if (n == 0) cn = u[3] * tau * pi[m][3];
+        if (n == 3) cn = u[0] * itau * pi[m][3] + u[3] * itau * pi[m][0];
+        chr = cm + cn;
+      }
+      out[pidx(m, n)] = (rhs - chr) * iu0 + pi[m][n] * divv;
+    }
+}
+
+// ---------------------------------------------------------------- root solve
+// f(e), f'(e) of NEWEST10/root.h:212-285 and the GSL Newton iteration
+// (x_{n+1} = x_n - f/f', stop on two equal iterates, error on f' == 0 or a
+// non-finite f or f'), NEWEST10/root.h:301-364.
+__device__ __forceinline__ double newton_cell(const Params& p, double M2, double Ttt, double pi1, double PI,
+                                              double guess, int& iters, unsigned& flags)
+{
+  double x = guess, fv, dfv;
+  {
+    double Pe = eos_p(p, x);
+    fv = x - (Ttt - pi1) + (M2 / ((Ttt - pi1) + Pe - PI));
+    double den = Ttt - pi1 + Pe - PI;
+    dfv = 1 - ((M2 / (den * den)) * eos_dpde(p, x));
+  }
+  int it = 0;
+  unsigned fl = 0;
+  double h1 = x, h2 = x, h3 = x, h4 = x;  // x_{it-1} .. x_{it-4}
+  for (;;) {
+    it++;
+    if (dfv == 0.0) {
+      fl |= 4u;
+      break;
+    }
+    double xn = x - (fv / dfv);
+    double Pe = eos_p(p, xn);
+    fv = xn - (Ttt - pi1) + (M2 / ((Ttt - pi1) + Pe - PI));
+    double den = Ttt - pi1 + Pe - PI;
+    dfv = 1 - ((M2 / (den * den)) * eos_dpde(p, xn));
+    double xp = x;
+    x = xn;
+    if (!isfinite(fv) || !isfinite(dfv)) {
+      fl |= 4u;
+      break;
+    }
+    if (x == xp) break;
+    if (it >= p.max_iter) {
+      fl |= 8u;
+      break;
+    }
+    h4 = h3;
+    h3 = h2;
+    h2 = h1;
+    h1 = xp;
+    // The Newton map depends on x alone: x_it == x_{it-p} means the iterates
+    // repeat with period p until max_iter, so jump to x_{max_iter} =
+    // x_{it-p+r}, r = (max_iter - it) mod p.  GSL's stopping rule (two equal
+    // iterates) leaves many cells in a period-2 orbit between neighbouring
+    // doubles; running those loops out would stall every wavefront.
+    int per = (it >= 2 && x == h2) ? 2 : (it >= 3 && x == h3) ? 3 : (it >= 4 && x == h4) ? 4 : 0;
+    if (per) {
+      int r = (p.max_iter - it) % per;
+      int q = per - r;
+      x = q == 1 ? h1 : q == 2 ? h2 : q == 3 ? h3 : h4;
+      it = p.max_iter;
+      fl |= 8u;
+      break;
+    }
+  }
+  iters = it;
+  flags = fl;
+  return x;
+}
+
+}  // namespace viral

@@ -1,0 +1,781 @@
+// viral.hip -- MI355X (gfx950) implementation of the VIRAL viscous RK step
+// behind the C ABI of include/viral.h.
+//
+// Device layout per event: structure of arrays, one ghosted FP64 array per
+// field in the reference's HydroGrid[i][j][k] order (eta fastest), two copies
+// of the state (the stage reads one and writes the other), RK registers
+// Var0/Var1/Var2, a Result accumulator, face fluxes and face speeds of the
+// current direction.
+//
+// One RK stage (NEWEST10/time.h:338-392 for RK2) is the launch sequence
+//   for each direction d:  face_speed<d>  face_flux<d>  divergence<d>
+//   update   (sources + RK combine + T/pi from Var + Newton root + velocities)
+//   k0_broadcast (q2)    ghost_fill
+// and the stages of one step are captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/viral.h"
+#include "physics.h"
+
+using namespace viral;
+
+namespace {
+
+constexpr int NF = VIRAL_NFIELDS;
+constexpr int SV = 15;
+constexpr int BLK = 256;
+
+struct Geom {
+  int gx, gy, gz;
+  int il, ir, jl, jr, kl, kr, k0;
+  long n;
+  long sx, sy;  // strides of i and j; k stride is 1
+};
+
+__device__ __forceinline__ long cidx(const Geom& g, int i, int j, int k) { return ((long)i * g.gy + j) * g.gz + k; }
+
+__device__ __forceinline__ bool interior(const Geom& g, int i, int j, int k)
+{
+  return i >= g.il && i < g.ir && j >= g.jl && j < g.jr && k >= g.kl && k < g.kr;
+}
+
+// Box of cells [i0,i1) x [j0,j1) x [k0,k1), thread t -> (i,j,k) with k fastest.
+struct Box {
+  int i0, j0, k0, ni, nj, nk;
+};
+__device__ __forceinline__ bool box_cell(const Box& b, long t, int& i, int& j, int& k)
+{
+  long nk = b.nk, nj = b.nj;
+  if (t >= (long)b.ni * nj * nk) return false;
+  k = b.k0 + (int)(t % nk);
+  t /= nk;
+  j = b.j0 + (int)(t % nj);
+  i = b.i0 + (int)(t / nj);
+  return true;
+}
+
+// faces c+1/2 along d for c in [lo-1, hi-1], interior in the other directions
+__host__ __device__ inline Box face_box(const Geom& g, int d)
+{
+  Box b{g.il, g.jl, g.kl, g.ir - g.il, g.jr - g.jl, g.kr - g.kl};
+  if (d == 0) { b.i0 -= 1; b.ni += 1; }
+  if (d == 1) { b.j0 -= 1; b.nj += 1; }
+  if (d == 2) { b.k0 -= 1; b.nk += 1; }
+  return b;
+}
+__host__ __device__ inline Box interior_box(const Geom& g) { return Box{g.il, g.jl, g.kl, g.ir - g.il, g.jr - g.jl, g.kr - g.kl}; }
+
+__device__ __forceinline__ long dstride(const Geom& g, int d) { return d == 0 ? g.sx : d == 1 ? g.sy : 1; }
+
+// quantity q (0..14 flux row, 15..29 Var row) of the centred flux / Var copy
+// at cell x for direction d, NEWEST10/hydroKT.h:16-98 and :364-381
+__device__ __forceinline__ double flux_q(const double* __restrict__ st, long n, long x, int d, int l, double tau,
+                                         int milne)
+{
+  const double e = st[VIRAL_F_EN * n + x], P = st[VIRAL_F_P * n + x], PI = st[VIRAL_F_BULK * n + x];
+  const double u0 = st[VIRAL_F_U0 * n + x];
+  const double ud = st[(VIRAL_F_U1 + d) * n + x];
+  if (l < 4) {
+    const int row = d + 1;
+    const double piv = st[(VIRAL_F_PI0 + pidx(row, l)) * n + x];
+    const double um = st[(VIRAL_F_U0 + l) * n + x];
+    double v;
+    if (l == row) {
+      double pp = (P - PI);
+      if (d == 2 && milne) pp = ((P - PI) / (tau * tau));
+      v = piv + pp + ud * um * (e + P - PI);
+    } else {
+      v = piv + ud * um * (e + P - PI);
+    }
+    return milne ? tau * (v) : v;
+  }
+  if (l < 14) return (st[(VIRAL_F_PI0 + l - 4) * n + x] * ud) / u0;
+  return (PI * ud) / u0;
+}
+
+__device__ __forceinline__ double var_q(const double* __restrict__ st, long n, long x, int l, double tau, int milne)
+{
+  if (l < 4) return milne ? tau * st[(VIRAL_F_T00 + l) * n + x] : st[(VIRAL_F_T00 + l) * n + x];
+  if (l < 14) return st[(VIRAL_F_PI0 + l - 4) * n + x];
+  return st[VIRAL_F_BULK * n + x];
+}
+
+// ---------------------------------------------------------------- kernels
+
+// face speed a(c+1/2) = max(|A_L(c)|, |A_R(c+1)|) with the WENO of the
+// eigenvalue u^d/u^0 (NEWEST10/hydroKT.h:176-258, 280-329, 453-455).  Under
+// q1 the eigenvalue is zero outside the interior and the face before the
+// first interior cell gets zero speed.
+__global__ void __launch_bounds__(BLK) k_face_speed(Geom g, Params p, int d, const double* __restrict__ st,
+                                                    double* __restrict__ amax)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  Box b = face_box(g, d);
+  if (!box_cell(b, t, i, j, k)) return;
+  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+  const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
+  double A[6];
+  int pos = d == 0 ? i : d == 1 ? j : k;
+#pragma unroll
+  for (int o = 0; o < 6; o++) {
+    long y = x + (o - 2) * s;
+    int pi_ = d == 0 ? i + o - 2 : i, pj = d == 1 ? j + o - 2 : j, pk = d == 2 ? k + o - 2 : k;
+    bool in = interior(g, pi_, pj, pk);
+    A[o] = (in || !q1) ? st[(VIRAL_F_U1 + d) * n + y] / st[VIRAL_F_U0 * n + y] : 0.0;
+  }
+  double aL = weno_L(A[0], A[1], A[2], A[3], A[4]);
+  double aR = weno_R(A[1], A[2], A[3], A[4], A[5]);
+  int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl;
+  double ml = 0, mr = 0;
+  if (!(q1 && pos < lo)) {
+    if (fabs(aL) > ml) ml = fabs(aL);
+    if (fabs(aR) > mr) mr = fabs(aR);
+  }
+  amax[x] = fmax(ml, mr);
+}
+
+// Rusanov/KT face flux of variable l at face c+1/2 (NEWEST10/hydroKT.h:444-506)
+__global__ void __launch_bounds__(BLK) k_face_flux(Geom g, Params p, int d, double tau, const double* __restrict__ st,
+                                                   const double* __restrict__ amax, double* __restrict__ flux)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.y;
+  int i, j, k;
+  Box b = face_box(g, d);
+  if (!box_cell(b, t, i, j, k)) return;
+  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+  double F[6], V[6];
+#pragma unroll
+  for (int o = 0; o < 6; o++) {
+    long y = x + (o - 2) * s;
+    F[o] = flux_q(st, n, y, d, l, tau, p.milne);
+    V[o] = var_q(st, n, y, l, tau, p.milne);
+  }
+  double FL = weno_L(F[0], F[1], F[2], F[3], F[4]);
+  double FR = weno_R(F[1], F[2], F[3], F[4], F[5]);
+  double VL = weno_L(V[0], V[1], V[2], V[3], V[4]);
+  double VR = weno_R(V[1], V[2], V[3], V[4], V[5]);
+  flux[l * n + x] = 0.5 * (FL + FR) - 0.5 * (amax[x]) * (VR - VL);
+}
+
+// Result (+)= -(1/h)(f(c+1/2) - f(c-1/2))  (NEWEST10/hydroKT.h:458-462, 393-403)
+__global__ void __launch_bounds__(BLK) k_divergence(Geom g, int d, double h, int first, const double* __restrict__ flux,
+                                                    double* __restrict__ res)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+#pragma unroll
+  for (int l = 0; l < SV; l++) {
+    double pr = -(1.0 / h) * (flux[l * n + x] - flux[l * n + x - s]);
+    res[l * n + x] = first ? (0.0 + pr) : (res[l * n + x] + pr);
+  }
+}
+
+struct StageArgs {
+  double tau_src;   // time of the sources / fluxes of this stage
+  double ts_src;    // step used in d_tau u = (u - prevu)/ts_src
+  double tau_new;   // time the RK combination lands on
+  double ts;        // full step
+  int stage;        // 0,1,2
+  int kind;         // 0: Var1 = Var0 + ts L; 1: RK2 final; 2: RK3 middle; 3: RK3 final
+};
+
+// Sources, L = Result + Source, the RK combination, T/pi from Var, the Newton
+// root and the velocities for one interior cell (NEWEST10/Source.h:195-536,
+// time.h:2-180, root.h:301-424).  Reads the stage-start state `si`, writes
+// the new state `so`.
+__global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, const double* __restrict__ si,
+                                                double* __restrict__ so, const double* __restrict__ res,
+                                                double* __restrict__ var0, double* __restrict__ var1,
+                                                double* __restrict__ var2, int32_t* __restrict__ iters,
+                                                uint8_t* __restrict__ flags, unsigned long long* __restrict__ bad)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  const long n = g.n, x = cidx(g, i, j, k);
+  const int milne = p.milne;
+  const double tau = a.tau_src;
+  // ---- sources
+  double S[SV];
+  const double e = si[VIRAL_F_EN * n + x], P = si[VIRAL_F_P * n + x], PI = si[VIRAL_F_BULK * n + x];
+  double u[4], pu[4], pi[10];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    u[l] = si[(VIRAL_F_U0 + l) * n + x];
+    pu[l] = si[(VIRAL_F_PU0 + l) * n + x];
+  }
+#pragma unroll
+  for (int l = 0; l < 10; l++) pi[l] = si[(VIRAL_F_PI0 + l) * n + x];
+  if (milne) {
+    double tau2 = tau * tau;
+    S[0] = -(P - PI) - tau2 * (pi[9] + (e + P - PI) * u[3] * u[3]);
+    S[3] = (-2.0 * (pi[3] + (e + P - PI) * u[0] * u[3]));
+  } else {
+    S[0] = 0;
+    S[3] = 0;
+  }
+  S[1] = 0;
+  S[2] = 0;
+  S[14] = 0;
+  double du[4][4];
+  const double hx[3] = {p.dx, p.dy, p.deta};
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const double* ul = si + (VIRAL_F_U0 + l) * n;
+    du[l][0] = (u[l] - pu[l]) / a.ts_src;
+    du[l][1] = weno_der(ul[x - 2 * g.sx], ul[x - g.sx], ul[x], ul[x + g.sx], ul[x + 2 * g.sx], hx[0]);
+    du[l][2] = weno_der(ul[x - 2 * g.sy], ul[x - g.sy], ul[x], ul[x + g.sy], ul[x + 2 * g.sy], hx[1]);
+    if (!p.lbi)
+      du[l][3] = weno_der(ul[x - 2], ul[x - 1], ul[x], ul[x + 1], ul[x + 2], hx[2]);
+    else
+      du[l][3] = 0;
+  }
+  {
+    double T = eos_T(p, e);
+    double s = eos_s(p, e, P, T);
+    double SH = eta_of(p, s);
+    double tpi = tpi_of(p, SH, P, e);
+    double out[10];
+    shear_source(milne, tau, u, pi, du, SH, tpi, out);
+#pragma unroll
+    for (int l = 0; l < 10; l++) S[4 + l] = out[l];
+  }
+  // ---- RK combination
+  double V[SV];
+#pragma unroll
+  for (int l = 0; l < SV; l++) {
+    double L = (res[l * n + x] + S[l]);
+    if (a.kind == 0) {
+      double v0 = l < 4 ? (milne ? tau * si[(VIRAL_F_T00 + l) * n + x] : si[(VIRAL_F_T00 + l) * n + x])
+                        : si[(VIRAL_F_T00 + l) * n + x];
+      double v1 = v0 + a.ts * (L);
+      var0[l * n + x] = v0;
+      var1[l * n + x] = v1;
+      V[l] = v1;
+    } else if (a.kind == 1) {
+      V[l] = 0.5 * var0[l * n + x] + 0.5 * var1[l * n + x] + 0.5 * a.ts * L;
+    } else if (a.kind == 2) {
+      double v2 = 0.75 * var0[l * n + x] + 0.25 * var1[l * n + x] + 0.25 * a.ts * L;
+      var2[l * n + x] = v2;
+      V[l] = v2;
+    } else {
+      V[l] = (1.0 / 3.0) * var0[l * n + x] + (2.0 / 3.0) * var2[l * n + x] + (2.0 / 3.0) * a.ts * L;
+    }
+  }
+  // ---- primary variables at tau_new (NEWEST10/time.h:100-162)
+  const double tn = a.tau_new;
+  double T0[4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) T0[l] = milne ? (V[l]) / (tn) : V[l];
+#pragma unroll
+  for (int l = 0; l < 10; l++) pi[l] = (V[4 + l]);
+  const double PIn = (V[14]);
+  // ---- root (NEWEST10/root.h:301-364)
+  double M2 = milne ? (T0[1] - pi[1]) * (T0[1] - pi[1]) + (T0[2] - pi[2]) * (T0[2] - pi[2]) +
+                          tn * tn * (T0[3] - pi[3]) * (T0[3] - pi[3])
+                    : (T0[1] - pi[1]) * (T0[1] - pi[1]) + (T0[2] - pi[2]) * (T0[2] - pi[2]) +
+                          (T0[3] - pi[3]) * (T0[3] - pi[3]);
+  int it;
+  unsigned fl;
+  double en = newton_cell(p, M2, T0[0], pi[0], PIn, e, it, fl);
+  double Temp = eos_T(p, en);
+  double Pn = eos_p(p, en);
+  // ---- velocities (NEWEST10/root.h:372-424)
+  double t2 = milne ? tn * tn : 1.0;
+  double vx = (T0[1] - pi[1]) / (T0[0] - pi[0] + Pn - PIn);
+  double vy = (T0[2] - pi[2]) / (T0[0] - pi[0] + Pn - PIn);
+  double ve = (T0[3] - pi[3]) / (T0[0] - pi[0] + Pn - PIn);
+  if (vx * vx + vy * vy + t2 * ve * ve >= 1) fl |= VIRAL_FLAG_ACAUSAL;
+  if (en <= 0) fl |= VIRAL_FLAG_ENEG;
+  double gm = 1.0 / sqrt(1 - vx * vx - vy * vy - t2 * ve * ve);
+  if (p.t_freeze > 0 && Temp < p.t_freeze) fl |= VIRAL_FLAG_FREEZEOUT;
+  // ---- store
+#pragma unroll
+  for (int l = 0; l < 4; l++) so[(VIRAL_F_T00 + l) * n + x] = T0[l];
+#pragma unroll
+  for (int l = 0; l < 10; l++) so[(VIRAL_F_PI0 + l) * n + x] = pi[l];
+  so[VIRAL_F_BULK * n + x] = PIn;
+  so[VIRAL_F_EN * n + x] = en;
+  so[VIRAL_F_P * n + x] = Pn;
+  so[VIRAL_F_TEMP * n + x] = Temp;
+  so[VIRAL_F_VX * n + x] = vx;
+  so[VIRAL_F_VY * n + x] = vy;
+  so[VIRAL_F_VE * n + x] = ve;
+  so[VIRAL_F_U0 * n + x] = gm;
+  so[VIRAL_F_U1 * n + x] = gm * vx;
+  so[VIRAL_F_U2 * n + x] = gm * vy;
+  so[VIRAL_F_U3 * n + x] = gm * ve;
+#pragma unroll
+  for (int l = 0; l < 4; l++) so[(VIRAL_F_PU0 + l) * n + x] = a.stage == 0 ? u[l] : pu[l];
+  iters[x] = it;
+  uint8_t prev = a.stage == 0 ? 0 : flags[x];
+  flags[x] = (uint8_t)((prev & ~VIRAL_FLAG_FREEZEOUT) | fl);
+  if (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER)) {
+    // stage in the high bits, reference loop-order index in the low bits
+    atomicMin(bad, ((unsigned long long)a.stage << 48) | (unsigned long long)x);
+  }
+}
+
+// NEWEST10/root.h:426-449: every eta slice takes the velocity of slice k0
+__global__ void __launch_bounds__(BLK) k_k0_broadcast(Geom g, Params p, double tau, double* __restrict__ st)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  if (k == g.k0) return;
+  const long n = g.n, x = cidx(g, i, j, k), x0 = cidx(g, i, j, g.k0);
+  double t2 = p.milne ? tau * tau : 1.0;
+  double vx = st[VIRAL_F_VX * n + x0], vy = st[VIRAL_F_VY * n + x0], ve = st[VIRAL_F_VE * n + x0];
+  st[VIRAL_F_VE * n + x] = ve;
+  st[VIRAL_F_VX * n + x] = vx;
+  st[VIRAL_F_VY * n + x] = vy;
+  double u0 = 1.0 / sqrt(1.0 - vx * vx - vy * vy - t2 * ve * ve);
+  st[VIRAL_F_U0 * n + x] = u0;
+  st[VIRAL_F_U1 * n + x] = u0 * vx;
+  st[VIRAL_F_U2 * n + x] = u0 * vy;
+  st[VIRAL_F_U3 * n + x] = u0 * ve;
+}
+
+// fields copied into ghosts, NEWEST10/boundary.h:13-46 (all but Temp)
+__constant__ int c_bfields[28] = {VIRAL_F_T00, VIRAL_F_T10, VIRAL_F_T20, VIRAL_F_T30, VIRAL_F_EN, VIRAL_F_VX,
+                                  VIRAL_F_VY, VIRAL_F_VE, VIRAL_F_P, VIRAL_F_U0, VIRAL_F_U1, VIRAL_F_U2, VIRAL_F_U3,
+                                  VIRAL_F_PU0, VIRAL_F_PU1, VIRAL_F_PU2, VIRAL_F_PU3, VIRAL_F_PI0, VIRAL_F_PI0 + 1,
+                                  VIRAL_F_PI0 + 2, VIRAL_F_PI0 + 3, VIRAL_F_PI0 + 4, VIRAL_F_PI0 + 5,
+                                  VIRAL_F_PI0 + 6, VIRAL_F_PI0 + 7, VIRAL_F_PI0 + 8, VIRAL_F_PI0 + 9, VIRAL_F_BULK};
+
+// Zero-gradient ghost fill of the x, y (and eta) layers of interior rows,
+// NEWEST10/boundary.h:1-275.  Thread = (ghost cell, field); the three ghost
+// slabs are disjoint and read only interior cells.
+__global__ void __launch_bounds__(BLK) k_ghost_fill(Geom g, Params p, double* __restrict__ st)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  const int f = c_bfields[q];
+  const long n = g.n;
+  const int nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
+  const long cx = 2L * VIRAL_BORDER * ny * nz;  // x slabs
+  const long cy = 2L * VIRAL_BORDER * nx * nz;  // y slabs
+  const long cz = p.lbi ? 0 : 2L * VIRAL_BORDER * nx * ny;
+  const bool mirror = (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
+  long dst, src;
+  if (t < cx) {
+    int k = g.kl + (int)(t % nz);
+    long r = t / nz;
+    int j = g.jl + (int)(r % ny);
+    int gg = (int)(r / ny);  // 0..2B-1
+    int side = gg / VIRAL_BORDER, h = gg % VIRAL_BORDER;
+    int i = side == 0 ? h : g.ir + h;
+    int is = side == 0 ? g.il : g.ir - 1;
+    if (mirror) is = side == 0 ? g.il + VIRAL_BORDER - h - 1 : g.ir - h - 1;
+    dst = cidx(g, i, j, k);
+    src = cidx(g, is, j, k);
+  } else if (t < cx + cy) {
+    t -= cx;
+    int k = g.kl + (int)(t % nz);
+    long r = t / nz;
+    int i = g.il + (int)(r % nx);
+    int gg = (int)(r / nx);
+    int side = gg / VIRAL_BORDER, h = gg % VIRAL_BORDER;
+    int j = side == 0 ? h : g.jr + h;
+    int js = side == 0 ? g.jl : g.jr - 1;
+    if (mirror) js = side == 0 ? g.jl + VIRAL_BORDER - h - 1 : g.jr - h - 1;
+    dst = cidx(g, i, j, k);
+    src = cidx(g, i, js, k);
+  } else if (t < cx + cy + cz) {
+    t -= cx + cy;
+    int h = (int)(t % (2 * VIRAL_BORDER));
+    long r = t / (2 * VIRAL_BORDER);
+    int j = g.jl + (int)(r % ny);
+    int i = g.il + (int)(r / ny);
+    int side = h / VIRAL_BORDER;
+    h = h % VIRAL_BORDER;
+    int k = side == 0 ? h : g.kr + h;
+    int ks = side == 0 ? g.kl : g.kr - 1;
+    dst = cidx(g, i, j, k);
+    src = cidx(g, i, j, ks);
+  } else {
+    return;
+  }
+  st[f * n + dst] = st[f * n + src];
+}
+
+__global__ void k_max_temp(Geom g, Params p, const double* __restrict__ st, unsigned long long* out)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  double v = 0;
+  if (box_cell(interior_box(g), t, i, j, k)) {
+    double tt = GEVFM * eos_T(p, st[VIRAL_F_EN * g.n + cidx(g, i, j, k)]);
+    if (tt > v) v = tt;
+  }
+  // wave reduction then one atomic per wave; non-negative doubles order like their bits
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_down(v, o);
+    if (w > v) v = w;
+  }
+  if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+struct Event {
+  viral_desc d;
+  Geom g;
+  Params p;
+  double* st[2] = {nullptr, nullptr};  // ping-pong state, NF*n each
+  int cur = 0;                          // which copy holds the state
+  double* var0 = nullptr;
+  double* var1 = nullptr;
+  double* var2 = nullptr;
+  double* res = nullptr;
+  double* flux = nullptr;
+  double* amax = nullptr;
+  int32_t* iters = nullptr;  // 3*n
+  uint8_t* flags = nullptr;
+  unsigned long long* bad = nullptr;   // device word
+  unsigned long long bad_host = ~0ull;
+};
+
+struct viral_ctx {
+  std::vector<Event> ev;
+  hipStream_t stream = nullptr;
+  std::string err;
+  unsigned long long* dtmp = nullptr;
+};
+
+static int fail(viral_ctx* c, int code, const std::string& msg)
+{
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                          \
+  do {                                                                                        \
+    hipError_t _e = (call);                                                                   \
+    if (_e != hipSuccess) return fail(ctx, VIRAL_ERR_HIP, std::string(#call ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+static unsigned nblocks(long cells) { return (unsigned)((cells + BLK - 1) / BLK); }
+static long box_count(const Box& b) { return (long)b.ni * b.nj * b.nk; }
+
+extern "C" {
+
+void viral_desc_default(viral_desc* d, int nx, int ny, int nz)
+{
+  memset(d, 0, sizeof(*d));
+  d->nx = nx;
+  d->ny = ny;
+  d->nz = nz;
+  d->lbi = 0;
+  d->coords = VIRAL_MILNE;
+  d->scheme = VIRAL_KT;
+  d->rk = 2;
+  d->eos = VIRAL_EOS_CONFORMAL;
+  d->dx = d->dy = d->deta = 0.1;
+  d->dof = 42.25;
+  d->eta_law = VIRAL_ETA_TIMES_S;
+  d->eta_coef = 0.2;
+  d->tpi_law = VIRAL_TPI_OVER_EP;
+  d->tpi_coef = 5.0;
+  d->quirks = VIRAL_Q_DEFAULT;
+  d->k0 = -1;
+  d->max_iter = 5000;
+  d->t_freeze = 0.0;
+  d->fast_math = 0;
+}
+
+void viral_ghost_dims(const viral_desc* d, int* gx, int* gy, int* gz)
+{
+  *gx = d->nx + 2 * VIRAL_BORDER;
+  *gy = d->ny + 2 * VIRAL_BORDER;
+  *gz = d->lbi ? 1 : d->nz + 2 * VIRAL_BORDER;
+}
+
+size_t viral_cells(const viral_desc* d)
+{
+  int gx, gy, gz;
+  viral_ghost_dims(d, &gx, &gy, &gz);
+  return (size_t)gx * gy * gz;
+}
+
+static int check_desc(viral_ctx* ctx, const viral_desc* d)
+{
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(ctx, VIRAL_ERR_ARG, "grid sizes must be positive");
+  if (d->lbi && d->nz != 1) return fail(ctx, VIRAL_ERR_ARG, "lbi grids need nz == 1");
+  if (d->rk < 1 || d->rk > 3) return fail(ctx, VIRAL_ERR_ARG, "rk must be 1, 2 or 3");
+  if (d->scheme != VIRAL_KT) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "only the KT scheme is implemented");
+  if (d->coords != VIRAL_MILNE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "only Milne coordinates are implemented");
+  if (d->eos != VIRAL_EOS_CONFORMAL && d->eos != VIRAL_EOS_S95P) return fail(ctx, VIRAL_ERR_ARG, "unknown eos");
+  if (d->max_iter < 1) return fail(ctx, VIRAL_ERR_ARG, "max_iter must be >= 1");
+  if (d->fast_math) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "fast_math is not implemented yet");
+  int gx, gy, gz;
+  viral_ghost_dims(d, &gx, &gy, &gz);
+  if (!d->lbi && d->k0 >= gz) return fail(ctx, VIRAL_ERR_ARG, "k0 outside the grid");
+  if ((double)gx * gy * gz > 4.0e9) return fail(ctx, VIRAL_ERR_ARG, "grid too large");
+  return VIRAL_OK;
+}
+
+int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
+{
+  if (!out || !descs || nev < 1) return VIRAL_ERR_ARG;
+  *out = nullptr;
+  viral_ctx* ctx = new viral_ctx();
+  for (int e = 0; e < nev; e++) {
+    int r = check_desc(ctx, &descs[e]);
+    if (r) {
+      std::string m = ctx->err;
+      delete ctx;
+      return r;
+    }
+  }
+  hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    delete ctx;
+    return VIRAL_ERR_HIP;
+  }
+  ctx->ev.resize(nev);
+  for (int e = 0; e < nev; e++) {
+    Event& E = ctx->ev[e];
+    const viral_desc* d = &descs[e];
+    E.d = *d;
+    Geom& g = E.g;
+    viral_ghost_dims(d, &g.gx, &g.gy, &g.gz);
+    g.il = VIRAL_BORDER; g.ir = VIRAL_BORDER + d->nx;
+    g.jl = VIRAL_BORDER; g.jr = VIRAL_BORDER + d->ny;
+    if (d->lbi) { g.kl = 0; g.kr = 1; g.k0 = 0; }
+    else { g.kl = VIRAL_BORDER; g.kr = VIRAL_BORDER + d->nz; g.k0 = d->k0 >= 0 ? d->k0 : (g.gz - 1) / 2; }
+    g.n = (long)g.gx * g.gy * g.gz;
+    g.sx = (long)g.gy * g.gz;
+    g.sy = g.gz;
+    Params& p = E.p;
+    p.milne = d->coords == VIRAL_MILNE;
+    p.lbi = d->lbi;
+    p.eos = d->eos;
+    p.eta_law = d->eta_law;
+    p.tpi_law = d->tpi_law;
+    p.max_iter = d->max_iter;
+    p.rk = d->rk;
+    p.quirks = d->quirks;
+    p.dx = d->dx; p.dy = d->dy; p.deta = d->deta;
+    p.eta_coef = d->eta_coef;
+    p.tpi_coef = d->tpi_coef;
+    p.factor = 3.0 * d->dof * PIE * PIE / 90.0;
+    p.t_freeze = d->t_freeze;
+    size_t n = (size_t)g.n;
+    bool ok = true;
+    ok = ok && hipMalloc(&E.st[0], NF * n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.st[1], NF * n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.var0, SV * n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.var1, SV * n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.var2, SV * n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.res, SV * n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.flux, SV * n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.amax, n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.iters, 3 * n * sizeof(int32_t)) == hipSuccess;
+    ok = ok && hipMalloc(&E.flags, n) == hipSuccess;
+    ok = ok && hipMalloc(&E.bad, sizeof(unsigned long long)) == hipSuccess;
+    if (!ok) {
+      viral_destroy(ctx);
+      return VIRAL_ERR_HIP;
+    }
+    hipMemset(E.st[0], 0, NF * n * sizeof(double));
+    hipMemset(E.st[1], 0, NF * n * sizeof(double));
+    hipMemset(E.iters, 0, 3 * n * sizeof(int32_t));
+    hipMemset(E.flags, 0, n);
+  }
+  if (hipMalloc(&ctx->dtmp, sizeof(unsigned long long)) != hipSuccess) {
+    viral_destroy(ctx);
+    return VIRAL_ERR_HIP;
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    viral_destroy(ctx);
+    return VIRAL_ERR_HIP;
+  }
+  *out = ctx;
+  return VIRAL_OK;
+}
+
+void viral_destroy(viral_ctx* ctx)
+{
+  if (!ctx) return;
+  for (Event& E : ctx->ev) {
+    hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1); hipFree(E.var2);
+    hipFree(E.res); hipFree(E.flux); hipFree(E.amax); hipFree(E.iters); hipFree(E.flags); hipFree(E.bad);
+  }
+  if (ctx->dtmp) hipFree(ctx->dtmp);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* viral_last_error(viral_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+static Event* get_ev(viral_ctx* ctx, int ev)
+{
+  if (!ctx || ev < 0 || ev >= (int)ctx->ev.size()) return nullptr;
+  return &ctx->ev[ev];
+}
+
+int viral_upload_state(viral_ctx* ctx, int ev, const double* const* fields)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !fields) return fail(ctx, VIRAL_ERR_ARG, "bad event or fields");
+  size_t n = (size_t)E->g.n;
+  for (int f = 0; f < NF; f++) {
+    if (!fields[f]) return fail(ctx, VIRAL_ERR_ARG, "null field pointer");
+    // both copies: cells the step never writes (corner ghosts, Temp in ghosts)
+    // keep their uploaded values whichever copy is current
+    HIPCHK(hipMemcpyAsync(E->st[0] + f * n, fields[f], n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(E->st[1] + f * n, fields[f], n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  E->cur = 0;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
+}
+
+int viral_download_field(viral_ctx* ctx, int ev, int f, double* out)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !out || f < 0 || f >= NF) return fail(ctx, VIRAL_ERR_ARG, "bad event, field or pointer");
+  size_t n = (size_t)E->g.n;
+  HIPCHK(hipMemcpyAsync(out, E->st[E->cur] + f * n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
+}
+
+int viral_download_state(viral_ctx* ctx, int ev, double* const* fields)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !fields) return fail(ctx, VIRAL_ERR_ARG, "bad event or fields");
+  size_t n = (size_t)E->g.n;
+  for (int f = 0; f < NF; f++) {
+    if (!fields[f]) return fail(ctx, VIRAL_ERR_ARG, "null field pointer");
+    HIPCHK(hipMemcpyAsync(fields[f], E->st[E->cur] + f * n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
+}
+
+// one stage on one event: fluxes at tau_src from copy `in`, update into `out`
+static int launch_stage(viral_ctx* ctx, Event& E, const StageArgs& a)
+{
+  const Geom& g = E.g;
+  const Params& p = E.p;
+  const double* si = E.st[E.cur];
+  double* so = E.st[E.cur ^ 1];
+  const int ndir = p.lbi ? 2 : 3;
+  const double hx[3] = {p.dx, p.dy, p.deta};
+  for (int d = 0; d < ndir; d++) {
+    Box fb = face_box(g, d);
+    long nf = box_count(fb);
+    hipLaunchKernelGGL(k_face_speed, dim3(nblocks(nf)), dim3(BLK), 0, ctx->stream, g, p, d, si, E.amax);
+    hipLaunchKernelGGL(k_face_flux, dim3(nblocks(nf), SV), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si, E.amax,
+                       E.flux);
+    hipLaunchKernelGGL(k_divergence, dim3(nblocks(box_count(interior_box(g)))), dim3(BLK), 0, ctx->stream, g, d,
+                       hx[d], d == 0 ? 1 : 0, E.flux, E.res);
+  }
+  long ni = box_count(interior_box(g));
+  hipLaunchKernelGGL(k_update, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, so, E.res, E.var0, E.var1,
+                     E.var2, E.iters + (long)a.stage * g.n, E.flags, E.bad);
+  if ((p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi)
+    hipLaunchKernelGGL(k_k0_broadcast, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a.tau_new, so);
+  int nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
+  long ng = 2L * VIRAL_BORDER * ((long)ny * nz + (long)nx * nz + (p.lbi ? 0 : (long)nx * ny));
+  hipLaunchKernelGGL(k_ghost_fill, dim3(nblocks(ng), 28), dim3(BLK), 0, ctx->stream, g, p, so);
+  E.cur ^= 1;
+  HIPCHK(hipGetLastError());
+  return VIRAL_OK;
+}
+
+int viral_step(viral_ctx* ctx, double tau, double ts)
+{
+  if (!ctx) return VIRAL_ERR_ARG;
+  for (Event& E : ctx->ev) {
+    HIPCHK(hipMemsetAsync(E.bad, 0xff, sizeof(unsigned long long), ctx->stream));
+    const int rk = E.p.rk;
+    StageArgs a{};
+    a.ts = ts;
+    // stage 1: NEWEST10/time.h:345-363 (RK2), 402-426 (RK3), 307-325 (RK1)
+    a.tau_src = tau; a.ts_src = ts; a.tau_new = tau + ts; a.stage = 0; a.kind = 0;
+    if (int r = launch_stage(ctx, E, a)) return r;
+    if (rk == 2) {
+      a.tau_src = tau + ts; a.ts_src = ts; a.tau_new = tau + ts; a.stage = 1; a.kind = 1;
+      if (int r = launch_stage(ctx, E, a)) return r;
+    } else if (rk == 3) {
+      a.tau_src = tau + ts; a.ts_src = ts; a.tau_new = tau + ts / 2.0; a.stage = 1; a.kind = 2;
+      if (int r = launch_stage(ctx, E, a)) return r;
+      a.tau_src = tau + ts / 2.0; a.ts_src = ts / 2.0; a.tau_new = tau + ts; a.stage = 2; a.kind = 3;
+      if (int r = launch_stage(ctx, E, a)) return r;
+    }
+    HIPCHK(hipMemcpyAsync(&E.bad_host, E.bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (Event& E : ctx->ev)
+    if (E.bad_host != ~0ull) return fail(ctx, VIRAL_ERR_PHYSICS, "a cell became acausal, non-positive or unsolvable");
+  return VIRAL_OK;
+}
+
+int viral_flags(viral_ctx* ctx, int ev, uint8_t* out)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !out) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
+  HIPCHK(hipMemcpyAsync(out, E->flags, (size_t)E->g.n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
+}
+
+int viral_iterations(viral_ctx* ctx, int ev, int stage, int32_t* out)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !out || stage < 0 || stage > 2) return fail(ctx, VIRAL_ERR_ARG, "bad event, stage or pointer");
+  HIPCHK(hipMemcpyAsync(out, E->iters + (long)stage * E->g.n, (size_t)E->g.n * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
+}
+
+int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* stage, unsigned* flags)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !i || !j || !k || !stage || !flags) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
+  if (E->bad_host == ~0ull) return -1;
+  long x = (long)(E->bad_host & ((1ull << 48) - 1));
+  *stage = (int)(E->bad_host >> 48);
+  *k = (int)(x % E->g.gz);
+  long r = x / E->g.gz;
+  *j = (int)(r % E->g.gy);
+  *i = (int)(r / E->g.gy);
+  uint8_t fl = 0;
+  HIPCHK(hipMemcpy(&fl, E->flags + x, 1, hipMemcpyDeviceToHost));
+  *flags = fl;
+  return 0;
+}
+
+int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !out) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
+  HIPCHK(hipMemsetAsync(ctx->dtmp, 0, sizeof(unsigned long long), ctx->stream));
+  long ni = box_count(interior_box(E->g));
+  hipLaunchKernelGGL(k_max_temp, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, E->g, E->p, E->st[E->cur], ctx->dtmp);
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, ctx->dtmp, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double v;
+  memcpy(&v, &bits, sizeof(v));
+  *out = v;
+  return VIRAL_OK;
+}
+
+}  // extern "C"
