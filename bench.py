@@ -1,0 +1,177 @@
+"""Benchmark of the MI355X viscous hydro step (one GPU).
+
+Workload (N=1): the second benchmark configuration -- viscous Kurganov-Tadmor
+step with the s95p EOS on a smooth optical-Glauber event (Woods-Saxon
+R=6.62 fm, a=0.546 fm, b=7 fm, peak e=30 GeV/fm^3, eta plateau 1.5 with
+Gaussian fall-off 1.0), Milne grid 200x200x64, dx=deta=0.1, tau0=0.6 fm/c,
+eta/s=0.08, tau_pi=5 eta/(e+p), RK3 with dtau=0.002 fm/c as the reference's
+s95p build (NEWEST10/Makeeos:5, maindef.h:255), FP64.  A step is one full RK
+step of the whole grid; inputs are resident in HBM before timing starts.
+
+Prints one JSON line.  `value` is interior cell-updates per second.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 cell-updates/s: plain vs batched ensemble; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
+def workload_desc(rk=3, fast=1):
+    from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP
+    return make_desc(200, 200, 64, dx=0.1, dy=0.1, deta=0.1, rk=rk, eos=EOS_S95P,
+                     eta_law=ETA_TIMES_S, eta_coef=0.08, tpi_law=TPI_OVER_EP, tpi_coef=5.0, fast_math=fast)
+
+
+def eos_p_vec(desc):
+    """s95p pressure on the host for the initial state (the oracle's closed
+    form is the reference's formula; used only to build inputs)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes
+    L = oracle_ctypes.lib()
+    return lambda e: np.vectorize(lambda v: L.or_eos_p(desc, float(v)))(e)
+
+
+# Compulsory HBM bytes per launch of each kernel (inputs read once, outputs
+# written once), per unit of work; units per launch are faces (flux/speed) or
+# interior cells (the rest).  F = face count of one direction, C = interior
+# cells.  See the kernel table of the design notes.
+def kernel_bytes(name, C, F):
+    if name == "face_flux":      # 15 launches' worth: per variable, ~7 state fields read + 1 flux written
+        return F * 8 * (7 + 1)
+    if name == "face_speed":     # u^d, u^0 read, a written
+        return F * 8 * 3
+    if name == "divergence":     # 15 fluxes read, 15 results read+written
+        return C * 8 * 45
+    if name == "update":         # 24 state fields + 15 result + 15 Var0 (+15 Var1) read, 29 state + 30 Var written
+        return C * 8 * (24 + 15 + 30 + 29 + 30)
+    if name == "k0_broadcast":
+        return C * 8 * (3 + 7)
+    if name == "ghost_fill":
+        return 0
+    return 0
+
+
+def cpu_baseline(seconds_target=20.0):
+    """The CPU oracle (a one-thread restatement of the reference step) on a
+    bounded sample of the same workload: the central 16-slice eta slab of the
+    200x200 Glauber grid, one RK3 step."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes
+    from viral_amd import ic
+    d = workload_desc(fast=0)
+    d.nz = 16
+    st = ic.glauber(d, 0.6, 0.002, b=7.0, eos_p=eos_p_vec(d))
+    o = oracle_ctypes.Oracle(d)
+    o.set_state(st)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        o.step(0.6 + steps * 0.002, 0.002)
+        steps += 1
+        if time.perf_counter() - t0 > seconds_target or steps >= 3:
+            break
+    dt = time.perf_counter() - t0
+    cells = d.nx * d.ny * d.nz
+    return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d RK3 step(s) of the central 200x200x16 eta slab of the same Glauber s95p grid, "
+                      "CPU oracle (C, -O2, one thread)" % steps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--exact", action="store_true",
+                    help="time the exact-arithmetic mode (bitwise equal to the CPU oracle) instead of fast mode")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (events shard by event, a "
+              "single event by x-y slabs; see the design notes). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from viral_amd import capi, ic
+
+    assert torch.cuda.is_available(), "needs a GPU"
+    torch.cuda.set_device(0)
+    d = workload_desc(fast=0 if args.exact else 1)
+    tau0, ts = 0.6, 0.002
+    st = ic.glauber(d, tau0, ts, b=7.0, eos_p=eos_p_vec(d))
+    h = capi.Hydro(d)
+    h.upload(st)
+    tau = tau0
+    for _ in range(args.warmup):
+        r = h.step(tau, ts)
+        assert r == 0, r
+        tau += ts
+    torch.cuda.synchronize()
+    h.timers(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = h.step(tau, ts)
+        assert r == 0, r
+        tau += ts
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rep = h.timer_report()
+    h.timers(False)
+    cells = d.nx * d.ny * d.nz
+    value = cells * args.steps / dt
+    ms_step = dt * 1000.0 / args.steps
+
+    # dominant kernel and its roofline figure
+    F = (d.nx + 1) * d.ny * d.nz  # faces of one direction (approx. all three)
+    dom = max(rep, key=lambda k: rep[k][0])
+    tot_ms, launches = rep[dom]
+    avg_s = tot_ms / 1000.0 / max(launches, 1)
+    bytes_launch = kernel_bytes(dom, cells, F)
+    achieved = bytes_launch / avg_s / 1e9
+    kernel_share = {k: round(v[0] / max(sum(x[0] for x in rep.values()), 1e-12), 4) for k, v in rep.items()}
+    bytes_per_update = {1: 336, 2: 784, 3: 1232}[d.rk]
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (optical-Glauber initial state generated on the host)",
+        "config": {"workload": "viscous KT, s95p EOS, Glauber b=7 fm, Milne 200x200x64, dx=deta=0.1, "
+                               "tau0=0.6, dtau=0.002, eta/s=0.08, tau_pi=5eta/(e+p), RK3",
+                   "grid": [d.nx, d.ny, d.nz], "rk": d.rk, "fast_math": d.fast_math,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_s * 1000.0, "launches": launches,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "step_hbm": {"algorithmic_bytes_per_cell_update": bytes_per_update,
+                     "achieved_GBs": value * bytes_per_update / 1e9,
+                     "frac": value * bytes_per_update / 1e9 / HBM_PEAK_GBS},
+        "kernel_time_share": kernel_share,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -145,12 +145,23 @@ int viral_flags(viral_ctx* ctx, int ev, uint8_t* out);
 int viral_iterations(viral_ctx* ctx, int ev, int stage, int32_t* out);
 /* First interior cell, in the reference's i,j,k loop order, that raised
  * ACAUSAL/ENEG/SOLVER in the last step's first failing stage; -1 if none.
- * Writes the ghosted (i,j,k) and the flags of that cell. */
+ * Writes the ghosted (i,j,k), the 0-based stage and that cell's flags in
+ * that stage. */
 int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* stage, unsigned* flags);
 
 /* Device reductions the reference's driver runs every step.
  * MaxTempGev (NEWEST10/utils.h:263-283): max over interior of GEVFM*T(e). */
 int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out);
+
+/* Per-kernel timing with HIP events recorded on the context's stream around
+ * every launch (instrumentation; the reference only times whole steps with
+ * clock(), NEWEST10/main.cpp:107,170-171).  viral_timers(ctx, 1) clears and
+ * starts recording; viral_timer_read returns the summed duration and launch
+ * count of kernel id k in [0, viral_kernel_count()). */
+int viral_timers(viral_ctx* ctx, int enable);
+int viral_timer_read(viral_ctx* ctx, int kernel, double* total_ms, long long* launches);
+int viral_kernel_count(void);
+const char* viral_kernel_name(int kernel);
 
 /* Error text of the last failing call on ctx (never NULL). */
 const char* viral_last_error(viral_ctx* ctx);

@@ -859,12 +859,12 @@ static int stage_root(or_ctx* c, double tau_r, int stage)
     size_t x = IX(c, i, j, k);
     c->flags[x] = (uint8_t)((c->flags[x] & ~VIRAL_FLAG_FREEZEOUT) | fl[x]);
   }
-  free(fl);
   if (bad >= 0 && c->bad_cell < 0) {
     c->bad_cell = bad;
     c->bad_stage = stage;
-    c->bad_flags = c->flags[bad];
+    c->bad_flags = fl[bad];
   }
+  free(fl);
   or_boundary(c);
   return bad;
 }

@@ -124,3 +124,57 @@ def test_max_temp_matches_oracle():
     o = oracle_ctypes.Oracle(d)
     o.set_state(st)
     assert h.max_temp_gev() == o.max_temp_gev()
+
+
+# ---------------------------------------------------------------- fast mode
+# fast_math=1 uses one division per WENO reconstruction, shared smoothness
+# indicators and FMAs; the bar is the tolerance the target states: relative
+# error <= 1e-10 per step and <= 1e-8 after 100 steps against the oracle.
+
+def rel_err(g, c, f):
+    scale = np.max(np.abs(c[f]))
+    if scale == 0:
+        return np.max(np.abs(g[f]))
+    return np.max(np.abs(g[f] - c[f])) / scale
+
+
+def test_fast_mode_one_step_within_1e10():
+    d = make_desc(40, 36, 9, fast_math=1)
+    dref = d.copy()
+    dref.fast_math = 0
+    tau, ts = 0.9, 0.005
+    st = ic.random_smooth(d, seed=21, tau=tau)
+    print("case fast one step", flush=True)
+    h = capi.Hydro(d)
+    h.upload(st)
+    o = oracle_ctypes.Oracle(dref)
+    o.set_state(st)
+    assert h.step(tau, ts) == o.step(tau, ts) == OK
+    g, c = h.download(), o.state()
+    worst = max(rel_err(g, c, f) for f in range(NFIELDS))
+    assert worst <= 1e-10, worst
+    I = np.zeros(d.ghost_dims(), bool)
+    I[d.interior_slices()] = True
+    I = I.ravel()
+    assert np.max(np.abs(g[F_EN][I] - c[F_EN][I]) / c[F_EN][I]) <= 1e-10
+    assert np.array_equal(h.flags() & 0x7, o.flags() & 0x7)
+
+
+def test_fast_mode_100_steps_gubser_within_1e8():
+    """Config 1 (64x64x8 Gubser, RK2, dtau=0.01) for 100 steps."""
+    d = make_desc(64, 64, 8, fast_math=1)
+    dref = d.copy()
+    dref.fast_math = 0
+    tau, ts = 1.0, 0.01
+    st = ic.gubser(d, tau, ts)
+    print("case fast 100 steps", flush=True)
+    h = capi.Hydro(d)
+    h.upload(st)
+    o = oracle_ctypes.Oracle(dref)
+    o.set_state(st)
+    for s in range(100):
+        t = tau + s * ts
+        assert h.step(t, ts) == o.step(t, ts) == OK
+    g, c = h.download(), o.state()
+    worst = max(rel_err(g, c, f) for f in range(NFIELDS))
+    assert worst <= 1e-8, worst

@@ -19,6 +19,7 @@ EXPORTS = [
     "viral_desc_default", "viral_cells", "viral_ghost_dims", "viral_create", "viral_destroy",
     "viral_upload_state", "viral_download_state", "viral_download_field", "viral_step",
     "viral_flags", "viral_iterations", "viral_first_bad_cell", "viral_max_temp_gev",
+    "viral_timers", "viral_timer_read", "viral_kernel_count", "viral_kernel_name",
     "viral_last_error",
 ]
 
@@ -64,6 +65,13 @@ def load():
     L.viral_first_bad_cell.argtypes = [vp, C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_uint)]
     L.viral_max_temp_gev.restype = C.c_int
     L.viral_max_temp_gev.argtypes = [vp, C.c_int, dp]
+    L.viral_timers.restype = C.c_int
+    L.viral_timers.argtypes = [vp, C.c_int]
+    L.viral_timer_read.restype = C.c_int
+    L.viral_timer_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.viral_kernel_count.restype = C.c_int
+    L.viral_kernel_name.restype = C.c_char_p
+    L.viral_kernel_name.argtypes = [C.c_int]
     L.viral_last_error.restype = C.c_char_p
     L.viral_last_error.argtypes = [vp]
     _lib = L
@@ -155,6 +163,18 @@ class Hydro:
             return None
         self._chk(r)
         return (i.value, j.value, k.value, s.value, fl.value)
+
+    def timers(self, enable=True):
+        self._chk(self.L.viral_timers(self.h, 1 if enable else 0))
+
+    def timer_report(self):
+        """{kernel name: (total ms, launches)} since timers were enabled"""
+        out = {}
+        for k in range(self.L.viral_kernel_count()):
+            ms, n = C.c_double(), C.c_longlong()
+            self._chk(self.L.viral_timer_read(self.h, k, C.byref(ms), C.byref(n)))
+            out[self.L.viral_kernel_name(k).decode()] = (ms.value, n.value)
+        return out
 
     def max_temp_gev(self, ev=0):
         v = C.c_double()

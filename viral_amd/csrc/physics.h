@@ -432,4 +432,98 @@ __device__ __forceinline__ double newton_cell(const Params& p, double M2, double
   return x;
 }
 
+
+// ---------------------------------------------------------------- fast forms
+// Same WENO5-JS weights written with one division per reconstruction:
+//   w_c = d_c / t_c / sum  ==  d_c P_c / sum_c d_c P_c,  t_c = (eps+b_c)^2,
+//   P_0 = t_1 t_2, P_1 = t_0 t_2, P_2 = t_0 t_1,
+// the 1/6 of the candidate polynomials folded into the denominator, L and R
+// sharing the smoothness indicators, and FMAs.  Agrees with weno_L/weno_R to
+// a few ulp (tests hold fast mode to 1e-10 relative per step).
+__device__ __forceinline__ void weno_LR_fast(double qm2, double qm1, double qc, double qp1, double qp2, double& L,
+                                             double& R)
+{
+  const double a0 = fma(-2.0, qp1, qc) + qp2, c0 = fma(3.0, qc, fma(-4.0, qp1, qp2));
+  const double a1 = fma(-2.0, qc, qm1) + qp1, c1 = qm1 - qp1;
+  const double a2 = fma(-2.0, qm1, qm2) + qc, c2 = fma(3.0, qc, fma(-4.0, qm1, qm2));
+  const double k = 13.0 / 12.0;
+  double t0 = WENO_EPS + fma(k * a0, a0, 0.25 * (c0 * c0));
+  double t1 = WENO_EPS + fma(k * a1, a1, 0.25 * (c1 * c1));
+  double t2 = WENO_EPS + fma(k * a2, a2, 0.25 * (c2 * c2));
+  t0 = t0 * t0;
+  t1 = t1 * t1;
+  t2 = t2 * t2;
+  const double P0 = t1 * t2, P1 = t0 * t2, P2 = t0 * t1;
+  // candidate values times 6
+  const double l0 = fma(2.0, qc, fma(5.0, qp1, -qp2));
+  const double l1 = fma(5.0, qc, fma(2.0, qp1, -qm1));
+  const double l2 = fma(2.0, qm2, fma(-7.0, qm1, 11.0 * qc));
+  const double r0 = fma(11.0, qc, fma(-7.0, qp1, 2.0 * qp2));
+  const double r1 = fma(2.0, qm1, fma(5.0, qc, -qp1));
+  const double r2 = fma(5.0, qm1, fma(2.0, qc, -qm2));
+  const double wl0 = 0.3 * P0, wl1 = 0.6 * P1, wl2 = 0.1 * P2;
+  const double wr0 = 0.1 * P0, wr1 = 0.6 * P1, wr2 = 0.3 * P2;
+  L = fma(wl0, l0, fma(wl1, l1, wl2 * l2)) / (6.0 * (wl0 + wl1 + wl2));
+  R = fma(wr0, r0, fma(wr1, r1, wr2 * r2)) / (6.0 * (wr0 + wr1 + wr2));
+}
+
+__device__ __forceinline__ double weno_L_fast(double qm2, double qm1, double qc, double qp1, double qp2)
+{
+  const double a0 = fma(-2.0, qp1, qc) + qp2, c0 = fma(3.0, qc, fma(-4.0, qp1, qp2));
+  const double a1 = fma(-2.0, qc, qm1) + qp1, c1 = qm1 - qp1;
+  const double a2 = fma(-2.0, qm1, qm2) + qc, c2 = fma(3.0, qc, fma(-4.0, qm1, qm2));
+  const double k = 13.0 / 12.0;
+  double t0 = WENO_EPS + fma(k * a0, a0, 0.25 * (c0 * c0));
+  double t1 = WENO_EPS + fma(k * a1, a1, 0.25 * (c1 * c1));
+  double t2 = WENO_EPS + fma(k * a2, a2, 0.25 * (c2 * c2));
+  t0 = t0 * t0;
+  t1 = t1 * t1;
+  t2 = t2 * t2;
+  const double wl0 = 0.3 * (t1 * t2), wl1 = 0.6 * (t0 * t2), wl2 = 0.1 * (t0 * t1);
+  const double l0 = fma(2.0, qc, fma(5.0, qp1, -qp2));
+  const double l1 = fma(5.0, qc, fma(2.0, qp1, -qm1));
+  const double l2 = fma(2.0, qm2, fma(-7.0, qm1, 11.0 * qc));
+  return fma(wl0, l0, fma(wl1, l1, wl2 * l2)) / (6.0 * (wl0 + wl1 + wl2));
+}
+
+__device__ __forceinline__ double weno_R_fast(double qm2, double qm1, double qc, double qp1, double qp2)
+{
+  const double a0 = fma(-2.0, qp1, qc) + qp2, c0 = fma(3.0, qc, fma(-4.0, qp1, qp2));
+  const double a1 = fma(-2.0, qc, qm1) + qp1, c1 = qm1 - qp1;
+  const double a2 = fma(-2.0, qm1, qm2) + qc, c2 = fma(3.0, qc, fma(-4.0, qm1, qm2));
+  const double k = 13.0 / 12.0;
+  double t0 = WENO_EPS + fma(k * a0, a0, 0.25 * (c0 * c0));
+  double t1 = WENO_EPS + fma(k * a1, a1, 0.25 * (c1 * c1));
+  double t2 = WENO_EPS + fma(k * a2, a2, 0.25 * (c2 * c2));
+  t0 = t0 * t0;
+  t1 = t1 * t1;
+  t2 = t2 * t2;
+  const double wr0 = 0.1 * (t1 * t2), wr1 = 0.6 * (t0 * t2), wr2 = 0.3 * (t0 * t1);
+  const double r0 = fma(11.0, qc, fma(-7.0, qp1, 2.0 * qp2));
+  const double r1 = fma(2.0, qm1, fma(5.0, qc, -qp1));
+  const double r2 = fma(5.0, qm1, fma(2.0, qc, -qm2));
+  return fma(wr0, r0, fma(wr1, r1, wr2 * r2)) / (6.0 * (wr0 + wr1 + wr2));
+}
+
+template <bool FAST>
+__device__ __forceinline__ double wL(double a, double b, double c, double d, double e)
+{
+  return FAST ? weno_L_fast(a, b, c, d, e) : weno_L(a, b, c, d, e);
+}
+template <bool FAST>
+__device__ __forceinline__ double wR(double a, double b, double c, double d, double e)
+{
+  return FAST ? weno_R_fast(a, b, c, d, e) : weno_R(a, b, c, d, e);
+}
+template <bool FAST>
+__device__ __forceinline__ double wder(double a, double b, double c, double d, double e, double h, double ih)
+{
+  if (FAST) {
+    double L, R;
+    weno_LR_fast(a, b, c, d, e, L, R);
+    return (L - R) * ih;
+  }
+  return weno_der(a, b, c, d, e, h);
+}
+
 }  // namespace viral

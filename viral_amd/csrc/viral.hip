@@ -8,10 +8,10 @@
 // current direction.
 //
 // One RK stage (NEWEST10/time.h:338-392 for RK2) is the launch sequence
-//   for each direction d:  face_speed<d>  face_flux<d>  divergence<d>
-//   update   (sources + RK combine + T/pi from Var + Newton root + velocities)
+//   for each direction d:  flux<d>   (face speeds + WENO + Rusanov face fluxes)
+//   update   (flux divergence + sources + RK combination + T/pi from Var +
+//             Newton root + velocities)
 //   k0_broadcast (q2)    ghost_fill
-// and the stages of one step are captured once into a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -73,110 +73,95 @@ __host__ __device__ inline Box interior_box(const Geom& g) { return Box{g.il, g.
 
 __device__ __forceinline__ long dstride(const Geom& g, int d) { return d == 0 ? g.sx : d == 1 ? g.sy : 1; }
 
-// quantity q (0..14 flux row, 15..29 Var row) of the centred flux / Var copy
-// at cell x for direction d, NEWEST10/hydroKT.h:16-98 and :364-381
-__device__ __forceinline__ double flux_q(const double* __restrict__ st, long n, long x, int d, int l, double tau,
-                                         int milne)
-{
-  const double e = st[VIRAL_F_EN * n + x], P = st[VIRAL_F_P * n + x], PI = st[VIRAL_F_BULK * n + x];
-  const double u0 = st[VIRAL_F_U0 * n + x];
-  const double ud = st[(VIRAL_F_U1 + d) * n + x];
-  if (l < 4) {
-    const int row = d + 1;
-    const double piv = st[(VIRAL_F_PI0 + pidx(row, l)) * n + x];
-    const double um = st[(VIRAL_F_U0 + l) * n + x];
-    double v;
-    if (l == row) {
-      double pp = (P - PI);
-      if (d == 2 && milne) pp = ((P - PI) / (tau * tau));
-      v = piv + pp + ud * um * (e + P - PI);
-    } else {
-      v = piv + ud * um * (e + P - PI);
-    }
-    return milne ? tau * (v) : v;
-  }
-  if (l < 14) return (st[(VIRAL_F_PI0 + l - 4) * n + x] * ud) / u0;
-  return (PI * ud) / u0;
-}
-
-__device__ __forceinline__ double var_q(const double* __restrict__ st, long n, long x, int l, double tau, int milne)
-{
-  if (l < 4) return milne ? tau * st[(VIRAL_F_T00 + l) * n + x] : st[(VIRAL_F_T00 + l) * n + x];
-  if (l < 14) return st[(VIRAL_F_PI0 + l - 4) * n + x];
-  return st[VIRAL_F_BULK * n + x];
-}
-
 // ---------------------------------------------------------------- kernels
 
-// face speed a(c+1/2) = max(|A_L(c)|, |A_R(c+1)|) with the WENO of the
-// eigenvalue u^d/u^0 (NEWEST10/hydroKT.h:176-258, 280-329, 453-455).  Under
-// q1 the eigenvalue is zero outside the interior and the face before the
-// first interior cell gets zero speed.
-__global__ void __launch_bounds__(BLK) k_face_speed(Geom g, Params p, int d, const double* __restrict__ st,
-                                                    double* __restrict__ amax)
+// One thread per face c+1/2 of direction d: the WENO of the eigenvalue u^d/u^0
+// gives the face speed a = max(|A_L(c)|, |A_R(c+1)|) (NEWEST10/hydroKT.h:
+// 176-258, 280-329; under q1 the eigenvalue is zero outside the interior and
+// the face before the first interior cell gets a = 0), then for each of the 15
+// variables the WENO of the centred flux (hydroKT.h:16-98) and of Var
+// (hydroKT.h:364-381) give the Rusanov flux
+//   f = (F_L + F_R)/2 - a (V_R - V_L)/2      (hydroKT.h:444-506).
+// The stencil cells' common terms are loaded once and kept in registers.
+template <bool FAST>
+__global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, double tau, const double* __restrict__ st,
+                                                  double* __restrict__ flux)
 {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   Box b = face_box(g, d);
   if (!box_cell(b, t, i, j, k)) return;
   const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+  const int milne = p.milne;
   const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
-  double A[6];
-  int pos = d == 0 ? i : d == 1 ? j : k;
+  const int row = d + 1;
+  double u0[6], ud[6], w[6], pp[6];
 #pragma unroll
   for (int o = 0; o < 6; o++) {
-    long y = x + (o - 2) * s;
-    int pi_ = d == 0 ? i + o - 2 : i, pj = d == 1 ? j + o - 2 : j, pk = d == 2 ? k + o - 2 : k;
-    bool in = interior(g, pi_, pj, pk);
-    A[o] = (in || !q1) ? st[(VIRAL_F_U1 + d) * n + y] / st[VIRAL_F_U0 * n + y] : 0.0;
+    const long y = x + (o - 2) * s;
+    const double e = st[VIRAL_F_EN * n + y], P = st[VIRAL_F_P * n + y], PI = st[VIRAL_F_BULK * n + y];
+    u0[o] = st[VIRAL_F_U0 * n + y];
+    ud[o] = st[(VIRAL_F_U1 + d) * n + y];
+    w[o] = e + P - PI;
+    pp[o] = (P - PI);
+    if (d == 2 && milne) pp[o] = FAST ? pp[o] * (1.0 / (tau * tau)) : ((P - PI) / (tau * tau));
   }
-  double aL = weno_L(A[0], A[1], A[2], A[3], A[4]);
-  double aR = weno_R(A[1], A[2], A[3], A[4], A[5]);
-  int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl;
-  double ml = 0, mr = 0;
-  if (!(q1 && pos < lo)) {
-    if (fabs(aL) > ml) ml = fabs(aL);
-    if (fabs(aR) > mr) mr = fabs(aR);
-  }
-  amax[x] = fmax(ml, mr);
-}
-
-// Rusanov/KT face flux of variable l at face c+1/2 (NEWEST10/hydroKT.h:444-506)
-__global__ void __launch_bounds__(BLK) k_face_flux(Geom g, Params p, int d, double tau, const double* __restrict__ st,
-                                                   const double* __restrict__ amax, double* __restrict__ flux)
-{
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int l = blockIdx.y;
-  int i, j, k;
-  Box b = face_box(g, d);
-  if (!box_cell(b, t, i, j, k)) return;
-  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
-  double F[6], V[6];
+  // face speed
+  double amax;
+  {
+    double A[6];
+    const int pos = d == 0 ? i : d == 1 ? j : k;
 #pragma unroll
-  for (int o = 0; o < 6; o++) {
-    long y = x + (o - 2) * s;
-    F[o] = flux_q(st, n, y, d, l, tau, p.milne);
-    V[o] = var_q(st, n, y, l, tau, p.milne);
+    for (int o = 0; o < 6; o++) {
+      const int ii = d == 0 ? i + o - 2 : i, jj = d == 1 ? j + o - 2 : j, kk = d == 2 ? k + o - 2 : k;
+      const bool in = interior(g, ii, jj, kk);
+      A[o] = (in || !q1) ? ud[o] / u0[o] : 0.0;
+    }
+    const double aL = wL<FAST>(A[0], A[1], A[2], A[3], A[4]);
+    const double aR = wR<FAST>(A[1], A[2], A[3], A[4], A[5]);
+    const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl;
+    double ml = 0, mr = 0;
+    if (!(q1 && pos < lo)) {
+      if (fabs(aL) > ml) ml = fabs(aL);
+      if (fabs(aR) > mr) mr = fabs(aR);
+    }
+    amax = fmax(ml, mr);
   }
-  double FL = weno_L(F[0], F[1], F[2], F[3], F[4]);
-  double FR = weno_R(F[1], F[2], F[3], F[4], F[5]);
-  double VL = weno_L(V[0], V[1], V[2], V[3], V[4]);
-  double VR = weno_R(V[1], V[2], V[3], V[4], V[5]);
-  flux[l * n + x] = 0.5 * (FL + FR) - 0.5 * (amax[x]) * (VR - VL);
-}
-
-// Result (+)= -(1/h)(f(c+1/2) - f(c-1/2))  (NEWEST10/hydroKT.h:458-462, 393-403)
-__global__ void __launch_bounds__(BLK) k_divergence(Geom g, int d, double h, int first, const double* __restrict__ flux,
-                                                    double* __restrict__ res)
-{
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!box_cell(interior_box(g), t, i, j, k)) return;
-  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+  double vd[6];
+  if (FAST) {
 #pragma unroll
+    for (int o = 0; o < 6; o++) vd[o] = ud[o] / u0[o];
+  }
+  const double tf = milne ? tau : 1.0;
+#pragma unroll 1
   for (int l = 0; l < SV; l++) {
-    double pr = -(1.0 / h) * (flux[l * n + x] - flux[l * n + x - s]);
-    res[l * n + x] = first ? (0.0 + pr) : (res[l * n + x] + pr);
+    double F[6], V[6];
+    // field feeding the flux row: pi^{row,l} for T rows, pi[l-4] / PI otherwise
+    const int ff = l < 4 ? VIRAL_F_PI0 + pidx(row, l) : (l < 14 ? VIRAL_F_PI0 + l - 4 : VIRAL_F_BULK);
+#pragma unroll
+    for (int o = 0; o < 6; o++) {
+      const long y = x + (o - 2) * s;
+      const double fv = st[(long)ff * n + y];
+      const double vv = st[(long)l * n + y];  // Var field index == state field index (T00..T30, pi0..pi9, PI)
+      if (l < 4) {
+        const double um = st[(VIRAL_F_U0 + l) * n + y];
+        double v;
+        if (l == row)
+          v = fv + pp[o] + ud[o] * um * w[o];
+        else
+          v = fv + ud[o] * um * w[o];
+        F[o] = milne ? tau * (v) : v;
+        V[o] = milne ? tf * vv : vv;
+      } else {
+        F[o] = FAST ? fv * vd[o] : (fv * ud[o]) / u0[o];
+        V[o] = vv;
+      }
+    }
+    const double FL = wL<FAST>(F[0], F[1], F[2], F[3], F[4]);
+    const double FR = wR<FAST>(F[1], F[2], F[3], F[4], F[5]);
+    const double VL = wL<FAST>(V[0], V[1], V[2], V[3], V[4]);
+    const double VR = wR<FAST>(V[1], V[2], V[3], V[4], V[5]);
+    flux[l * n + x] = FAST ? fma(-0.5 * amax, VR - VL, 0.5 * (FL + FR)) : 0.5 * (FL + FR) - 0.5 * (amax) * (VR - VL);
   }
 }
 
@@ -189,12 +174,15 @@ struct StageArgs {
   int kind;         // 0: Var1 = Var0 + ts L; 1: RK2 final; 2: RK3 middle; 3: RK3 final
 };
 
-// Sources, L = Result + Source, the RK combination, T/pi from Var, the Newton
-// root and the velocities for one interior cell (NEWEST10/Source.h:195-536,
-// time.h:2-180, root.h:301-424).  Reads the stage-start state `si`, writes
-// the new state `so`.
+// Flux divergence, sources, L = Result + Source, the RK combination, T/pi
+// from Var, the Newton root and the velocities for one interior cell
+// (NEWEST10/hydroKT.h:393-403,458-504; Source.h:195-536; time.h:2-180;
+// root.h:301-424).  Reads the stage-start state `si` and the face fluxes of
+// the three directions, writes the new state `so`.
+template <bool FAST>
 __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, const double* __restrict__ si,
-                                                double* __restrict__ so, const double* __restrict__ res,
+                                                double* __restrict__ so, const double* __restrict__ fx,
+                                                const double* __restrict__ fy, const double* __restrict__ fz,
                                                 double* __restrict__ var0, double* __restrict__ var1,
                                                 double* __restrict__ var2, int32_t* __restrict__ iters,
                                                 uint8_t* __restrict__ flags, unsigned long long* __restrict__ bad)
@@ -205,6 +193,27 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
   const long n = g.n, x = cidx(g, i, j, k);
   const int milne = p.milne;
   const double tau = a.tau_src;
+  // ---- Result = sum over directions of -(1/h)(f(c+1/2) - f(c-1/2)), x then y then eta
+  double Res[SV];
+  {
+    const double* fl[3] = {fx, fy, fz};
+    const long sd[3] = {g.sx, g.sy, 1};
+    const double hh[3] = {p.dx, p.dy, p.deta};
+    const int ndir = p.lbi ? 2 : 3;
+#pragma unroll
+    for (int l = 0; l < SV; l++) Res[l] = 0.0;
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) {
+      if (dd >= ndir) break;
+      const double* f = fl[dd];
+#pragma unroll
+      for (int l = 0; l < SV; l++) {
+        const double df = f[l * n + x] - f[l * n + x - sd[dd]];
+        const double pr = FAST ? -df * (1.0 / hh[dd]) : -(1.0 / hh[dd]) * (df);
+        Res[l] = Res[l] + pr;
+      }
+    }
+  }
   // ---- sources
   double S[SV];
   const double e = si[VIRAL_F_EN * n + x], P = si[VIRAL_F_P * n + x], PI = si[VIRAL_F_BULK * n + x];
@@ -232,11 +241,11 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
 #pragma unroll
   for (int l = 0; l < 4; l++) {
     const double* ul = si + (VIRAL_F_U0 + l) * n;
-    du[l][0] = (u[l] - pu[l]) / a.ts_src;
-    du[l][1] = weno_der(ul[x - 2 * g.sx], ul[x - g.sx], ul[x], ul[x + g.sx], ul[x + 2 * g.sx], hx[0]);
-    du[l][2] = weno_der(ul[x - 2 * g.sy], ul[x - g.sy], ul[x], ul[x + g.sy], ul[x + 2 * g.sy], hx[1]);
+    du[l][0] = FAST ? (u[l] - pu[l]) * (1.0 / a.ts_src) : (u[l] - pu[l]) / a.ts_src;
+    du[l][1] = wder<FAST>(ul[x - 2 * g.sx], ul[x - g.sx], ul[x], ul[x + g.sx], ul[x + 2 * g.sx], hx[0], 1.0 / hx[0]);
+    du[l][2] = wder<FAST>(ul[x - 2 * g.sy], ul[x - g.sy], ul[x], ul[x + g.sy], ul[x + 2 * g.sy], hx[1], 1.0 / hx[1]);
     if (!p.lbi)
-      du[l][3] = weno_der(ul[x - 2], ul[x - 1], ul[x], ul[x + 1], ul[x + 2], hx[2]);
+      du[l][3] = wder<FAST>(ul[x - 2], ul[x - 1], ul[x], ul[x + 1], ul[x + 2], hx[2], 1.0 / hx[2]);
     else
       du[l][3] = 0;
   }
@@ -250,15 +259,15 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
 #pragma unroll
     for (int l = 0; l < 10; l++) S[4 + l] = out[l];
   }
-  // ---- RK combination
+  // ---- RK combination (NEWEST10/time.h:2-98)
   double V[SV];
 #pragma unroll
   for (int l = 0; l < SV; l++) {
-    double L = (res[l * n + x] + S[l]);
+    double L = (Res[l] + S[l]);
     if (a.kind == 0) {
       double v0 = l < 4 ? (milne ? tau * si[(VIRAL_F_T00 + l) * n + x] : si[(VIRAL_F_T00 + l) * n + x])
                         : si[(VIRAL_F_T00 + l) * n + x];
-      double v1 = v0 + a.ts * (L);
+      double v1 = FAST ? fma(a.ts, L, v0) : v0 + a.ts * (L);
       var0[l * n + x] = v0;
       var1[l * n + x] = v1;
       V[l] = v1;
@@ -276,7 +285,7 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
   const double tn = a.tau_new;
   double T0[4];
 #pragma unroll
-  for (int l = 0; l < 4; l++) T0[l] = milne ? (V[l]) / (tn) : V[l];
+  for (int l = 0; l < 4; l++) T0[l] = milne ? (FAST ? V[l] * (1.0 / tn) : (V[l]) / (tn)) : V[l];
 #pragma unroll
   for (int l = 0; l < 10; l++) pi[l] = (V[4 + l]);
   const double PIn = (V[14]);
@@ -292,9 +301,17 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
   double Pn = eos_p(p, en);
   // ---- velocities (NEWEST10/root.h:372-424)
   double t2 = milne ? tn * tn : 1.0;
-  double vx = (T0[1] - pi[1]) / (T0[0] - pi[0] + Pn - PIn);
-  double vy = (T0[2] - pi[2]) / (T0[0] - pi[0] + Pn - PIn);
-  double ve = (T0[3] - pi[3]) / (T0[0] - pi[0] + Pn - PIn);
+  double vx, vy, ve;
+  if (FAST) {
+    const double r = 1.0 / (T0[0] - pi[0] + Pn - PIn);
+    vx = (T0[1] - pi[1]) * r;
+    vy = (T0[2] - pi[2]) * r;
+    ve = (T0[3] - pi[3]) * r;
+  } else {
+    vx = (T0[1] - pi[1]) / (T0[0] - pi[0] + Pn - PIn);
+    vy = (T0[2] - pi[2]) / (T0[0] - pi[0] + Pn - PIn);
+    ve = (T0[3] - pi[3]) / (T0[0] - pi[0] + Pn - PIn);
+  }
   if (vx * vx + vy * vy + t2 * ve * ve >= 1) fl |= VIRAL_FLAG_ACAUSAL;
   if (en <= 0) fl |= VIRAL_FLAG_ENEG;
   double gm = 1.0 / sqrt(1 - vx * vx - vy * vy - t2 * ve * ve);
@@ -321,8 +338,8 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
   uint8_t prev = a.stage == 0 ? 0 : flags[x];
   flags[x] = (uint8_t)((prev & ~VIRAL_FLAG_FREEZEOUT) | fl);
   if (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER)) {
-    // stage in the high bits, reference loop-order index in the low bits
-    atomicMin(bad, ((unsigned long long)a.stage << 48) | (unsigned long long)x);
+    // ordered by stage, then reference loop-order index; this stage's flags in the low byte
+    atomicMin(bad, ((unsigned long long)a.stage << 56) | ((unsigned long long)x << 8) | (unsigned long long)fl);
   }
 }
 
@@ -439,20 +456,59 @@ struct Event {
   double* var0 = nullptr;
   double* var1 = nullptr;
   double* var2 = nullptr;
-  double* res = nullptr;
-  double* flux = nullptr;
-  double* amax = nullptr;
+  double* flux[3] = {nullptr, nullptr, nullptr};  // face fluxes of x, y, eta: [15][n] each
   int32_t* iters = nullptr;  // 3*n
   uint8_t* flags = nullptr;
   unsigned long long* bad = nullptr;   // device word
   unsigned long long bad_host = ~0ull;
 };
 
+enum KernelId { K_FLUX = 0, K_UPDATE, K_K0, K_GHOST, K_COUNT };
+static const char* const kKernelNames[K_COUNT] = {"flux", "update", "k0_broadcast", "ghost_fill"};
+
 struct viral_ctx {
   std::vector<Event> ev;
   hipStream_t stream = nullptr;
   std::string err;
   unsigned long long* dtmp = nullptr;
+  // per-kernel HIP-event timing (viral_timers)
+  bool timing = false;
+  std::vector<hipEvent_t> tev[K_COUNT];  // start/stop pairs
+  std::vector<hipEvent_t> pool;
+};
+
+static hipEvent_t pool_event(viral_ctx* c)
+{
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+// brackets one kernel launch with events on the context stream when timing
+struct KTimer {
+  viral_ctx* c;
+  int id;
+  KTimer(viral_ctx* c_, int id_) : c(c_), id(id_)
+  {
+    if (c->timing) {
+      hipEvent_t e = pool_event(c);
+      hipEventRecord(e, c->stream);
+      c->tev[id].push_back(e);
+    }
+  }
+  ~KTimer()
+  {
+    if (c->timing) {
+      hipEvent_t e = pool_event(c);
+      hipEventRecord(e, c->stream);
+      c->tev[id].push_back(e);
+    }
+  }
 };
 
 static int fail(viral_ctx* c, int code, const std::string& msg)
@@ -519,7 +575,6 @@ static int check_desc(viral_ctx* ctx, const viral_desc* d)
   if (d->coords != VIRAL_MILNE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "only Milne coordinates are implemented");
   if (d->eos != VIRAL_EOS_CONFORMAL && d->eos != VIRAL_EOS_S95P) return fail(ctx, VIRAL_ERR_ARG, "unknown eos");
   if (d->max_iter < 1) return fail(ctx, VIRAL_ERR_ARG, "max_iter must be >= 1");
-  if (d->fast_math) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "fast_math is not implemented yet");
   int gx, gy, gz;
   viral_ghost_dims(d, &gx, &gy, &gz);
   if (!d->lbi && d->k0 >= gz) return fail(ctx, VIRAL_ERR_ARG, "k0 outside the grid");
@@ -580,9 +635,7 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     ok = ok && hipMalloc(&E.var0, SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.var1, SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.var2, SV * n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&E.res, SV * n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&E.flux, SV * n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&E.amax, n * sizeof(double)) == hipSuccess;
+    for (int q = 0; q < 3; q++) ok = ok && hipMalloc(&E.flux[q], SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.iters, 3 * n * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc(&E.flags, n) == hipSuccess;
     ok = ok && hipMalloc(&E.bad, sizeof(unsigned long long)) == hipSuccess;
@@ -612,9 +665,13 @@ void viral_destroy(viral_ctx* ctx)
   if (!ctx) return;
   for (Event& E : ctx->ev) {
     hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1); hipFree(E.var2);
-    hipFree(E.res); hipFree(E.flux); hipFree(E.amax); hipFree(E.iters); hipFree(E.flags); hipFree(E.bad);
+    for (int q = 0; q < 3; q++) hipFree(E.flux[q]);
+    hipFree(E.iters); hipFree(E.flags); hipFree(E.bad);
   }
   if (ctx->dtmp) hipFree(ctx->dtmp);
+  for (int k = 0; k < K_COUNT; k++)
+    for (hipEvent_t e : ctx->tev[k]) hipEventDestroy(e);
+  for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -675,24 +732,38 @@ static int launch_stage(viral_ctx* ctx, Event& E, const StageArgs& a)
   const double* si = E.st[E.cur];
   double* so = E.st[E.cur ^ 1];
   const int ndir = p.lbi ? 2 : 3;
-  const double hx[3] = {p.dx, p.dy, p.deta};
+  const bool fast = E.d.fast_math != 0;
   for (int d = 0; d < ndir; d++) {
     Box fb = face_box(g, d);
     long nf = box_count(fb);
-    hipLaunchKernelGGL(k_face_speed, dim3(nblocks(nf)), dim3(BLK), 0, ctx->stream, g, p, d, si, E.amax);
-    hipLaunchKernelGGL(k_face_flux, dim3(nblocks(nf), SV), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si, E.amax,
-                       E.flux);
-    hipLaunchKernelGGL(k_divergence, dim3(nblocks(box_count(interior_box(g)))), dim3(BLK), 0, ctx->stream, g, d,
-                       hx[d], d == 0 ? 1 : 0, E.flux, E.res);
+    KTimer t(ctx, K_FLUX);
+    if (fast)
+      hipLaunchKernelGGL(k_flux_dir<true>, dim3(nblocks(nf)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si,
+                         E.flux[d]);
+    else
+      hipLaunchKernelGGL(k_flux_dir<false>, dim3(nblocks(nf)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si,
+                         E.flux[d]);
   }
   long ni = box_count(interior_box(g));
-  hipLaunchKernelGGL(k_update, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, so, E.res, E.var0, E.var1,
-                     E.var2, E.iters + (long)a.stage * g.n, E.flags, E.bad);
-  if ((p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi)
+  {
+    KTimer t(ctx, K_UPDATE);
+    if (fast)
+      hipLaunchKernelGGL(k_update<true>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, so, E.flux[0],
+                         E.flux[1], E.flux[2], E.var0, E.var1, E.var2, E.iters + (long)a.stage * g.n, E.flags, E.bad);
+    else
+      hipLaunchKernelGGL(k_update<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, so, E.flux[0],
+                         E.flux[1], E.flux[2], E.var0, E.var1, E.var2, E.iters + (long)a.stage * g.n, E.flags, E.bad);
+  }
+  if ((p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi) {
+    KTimer t(ctx, K_K0);
     hipLaunchKernelGGL(k_k0_broadcast, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a.tau_new, so);
+  }
   int nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
   long ng = 2L * VIRAL_BORDER * ((long)ny * nz + (long)nx * nz + (p.lbi ? 0 : (long)nx * ny));
-  hipLaunchKernelGGL(k_ghost_fill, dim3(nblocks(ng), 28), dim3(BLK), 0, ctx->stream, g, p, so);
+  {
+    KTimer t(ctx, K_GHOST);
+    hipLaunchKernelGGL(k_ghost_fill, dim3(nblocks(ng), 28), dim3(BLK), 0, ctx->stream, g, p, so);
+  }
   E.cur ^= 1;
   HIPCHK(hipGetLastError());
   return VIRAL_OK;
@@ -750,15 +821,13 @@ int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* st
   Event* E = get_ev(ctx, ev);
   if (!E || !i || !j || !k || !stage || !flags) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
   if (E->bad_host == ~0ull) return -1;
-  long x = (long)(E->bad_host & ((1ull << 48) - 1));
-  *stage = (int)(E->bad_host >> 48);
+  long x = (long)((E->bad_host >> 8) & ((1ull << 48) - 1));
+  *stage = (int)(E->bad_host >> 56);
   *k = (int)(x % E->g.gz);
   long r = x / E->g.gz;
   *j = (int)(r % E->g.gy);
   *i = (int)(r / E->g.gy);
-  uint8_t fl = 0;
-  HIPCHK(hipMemcpy(&fl, E->flags + x, 1, hipMemcpyDeviceToHost));
-  *flags = fl;
+  *flags = (unsigned)(E->bad_host & 0xff);
   return 0;
 }
 
@@ -775,6 +844,37 @@ int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)
   double v;
   memcpy(&v, &bits, sizeof(v));
   *out = v;
+  return VIRAL_OK;
+}
+
+int viral_kernel_count(void) { return K_COUNT; }
+const char* viral_kernel_name(int kernel) { return (kernel >= 0 && kernel < K_COUNT) ? kKernelNames[kernel] : ""; }
+
+int viral_timers(viral_ctx* ctx, int enable)
+{
+  if (!ctx) return VIRAL_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < K_COUNT; k++) {
+    for (hipEvent_t e : ctx->tev[k]) ctx->pool.push_back(e);
+    ctx->tev[k].clear();
+  }
+  ctx->timing = enable != 0;
+  return VIRAL_OK;
+}
+
+int viral_timer_read(viral_ctx* ctx, int kernel, double* total_ms, long long* launches)
+{
+  if (!ctx || kernel < 0 || kernel >= K_COUNT || !total_ms || !launches) return fail(ctx, VIRAL_ERR_ARG, "bad timer");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double t = 0;
+  const std::vector<hipEvent_t>& v = ctx->tev[kernel];
+  for (size_t q = 0; q + 1 < v.size(); q += 2) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, v[q], v[q + 1]));
+    t += ms;
+  }
+  *total_ms = t;
+  *launches = (long long)(v.size() / 2);
   return VIRAL_OK;
 }
 

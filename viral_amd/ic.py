@@ -120,3 +120,81 @@ def random_smooth(desc, seed, tau, e_scale=4.0, v_scale=0.3, pi_frac=0.05, prevu
         pi[l] = pi[l] / t3
     pi[9] = pi[9] / (t3 * t3)
     return assemble(desc, e, P, u, pi, prevu, tau)
+
+
+def _thickness(xs, ys, R=6.62, a=0.546, nz=801, zmax=20.0):
+    """Woods-Saxon thickness T(x,y) = int rho dz with rho0 = 1 (shape only)."""
+    z = np.linspace(-zmax, zmax, nz)
+    dz = z[1] - z[0]
+    r = np.sqrt(xs[..., None] ** 2 + ys[..., None] ** 2 + z ** 2)
+    rho = 1.0 / (1.0 + np.exp((r - R) / a))
+    return rho.sum(axis=-1) * dz
+
+
+def eta_profile(eta, plateau=1.5, falloff=1.0):
+    """Flat plateau of half-width `plateau` with Gaussian tails of width
+    `falloff` (the form of NEWEST10/ginit.h:876-879)."""
+    over = np.maximum(np.abs(eta) - plateau, 0.0)
+    return np.exp(-over * over / (2.0 * falloff * falloff))
+
+
+def glauber(desc, tau, ts, b=7.0, e_peak_gev=30.0, R=6.62, a=0.546, plateau=1.5, falloff=1.0, eos_p=None):
+    """Smooth optical-Glauber event: e(x,y,eta) = N T_A(x+b/2,y) T_B(x-b/2,y)
+    H(eta), normalised to peak e_peak_gev GeV/fm^3 (converted to fm^-4 with
+    GEVFM), fluid at rest, pi = 0, prevu = u.  The reference builds its
+    Glauber state in ginit.h (GSL integration); this is the same shape of
+    input, generated here."""
+    X, Y, E = coords(desc)
+    x1, y1 = X[:, :, 0], Y[:, :, 0]
+    TA = _thickness(x1 + b / 2.0, y1, R, a)
+    TB = _thickness(x1 - b / 2.0, y1, R, a)
+    prof = TA * TB
+    prof = prof / prof.max()
+    e = (e_peak_gev / GEVFM) * prof[:, :, None] * eta_profile(E[0, 0, :], plateau, falloff)[None, None, :]
+    e = np.maximum(e, 1e-7 * e.max())
+    P = e / 3.0 if eos_p is None else eos_p(e)
+    z = np.zeros_like(e)
+    o = np.ones_like(e)
+    u = [o, z, z, z]
+    prevu = [o.copy(), z.copy(), z.copy(), z.copy()]
+    pi = [np.zeros_like(e) for _ in range(10)]
+    return assemble(desc, e, P, u, pi, prevu, tau)
+
+
+def hotspots(desc, tau, ts, seed, n_spots=200, width=0.4, b=None, b_max=10.0, R=6.62, a=0.546, gamma_k=1.0,
+             e_scale_gev=10.0, plateau=1.5, falloff=1.0, eos_p=None):
+    """Fluctuating event: n_spots Gaussian hot spots (width fm) whose centres
+    are drawn from two Woods-Saxon nuclei at +-b/2 (b ~ U(0, b_max) unless
+    given) with Gamma(k)-distributed weights, times the plateau eta profile.
+    Seeded; fluid at rest, pi = 0."""
+    rng = np.random.default_rng(seed)
+    if b is None:
+        b = rng.uniform(0.0, b_max)
+    # sample nucleon-like positions from the Woods-Saxon density by rejection
+    pts = []
+    while len(pts) < n_spots:
+        c = rng.uniform(-R - 4 * a, R + 4 * a, size=(4 * n_spots, 3))
+        r = np.sqrt((c ** 2).sum(axis=1))
+        keep = rng.uniform(0, 1, len(r)) < 1.0 / (1.0 + np.exp((r - R) / a))
+        for p in c[keep]:
+            if len(pts) < n_spots:
+                pts.append(p)
+    pts = np.array(pts)
+    side = np.where(np.arange(n_spots) % 2 == 0, 0.5 * b, -0.5 * b)
+    cx, cy = pts[:, 0] + side, pts[:, 1]
+    w = rng.gamma(gamma_k, 1.0 / gamma_k, n_spots)
+    X, Y, E = coords(desc)
+    x1, y1 = X[:, :, 0], Y[:, :, 0]
+    tr = np.zeros_like(x1)
+    for q in range(n_spots):
+        tr += w[q] * np.exp(-((x1 - cx[q]) ** 2 + (y1 - cy[q]) ** 2) / (2 * width * width))
+    tr = tr / max(tr.max(), 1e-300)
+    e = (e_scale_gev / GEVFM) * tr[:, :, None] * eta_profile(E[0, 0, :], plateau, falloff)[None, None, :]
+    e = np.maximum(e, 1e-7 * e.max())
+    P = e / 3.0 if eos_p is None else eos_p(e)
+    z = np.zeros_like(e)
+    o = np.ones_like(e)
+    u = [o, z, z, z]
+    prevu = [o.copy(), z.copy(), z.copy(), z.copy()]
+    pi = [np.zeros_like(e) for _ in range(10)]
+    return assemble(desc, e, P, u, pi, prevu, tau)
