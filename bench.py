@@ -40,24 +40,26 @@ def eos_p_vec(desc):
     return lambda e: np.vectorize(lambda v: L.or_eos_p(desc, float(v)))(e)
 
 
-# Compulsory HBM bytes per launch of each kernel (inputs read once, outputs
-# written once), per unit of work; units per launch are faces (flux/speed) or
-# interior cells (the rest).  F = face count of one direction, C = interior
-# cells.  See the kernel table of the design notes.
-def kernel_bytes(name, C, F):
-    if name == "face_flux":      # 15 launches' worth: per variable, ~7 state fields read + 1 flux written
-        return F * 8 * (7 + 1)
-    if name == "face_speed":     # u^d, u^0 read, a written
-        return F * 8 * 3
-    if name == "divergence":     # 15 fluxes read, 15 results read+written
-        return C * 8 * 45
-    if name == "update":         # 24 state fields + 15 result + 15 Var0 (+15 Var1) read, 29 state + 30 Var written
-        return C * 8 * (24 + 15 + 30 + 29 + 30)
+# Compulsory HBM bytes per launch of each kernel: every input field read
+# once and every output written once over the cells/faces of one launch.
+#   flux (one direction, per face): 21 state fields read (e, P, PI, u^mu,
+#     pi^{mu nu}, T^{tau mu}) + 15 face fluxes written = 36 doubles = 288 B
+#   update (per interior cell, RK3 stage average): 45 face fluxes + 25 state
+#     fields + Var registers (30, 30, 30 over the three stages) read, 29 state
+#     fields + Var registers (30, 15, 0) written, + 5 B of iteration count and
+#     flags = 134 doubles + 5 B = 1077 B
+def kernel_bytes(name, d):
+    faces = [(d.nx + 1) * d.ny * d.nz, d.nx * (d.ny + 1) * d.nz, d.nx * d.ny * (d.nz + 1)]
+    cells = d.nx * d.ny * d.nz
+    if name == "flux":
+        return sum(faces) / len(faces) * 288.0
+    if name == "update":
+        return cells * 1077.0
     if name == "k0_broadcast":
-        return C * 8 * (3 + 7)
+        return cells * 8.0 * 10
     if name == "ghost_fill":
-        return 0
-    return 0
+        return 0.0
+    return 0.0
 
 
 def cpu_baseline(seconds_target=20.0):
@@ -134,11 +136,10 @@ def main():
     ms_step = dt * 1000.0 / args.steps
 
     # dominant kernel and its roofline figure
-    F = (d.nx + 1) * d.ny * d.nz  # faces of one direction (approx. all three)
     dom = max(rep, key=lambda k: rep[k][0])
     tot_ms, launches = rep[dom]
     avg_s = tot_ms / 1000.0 / max(launches, 1)
-    bytes_launch = kernel_bytes(dom, cells, F)
+    bytes_launch = kernel_bytes(dom, d)
     achieved = bytes_launch / avg_s / 1e9
     kernel_share = {k: round(v[0] / max(sum(x[0] for x in rep.values()), 1e-12), 4) for k, v in rep.items()}
     bytes_per_update = {1: 336, 2: 784, 3: 1232}[d.rk]

@@ -27,4 +27,6 @@ build() {
 build ref_kt -DKT
 build ref_kt_gubser -DKT -DGUBSER
 build ref_kt_ginit -DKT -DGINIT
+build ref_shas -DSHAS
+build ref_shas_ginit -DSHAS -DGINIT
 echo "built $(ls "$OUT" | tr '\n' ' ')"

@@ -427,6 +427,48 @@ void or_calc_source(or_ctx* c, double tau, double ts, double* S)
         S[2 * n + x] = 0;
         S[14 * n + x] = 0;
       }
+  /* CON (SHASTA) pressure/shear gradient sources, NEWEST10/Source.h:223-336:
+   * S^mu += d/dx^i of q^mu_i, WENO derivative over the 5-cell stencil */
+  if (d->scheme == VIRAL_SHASTA) {
+    int ndir = d->lbi ? 2 : 3;
+    const double hh[3] = {d->dx, d->dy, d->deta};
+    for (int dir = 0; dir < ndir; dir++) {
+      size_t st = dir == 0 ? IX(c, 1, 0, 0) : dir == 1 ? IX(c, 0, 1, 0) : 1;
+      for (int i = c->il; i < c->ir; i++)
+        for (int j = c->jl; j < c->jr; j++)
+          for (int k = c->kl; k < c->kr; k++) {
+            size_t x = IX(c, i, j, k);
+            double q[4][5];
+            for (int o = 0; o < 5; o++) {
+              size_t y = x + (o - 2) * st;
+              double P = f[VIRAL_F_P][y], PI = f[VIRAL_F_BULK][y];
+              double u0 = f[VIRAL_F_U0][y], ud = f[VIRAL_F_U1 + dir][y];
+              double p2 = f[VIRAL_F_PI0 + 1][y], p3 = f[VIRAL_F_PI0 + 2][y], p4 = f[VIRAL_F_PI0 + 3][y];
+              double tt = milne ? tau : 1.0;
+              q[0][o] = -tt * (0);
+              if (dir == 0) {
+                double p5 = f[VIRAL_F_PI0 + 4][y], p6 = f[VIRAL_F_PI0 + 5][y], p7 = f[VIRAL_F_PI0 + 6][y];
+                q[1][o] = -tt * (P - PI - (ud / u0) * p2 + p5);
+                q[2][o] = -tt * (-(ud / u0) * p3 + p6);
+                q[3][o] = -tt * (-(ud / u0) * p4 + p7);
+              } else if (dir == 1) {
+                double p6 = f[VIRAL_F_PI0 + 5][y], p8 = f[VIRAL_F_PI0 + 7][y], p9 = f[VIRAL_F_PI0 + 8][y];
+                q[1][o] = -tt * (-(ud / u0) * p2 + p6);
+                q[2][o] = -tt * (P - PI - (ud / u0) * p3 + p8);
+                q[3][o] = -tt * (-(ud / u0) * p4 + p9);
+              } else {
+                double p7 = f[VIRAL_F_PI0 + 6][y], p9 = f[VIRAL_F_PI0 + 8][y], p10 = f[VIRAL_F_PI0 + 9][y];
+                q[1][o] = -tt * (-(ud / u0) * p2 + p7);
+                q[2][o] = -tt * (-(ud / u0) * p3 + p9);
+                q[3][o] = milne ? -tt * ((P - PI) / (tau * tau) - (ud / u0) * p4 + p10)
+                                : -tt * ((P - PI) - (ud / u0) * p4 + p10);
+              }
+            }
+            for (int m = 0; m < 4; m++)
+              S[m * n + x] += or_weno_der(q[m][0], q[m][1], q[m][2], q[m][3], q[m][4], hh[dir]);
+          }
+    }
+  }
   /* velocity gradients, NEWEST10/Source.h:3-101 */
   const double hx[3] = {d->dx, d->dy, d->deta};
   for (int i = c->il; i < c->ir; i++)
@@ -520,10 +562,87 @@ static void weno_rows(or_ctx* c, int dir, const double* q, double* L, double* R)
     }
 }
 
+/* SHASTA flux-corrected transport, NEWEST10/hydroshasta.h:8-32 (transport
+ * velocities), 81-98 (stage), 101-284 (one direction): low-order transported
+ * UTD with Q+- = (1/2 -+ v lambda)/(1 +- (v_{+-1} - v) lambda), flux-limited
+ * antidiffusion A = s max(0, min(s dUTD_{+1}, |dUTD/8|, s dUTD_{-1})), and
+ * the rate (-U + UTD - A_i + A_{i-1}) / taustep. */
+static void shasta_explicit(or_ctx* c, double tau, double taustep, double* Result)
+{
+  const viral_desc* d = &c->d;
+  const int milne = d->coords == VIRAL_MILNE;
+  size_t n = c->n;
+  double** f = c->f;
+  memset(Result, 0, SVAR * n * sizeof(double));
+  for (size_t x = 0; x < n; x++) {
+    double tf = milne ? tau : 1.0;
+    for (int mu = 0; mu < 4; mu++) c->var[mu * n + x] = milne ? tf * f[VIRAL_F_T00 + mu][x] : f[VIRAL_F_T00 + mu][x];
+    for (int l = 0; l < 10; l++) c->var[(4 + l) * n + x] = f[VIRAL_F_PI0 + l][x];
+    c->var[14 * n + x] = f[VIRAL_F_BULK][x];
+  }
+  int ndir = d->lbi ? 2 : 3;
+  const double hx[3] = {d->dx, d->dy, d->deta};
+  double* NV = c->F;    /* transport velocity of one direction, [15][n] */
+  double* UTD = c->FL;  /* [n] */
+  double* AD = c->FR;   /* [n] */
+  for (int dir = 0; dir < ndir; dir++) {
+    /* CalcNumVel: row 0 moves with T^{dir,tau}/T^{tau tau}, the rest with u^dir/u^0 */
+    for (size_t x = 0; x < n; x++) {
+      double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x], PI = f[VIRAL_F_BULK][x];
+      double u0 = f[VIRAL_F_U0][x], ud = f[VIRAL_F_U1 + dir][x];
+      double pd = f[VIRAL_F_PI0 + 1 + dir][x];
+      NV[0 * n + x] = (pd + ud * u0 * (e + P - PI)) / f[VIRAL_F_T00][x];
+      for (int l = 1; l < SVAR; l++) NV[l * n + x] = ud / u0;
+    }
+    int G = dir == 0 ? c->gx : dir == 1 ? c->gy : c->gz;
+    size_t st = dir == 0 ? IX(c, 1, 0, 0) : dir == 1 ? IX(c, 0, 1, 0) : 1;
+    double lam = taustep / hx[dir];
+    int a0, a1, b0, b1;
+    if (dir == 0) { a0 = c->jl; a1 = c->jr; b0 = c->kl; b1 = c->kr; }
+    else if (dir == 1) { a0 = c->il; a1 = c->ir; b0 = c->kl; b1 = c->kr; }
+    else { a0 = c->il; a1 = c->ir; b0 = c->jl; b1 = c->jr; }
+    int lo = dir == 0 ? c->il : dir == 1 ? c->jl : c->kl;
+    int hi = dir == 0 ? c->ir : dir == 1 ? c->jr : c->kr;
+    for (int l = 0; l < SVAR; l++) {
+      const double* U = c->var + l * n;
+      const double* V = NV + l * n;
+      for (int a = a0; a < a1; a++)
+        for (int b = b0; b < b1; b++) {
+          size_t base = dir == 0 ? IX(c, 0, a, b) : dir == 1 ? IX(c, a, 0, b) : IX(c, a, b, 0);
+          for (int p = 1; p < G - 1; p++) {
+            size_t x = base + (size_t)p * st;
+            double Vc = V[x], Vp = V[x + st], Vm = V[x - st];
+            double Qp = (0.5 - Vc * lam) / (1.0 + (Vp - Vc) * lam);
+            double Qm = (0.5 + Vc * lam) / (1.0 - (Vm - Vc) * lam);
+            UTD[x] = 0.5 * ((Qp * Qp * (U[x + st] - U[x])) + (Qm * Qm * (U[x - st] - U[x]))) + (Qp + Qm) * U[x];
+          }
+          for (int p = 2; p < G - 3; p++) {
+            size_t x = base + (size_t)p * st;
+            double signa = 1;
+            double temp = 1 * 0.125 * (UTD[x + st] - UTD[x]);
+            if (temp < 0) signa = -1;
+            double term1 = signa * (UTD[x + 2 * st] - UTD[x + st]);
+            double term3 = signa * (UTD[x] - UTD[x - st]);
+            double mn = fmin(fmin(term1, fabs(temp)), term3);
+            AD[x] = signa * fmax(0, mn);
+          }
+          for (int p = lo; p < hi; p++) {
+            size_t x = base + (size_t)p * st;
+            double pr = (-U[x] + UTD[x] - AD[x] + AD[x - st]) / taustep;
+            Result[l * n + x] += pr;
+          }
+        }
+    }
+  }
+}
+
 /* NEWEST10/hydroKT.h:406-436 with fvX/fvY/fvZ (444-506) */
 void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* Result)
 {
-  (void)taustep;
+  if (c->d.scheme == VIRAL_SHASTA) {
+    shasta_explicit(c, tau, taustep, Result);
+    return;
+  }
   const viral_desc* d = &c->d;
   const int milne = d->coords == VIRAL_MILNE;
   size_t n = c->n;

@@ -5,7 +5,7 @@
  * TEST INFRASTRUCTURE.  The reference as a whole cannot be built here:
  * main.cpp, root.h and ginit.h need GSL, which the image lacks.  The other
  * step headers (maindef.h, s95p.h, alloc.h, init.h, utils.h, pack.h,
- * boundary.h, debug.h, Source.h, write.h, hydroKT.h) need only MPI, which the
+ * boundary.h, debug.h, Source.h, write.h, hydroKT.h, hydroshasta.h) need only MPI, which the
  * image has (MPICH under /opt/conda).  This harness includes them exactly as
  * NEWEST10/main.cpp:1-81 does (same defines, same order, minus the GSL
  * headers and root.h), fills the grid with a seeded state and calls
@@ -54,6 +54,9 @@ using namespace std;
 #include "write.h"
 #ifdef KT
 #include "hydroKT.h"
+#endif
+#ifdef SHAS
+#include "hydroshasta.h"
 #endif
 
 /* splitmix64, used only to fill the grid */
@@ -176,6 +179,10 @@ int main(int argc, char** argv)
   for (int l = 0; l < SVAR; l++) DUMP(c.Source[l]);
 #ifdef KT
   CalcCentreFlux(HydroGrid, tau);
+  hydroExplicit(HydroGrid, tau, ts / 2);
+  for (int l = 0; l < SVAR; l++) DUMP(c.Result[l]);
+#elif defined SHAS
+  CalcNumVel(HydroGrid, tau);
   hydroExplicit(HydroGrid, tau, ts / 2);
   for (int l = 0; l < SVAR; l++) DUMP(c.Result[l]);
 #else

@@ -7,7 +7,7 @@ import tempfile
 
 import numpy as np
 
-from viral_amd.desc import NFIELDS, make_desc, ETA_S_OVER_4PI, TPI_OVER_P, EOS_S95P
+from viral_amd.desc import NFIELDS, make_desc, ETA_S_OVER_4PI, TPI_OVER_P, EOS_S95P, SHASTA
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
@@ -20,6 +20,10 @@ PROFILES = {
     "ref_kt_ginit": dict(nx=30, ny=30, nz=101, dx=0.1, dy=0.1, deta=0.1, eos=EOS_S95P,
                          eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5),
     "ref_kt_gubser": dict(nx=100, ny=100, nz=1, lbi=1, dx=0.05, dy=0.05, deta=0.1),
+    "ref_shas": dict(nx=40, ny=40, nz=101, dx=0.1, dy=0.1, deta=0.1, scheme=SHASTA,
+                     eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5),
+    "ref_shas_ginit": dict(nx=30, ny=30, nz=101, dx=0.1, dy=0.1, deta=0.1, eos=EOS_S95P, scheme=SHASTA,
+                           eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5),
 }
 
 

@@ -99,6 +99,28 @@ def test_lbi_grid_bitwise():
     assert_bitwise(h, o, 2)
 
 
+@pytest.mark.parametrize("rk", [2, 3])
+def test_shasta_bitwise(rk):
+    """SHASTA scheme with the CON gradient sources (-DSHAS)."""
+    from viral_amd.desc import SHASTA
+    d = make_desc(17, 15, 9, scheme=SHASTA, rk=rk, eta_law=ETA_S_OVER_4PI, eta_coef=1.0,
+                  tpi_law=TPI_OVER_P, tpi_coef=1.5)
+    tau, ts = 0.7, 0.004
+    st = ic.random_smooth(d, seed=31 + rk, tau=tau)
+    h, o = run_both(d, st, tau, ts, 2)
+    assert_bitwise(h, o, rk)
+
+
+def test_shasta_s95p_lbi_bitwise():
+    from viral_amd.desc import SHASTA
+    d = make_desc(20, 18, 1, lbi=1, scheme=SHASTA, eos=EOS_S95P)
+    tau, ts = 0.6, 0.004
+    eosp = lambda e: oracle_ctypes.lib().or_eos_p(d, e)
+    st = ic.random_smooth(d, seed=41, tau=tau, e_scale=30.0, eos_p=eosp)
+    h, o = run_both(d, st, tau, ts, 2)
+    assert_bitwise(h, o, 2)
+
+
 def test_acausal_cell_reported_like_reference():
     """A cell pushed to v >= 1 is flagged and reported as the first bad cell
     in (i,j,k) order, where the reference exits (NEWEST10/root.h:396-421)."""
@@ -129,13 +151,11 @@ def test_max_temp_matches_oracle():
 # ---------------------------------------------------------------- fast mode
 # fast_math=1 uses one division per WENO reconstruction, shared smoothness
 # indicators and FMAs; the bar is the tolerance the target states: relative
-# error <= 1e-10 per step and <= 1e-8 after 100 steps against the oracle.
-
-def rel_err(g, c, f):
-    scale = np.max(np.abs(c[f]))
-    if scale == 0:
-        return np.max(np.abs(g[f]))
-    return np.max(np.abs(g[f] - c[f])) / scale
+# error <= 1e-10 per step and <= 1e-8 after 100 steps against the oracle,
+# per cell on physical scales (tests/metrics.py: e vs e, u vs u^0, pi vs e+P,
+# T^{tau mu} vs T^{tau tau}).  A one-ulp perturbation of the oracle's own
+# input moves these by ~1e-13 over 100 Gubser steps.
+import metrics
 
 
 def test_fast_mode_one_step_within_1e10():
@@ -151,12 +171,9 @@ def test_fast_mode_one_step_within_1e10():
     o.set_state(st)
     assert h.step(tau, ts) == o.step(tau, ts) == OK
     g, c = h.download(), o.state()
-    worst = max(rel_err(g, c, f) for f in range(NFIELDS))
-    assert worst <= 1e-10, worst
-    I = np.zeros(d.ghost_dims(), bool)
-    I[d.interior_slices()] = True
-    I = I.ravel()
-    assert np.max(np.abs(g[F_EN][I] - c[F_EN][I]) / c[F_EN][I]) <= 1e-10
+    errs = metrics.state_errors(d, g, c)
+    print(errs)
+    assert metrics.worst(errs) <= 1e-10, errs
     assert np.array_equal(h.flags() & 0x7, o.flags() & 0x7)
 
 
@@ -176,5 +193,6 @@ def test_fast_mode_100_steps_gubser_within_1e8():
         t = tau + s * ts
         assert h.step(t, ts) == o.step(t, ts) == OK
     g, c = h.download(), o.state()
-    worst = max(rel_err(g, c, f) for f in range(NFIELDS))
-    assert worst <= 1e-8, worst
+    errs = metrics.state_errors(d, g, c)
+    print(errs)
+    assert metrics.worst(errs) <= 1e-8, errs

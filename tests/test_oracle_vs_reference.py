@@ -15,7 +15,7 @@ import pytest
 import oracle_ctypes
 import refdata
 
-PROFILES = ["ref_kt", "ref_kt_gubser", "ref_kt_ginit"]
+PROFILES = ["ref_kt", "ref_kt_gubser", "ref_kt_ginit", "ref_shas", "ref_shas_ginit"]
 
 
 def need(name):

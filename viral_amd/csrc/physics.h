@@ -85,7 +85,7 @@ __device__ __forceinline__ double dpow(double x, double y) { return dexp(y * dlo
 
 // ---------------------------------------------------------------- parameters
 struct Params {
-  int milne, lbi, eos, eta_law, tpi_law, max_iter, rk;
+  int milne, lbi, eos, eta_law, tpi_law, max_iter, rk, scheme;
   unsigned quirks;
   double dx, dy, deta;
   double eta_coef, tpi_coef;

@@ -165,11 +165,83 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, doubl
   }
 }
 
+// SHASTA flux-corrected transport along direction d for one interior cell c
+// (NEWEST10/hydroshasta.h:8-32, 101-284): transport velocity v (row 0:
+// T^{d,tau}/T^{tau tau}, other rows u^d/u^0), low-order transported UTD at
+// c-2..c+2, antidiffusion A at c-1 and c, partial result
+//   (-U + UTD - A_c + A_{c-1}) / taustep.
+// The Q+- factors of the rows that share u^d/u^0 are computed once.
+template <bool FAST>
+__global__ void __launch_bounds__(BLK) k_shasta_dir(Geom g, Params p, int d, double tau, double taustep,
+                                                    const double* __restrict__ st, double* __restrict__ pr)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+  const int milne = p.milne;
+  const double hh = d == 0 ? p.dx : d == 1 ? p.dy : p.deta;
+  const double lam = taustep / hh;
+  double v0[7], vr[7];
+#pragma unroll
+  for (int o = 0; o < 7; o++) {
+    const long y = x + (o - 3) * s;
+    const double e = st[VIRAL_F_EN * n + y], P = st[VIRAL_F_P * n + y], PI = st[VIRAL_F_BULK * n + y];
+    const double u0 = st[VIRAL_F_U0 * n + y], ud = st[(VIRAL_F_U1 + d) * n + y];
+    const double pd = st[(VIRAL_F_PI0 + 1 + d) * n + y];
+    v0[o] = (pd + ud * u0 * (e + P - PI)) / st[VIRAL_F_T00 * n + y];
+    vr[o] = ud / u0;
+  }
+  // Q+ and Q- at stencil positions 1..5 (cells c-2..c+2)
+  double qp0[5], qm0[5], qpr[5], qmr[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const int o = m + 1;
+    qp0[m] = (0.5 - v0[o] * lam) / (1.0 + (v0[o + 1] - v0[o]) * lam);
+    qm0[m] = (0.5 + v0[o] * lam) / (1.0 - (v0[o - 1] - v0[o]) * lam);
+    qpr[m] = (0.5 - vr[o] * lam) / (1.0 + (vr[o + 1] - vr[o]) * lam);
+    qmr[m] = (0.5 + vr[o] * lam) / (1.0 - (vr[o - 1] - vr[o]) * lam);
+  }
+  const double tf = milne ? tau : 1.0;
+#pragma unroll 1
+  for (int l = 0; l < SV; l++) {
+    double U[7];
+#pragma unroll
+    for (int o = 0; o < 7; o++) {
+      const long y = x + (o - 3) * s;
+      const double vv = st[(long)l * n + y];
+      U[o] = (l < 4 && milne) ? tf * vv : vv;
+    }
+    double utd[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const int o = m + 1;
+      const double Qp = l == 0 ? qp0[m] : qpr[m], Qm = l == 0 ? qm0[m] : qmr[m];
+      utd[m] = 0.5 * ((Qp * Qp * (U[o + 1] - U[o])) + (Qm * Qm * (U[o - 1] - U[o]))) + (Qp + Qm) * U[o];
+    }
+    // antidiffusion at c-1 (uses utd[0..3]) and c (utd[1..4])
+    double A[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      double signa = 1;
+      const double temp = 1 * 0.125 * (utd[q + 2] - utd[q + 1]);
+      if (temp < 0) signa = -1;
+      const double term1 = signa * (utd[q + 3] - utd[q + 2]);
+      const double term3 = signa * (utd[q + 1] - utd[q]);
+      const double mn = fmin(fmin(term1, fabs(temp)), term3);
+      A[q] = signa * fmax(0, mn);
+    }
+    const double num = -U[3] + utd[2] - A[1] + A[0];
+    pr[l * n + x] = FAST ? num * (1.0 / taustep) : num / taustep;
+  }
+}
+
 struct StageArgs {
   double tau_src;   // time of the sources / fluxes of this stage
   double ts_src;    // step used in d_tau u = (u - prevu)/ts_src
   double tau_new;   // time the RK combination lands on
   double ts;        // full step
+  double taustep;   // SHASTA sub-step: ts, ts/2, ts/3 for RK1/2/3 (NEWEST10/time.h:314,352,411)
   int stage;        // 0,1,2
   int kind;         // 0: Var1 = Var0 + ts L; 1: RK2 final; 2: RK3 middle; 3: RK3 final
 };
@@ -208,8 +280,13 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
       const double* f = fl[dd];
 #pragma unroll
       for (int l = 0; l < SV; l++) {
-        const double df = f[l * n + x] - f[l * n + x - sd[dd]];
-        const double pr = FAST ? -df * (1.0 / hh[dd]) : -(1.0 / hh[dd]) * (df);
+        double pr;
+        if (p.scheme == VIRAL_SHASTA) {
+          pr = f[l * n + x];  // SHASTA kernels store the partial result itself
+        } else {
+          const double df = f[l * n + x] - f[l * n + x - sd[dd]];
+          pr = FAST ? -df * (1.0 / hh[dd]) : -(1.0 / hh[dd]) * (df);
+        }
         Res[l] = Res[l] + pr;
       }
     }
@@ -236,6 +313,50 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
   S[1] = 0;
   S[2] = 0;
   S[14] = 0;
+  if (p.scheme == VIRAL_SHASTA) {
+    // CON gradient sources, NEWEST10/Source.h:223-336
+    const long sd[3] = {g.sx, g.sy, 1};
+    const double hh[3] = {p.dx, p.dy, p.deta};
+    const int ndir = p.lbi ? 2 : 3;
+    const double tt = milne ? tau : 1.0;
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) {
+      if (dd >= ndir) break;
+      double q[4][5];
+#pragma unroll
+      for (int o = 0; o < 5; o++) {
+        const long y = x + (o - 2) * sd[dd];
+        const double Py = si[VIRAL_F_P * n + y], PIy = si[VIRAL_F_BULK * n + y];
+        const double u0y = si[VIRAL_F_U0 * n + y], udy = si[(VIRAL_F_U1 + dd) * n + y];
+        const double p2 = si[(VIRAL_F_PI0 + 1) * n + y], p3 = si[(VIRAL_F_PI0 + 2) * n + y],
+                     p4 = si[(VIRAL_F_PI0 + 3) * n + y];
+        q[0][o] = -tt * (0);
+        if (dd == 0) {
+          const double p5 = si[(VIRAL_F_PI0 + 4) * n + y], p6 = si[(VIRAL_F_PI0 + 5) * n + y],
+                       p7 = si[(VIRAL_F_PI0 + 6) * n + y];
+          q[1][o] = -tt * (Py - PIy - (udy / u0y) * p2 + p5);
+          q[2][o] = -tt * (-(udy / u0y) * p3 + p6);
+          q[3][o] = -tt * (-(udy / u0y) * p4 + p7);
+        } else if (dd == 1) {
+          const double p6 = si[(VIRAL_F_PI0 + 5) * n + y], p8 = si[(VIRAL_F_PI0 + 7) * n + y],
+                       p9 = si[(VIRAL_F_PI0 + 8) * n + y];
+          q[1][o] = -tt * (-(udy / u0y) * p2 + p6);
+          q[2][o] = -tt * (Py - PIy - (udy / u0y) * p3 + p8);
+          q[3][o] = -tt * (-(udy / u0y) * p4 + p9);
+        } else {
+          const double p7 = si[(VIRAL_F_PI0 + 6) * n + y], p9 = si[(VIRAL_F_PI0 + 8) * n + y],
+                       p10 = si[(VIRAL_F_PI0 + 9) * n + y];
+          q[1][o] = -tt * (-(udy / u0y) * p2 + p7);
+          q[2][o] = -tt * (-(udy / u0y) * p3 + p9);
+          q[3][o] = milne ? -tt * ((Py - PIy) / (tau * tau) - (udy / u0y) * p4 + p10)
+                          : -tt * ((Py - PIy) - (udy / u0y) * p4 + p10);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+        S[m] += wder<FAST>(q[m][0], q[m][1], q[m][2], q[m][3], q[m][4], hh[dd], 1.0 / hh[dd]);
+    }
+  }
   double du[4][4];
   const double hx[3] = {p.dx, p.dy, p.deta};
 #pragma unroll
@@ -571,7 +692,7 @@ static int check_desc(viral_ctx* ctx, const viral_desc* d)
   if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(ctx, VIRAL_ERR_ARG, "grid sizes must be positive");
   if (d->lbi && d->nz != 1) return fail(ctx, VIRAL_ERR_ARG, "lbi grids need nz == 1");
   if (d->rk < 1 || d->rk > 3) return fail(ctx, VIRAL_ERR_ARG, "rk must be 1, 2 or 3");
-  if (d->scheme != VIRAL_KT) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "only the KT scheme is implemented");
+  if (d->scheme != VIRAL_KT && d->scheme != VIRAL_SHASTA) return fail(ctx, VIRAL_ERR_ARG, "unknown scheme");
   if (d->coords != VIRAL_MILNE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "only Milne coordinates are implemented");
   if (d->eos != VIRAL_EOS_CONFORMAL && d->eos != VIRAL_EOS_S95P) return fail(ctx, VIRAL_ERR_ARG, "unknown eos");
   if (d->max_iter < 1) return fail(ctx, VIRAL_ERR_ARG, "max_iter must be >= 1");
@@ -622,6 +743,7 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     p.tpi_law = d->tpi_law;
     p.max_iter = d->max_iter;
     p.rk = d->rk;
+    p.scheme = d->scheme;
     p.quirks = d->quirks;
     p.dx = d->dx; p.dy = d->dy; p.deta = d->deta;
     p.eta_coef = d->eta_coef;
@@ -737,7 +859,15 @@ static int launch_stage(viral_ctx* ctx, Event& E, const StageArgs& a)
     Box fb = face_box(g, d);
     long nf = box_count(fb);
     KTimer t(ctx, K_FLUX);
-    if (fast)
+    if (p.scheme == VIRAL_SHASTA) {
+      long ni = box_count(interior_box(g));
+      if (fast)
+        hipLaunchKernelGGL(k_shasta_dir<true>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src,
+                           a.taustep, si, E.flux[d]);
+      else
+        hipLaunchKernelGGL(k_shasta_dir<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src,
+                           a.taustep, si, E.flux[d]);
+    } else if (fast)
       hipLaunchKernelGGL(k_flux_dir<true>, dim3(nblocks(nf)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si,
                          E.flux[d]);
     else
@@ -777,6 +907,7 @@ int viral_step(viral_ctx* ctx, double tau, double ts)
     const int rk = E.p.rk;
     StageArgs a{};
     a.ts = ts;
+    a.taustep = rk == 3 ? ts / 3.0 : rk == 2 ? ts / 2 : ts;
     // stage 1: NEWEST10/time.h:345-363 (RK2), 402-426 (RK3), 307-325 (RK1)
     a.tau_src = tau; a.ts_src = ts; a.tau_new = tau + ts; a.stage = 0; a.kind = 0;
     if (int r = launch_stage(ctx, E, a)) return r;
