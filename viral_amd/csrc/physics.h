@@ -434,6 +434,18 @@ __device__ __forceinline__ double newton_cell(const Params& p, double M2, double
 
 
 // ---------------------------------------------------------------- fast forms
+// 1/x from the hardware estimate (v_rcp_f64) refined by two Newton-Raphson
+// steps r <- r + r(1 - x r): within an ulp or two of the IEEE quotient for
+// the normal, finite denominators the fast path feeds it.
+__device__ __forceinline__ double frcp(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 // Same WENO5-JS weights written with one division per reconstruction:
 //   w_c = d_c / t_c / sum  ==  d_c P_c / sum_c d_c P_c,  t_c = (eps+b_c)^2,
 //   P_0 = t_1 t_2, P_1 = t_0 t_2, P_2 = t_0 t_1,
@@ -463,8 +475,8 @@ __device__ __forceinline__ void weno_LR_fast(double qm2, double qm1, double qc, 
   const double r2 = fma(5.0, qm1, fma(2.0, qc, -qm2));
   const double wl0 = 0.3 * P0, wl1 = 0.6 * P1, wl2 = 0.1 * P2;
   const double wr0 = 0.1 * P0, wr1 = 0.6 * P1, wr2 = 0.3 * P2;
-  L = fma(wl0, l0, fma(wl1, l1, wl2 * l2)) / (6.0 * (wl0 + wl1 + wl2));
-  R = fma(wr0, r0, fma(wr1, r1, wr2 * r2)) / (6.0 * (wr0 + wr1 + wr2));
+  L = fma(wl0, l0, fma(wl1, l1, wl2 * l2)) * frcp(6.0 * (wl0 + wl1 + wl2));
+  R = fma(wr0, r0, fma(wr1, r1, wr2 * r2)) * frcp(6.0 * (wr0 + wr1 + wr2));
 }
 
 __device__ __forceinline__ double weno_L_fast(double qm2, double qm1, double qc, double qp1, double qp2)
@@ -483,7 +495,7 @@ __device__ __forceinline__ double weno_L_fast(double qm2, double qm1, double qc,
   const double l0 = fma(2.0, qc, fma(5.0, qp1, -qp2));
   const double l1 = fma(5.0, qc, fma(2.0, qp1, -qm1));
   const double l2 = fma(2.0, qm2, fma(-7.0, qm1, 11.0 * qc));
-  return fma(wl0, l0, fma(wl1, l1, wl2 * l2)) / (6.0 * (wl0 + wl1 + wl2));
+  return fma(wl0, l0, fma(wl1, l1, wl2 * l2)) * frcp(6.0 * (wl0 + wl1 + wl2));
 }
 
 __device__ __forceinline__ double weno_R_fast(double qm2, double qm1, double qc, double qp1, double qp2)
@@ -502,7 +514,7 @@ __device__ __forceinline__ double weno_R_fast(double qm2, double qm1, double qc,
   const double r0 = fma(11.0, qc, fma(-7.0, qp1, 2.0 * qp2));
   const double r1 = fma(2.0, qm1, fma(5.0, qc, -qp1));
   const double r2 = fma(5.0, qm1, fma(2.0, qc, -qm2));
-  return fma(wr0, r0, fma(wr1, r1, wr2 * r2)) / (6.0 * (wr0 + wr1 + wr2));
+  return fma(wr0, r0, fma(wr1, r1, wr2 * r2)) * frcp(6.0 * (wr0 + wr1 + wr2));
 }
 
 template <bool FAST>

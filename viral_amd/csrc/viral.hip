@@ -106,6 +106,11 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, doubl
     pp[o] = (P - PI);
     if (d == 2 && milne) pp[o] = FAST ? pp[o] * (1.0 / (tau * tau)) : ((P - PI) / (tau * tau));
   }
+  double vd[6];
+  if (FAST) {
+#pragma unroll
+    for (int o = 0; o < 6; o++) vd[o] = ud[o] * frcp(u0[o]);
+  }
   // face speed
   double amax;
   {
@@ -115,7 +120,7 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, doubl
     for (int o = 0; o < 6; o++) {
       const int ii = d == 0 ? i + o - 2 : i, jj = d == 1 ? j + o - 2 : j, kk = d == 2 ? k + o - 2 : k;
       const bool in = interior(g, ii, jj, kk);
-      A[o] = (in || !q1) ? ud[o] / u0[o] : 0.0;
+      A[o] = (in || !q1) ? (FAST ? vd[o] : ud[o] / u0[o]) : 0.0;
     }
     const double aL = wL<FAST>(A[0], A[1], A[2], A[3], A[4]);
     const double aR = wR<FAST>(A[1], A[2], A[3], A[4], A[5]);
@@ -127,24 +132,21 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, doubl
     }
     amax = fmax(ml, mr);
   }
-  double vd[6];
-  if (FAST) {
-#pragma unroll
-    for (int o = 0; o < 6; o++) vd[o] = ud[o] / u0[o];
-  }
   const double tf = milne ? tau : 1.0;
-#pragma unroll 1
+  const double* __restrict__ base[6];
+#pragma unroll
+  for (int o = 0; o < 6; o++) base[o] = st + x + (o - 2) * s;
+#pragma unroll
   for (int l = 0; l < SV; l++) {
     double F[6], V[6];
     // field feeding the flux row: pi^{row,l} for T rows, pi[l-4] / PI otherwise
     const int ff = l < 4 ? VIRAL_F_PI0 + pidx(row, l) : (l < 14 ? VIRAL_F_PI0 + l - 4 : VIRAL_F_BULK);
 #pragma unroll
     for (int o = 0; o < 6; o++) {
-      const long y = x + (o - 2) * s;
-      const double fv = st[(long)ff * n + y];
-      const double vv = st[(long)l * n + y];  // Var field index == state field index (T00..T30, pi0..pi9, PI)
+      const double fv = base[o][(long)ff * n];
+      const double vv = base[o][(long)l * n];  // Var field index == state field index (T00..T30, pi0..pi9, PI)
       if (l < 4) {
-        const double um = st[(VIRAL_F_U0 + l) * n + y];
+        const double um = base[o][(long)(VIRAL_F_U0 + l) * n];
         double v;
         if (l == row)
           v = fv + pp[o] + ud[o] * um * w[o];
