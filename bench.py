@@ -42,7 +42,7 @@ def eos_p_vec(desc):
 
 # Compulsory HBM bytes per launch of each kernel: every input field read
 # once and every output written once over the cells/faces of one launch.
-#   flux (one direction, per face): 21 state fields read (e, P, PI, u^mu,
+#   flux_x/_y/_eta (per face): 21 state fields read (e, P, PI, u^mu,
 #     pi^{mu nu}, T^{tau mu}) + 15 face fluxes written = 36 doubles = 288 B
 #   update (per interior cell, RK3 stage average): 45 face fluxes + 25 state
 #     fields + Var registers (30, 30, 30 over the three stages) read, 29 state
@@ -51,8 +51,8 @@ def eos_p_vec(desc):
 def kernel_bytes(name, d):
     faces = [(d.nx + 1) * d.ny * d.nz, d.nx * (d.ny + 1) * d.nz, d.nx * d.ny * (d.nz + 1)]
     cells = d.nx * d.ny * d.nz
-    if name == "flux":
-        return sum(faces) / len(faces) * 288.0
+    if name.startswith("flux_"):
+        return faces[{"flux_x": 0, "flux_y": 1, "flux_eta": 2}[name]] * 288.0
     if name == "update":
         return cells * 1077.0
     if name == "k0_broadcast":
@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, run one plain 1 GiB device copy so a counter run can scale "
+                         "FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     ap.add_argument("--exact", action="store_true",
                     help="time the exact-arithmetic mode (bitwise equal to the CPU oracle) instead of fast mode")
     args = ap.parse_args()
@@ -131,6 +134,13 @@ def main():
     dt = time.perf_counter() - t0
     rep = h.timer_report()
     h.timers(False)
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cells = d.nx * d.ny * d.nz
     value = cells * args.steps / dt
     ms_step = dt * 1000.0 / args.steps

@@ -586,8 +586,9 @@ struct Event {
   unsigned long long bad_host = ~0ull;
 };
 
-enum KernelId { K_FLUX = 0, K_UPDATE, K_K0, K_GHOST, K_COUNT };
-static const char* const kKernelNames[K_COUNT] = {"flux", "update", "k0_broadcast", "ghost_fill"};
+enum KernelId { K_FLUX_X = 0, K_FLUX_Y, K_FLUX_Z, K_UPDATE, K_K0, K_GHOST, K_COUNT };
+static const char* const kKernelNames[K_COUNT] = {"flux_x", "flux_y", "flux_eta", "update", "k0_broadcast",
+                                                  "ghost_fill"};
 
 struct viral_ctx {
   std::vector<Event> ev;
@@ -860,7 +861,7 @@ static int launch_stage(viral_ctx* ctx, Event& E, const StageArgs& a)
   for (int d = 0; d < ndir; d++) {
     Box fb = face_box(g, d);
     long nf = box_count(fb);
-    KTimer t(ctx, K_FLUX);
+    KTimer t(ctx, K_FLUX_X + d);
     if (p.scheme == VIRAL_SHASTA) {
       long ni = box_count(interior_box(g));
       if (fast)
