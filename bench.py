@@ -27,7 +27,8 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 def workload_desc(rk=3, fast=1):
     from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP
     return make_desc(200, 200, 64, dx=0.1, dy=0.1, deta=0.1, rk=rk, eos=EOS_S95P,
-                     eta_law=ETA_TIMES_S, eta_coef=0.08, tpi_law=TPI_OVER_EP, tpi_coef=5.0, fast_math=fast)
+                     eta_law=ETA_TIMES_S, eta_coef=0.08, tpi_law=TPI_OVER_EP, tpi_coef=5.0, fast_math=fast,
+                     eos_table=fast)
 
 
 def eos_p_vec(desc):

@@ -104,6 +104,10 @@ typedef struct viral_desc {
   int fast_math;            /* 0: arithmetic in the reference's operation order, bitwise
                                equal to the CPU restatement; 1: reciprocal-multiply/FMA
                                forms, within 1e-10 relative per step                     */
+  int eos_table;            /* fast mode only: 1 evaluates s95p from a table (4-point
+                               interpolation, 1024 nodes per octave of e, closed form next
+                               to its branch points and below 1.2e-4 fm^-4) instead of
+                               the closed form                                            */
 } viral_desc;
 
 /* Fill *d with the reference's defaults for an nx*ny*nz Milne KT RK2 grid. */
@@ -152,6 +156,13 @@ int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* st
 /* Device reductions the reference's driver runs every step.
  * MaxTempGev (NEWEST10/utils.h:263-283): max over interior of GEVFM*T(e). */
 int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out);
+
+/* Host evaluation of the equation of state a descriptor selects: pressure,
+ * dp/de, temperature and entropy density at n energy densities (1/fm^4).
+ * The reference's EOS/DPDE/FT/FS profile inlines (NEWEST10/maindef.h:68-72,
+ * 274-278) and s95p_p/a/T/s (NEWEST10/s95p.h); with eos_table set, the
+ * same table the fast-mode kernels read. */
+int viral_eos_eval(const viral_desc* d, size_t n, const double* e, double* p, double* dpde, double* T, double* s);
 
 /* Per-kernel timing with HIP events recorded on the context's stream around
  * every launch (instrumentation; the reference only times whole steps with

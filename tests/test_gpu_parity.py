@@ -196,3 +196,23 @@ def test_fast_mode_100_steps_gubser_within_1e8():
     errs = metrics.state_errors(d, g, c)
     print(errs)
     assert metrics.worst(errs) <= 1e-8, errs
+
+
+def test_fast_mode_s95p_table_within_1e10():
+    """Fast mode with the tabulated s95p EOS against the closed-form oracle."""
+    d = make_desc(30, 28, 9, eos=EOS_S95P, fast_math=1, eos_table=1, eta_law=ETA_S_OVER_4PI, eta_coef=1.0)
+    dref = d.copy()
+    dref.fast_math = 0
+    dref.eos_table = 0
+    tau, ts = 0.6, 0.002
+    eosp = lambda e: oracle_ctypes.lib().or_eos_p(dref, e)
+    st = ic.random_smooth(d, seed=51, tau=tau, e_scale=60.0, eos_p=eosp)
+    print("case fast s95p table", flush=True)
+    h = capi.Hydro(d)
+    h.upload(st)
+    o = oracle_ctypes.Oracle(dref)
+    o.set_state(st)
+    assert h.step(tau, ts) == o.step(tau, ts) == OK
+    errs = metrics.state_errors(d, h.download(), o.state())
+    print(errs)
+    assert metrics.worst(errs) <= 1e-10, errs

@@ -19,7 +19,7 @@ EXPORTS = [
     "viral_desc_default", "viral_cells", "viral_ghost_dims", "viral_create", "viral_destroy",
     "viral_upload_state", "viral_download_state", "viral_download_field", "viral_step",
     "viral_flags", "viral_iterations", "viral_first_bad_cell", "viral_max_temp_gev",
-    "viral_timers", "viral_timer_read", "viral_kernel_count", "viral_kernel_name",
+    "viral_eos_eval", "viral_timers", "viral_timer_read", "viral_kernel_count", "viral_kernel_name",
     "viral_last_error",
 ]
 
@@ -65,6 +65,8 @@ def load():
     L.viral_first_bad_cell.argtypes = [vp, C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_uint)]
     L.viral_max_temp_gev.restype = C.c_int
     L.viral_max_temp_gev.argtypes = [vp, C.c_int, dp]
+    L.viral_eos_eval.restype = C.c_int
+    L.viral_eos_eval.argtypes = [pdesc, C.c_size_t] + [dp] * 5
     L.viral_timers.restype = C.c_int
     L.viral_timers.argtypes = [vp, C.c_int]
     L.viral_timer_read.restype = C.c_int
@@ -83,6 +85,18 @@ def _ptr_array(arrays):
     for q, a in enumerate(arrays):
         arr[q] = a.ctypes.data_as(C.POINTER(C.c_double))
     return arr
+
+
+def eos_eval(desc, e):
+    """(p, dp/de, T, s) of the descriptor's EOS at energy densities e (host)."""
+    L = load()
+    e = np.ascontiguousarray(e, dtype=np.float64).ravel()
+    outs = [np.empty_like(e) for _ in range(4)]
+    ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in [e] + outs]
+    r = L.viral_eos_eval(C.byref(desc), e.size, *ptrs)
+    if r != OK:
+        raise ViralError(r, "viral_eos_eval")
+    return outs
 
 
 class Hydro:

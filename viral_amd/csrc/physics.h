@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <math.h>
 
 namespace viral {
 
@@ -25,10 +26,10 @@ constexpr double WENO_EPS = 1e-6;              // NEWEST10/maindef.h:355
 #define VM_LN2_LO 1.90821492927058770002e-10
 #define VM_INV_LN2 1.44269504088896338700e+00
 
-__device__ __forceinline__ double dexp(double x)
+__host__ __device__ __forceinline__ double dexp(double x)
 {
   if (x != x) return x;
-  if (x > 709.0) return __builtin_huge_val();
+  if (x > 709.0) return HUGE_VAL;
   if (x < -745.0) return 0.0;
   double k = rint(x * VM_INV_LN2);
   double r = (x - k * VM_LN2_HI) - k * VM_LN2_LO;
@@ -51,11 +52,11 @@ __device__ __forceinline__ double dexp(double x)
   return ldexp(p, (int)k);
 }
 
-__device__ __forceinline__ double dlog(double x)
+__host__ __device__ __forceinline__ double dlog(double x)
 {
-  if (x != x || x < 0.0) return __builtin_nan("");
-  if (x == 0.0) return -__builtin_huge_val();
-  if (x == __builtin_huge_val()) return x;
+  if (x != x || x < 0.0) return NAN;
+  if (x == 0.0) return -HUGE_VAL;
+  if (x == HUGE_VAL) return x;
   int e;
   double m = frexp(x, &e);
   if (m < 0.70710678118654752440) {
@@ -81,9 +82,19 @@ __device__ __forceinline__ double dlog(double x)
   return de * VM_LN2_HI + (de * VM_LN2_LO + lm);
 }
 
-__device__ __forceinline__ double dpow(double x, double y) { return dexp(y * dlog(x)); }
+__host__ __device__ __forceinline__ double dpow(double x, double y) { return dexp(y * dlog(x)); }
 
 // ---------------------------------------------------------------- parameters
+// Tabulated s95p EOS: nodes e = 2^(emin+o) (1 + q/K), q = -1..K+1, in each
+// of noct octaves, holding (p, dp/de, T, s); 4-point Lagrange interpolation
+// inside an octave, closed form where a node stencil straddles a branch point
+// of the piecewise EOS (flag array, one byte per interval).
+struct EosTab {
+  const double* v;             // [noct][K+3][4]
+  const unsigned char* closed; // [noct][K]
+  int emin, noct, K;
+};
+
 struct Params {
   int milne, lbi, eos, eta_law, tpi_law, max_iter, rk, scheme;
   unsigned quirks;
@@ -91,6 +102,8 @@ struct Params {
   double eta_coef, tpi_coef;
   double factor;    // conformal FACTOR = 3*DOF*pi^2/90 (NEWEST10/maindef.h:10)
   double t_freeze;
+  int use_tab;      // fast mode with the tabulated s95p EOS
+  EosTab tab;
 };
 
 // ---------------------------------------------------------------- EOS
@@ -105,7 +118,7 @@ struct Params {
 #define S95_ES3 1.9402832534193788
 #define S95_ES4 3.7292474570977285
 
-__device__ __forceinline__ double s95p_p(double e)
+__host__ __device__ __forceinline__ double s95p_p(double e)
 {
   if (e < 1e-5) return 0.143375 * e;
   double x = e * GEVFM, r;
@@ -123,7 +136,7 @@ __device__ __forceinline__ double s95p_p(double e)
   return r / GEVFM;
 }
 
-__device__ __forceinline__ double s95p_a(double e)
+__host__ __device__ __forceinline__ double s95p_a(double e)
 {
   if (e < 1e-5) return 0.143375;
   double x = e * GEVFM, r;
@@ -144,7 +157,7 @@ __device__ __forceinline__ double s95p_a(double e)
   return r;
 }
 
-__device__ __forceinline__ double s95p_s(double e)
+__host__ __device__ __forceinline__ double s95p_s(double e)
 {
   double x = e * GEVFM, r;
   if (x < S95_ES1)
@@ -162,7 +175,7 @@ __device__ __forceinline__ double s95p_s(double e)
   return dpow(r, 0.75);
 }
 
-__device__ __forceinline__ double s95p_T(double e)
+__host__ __device__ __forceinline__ double s95p_T(double e)
 {
   double x = e * GEVFM, r;
   if (x < 0.5143939846236409)
@@ -184,6 +197,40 @@ __device__ __forceinline__ double eos_s(const Params& p, double e, double P, dou
 {
   return p.eos ? s95p_s(e) : (e + P) * (1.0 / T);
 }
+// Interpolated (p, dp/de, T, s) at e from the table; false where the table
+// does not apply (outside it, or an interval next to a branch point).
+__host__ __device__ __forceinline__ bool eos_tab_lookup(const EosTab& t, double e, double out[4])
+{
+  if (!(e >= 1e-5)) return false;
+  int E;
+  const double m = frexp(e, &E);  // e = m 2^E, m in [0.5, 1)
+  const int oct = E - 1 - t.emin;
+  if (oct < 0 || oct >= t.noct) return false;
+  const double u = (2.0 * m - 1.0) * t.K;
+  int q = (int)u;
+  if (q > t.K - 1) q = t.K - 1;
+  if (t.closed[oct * t.K + q]) return false;
+  const double x = u - q;
+  const double xm = x - 1.0, xp = x + 1.0, x2 = x - 2.0;
+  const double w0 = -(x * xm * x2) * (1.0 / 6.0);
+  const double w1 = (xp * xm * x2) * 0.5;
+  const double w2 = -(xp * x * x2) * 0.5;
+  const double w3 = (xp * x * xm) * (1.0 / 6.0);
+  const double* b = t.v + ((long)oct * (t.K + 3) + q) * 4;
+#pragma unroll
+  for (int c = 0; c < 4; c++) out[c] = fma(w0, b[c], fma(w1, b[4 + c], fma(w2, b[8 + c], w3 * b[12 + c])));
+  return true;
+}
+
+// closed-form (p, dp/de, T, s)
+__host__ __device__ __forceinline__ void s95p_all(double e, double out[4])
+{
+  out[0] = s95p_p(e);
+  out[1] = s95p_a(e);
+  out[2] = s95p_T(e);
+  out[3] = s95p_s(e);
+}
+
 // Feta / Ftaupi laws, NEWEST10/maindef.h:76-77,187-188,282-283,328-329
 __device__ __forceinline__ double eta_of(const Params& p, double s)
 {
@@ -536,6 +583,102 @@ __device__ __forceinline__ double wder(double a, double b, double c, double d, d
     return (L - R) * ih;
   }
   return weno_der(a, b, c, d, e, h);
+}
+
+
+// Fast-mode Newton: same iteration, stopping rule and period shortcut as
+// newton_cell, with reciprocal forms and, for s95p with the table, the
+// interpolated p(e) and dp/de.
+__device__ __forceinline__ void eos_pa_fast(const Params& p, double e, double& P, double& a)
+{
+  if (!p.eos) {
+    P = e * (1.0 / 3.0);
+    a = 1.0 / 3.0;
+    return;
+  }
+  double v[4];
+  if (p.use_tab && eos_tab_lookup(p.tab, e, v)) {
+    P = v[0];
+    a = v[1];
+    return;
+  }
+  P = s95p_p(e);
+  a = s95p_a(e);
+}
+
+__device__ __forceinline__ double newton_cell_fast(const Params& p, double M2, double Ttt, double pi1, double PI,
+                                                   double guess, int& iters, unsigned& flags)
+{
+  const double A = Ttt - pi1;
+  double x = guess, fv, dfv;
+  {
+    double Pe, a;
+    eos_pa_fast(p, x, Pe, a);
+    const double r = frcp(A + Pe - PI);
+    fv = x - A + M2 * r;
+    dfv = 1 - (M2 * r * r) * a;
+  }
+  int it = 0;
+  unsigned fl = 0;
+  double h1 = x, h2 = x, h3 = x, h4 = x;
+  for (;;) {
+    it++;
+    if (dfv == 0.0) {
+      fl |= 4u;
+      break;
+    }
+    double xn = x - fv * frcp(dfv);
+    double Pe, a;
+    eos_pa_fast(p, xn, Pe, a);
+    const double r = frcp(A + Pe - PI);
+    fv = xn - A + M2 * r;
+    dfv = 1 - (M2 * r * r) * a;
+    double xp = x;
+    x = xn;
+    if (!isfinite(fv) || !isfinite(dfv)) {
+      fl |= 4u;
+      break;
+    }
+    if (x == xp) break;
+    if (it >= p.max_iter) {
+      fl |= 8u;
+      break;
+    }
+    h4 = h3;
+    h3 = h2;
+    h2 = h1;
+    h1 = xp;
+    int per = (it >= 2 && x == h2) ? 2 : (it >= 3 && x == h3) ? 3 : (it >= 4 && x == h4) ? 4 : 0;
+    if (per) {
+      int rr = (p.max_iter - it) % per;
+      int q = per - rr;
+      x = q == 1 ? h1 : q == 2 ? h2 : q == 3 ? h3 : h4;
+      it = p.max_iter;
+      fl |= 8u;
+      break;
+    }
+  }
+  iters = it;
+  flags = fl;
+  return x;
+}
+
+// Fast-mode T(e) and s(e) (and P when asked) for the sources and the output.
+__device__ __forceinline__ void eos_Ts_fast(const Params& p, double e, double P, double& T, double& s)
+{
+  if (!p.eos) {
+    T = sqrt(sqrt(e * (1.0 / p.factor)));
+    s = (e + P) * frcp(T);
+    return;
+  }
+  double v[4];
+  if (p.use_tab && eos_tab_lookup(p.tab, e, v)) {
+    T = v[2];
+    s = v[3];
+    return;
+  }
+  T = s95p_T(e);
+  s = s95p_s(e);
 }
 
 }  // namespace viral

@@ -373,8 +373,13 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
       du[l][3] = 0;
   }
   {
-    double T = eos_T(p, e);
-    double s = eos_s(p, e, P, T);
+    double T, s;
+    if (FAST) {
+      eos_Ts_fast(p, e, P, T, s);
+    } else {
+      T = eos_T(p, e);
+      s = eos_s(p, e, P, T);
+    }
     double SH = eta_of(p, s);
     double tpi = tpi_of(p, SH, P, e);
     double out[10];
@@ -419,9 +424,17 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
                           (T0[3] - pi[3]) * (T0[3] - pi[3]);
   int it;
   unsigned fl;
-  double en = newton_cell(p, M2, T0[0], pi[0], PIn, e, it, fl);
-  double Temp = eos_T(p, en);
-  double Pn = eos_p(p, en);
+  double en, Temp, Pn;
+  if (FAST) {
+    en = newton_cell_fast(p, M2, T0[0], pi[0], PIn, e, it, fl);
+    double a_, s_;
+    eos_pa_fast(p, en, Pn, a_);
+    eos_Ts_fast(p, en, Pn, Temp, s_);
+  } else {
+    en = newton_cell(p, M2, T0[0], pi[0], PIn, e, it, fl);
+    Temp = eos_T(p, en);
+    Pn = eos_p(p, en);
+  }
   // ---- velocities (NEWEST10/root.h:372-424)
   double t2 = milne ? tn * tn : 1.0;
   double vx, vy, ve;
@@ -584,6 +597,8 @@ struct Event {
   uint8_t* flags = nullptr;
   unsigned long long* bad = nullptr;   // device word
   unsigned long long bad_host = ~0ull;
+  double* tab_v = nullptr;             // tabulated s95p EOS (fast mode)
+  unsigned char* tab_closed = nullptr;
 };
 
 enum KernelId { K_FLUX_X = 0, K_FLUX_Y, K_FLUX_Z, K_UPDATE, K_K0, K_GHOST, K_COUNT };
@@ -650,6 +665,52 @@ static int fail(viral_ctx* c, int code, const std::string& msg)
 static unsigned nblocks(long cells) { return (unsigned)((cells + BLK - 1) / BLK); }
 static long box_count(const Box& b) { return (long)b.ni * b.nj * b.nk; }
 
+// Host build of the tabulated s95p EOS (see EosTab in physics.h): octaves
+// 2^-13 .. 2^13 fm^-4 (1.2e-4 .. 8192 fm^-4, i.e. up to ~1600 GeV/fm^3;
+// below, the closed form's own exp(x)-1 cancellation is already ~1e-10, so
+// those energies keep it), 1024 intervals per octave, 4-point Lagrange
+// stencils (interpolation error ~1e-11 on the steepest branch); intervals whose
+// stencil [e_{q-1}, e_{q+2}] contains a branch point of s95p_p/a/s/T (or the
+// 1e-5 fm^-4 low-density switch) are flagged to use the closed form.
+static const int kTabEmin = -13, kTabNoct = 26, kTabK = 1024;
+
+static void build_eos_table(std::vector<double>& v, std::vector<unsigned char>& closed)
+{
+  const int K = kTabK, W = K + 3;
+  v.assign((size_t)kTabNoct * W * 4, 0.0);
+  closed.assign((size_t)kTabNoct * K, 0);
+  const double brk[] = {1e-5,
+                        S95_E1 / GEVFM, S95_E2 / GEVFM, S95_E3 / GEVFM, S95_E4 / GEVFM,
+                        S95_ES1 / GEVFM, S95_ES2 / GEVFM, S95_ES3 / GEVFM, S95_ES4 / GEVFM,
+                        0.5143939846236409 / GEVFM};
+  for (int o = 0; o < kTabNoct; o++) {
+    const double base = ldexp(1.0, kTabEmin + o);
+    for (int q = -1; q <= K + 1; q++) {
+      const double e = base * (1.0 + (double)q / K);
+      double out[4];
+      s95p_all(e, out);
+      for (int c = 0; c < 4; c++) v[((size_t)o * W + (q + 1)) * 4 + c] = out[c];
+    }
+    for (int q = 0; q < K; q++) {
+      const double lo = base * (1.0 + (double)(q - 1) / K), hi = base * (1.0 + (double)(q + 2) / K);
+      for (double b : brk)
+        if (b >= lo && b <= hi) closed[(size_t)o * K + q] = 1;
+    }
+  }
+}
+
+static EosTab host_tab()
+{
+  static std::vector<double> v;
+  static std::vector<unsigned char> c;
+  static bool built = false;
+  if (!built) {
+    build_eos_table(v, c);
+    built = true;
+  }
+  return EosTab{v.data(), c.data(), kTabEmin, kTabNoct, kTabK};
+}
+
 extern "C" {
 
 void viral_desc_default(viral_desc* d, int nx, int ny, int nz)
@@ -674,6 +735,7 @@ void viral_desc_default(viral_desc* d, int nx, int ny, int nz)
   d->max_iter = 5000;
   d->t_freeze = 0.0;
   d->fast_math = 0;
+  d->eos_table = 0;
 }
 
 void viral_ghost_dims(const viral_desc* d, int* gx, int* gy, int* gz)
@@ -699,6 +761,8 @@ static int check_desc(viral_ctx* ctx, const viral_desc* d)
   if (d->coords != VIRAL_MILNE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "only Milne coordinates are implemented");
   if (d->eos != VIRAL_EOS_CONFORMAL && d->eos != VIRAL_EOS_S95P) return fail(ctx, VIRAL_ERR_ARG, "unknown eos");
   if (d->max_iter < 1) return fail(ctx, VIRAL_ERR_ARG, "max_iter must be >= 1");
+  if (d->eos_table && !d->fast_math)
+    return fail(ctx, VIRAL_ERR_ARG, "the tabulated EOS is a fast-mode option (exact mode evaluates the closed form)");
   int gx, gy, gz;
   viral_ghost_dims(d, &gx, &gy, &gz);
   if (!d->lbi && d->k0 >= gz) return fail(ctx, VIRAL_ERR_ARG, "k0 outside the grid");
@@ -753,6 +817,8 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     p.tpi_coef = d->tpi_coef;
     p.factor = 3.0 * d->dof * PIE * PIE / 90.0;
     p.t_freeze = d->t_freeze;
+    p.use_tab = 0;
+    p.tab = EosTab{nullptr, nullptr, 0, 0, 0};
     size_t n = (size_t)g.n;
     bool ok = true;
     ok = ok && hipMalloc(&E.st[0], NF * n * sizeof(double)) == hipSuccess;
@@ -767,6 +833,21 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     if (!ok) {
       viral_destroy(ctx);
       return VIRAL_ERR_HIP;
+    }
+    if (d->eos_table && d->eos == VIRAL_EOS_S95P) {
+      std::vector<double> tv;
+      std::vector<unsigned char> tc;
+      build_eos_table(tv, tc);
+      ok = ok && hipMalloc(&E.tab_v, tv.size() * sizeof(double)) == hipSuccess;
+      ok = ok && hipMalloc(&E.tab_closed, tc.size()) == hipSuccess;
+      if (!ok) {
+        viral_destroy(ctx);
+        return VIRAL_ERR_HIP;
+      }
+      hipMemcpy(E.tab_v, tv.data(), tv.size() * sizeof(double), hipMemcpyHostToDevice);
+      hipMemcpy(E.tab_closed, tc.data(), tc.size(), hipMemcpyHostToDevice);
+      p.use_tab = 1;
+      p.tab = EosTab{E.tab_v, E.tab_closed, kTabEmin, kTabNoct, kTabK};
     }
     hipMemset(E.st[0], 0, NF * n * sizeof(double));
     hipMemset(E.st[1], 0, NF * n * sizeof(double));
@@ -792,6 +873,8 @@ void viral_destroy(viral_ctx* ctx)
     hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1); hipFree(E.var2);
     for (int q = 0; q < 3; q++) hipFree(E.flux[q]);
     hipFree(E.iters); hipFree(E.flags); hipFree(E.bad);
+    if (E.tab_v) hipFree(E.tab_v);
+    if (E.tab_closed) hipFree(E.tab_closed);
   }
   if (ctx->dtmp) hipFree(ctx->dtmp);
   for (int k = 0; k < K_COUNT; k++)
@@ -978,6 +1061,31 @@ int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)
   double v;
   memcpy(&v, &bits, sizeof(v));
   *out = v;
+  return VIRAL_OK;
+}
+
+int viral_eos_eval(const viral_desc* d, size_t n, const double* e, double* p, double* dpde, double* T, double* s)
+{
+  if (!d || !e || !p || !dpde || !T || !s) return VIRAL_ERR_ARG;
+  const double factor = 3.0 * d->dof * PIE * PIE / 90.0;
+  const bool tab = d->eos == VIRAL_EOS_S95P && d->eos_table;
+  EosTab t = tab ? host_tab() : EosTab{nullptr, nullptr, 0, 0, 0};
+  for (size_t q = 0; q < n; q++) {
+    const double x = e[q];
+    if (d->eos == VIRAL_EOS_S95P) {
+      double out[4];
+      if (!(tab && eos_tab_lookup(t, x, out))) s95p_all(x, out);
+      p[q] = out[0];
+      dpde[q] = out[1];
+      T[q] = out[2];
+      s[q] = out[3];
+    } else {
+      p[q] = x / 3.0;
+      dpde[q] = 1.0 / 3.0;
+      T[q] = sqrt(sqrt(x / factor));
+      s[q] = (x + p[q]) * (1.0 / T[q]);
+    }
+  }
   return VIRAL_OK;
 }
 

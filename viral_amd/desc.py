@@ -59,6 +59,7 @@ class ViralDesc(C.Structure):
         ("max_iter", C.c_int),
         ("t_freeze", C.c_double),
         ("fast_math", C.c_int),
+        ("eos_table", C.c_int),
     ]
 
     def ghost_dims(self):
@@ -107,6 +108,7 @@ def make_desc(nx, ny, nz, **kw):
     d.max_iter = 5000
     d.t_freeze = 0.0
     d.fast_math = 0
+    d.eos_table = 0
     for k, v in kw.items():
         if not hasattr(d, k):
             raise AttributeError(k)
