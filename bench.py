@@ -123,6 +123,7 @@ def main():
         r = h.step(tau, ts)
         assert r == 0, r
         tau += ts
+    tiles = h.autotune(reps=2) if d.fast_math else None   # LDS tile shape per direction, on this state
     torch.cuda.synchronize()
     h.timers(True)
     torch.cuda.synchronize()
@@ -169,7 +170,8 @@ def main():
         "data": "synthetic (optical-Glauber initial state generated on the host)",
         "config": {"workload": "viscous KT, s95p EOS, Glauber b=7 fm, Milne 200x200x64, dx=deta=0.1, "
                                "tau0=0.6, dtau=0.002, eta/s=0.08, tau_pi=5eta/(e+p), RK3",
-                   "grid": [d.nx, d.ny, d.nz], "rk": d.rk, "fast_math": d.fast_math,
+                   "grid": [d.nx, d.ny, d.nz], "rk": d.rk, "fast_math": d.fast_math, "eos_table": d.eos_table,
+                   "flux_tiles": tiles,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

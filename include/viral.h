@@ -164,6 +164,15 @@ int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out);
  * same table the fast-mode kernels read. */
 int viral_eos_eval(const viral_desc* d, size_t n, const double* e, double* p, double* dpde, double* T, double* s);
 
+/* LDS tile shape of the fast KT flux kernel of direction dir (0 x, 1 y,
+ * 2 eta): tp cells along the direction (tp-5 faces per tile) times tk lines,
+ * tp*tk <= 1024.  viral_autotune times the candidate shapes on event ev's
+ * current state (reps launches each) and keeps the fastest per direction.
+ * No counterpart in the reference (a CPU code with no tiling). */
+int viral_set_tile(viral_ctx* ctx, int ev, int dir, int tp, int tk);
+int viral_get_tile(viral_ctx* ctx, int ev, int dir, int* tp, int* tk);
+int viral_autotune(viral_ctx* ctx, int ev, int reps);
+
 /* Per-kernel timing with HIP events recorded on the context's stream around
  * every launch (instrumentation; the reference only times whole steps with
  * clock(), NEWEST10/main.cpp:107,170-171).  viral_timers(ctx, 1) clears and

@@ -216,3 +216,40 @@ def test_fast_mode_s95p_table_within_1e10():
     errs = metrics.state_errors(d, h.download(), o.state())
     print(errs)
     assert metrics.worst(errs) <= 1e-10, errs
+
+
+@pytest.mark.parametrize("tiles", [((32, 16), (64, 8), (16, 16)), ((64, 16), (128, 8), (23, 4)), ((256, 4), (16, 64), (8, 128))])
+def test_fast_flux_tile_shapes_agree(tiles):
+    """The LDS tile shape is a launch parameter only: every shape gives the
+    same fluxes (bitwise) on a grid that is not a multiple of any tile."""
+    d = make_desc(37, 29, 17, fast_math=1)
+    tau, ts = 0.9, 0.005
+    st = ic.random_smooth(d, seed=61, tau=tau)
+    print("case tiles", tiles, flush=True)
+    outs = []
+    for tl in [None, tiles]:
+        h = capi.Hydro(d)
+        h.upload(st)
+        if tl:
+            for dd, (tp, tk) in enumerate(tl):
+                h.set_tile(dd, tp, tk)
+        assert h.step(tau, ts) == OK
+        outs.append(h.download())
+    for f in range(NFIELDS):
+        assert same_bits(outs[0][f], outs[1][f]), FIELD_NAMES[f]
+
+
+def test_autotune_keeps_results():
+    d = make_desc(40, 30, 12, fast_math=1)
+    st = ic.random_smooth(d, seed=62, tau=1.0)
+    h = capi.Hydro(d)
+    h.upload(st)
+    tiles = h.autotune(reps=1)
+    assert all(tp * tk <= 1024 for tp, tk in tiles)
+    h.step(1.0, 0.005)
+    h2 = capi.Hydro(d)
+    h2.upload(st)
+    h2.step(1.0, 0.005)
+    a, b = h.download(), h2.download()
+    for f in range(NFIELDS):
+        assert same_bits(a[f], b[f]), FIELD_NAMES[f]

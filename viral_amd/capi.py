@@ -19,7 +19,7 @@ EXPORTS = [
     "viral_desc_default", "viral_cells", "viral_ghost_dims", "viral_create", "viral_destroy",
     "viral_upload_state", "viral_download_state", "viral_download_field", "viral_step",
     "viral_flags", "viral_iterations", "viral_first_bad_cell", "viral_max_temp_gev",
-    "viral_eos_eval", "viral_timers", "viral_timer_read", "viral_kernel_count", "viral_kernel_name",
+    "viral_eos_eval", "viral_set_tile", "viral_get_tile", "viral_autotune", "viral_timers", "viral_timer_read", "viral_kernel_count", "viral_kernel_name",
     "viral_last_error",
 ]
 
@@ -67,6 +67,12 @@ def load():
     L.viral_max_temp_gev.argtypes = [vp, C.c_int, dp]
     L.viral_eos_eval.restype = C.c_int
     L.viral_eos_eval.argtypes = [pdesc, C.c_size_t] + [dp] * 5
+    L.viral_set_tile.restype = C.c_int
+    L.viral_set_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.viral_get_tile.restype = C.c_int
+    L.viral_get_tile.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.viral_autotune.restype = C.c_int
+    L.viral_autotune.argtypes = [vp, C.c_int, C.c_int]
     L.viral_timers.restype = C.c_int
     L.viral_timers.argtypes = [vp, C.c_int]
     L.viral_timer_read.restype = C.c_int
@@ -177,6 +183,19 @@ class Hydro:
             return None
         self._chk(r)
         return (i.value, j.value, k.value, s.value, fl.value)
+
+    def set_tile(self, d, tp, tk, ev=0):
+        self._chk(self.L.viral_set_tile(self.h, ev, d, tp, tk))
+
+    def get_tile(self, d, ev=0):
+        tp, tk = C.c_int(), C.c_int()
+        self._chk(self.L.viral_get_tile(self.h, ev, d, C.byref(tp), C.byref(tk)))
+        return tp.value, tk.value
+
+    def autotune(self, reps=3, ev=0):
+        """Pick the fastest LDS tile shape per direction on the current state."""
+        self._chk(self.L.viral_autotune(self.h, ev, reps))
+        return [self.get_tile(d, ev) for d in range(3)]
 
     def timers(self, enable=True):
         self._chk(self.L.viral_timers(self.h, 1 if enable else 0))

@@ -167,6 +167,159 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, doubl
   }
 }
 
+// Fast-mode KT flux of direction D on an LDS tile (same physics as
+// k_flux_dir).  A block takes TK lines along D (interior in the other two
+// directions) and TP consecutive cells of each; every cell's centred flux
+// and Var of the current variable are computed once and staged in LDS, every
+// cell reconstructs its left and right face values from one set of WENO
+// smoothness indicators, and a face c+1/2 takes L from cell c and R from cell
+// c+1 through a second, double-buffered LDS array.  TP-5 faces per line per
+// tile; the tile shape is a launch parameter.  Slots map to cells with the
+// contiguous (eta) index fastest so loads coalesce in every direction.
+template <int D>
+__global__ void __launch_bounds__(BLK) k_flux_tile(Geom g, Params p, double tau, const double* __restrict__ st,
+                                                   double* __restrict__ flux, int TP, int TK)
+{
+  extern __shared__ double lds[];
+  constexpr int MAXR = 4;  // slots per thread: TP*TK <= MAXR*BLK
+  const int S = TP * TK;
+  double* sF = lds;
+  double* sV = lds + S;
+  double* rF0 = lds + 2 * S;
+  double* rV0 = lds + 3 * S;
+  double* rF1 = lds + 4 * S;
+  double* rV1 = lds + 5 * S;
+  const int na = D == 0 ? g.jr - g.jl : g.ir - g.il;  // slow line coordinate
+  const int nb = D == 2 ? g.jr - g.jl : g.kr - g.kl;  // fast line coordinate
+  const long nlines = (long)na * nb;
+  const int lo = D == 0 ? g.il : D == 1 ? g.jl : g.kl;
+  const int hi = D == 0 ? g.ir : D == 1 ? g.jr : g.kr;
+  const int GD = D == 0 ? g.gx : D == 1 ? g.gy : g.gz;
+  const int fpt = TP - 5;
+  const int nface = hi - lo + 1;
+  const int nseg = (nface + fpt - 1) / fpt;
+  const long lgrp = blockIdx.x / nseg;
+  const int sg = blockIdx.x % nseg;
+  const int c0 = lo - 1 + sg * fpt;  // first face cell of this tile
+  const int p0 = c0 - 2;             // cell at tile position 0
+  const long n = g.n;
+  const int milne = p.milne;
+  const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
+  const long sd = D == 0 ? g.sx : D == 1 ? g.sy : 1;
+  const int row = D + 1;
+  const double tf = milne ? tau : 1.0;
+
+  long xs[MAXR];
+  int pos[MAXR];
+  bool cell_ok[MAXR], face_ok[MAXR], in_[MAXR];
+  double u0v[MAXR], udv[MAXR], vdv[MAXR], wv[MAXR], ppv[MAXR], amax[MAXR], LF[MAXR], LV[MAXR];
+#pragma unroll
+  for (int r = 0; r < MAXR; r++) {
+    const int sl = threadIdx.x + r * BLK;
+    cell_ok[r] = false;
+    face_ok[r] = false;
+    in_[r] = false;
+    xs[r] = 0;
+    pos[r] = -100;
+    if (sl >= S) continue;
+    const int ps = D < 2 ? sl / TK : sl % TP;
+    const int li = D < 2 ? sl % TK : sl / TP;
+    const long line = lgrp * TK + li;
+    pos[r] = ps;
+    const int cd = p0 + ps;
+    if (line >= nlines || cd < 0 || cd >= GD) continue;
+    const int a = (int)(line / nb), b = (int)(line % nb);
+    int i, j, k;
+    if (D == 0) { i = cd; j = g.jl + a; k = g.kl + b; }
+    else if (D == 1) { i = g.il + a; j = cd; k = g.kl + b; }
+    else { i = g.il + a; j = g.jl + b; k = cd; }
+    xs[r] = cidx(g, i, j, k);
+    cell_ok[r] = true;
+    in_[r] = cd >= lo && cd < hi;
+    face_ok[r] = ps >= 2 && ps < 2 + fpt && cd <= hi - 1;
+  }
+  // common terms of each owned cell
+#pragma unroll
+  for (int r = 0; r < MAXR; r++) {
+    u0v[r] = 1.0;
+    udv[r] = vdv[r] = wv[r] = ppv[r] = 0.0;
+    amax[r] = LF[r] = LV[r] = 0.0;
+    if (!cell_ok[r]) continue;
+    const long y = xs[r];
+    const double e = st[VIRAL_F_EN * n + y], P = st[VIRAL_F_P * n + y], PI = st[VIRAL_F_BULK * n + y];
+    u0v[r] = st[VIRAL_F_U0 * n + y];
+    udv[r] = st[(VIRAL_F_U1 + D) * n + y];
+    vdv[r] = udv[r] * frcp(u0v[r]);
+    wv[r] = e + P - PI;
+    ppv[r] = (D == 2 && milne) ? (P - PI) * (1.0 / (tau * tau)) : (P - PI);
+  }
+  // quantity -1: eigenvalue -> face speeds; 0..14: flux rows
+  for (int q = -1; q < SV; q++) {
+    double* rF = (q & 1) ? rF1 : rF0;
+    double* rV = (q & 1) ? rV1 : rV0;
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) {
+      const int sl = threadIdx.x + r * BLK;
+      if (sl >= S) continue;
+      double F = 0.0, V = 0.0;
+      if (cell_ok[r]) {
+        const long y = xs[r];
+        if (q < 0) {
+          V = (in_[r] || !q1) ? vdv[r] : 0.0;
+        } else if (q < 4) {
+          const double piv = st[(long)(VIRAL_F_PI0 + pidx(row, q)) * n + y];
+          const double um = st[(long)(VIRAL_F_U0 + q) * n + y];
+          const double v = (q == row) ? piv + ppv[r] + udv[r] * um * wv[r] : piv + udv[r] * um * wv[r];
+          F = milne ? tau * v : v;
+          V = milne ? tf * st[(long)q * n + y] : st[(long)q * n + y];
+        } else {
+          V = st[(long)q * n + y];  // pi[q-4] or PI: Var row q is state field q
+          F = V * vdv[r];
+        }
+      }
+      sF[sl] = F;
+      sV[sl] = V;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) {
+      const int sl = threadIdx.x + r * BLK;
+      const int ps = pos[r];
+      if (sl >= S || ps < 2 || ps > TP - 3) continue;
+      const int st1 = D < 2 ? TK : 1;  // slot stride along the line
+      double Lq, Rq;
+      weno_LR_fast(sV[sl - 2 * st1], sV[sl - st1], sV[sl], sV[sl + st1], sV[sl + 2 * st1], Lq, Rq);
+      LV[r] = Lq;
+      rV[sl] = Rq;
+      if (q >= 0) {
+        weno_LR_fast(sF[sl - 2 * st1], sF[sl - st1], sF[sl], sF[sl + st1], sF[sl + 2 * st1], Lq, Rq);
+        LF[r] = Lq;
+        rF[sl] = Rq;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) {
+      if (!face_ok[r]) continue;
+      const int sl = threadIdx.x + r * BLK;
+      const int st1 = D < 2 ? TK : 1;
+      const double VR = rV[sl + st1];
+      if (q < 0) {
+        const int cd = p0 + pos[r];
+        double ml = 0, mr = 0;
+        if (!(q1 && cd < lo)) {
+          if (fabs(LV[r]) > ml) ml = fabs(LV[r]);
+          if (fabs(VR) > mr) mr = fabs(VR);
+        }
+        amax[r] = fmax(ml, mr);
+      } else {
+        const double FR = rF[sl + st1];
+        flux[(long)q * n + xs[r]] = fma(-0.5 * amax[r], VR - LV[r], 0.5 * (LF[r] + FR));
+      }
+    }
+  }
+}
+
 // SHASTA flux-corrected transport along direction d for one interior cell c
 // (NEWEST10/hydroshasta.h:8-32, 101-284): transport velocity v (row 0:
 // T^{d,tau}/T^{tau tau}, other rows u^d/u^0), low-order transported UTD at
@@ -597,6 +750,8 @@ struct Event {
   uint8_t* flags = nullptr;
   unsigned long long* bad = nullptr;   // device word
   unsigned long long bad_host = ~0ull;
+  int tile_tp[3] = {64, 64, 64};        // LDS tile of the fast flux kernel: cells per line
+  int tile_tk[3] = {16, 16, 4};         //   and lines per tile, per direction
   double* tab_v = nullptr;             // tabulated s95p EOS (fast mode)
   unsigned char* tab_closed = nullptr;
 };
@@ -834,6 +989,14 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
       viral_destroy(ctx);
       return VIRAL_ERR_HIP;
     }
+    {
+      // default LDS tiles: 64-cell lines x 16 lines across x and y; along eta
+      // the whole ghosted line when it fits (one tile per line)
+      const int gz = g.gz;
+      E.tile_tp[2] = gz <= 128 ? gz : 64;
+      E.tile_tk[2] = 1024 / E.tile_tp[2] > 0 ? 1024 / E.tile_tp[2] : 1;
+      if (E.tile_tp[2] * E.tile_tk[2] > 1024) E.tile_tk[2] = 1024 / E.tile_tp[2];
+    }
     if (d->eos_table && d->eos == VIRAL_EOS_S95P) {
       std::vector<double> tv;
       std::vector<unsigned char> tc;
@@ -932,6 +1095,25 @@ int viral_download_state(viral_ctx* ctx, int ev, double* const* fields)
   return VIRAL_OK;
 }
 
+// fast KT flux of direction d on LDS tiles of TP cells x TK lines
+static void launch_flux_tile(viral_ctx* ctx, Event& E, int d, double tau, const double* si, int TP, int TK)
+{
+  const Geom& g = E.g;
+  const int na = d == 0 ? g.jr - g.jl : g.ir - g.il;
+  const int nb = d == 2 ? g.jr - g.jl : g.kr - g.kl;
+  const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
+  const long nlines = (long)na * nb;
+  const int nseg = (hi - lo + 1 + TP - 6) / (TP - 5);
+  const unsigned blocks = (unsigned)(((nlines + TK - 1) / TK) * nseg);
+  const size_t shm = 6 * (size_t)TP * TK * sizeof(double);
+  if (d == 0)
+    hipLaunchKernelGGL(k_flux_tile<0>, dim3(blocks), dim3(BLK), shm, ctx->stream, g, E.p, tau, si, E.flux[d], TP, TK);
+  else if (d == 1)
+    hipLaunchKernelGGL(k_flux_tile<1>, dim3(blocks), dim3(BLK), shm, ctx->stream, g, E.p, tau, si, E.flux[d], TP, TK);
+  else
+    hipLaunchKernelGGL(k_flux_tile<2>, dim3(blocks), dim3(BLK), shm, ctx->stream, g, E.p, tau, si, E.flux[d], TP, TK);
+}
+
 // one stage on one event: fluxes at tau_src from copy `in`, update into `out`
 static int launch_stage(viral_ctx* ctx, Event& E, const StageArgs& a)
 {
@@ -953,10 +1135,9 @@ static int launch_stage(viral_ctx* ctx, Event& E, const StageArgs& a)
       else
         hipLaunchKernelGGL(k_shasta_dir<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src,
                            a.taustep, si, E.flux[d]);
-    } else if (fast)
-      hipLaunchKernelGGL(k_flux_dir<true>, dim3(nblocks(nf)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si,
-                         E.flux[d]);
-    else
+    } else if (fast) {
+      launch_flux_tile(ctx, E, d, a.tau_src, si, E.tile_tp[d], E.tile_tk[d]);
+    } else
       hipLaunchKernelGGL(k_flux_dir<false>, dim3(nblocks(nf)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si,
                          E.flux[d]);
   }
@@ -1086,6 +1267,74 @@ int viral_eos_eval(const viral_desc* d, size_t n, const double* e, double* p, do
       s[q] = (x + p[q]) * (1.0 / T[q]);
     }
   }
+  return VIRAL_OK;
+}
+
+static bool tile_ok(const Event& E, int d, int TP, int TK)
+{
+  if (TP < 8 || TK < 1 || TP * TK > 4 * BLK) return false;
+  const int G = d == 0 ? E.g.gx : d == 1 ? E.g.gy : E.g.gz;
+  return TP <= G + 8;
+}
+
+int viral_set_tile(viral_ctx* ctx, int ev, int dir, int tp, int tk)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || dir < 0 || dir > 2 || !tile_ok(*E, dir, tp, tk)) return fail(ctx, VIRAL_ERR_ARG, "bad tile");
+  E->tile_tp[dir] = tp;
+  E->tile_tk[dir] = tk;
+  return VIRAL_OK;
+}
+
+int viral_get_tile(viral_ctx* ctx, int ev, int dir, int* tp, int* tk)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || dir < 0 || dir > 2 || !tp || !tk) return fail(ctx, VIRAL_ERR_ARG, "bad tile query");
+  *tp = E->tile_tp[dir];
+  *tk = E->tile_tk[dir];
+  return VIRAL_OK;
+}
+
+int viral_autotune(viral_ctx* ctx, int ev, int reps)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || reps < 1) return fail(ctx, VIRAL_ERR_ARG, "bad autotune call");
+  if (!E->d.fast_math || E->d.scheme != VIRAL_KT) return VIRAL_OK;  // only the fast KT flux is tiled
+  static const int shapes[][2] = {{32, 16}, {32, 32}, {64, 8}, {64, 16}, {128, 4}, {128, 8}, {256, 4}, {16, 64}};
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  const int ndir = E->p.lbi ? 2 : 3;
+  for (int d = 0; d < ndir; d++) {
+    std::vector<std::pair<int, int>> cand;
+    for (auto& sh : shapes) cand.push_back({sh[0], sh[1]});
+    if (d == 2) {  // whole eta lines
+      const int tp = E->g.gz;
+      for (int tk = 1; tp * tk <= 4 * BLK; tk *= 2) cand.push_back({tp, tk});
+    }
+    float best = 1e30f;
+    int btp = E->tile_tp[d], btk = E->tile_tk[d];
+    for (auto& c : cand) {
+      if (!tile_ok(*E, d, c.first, c.second)) continue;
+      launch_flux_tile(ctx, *E, d, 1.0, E->st[E->cur], c.first, c.second);  // warm
+      HIPCHK(hipEventRecord(t0, ctx->stream));
+      for (int r = 0; r < reps; r++) launch_flux_tile(ctx, *E, d, 1.0, E->st[E->cur], c.first, c.second);
+      HIPCHK(hipEventRecord(t1, ctx->stream));
+      HIPCHK(hipEventSynchronize(t1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+      if (ms < best) {
+        best = ms;
+        btp = c.first;
+        btk = c.second;
+      }
+    }
+    E->tile_tp[d] = btp;
+    E->tile_tk[d] = btk;
+  }
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  HIPCHK(hipGetLastError());
   return VIRAL_OK;
 }
 
