@@ -327,7 +327,11 @@ static void shear_source(int milne, double tau, const double u[4], const double 
     for (int n = 0; n < 4; n++) pi[m][n] = pi10[PIDX[m][n]];
   double ul[4];
   for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
-  /* covariant derivative Dc[a][b] = d_b u^a */
+  /* d_i(u^i/u^0) from the partial derivatives */
+  double iu0 = 1.0 / u[0];
+  double divv = 0;
+  for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
+  /* covariant derivative Dc[a][b] = d_b u^a (Christoffel terms added) */
   double Dc[4][4];
   for (int a = 0; a < 4; a++)
     for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
@@ -338,38 +342,6 @@ static void shear_source(int milne, double tau, const double u[4], const double 
     Dc[3][3] = Dc[3][3] + itau * u[0];
   }
   double theta = Dc[0][0] + Dc[1][1] + Dc[2][2] + Dc[3][3];
-  /* G^{ab} = g^{aa} d_a u^b */
-  double G[4][4];
-  for (int a = 0; a < 4; a++)
-    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
-  /* projectors */
-  double Pm[4][4]; /* Delta^m_a */
-  double Pu[4][4]; /* Delta^{mn} */
-  double Pl[4][4]; /* Delta_{ab} */
-  for (int m = 0; m < 4; m++)
-    for (int a = 0; a < 4; a++) {
-      Pm[m][a] = (m == a ? 1.0 : 0.0) - u[m] * ul[a];
-      Pu[m][a] = (m == a ? gi[m] : 0.0) - u[m] * u[a];
-      Pl[m][a] = (m == a ? g[m] : 0.0) - ul[m] * ul[a];
-    }
-  /* trace Delta_{ab} G^{ab} */
-  double tr = 0;
-  for (int a = 0; a < 4; a++)
-    for (int b = 0; b < 4; b++) tr += Pl[a][b] * G[a][b];
-  /* M = Pm G, A = M Pm^T */
-  double M[4][4], Am[4][4];
-  for (int m = 0; m < 4; m++)
-    for (int b = 0; b < 4; b++) {
-      double s = 0;
-      for (int a = 0; a < 4; a++) s += Pm[m][a] * G[a][b];
-      M[m][b] = s;
-    }
-  for (int m = 0; m < 4; m++)
-    for (int n = 0; n < 4; n++) {
-      double s = 0;
-      for (int b = 0; b < 4; b++) s += M[m][b] * Pm[n][b];
-      Am[m][n] = s;
-    }
   /* D u_a = g_aa u^b d_b u^a */
   double Dul[4];
   for (int a = 0; a < 4; a++) {
@@ -377,15 +349,43 @@ static void shear_source(int milne, double tau, const double u[4], const double 
     for (int b = 0; b < 4; b++) s += u[b] * Dc[a][b];
     Dul[a] = g[a] * s;
   }
-  /* d_i(u^i/u^0) */
-  double iu0 = 1.0 / u[0];
-  double divv = 0;
-  for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
+  /* G^{ab} = g^{aa} d_a u^b */
+  double G[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
+  /* Delta^{mn}_{ab} G^{ab} with Delta^m_a = delta - u^m u_a expands to
+   *   G^{mn} - u^n (G u_)^m - u^m (u_ G)^n + u^m u^n (u_ G u_),
+   * and Delta_{ab} G^{ab} = sum_a g_aa G^{aa} - u_ G u_ */
+  double Gu[4], uG[4], H[4];
+  for (int m = 0; m < 4; m++) {
+    double s1 = 0, s2 = 0;
+    for (int b = 0; b < 4; b++) {
+      s1 += G[m][b] * ul[b];
+      s2 += ul[b] * G[b][m];
+    }
+    Gu[m] = s1;
+    uG[m] = s2;
+  }
+  double uGu = 0, trg = 0;
+  for (int a = 0; a < 4; a++) {
+    uGu += ul[a] * Gu[a];
+    trg += g[a] * G[a][a];
+  }
+  double tr = trg - uGu;
+  for (int m = 0; m < 4; m++) H[m] = Gu[m] + uG[m];
+  double c3 = uGu + (1.0 / 3.0) * tr;
+  /* pi^{ma} D u_a */
+  double PD[4];
+  for (int m = 0; m < 4; m++) {
+    double s = 0;
+    for (int a = 0; a < 4; a++) s += pi[m][a] * Dul[a];
+    PD[m] = s;
+  }
   for (int m = 0; m < 4; m++)
     for (int n = m; n < 4; n++) {
-      double sigma = 0.5 * (Am[m][n] + Am[n][m]) - (1.0 / 3.0) * Pu[m][n] * tr;
-      double pu = 0;
-      for (int a = 0; a < 4; a++) pu += (u[m] * pi[n][a] + u[n] * pi[m][a]) * Dul[a];
+      double sigma = 0.5 * (G[m][n] + G[n][m]) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
+      if (m == n) sigma = sigma - (1.0 / 3.0) * gi[m] * tr;
+      double pu = u[m] * PD[n] + u[n] * PD[m];
       double rhs = -(pi[m][n] - 2.0 * SH * sigma) / tpi - (4.0 / 3.0) * pi[m][n] * theta - pu;
       double chr = 0;
       if (milne) {

@@ -319,102 +319,9 @@ __device__ __forceinline__ int pidx(int m, int n)
   return a == 0 ? b : a == 1 ? 3 + b : a == 2 ? 5 + b : 9;
 }
 
+template <bool FAST = false>
 __device__ __forceinline__ void shear_source(int milne, double tau, const double u[4], const double pi10[10],
-                                             const double du[4][4], double SH, double tpi, double out[10])
-{
-  double g[4] = {1.0, -1.0, -1.0, milne ? -(tau * tau) : -1.0};
-  double gi[4] = {1.0, -1.0, -1.0, milne ? -(1.0 / (tau * tau)) : -1.0};
-  double pi[4][4];
-#pragma unroll
-  for (int m = 0; m < 4; m++)
-#pragma unroll
-    for (int n = 0; n < 4; n++) pi[m][n] = pi10[pidx(m, n)];
-  double ul[4];
-#pragma unroll
-  for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
-  double Dc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
-  double itau = milne ? 1.0 / tau : 0.0;
-  if (milne) {
-    Dc[0][3] = Dc[0][3] + tau * u[3];
-    Dc[3][0] = Dc[3][0] + itau * u[3];
-    Dc[3][3] = Dc[3][3] + itau * u[0];
-  }
-  double theta = Dc[0][0] + Dc[1][1] + Dc[2][2] + Dc[3][3];
-  double G[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
-  double Pm[4][4], Pu[4][4], Pl[4][4];
-#pragma unroll
-  for (int m = 0; m < 4; m++)
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-      Pm[m][a] = (m == a ? 1.0 : 0.0) - u[m] * ul[a];
-      Pu[m][a] = (m == a ? gi[m] : 0.0) - u[m] * u[a];
-      Pl[m][a] = (m == a ? g[m] : 0.0) - ul[m] * ul[a];
-    }
-  double tr = 0;
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) tr += Pl[a][b] * G[a][b];
-  double M[4][4], Am[4][4];
-#pragma unroll
-  for (int m = 0; m < 4; m++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      double s = 0;
-#pragma unroll
-      for (int a = 0; a < 4; a++) s += Pm[m][a] * G[a][b];
-      M[m][b] = s;
-    }
-#pragma unroll
-  for (int m = 0; m < 4; m++)
-#pragma unroll
-    for (int n = 0; n < 4; n++) {
-      double s = 0;
-#pragma unroll
-      for (int b = 0; b < 4; b++) s += M[m][b] * Pm[n][b];
-      Am[m][n] = s;
-    }
-  double Dul[4];
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    double s = 0;
-#pragma unroll
-    for (int b = 0; b < 4; b++) s += u[b] * Dc[a][b];
-    Dul[a] = g[a] * s;
-  }
-  double iu0 = 1.0 / u[0];
-  double divv = 0;
-#pragma unroll
-  for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
-#pragma unroll
-  for (int m = 0; m < 4; m++)
-#pragma unroll
-    for (int n = m; n < 4; n++) {
-      double sigma = 0.5 * (Am[m][n] + Am[n][m]) - (1.0 / 3.0) * Pu[m][n] * tr;
-      double pu = 0;
-#pragma unroll
-      for (int a = 0; a < 4; a++) pu += (u[m] * pi[n][a] + u[n] * pi[m][a]) * Dul[a];
-      double rhs = -(pi[m][n] - 2.0 * SH * sigma) / tpi - (4.0 / 3.0) * pi[m][n] * theta - pu;
-      double chr = 0;
-      if (milne) {
-        double cm = 0, cn = 0;
-        if (m == 0) cm = u[3] * tau * pi[3][n];
-        if (m == 3) cm = u[0] * itau * pi[3][n] + u[3] * itau * pi[0][n];
-        if (n == 0) cn = u[3] * tau * pi[m][3];
-        if (n == 3) cn = u[0] * itau * pi[m][3] + u[3] * itau * pi[m][0];
-        chr = cm + cn;
-      }
-      out[pidx(m, n)] = (rhs - chr) * iu0 + pi[m][n] * divv;
-    }
-}
+                                             const double du[4][4], double SH, double tpi, double out[10]);
 
 // ---------------------------------------------------------------- root solve
 // f(e), f'(e) of NEWEST10/root.h:212-285 and the GSL Newton iteration
@@ -679,6 +586,104 @@ __device__ __forceinline__ void eos_Ts_fast(const Params& p, double e, double P,
   }
   T = s95p_T(e);
   s = s95p_s(e);
+}
+
+
+// Israel-Stewart shear source, register-light form of the double projector:
+//   Delta^{mn}_{ab} G^{ab} = G^{mn} - u^n (G u_)^m - u^m (u_ G)^n + u^m u^n (u_ G u_),
+//   Delta_{ab} G^{ab} = sum_a g_aa G^{aa} - u_ G u_,   G^{ab} = g^{aa} d_a u^b.
+// Exact mode follows the CPU restatement's operation order; FAST replaces
+// the divisions by reciprocals.
+template <bool FAST>
+__device__ __forceinline__ void shear_source(int milne, double tau, const double u[4], const double pi10[10],
+                                             const double du[4][4], double SH, double tpi, double out[10])
+{
+  const double g[4] = {1.0, -1.0, -1.0, milne ? -(tau * tau) : -1.0};
+  const double gi[4] = {1.0, -1.0, -1.0, milne ? -(1.0 / (tau * tau)) : -1.0};
+  double ul[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
+  const double iu0 = FAST ? frcp(u[0]) : 1.0 / u[0];
+  double divv = 0;
+#pragma unroll
+  for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
+  double Dc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
+  const double itau = milne ? 1.0 / tau : 0.0;
+  if (milne) {
+    Dc[0][3] = Dc[0][3] + tau * u[3];
+    Dc[3][0] = Dc[3][0] + itau * u[3];
+    Dc[3][3] = Dc[3][3] + itau * u[0];
+  }
+  const double theta = Dc[0][0] + Dc[1][1] + Dc[2][2] + Dc[3][3];
+  double Dul[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double s = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) s += u[b] * Dc[a][b];
+    Dul[a] = g[a] * s;
+  }
+  double G[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
+  double Gu[4], uG[4], H[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    double s1 = 0, s2 = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      s1 += G[m][b] * ul[b];
+      s2 += ul[b] * G[b][m];
+    }
+    Gu[m] = s1;
+    uG[m] = s2;
+  }
+  double uGu = 0, trg = 0;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    uGu += ul[a] * Gu[a];
+    trg += g[a] * G[a][a];
+  }
+  const double tr = trg - uGu;
+#pragma unroll
+  for (int m = 0; m < 4; m++) H[m] = Gu[m] + uG[m];
+  const double c3 = uGu + (1.0 / 3.0) * tr;
+  double PD[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    double s = 0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) s += pi10[pidx(m, a)] * Dul[a];
+    PD[m] = s;
+  }
+  const double itpi = FAST ? frcp(tpi) : 0.0;
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int n = m; n < 4; n++) {
+      const double pmn = pi10[pidx(m, n)];
+      double sigma = 0.5 * (G[m][n] + G[n][m]) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
+      if (m == n) sigma = sigma - (1.0 / 3.0) * gi[m] * tr;
+      const double pu = u[m] * PD[n] + u[n] * PD[m];
+      const double rel = FAST ? -(pmn - 2.0 * SH * sigma) * itpi : -(pmn - 2.0 * SH * sigma) / tpi;
+      const double rhs = rel - (4.0 / 3.0) * pmn * theta - pu;
+      double chr = 0;
+      if (milne) {
+        double cm = 0, cn = 0;
+        if (m == 0) cm = u[3] * tau * pi10[pidx(3, n)];
+        if (m == 3) cm = u[0] * itau * pi10[pidx(3, n)] + u[3] * itau * pi10[pidx(0, n)];
+        if (n == 0) cn = u[3] * tau * pi10[pidx(m, 3)];
+        if (n == 3) cn = u[0] * itau * pi10[pidx(m, 3)] + u[3] * itau * pi10[pidx(m, 0)];
+        chr = cm + cn;
+      }
+      out[pidx(m, n)] = (rhs - chr) * iu0 + pmn * divv;
+    }
 }
 
 }  // namespace viral

@@ -401,18 +401,11 @@ struct StageArgs {
   int kind;         // 0: Var1 = Var0 + ts L; 1: RK2 final; 2: RK3 middle; 3: RK3 final
 };
 
-// Flux divergence, sources, L = Result + Source, the RK combination, T/pi
-// from Var, the Newton root and the velocities for one interior cell
-// (NEWEST10/hydroKT.h:393-403,458-504; Source.h:195-536; time.h:2-180;
-// root.h:301-424).  Reads the stage-start state `si` and the face fluxes of
-// the three directions, writes the new state `so`.
+// Geometric, CON-gradient and Israel-Stewart sources of one interior cell
+// (NEWEST10/Source.h:3-101,195-536) from the stage-start state, into src.
 template <bool FAST>
-__global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, const double* __restrict__ si,
-                                                double* __restrict__ so, const double* __restrict__ fx,
-                                                const double* __restrict__ fy, const double* __restrict__ fz,
-                                                double* __restrict__ var0, double* __restrict__ var1,
-                                                double* __restrict__ var2, int32_t* __restrict__ iters,
-                                                uint8_t* __restrict__ flags, unsigned long long* __restrict__ bad)
+__global__ void __launch_bounds__(BLK) k_source(Geom g, Params p, StageArgs a, const double* __restrict__ si,
+                                                double* __restrict__ src)
 {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
@@ -420,32 +413,6 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
   const long n = g.n, x = cidx(g, i, j, k);
   const int milne = p.milne;
   const double tau = a.tau_src;
-  // ---- Result = sum over directions of -(1/h)(f(c+1/2) - f(c-1/2)), x then y then eta
-  double Res[SV];
-  {
-    const double* fl[3] = {fx, fy, fz};
-    const long sd[3] = {g.sx, g.sy, 1};
-    const double hh[3] = {p.dx, p.dy, p.deta};
-    const int ndir = p.lbi ? 2 : 3;
-#pragma unroll
-    for (int l = 0; l < SV; l++) Res[l] = 0.0;
-#pragma unroll
-    for (int dd = 0; dd < 3; dd++) {
-      if (dd >= ndir) break;
-      const double* f = fl[dd];
-#pragma unroll
-      for (int l = 0; l < SV; l++) {
-        double pr;
-        if (p.scheme == VIRAL_SHASTA) {
-          pr = f[l * n + x];  // SHASTA kernels store the partial result itself
-        } else {
-          const double df = f[l * n + x] - f[l * n + x - sd[dd]];
-          pr = FAST ? -df * (1.0 / hh[dd]) : -(1.0 / hh[dd]) * (df);
-        }
-        Res[l] = Res[l] + pr;
-      }
-    }
-  }
   // ---- sources
   double S[SV];
   const double e = si[VIRAL_F_EN * n + x], P = si[VIRAL_F_P * n + x], PI = si[VIRAL_F_BULK * n + x];
@@ -536,9 +503,69 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
     double SH = eta_of(p, s);
     double tpi = tpi_of(p, SH, P, e);
     double out[10];
-    shear_source(milne, tau, u, pi, du, SH, tpi, out);
+    shear_source<FAST>(milne, tau, u, pi, du, SH, tpi, out);
 #pragma unroll
     for (int l = 0; l < 10; l++) S[4 + l] = out[l];
+  }
+#pragma unroll
+  for (int l = 0; l < SV; l++) src[l * n + x] = S[l];
+}
+
+// Flux divergence, L = Result + Source, the RK combination, T/pi from Var,
+// the Newton root and the velocities for one interior cell
+// (NEWEST10/hydroKT.h:393-403,458-504; time.h:2-180; root.h:301-424).  Reads
+// the stage-start state `si`, the face fluxes of the three directions and the
+// sources, writes the new state `so`.
+template <bool FAST>
+__global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, const double* __restrict__ si,
+                                                double* __restrict__ so, const double* __restrict__ fx,
+                                                const double* __restrict__ fy, const double* __restrict__ fz,
+                                                const double* __restrict__ src, double* __restrict__ var0,
+                                                double* __restrict__ var1, double* __restrict__ var2,
+                                                int32_t* __restrict__ iters, uint8_t* __restrict__ flags,
+                                                unsigned long long* __restrict__ bad)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  const long n = g.n, x = cidx(g, i, j, k);
+  const int milne = p.milne;
+  const double tau = a.tau_src;
+  // ---- Result = sum over directions of -(1/h)(f(c+1/2) - f(c-1/2)), x then y then eta
+  double Res[SV];
+  {
+    const double* fl[3] = {fx, fy, fz};
+    const long sd[3] = {g.sx, g.sy, 1};
+    const double hh[3] = {p.dx, p.dy, p.deta};
+    const int ndir = p.lbi ? 2 : 3;
+#pragma unroll
+    for (int l = 0; l < SV; l++) Res[l] = 0.0;
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) {
+      if (dd >= ndir) break;
+      const double* f = fl[dd];
+#pragma unroll
+      for (int l = 0; l < SV; l++) {
+        double pr;
+        if (p.scheme == VIRAL_SHASTA) {
+          pr = f[l * n + x];  // SHASTA kernels store the partial result itself
+        } else {
+          const double df = f[l * n + x] - f[l * n + x - sd[dd]];
+          pr = FAST ? -df * (1.0 / hh[dd]) : -(1.0 / hh[dd]) * (df);
+        }
+        Res[l] = Res[l] + pr;
+      }
+    }
+  }
+  double S[SV];
+#pragma unroll
+  for (int l = 0; l < SV; l++) S[l] = src[l * n + x];
+  const double e = si[VIRAL_F_EN * n + x];
+  double u[4], pu[4], pi[10];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    u[l] = si[(VIRAL_F_U0 + l) * n + x];
+    pu[l] = si[(VIRAL_F_PU0 + l) * n + x];
   }
   // ---- RK combination (NEWEST10/time.h:2-98)
   double V[SV];
@@ -746,6 +773,7 @@ struct Event {
   double* var1 = nullptr;
   double* var2 = nullptr;
   double* flux[3] = {nullptr, nullptr, nullptr};  // face fluxes of x, y, eta: [15][n] each
+  double* src = nullptr;                          // sources of the stage: [15][n]
   int32_t* iters = nullptr;  // 3*n
   uint8_t* flags = nullptr;
   unsigned long long* bad = nullptr;   // device word
@@ -756,9 +784,9 @@ struct Event {
   unsigned char* tab_closed = nullptr;
 };
 
-enum KernelId { K_FLUX_X = 0, K_FLUX_Y, K_FLUX_Z, K_UPDATE, K_K0, K_GHOST, K_COUNT };
-static const char* const kKernelNames[K_COUNT] = {"flux_x", "flux_y", "flux_eta", "update", "k0_broadcast",
-                                                  "ghost_fill"};
+enum KernelId { K_FLUX_X = 0, K_FLUX_Y, K_FLUX_Z, K_SOURCE, K_UPDATE, K_K0, K_GHOST, K_COUNT };
+static const char* const kKernelNames[K_COUNT] = {"flux_x", "flux_y", "flux_eta", "source", "update",
+                                                  "k0_broadcast", "ghost_fill"};
 
 struct viral_ctx {
   std::vector<Event> ev;
@@ -982,6 +1010,7 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     ok = ok && hipMalloc(&E.var1, SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.var2, SV * n * sizeof(double)) == hipSuccess;
     for (int q = 0; q < 3; q++) ok = ok && hipMalloc(&E.flux[q], SV * n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&E.src, SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.iters, 3 * n * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc(&E.flags, n) == hipSuccess;
     ok = ok && hipMalloc(&E.bad, sizeof(unsigned long long)) == hipSuccess;
@@ -1035,6 +1064,7 @@ void viral_destroy(viral_ctx* ctx)
   for (Event& E : ctx->ev) {
     hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1); hipFree(E.var2);
     for (int q = 0; q < 3; q++) hipFree(E.flux[q]);
+    hipFree(E.src);
     hipFree(E.iters); hipFree(E.flags); hipFree(E.bad);
     if (E.tab_v) hipFree(E.tab_v);
     if (E.tab_closed) hipFree(E.tab_closed);
@@ -1143,13 +1173,22 @@ static int launch_stage(viral_ctx* ctx, Event& E, const StageArgs& a)
   }
   long ni = box_count(interior_box(g));
   {
+    KTimer t(ctx, K_SOURCE);
+    if (fast)
+      hipLaunchKernelGGL(k_source<true>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, E.src);
+    else
+      hipLaunchKernelGGL(k_source<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, E.src);
+  }
+  {
     KTimer t(ctx, K_UPDATE);
     if (fast)
       hipLaunchKernelGGL(k_update<true>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, so, E.flux[0],
-                         E.flux[1], E.flux[2], E.var0, E.var1, E.var2, E.iters + (long)a.stage * g.n, E.flags, E.bad);
+                         E.flux[1], E.flux[2], E.src, E.var0, E.var1, E.var2, E.iters + (long)a.stage * g.n, E.flags,
+                         E.bad);
     else
       hipLaunchKernelGGL(k_update<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, so, E.flux[0],
-                         E.flux[1], E.flux[2], E.var0, E.var1, E.var2, E.iters + (long)a.stage * g.n, E.flags, E.bad);
+                         E.flux[1], E.flux[2], E.src, E.var0, E.var1, E.var2, E.iters + (long)a.stage * g.n, E.flags,
+                         E.bad);
   }
   if ((p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi) {
     KTimer t(ctx, K_K0);
@@ -1270,11 +1309,13 @@ int viral_eos_eval(const viral_desc* d, size_t n, const double* e, double* p, do
   return VIRAL_OK;
 }
 
+// any tile with at least 3 faces per line and at most 4 cells per thread
+// (cells past the end of a line are masked)
 static bool tile_ok(const Event& E, int d, int TP, int TK)
 {
-  if (TP < 8 || TK < 1 || TP * TK > 4 * BLK) return false;
-  const int G = d == 0 ? E.g.gx : d == 1 ? E.g.gy : E.g.gz;
-  return TP <= G + 8;
+  (void)E;
+  (void)d;
+  return TP >= 8 && TK >= 1 && TP * TK <= 4 * BLK;
 }
 
 int viral_set_tile(viral_ctx* ctx, int ev, int dir, int tp, int tk)
@@ -1314,8 +1355,9 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps)
     }
     float best = 1e30f;
     int btp = E->tile_tp[d], btk = E->tile_tk[d];
+    const int G = d == 0 ? E->g.gx : d == 1 ? E->g.gy : E->g.gz;
     for (auto& c : cand) {
-      if (!tile_ok(*E, d, c.first, c.second)) continue;
+      if (!tile_ok(*E, d, c.first, c.second) || c.first > G + 8) continue;
       launch_flux_tile(ctx, *E, d, 1.0, E->st[E->cur], c.first, c.second);  // warm
       HIPCHK(hipEventRecord(t0, ctx->stream));
       for (int r = 0; r < reps; r++) launch_flux_tile(ctx, *E, d, 1.0, E->st[E->cur], c.first, c.second);
