@@ -45,21 +45,33 @@ def eos_p_vec(desc):
 # once and every output written once over the cells/faces of one launch.
 #   flux_x/_y/_eta (per face): 21 state fields read (e, P, PI, u^mu,
 #     pi^{mu nu}, T^{tau mu}) + 15 face fluxes written = 36 doubles = 288 B
-#   update (per interior cell, RK3 stage average): 45 face fluxes + 25 state
-#     fields + Var registers (30, 30, 30 over the three stages) read, 29 state
-#     fields + Var registers (30, 15, 0) written, + 5 B of iteration count and
-#     flags = 134 doubles + 5 B = 1077 B
+#   source (per interior cell): e, P, PI, u (4), prevu (4), pi (10) read,
+#     15 sources written = 36 doubles = 288 B
+#   update (per interior cell): 45 face fluxes + 15 sources + e, u, prevu (9)
+#     read, 29 state fields written, plus per stage kind the RK registers:
+#     first stage reads the 15 evolved fields and writes Var0, Var1 (30);
+#     RK2 final reads Var0, Var1 (30); RK3 middle reads Var0, Var1 and writes
+#     Var2 (30 + 15); RK3 final reads Var0, Var2 (30); + 4 B iteration count,
+#     1 B flags (+1 B read after the first stage).  Averaged over the stages.
+def update_doubles(rk):
+    base = 45 + 15 + 9 + 29
+    kinds = {1: [15 + 30], 2: [15 + 30, 30], 3: [15 + 30, 30 + 15, 30]}[rk]
+    return [base + k for k in kinds]
+
+
 def kernel_bytes(name, d):
     faces = [(d.nx + 1) * d.ny * d.nz, d.nx * (d.ny + 1) * d.nz, d.nx * d.ny * (d.nz + 1)]
     cells = d.nx * d.ny * d.nz
     if name.startswith("flux_"):
         return faces[{"flux_x": 0, "flux_y": 1, "flux_eta": 2}[name]] * 288.0
+    if name == "source":
+        return cells * 288.0
     if name == "update":
-        return cells * 1077.0
+        per = update_doubles(d.rk)
+        avg = sum(8.0 * x + 5.0 + (1.0 if q else 0.0) for q, x in enumerate(per)) / len(per)
+        return cells * avg
     if name == "k0_broadcast":
         return cells * 8.0 * 10
-    if name == "ghost_fill":
-        return 0.0
     return 0.0
 
 
@@ -90,11 +102,83 @@ def cpu_baseline(seconds_target=20.0):
                       "CPU oracle (C, -O2, one thread)" % steps}
 
 
+def ensemble_descs(n_events, seed=4):
+    """Uneven ensemble: square transverse side ~U{96..256} cells, eta cells
+    ~U{32..64}, b ~U(0,12) fm; viscous KT, s95p EOS (table), eta/s=0.08,
+    tau_pi=5eta/(e+p), RK3, dx=deta=0.1."""
+    import numpy as np
+    from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP
+    rng = np.random.default_rng(seed)
+    descs, bs = [], []
+    for _ in range(n_events):
+        side = int(rng.integers(96, 257))
+        neta = int(rng.integers(32, 65))
+        descs.append(make_desc(side, side, neta, rk=3, eos=EOS_S95P, eta_law=ETA_TIMES_S, eta_coef=0.08,
+                               tpi_law=TPI_OVER_EP, tpi_coef=5.0, fast_math=1, eos_table=1))
+        bs.append(float(rng.uniform(0.0, 12.0)))
+    return descs, bs
+
+
+def bench_ensemble(n_events, steps, tau0=0.6, ts=0.002):
+    """Plain (one launch per event and kernel phase) vs batched (one launch per
+    phase over all events, hipGraph replay, autotuned launch configuration)
+    on the same uneven hotspot ensemble in one context; both synchronise once
+    per step."""
+    from concurrent.futures import ThreadPoolExecutor
+    from viral_amd import capi, ic
+    descs, bs = ensemble_descs(n_events)
+    cells = sum(d.nx * d.ny * d.nz for d in descs)
+
+    def make_state(e):
+        d = descs[e]
+        dref = d.copy()
+        dref.fast_math = dref.eos_table = 0
+        return ic.hotspots(d, tau0, ts, seed=1000 + e, b=bs[e], eos_p=lambda x: capi.eos_eval(dref, x)[0].reshape(x.shape))
+
+    t_ic = time.perf_counter()
+    h = capi.Hydro(descs)
+    with ThreadPoolExecutor(4) as pool:
+        for e, st in enumerate(pool.map(make_state, range(n_events))):
+            h.upload(st, ev=e)
+            del st
+    t_ic = time.perf_counter() - t_ic
+
+    def timed(n):
+        t = [tau0]
+        r = h.step(t[0], ts)
+        assert r == 0, r
+        t[0] += ts
+        t0 = time.perf_counter()
+        for _ in range(n):
+            r = h.step(t[0], ts)
+            assert r == 0, r
+            t[0] += ts
+        return time.perf_counter() - t0
+
+    out = {"workload": "%d hotspot events (200 spots, width 0.4 fm, b~U(0,12) fm), transverse side ~U{96..256}, "
+                       "eta cells ~U{32..64}, viscous KT, s95p table, eta/s=0.08, RK3, dtau=%g" % (n_events, ts),
+           "events": n_events, "interior_cells": cells, "ic_seconds": round(t_ic, 1)}
+    h.set_launch(batched=0, graph=0, blocks_per_cu=0, fusion=0, xcd=0)
+    dt = timed(steps)
+    out["plain"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps}
+    h.set_launch(batched=1, graph=1)
+    h.autotune(reps=1)
+    out["launch"] = h.launch().as_dict()
+    dt = timed(steps)
+    out["batched"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps}
+    out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    h.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ensemble-events", type=int, default=64,
+                    help="events of the batched-ensemble leg (0 skips it)")
+    ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy so a counter run can scale "
@@ -123,7 +207,8 @@ def main():
         r = h.step(tau, ts)
         assert r == 0, r
         tau += ts
-    tiles = h.autotune(reps=2) if d.fast_math else None   # LDS tile shape per direction, on this state
+    tiles = h.autotune(reps=2) if d.fast_math else None   # launch configuration, on this state
+    launch = h.launch().as_dict()
     torch.cuda.synchronize()
     h.timers(True)
     torch.cuda.synchronize()
@@ -171,7 +256,7 @@ def main():
         "config": {"workload": "viscous KT, s95p EOS, Glauber b=7 fm, Milne 200x200x64, dx=deta=0.1, "
                                "tau0=0.6, dtau=0.002, eta/s=0.08, tau_pi=5eta/(e+p), RK3",
                    "grid": [d.nx, d.ny, d.nz], "rk": d.rk, "fast_math": d.fast_math, "eos_table": d.eos_table,
-                   "flux_tiles": tiles,
+                   "flux_tiles": tiles, "launch": launch,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -182,6 +267,9 @@ def main():
                      "frac": value * bytes_per_update / 1e9 / HBM_PEAK_GBS},
         "kernel_time_share": kernel_share,
     }
+    h.close()
+    if args.ensemble_events > 0:
+        out["ensemble"] = bench_ensemble(args.ensemble_events, args.ensemble_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

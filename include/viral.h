@@ -120,9 +120,9 @@ void viral_ghost_dims(const viral_desc* d, int* gx, int* gy, int* gz);
 typedef struct viral_ctx viral_ctx;
 
 /* Create a context on the current HIP device holding nev events described by
- * descs[0..nev-1] (events may have different grid sizes; all other fields
- * must agree).  Replaces init()'s allocation (NEWEST10/init.h:548-552,
- * NEWEST10/alloc.h:33).  Returns VIRAL_OK or an error. */
+ * descs[0..nev-1] (an ensemble: events may differ in grid size, EOS and
+ * transport coefficients; scheme, rk, fast_math and coords must agree).
+ * Replaces init()'s allocation (NEWEST10/init.h:548-552, NEWEST10/alloc.h:33).  Returns VIRAL_OK or an error. */
 int viral_create(const viral_desc* descs, int nev, viral_ctx** out);
 void viral_destroy(viral_ctx* ctx);
 
@@ -142,6 +142,16 @@ int viral_download_field(viral_ctx* ctx, int ev, int f, double* out);
  * VIRAL_ERR_PHYSICS where the reference would have exited
  * (NEWEST10/root.h:396-421). */
 int viral_step(viral_ctx* ctx, double tau, double dtau);
+
+/* nsteps steps from tau, tau advancing by dtau between them as in the
+ * reference's time loop (NEWEST10/main.cpp:144-156), with one host
+ * synchronisation at the end.  *done (may be NULL) receives the number of
+ * steps that completed before the first failing one (nsteps if none failed);
+ * on VIRAL_ERR_PHYSICS the state is that of the last step run and
+ * viral_first_bad_cell describes the first failing step.  The reference
+ * checks every step and exits on the first failure; drivers that act on a
+ * failure mid-run call viral_step once per step instead. */
+int viral_steps(viral_ctx* ctx, double tau, double dtau, int nsteps, int* done);
 
 /* Per-cell flags (uint8, ghosted layout, OR over the stages of the last step)
  * and Newton iteration counts of stage s (0-based) of the last step (int32). */
@@ -164,11 +174,60 @@ int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out);
  * same table the fast-mode kernels read. */
 int viral_eos_eval(const viral_desc* d, size_t n, const double* e, double* p, double* dpde, double* T, double* s);
 
+/* ---- Launch layer (no counterpart in the reference, a CPU code with no
+ * tiling, batching or launches).
+ *
+ * Every RK stage is a fixed sequence of kernel phases (flux x, flux y, flux
+ * eta, sources, update, k0 broadcast, ghost fill).  Each phase splits every
+ * event into tiles (one block's worth of work); the tile table lists, per
+ * phase, the first tile of each event.
+ *
+ *   batched        1: each phase is ONE launch over all events of the context,
+ *                  a block finding its (event, tile) in the table; 0: one
+ *                  launch per event and phase.  Default: 1 when nev > 1.
+ *   graph          1: the launches of one RK step are captured in a hipGraph
+ *                  once (per starting state copy) and replayed every step; step
+ *                  times reach the kernels through a device-resident clock.
+ *                  Per-kernel timers force direct launches while on.
+ *   blocks_per_cu  0: a block per tile; k > 0: persistent launch of k blocks per
+ *                  compute unit striding over the tiles.
+ *   fusion         stage-fusion depth: 0 sources in their own kernel, 1 sources
+ *                  evaluated inside the update kernel.
+ *   xcd            1: permute tile ids so each XCD (blocks b with equal b % 8)
+ *                  receives a contiguous range, sharing halos in its L2.
+ *   flux_nt[3]     threads per block of the fast KT flux kernel per direction:
+ *                  64, 128, 256 or 512 (1, 2, 4, 8 wavefronts).
+ * The launch configuration never changes results: every cell and face sees
+ * the same arithmetic whatever the tiling, batching or fusion. */
+typedef struct viral_launch {
+  int batched, graph, blocks_per_cu, fusion, xcd;
+  int flux_nt[3];
+} viral_launch;
+int viral_get_launch(viral_ctx* ctx, viral_launch* out);
+int viral_set_launch(viral_ctx* ctx, const viral_launch* in);
+
+/* kernel phases of the tile table */
+enum { VIRAL_PHASE_FLUX_X = 0, VIRAL_PHASE_FLUX_Y, VIRAL_PHASE_FLUX_ETA, VIRAL_PHASE_CELLS, VIRAL_PHASE_K0,
+       VIRAL_PHASE_GHOST, VIRAL_NPHASE };
+/* Tiles of one event in a phase for flux tile shape (tp, tk) (used by the fast
+ * KT flux phases only; host arithmetic, no device needed); -1 on bad input. */
+long long viral_tile_count(const viral_desc* d, int phase, int tp, int tk);
+/* The context's tile table of a phase: start[0..nev], event e owning tiles
+ * [start[e], start[e+1]). */
+int viral_tile_table(viral_ctx* ctx, int phase, long long* start);
+/* Tile that block slot w of a `total`-tile launch processes when the XCD order
+ * is on (a permutation of [0, total)); -1 on bad input. */
+long long viral_xcd_tile(long long w, long long total);
+
 /* LDS tile shape of the fast KT flux kernel of direction dir (0 x, 1 y,
  * 2 eta): tp cells along the direction (tp-5 faces per tile) times tk lines,
- * tp*tk <= 1024.  viral_autotune times the candidate shapes on event ev's
- * current state (reps launches each) and keeps the fastest per direction.
- * No counterpart in the reference (a CPU code with no tiling). */
+ * tp >= 8, tp*tk at most 4*flux_nt[dir] and at most 1024.
+ * viral_autotune times candidates on the current state at the clock of the
+ * last step (reps launches each) and keeps the fastest: with ev >= 0 the tile
+ * shapes of event ev; with ev < 0 the whole launch configuration (per
+ * direction the block size and one shape for all events, then fusion depth,
+ * persistent blocks and XCD order).  It overwrites the last step's flags and
+ * iteration counts, never the state. */
 int viral_set_tile(viral_ctx* ctx, int ev, int dir, int tp, int tk);
 int viral_get_tile(viral_ctx* ctx, int ev, int dir, int* tp, int* tk);
 int viral_autotune(viral_ctx* ctx, int ev, int reps);

@@ -19,9 +19,24 @@ EXPORTS = [
     "viral_desc_default", "viral_cells", "viral_ghost_dims", "viral_create", "viral_destroy",
     "viral_upload_state", "viral_download_state", "viral_download_field", "viral_step",
     "viral_flags", "viral_iterations", "viral_first_bad_cell", "viral_max_temp_gev",
-    "viral_eos_eval", "viral_set_tile", "viral_get_tile", "viral_autotune", "viral_timers", "viral_timer_read", "viral_kernel_count", "viral_kernel_name",
-    "viral_last_error",
+    "viral_eos_eval", "viral_set_tile", "viral_get_tile", "viral_autotune", "viral_timers", "viral_timer_read",
+    "viral_kernel_count", "viral_kernel_name", "viral_last_error", "viral_steps", "viral_get_launch",
+    "viral_set_launch", "viral_tile_count", "viral_tile_table", "viral_xcd_tile",
 ]
+
+# kernel phases of the tile table (VIRAL_PHASE_*)
+PHASE_FLUX_X, PHASE_FLUX_Y, PHASE_FLUX_ETA, PHASE_CELLS, PHASE_K0, PHASE_GHOST = range(6)
+
+
+class ViralLaunch(C.Structure):
+    """viral_launch: how the kernel phases of a stage are launched."""
+    _fields_ = [("batched", C.c_int), ("graph", C.c_int), ("blocks_per_cu", C.c_int), ("fusion", C.c_int),
+                ("xcd", C.c_int), ("flux_nt", C.c_int * 3)]
+
+    def as_dict(self):
+        return {"batched": self.batched, "graph": self.graph, "blocks_per_cu": self.blocks_per_cu,
+                "fusion": self.fusion, "xcd": self.xcd, "flux_nt": list(self.flux_nt)}
+
 
 _lib = None
 
@@ -82,6 +97,18 @@ def load():
     L.viral_kernel_name.argtypes = [C.c_int]
     L.viral_last_error.restype = C.c_char_p
     L.viral_last_error.argtypes = [vp]
+    L.viral_steps.restype = C.c_int
+    L.viral_steps.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int)]
+    L.viral_get_launch.restype = C.c_int
+    L.viral_get_launch.argtypes = [vp, C.POINTER(ViralLaunch)]
+    L.viral_set_launch.restype = C.c_int
+    L.viral_set_launch.argtypes = [vp, C.POINTER(ViralLaunch)]
+    L.viral_tile_count.restype = C.c_longlong
+    L.viral_tile_count.argtypes = [pdesc, C.c_int, C.c_int, C.c_int]
+    L.viral_tile_table.restype = C.c_int
+    L.viral_tile_table.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong)]
+    L.viral_xcd_tile.restype = C.c_longlong
+    L.viral_xcd_tile.argtypes = [C.c_longlong, C.c_longlong]
     _lib = L
     return L
 
@@ -91,6 +118,15 @@ def _ptr_array(arrays):
     for q, a in enumerate(arrays):
         arr[q] = a.ctypes.data_as(C.POINTER(C.c_double))
     return arr
+
+
+def tile_count(desc, phase, tp=64, tk=16):
+    """Tiles (blocks' worth of work) of one event in one kernel phase."""
+    return load().viral_tile_count(C.byref(desc), phase, tp, tk)
+
+
+def xcd_tile(w, total):
+    return load().viral_xcd_tile(w, total)
 
 
 def eos_eval(desc, e):
@@ -192,10 +228,43 @@ class Hydro:
         self._chk(self.L.viral_get_tile(self.h, ev, d, C.byref(tp), C.byref(tk)))
         return tp.value, tk.value
 
-    def autotune(self, reps=3, ev=0):
-        """Pick the fastest LDS tile shape per direction on the current state."""
+    def autotune(self, reps=3, ev=-1):
+        """Time candidate launch configurations on the current state and keep
+        the fastest: ev < 0 tunes the whole context (flux block size and tile
+        shapes, fusion depth, persistent blocks, XCD order), ev >= 0 only that
+        event's flux tile shapes.  Returns event 0's (or ev's) tile shapes."""
         self._chk(self.L.viral_autotune(self.h, ev, reps))
-        return [self.get_tile(d, ev) for d in range(3)]
+        return [self.get_tile(d, max(ev, 0)) for d in range(3)]
+
+    def steps(self, tau, ts, n):
+        """n steps with one synchronisation; returns (status, steps done
+        before the first failure)."""
+        done = C.c_int()
+        r = self.L.viral_steps(self.h, tau, ts, n, C.byref(done))
+        if r not in (OK, ERR_PHYSICS):
+            self._chk(r)
+        return r, done.value
+
+    def launch(self):
+        c = ViralLaunch()
+        self._chk(self.L.viral_get_launch(self.h, C.byref(c)))
+        return c
+
+    def set_launch(self, **kw):
+        """Change fields of the launch configuration (flux_nt as a 3-list)."""
+        c = self.launch()
+        for k, v in kw.items():
+            if k == "flux_nt":
+                for d in range(3):
+                    c.flux_nt[d] = v[d]
+            else:
+                setattr(c, k, int(v))
+        self._chk(self.L.viral_set_launch(self.h, C.byref(c)))
+
+    def tile_table(self, phase):
+        out = (C.c_longlong * (len(self.descs) + 1))()
+        self._chk(self.L.viral_tile_table(self.h, phase, out))
+        return list(out)
 
     def timers(self, enable=True):
         self._chk(self.L.viral_timers(self.h, 1 if enable else 0))

@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -73,6 +75,98 @@ __host__ __device__ inline Box interior_box(const Geom& g) { return Box{g.il, g.
 
 __device__ __forceinline__ long dstride(const Geom& g, int d) { return d == 0 ? g.sx : d == 1 ? g.sy : 1; }
 
+// ------------------------------------------------- stage clock and event table
+
+struct StageArgs {
+  double tau_src;   // time of the sources / fluxes of this stage
+  double ts_src;    // step used in d_tau u = (u - prevu)/ts_src
+  double tau_new;   // time the RK combination lands on
+  double ts;        // full step
+  double taustep;   // SHASTA sub-step: ts, ts/2, ts/3 for RK1/2/3 (NEWEST10/time.h:314,352,411)
+  int stage;        // 0,1,2
+  int kind;         // 0: Var1 = Var0 + ts L; 1: RK2 final; 2: RK3 middle; 3: RK3 final
+};
+
+// Device-resident step clock.  Kernels read tau and dtau from here, so one
+// captured graph serves every step; the last kernel of a step advances it
+// (tau += dtau, as the reference's time loop, NEWEST10/main.cpp:156).
+struct Clock {
+  double tau, ts;
+  long long step;  // steps taken since the clock was set (first-failure bookkeeping)
+};
+
+// Stage times of stage `stage` of an RK`rk` step from (tau, ts):
+// NEWEST10/time.h:307-325 (RK1), 345-363 (RK2), 402-426 (RK3).
+__host__ __device__ inline StageArgs stage_args(double tau, double ts, int rk, int stage)
+{
+  StageArgs a{};
+  a.ts = ts;
+  a.taustep = rk == 3 ? ts / 3.0 : rk == 2 ? ts / 2 : ts;
+  a.stage = stage;
+  if (stage == 0) { a.tau_src = tau; a.ts_src = ts; a.tau_new = tau + ts; a.kind = 0; }
+  else if (rk == 2) { a.tau_src = tau + ts; a.ts_src = ts; a.tau_new = tau + ts; a.kind = 1; }
+  else if (stage == 1) { a.tau_src = tau + ts; a.ts_src = ts; a.tau_new = tau + ts / 2.0; a.kind = 2; }
+  else { a.tau_src = tau + ts / 2.0; a.ts_src = ts / 2.0; a.tau_new = tau + ts; a.kind = 3; }
+  return a;
+}
+
+// One row of the device event table: geometry, physics and buffers of one
+// event of the ensemble, plus its flux tile shape per direction.
+struct EvDev {
+  Geom g;
+  Params p;
+  double* st[2];
+  double* var0;
+  double* var1;
+  double* var2;
+  double* flux[3];
+  double* src;
+  int32_t* iters;
+  uint8_t* flags;
+  unsigned long long* bad;
+  int tp[3], tk[3];
+};
+
+// Work of one launch: `total` tiles (blocks' worth of work) spread over the
+// events ev0..ev0+nev-1; event e owns tiles [start[e-ev0], start[e-ev0+1]).
+// A launch either gives every tile its own block (gridDim == total) or runs
+// persistent blocks that stride over the tiles.  With xcd set, tile ids are
+// permuted so the tiles one XCD (blocks b with b % 8 equal) receives are
+// contiguous, keeping neighbouring tiles' halos in that XCD's L2.
+struct Sched {
+  const long* start;
+  long total;
+  int ev0, nev, xcd;
+};
+
+__host__ __device__ inline long xcd_tile(long w, long total)
+{
+  const long per = total / 8, rem = total % 8;
+  const long x = w % 8, r = w / 8;
+  return x * per + (x < rem ? x : rem) + r;
+}
+
+__device__ __forceinline__ int sched_find(const Sched& s, long w, long& local)
+{
+  const long t = s.xcd ? xcd_tile(w, s.total) : w;
+  if (s.nev == 1) {
+    local = t;
+    return s.ev0;
+  }
+  int lo = 0, hi = s.nev - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s.start[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  local = t - s.start[lo];
+  return s.ev0 + lo;
+}
+
+// PERSIST: blocks stride over the tiles; otherwise one tile per block (the
+// loop runs once, and the compiler keeps the event row in scalar registers)
+#define TILE_LOOP(s) for (long w_ = blockIdx.x; w_ < (s).total; w_ += PERSIST ? (long)gridDim.x : (s).total)
+
 // ---------------------------------------------------------------- kernels
 
 // One thread per face c+1/2 of direction d: the WENO of the eigenvalue u^d/u^0
@@ -84,10 +178,9 @@ __device__ __forceinline__ long dstride(const Geom& g, int d) { return d == 0 ? 
 //   f = (F_L + F_R)/2 - a (V_R - V_L)/2      (hydroKT.h:444-506).
 // The stencil cells' common terms are loaded once and kept in registers.
 template <bool FAST>
-__global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, double tau, const double* __restrict__ st,
-                                                  double* __restrict__ flux)
+__device__ __forceinline__ void flux_face(const Geom& g, const Params& p, int d, double tau,
+                                          const double* __restrict__ st, double* __restrict__ flux, long t)
 {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   Box b = face_box(g, d);
   if (!box_cell(b, t, i, j, k)) return;
@@ -167,6 +260,19 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, doubl
   }
 }
 
+template <bool FAST, bool PERSIST>
+__global__ void __launch_bounds__(BLK) k_flux_dir(const EvDev* __restrict__ tab, Sched s, const Clock* __restrict__ clk,
+                                                  int stage, int cur, int d)
+{
+  TILE_LOOP(s)
+  {
+    long lb;
+    const EvDev& E = tab[sched_find(s, w_, lb)];
+    const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
+    flux_face<FAST>(E.g, E.p, d, a.tau_src, E.st[cur], E.flux[d], lb * BLK + threadIdx.x);
+  }
+}
+
 // Fast-mode KT flux of direction D on an LDS tile (same physics as
 // k_flux_dir).  A block takes TK lines along D (interior in the other two
 // directions) and TP consecutive cells of each; every cell's centred flux
@@ -176,12 +282,22 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(Geom g, Params p, int d, doubl
 // c+1 through a second, double-buffered LDS array.  TP-5 faces per line per
 // tile; the tile shape is a launch parameter.  Slots map to cells with the
 // contiguous (eta) index fastest so loads coalesce in every direction.
-template <int D>
-__global__ void __launch_bounds__(BLK) k_flux_tile(Geom g, Params p, double tau, const double* __restrict__ st,
-                                                   double* __restrict__ flux, int TP, int TK)
+//
+// NT threads per block; each thread owns up to MAXR slots (TP*TK <= MAXR*NT,
+// at most 1024 slots = 48 KiB of LDS).
+template <int NT>
+struct TileSlots {
+  static constexpr int MAXR = NT >= 512 ? 2 : 4;
+  static constexpr int MAXS = MAXR * NT < 1024 ? MAXR * NT : 1024;
+};
+
+template <int D, int NT>
+__device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, double tau,
+                                               const double* __restrict__ st, double* __restrict__ flux, int TP,
+                                               int TK, long tile, double* lds)
 {
-  extern __shared__ double lds[];
-  constexpr int MAXR = 4;  // slots per thread: TP*TK <= MAXR*BLK
+  constexpr int MAXR = TileSlots<NT>::MAXR;
+  constexpr int BLK = NT;
   const int S = TP * TK;
   double* sF = lds;
   double* sV = lds + S;
@@ -198,8 +314,8 @@ __global__ void __launch_bounds__(BLK) k_flux_tile(Geom g, Params p, double tau,
   const int fpt = TP - 5;
   const int nface = hi - lo + 1;
   const int nseg = (nface + fpt - 1) / fpt;
-  const long lgrp = blockIdx.x / nseg;
-  const int sg = blockIdx.x % nseg;
+  const long lgrp = tile / nseg;
+  const int sg = (int)(tile % nseg);
   const int c0 = lo - 1 + sg * fpt;  // first face cell of this tile
   const int p0 = c0 - 2;             // cell at tile position 0
   const long n = g.n;
@@ -320,6 +436,21 @@ __global__ void __launch_bounds__(BLK) k_flux_tile(Geom g, Params p, double tau,
   }
 }
 
+template <int D, int NT, bool PERSIST>
+__global__ void __launch_bounds__(NT) k_flux_tile(const EvDev* __restrict__ tab, Sched s,
+                                                  const Clock* __restrict__ clk, int stage, int cur)
+{
+  extern __shared__ double lds[];
+  TILE_LOOP(s)
+  {
+    long lb;
+    const EvDev& E = tab[sched_find(s, w_, lb)];
+    const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
+    flux_tile_body<D, NT>(E.g, E.p, a.tau_src, E.st[cur], E.flux[D], E.tp[D], E.tk[D], lb, lds);
+    __syncthreads();  // the next tile may lay its LDS arrays out differently
+  }
+}
+
 // SHASTA flux-corrected transport along direction d for one interior cell c
 // (NEWEST10/hydroshasta.h:8-32, 101-284): transport velocity v (row 0:
 // T^{d,tau}/T^{tau tau}, other rows u^d/u^0), low-order transported UTD at
@@ -327,10 +458,9 @@ __global__ void __launch_bounds__(BLK) k_flux_tile(Geom g, Params p, double tau,
 //   (-U + UTD - A_c + A_{c-1}) / taustep.
 // The Q+- factors of the rows that share u^d/u^0 are computed once.
 template <bool FAST>
-__global__ void __launch_bounds__(BLK) k_shasta_dir(Geom g, Params p, int d, double tau, double taustep,
-                                                    const double* __restrict__ st, double* __restrict__ pr)
+__device__ __forceinline__ void shasta_cell(const Geom& g, const Params& p, int d, double tau, double taustep,
+                                            const double* __restrict__ st, double* __restrict__ pr, long t)
 {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_cell(interior_box(g), t, i, j, k)) return;
   const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
@@ -391,30 +521,29 @@ __global__ void __launch_bounds__(BLK) k_shasta_dir(Geom g, Params p, int d, dou
   }
 }
 
-struct StageArgs {
-  double tau_src;   // time of the sources / fluxes of this stage
-  double ts_src;    // step used in d_tau u = (u - prevu)/ts_src
-  double tau_new;   // time the RK combination lands on
-  double ts;        // full step
-  double taustep;   // SHASTA sub-step: ts, ts/2, ts/3 for RK1/2/3 (NEWEST10/time.h:314,352,411)
-  int stage;        // 0,1,2
-  int kind;         // 0: Var1 = Var0 + ts L; 1: RK2 final; 2: RK3 middle; 3: RK3 final
-};
+template <bool FAST, bool PERSIST>
+__global__ void __launch_bounds__(BLK) k_shasta_dir(const EvDev* __restrict__ tab, Sched s,
+                                                    const Clock* __restrict__ clk, int stage, int cur, int d)
+{
+  TILE_LOOP(s)
+  {
+    long lb;
+    const EvDev& E = tab[sched_find(s, w_, lb)];
+    const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
+    shasta_cell<FAST>(E.g, E.p, d, a.tau_src, a.taustep, E.st[cur], E.flux[d], lb * BLK + threadIdx.x);
+  }
+}
+
 
 // Geometric, CON-gradient and Israel-Stewart sources of one interior cell
 // (NEWEST10/Source.h:3-101,195-536) from the stage-start state, into src.
 template <bool FAST>
-__global__ void __launch_bounds__(BLK) k_source(Geom g, Params p, StageArgs a, const double* __restrict__ si,
-                                                double* __restrict__ src)
+__device__ __forceinline__ void source_terms(const Geom& g, const Params& p, const StageArgs& a,
+                                             const double* __restrict__ si, long x, double* S)
 {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!box_cell(interior_box(g), t, i, j, k)) return;
-  const long n = g.n, x = cidx(g, i, j, k);
+  const long n = g.n;
   const int milne = p.milne;
   const double tau = a.tau_src;
-  // ---- sources
-  double S[SV];
   const double e = si[VIRAL_F_EN * n + x], P = si[VIRAL_F_P * n + x], PI = si[VIRAL_F_BULK * n + x];
   double u[4], pu[4], pi[10];
 #pragma unroll
@@ -507,8 +636,25 @@ __global__ void __launch_bounds__(BLK) k_source(Geom g, Params p, StageArgs a, c
 #pragma unroll
     for (int l = 0; l < 10; l++) S[4 + l] = out[l];
   }
+}
+
+template <bool FAST, bool PERSIST>
+__global__ void __launch_bounds__(BLK) k_source(const EvDev* __restrict__ tab, Sched s, const Clock* __restrict__ clk,
+                                                int stage, int cur)
+{
+  TILE_LOOP(s)
+  {
+    long lb;
+    const EvDev& E = tab[sched_find(s, w_, lb)];
+    int i, j, k;
+    if (!box_cell(interior_box(E.g), lb * BLK + threadIdx.x, i, j, k)) continue;
+    const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
+    const long x = cidx(E.g, i, j, k), n = E.g.n;
+    double S[SV];
+    source_terms<FAST>(E.g, E.p, a, E.st[cur], x, S);
 #pragma unroll
-  for (int l = 0; l < SV; l++) src[l * n + x] = S[l];
+    for (int l = 0; l < SV; l++) E.src[l * n + x] = S[l];
+  }
 }
 
 // Flux divergence, L = Result + Source, the RK combination, T/pi from Var,
@@ -516,16 +662,19 @@ __global__ void __launch_bounds__(BLK) k_source(Geom g, Params p, StageArgs a, c
 // (NEWEST10/hydroKT.h:393-403,458-504; time.h:2-180; root.h:301-424).  Reads
 // the stage-start state `si`, the face fluxes of the three directions and the
 // sources, writes the new state `so`.
-template <bool FAST>
-__global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, const double* __restrict__ si,
-                                                double* __restrict__ so, const double* __restrict__ fx,
-                                                const double* __restrict__ fy, const double* __restrict__ fz,
-                                                const double* __restrict__ src, double* __restrict__ var0,
-                                                double* __restrict__ var1, double* __restrict__ var2,
-                                                int32_t* __restrict__ iters, uint8_t* __restrict__ flags,
-                                                unsigned long long* __restrict__ bad)
+//
+// FUSED evaluates the sources in place (source_terms) instead of reading
+// them from the source kernel's output: one launch and 15 doubles per cell
+// less traffic, at the price of the union of both kernels' registers.
+template <bool FAST, bool FUSED>
+__device__ __forceinline__ void update_cell(const Geom& g, const Params& p, const StageArgs& a, long long step,
+                                            const double* __restrict__ si, double* __restrict__ so,
+                                            const double* __restrict__ fx, const double* __restrict__ fy,
+                                            const double* __restrict__ fz, const double* __restrict__ src,
+                                            double* __restrict__ var0, double* __restrict__ var1,
+                                            double* __restrict__ var2, int32_t* __restrict__ iters,
+                                            uint8_t* __restrict__ flags, unsigned long long* __restrict__ bad, long t)
 {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_cell(interior_box(g), t, i, j, k)) return;
   const long n = g.n, x = cidx(g, i, j, k);
@@ -558,8 +707,12 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
     }
   }
   double S[SV];
+  if (FUSED) {
+    source_terms<FAST>(g, p, a, si, x, S);
+  } else {
 #pragma unroll
-  for (int l = 0; l < SV; l++) S[l] = src[l * n + x];
+    for (int l = 0; l < SV; l++) S[l] = src[l * n + x];
+  }
   const double e = si[VIRAL_F_EN * n + x];
   double u[4], pu[4], pi[10];
 #pragma unroll
@@ -654,15 +807,31 @@ __global__ void __launch_bounds__(BLK) k_update(Geom g, Params p, StageArgs a, c
   uint8_t prev = a.stage == 0 ? 0 : flags[x];
   flags[x] = (uint8_t)((prev & ~VIRAL_FLAG_FREEZEOUT) | fl);
   if (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER)) {
-    // ordered by stage, then reference loop-order index; this stage's flags in the low byte
-    atomicMin(bad, ((unsigned long long)a.stage << 56) | ((unsigned long long)x << 8) | (unsigned long long)fl);
+    // ordered by step, stage, then reference loop-order index (x < 2^32);
+    // this stage's flags in the low byte
+    atomicMin(bad, ((unsigned long long)step << 42) | ((unsigned long long)a.stage << 40) |
+                       ((unsigned long long)x << 8) | (unsigned long long)fl);
+  }
+}
+
+template <bool FAST, bool FUSED, bool PERSIST>
+__global__ void __launch_bounds__(BLK) k_update(const EvDev* __restrict__ tab, Sched s, const Clock* __restrict__ clk,
+                                                int stage, int cur)
+{
+  TILE_LOOP(s)
+  {
+    long lb;
+    const EvDev& E = tab[sched_find(s, w_, lb)];
+    const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
+    update_cell<FAST, FUSED>(E.g, E.p, a, clk->step, E.st[cur], E.st[cur ^ 1], E.flux[0], E.flux[1], E.flux[2],
+                             E.src, E.var0, E.var1, E.var2, E.iters + (long)stage * E.g.n, E.flags, E.bad,
+                             lb * BLK + threadIdx.x);
   }
 }
 
 // NEWEST10/root.h:426-449: every eta slice takes the velocity of slice k0
-__global__ void __launch_bounds__(BLK) k_k0_broadcast(Geom g, Params p, double tau, double* __restrict__ st)
+__device__ __forceinline__ void k0_cell(const Geom& g, const Params& p, double tau, double* __restrict__ st, long t)
 {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_cell(interior_box(g), t, i, j, k)) return;
   if (k == g.k0) return;
@@ -679,6 +848,19 @@ __global__ void __launch_bounds__(BLK) k_k0_broadcast(Geom g, Params p, double t
   st[VIRAL_F_U3 * n + x] = u0 * ve;
 }
 
+template <bool PERSIST>
+__global__ void __launch_bounds__(BLK) k_k0_broadcast(const EvDev* __restrict__ tab, Sched s,
+                                                      const Clock* __restrict__ clk, int stage, int cur)
+{
+  TILE_LOOP(s)
+  {
+    long lb;
+    const EvDev& E = tab[sched_find(s, w_, lb)];
+    const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
+    k0_cell(E.g, E.p, a.tau_new, E.st[cur], lb * BLK + threadIdx.x);
+  }
+}
+
 // fields copied into ghosts, NEWEST10/boundary.h:13-46 (all but Temp)
 __constant__ int c_bfields[28] = {VIRAL_F_T00, VIRAL_F_T10, VIRAL_F_T20, VIRAL_F_T30, VIRAL_F_EN, VIRAL_F_VX,
                                   VIRAL_F_VY, VIRAL_F_VE, VIRAL_F_P, VIRAL_F_U0, VIRAL_F_U1, VIRAL_F_U2, VIRAL_F_U3,
@@ -689,10 +871,14 @@ __constant__ int c_bfields[28] = {VIRAL_F_T00, VIRAL_F_T10, VIRAL_F_T20, VIRAL_F
 // Zero-gradient ghost fill of the x, y (and eta) layers of interior rows,
 // NEWEST10/boundary.h:1-275.  Thread = (ghost cell, field); the three ghost
 // slabs are disjoint and read only interior cells.
-__global__ void __launch_bounds__(BLK) k_ghost_fill(Geom g, Params p, double* __restrict__ st)
+__host__ __device__ inline long ghost_count(const Geom& g, int lbi)
 {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int q = blockIdx.y;
+  const long nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
+  return 2L * VIRAL_BORDER * (ny * nz + nx * nz + (lbi ? 0 : nx * ny));
+}
+
+__device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, double* __restrict__ st, int q, long t)
+{
   const int f = c_bfields[q];
   const long n = g.n;
   const int nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
@@ -742,6 +928,36 @@ __global__ void __launch_bounds__(BLK) k_ghost_fill(Geom g, Params p, double* __
   st[f * n + dst] = st[f * n + src];
 }
 
+// tiles of one event: 28 fields x ceil(ghost cells / BLK)
+template <bool PERSIST>
+__global__ void __launch_bounds__(BLK) k_ghost_fill(const EvDev* __restrict__ tab, Sched s, int cur)
+{
+  TILE_LOOP(s)
+  {
+    long lb;
+    const EvDev& E = tab[sched_find(s, w_, lb)];
+    const long nb = (ghost_count(E.g, E.p.lbi) + BLK - 1) / BLK;
+    const int q = (int)(lb / nb);
+    ghost_cell(E.g, E.p, E.st[cur], q, (lb % nb) * BLK + threadIdx.x);
+  }
+}
+
+// first kernel of a step sequence: no failure recorded yet
+__global__ void k_clear_bad(unsigned long long* __restrict__ bad, int nev)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nev) bad[e] = ~0ull;
+}
+
+// last kernel of a step: tau += dtau (NEWEST10/main.cpp:156)
+__global__ void k_clock_advance(Clock* __restrict__ clk)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    clk->tau = clk->tau + clk->ts;
+    clk->step = clk->step + 1;
+  }
+}
+
 __global__ void k_max_temp(Geom g, Params p, const double* __restrict__ st, unsigned long long* out)
 {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -763,24 +979,26 @@ __global__ void k_max_temp(Geom g, Params p, const double* __restrict__ st, unsi
 
 // ---------------------------------------------------------------- host side
 
+// kernel phases of one RK stage; each phase is one launch per event, or one
+// launch over every event of the ensemble
+enum Phase { PH_FLUX_X = 0, PH_FLUX_Y, PH_FLUX_Z, PH_CELLS, PH_K0, PH_GHOST, PH_COUNT };
+
 struct Event {
   viral_desc d;
   Geom g;
   Params p;
   double* st[2] = {nullptr, nullptr};  // ping-pong state, NF*n each
-  int cur = 0;                          // which copy holds the state
   double* var0 = nullptr;
   double* var1 = nullptr;
-  double* var2 = nullptr;
+  double* var2 = nullptr;                         // RK3 only
   double* flux[3] = {nullptr, nullptr, nullptr};  // face fluxes of x, y, eta: [15][n] each
   double* src = nullptr;                          // sources of the stage: [15][n]
   int32_t* iters = nullptr;  // 3*n
   uint8_t* flags = nullptr;
-  unsigned long long* bad = nullptr;   // device word
   unsigned long long bad_host = ~0ull;
-  int tile_tp[3] = {64, 64, 64};        // LDS tile of the fast flux kernel: cells per line
-  int tile_tk[3] = {16, 16, 4};         //   and lines per tile, per direction
-  double* tab_v = nullptr;             // tabulated s95p EOS (fast mode)
+  int tile_tp[3] = {64, 64, 64};  // LDS tile of the fast flux kernel: cells per line
+  int tile_tk[3] = {16, 16, 4};   //   and lines per tile, per direction
+  double* tab_v = nullptr;        // tabulated s95p EOS (fast mode)
   unsigned char* tab_closed = nullptr;
 };
 
@@ -793,6 +1011,26 @@ struct viral_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   unsigned long long* dtmp = nullptr;
+  int cur = 0;    // which state copy is current (all events step together)
+  int ncu = 256;  // compute units of the device
+  int rk = 2, scheme = VIRAL_KT, fast = 0;
+  viral_launch cfg{};
+  // tile table: device event rows and, per phase, the first tile of every event
+  EvDev* d_tab = nullptr;
+  long* d_start = nullptr;  // PH_COUNT x (nev+1)
+  std::vector<long> cnt[PH_COUNT];
+  long total[PH_COUNT] = {};
+  size_t shm[3] = {};  // LDS bytes of the batched flux launches
+  bool dirty = true;   // table or launch configuration changed since the last upload / capture
+  // step clock and failure words (device + pinned host mirrors)
+  Clock* d_clk = nullptr;
+  Clock* h_clk = nullptr;
+  unsigned long long* d_bad = nullptr;
+  unsigned long long* h_bad = nullptr;
+  double last_tau = 1.0, last_ts = 0.01;  // clock the autotuner times with
+  // captured RK step per starting parity
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
   // per-kernel HIP-event timing (viral_timers)
   bool timing = false;
   std::vector<hipEvent_t> tev[K_COUNT];  // start/stop pairs
@@ -848,6 +1086,274 @@ static int fail(viral_ctx* c, int code, const std::string& msg)
 static unsigned nblocks(long cells) { return (unsigned)((cells + BLK - 1) / BLK); }
 static long box_count(const Box& b) { return (long)b.ni * b.nj * b.nk; }
 
+static int max_slots(int nt)
+{
+  switch (nt) {
+    case 64: return TileSlots<64>::MAXS;
+    case 128: return TileSlots<128>::MAXS;
+    case 256: return TileSlots<256>::MAXS;
+    case 512: return TileSlots<512>::MAXS;
+    default: return 0;
+  }
+}
+
+// a tile needs at least 3 faces per line; cells past the end of a line are masked
+static bool tile_ok(int TP, int TK, int nt) { return TP >= 8 && TK >= 1 && (long)TP * TK <= max_slots(nt); }
+
+static void fill_geom(const viral_desc* d, Geom& g)
+{
+  viral_ghost_dims(d, &g.gx, &g.gy, &g.gz);
+  g.il = VIRAL_BORDER; g.ir = VIRAL_BORDER + d->nx;
+  g.jl = VIRAL_BORDER; g.jr = VIRAL_BORDER + d->ny;
+  if (d->lbi) { g.kl = 0; g.kr = 1; g.k0 = 0; }
+  else { g.kl = VIRAL_BORDER; g.kr = VIRAL_BORDER + d->nz; g.k0 = d->k0 >= 0 ? d->k0 : (g.gz - 1) / 2; }
+  g.n = (long)g.gx * g.gy * g.gz;
+  g.sx = (long)g.gy * g.gz;
+  g.sy = g.gz;
+}
+
+// tiles of one event in one phase (a tile is one block's worth of work)
+static long tile_count(const viral_desc* d, int ph, int tp, int tk)
+{
+  Geom g;
+  fill_geom(d, g);
+  const long ni = box_count(interior_box(g));
+  switch (ph) {
+    case PH_FLUX_X:
+    case PH_FLUX_Y:
+    case PH_FLUX_Z: {
+      const int dd = ph - PH_FLUX_X;
+      if (dd == 2 && d->lbi) return 0;
+      if (d->scheme == VIRAL_SHASTA) return nblocks(ni);
+      if (!d->fast_math) return nblocks(box_count(face_box(g, dd)));
+      const int na = dd == 0 ? g.jr - g.jl : g.ir - g.il;
+      const int nb = dd == 2 ? g.jr - g.jl : g.kr - g.kl;
+      const int lo = dd == 0 ? g.il : dd == 1 ? g.jl : g.kl, hi = dd == 0 ? g.ir : dd == 1 ? g.jr : g.kr;
+      const long nlines = (long)na * nb;
+      const long nseg = (hi - lo + 1 + tp - 6) / (tp - 5);
+      return ((nlines + tk - 1) / tk) * nseg;
+    }
+    case PH_CELLS: return nblocks(ni);
+    case PH_K0: return ((d->quirks & VIRAL_Q_K0_BROADCAST) && !d->lbi) ? nblocks(ni) : 0;
+    case PH_GHOST: return 28L * (long)nblocks(ghost_count(g, d->lbi));
+    default: return 0;
+  }
+}
+
+static void drop_graphs(viral_ctx* c)
+{
+  for (int q = 0; q < 2; q++) {
+    if (c->gexec[q]) hipGraphExecDestroy(c->gexec[q]);
+    if (c->graph[q]) hipGraphDestroy(c->graph[q]);
+    c->gexec[q] = nullptr;
+    c->graph[q] = nullptr;
+  }
+}
+
+// rebuild the tile table from the events' tile shapes and upload it with the
+// event rows; drops captured graphs (their launches baked the old table in)
+static int refresh_table(viral_ctx* ctx)
+{
+  if (!ctx->dirty) return VIRAL_OK;
+  drop_graphs(ctx);
+  const int nev = (int)ctx->ev.size();
+  std::vector<EvDev> rows(nev);
+  std::vector<long> start((size_t)PH_COUNT * (nev + 1));
+  for (int ph = 0; ph < PH_COUNT; ph++) {
+    ctx->cnt[ph].assign(nev, 0);
+    long acc = 0;
+    for (int e = 0; e < nev; e++) {
+      const Event& E = ctx->ev[e];
+      const int dd = ph <= PH_FLUX_Z ? ph : 0;
+      const long n = tile_count(&E.d, ph, E.tile_tp[dd], E.tile_tk[dd]);
+      start[(size_t)ph * (nev + 1) + e] = acc;
+      ctx->cnt[ph][e] = n;
+      acc += n;
+    }
+    start[(size_t)ph * (nev + 1) + nev] = acc;
+    ctx->total[ph] = acc;
+  }
+  for (int d = 0; d < 3; d++) {
+    size_t m = 0;
+    for (const Event& E : ctx->ev) m = std::max(m, 6 * (size_t)E.tile_tp[d] * E.tile_tk[d] * sizeof(double));
+    ctx->shm[d] = m;
+  }
+  for (int e = 0; e < nev; e++) {
+    const Event& E = ctx->ev[e];
+    EvDev& r = rows[e];
+    r.g = E.g;
+    r.p = E.p;
+    r.st[0] = E.st[0];
+    r.st[1] = E.st[1];
+    r.var0 = E.var0;
+    r.var1 = E.var1;
+    r.var2 = E.var2;
+    for (int q = 0; q < 3; q++) r.flux[q] = E.flux[q];
+    r.src = E.src;
+    r.iters = E.iters;
+    r.flags = E.flags;
+    r.bad = ctx->d_bad + e;
+    for (int q = 0; q < 3; q++) {
+      r.tp[q] = E.tile_tp[q];
+      r.tk[q] = E.tile_tk[q];
+    }
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(ctx->d_tab, rows.data(), nev * sizeof(EvDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_start, start.data(), start.size() * sizeof(long), hipMemcpyHostToDevice));
+  ctx->dirty = false;
+  return VIRAL_OK;
+}
+
+static unsigned grid_for(const viral_ctx* c, long tiles)
+{
+  long g = tiles;
+  if (c->cfg.blocks_per_cu > 0) g = std::min(g, (long)c->ncu * c->cfg.blocks_per_cu);
+  return (unsigned)g;
+}
+
+// run f(sched, grid, event or -1) once over all events (batched) or once per event
+template <class F>
+static void for_phase(viral_ctx* c, int ph, F&& f)
+{
+  const int nev = (int)c->ev.size();
+  if (c->cfg.batched) {
+    if (c->total[ph] == 0) return;
+    Sched s{c->d_start + (size_t)ph * (nev + 1), c->total[ph], 0, nev, c->cfg.xcd};
+    f(s, grid_for(c, c->total[ph]), -1);
+  } else {
+    for (int e = 0; e < nev; e++) {
+      const long n = c->cnt[ph][e];
+      if (n == 0) continue;
+      Sched s{nullptr, n, e, 1, c->cfg.xcd};
+      f(s, grid_for(c, n), e);
+    }
+  }
+}
+
+template <class F>
+static void with_bool(bool b, F&& f)
+{
+  if (b) f(std::true_type{});
+  else f(std::false_type{});
+}
+
+template <bool P>
+static void launch_tile(viral_ctx* c, int d, int nt, const Sched& s, unsigned grid, size_t shm, int stage, int cur)
+{
+#define TILE_CASE(D, NT)                                                                                    \
+  hipLaunchKernelGGL((k_flux_tile<D, NT, P>), dim3(grid), dim3(NT), shm, c->stream, c->d_tab, s, c->d_clk, \
+                     stage, cur)
+#define TILE_NT(D)                       \
+  switch (nt) {                          \
+    case 64: TILE_CASE(D, 64); break;    \
+    case 128: TILE_CASE(D, 128); break;  \
+    case 512: TILE_CASE(D, 512); break;  \
+    default: TILE_CASE(D, 256); break;   \
+  }
+  if (d == 0) { TILE_NT(0) }
+  else if (d == 1) { TILE_NT(1) }
+  else { TILE_NT(2) }
+#undef TILE_NT
+#undef TILE_CASE
+}
+
+static void launch_flux(viral_ctx* c, int d, int stage, int cur)
+{
+  KTimer t(c, K_FLUX_X + d);
+  with_bool(c->cfg.blocks_per_cu > 0, [&](auto PT) {
+    constexpr bool P = decltype(PT)::value;
+    for_phase(c, PH_FLUX_X + d, [&](const Sched& s, unsigned grid, int e) {
+      if (c->scheme == VIRAL_SHASTA) {
+        if (c->fast)
+          hipLaunchKernelGGL((k_shasta_dir<true, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
+                             stage, cur, d);
+        else
+          hipLaunchKernelGGL((k_shasta_dir<false, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
+                             stage, cur, d);
+      } else if (c->fast) {
+        const size_t shm =
+            e < 0 ? c->shm[d] : 6 * (size_t)c->ev[e].tile_tp[d] * c->ev[e].tile_tk[d] * sizeof(double);
+        launch_tile<P>(c, d, c->cfg.flux_nt[d], s, grid, shm, stage, cur);
+      } else {
+        hipLaunchKernelGGL((k_flux_dir<false, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
+                           stage, cur, d);
+      }
+    });
+  });
+}
+
+static void launch_cells(viral_ctx* c, int stage, int cur)
+{
+  with_bool(c->cfg.blocks_per_cu > 0, [&](auto PT) {
+    constexpr bool P = decltype(PT)::value;
+    with_bool(c->fast, [&](auto FT) {
+      constexpr bool FAST = decltype(FT)::value;
+      if (!c->cfg.fusion) {
+        KTimer t(c, K_SOURCE);
+        for_phase(c, PH_CELLS, [&](const Sched& s, unsigned grid, int) {
+          hipLaunchKernelGGL((k_source<FAST, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk, stage,
+                             cur);
+        });
+      }
+      KTimer t(c, K_UPDATE);
+      for_phase(c, PH_CELLS, [&](const Sched& s, unsigned grid, int) {
+        if (c->cfg.fusion)
+          hipLaunchKernelGGL((k_update<FAST, true, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
+                             stage, cur);
+        else
+          hipLaunchKernelGGL((k_update<FAST, false, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
+                             stage, cur);
+      });
+    });
+  });
+}
+
+// one RK stage of every event: fluxes and sources from copy `cur`, the new
+// state into copy cur^1, then the k0 broadcast and the ghost fill of it
+static void launch_stage(viral_ctx* c, int stage, int cur)
+{
+  for (int d = 0; d < 3; d++) launch_flux(c, d, stage, cur);
+  launch_cells(c, stage, cur);
+  with_bool(c->cfg.blocks_per_cu > 0, [&](auto PT) {
+    constexpr bool P = decltype(PT)::value;
+    {
+      KTimer t(c, K_K0);
+      for_phase(c, PH_K0, [&](const Sched& s, unsigned grid, int) {
+        hipLaunchKernelGGL((k_k0_broadcast<P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk, stage,
+                           cur ^ 1);
+      });
+    }
+    KTimer t(c, K_GHOST);
+    for_phase(c, PH_GHOST, [&](const Sched& s, unsigned grid, int) {
+      hipLaunchKernelGGL((k_ghost_fill<P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, cur ^ 1);
+    });
+  });
+}
+
+// every stage of one step starting from copy cur0, then the clock advance
+static void launch_step(viral_ctx* c, int cur0)
+{
+  int cur = cur0;
+  for (int stage = 0; stage < c->rk; stage++) {
+    launch_stage(c, stage, cur);
+    cur ^= 1;
+  }
+  hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, c->stream, c->d_clk);
+}
+
+static int capture_step(viral_ctx* ctx, int cur0)
+{
+  if (ctx->gexec[cur0]) return VIRAL_OK;
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  launch_step(ctx, cur0);
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamEndCapture(ctx->stream, &g));
+  ctx->graph[cur0] = g;
+  HIPCHK(hipGraphInstantiate(&ctx->gexec[cur0], g, nullptr, nullptr, 0));
+  return VIRAL_OK;
+}
+
 // Host build of the tabulated s95p EOS (see EosTab in physics.h): octaves
 // 2^-13 .. 2^13 fm^-4 (1.2e-4 .. 8192 fm^-4, i.e. up to ~1600 GeV/fm^3;
 // below, the closed form's own exp(x)-1 cancellation is already ~1e-10, so
@@ -894,6 +1400,25 @@ static EosTab host_tab()
   return EosTab{v.data(), c.data(), kTabEmin, kTabNoct, kTabK};
 }
 
+// elapsed ms of reps runs of fn on the context stream (after one warm run)
+template <class F>
+static int time_ms(viral_ctx* ctx, int reps, float* ms, F&& fn)
+{
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  fn();
+  HIPCHK(hipEventRecord(t0, ctx->stream));
+  for (int r = 0; r < reps; r++) fn();
+  HIPCHK(hipEventRecord(t1, ctx->stream));
+  HIPCHK(hipEventSynchronize(t1));
+  HIPCHK(hipEventElapsedTime(ms, t0, t1));
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  HIPCHK(hipGetLastError());
+  return VIRAL_OK;
+}
+
 extern "C" {
 
 void viral_desc_default(viral_desc* d, int nx, int ny, int nz)
@@ -935,6 +1460,19 @@ size_t viral_cells(const viral_desc* d)
   return (size_t)gx * gy * gz;
 }
 
+long long viral_tile_count(const viral_desc* d, int phase, int tp, int tk)
+{
+  if (!d || phase < 0 || phase >= PH_COUNT) return -1;
+  if (phase <= PH_FLUX_Z && d->fast_math && d->scheme == VIRAL_KT && (tp < 8 || tk < 1)) return -1;
+  return tile_count(d, phase, tp, tk);
+}
+
+long long viral_xcd_tile(long long w, long long total)
+{
+  if (total <= 0 || w < 0 || w >= total) return -1;
+  return xcd_tile(w, total);
+}
+
 static int check_desc(viral_ctx* ctx, const viral_desc* d)
 {
   if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(ctx, VIRAL_ERR_ARG, "grid sizes must be positive");
@@ -960,31 +1498,49 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
   viral_ctx* ctx = new viral_ctx();
   for (int e = 0; e < nev; e++) {
     int r = check_desc(ctx, &descs[e]);
+    if (!r && (descs[e].scheme != descs[0].scheme || descs[e].rk != descs[0].rk ||
+               descs[e].fast_math != descs[0].fast_math || descs[e].coords != descs[0].coords))
+      r = VIRAL_ERR_ARG;  // the events of one context step through one launch sequence
     if (r) {
-      std::string m = ctx->err;
       delete ctx;
       return r;
     }
+  }
+  ctx->rk = descs[0].rk;
+  ctx->scheme = descs[0].scheme;
+  ctx->fast = descs[0].fast_math != 0;
+  ctx->cfg.batched = nev > 1;
+  ctx->cfg.graph = 0;
+  ctx->cfg.blocks_per_cu = 0;
+  ctx->cfg.fusion = 0;
+  ctx->cfg.xcd = 0;
+  for (int d = 0; d < 3; d++) ctx->cfg.flux_nt[d] = 256;
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      ctx->ncu = ncu;
   }
   hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (he != hipSuccess) {
     delete ctx;
     return VIRAL_ERR_HIP;
   }
+  bool ok = true;
+  ok = ok && hipMalloc(&ctx->d_tab, nev * sizeof(EvDev)) == hipSuccess;
+  ok = ok && hipMalloc(&ctx->d_start, (size_t)PH_COUNT * (nev + 1) * sizeof(long)) == hipSuccess;
+  ok = ok && hipMalloc(&ctx->d_clk, sizeof(Clock)) == hipSuccess;
+  ok = ok && hipHostMalloc(&ctx->h_clk, sizeof(Clock), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipMalloc(&ctx->d_bad, nev * sizeof(unsigned long long)) == hipSuccess;
+  ok = ok && hipHostMalloc(&ctx->h_bad, nev * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipMalloc(&ctx->dtmp, sizeof(unsigned long long)) == hipSuccess;
   ctx->ev.resize(nev);
-  for (int e = 0; e < nev; e++) {
+  for (int e = 0; e < nev && ok; e++) {
     Event& E = ctx->ev[e];
     const viral_desc* d = &descs[e];
     E.d = *d;
     Geom& g = E.g;
-    viral_ghost_dims(d, &g.gx, &g.gy, &g.gz);
-    g.il = VIRAL_BORDER; g.ir = VIRAL_BORDER + d->nx;
-    g.jl = VIRAL_BORDER; g.jr = VIRAL_BORDER + d->ny;
-    if (d->lbi) { g.kl = 0; g.kr = 1; g.k0 = 0; }
-    else { g.kl = VIRAL_BORDER; g.kr = VIRAL_BORDER + d->nz; g.k0 = d->k0 >= 0 ? d->k0 : (g.gz - 1) / 2; }
-    g.n = (long)g.gx * g.gy * g.gz;
-    g.sx = (long)g.gy * g.gz;
-    g.sy = g.gz;
+    fill_geom(d, g);
     Params& p = E.p;
     p.milne = d->coords == VIRAL_MILNE;
     p.lbi = d->lbi;
@@ -1003,28 +1559,22 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     p.use_tab = 0;
     p.tab = EosTab{nullptr, nullptr, 0, 0, 0};
     size_t n = (size_t)g.n;
-    bool ok = true;
     ok = ok && hipMalloc(&E.st[0], NF * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.st[1], NF * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.var0, SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.var1, SV * n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&E.var2, SV * n * sizeof(double)) == hipSuccess;
-    for (int q = 0; q < 3; q++) ok = ok && hipMalloc(&E.flux[q], SV * n * sizeof(double)) == hipSuccess;
+    if (d->rk == 3) ok = ok && hipMalloc(&E.var2, SV * n * sizeof(double)) == hipSuccess;
+    for (int q = 0; q < (d->lbi ? 2 : 3); q++) ok = ok && hipMalloc(&E.flux[q], SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.src, SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.iters, 3 * n * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc(&E.flags, n) == hipSuccess;
-    ok = ok && hipMalloc(&E.bad, sizeof(unsigned long long)) == hipSuccess;
-    if (!ok) {
-      viral_destroy(ctx);
-      return VIRAL_ERR_HIP;
-    }
+    if (!ok) break;
     {
       // default LDS tiles: 64-cell lines x 16 lines across x and y; along eta
       // the whole ghosted line when it fits (one tile per line)
       const int gz = g.gz;
-      E.tile_tp[2] = gz <= 128 ? gz : 64;
-      E.tile_tk[2] = 1024 / E.tile_tp[2] > 0 ? 1024 / E.tile_tp[2] : 1;
-      if (E.tile_tp[2] * E.tile_tk[2] > 1024) E.tile_tk[2] = 1024 / E.tile_tp[2];
+      E.tile_tp[2] = gz <= 128 ? std::max(gz, 8) : 64;  // lbi (gz = 1): the eta phase is empty anyway
+      E.tile_tk[2] = std::max(1, 1024 / E.tile_tp[2]);
     }
     if (d->eos_table && d->eos == VIRAL_EOS_S95P) {
       std::vector<double> tv;
@@ -1032,10 +1582,7 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
       build_eos_table(tv, tc);
       ok = ok && hipMalloc(&E.tab_v, tv.size() * sizeof(double)) == hipSuccess;
       ok = ok && hipMalloc(&E.tab_closed, tc.size()) == hipSuccess;
-      if (!ok) {
-        viral_destroy(ctx);
-        return VIRAL_ERR_HIP;
-      }
+      if (!ok) break;
       hipMemcpy(E.tab_v, tv.data(), tv.size() * sizeof(double), hipMemcpyHostToDevice);
       hipMemcpy(E.tab_closed, tc.data(), tc.size(), hipMemcpyHostToDevice);
       p.use_tab = 1;
@@ -1046,11 +1593,14 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     hipMemset(E.iters, 0, 3 * n * sizeof(int32_t));
     hipMemset(E.flags, 0, n);
   }
-  if (hipMalloc(&ctx->dtmp, sizeof(unsigned long long)) != hipSuccess) {
+  if (!ok || hipDeviceSynchronize() != hipSuccess) {
     viral_destroy(ctx);
     return VIRAL_ERR_HIP;
   }
-  if (hipDeviceSynchronize() != hipSuccess) {
+  ctx->h_clk->tau = 1.0;
+  ctx->h_clk->ts = 0.01;
+  ctx->h_clk->step = 0;
+  if (refresh_table(ctx) != VIRAL_OK) {
     viral_destroy(ctx);
     return VIRAL_ERR_HIP;
   }
@@ -1061,14 +1611,24 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
 void viral_destroy(viral_ctx* ctx)
 {
   if (!ctx) return;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  drop_graphs(ctx);
   for (Event& E : ctx->ev) {
-    hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1); hipFree(E.var2);
-    for (int q = 0; q < 3; q++) hipFree(E.flux[q]);
+    hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1);
+    if (E.var2) hipFree(E.var2);
+    for (int q = 0; q < 3; q++)
+      if (E.flux[q]) hipFree(E.flux[q]);
     hipFree(E.src);
-    hipFree(E.iters); hipFree(E.flags); hipFree(E.bad);
+    hipFree(E.iters); hipFree(E.flags);
     if (E.tab_v) hipFree(E.tab_v);
     if (E.tab_closed) hipFree(E.tab_closed);
   }
+  if (ctx->d_tab) hipFree(ctx->d_tab);
+  if (ctx->d_start) hipFree(ctx->d_start);
+  if (ctx->d_clk) hipFree(ctx->d_clk);
+  if (ctx->h_clk) hipHostFree(ctx->h_clk);
+  if (ctx->d_bad) hipFree(ctx->d_bad);
+  if (ctx->h_bad) hipHostFree(ctx->h_bad);
   if (ctx->dtmp) hipFree(ctx->dtmp);
   for (int k = 0; k < K_COUNT; k++)
     for (hipEvent_t e : ctx->tev[k]) hipEventDestroy(e);
@@ -1097,7 +1657,6 @@ int viral_upload_state(viral_ctx* ctx, int ev, const double* const* fields)
     HIPCHK(hipMemcpyAsync(E->st[0] + f * n, fields[f], n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(E->st[1] + f * n, fields[f], n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   }
-  E->cur = 0;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return VIRAL_OK;
 }
@@ -1107,7 +1666,7 @@ int viral_download_field(viral_ctx* ctx, int ev, int f, double* out)
   Event* E = get_ev(ctx, ev);
   if (!E || !out || f < 0 || f >= NF) return fail(ctx, VIRAL_ERR_ARG, "bad event, field or pointer");
   size_t n = (size_t)E->g.n;
-  HIPCHK(hipMemcpyAsync(out, E->st[E->cur] + f * n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(out, E->st[ctx->cur] + f * n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return VIRAL_OK;
 }
@@ -1119,120 +1678,60 @@ int viral_download_state(viral_ctx* ctx, int ev, double* const* fields)
   size_t n = (size_t)E->g.n;
   for (int f = 0; f < NF; f++) {
     if (!fields[f]) return fail(ctx, VIRAL_ERR_ARG, "null field pointer");
-    HIPCHK(hipMemcpyAsync(fields[f], E->st[E->cur] + f * n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(fields[f], E->st[ctx->cur] + f * n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return VIRAL_OK;
 }
 
-// fast KT flux of direction d on LDS tiles of TP cells x TK lines
-static void launch_flux_tile(viral_ctx* ctx, Event& E, int d, double tau, const double* si, int TP, int TK)
+int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
 {
-  const Geom& g = E.g;
-  const int na = d == 0 ? g.jr - g.jl : g.ir - g.il;
-  const int nb = d == 2 ? g.jr - g.jl : g.kr - g.kl;
-  const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
-  const long nlines = (long)na * nb;
-  const int nseg = (hi - lo + 1 + TP - 6) / (TP - 5);
-  const unsigned blocks = (unsigned)(((nlines + TK - 1) / TK) * nseg);
-  const size_t shm = 6 * (size_t)TP * TK * sizeof(double);
-  if (d == 0)
-    hipLaunchKernelGGL(k_flux_tile<0>, dim3(blocks), dim3(BLK), shm, ctx->stream, g, E.p, tau, si, E.flux[d], TP, TK);
-  else if (d == 1)
-    hipLaunchKernelGGL(k_flux_tile<1>, dim3(blocks), dim3(BLK), shm, ctx->stream, g, E.p, tau, si, E.flux[d], TP, TK);
-  else
-    hipLaunchKernelGGL(k_flux_tile<2>, dim3(blocks), dim3(BLK), shm, ctx->stream, g, E.p, tau, si, E.flux[d], TP, TK);
-}
-
-// one stage on one event: fluxes at tau_src from copy `in`, update into `out`
-static int launch_stage(viral_ctx* ctx, Event& E, const StageArgs& a)
-{
-  const Geom& g = E.g;
-  const Params& p = E.p;
-  const double* si = E.st[E.cur];
-  double* so = E.st[E.cur ^ 1];
-  const int ndir = p.lbi ? 2 : 3;
-  const bool fast = E.d.fast_math != 0;
-  for (int d = 0; d < ndir; d++) {
-    Box fb = face_box(g, d);
-    long nf = box_count(fb);
-    KTimer t(ctx, K_FLUX_X + d);
-    if (p.scheme == VIRAL_SHASTA) {
-      long ni = box_count(interior_box(g));
-      if (fast)
-        hipLaunchKernelGGL(k_shasta_dir<true>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src,
-                           a.taustep, si, E.flux[d]);
-      else
-        hipLaunchKernelGGL(k_shasta_dir<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src,
-                           a.taustep, si, E.flux[d]);
-    } else if (fast) {
-      launch_flux_tile(ctx, E, d, a.tau_src, si, E.tile_tp[d], E.tile_tk[d]);
-    } else
-      hipLaunchKernelGGL(k_flux_dir<false>, dim3(nblocks(nf)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si,
-                         E.flux[d]);
-  }
-  long ni = box_count(interior_box(g));
-  {
-    KTimer t(ctx, K_SOURCE);
-    if (fast)
-      hipLaunchKernelGGL(k_source<true>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, E.src);
-    else
-      hipLaunchKernelGGL(k_source<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, E.src);
-  }
-  {
-    KTimer t(ctx, K_UPDATE);
-    if (fast)
-      hipLaunchKernelGGL(k_update<true>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, so, E.flux[0],
-                         E.flux[1], E.flux[2], E.src, E.var0, E.var1, E.var2, E.iters + (long)a.stage * g.n, E.flags,
-                         E.bad);
-    else
-      hipLaunchKernelGGL(k_update<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a, si, so, E.flux[0],
-                         E.flux[1], E.flux[2], E.src, E.var0, E.var1, E.var2, E.iters + (long)a.stage * g.n, E.flags,
-                         E.bad);
-  }
-  if ((p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi) {
-    KTimer t(ctx, K_K0);
-    hipLaunchKernelGGL(k_k0_broadcast, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, g, p, a.tau_new, so);
-  }
-  int nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
-  long ng = 2L * VIRAL_BORDER * ((long)ny * nz + (long)nx * nz + (p.lbi ? 0 : (long)nx * ny));
-  {
-    KTimer t(ctx, K_GHOST);
-    hipLaunchKernelGGL(k_ghost_fill, dim3(nblocks(ng), 28), dim3(BLK), 0, ctx->stream, g, p, so);
-  }
-  E.cur ^= 1;
-  HIPCHK(hipGetLastError());
-  return VIRAL_OK;
-}
-
-int viral_step(viral_ctx* ctx, double tau, double ts)
-{
-  if (!ctx) return VIRAL_ERR_ARG;
-  for (Event& E : ctx->ev) {
-    HIPCHK(hipMemsetAsync(E.bad, 0xff, sizeof(unsigned long long), ctx->stream));
-    const int rk = E.p.rk;
-    StageArgs a{};
-    a.ts = ts;
-    a.taustep = rk == 3 ? ts / 3.0 : rk == 2 ? ts / 2 : ts;
-    // stage 1: NEWEST10/time.h:345-363 (RK2), 402-426 (RK3), 307-325 (RK1)
-    a.tau_src = tau; a.ts_src = ts; a.tau_new = tau + ts; a.stage = 0; a.kind = 0;
-    if (int r = launch_stage(ctx, E, a)) return r;
-    if (rk == 2) {
-      a.tau_src = tau + ts; a.ts_src = ts; a.tau_new = tau + ts; a.stage = 1; a.kind = 1;
-      if (int r = launch_stage(ctx, E, a)) return r;
-    } else if (rk == 3) {
-      a.tau_src = tau + ts; a.ts_src = ts; a.tau_new = tau + ts / 2.0; a.stage = 1; a.kind = 2;
-      if (int r = launch_stage(ctx, E, a)) return r;
-      a.tau_src = tau + ts / 2.0; a.ts_src = ts / 2.0; a.tau_new = tau + ts; a.stage = 2; a.kind = 3;
-      if (int r = launch_stage(ctx, E, a)) return r;
+  if (!ctx || nsteps < 0) return VIRAL_ERR_ARG;
+  if (done) *done = 0;
+  if (nsteps == 0) return VIRAL_OK;
+  if (nsteps >= (1 << 22)) return fail(ctx, VIRAL_ERR_ARG, "at most 2^22 - 1 steps per call");
+  if (int r = refresh_table(ctx)) return r;
+  const int nev = (int)ctx->ev.size();
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // the pinned clock may still be in flight
+  ctx->h_clk->tau = tau;
+  ctx->h_clk->ts = ts;
+  ctx->h_clk->step = 0;
+  HIPCHK(hipMemcpyAsync(ctx->d_clk, ctx->h_clk, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_clear_bad, dim3((nev + 63) / 64), dim3(64), 0, ctx->stream, ctx->d_bad, nev);
+  const bool graph = ctx->cfg.graph && !ctx->timing;
+  for (int s = 0; s < nsteps; s++) {
+    if (graph) {
+      if (int r = capture_step(ctx, ctx->cur)) return r;
+      HIPCHK(hipGraphLaunch(ctx->gexec[ctx->cur], ctx->stream));
+    } else {
+      launch_step(ctx, ctx->cur);
     }
-    HIPCHK(hipMemcpyAsync(&E.bad_host, E.bad, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->cur ^= (ctx->rk & 1);
   }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(ctx->h_bad, ctx->d_bad, nev * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  for (Event& E : ctx->ev)
-    if (E.bad_host != ~0ull) return fail(ctx, VIRAL_ERR_PHYSICS, "a cell became acausal, non-positive or unsolvable");
+  double t = tau;
+  for (int s = 0; s < nsteps; s++) t = t + ts;
+  ctx->last_tau = t;
+  ctx->last_ts = ts;
+  long long first = -1;
+  for (int e = 0; e < nev; e++) {
+    ctx->ev[e].bad_host = ctx->h_bad[e];
+    if (ctx->h_bad[e] != ~0ull) {
+      const long long st = (long long)(ctx->h_bad[e] >> 42);
+      if (first < 0 || st < first) first = st;
+    }
+  }
+  if (first >= 0) {
+    if (done) *done = (int)first;
+    return fail(ctx, VIRAL_ERR_PHYSICS, "a cell became acausal, non-positive or unsolvable");
+  }
+  if (done) *done = nsteps;
   return VIRAL_OK;
 }
+
+int viral_step(viral_ctx* ctx, double tau, double ts) { return viral_steps(ctx, tau, ts, 1, nullptr); }
 
 int viral_flags(viral_ctx* ctx, int ev, uint8_t* out)
 {
@@ -1258,8 +1757,8 @@ int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* st
   Event* E = get_ev(ctx, ev);
   if (!E || !i || !j || !k || !stage || !flags) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
   if (E->bad_host == ~0ull) return -1;
-  long x = (long)((E->bad_host >> 8) & ((1ull << 48) - 1));
-  *stage = (int)(E->bad_host >> 56);
+  long x = (long)((E->bad_host >> 8) & 0xffffffffull);
+  *stage = (int)((E->bad_host >> 40) & 3);
   *k = (int)(x % E->g.gz);
   long r = x / E->g.gz;
   *j = (int)(r % E->g.gy);
@@ -1274,7 +1773,7 @@ int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)
   if (!E || !out) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
   HIPCHK(hipMemsetAsync(ctx->dtmp, 0, sizeof(unsigned long long), ctx->stream));
   long ni = box_count(interior_box(E->g));
-  hipLaunchKernelGGL(k_max_temp, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, E->g, E->p, E->st[E->cur], ctx->dtmp);
+  hipLaunchKernelGGL(k_max_temp, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, E->g, E->p, E->st[ctx->cur], ctx->dtmp);
   unsigned long long bits = 0;
   HIPCHK(hipMemcpyAsync(&bits, ctx->dtmp, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1309,21 +1808,13 @@ int viral_eos_eval(const viral_desc* d, size_t n, const double* e, double* p, do
   return VIRAL_OK;
 }
 
-// any tile with at least 3 faces per line and at most 4 cells per thread
-// (cells past the end of a line are masked)
-static bool tile_ok(const Event& E, int d, int TP, int TK)
-{
-  (void)E;
-  (void)d;
-  return TP >= 8 && TK >= 1 && TP * TK <= 4 * BLK;
-}
-
 int viral_set_tile(viral_ctx* ctx, int ev, int dir, int tp, int tk)
 {
   Event* E = get_ev(ctx, ev);
-  if (!E || dir < 0 || dir > 2 || !tile_ok(*E, dir, tp, tk)) return fail(ctx, VIRAL_ERR_ARG, "bad tile");
+  if (!E || dir < 0 || dir > 2 || !tile_ok(tp, tk, ctx->cfg.flux_nt[dir])) return fail(ctx, VIRAL_ERR_ARG, "bad tile");
   E->tile_tp[dir] = tp;
   E->tile_tk[dir] = tk;
+  ctx->dirty = true;
   return VIRAL_OK;
 }
 
@@ -1336,47 +1827,178 @@ int viral_get_tile(viral_ctx* ctx, int ev, int dir, int* tp, int* tk)
   return VIRAL_OK;
 }
 
+int viral_get_launch(viral_ctx* ctx, viral_launch* out)
+{
+  if (!ctx || !out) return VIRAL_ERR_ARG;
+  *out = ctx->cfg;
+  return VIRAL_OK;
+}
+
+int viral_set_launch(viral_ctx* ctx, const viral_launch* in)
+{
+  if (!ctx || !in) return VIRAL_ERR_ARG;
+  if (in->blocks_per_cu < 0 || in->blocks_per_cu > 64) return fail(ctx, VIRAL_ERR_ARG, "blocks_per_cu in [0, 64]");
+  for (int d = 0; d < 3; d++) {
+    if (!max_slots(in->flux_nt[d])) return fail(ctx, VIRAL_ERR_ARG, "flux_nt must be 64, 128, 256 or 512");
+    for (const Event& E : ctx->ev)
+      if (!tile_ok(E.tile_tp[d], E.tile_tk[d], in->flux_nt[d]))
+        return fail(ctx, VIRAL_ERR_ARG, "an event's flux tile does not fit flux_nt (set smaller tiles first)");
+  }
+  viral_launch c = *in;
+  c.batched = in->batched != 0;
+  c.graph = in->graph != 0;
+  c.fusion = in->fusion != 0;
+  c.xcd = in->xcd != 0;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->cfg = c;
+  ctx->dirty = true;
+  return VIRAL_OK;
+}
+
+int viral_tile_table(viral_ctx* ctx, int phase, long long* start)
+{
+  if (!ctx || !start || phase < 0 || phase >= PH_COUNT) return fail(ctx, VIRAL_ERR_ARG, "bad tile table query");
+  if (int r = refresh_table(ctx)) return r;
+  long long acc = 0;
+  for (size_t e = 0; e < ctx->ev.size(); e++) {
+    start[e] = acc;
+    acc += ctx->cnt[phase][e];
+  }
+  start[ctx->ev.size()] = acc;
+  return VIRAL_OK;
+}
+
+// Autotuner.  Times launches on the current state at the clock of the last
+// step (stage 0 writes only the spare state copy, the RK registers and the
+// per-step diagnostics, which the next step rewrites) and keeps the fastest:
+//   ev >= 0: the fast KT flux tile shape of event ev per direction;
+//   ev <  0: for the whole context, per direction the flux block size and one
+//            tile shape for every event (or each event's whole eta line),
+//            then the fusion depth, the persistent-block count and the XCD
+//            tile order, each timed over a whole stage.
 int viral_autotune(viral_ctx* ctx, int ev, int reps)
 {
-  Event* E = get_ev(ctx, ev);
-  if (!E || reps < 1) return fail(ctx, VIRAL_ERR_ARG, "bad autotune call");
-  if (!E->d.fast_math || E->d.scheme != VIRAL_KT) return VIRAL_OK;  // only the fast KT flux is tiled
-  static const int shapes[][2] = {{32, 16}, {32, 32}, {64, 8}, {64, 16}, {128, 4}, {128, 8}, {256, 4}, {16, 64}};
-  hipEvent_t t0, t1;
-  HIPCHK(hipEventCreate(&t0));
-  HIPCHK(hipEventCreate(&t1));
-  const int ndir = E->p.lbi ? 2 : 3;
-  for (int d = 0; d < ndir; d++) {
-    std::vector<std::pair<int, int>> cand;
-    for (auto& sh : shapes) cand.push_back({sh[0], sh[1]});
-    if (d == 2) {  // whole eta lines
-      const int tp = E->g.gz;
-      for (int tk = 1; tp * tk <= 4 * BLK; tk *= 2) cand.push_back({tp, tk});
-    }
-    float best = 1e30f;
-    int btp = E->tile_tp[d], btk = E->tile_tk[d];
-    const int G = d == 0 ? E->g.gx : d == 1 ? E->g.gy : E->g.gz;
-    for (auto& c : cand) {
-      if (!tile_ok(*E, d, c.first, c.second) || c.first > G + 8) continue;
-      launch_flux_tile(ctx, *E, d, 1.0, E->st[E->cur], c.first, c.second);  // warm
-      HIPCHK(hipEventRecord(t0, ctx->stream));
-      for (int r = 0; r < reps; r++) launch_flux_tile(ctx, *E, d, 1.0, E->st[E->cur], c.first, c.second);
-      HIPCHK(hipEventRecord(t1, ctx->stream));
-      HIPCHK(hipEventSynchronize(t1));
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, t0, t1));
-      if (ms < best) {
-        best = ms;
-        btp = c.first;
-        btk = c.second;
+  if (!ctx || reps < 1 || ev >= (int)ctx->ev.size()) return fail(ctx, VIRAL_ERR_ARG, "bad autotune call");
+  if (ctx->timing) return fail(ctx, VIRAL_ERR_ARG, "switch the timers off before autotuning");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->h_clk->tau = ctx->last_tau;
+  ctx->h_clk->ts = ctx->last_ts;
+  ctx->h_clk->step = 0;
+  HIPCHK(hipMemcpyAsync(ctx->d_clk, ctx->h_clk, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+  static const int shapes[][2] = {{32, 8},  {32, 16}, {32, 32}, {64, 4},  {64, 8},   {64, 16},
+                                  {128, 4}, {128, 8}, {256, 4}, {16, 32}, {16, 64}, {0, 0}};
+  static const int nts[] = {64, 128, 256, 512};
+  const int nev = (int)ctx->ev.size();
+  const int cur = ctx->cur;
+  if (ctx->fast && ctx->scheme == VIRAL_KT) {
+    for (int d = 0; d < 3; d++) {
+      bool any = false;
+      for (int e = 0; e < nev; e++) any = any || !(d == 2 && ctx->ev[e].d.lbi);
+      if (!any) continue;
+      float best = 1e30f;
+      int bnt = ctx->cfg.flux_nt[d];
+      std::vector<int> btp(nev), btk(nev);
+      for (int e = 0; e < nev; e++) {
+        btp[e] = ctx->ev[e].tile_tp[d];
+        btk[e] = ctx->ev[e].tile_tk[d];
       }
+      for (int nt : nts) {
+        if (ev >= 0 && nt != ctx->cfg.flux_nt[d]) continue;  // one event: keep the context's block size
+        for (auto& sh : shapes) {
+          // candidate: every event (or just ev) gets shape sh; {0,0} = whole ghosted line along eta
+          if (sh[0] == 0 && d != 2) continue;
+          bool valid = true;
+          for (int e = 0; e < nev; e++) {
+            if (ev >= 0 && e != ev) continue;
+            Event& E = ctx->ev[e];
+            if (d == 2 && E.d.lbi) continue;  // no eta phase
+            const int G = d == 0 ? E.g.gx : d == 1 ? E.g.gy : E.g.gz;
+            int tp = sh[0], tk = sh[1];
+            if (tp == 0) {
+              tp = std::max(8, G);
+              tk = 1;
+              while ((long)tp * tk * 2 <= max_slots(nt)) tk *= 2;
+            }
+            if (!tile_ok(tp, tk, nt) || (tp > G + 8 && sh[0] != 0)) {
+              valid = false;
+              break;
+            }
+            E.tile_tp[d] = tp;
+            E.tile_tk[d] = tk;
+          }
+          if (!valid) continue;
+          const int keep_nt = ctx->cfg.flux_nt[d];
+          ctx->cfg.flux_nt[d] = nt;
+          ctx->dirty = true;
+          if (int r = refresh_table(ctx)) return r;
+          float ms = 0;
+          int r;
+          if (ev >= 0) {
+            Sched s{nullptr, ctx->cnt[PH_FLUX_X + d][ev], ev, 1, ctx->cfg.xcd};
+            const size_t shm = 6 * (size_t)ctx->ev[ev].tile_tp[d] * ctx->ev[ev].tile_tk[d] * sizeof(double);
+            const unsigned grid = grid_for(ctx, s.total);
+            if (ctx->cfg.blocks_per_cu > 0)
+              r = time_ms(ctx, reps, &ms, [&] { launch_tile<true>(ctx, d, nt, s, grid, shm, 0, cur); });
+            else
+              r = time_ms(ctx, reps, &ms, [&] { launch_tile<false>(ctx, d, nt, s, grid, shm, 0, cur); });
+          } else {
+            r = time_ms(ctx, reps, &ms, [&] { launch_flux(ctx, d, 0, cur); });
+          }
+          ctx->cfg.flux_nt[d] = keep_nt;
+          if (r) return r;
+          if (ms < best) {
+            best = ms;
+            bnt = nt;
+            for (int e = 0; e < nev; e++) {
+              btp[e] = ctx->ev[e].tile_tp[d];
+              btk[e] = ctx->ev[e].tile_tk[d];
+            }
+          }
+        }
+      }
+      ctx->cfg.flux_nt[d] = bnt;
+      for (int e = 0; e < nev; e++) {
+        ctx->ev[e].tile_tp[d] = btp[e];
+        ctx->ev[e].tile_tk[d] = btk[e];
+      }
+      ctx->dirty = true;
     }
-    E->tile_tp[d] = btp;
-    E->tile_tk[d] = btk;
   }
-  hipEventDestroy(t0);
-  hipEventDestroy(t1);
-  HIPCHK(hipGetLastError());
+  if (ev < 0) {
+    auto stage_ms = [&](float* ms) -> int {
+      ctx->dirty = true;
+      if (int r = refresh_table(ctx)) return r;
+      return time_ms(ctx, reps, ms, [&] { launch_stage(ctx, 0, cur); });
+    };
+    float best = 1e30f, ms = 0;
+    int bf = ctx->cfg.fusion;
+    for (int f = 0; f < 2; f++) {
+      ctx->cfg.fusion = f;
+      if (int r = stage_ms(&ms)) return r;
+      if (ms < best) { best = ms; bf = f; }
+    }
+    ctx->cfg.fusion = bf;
+    static const int bpcs[] = {0, 1, 2, 4, 8};
+    best = 1e30f;
+    int bb = ctx->cfg.blocks_per_cu;
+    for (int b : bpcs) {
+      ctx->cfg.blocks_per_cu = b;
+      if (int r = stage_ms(&ms)) return r;
+      if (ms < best) { best = ms; bb = b; }
+    }
+    ctx->cfg.blocks_per_cu = bb;
+    best = 1e30f;
+    int bx = ctx->cfg.xcd;
+    for (int x = 0; x < 2; x++) {
+      ctx->cfg.xcd = x;
+      if (int r = stage_ms(&ms)) return r;
+      if (ms < best) { best = ms; bx = x; }
+    }
+    ctx->cfg.xcd = bx;
+  }
+  ctx->dirty = true;
+  if (int r = refresh_table(ctx)) return r;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return VIRAL_OK;
 }
 
