@@ -54,10 +54,20 @@ enum { VIRAL_TPI_OVER_EP = 0,      /* tau_pi = (c*eta)/(e+p)   */
 #define VIRAL_Q_DPDE_OF_P     (1u << 2) /* q3: a = DPDE(P) in the CON eigenvalues (NEWEST10/maindef.h:490) */
 #define VIRAL_Q_GUBSER_MIRROR (1u << 3) /* q7: mirror u^1,u^2 into x/y ghosts (NEWEST10/boundary.h:32-37,77-81,
                                            118-122,160-165); off by default: only the GUBSER profile has it */
-#define VIRAL_Q_DEFAULT (VIRAL_Q_GHOST_EIGEN | VIRAL_Q_K0_BROADCAST | VIRAL_Q_DPDE_OF_P)
+#define VIRAL_Q_SHEAR_SLICES  (1u << 4) /* q9, Cartesian only: shear sources only in ghosted eta slices k < nz,
+                                           as the reference's loop k = 0..ZCMA-1 (TXYZ5/Source.h:272); the top 3
+                                           interior slices keep the zero sources of the freshly allocated grid */
+#define VIRAL_Q_DEFAULT (VIRAL_Q_GHOST_EIGEN | VIRAL_Q_K0_BROADCAST | VIRAL_Q_DPDE_OF_P | VIRAL_Q_SHEAR_SLICES)
 
 /* State fields, each one ghosted array of doubles.  Names follow the
- * reference's cell members (NEWEST10/maindef.h:399-405). */
+ * reference's cell members (NEWEST10/maindef.h:399-405).
+ *
+ * Cartesian (TXYZ5) events use the same arrays: T^{t mu} in VIRAL_F_T00..T30,
+ * the five evolved shear components of TXYZ5/maindef.h:240 (pi[0..4] =
+ * pi^{xx}, pi^{yy}, pi^{xy}, pi^{xz}, pi^{yz}) in VIRAL_F_PI0..PI0+4, and
+ * VIRAL_F_VE = v^z.  Slots VIRAL_F_PI0+5..PI0+9 are not evolved (the step only
+ * copies them into ghosts).  The other five components follow from
+ * u_mu pi^{mu nu} = 0 and pi^mu_mu = 0. */
 enum {
   VIRAL_F_T00 = 0, VIRAL_F_T10, VIRAL_F_T20, VIRAL_F_T30,       /* T^{tau mu}                     */
   VIRAL_F_PI0,                                                  /* pi[0..9] = pi^{tt,tx,ty,te,xx,xy,xe,yy,ye,ee}

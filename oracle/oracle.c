@@ -1,5 +1,6 @@
 /*
- * oracle.c -- CPU restatement of one VIRAL viscous hydro step (Milne KT path).
+ * oracle.c -- CPU restatement of one VIRAL viscous hydro step (Milne KT and
+ * SHASTA paths of NEWEST10, Cartesian KT path of TXYZ5).
  *
  * TEST INFRASTRUCTURE: the parity oracle (see oracle.h).  Built with
  * -O2 -ffp-contract=off so every expression rounds in the order written.
@@ -18,6 +19,11 @@
  *   EOS .................. NEWEST10/maindef.h:68-72,320-324 (conformal),
  *                          NEWEST10/s95p.h:13-123 (s95p)
  *   ghost fill ........... NEWEST10/boundary.h:1-275
+ *   Cartesian ............ TXYZ5/hydroKT.h:16-62 (flux), TXYZ5/Source.h:
+ *                          136-157,270-330 (sources), TXYZ5/time.h:100-165,
+ *                          TXYZ5/multiroot.h:187-479 (4-D root),
+ *                          TXYZ5/boundary.h; derived shear components from
+ *                          NEWEST5/maindef.h:497-501 at tau = 1
  *
  * Deliberate differences from the reference text (all below 1e-15 relative):
  *  - x^(1/4) of the conformal temperature is sqrt(sqrt(x)) instead of
@@ -26,10 +32,14 @@
  *  - the Israel-Stewart shear source is evaluated in covariant tensor form
  *    (see shear_source below) instead of the reference's expanded
  *    computer-algebra output; agreement with the reference's own CalcSource
- *    is pinned by tests/test_oracle_vs_reference.py through oracle/_ref.
+ *    is pinned by tests/test_oracle_vs_reference.py through oracle/_ref;
+ *  - the Cartesian root restates GSL's gnewton and LU routines (GSL is not
+ *    in the reference tree; version unpinned), and flags a cell whose
+ *    solution is not finite, which the reference would carry on silently.
  */
 #include "oracle.h"
 #include "vmath.h"
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -65,6 +75,51 @@ struct or_ctx {
 };
 
 static inline size_t IX(const or_ctx* c, int i, int j, int k) { return ((size_t)i * c->gy + j) * c->gz + k; }
+
+/* Rows of the 15-row Var/flux/source layout a path evolves: Milne all 15
+ * (T^{tau mu}, 10 pi, PI); Cartesian (TXYZ5 SVAR = 10, TXYZ5/maindef.h:196)
+ * T^{t mu}, the 5 evolved pi in rows 4..8 and PI in row 14. */
+static inline int row_used(const or_ctx* c, int l) { return c->d.coords == VIRAL_MILNE || l < 9 || l == 14; }
+
+/* Full Cartesian pi^{mu nu} (slots {tt,tx,ty,tz,xx,xy,xz,yy,yz,zz}) from the
+ * five evolved components p = {xx, yy, xy, xz, yz} (TXYZ5/maindef.h:240):
+ *   u_mu pi^{mu x} = 0 -> pi^{tx} = (u^x pi^{xx} + u^y pi^{xy} + u^z pi^{xz})/u^t
+ *   u_mu pi^{mu y} = 0 -> pi^{ty} = (u^x pi^{xy} + u^y pi^{yy} + u^z pi^{yz})/u^t
+ *   u_mu pi^{mu t} = 0 and pi^mu_mu = 0 -> pi^{zz} below, pi^{tt} = xx+yy+zz
+ *   u_mu pi^{mu z} = 0 -> pi^{tz} = (u^x pi^{xz} + u^y pi^{yz} + u^z pi^{zz})/u^t
+ * the A1..A5 of NEWEST5/maindef.h:497-501 at tau = 1. */
+void or_cart_pi10(const double u[4], const double* p, double pi10[10])
+{
+  const double u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
+  const double p1 = p[0], p2 = p[1], p3 = p[2], p4 = p[3], p5 = p[4];
+  const double zz = (p1 * u1 * u1 + 2.0 * p3 * u1 * u2 + p2 * u2 * u2 + 2.0 * u3 * (p4 * u1 + p5 * u2) -
+                     u0 * u0 * (p1 + p2)) /
+                    (u0 * u0 - u3 * u3);
+  pi10[0] = p1 + p2 + zz;
+  pi10[1] = (p1 * u1 + p3 * u2 + p4 * u3) / u0;
+  pi10[2] = (p3 * u1 + p2 * u2 + p5 * u3) / u0;
+  pi10[3] = (p4 * u1 + p5 * u2 + zz * u3) / u0;
+  pi10[4] = p1;
+  pi10[5] = p3;
+  pi10[6] = p4;
+  pi10[7] = p2;
+  pi10[8] = p5;
+  pi10[9] = zz;
+}
+
+/* slot of the full tensor holding Cartesian evolved component l */
+static const int CART5[5] = {4, 7, 5, 6, 8};
+
+static void cell_pi10(const or_ctx* c, size_t x, const double u[4], double pi10[10])
+{
+  if (c->d.coords == VIRAL_MILNE) {
+    for (int l = 0; l < 10; l++) pi10[l] = c->f[VIRAL_F_PI0 + l][x];
+  } else {
+    double p[5];
+    for (int l = 0; l < 5; l++) p[l] = c->f[VIRAL_F_PI0 + l][x];
+    or_cart_pi10(u, p, pi10);
+  }
+}
 
 /* ------------------------------------------------------------------ EOS */
 
@@ -401,7 +456,18 @@ static void shear_source(int milne, double tau, const double u[4], const double 
     }
 }
 
-/* geometric + shear sources on the interior, NEWEST10/Source.h:195-536 */
+void or_shear_source(int milne, double tau, const double u[4], const double pi10[10], const double du16[16],
+                     double SH, double tpi, double out[10])
+{
+  double du[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) du[a][b] = du16[4 * a + b];
+  shear_source(milne, tau, u, pi10, du, SH, tpi, out);
+}
+
+/* geometric + shear sources on the interior, NEWEST10/Source.h:195-536
+ * (Cartesian: TXYZ5/Source.h:136-157 zero geometric terms, 270-330 shear for
+ * the five evolved components; under q9 only ghosted k < nz) */
 void or_calc_source(or_ctx* c, double tau, double ts, double* S)
 {
   const viral_desc* d = &c->d;
@@ -489,7 +555,7 @@ void or_calc_source(or_ctx* c, double tau, double ts, double* S)
           else
             du[l][3] = 0;
         }
-        for (int l = 0; l < 10; l++) pi[l] = f[VIRAL_F_PI0 + l][x];
+        cell_pi10(c, x, u, pi);
         double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x];
         double T = or_eos_T(d, e);
         double s = or_eos_s(d, e, P, T);
@@ -497,7 +563,13 @@ void or_calc_source(or_ctx* c, double tau, double ts, double* S)
         double tpi = tpi_of(d, SH, P, e);
         double out[10];
         shear_source(milne, tau, u, pi, du, SH, tpi, out);
-        for (int l = 0; l < 10; l++) S[(4 + l) * n + x] = out[l];
+        if (milne) {
+          for (int l = 0; l < 10; l++) S[(4 + l) * n + x] = out[l];
+        } else {
+          const int live = !(d->quirks & VIRAL_Q_SHEAR_SLICES) || k < d->nz;
+          for (int l = 0; l < 5; l++) S[(4 + l) * n + x] = live ? out[CART5[l]] : 0.0;
+          for (int l = 5; l < 10; l++) S[(4 + l) * n + x] = 0.0;
+        }
       }
 }
 
@@ -513,7 +585,7 @@ static void centre_flux(or_ctx* c, int dir, double tau, double* F)
     double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x], PI = f[VIRAL_F_BULK][x];
     double u[4] = {f[VIRAL_F_U0][x], f[VIRAL_F_U1][x], f[VIRAL_F_U2][x], f[VIRAL_F_U3][x]};
     double p[10];
-    for (int l = 0; l < 10; l++) p[l] = f[VIRAL_F_PI0 + l][x];
+    cell_pi10(c, x, u, p);
     double ud = u[dir + 1];
     double tf = milne ? tau : 1.0;
     int row = dir + 1;
@@ -530,10 +602,17 @@ static void centre_flux(or_ctx* c, int dir, double tau, double* F)
       }
       F[mu * n + x] = milne ? tf * (v) : v;
     }
-    for (int l = 0; l < 10; l++) F[(4 + l) * n + x] = (p[l] * ud) / u[0];
+    if (milne) {
+      for (int l = 0; l < 10; l++) F[(4 + l) * n + x] = (p[l] * ud) / u[0];
+    } else {  /* TXYZ5/hydroKT.h:31-35 */
+      for (int l = 0; l < 5; l++) F[(4 + l) * n + x] = (p[CART5[l]] * ud) / u[0];
+      for (int l = 5; l < 10; l++) F[(4 + l) * n + x] = 0.0;
+    }
     F[14 * n + x] = (PI * ud) / u[0];
   }
 }
+
+void or_centre_flux(or_ctx* c, int dir, double tau, double* F) { centre_flux(c, dir, tau, F); }
 
 /* WENO of one quantity along a direction over interior rows; L stored at the
  * cell, R stored at the cell before it (NEWEST10/hydroKT.h:103-171). */
@@ -676,6 +755,7 @@ void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* Result)
     memset(c->AL, 0, n * sizeof(double));
     memset(c->AR, 0, n * sizeof(double));
     for (int l = 0; l < SVAR; l++) {
+      if (!row_used(c, l)) continue;
       weno_rows(c, dir, c->F + l * n, c->FL + l * n, c->FR + l * n);
       weno_rows(c, dir, c->var + l * n, c->VL + l * n, c->VR + l * n);
     }
@@ -709,6 +789,7 @@ void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* Result)
       for (int i = c->il; i < c->ir; i++)
         for (int j = c->jl; j < c->jr; j++)
           for (int k = c->kl; k < c->kr; k++) {
+            if (!row_used(c, l)) continue;
             size_t x = IX(c, i, j, k);
             double pr = -(1.0 / hx[dir]) * (c->flux[l * n + x] - c->flux[l * n + x - st]);
             Result[l * n + x] += pr;
@@ -781,12 +862,215 @@ double or_newton_cell(const viral_desc* d, double tau, double T00, double T10, d
 void or_set_newton_shortcut(int on) { newton_shortcut = on; }
 long or_newton_cycle_count(int period) { return (period >= 0 && period < 5) ? newton_cycles[period] : -1; }
 
+/* ---- Cartesian root (TXYZ5/multiroot.h) */
+
+/* conservation_f of TXYZ5/multiroot.h:203-240: residuals of T^{t mu} =
+ * (e+P-PI) u^t u^mu - (P-PI) g^{t mu} + pi^{t mu} in the unknowns
+ * (e, u^x, u^y, u^z), u^t = sqrt(1 + u.u), multiplied out as the reference
+ * writes them.  prm = {W, X, Y, Z, p1..p5, PI}, W..Z = T^{tt}..T^{tz}. */
+static void cart_f(const viral_desc* d, const double* prm, const double* x, double* y)
+{
+  const double W = prm[0], X = prm[1], Y = prm[2], Z = prm[3];
+  const double p1 = prm[4], p2 = prm[5], p3 = prm[6], p4 = prm[7], p5 = prm[8], PI = prm[9];
+  const double e = x[0], u1 = x[1], u2 = x[2], u3 = x[3];
+  const double P = or_eos_p(d, e);
+  const double q = 1 + u1 * u1 + u2 * u2 + u3 * u3;
+  const double sq = sqrt(q);
+  y[0] = -(u1 * X) - u2 * Y - u3 * Z + (-e + W) * sq;
+  y[1] = -(p3 * u2) - p4 * u3 - P * u1 * q - u1 * (p1 + e * q - PI * q) + X * sq;
+  y[2] = -(p3 * u1) - (e + p2 - PI) * u2 - p5 * u3 - (e - PI) * u2 * (u1 * u1 + u2 * u2 + u3 * u3) - P * u2 * q + Y * sq;
+  y[3] = -(p4 * u1) - p5 * u2 - P * u3 + u3 * (p1 + p2 + PI - W) + Z * sq;
+}
+
+/* conservation_df of TXYZ5/multiroot.h:243-319: J[r][c] = d y_r / d x_c */
+static void cart_df(const viral_desc* d, const double* prm, const double* x, double J[4][4])
+{
+  const double W = prm[0], X = prm[1], Y = prm[2], Z = prm[3];
+  const double p1 = prm[4], p2 = prm[5], p3 = prm[6], p4 = prm[7], p5 = prm[8], PI = prm[9];
+  const double e = x[0], u1 = x[1], u2 = x[2], u3 = x[3];
+  const double P = or_eos_p(d, e);
+  const double a = or_eos_dpde(d, e);
+  const double q = 1 + u1 * u1 + u2 * u2 + u3 * u3;
+  const double sq = sqrt(q), isq = 1.0 / sqrt(q);
+  const double w = e + P - PI;
+  J[0][0] = -sq;
+  J[0][1] = -X + u1 * (-e + W) * isq;
+  J[0][2] = -Y + u2 * (-e + W) * isq;
+  J[0][3] = -Z + u3 * (-e + W) * isq;
+  J[1][0] = -((1 + a) * u1 * q);
+  J[1][1] = -p1 - w * (1 + 3 * u1 * u1 + u2 * u2 + u3 * u3) + u1 * X * isq;
+  J[1][2] = -p3 - 2 * w * u1 * u2 + u2 * X * isq;
+  J[1][3] = -p4 - 2 * w * u1 * u3 + u3 * X * isq;
+  J[2][0] = -((1 + a) * u2 * q);
+  J[2][1] = -p3 - 2 * w * u1 * u2 + u1 * Y * isq;
+  J[2][2] = -p2 - w * (1 + u1 * u1 + 3 * u2 * u2 + u3 * u3) + u2 * Y * isq;
+  J[2][3] = -p5 - 2 * w * u2 * u3 + u3 * Y * isq;
+  J[3][0] = (-a * u3);
+  J[3][1] = -p4 + u1 * Z * isq;
+  J[3][2] = -p5 + u2 * Z * isq;
+  J[3][3] = -P + p1 + p2 + PI - W + u3 * Z * isq;
+}
+
+void or_cart_f(const viral_desc* d, const double* prm, const double* x, double* y) { cart_f(d, prm, x, y); }
+void or_cart_df(const viral_desc* d, const double* prm, const double* x, double* J16)
+{
+  double J[4][4];
+  cart_df(d, prm, x, J);
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) J16[4 * r + c] = J[r][c];
+}
+
+static double enorm4(const double* f)
+{
+  double e2 = 0;
+  for (int i = 0; i < 4; i++) e2 += f[i] * f[i];
+  return sqrt(e2);
+}
+
+#define GN_MAX_ITER 100 /* TXYZ5/multiroot.h:388 */
+
+/* gsl_multiroot_fdfsolver_gnewton driven as TXYZ5/multiroot.h:385-418:
+ * per iteration an LU solve J d = f with partial pivoting (GSL
+ * gsl_linalg_LU_decomp / LU_solve), trial x - t d from t = 1, and while the
+ * residual norm rises and t > DBL_EPSILON the step shrinks by
+ * t *= (sqrt(1 + 6 theta) - 1)/(3 theta), theta = |f_trial|/|f|; stop when
+ * gsl_multiroot_test_delta(dx, x, 0, 1e-30) holds for every component or after
+ * 100 iterations.  A zero pivot (GSL's "matrix is singular" error, which
+ * aborts the reference) or a non-finite result sets VIRAL_FLAG_SOLVER. */
+int or_cart_gnewton(const viral_desc* d, const double* prm, double* x, unsigned* flags)
+{
+  double f[4], J[4][4];
+  unsigned fl = 0;
+  cart_f(d, prm, x, f);
+  cart_df(d, prm, x, J);
+  double phi = enorm4(f);
+  int iter = 0;
+  for (;;) {
+    iter++;
+    double LU[4][4];
+    int perm[4] = {0, 1, 2, 3};
+    memcpy(LU, J, sizeof(LU));
+    for (int j = 0; j < 3; j++) {
+      double mx = fabs(LU[j][j]);
+      int ip = j;
+      for (int i = j + 1; i < 4; i++) {
+        double aij = fabs(LU[i][j]);
+        if (aij > mx) { mx = aij; ip = i; }
+      }
+      if (ip != j) {
+        for (int k = 0; k < 4; k++) { double t = LU[j][k]; LU[j][k] = LU[ip][k]; LU[ip][k] = t; }
+        int t = perm[j]; perm[j] = perm[ip]; perm[ip] = t;
+      }
+      double ajj = LU[j][j];
+      if (ajj != 0.0)
+        for (int i = j + 1; i < 4; i++) {
+          double aij = LU[i][j] / ajj;
+          LU[i][j] = aij;
+          for (int k = j + 1; k < 4; k++) LU[i][k] = LU[i][k] - aij * LU[j][k];
+        }
+    }
+    if (LU[0][0] == 0.0 || LU[1][1] == 0.0 || LU[2][2] == 0.0 || LU[3][3] == 0.0) { fl |= VIRAL_FLAG_SOLVER; break; }
+    double dv[4];
+    for (int i = 0; i < 4; i++) dv[i] = f[perm[i]];
+    for (int i = 1; i < 4; i++) {
+      double t = dv[i];
+      for (int j = 0; j < i; j++) t -= LU[i][j] * dv[j];
+      dv[i] = t;
+    }
+    dv[3] = dv[3] / LU[3][3];
+    for (int i = 2; i >= 0; i--) {
+      double t = dv[i];
+      for (int j = i + 1; j < 4; j++) t -= LU[i][j] * dv[j];
+      dv[i] = t / LU[i][i];
+    }
+    double t = 1, phi0 = phi, phi1, xt[4];
+    for (;;) {
+      for (int i = 0; i < 4; i++) xt[i] = x[i] - t * dv[i];
+      cart_f(d, prm, xt, f);
+      phi1 = enorm4(f);
+      if (phi1 > phi0 && t > DBL_EPSILON) {
+        double theta = phi1 / phi0;
+        double u = (sqrt(1.0 + 6.0 * theta) - 1.0) / (3.0 * theta);
+        t *= u;
+        continue;
+      }
+      break;
+    }
+    double dx[4];
+    for (int i = 0; i < 4; i++) {
+      x[i] = xt[i];
+      dx[i] = -t * dv[i];
+    }
+    cart_df(d, prm, x, J);
+    phi = phi1;
+    int ok = 1;
+    for (int i = 0; i < 4; i++) {
+      double tol = 1e-30 * fabs(x[i]);
+      if (!(fabs(dx[i]) < tol || dx[i] == 0)) { ok = 0; break; }
+    }
+    if (ok) break;
+    if (iter >= GN_MAX_ITER) { fl |= VIRAL_FLAG_MAXITER; break; }
+  }
+  for (int i = 0; i < 4; i++)
+    if (!isfinite(x[i])) fl |= VIRAL_FLAG_SOLVER;
+  *flags = fl;
+  return iter;
+}
+
+/* TXYZ5/multiroot.h:322-479 on the interior: gnewton from the guess
+ * (e, u^x, u^y, u^z) of the cell, then Temp, u^t = sqrt(1 + u.u), v = u/u^t,
+ * P; a cell with |v| >= 1 or e <= 0 is where the reference exits. */
+static int cart_root(or_ctx* c, int32_t* iters, uint8_t* flags)
+{
+  const viral_desc* d = &c->d;
+  double** f = c->f;
+  int bad = -1;
+  for (int i = c->il; i < c->ir; i++)
+    for (int j = c->jl; j < c->jr; j++)
+      for (int k = c->kl; k < c->kr; k++) {
+        size_t x = IX(c, i, j, k);
+        double prm[10] = {f[VIRAL_F_T00][x], f[VIRAL_F_T10][x], f[VIRAL_F_T20][x], f[VIRAL_F_T30][x],
+                          f[VIRAL_F_PI0][x], f[VIRAL_F_PI0 + 1][x], f[VIRAL_F_PI0 + 2][x], f[VIRAL_F_PI0 + 3][x],
+                          f[VIRAL_F_PI0 + 4][x], f[VIRAL_F_BULK][x]};
+        double r[4] = {f[VIRAL_F_EN][x], f[VIRAL_F_U1][x], f[VIRAL_F_U2][x], f[VIRAL_F_U3][x]};
+        unsigned fl;
+        int it = or_cart_gnewton(d, prm, r, &fl);
+        f[VIRAL_F_EN][x] = r[0];
+        f[VIRAL_F_TEMP][x] = or_eos_T(d, r[0]);
+        f[VIRAL_F_U1][x] = r[1];
+        f[VIRAL_F_U2][x] = r[2];
+        f[VIRAL_F_U3][x] = r[3];
+        f[VIRAL_F_U0][x] = sqrt(1 + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+        if (iters) iters[x] = it;
+        if (flags) flags[x] = (uint8_t)fl;
+      }
+  for (int i = c->il; i < c->ir; i++)
+    for (int j = c->jl; j < c->jr; j++)
+      for (int k = c->kl; k < c->kr; k++) {
+        size_t x = IX(c, i, j, k);
+        double En = f[VIRAL_F_EN][x], u0 = f[VIRAL_F_U0][x];
+        double vx = f[VIRAL_F_U1][x] / u0, vy = f[VIRAL_F_U2][x] / u0, ve = f[VIRAL_F_U3][x] / u0;
+        unsigned fl = flags ? flags[x] : 0;
+        if (sqrt(vx * vx + vy * vy + ve * ve) >= 1) fl |= VIRAL_FLAG_ACAUSAL;
+        if (En <= 0) fl |= VIRAL_FLAG_ENEG;
+        f[VIRAL_F_VX][x] = vx;
+        f[VIRAL_F_VY][x] = vy;
+        f[VIRAL_F_VE][x] = ve;
+        f[VIRAL_F_P][x] = or_eos_p(d, En);
+        if (d->t_freeze > 0 && f[VIRAL_F_TEMP][x] < d->t_freeze) fl |= VIRAL_FLAG_FREEZEOUT;
+        if (flags) flags[x] = (uint8_t)fl;
+        if (bad < 0 && (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER))) bad = (int)x;
+      }
+  return bad;
+}
+
 /* NEWEST10/root.h:301-450 on the interior */
 int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)
 {
   const viral_desc* d = &c->d;
   const int milne = d->coords == VIRAL_MILNE;
   double** f = c->f;
+  if (!milne) return cart_root(c, iters, flags);
   double t2 = milne ? tau * tau : 1.0;
   int bad = -1;
   for (int i = c->il; i < c->ir; i++)
@@ -830,7 +1114,7 @@ int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)
         if (flags) flags[x] = (uint8_t)fl;
         if (bad < 0 && (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER))) bad = (int)x;
       }
-  if ((d->quirks & VIRAL_Q_K0_BROADCAST) && !d->lbi) {
+  if ((d->quirks & VIRAL_Q_K0_BROADCAST) && !d->lbi && milne) {
     for (int i = c->il; i < c->ir; i++)
       for (int j = c->jl; j < c->jr; j++)
         for (int k = c->kl; k < c->kr; k++) {
@@ -865,7 +1149,7 @@ static const int BFIELDS[] = {VIRAL_F_T00, VIRAL_F_T10, VIRAL_F_T20, VIRAL_F_T30
 void or_boundary(or_ctx* c)
 {
   double** f = c->f;
-  const int mirror = (c->d.quirks & VIRAL_Q_GUBSER_MIRROR) != 0;
+  const int mirror = (c->d.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && c->d.coords == VIRAL_MILNE;
   for (int j = c->jl; j < c->jr; j++)
     for (int k = c->kl; k < c->kr; k++)
       for (int g = 0; g < B; g++) {
@@ -927,7 +1211,8 @@ double or_max_temp_gev(or_ctx* c)
 
 /* ------------------------------------------------------------------ RK */
 
-/* T = Var/tau, pi = Var, PI = Var on the interior (NEWEST10/time.h:100-162) */
+/* T = Var/tau, pi = Var, PI = Var on the interior (NEWEST10/time.h:100-162;
+ * Cartesian T = Var and five pi, TXYZ5/time.h:100-165) */
 static void update_primary(or_ctx* c, const double* V, double tau)
 {
   const int milne = c->d.coords == VIRAL_MILNE;
@@ -937,7 +1222,7 @@ static void update_primary(or_ctx* c, const double* V, double tau)
       for (int k = c->kl; k < c->kr; k++) {
         size_t x = IX(c, i, j, k);
         for (int mu = 0; mu < 4; mu++) c->f[VIRAL_F_T00 + mu][x] = milne ? (V[mu * n + x]) / (tau) : V[mu * n + x];
-        for (int l = 0; l < 10; l++) c->f[VIRAL_F_PI0 + l][x] = (V[(4 + l) * n + x]);
+        for (int l = 0; l < (milne ? 10 : 5); l++) c->f[VIRAL_F_PI0 + l][x] = (V[(4 + l) * n + x]);
         c->f[VIRAL_F_BULK][x] = (V[14 * n + x]);
       }
 }

@@ -59,6 +59,22 @@ double or_newton_cell(const viral_desc* d, double tau, double T00, double T10, d
 void or_set_newton_shortcut(int on);
 long or_newton_cycle_count(int period);
 
+/* Cartesian pieces (TXYZ5): the full pi^{mu nu} (slots tt,tx,ty,tz,xx,xy,xz,
+ * yy,yz,zz) from the five evolved components {xx,yy,xy,xz,yz}; the 4-D root
+ * solve of one cell, prm = {T^tt, T^tx, T^ty, T^tz, p1..p5, PI}, x = guess
+ * (e, u^x, u^y, u^z) in / root out; returns the iteration count. */
+void or_cart_pi10(const double u[4], const double* p5, double pi10[10]);
+int or_cart_gnewton(const viral_desc* d, const double* prm, double* x, unsigned* flags);
+/* its residuals and Jacobian (J16[4r+c] = d y_r / d x_c) */
+void or_cart_f(const viral_desc* d, const double* prm, const double* x, double* y);
+void or_cart_df(const viral_desc* d, const double* prm, const double* x, double* J16);
+/* centred fluxes of direction dir on every cell, [15][cells] */
+void or_centre_flux(or_ctx* c, int dir, double tau, double* F);
+
+/* The Israel-Stewart shear source of one cell (du16[4a+b] = d_b u^a). */
+void or_shear_source(int milne, double tau, const double u[4], const double pi10[10], const double du16[16],
+                     double SH, double tpi, double out[10]);
+
 /* WENO kernels of NEWEST10/utils.h:57-160 */
 double or_weno_L(double qm2, double qm1, double qc, double qp1, double qp2);
 double or_weno_R(double qm2, double qm1, double qc, double qp1, double qp2);

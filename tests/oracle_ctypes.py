@@ -62,8 +62,62 @@ def lib():
             getattr(L, name).argtypes = [C.c_double]
         L.or_vm_pow.restype = C.c_double
         L.or_vm_pow.argtypes = [C.c_double, C.c_double]
+        L.or_cart_pi10.argtypes = [dp, dp, dp]
+        L.or_cart_gnewton.restype = C.c_int
+        L.or_cart_gnewton.argtypes = [C.POINTER(ViralDesc), dp, dp, C.POINTER(C.c_uint)]
+        L.or_cart_f.argtypes = [C.POINTER(ViralDesc), dp, dp, dp]
+        L.or_cart_df.argtypes = [C.POINTER(ViralDesc), dp, dp, dp]
+        L.or_centre_flux.argtypes = [vp, C.c_int, C.c_double, dp]
+        L.or_shear_source.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_double, C.c_double, dp]
         _lib = L
     return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def cart_pi10(u, p5):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    p5 = np.ascontiguousarray(p5, dtype=np.float64)
+    out = np.zeros(10)
+    lib().or_cart_pi10(_dp(u), _dp(p5), _dp(out))
+    return out
+
+
+def cart_gnewton(desc, prm, guess):
+    """(root, iterations, flags) of the Cartesian 4-D solve"""
+    prm = np.ascontiguousarray(prm, dtype=np.float64)
+    x = np.array(guess, dtype=np.float64)
+    fl = C.c_uint()
+    it = lib().or_cart_gnewton(C.byref(desc), _dp(prm), _dp(x), C.byref(fl))
+    return x, it, fl.value
+
+
+def cart_f(desc, prm, x):
+    prm = np.ascontiguousarray(prm, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(4)
+    lib().or_cart_f(C.byref(desc), _dp(prm), _dp(x), _dp(y))
+    return y
+
+
+def cart_df(desc, prm, x):
+    prm = np.ascontiguousarray(prm, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    J = np.zeros(16)
+    lib().or_cart_df(C.byref(desc), _dp(prm), _dp(x), _dp(J))
+    return J.reshape(4, 4)
+
+
+def shear_source(milne, tau, u, pi10, du, SH, tpi):
+    """du[a][b] = d_b u^a"""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    pi10 = np.ascontiguousarray(pi10, dtype=np.float64)
+    du = np.ascontiguousarray(du, dtype=np.float64).ravel()
+    out = np.zeros(10)
+    lib().or_shear_source(int(milne), tau, _dp(u), _dp(pi10), _dp(du), SH, tpi, _dp(out))
+    return out
 
 
 class Oracle:
@@ -129,6 +183,11 @@ class Oracle:
 
     def boundary(self):
         self.L.or_boundary(self.h)
+
+    def centre_flux(self, d, tau):
+        out = np.zeros(15 * self.n)
+        self.L.or_centre_flux(self.h, d, tau, _dp(out))
+        return out.reshape(15, self.n)
 
     def max_temp_gev(self):
         return self.L.or_max_temp_gev(self.h)

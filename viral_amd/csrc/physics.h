@@ -686,4 +686,201 @@ __device__ __forceinline__ void shear_source(int milne, double tau, const double
     }
 }
 
+// ---------------------------------------------------------------- Cartesian (TXYZ5)
+// Full pi^{mu nu} (slots tt,tx,ty,tz,xx,xy,xz,yy,yz,zz) from the five evolved
+// components {xx, yy, xy, xz, yz} (TXYZ5/maindef.h:240) by u_mu pi^{mu nu} = 0
+// and pi^mu_mu = 0 -- the A1..A5 of NEWEST5/maindef.h:497-501 at tau = 1.
+// Same operation order as the CPU restatement.
+__device__ __forceinline__ void cart_pi10(const double u[4], double p1, double p2, double p3, double p4, double p5,
+                                          double pi10[10])
+{
+  const double u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
+  const double zz = (p1 * u1 * u1 + 2.0 * p3 * u1 * u2 + p2 * u2 * u2 + 2.0 * u3 * (p4 * u1 + p5 * u2) -
+                     u0 * u0 * (p1 + p2)) /
+                    (u0 * u0 - u3 * u3);
+  pi10[0] = p1 + p2 + zz;
+  pi10[1] = (p1 * u1 + p3 * u2 + p4 * u3) / u0;
+  pi10[2] = (p3 * u1 + p2 * u2 + p5 * u3) / u0;
+  pi10[3] = (p4 * u1 + p5 * u2 + zz * u3) / u0;
+  pi10[4] = p1;
+  pi10[5] = p3;
+  pi10[6] = p4;
+  pi10[7] = p2;
+  pi10[8] = p5;
+  pi10[9] = zz;
+}
+
+// slot of the full tensor holding evolved Cartesian component l
+__device__ __forceinline__ int cart5(int l) { return l == 0 ? 4 : l == 1 ? 7 : l == 2 ? 5 : l == 3 ? 6 : 8; }
+
+// conservation_f / conservation_df of TXYZ5/multiroot.h:203-319 (residuals of
+// T^{t mu} in the unknowns e, u^x, u^y, u^z), prm = {T^tt, T^tx, T^ty, T^tz,
+// p1..p5, PI}.
+__device__ __forceinline__ void cart_f(const Params& p, const double prm[10], const double x[4], double y[4])
+{
+  const double W = prm[0], X = prm[1], Y = prm[2], Z = prm[3];
+  const double p1 = prm[4], p2 = prm[5], p3 = prm[6], p4 = prm[7], p5 = prm[8], PI = prm[9];
+  const double e = x[0], u1 = x[1], u2 = x[2], u3 = x[3];
+  const double P = eos_p(p, e);
+  const double q = 1 + u1 * u1 + u2 * u2 + u3 * u3;
+  const double sq = sqrt(q);
+  y[0] = -(u1 * X) - u2 * Y - u3 * Z + (-e + W) * sq;
+  y[1] = -(p3 * u2) - p4 * u3 - P * u1 * q - u1 * (p1 + e * q - PI * q) + X * sq;
+  y[2] = -(p3 * u1) - (e + p2 - PI) * u2 - p5 * u3 - (e - PI) * u2 * (u1 * u1 + u2 * u2 + u3 * u3) - P * u2 * q + Y * sq;
+  y[3] = -(p4 * u1) - p5 * u2 - P * u3 + u3 * (p1 + p2 + PI - W) + Z * sq;
+}
+
+__device__ __forceinline__ void cart_df(const Params& p, const double prm[10], const double x[4], double J[4][4])
+{
+  const double W = prm[0], X = prm[1], Y = prm[2], Z = prm[3];
+  const double p1 = prm[4], p2 = prm[5], p3 = prm[6], p4 = prm[7], p5 = prm[8], PI = prm[9];
+  const double e = x[0], u1 = x[1], u2 = x[2], u3 = x[3];
+  const double P = eos_p(p, e);
+  const double a = eos_dpde(p, e);
+  const double q = 1 + u1 * u1 + u2 * u2 + u3 * u3;
+  const double sq = sqrt(q), isq = 1.0 / sqrt(q);
+  const double w = e + P - PI;
+  J[0][0] = -sq;
+  J[0][1] = -X + u1 * (-e + W) * isq;
+  J[0][2] = -Y + u2 * (-e + W) * isq;
+  J[0][3] = -Z + u3 * (-e + W) * isq;
+  J[1][0] = -((1 + a) * u1 * q);
+  J[1][1] = -p1 - w * (1 + 3 * u1 * u1 + u2 * u2 + u3 * u3) + u1 * X * isq;
+  J[1][2] = -p3 - 2 * w * u1 * u2 + u2 * X * isq;
+  J[1][3] = -p4 - 2 * w * u1 * u3 + u3 * X * isq;
+  J[2][0] = -((1 + a) * u2 * q);
+  J[2][1] = -p3 - 2 * w * u1 * u2 + u1 * Y * isq;
+  J[2][2] = -p2 - w * (1 + u1 * u1 + 3 * u2 * u2 + u3 * u3) + u2 * Y * isq;
+  J[2][3] = -p5 - 2 * w * u2 * u3 + u3 * Y * isq;
+  J[3][0] = (-a * u3);
+  J[3][1] = -p4 + u1 * Z * isq;
+  J[3][2] = -p5 + u2 * Z * isq;
+  J[3][3] = -P + p1 + p2 + PI - W + u3 * Z * isq;
+}
+
+__device__ __forceinline__ double enorm4(const double f[4])
+{
+  double e2 = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) e2 += f[i] * f[i];
+  return sqrt(e2);
+}
+
+// GSL gnewton as driven by TXYZ5/multiroot.h:385-418 (max 100 iterations,
+// test_delta(dx, x, 0, 1e-30)): LU with partial pivoting (first largest
+// pivot; the row swaps are applied to the right-hand side as they happen,
+// which equals GSL's permutation of it), forward/back substitution, and the
+// step halving rule t *= (sqrt(1+6 theta)-1)/(3 theta) while the residual
+// norm rises and t > DBL_EPSILON.  All indices are compile-time (no scratch).
+// A zero pivot or a non-finite root sets the solver flag (4).
+__device__ __forceinline__ int cart_gnewton(const Params& p, const double prm[10], double x[4], unsigned& flags)
+{
+  double f[4], J[4][4];
+  unsigned fl = 0;
+  cart_f(p, prm, x, f);
+  cart_df(p, prm, x, J);
+  double phi = enorm4(f);
+  int iter = 0;
+  for (;;) {
+    iter++;
+    double LU[4][4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      b[i] = f[i];
+#pragma unroll
+      for (int k = 0; k < 4; k++) LU[i][k] = J[i][k];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double mx = fabs(LU[j][j]);
+      int ip = j;
+#pragma unroll
+      for (int i = j + 1; i < 4; i++) {
+        const double aij = fabs(LU[i][j]);
+        if (aij > mx) {
+          mx = aij;
+          ip = i;
+        }
+      }
+#pragma unroll
+      for (int i = j + 1; i < 4; i++) {
+        if (ip == i) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const double t = LU[j][k];
+            LU[j][k] = LU[i][k];
+            LU[i][k] = t;
+          }
+          const double t = b[j];
+          b[j] = b[i];
+          b[i] = t;
+        }
+      }
+      const double ajj = LU[j][j];
+      if (ajj != 0.0) {
+#pragma unroll
+        for (int i = j + 1; i < 4; i++) {
+          const double aij = LU[i][j] / ajj;
+          LU[i][j] = aij;
+#pragma unroll
+          for (int k = j + 1; k < 4; k++) LU[i][k] = LU[i][k] - aij * LU[j][k];
+        }
+      }
+    }
+    if (LU[0][0] == 0.0 || LU[1][1] == 0.0 || LU[2][2] == 0.0 || LU[3][3] == 0.0) {
+      fl |= 4u;
+      break;
+    }
+#pragma unroll
+    for (int i = 1; i < 4; i++) {
+      double t = b[i];
+#pragma unroll
+      for (int j = 0; j < i; j++) t -= LU[i][j] * b[j];
+      b[i] = t;
+    }
+    b[3] = b[3] / LU[3][3];
+#pragma unroll
+    for (int i = 2; i >= 0; i--) {
+      double t = b[i];
+#pragma unroll
+      for (int j = i + 1; j < 4; j++) t -= LU[i][j] * b[j];
+      b[i] = t / LU[i][i];
+    }
+    double t = 1, phi0 = phi, phi1, xt[4];
+    for (;;) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) xt[i] = x[i] - t * b[i];
+      cart_f(p, prm, xt, f);
+      phi1 = enorm4(f);
+      if (phi1 > phi0 && t > 2.220446049250313080847e-16) {
+        const double theta = phi1 / phi0;
+        const double u = (sqrt(1.0 + 6.0 * theta) - 1.0) / (3.0 * theta);
+        t *= u;
+        continue;
+      }
+      break;
+    }
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      x[i] = xt[i];
+      const double dxi = -t * b[i];
+      const double tol = 1e-30 * fabs(x[i]);
+      ok = ok && (fabs(dxi) < tol || dxi == 0);
+    }
+    cart_df(p, prm, x, J);
+    phi = phi1;
+    if (ok) break;
+    if (iter >= 100) {
+      fl |= 8u;
+      break;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (!isfinite(x[i])) fl |= 4u;
+  flags = fl;
+  return iter;
+}
+
 }  // namespace viral

@@ -177,7 +177,10 @@ __device__ __forceinline__ int sched_find(const Sched& s, long w, long& local)
 // (hydroKT.h:364-381) give the Rusanov flux
 //   f = (F_L + F_R)/2 - a (V_R - V_L)/2      (hydroKT.h:444-506).
 // The stencil cells' common terms are loaded once and kept in registers.
-template <bool FAST>
+// CART: the Cartesian (TXYZ5/hydroKT.h:16-62) form -- no tau factors, the
+// five evolved shear rows 4..8 and PI in row 14, and the T rows built from
+// the full pi^{mu nu} that cart_pi10 derives per stencil cell.
+template <bool FAST, bool CART>
 __device__ __forceinline__ void flux_face(const Geom& g, const Params& p, int d, double tau,
                                           const double* __restrict__ st, double* __restrict__ flux, long t)
 {
@@ -185,7 +188,7 @@ __device__ __forceinline__ void flux_face(const Geom& g, const Params& p, int d,
   Box b = face_box(g, d);
   if (!box_cell(b, t, i, j, k)) return;
   const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
-  const int milne = p.milne;
+  const int milne = CART ? 0 : p.milne;
   const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
   const int row = d + 1;
   double u0[6], ud[6], w[6], pp[6];
@@ -229,14 +232,29 @@ __device__ __forceinline__ void flux_face(const Geom& g, const Params& p, int d,
   const double* __restrict__ base[6];
 #pragma unroll
   for (int o = 0; o < 6; o++) base[o] = st + x + (o - 2) * s;
+  double piT[6][4];  // Cartesian pi^{row,m} of each stencil cell
+  if (CART) {
+#pragma unroll
+    for (int o = 0; o < 6; o++) {
+      const double uu[4] = {u0[o], base[o][(long)VIRAL_F_U1 * n], base[o][(long)VIRAL_F_U2 * n],
+                            base[o][(long)VIRAL_F_U3 * n]};
+      double pt[10];
+      cart_pi10(uu, base[o][(long)VIRAL_F_PI0 * n], base[o][(long)(VIRAL_F_PI0 + 1) * n],
+                base[o][(long)(VIRAL_F_PI0 + 2) * n], base[o][(long)(VIRAL_F_PI0 + 3) * n],
+                base[o][(long)(VIRAL_F_PI0 + 4) * n], pt);
+#pragma unroll
+      for (int m = 0; m < 4; m++) piT[o][m] = pt[pidx(row, m)];
+    }
+  }
 #pragma unroll
   for (int l = 0; l < SV; l++) {
+    if (CART && l >= 9 && l < 14) continue;
     double F[6], V[6];
     // field feeding the flux row: pi^{row,l} for T rows, pi[l-4] / PI otherwise
     const int ff = l < 4 ? VIRAL_F_PI0 + pidx(row, l) : (l < 14 ? VIRAL_F_PI0 + l - 4 : VIRAL_F_BULK);
 #pragma unroll
     for (int o = 0; o < 6; o++) {
-      const double fv = base[o][(long)ff * n];
+      const double fv = (CART && l < 4) ? piT[o][l < 4 ? l : 0] : base[o][(long)ff * n];
       const double vv = base[o][(long)l * n];  // Var field index == state field index (T00..T30, pi0..pi9, PI)
       if (l < 4) {
         const double um = base[o][(long)(VIRAL_F_U0 + l) * n];
@@ -260,7 +278,7 @@ __device__ __forceinline__ void flux_face(const Geom& g, const Params& p, int d,
   }
 }
 
-template <bool FAST, bool PERSIST>
+template <bool FAST, bool PERSIST, bool CART>
 __global__ void __launch_bounds__(BLK) k_flux_dir(const EvDev* __restrict__ tab, Sched s, const Clock* __restrict__ clk,
                                                   int stage, int cur, int d)
 {
@@ -269,7 +287,7 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(const EvDev* __restrict__ tab,
     long lb;
     const EvDev& E = tab[sched_find(s, w_, lb)];
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
-    flux_face<FAST>(E.g, E.p, d, a.tau_src, E.st[cur], E.flux[d], lb * BLK + threadIdx.x);
+    flux_face<FAST, CART>(E.g, E.p, d, a.tau_src, E.st[cur], E.flux[d], lb * BLK + threadIdx.x);
   }
 }
 
@@ -291,7 +309,7 @@ struct TileSlots {
   static constexpr int MAXS = MAXR * NT < 1024 ? MAXR * NT : 1024;
 };
 
-template <int D, int NT>
+template <int D, int NT, bool CART>
 __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, double tau,
                                                const double* __restrict__ st, double* __restrict__ flux, int TP,
                                                int TK, long tile, double* lds)
@@ -319,7 +337,7 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
   const int c0 = lo - 1 + sg * fpt;  // first face cell of this tile
   const int p0 = c0 - 2;             // cell at tile position 0
   const long n = g.n;
-  const int milne = p.milne;
+  const int milne = CART ? 0 : p.milne;
   const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
   const long sd = D == 0 ? g.sx : D == 1 ? g.sy : 1;
   const int row = D + 1;
@@ -329,6 +347,7 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
   int pos[MAXR];
   bool cell_ok[MAXR], face_ok[MAXR], in_[MAXR];
   double u0v[MAXR], udv[MAXR], vdv[MAXR], wv[MAXR], ppv[MAXR], amax[MAXR], LF[MAXR], LV[MAXR];
+  double pT[MAXR][4];  // Cartesian pi^{row,m} of each owned cell
 #pragma unroll
   for (int r = 0; r < MAXR; r++) {
     const int sl = threadIdx.x + r * BLK;
@@ -368,9 +387,18 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     vdv[r] = udv[r] * frcp(u0v[r]);
     wv[r] = e + P - PI;
     ppv[r] = (D == 2 && milne) ? (P - PI) * (1.0 / (tau * tau)) : (P - PI);
+    if (CART) {
+      const double uu[4] = {u0v[r], st[VIRAL_F_U1 * n + y], st[VIRAL_F_U2 * n + y], st[VIRAL_F_U3 * n + y]};
+      double pt[10];
+      cart_pi10(uu, st[VIRAL_F_PI0 * n + y], st[(VIRAL_F_PI0 + 1) * n + y], st[(VIRAL_F_PI0 + 2) * n + y],
+                st[(VIRAL_F_PI0 + 3) * n + y], st[(VIRAL_F_PI0 + 4) * n + y], pt);
+#pragma unroll
+      for (int m = 0; m < 4; m++) pT[r][m] = pt[pidx(row, m)];
+    }
   }
   // quantity -1: eigenvalue -> face speeds; 0..14: flux rows
   for (int q = -1; q < SV; q++) {
+    if (CART && q >= 9 && q < 14) continue;  // block-uniform: no barrier is skipped by part of the block
     double* rF = (q & 1) ? rF1 : rF0;
     double* rV = (q & 1) ? rV1 : rV0;
 #pragma unroll
@@ -383,7 +411,8 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
         if (q < 0) {
           V = (in_[r] || !q1) ? vdv[r] : 0.0;
         } else if (q < 4) {
-          const double piv = st[(long)(VIRAL_F_PI0 + pidx(row, q)) * n + y];
+          const double piv = CART ? (q == 0 ? pT[r][0] : q == 1 ? pT[r][1] : q == 2 ? pT[r][2] : pT[r][3])
+                                  : st[(long)(VIRAL_F_PI0 + pidx(row, q)) * n + y];
           const double um = st[(long)(VIRAL_F_U0 + q) * n + y];
           const double v = (q == row) ? piv + ppv[r] + udv[r] * um * wv[r] : piv + udv[r] * um * wv[r];
           F = milne ? tau * v : v;
@@ -436,7 +465,7 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
   }
 }
 
-template <int D, int NT, bool PERSIST>
+template <int D, int NT, bool PERSIST, bool CART>
 __global__ void __launch_bounds__(NT) k_flux_tile(const EvDev* __restrict__ tab, Sched s,
                                                   const Clock* __restrict__ clk, int stage, int cur)
 {
@@ -446,7 +475,7 @@ __global__ void __launch_bounds__(NT) k_flux_tile(const EvDev* __restrict__ tab,
     long lb;
     const EvDev& E = tab[sched_find(s, w_, lb)];
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
-    flux_tile_body<D, NT>(E.g, E.p, a.tau_src, E.st[cur], E.flux[D], E.tp[D], E.tk[D], lb, lds);
+    flux_tile_body<D, NT, CART>(E.g, E.p, a.tau_src, E.st[cur], E.flux[D], E.tp[D], E.tk[D], lb, lds);
     __syncthreads();  // the next tile may lay its LDS arrays out differently
   }
 }
@@ -537,12 +566,16 @@ __global__ void __launch_bounds__(BLK) k_shasta_dir(const EvDev* __restrict__ ta
 
 // Geometric, CON-gradient and Israel-Stewart sources of one interior cell
 // (NEWEST10/Source.h:3-101,195-536) from the stage-start state, into src.
-template <bool FAST>
+// CART: TXYZ5/Source.h -- no geometric terms, the shear source of the five
+// evolved components (rows 4..8) from the full tensor, and under q9 only in
+// ghosted eta slices k < nz (the reference's loop k = 0..ZCMA-1,
+// TXYZ5/Source.h:272; other slices keep zero sources).
+template <bool FAST, bool CART>
 __device__ __forceinline__ void source_terms(const Geom& g, const Params& p, const StageArgs& a,
                                              const double* __restrict__ si, long x, double* S)
 {
   const long n = g.n;
-  const int milne = p.milne;
+  const int milne = CART ? 0 : p.milne;
   const double tau = a.tau_src;
   const double e = si[VIRAL_F_EN * n + x], P = si[VIRAL_F_P * n + x], PI = si[VIRAL_F_BULK * n + x];
   double u[4], pu[4], pi[10];
@@ -551,8 +584,13 @@ __device__ __forceinline__ void source_terms(const Geom& g, const Params& p, con
     u[l] = si[(VIRAL_F_U0 + l) * n + x];
     pu[l] = si[(VIRAL_F_PU0 + l) * n + x];
   }
+  if (CART) {
+    cart_pi10(u, si[VIRAL_F_PI0 * n + x], si[(VIRAL_F_PI0 + 1) * n + x], si[(VIRAL_F_PI0 + 2) * n + x],
+              si[(VIRAL_F_PI0 + 3) * n + x], si[(VIRAL_F_PI0 + 4) * n + x], pi);
+  } else {
 #pragma unroll
-  for (int l = 0; l < 10; l++) pi[l] = si[(VIRAL_F_PI0 + l) * n + x];
+    for (int l = 0; l < 10; l++) pi[l] = si[(VIRAL_F_PI0 + l) * n + x];
+  }
   if (milne) {
     double tau2 = tau * tau;
     S[0] = -(P - PI) - tau2 * (pi[9] + (e + P - PI) * u[3] * u[3]);
@@ -633,12 +671,20 @@ __device__ __forceinline__ void source_terms(const Geom& g, const Params& p, con
     double tpi = tpi_of(p, SH, P, e);
     double out[10];
     shear_source<FAST>(milne, tau, u, pi, du, SH, tpi, out);
+    if (CART) {
+      const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)(x % g.gz) < g.kr - g.kl;
 #pragma unroll
-    for (int l = 0; l < 10; l++) S[4 + l] = out[l];
+      for (int l = 0; l < 5; l++) S[4 + l] = live ? out[cart5(l)] : 0.0;
+#pragma unroll
+      for (int l = 5; l < 10; l++) S[4 + l] = 0.0;
+    } else {
+#pragma unroll
+      for (int l = 0; l < 10; l++) S[4 + l] = out[l];
+    }
   }
 }
 
-template <bool FAST, bool PERSIST>
+template <bool FAST, bool PERSIST, bool CART>
 __global__ void __launch_bounds__(BLK) k_source(const EvDev* __restrict__ tab, Sched s, const Clock* __restrict__ clk,
                                                 int stage, int cur)
 {
@@ -651,9 +697,10 @@ __global__ void __launch_bounds__(BLK) k_source(const EvDev* __restrict__ tab, S
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
     const long x = cidx(E.g, i, j, k), n = E.g.n;
     double S[SV];
-    source_terms<FAST>(E.g, E.p, a, E.st[cur], x, S);
+    source_terms<FAST, CART>(E.g, E.p, a, E.st[cur], x, S);
 #pragma unroll
-    for (int l = 0; l < SV; l++) E.src[l * n + x] = S[l];
+    for (int l = 0; l < SV; l++)
+      if (!CART || l < 9 || l == 14) E.src[l * n + x] = S[l];
   }
 }
 
@@ -666,7 +713,11 @@ __global__ void __launch_bounds__(BLK) k_source(const EvDev* __restrict__ tab, S
 // FUSED evaluates the sources in place (source_terms) instead of reading
 // them from the source kernel's output: one launch and 15 doubles per cell
 // less traffic, at the price of the union of both kernels' registers.
-template <bool FAST, bool FUSED>
+//
+// CART: the Cartesian update (TXYZ5/time.h:100-165, TXYZ5/multiroot.h:322-479):
+// T^{t mu} = Var, five pi rows, the 4-D gnewton root from (e, u^x, u^y, u^z),
+// u^t = sqrt(1 + u.u), no k0 broadcast.
+template <bool FAST, bool FUSED, bool CART>
 __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, const StageArgs& a, long long step,
                                             const double* __restrict__ si, double* __restrict__ so,
                                             const double* __restrict__ fx, const double* __restrict__ fy,
@@ -678,7 +729,7 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
   int i, j, k;
   if (!box_cell(interior_box(g), t, i, j, k)) return;
   const long n = g.n, x = cidx(g, i, j, k);
-  const int milne = p.milne;
+  const int milne = CART ? 0 : p.milne;
   const double tau = a.tau_src;
   // ---- Result = sum over directions of -(1/h)(f(c+1/2) - f(c-1/2)), x then y then eta
   double Res[SV];
@@ -695,6 +746,7 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
       const double* f = fl[dd];
 #pragma unroll
       for (int l = 0; l < SV; l++) {
+        if (CART && l >= 9 && l < 14) continue;
         double pr;
         if (p.scheme == VIRAL_SHASTA) {
           pr = f[l * n + x];  // SHASTA kernels store the partial result itself
@@ -708,10 +760,10 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
   }
   double S[SV];
   if (FUSED) {
-    source_terms<FAST>(g, p, a, si, x, S);
+    source_terms<FAST, CART>(g, p, a, si, x, S);
   } else {
 #pragma unroll
-    for (int l = 0; l < SV; l++) S[l] = src[l * n + x];
+    for (int l = 0; l < SV; l++) S[l] = (CART && l >= 9 && l < 14) ? 0.0 : src[l * n + x];
   }
   const double e = si[VIRAL_F_EN * n + x];
   double u[4], pu[4], pi[10];
@@ -724,6 +776,10 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
   double V[SV];
 #pragma unroll
   for (int l = 0; l < SV; l++) {
+    if (CART && l >= 9 && l < 14) {
+      V[l] = 0.0;
+      continue;
+    }
     double L = (Res[l] + S[l]);
     if (a.kind == 0) {
       double v0 = l < 4 ? (milne ? tau * si[(VIRAL_F_T00 + l) * n + x] : si[(VIRAL_F_T00 + l) * n + x])
@@ -741,6 +797,44 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
     } else {
       V[l] = (1.0 / 3.0) * var0[l * n + x] + (2.0 / 3.0) * var2[l * n + x] + (2.0 / 3.0) * a.ts * L;
     }
+  }
+  if (CART) {
+    // ---- TXYZ5/time.h:100-165 and TXYZ5/multiroot.h:322-479
+    const double prm[10] = {V[0], V[1], V[2], V[3], V[4], V[5], V[6], V[7], V[8], V[14]};
+    double r[4] = {e, u[1], u[2], u[3]};
+    unsigned fl;
+    const int it = cart_gnewton(p, prm, r, fl);
+    const double En = r[0];
+    const double Temp = eos_T(p, En);
+    const double c0 = sqrt(1 + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+    const double vx = r[1] / c0, vy = r[2] / c0, ve = r[3] / c0;
+    if (sqrt(vx * vx + vy * vy + ve * ve) >= 1) fl |= VIRAL_FLAG_ACAUSAL;
+    if (En <= 0) fl |= VIRAL_FLAG_ENEG;
+    if (p.t_freeze > 0 && Temp < p.t_freeze) fl |= VIRAL_FLAG_FREEZEOUT;
+#pragma unroll
+    for (int l = 0; l < 4; l++) so[(VIRAL_F_T00 + l) * n + x] = V[l];
+#pragma unroll
+    for (int l = 0; l < 5; l++) so[(VIRAL_F_PI0 + l) * n + x] = V[4 + l];
+    so[VIRAL_F_BULK * n + x] = V[14];
+    so[VIRAL_F_EN * n + x] = En;
+    so[VIRAL_F_P * n + x] = eos_p(p, En);
+    so[VIRAL_F_TEMP * n + x] = Temp;
+    so[VIRAL_F_VX * n + x] = vx;
+    so[VIRAL_F_VY * n + x] = vy;
+    so[VIRAL_F_VE * n + x] = ve;
+    so[VIRAL_F_U0 * n + x] = c0;
+    so[VIRAL_F_U1 * n + x] = r[1];
+    so[VIRAL_F_U2 * n + x] = r[2];
+    so[VIRAL_F_U3 * n + x] = r[3];
+#pragma unroll
+    for (int l = 0; l < 4; l++) so[(VIRAL_F_PU0 + l) * n + x] = a.stage == 0 ? u[l] : pu[l];
+    iters[x] = it;
+    const uint8_t prev = a.stage == 0 ? 0 : flags[x];
+    flags[x] = (uint8_t)((prev & ~VIRAL_FLAG_FREEZEOUT) | fl);
+    if (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER))
+      atomicMin(bad, ((unsigned long long)step << 42) | ((unsigned long long)a.stage << 40) |
+                         ((unsigned long long)x << 8) | (unsigned long long)fl);
+    return;
   }
   // ---- primary variables at tau_new (NEWEST10/time.h:100-162)
   const double tn = a.tau_new;
@@ -814,7 +908,7 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
   }
 }
 
-template <bool FAST, bool FUSED, bool PERSIST>
+template <bool FAST, bool FUSED, bool PERSIST, bool CART>
 __global__ void __launch_bounds__(BLK) k_update(const EvDev* __restrict__ tab, Sched s, const Clock* __restrict__ clk,
                                                 int stage, int cur)
 {
@@ -823,7 +917,7 @@ __global__ void __launch_bounds__(BLK) k_update(const EvDev* __restrict__ tab, S
     long lb;
     const EvDev& E = tab[sched_find(s, w_, lb)];
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
-    update_cell<FAST, FUSED>(E.g, E.p, a, clk->step, E.st[cur], E.st[cur ^ 1], E.flux[0], E.flux[1], E.flux[2],
+    update_cell<FAST, FUSED, CART>(E.g, E.p, a, clk->step, E.st[cur], E.st[cur ^ 1], E.flux[0], E.flux[1], E.flux[2],
                              E.src, E.var0, E.var1, E.var2, E.iters + (long)stage * E.g.n, E.flags, E.bad,
                              lb * BLK + threadIdx.x);
   }
@@ -885,7 +979,7 @@ __device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, doubl
   const long cx = 2L * VIRAL_BORDER * ny * nz;  // x slabs
   const long cy = 2L * VIRAL_BORDER * nx * nz;  // y slabs
   const long cz = p.lbi ? 0 : 2L * VIRAL_BORDER * nx * ny;
-  const bool mirror = (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
+  const bool mirror = (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && p.milne && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
   long dst, src;
   if (t < cx) {
     int k = g.kl + (int)(t % nz);
@@ -1013,7 +1107,7 @@ struct viral_ctx {
   unsigned long long* dtmp = nullptr;
   int cur = 0;    // which state copy is current (all events step together)
   int ncu = 256;  // compute units of the device
-  int rk = 2, scheme = VIRAL_KT, fast = 0;
+  int rk = 2, scheme = VIRAL_KT, fast = 0, cart = 0;
   viral_launch cfg{};
   // tile table: device event rows and, per phase, the first tile of every event
   EvDev* d_tab = nullptr;
@@ -1134,7 +1228,8 @@ static long tile_count(const viral_desc* d, int ph, int tp, int tk)
       return ((nlines + tk - 1) / tk) * nseg;
     }
     case PH_CELLS: return nblocks(ni);
-    case PH_K0: return ((d->quirks & VIRAL_Q_K0_BROADCAST) && !d->lbi) ? nblocks(ni) : 0;
+    case PH_K0:
+      return ((d->quirks & VIRAL_Q_K0_BROADCAST) && !d->lbi && d->coords == VIRAL_MILNE) ? nblocks(ni) : 0;
     case PH_GHOST: return 28L * (long)nblocks(ghost_count(g, d->lbi));
     default: return 0;
   }
@@ -1238,11 +1333,11 @@ static void with_bool(bool b, F&& f)
   else f(std::false_type{});
 }
 
-template <bool P>
+template <bool P, bool CART>
 static void launch_tile(viral_ctx* c, int d, int nt, const Sched& s, unsigned grid, size_t shm, int stage, int cur)
 {
-#define TILE_CASE(D, NT)                                                                                    \
-  hipLaunchKernelGGL((k_flux_tile<D, NT, P>), dim3(grid), dim3(NT), shm, c->stream, c->d_tab, s, c->d_clk, \
+#define TILE_CASE(D, NT)                                                                                          \
+  hipLaunchKernelGGL((k_flux_tile<D, NT, P, CART>), dim3(grid), dim3(NT), shm, c->stream, c->d_tab, s, c->d_clk, \
                      stage, cur)
 #define TILE_NT(D)                       \
   switch (nt) {                          \
@@ -1263,22 +1358,25 @@ static void launch_flux(viral_ctx* c, int d, int stage, int cur)
   KTimer t(c, K_FLUX_X + d);
   with_bool(c->cfg.blocks_per_cu > 0, [&](auto PT) {
     constexpr bool P = decltype(PT)::value;
-    for_phase(c, PH_FLUX_X + d, [&](const Sched& s, unsigned grid, int e) {
-      if (c->scheme == VIRAL_SHASTA) {
-        if (c->fast)
-          hipLaunchKernelGGL((k_shasta_dir<true, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
-                             stage, cur, d);
-        else
-          hipLaunchKernelGGL((k_shasta_dir<false, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
-                             stage, cur, d);
-      } else if (c->fast) {
-        const size_t shm =
-            e < 0 ? c->shm[d] : 6 * (size_t)c->ev[e].tile_tp[d] * c->ev[e].tile_tk[d] * sizeof(double);
-        launch_tile<P>(c, d, c->cfg.flux_nt[d], s, grid, shm, stage, cur);
-      } else {
-        hipLaunchKernelGGL((k_flux_dir<false, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
-                           stage, cur, d);
-      }
+    with_bool(c->cart, [&](auto CT) {
+      constexpr bool CART = decltype(CT)::value;
+      for_phase(c, PH_FLUX_X + d, [&](const Sched& s, unsigned grid, int e) {
+        if (c->scheme == VIRAL_SHASTA) {
+          if (c->fast)
+            hipLaunchKernelGGL((k_shasta_dir<true, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
+                               stage, cur, d);
+          else
+            hipLaunchKernelGGL((k_shasta_dir<false, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
+                               stage, cur, d);
+        } else if (c->fast) {
+          const size_t shm =
+              e < 0 ? c->shm[d] : 6 * (size_t)c->ev[e].tile_tp[d] * c->ev[e].tile_tk[d] * sizeof(double);
+          launch_tile<P, CART>(c, d, c->cfg.flux_nt[d], s, grid, shm, stage, cur);
+        } else {
+          hipLaunchKernelGGL((k_flux_dir<false, P, CART>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s,
+                             c->d_clk, stage, cur, d);
+        }
+      });
     });
   });
 }
@@ -1289,21 +1387,24 @@ static void launch_cells(viral_ctx* c, int stage, int cur)
     constexpr bool P = decltype(PT)::value;
     with_bool(c->fast, [&](auto FT) {
       constexpr bool FAST = decltype(FT)::value;
-      if (!c->cfg.fusion) {
-        KTimer t(c, K_SOURCE);
+      with_bool(c->cart, [&](auto CT) {
+        constexpr bool CART = decltype(CT)::value;
+        if (!c->cfg.fusion) {
+          KTimer t(c, K_SOURCE);
+          for_phase(c, PH_CELLS, [&](const Sched& s, unsigned grid, int) {
+            hipLaunchKernelGGL((k_source<FAST, P, CART>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
+                               stage, cur);
+          });
+        }
+        KTimer t(c, K_UPDATE);
         for_phase(c, PH_CELLS, [&](const Sched& s, unsigned grid, int) {
-          hipLaunchKernelGGL((k_source<FAST, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk, stage,
-                             cur);
+          if (c->cfg.fusion)
+            hipLaunchKernelGGL((k_update<FAST, true, P, CART>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s,
+                               c->d_clk, stage, cur);
+          else
+            hipLaunchKernelGGL((k_update<FAST, false, P, CART>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s,
+                               c->d_clk, stage, cur);
         });
-      }
-      KTimer t(c, K_UPDATE);
-      for_phase(c, PH_CELLS, [&](const Sched& s, unsigned grid, int) {
-        if (c->cfg.fusion)
-          hipLaunchKernelGGL((k_update<FAST, true, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
-                             stage, cur);
-        else
-          hipLaunchKernelGGL((k_update<FAST, false, P>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s, c->d_clk,
-                             stage, cur);
       });
     });
   });
@@ -1479,7 +1580,12 @@ static int check_desc(viral_ctx* ctx, const viral_desc* d)
   if (d->lbi && d->nz != 1) return fail(ctx, VIRAL_ERR_ARG, "lbi grids need nz == 1");
   if (d->rk < 1 || d->rk > 3) return fail(ctx, VIRAL_ERR_ARG, "rk must be 1, 2 or 3");
   if (d->scheme != VIRAL_KT && d->scheme != VIRAL_SHASTA) return fail(ctx, VIRAL_ERR_ARG, "unknown scheme");
-  if (d->coords != VIRAL_MILNE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "only Milne coordinates are implemented");
+  if (d->coords != VIRAL_MILNE && d->coords != VIRAL_CARTESIAN) return fail(ctx, VIRAL_ERR_ARG, "unknown coordinates");
+  if (d->coords == VIRAL_CARTESIAN && d->scheme != VIRAL_KT)
+    return fail(ctx, VIRAL_ERR_UNSUPPORTED,
+                "Cartesian coordinates run the KT scheme only (the reference's TXYZ5 SHASTA sources do not compile)");
+  if (d->coords == VIRAL_CARTESIAN && d->lbi)
+    return fail(ctx, VIRAL_ERR_UNSUPPORTED, "boost invariance needs Milne coordinates");
   if (d->eos != VIRAL_EOS_CONFORMAL && d->eos != VIRAL_EOS_S95P) return fail(ctx, VIRAL_ERR_ARG, "unknown eos");
   if (d->max_iter < 1) return fail(ctx, VIRAL_ERR_ARG, "max_iter must be >= 1");
   if (d->eos_table && !d->fast_math)
@@ -1509,6 +1615,7 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
   ctx->rk = descs[0].rk;
   ctx->scheme = descs[0].scheme;
   ctx->fast = descs[0].fast_math != 0;
+  ctx->cart = descs[0].coords == VIRAL_CARTESIAN;
   ctx->cfg.batched = nev > 1;
   ctx->cfg.graph = 0;
   ctx->cfg.blocks_per_cu = 0;
@@ -1937,10 +2044,14 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps)
             Sched s{nullptr, ctx->cnt[PH_FLUX_X + d][ev], ev, 1, ctx->cfg.xcd};
             const size_t shm = 6 * (size_t)ctx->ev[ev].tile_tp[d] * ctx->ev[ev].tile_tk[d] * sizeof(double);
             const unsigned grid = grid_for(ctx, s.total);
-            if (ctx->cfg.blocks_per_cu > 0)
-              r = time_ms(ctx, reps, &ms, [&] { launch_tile<true>(ctx, d, nt, s, grid, shm, 0, cur); });
-            else
-              r = time_ms(ctx, reps, &ms, [&] { launch_tile<false>(ctx, d, nt, s, grid, shm, 0, cur); });
+            r = VIRAL_OK;
+            with_bool(ctx->cfg.blocks_per_cu > 0, [&](auto PT) {
+              with_bool(ctx->cart, [&](auto CT) {
+                r = time_ms(ctx, reps, &ms, [&] {
+                  launch_tile<decltype(PT)::value, decltype(CT)::value>(ctx, d, nt, s, grid, shm, 0, cur);
+                });
+              });
+            });
           } else {
             r = time_ms(ctx, reps, &ms, [&] { launch_flux(ctx, d, 0, cur); });
           }

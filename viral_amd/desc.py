@@ -16,7 +16,8 @@ Q_GHOST_EIGEN = 1 << 0
 Q_K0_BROADCAST = 1 << 1
 Q_DPDE_OF_P = 1 << 2
 Q_GUBSER_MIRROR = 1 << 3
-Q_DEFAULT = Q_GHOST_EIGEN | Q_K0_BROADCAST | Q_DPDE_OF_P
+Q_SHEAR_SLICES = 1 << 4
+Q_DEFAULT = Q_GHOST_EIGEN | Q_K0_BROADCAST | Q_DPDE_OF_P | Q_SHEAR_SLICES
 
 (F_T00, F_T10, F_T20, F_T30) = range(4)
 F_PI0 = 4

@@ -64,6 +64,33 @@ def assemble(desc, e, P, u, pi, prevu, tau, Temp=None, PI=None):
     return st
 
 
+def cart_pi10(u, p):
+    """Full Cartesian pi^{mu nu} (slots tt,tx,ty,tz,xx,xy,xz,yy,yz,zz) from the
+    five evolved components p = (xx, yy, xy, xz, yz) by u_mu pi^{mu nu} = 0
+    and pi^mu_mu = 0 (the A1..A5 forms of NEWEST5/maindef.h:497-501 at
+    tau = 1, which the Cartesian TXYZ5 code uses)."""
+    u0, u1, u2, u3 = u
+    p1, p2, p3, p4, p5 = p
+    zz = ((p1 * u1 * u1 + 2.0 * p3 * u1 * u2 + p2 * u2 * u2 + 2.0 * u3 * (p4 * u1 + p5 * u2) - u0 * u0 * (p1 + p2))
+          / (u0 * u0 - u3 * u3))
+    tx = (p1 * u1 + p3 * u2 + p4 * u3) / u0
+    ty = (p3 * u1 + p2 * u2 + p5 * u3) / u0
+    tz = (p4 * u1 + p5 * u2 + zz * u3) / u0
+    return [p1 + p2 + zz, tx, ty, tz, p1, p3, p4, p2, p5, zz]
+
+
+def cartesian_state(desc, e, P, u, p5, prevu, Temp=None):
+    """29-field Cartesian state: T^{t mu} from the full pi, the five evolved
+    components in pi slots 0..4, slots 5..9 zero."""
+    full = cart_pi10(u, p5)
+    st = assemble(desc, e, P, u, full, prevu, 1.0, Temp=Temp)
+    for l in range(5):
+        st[F_PI0 + l] = np.asarray(p5[l]).ravel()
+    for l in range(5, 10):
+        st[F_PI0 + l] = 0.0
+    return st
+
+
 def gubser(desc, tau, ts, q=1.0, eps0=1.0):
     """Ideal Gubser flow (closed form of NEWideal/init.h:139-195, velocity of
     NEWEST10/init.h:470-521): e = eps0 (2q)^(8/3) / (tau (1+g))^(4/3),
@@ -116,6 +143,8 @@ def random_smooth(desc, seed, tau, e_scale=4.0, v_scale=0.3, pi_frac=0.05, prevu
     u = [u0, u1, u2, u3]
     prevu = [c * (1.0 + prevu_jitter * rng.standard_normal(e.shape)) for c in u]
     pi = [pi_frac * e * rng.standard_normal(e.shape) for _ in range(10)]
+    if desc.coords != MILNE:
+        return cartesian_state(desc, e, P, u, pi[:5], prevu)
     for l in (3, 6, 8):
         pi[l] = pi[l] / t3
     pi[9] = pi[9] / (t3 * t3)
