@@ -1,0 +1,421 @@
+// adjoint.h -- hand-written reverse-mode (vector-Jacobian) forms of the
+// pieces of one viscous KT stage in Milne coordinates: WENO5 reconstruction,
+// the centred flux / Var / eigenvalue of a cell, the Israel-Stewart shear
+// source, the geometric source, the EOS and transport coefficients, the
+// velocities and the Newton root (implicit-function theorem at the converged
+// root: de = -(df/dtheta) dtheta / (df/de)).
+//
+// Each *_vjp function takes the adjoint of its outputs and ADDS the adjoint of
+// its inputs into the caller's accumulators.  All functions are host- and
+// device-callable so they can be checked against finite differences on the CPU.
+#pragma once
+
+#include "physics.h"
+
+namespace viral {
+
+#define VHD __host__ __device__ __forceinline__
+
+// d(L or R)/d(x0..x4) of the WENO5-JS reconstruction of NEWEST10/utils.h:
+// 102-160 on the stencil x = (q_{c-2}, .., q_{c+2}).  With w_c = a_c / sum a,
+// a_c = d_c / (eps + b_c)^2 and candidate values q_c (linear in x):
+//   dOut/dx_i = sum_c w_c dq_c/dx_i + (1/sum a) sum_c (q_c - Out) da_c/dx_i,
+//   da_c/dx_i = -2 a_c / (eps + b_c) db_c/dx_i,
+//   b_c = 13/12 A_c^2 + 1/4 C_c^2 (A_c, C_c linear in x).
+VHD void weno_grad(const double x[5], bool right, double g[5])
+{
+  // linear forms of A_c and C_c (smoothness indicators), candidates q_c
+  const double Ac[3][5] = {{0, 0, 1, -2, 1}, {0, 1, -2, 1, 0}, {1, -2, 1, 0, 0}};
+  const double Cc[3][5] = {{0, 0, 3, -4, 1}, {0, 1, 0, -1, 0}, {1, -4, 3, 0, 0}};
+  const double QL[3][5] = {{0, 0, 2, 5, -1}, {0, -1, 5, 2, 0}, {2, -7, 11, 0, 0}};
+  const double QR[3][5] = {{0, 0, 11, -7, 2}, {0, 2, 5, -1, 0}, {-1, 5, 2, 0, 0}};
+  const double dL[3] = {0.3, 0.6, 0.1}, dR[3] = {0.1, 0.6, 0.3};
+  double A[3], C[3], b[3], al[3], q[3], sum = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    A[c] = C[c] = q[c] = 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      A[c] += Ac[c][i] * x[i];
+      C[c] += Cc[c][i] * x[i];
+      q[c] += (right ? QR[c][i] : QL[c][i]) * x[i];
+    }
+    q[c] = q[c] / 6.0;
+    b[c] = (13. / 12.) * (A[c] * A[c]) + 0.25 * (C[c] * C[c]);
+    const double t = WENO_EPS + b[c];
+    al[c] = (right ? dR[c] : dL[c]) / (t * t);
+    sum += al[c];
+  }
+  double out = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) out += (al[c] / sum) * q[c];
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    double s1 = 0, s2 = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double w = al[c] / sum;
+      s1 += w * (right ? QR[c][i] : QL[c][i]) / 6.0;
+      const double db = (13. / 6.) * A[c] * Ac[c][i] + 0.5 * C[c] * Cc[c][i];
+      s2 += (q[c] - out) * (-2.0 * al[c] / (WENO_EPS + b[c])) * db;
+    }
+    g[i] = s1 + s2 / sum;
+  }
+}
+
+// conformal dT/de of T = (e/FACTOR)^(1/4)
+VHD double eos_dTde(const Params& p, double e, double T) { return T / (4.0 * e); }
+
+// u = gamma (1, v), gamma = (1 - vx^2 - vy^2 - t2 ve^2)^(-1/2):
+// adds d<ub, u>/dv to vb
+VHD void gamma_vjp(const double v[3], double t2, const double ub[4], double vb[3])
+{
+  const double gm = 1.0 / sqrt(1 - v[0] * v[0] - v[1] * v[1] - t2 * v[2] * v[2]);
+  const double s = ub[0] + ub[1] * v[0] + ub[2] * v[1] + ub[3] * v[2];
+  const double g3 = gm * gm * gm;
+  vb[0] += gm * ub[1] + g3 * v[0] * s;
+  vb[1] += gm * ub[2] + g3 * v[1] * s;
+  vb[2] += gm * ub[3] + g3 * t2 * v[2] * s;
+}
+
+// Centred KT flux F_l (15 rows), Var V_l and eigenvalue A = u^d/u^0 of one
+// cell for direction d (NEWEST10/hydroKT.h:16-98, 176-258, 364-381):
+//   T rows  F_l = tau (pi^{d+1,l} + [l = d+1] pp + u^{d+1} u^l w),  V_l = tau T^{tau l}
+//   pi rows F_l = pi u^{d+1}/u^0, V = pi;  PI row likewise;
+//   w = e + P - PI, pp = P - PI (eta direction: (P - PI)/tau^2).
+// Adds the adjoint of (e, P, PI, u, pi, T) given Fb, Vb, Ab.
+VHD void flux_cell_vjp(int d, double tau, double e, double P, double PI, const double u[4], const double pi10[10],
+                       const double Fb[15], const double Vb[15], double Ab, double& eb, double& Pb, double& PIb,
+                       double ub[4], double pib[10], double Tb[4])
+{
+  const int row = d + 1;
+  const double ud = u[row], u0 = u[0];
+  const double w = e + P - PI;
+  double wb = 0, ppb = 0, udb = 0;
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const double vb = tau * Fb[l];
+    pib[pidx(row, l)] += vb;
+    if (l == row) ppb += vb;
+    udb += vb * u[l] * w;
+    ub[l] += vb * ud * w;
+    wb += vb * ud * u[l];
+  }
+  const double iu0 = 1.0 / u0;
+#pragma unroll
+  for (int l = 0; l < 11; l++) {
+    const double f = l < 10 ? pi10[l] : PI;
+    const double fb = Fb[4 + l];
+    if (l < 10) pib[l] += fb * ud * iu0;
+    else PIb += fb * ud * iu0;
+    udb += fb * f * iu0;
+    ub[0] -= fb * f * ud * iu0 * iu0;
+  }
+  const double k = d == 2 ? 1.0 / (tau * tau) : 1.0;
+  Pb += ppb * k;
+  PIb -= ppb * k;
+  eb += wb;
+  Pb += wb;
+  PIb -= wb;
+#pragma unroll
+  for (int l = 0; l < 4; l++) Tb[l] += tau * Vb[l];
+#pragma unroll
+  for (int l = 0; l < 10; l++) pib[l] += Vb[4 + l];
+  PIb += Vb[14];
+  udb += Ab * iu0;
+  ub[0] -= Ab * ud * iu0 * iu0;
+  ub[row] += udb;
+}
+
+// Reverse of shear_source (physics.h): given ob = d(J)/d(out), adds the
+// adjoint of u, pi, du, SH and tpi.  The forward intermediates are recomputed
+// in the same order.
+VHD void shear_source_vjp(int milne, double tau, const double u[4], const double pi10[10], const double du[4][4],
+                          double SH, double tpi, const double ob[10], double ub[4], double pib[10], double dub[4][4],
+                          double& SHb, double& tpib)
+{
+  const double g[4] = {1.0, -1.0, -1.0, milne ? -(tau * tau) : -1.0};
+  const double gi[4] = {1.0, -1.0, -1.0, milne ? -(1.0 / (tau * tau)) : -1.0};
+  double ul[4];
+  for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
+  const double iu0 = 1.0 / u[0];
+  double divv = 0;
+  for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
+  double Dc[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
+  const double itau = milne ? 1.0 / tau : 0.0;
+  if (milne) {
+    Dc[0][3] = Dc[0][3] + tau * u[3];
+    Dc[3][0] = Dc[3][0] + itau * u[3];
+    Dc[3][3] = Dc[3][3] + itau * u[0];
+  }
+  const double theta = Dc[0][0] + Dc[1][1] + Dc[2][2] + Dc[3][3];
+  double Dul[4];
+  for (int a = 0; a < 4; a++) {
+    double s = 0;
+    for (int b = 0; b < 4; b++) s += u[b] * Dc[a][b];
+    Dul[a] = g[a] * s;
+  }
+  double G[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
+  double Gu[4], uG[4], H[4];
+  for (int m = 0; m < 4; m++) {
+    double s1 = 0, s2 = 0;
+    for (int b = 0; b < 4; b++) {
+      s1 += G[m][b] * ul[b];
+      s2 += ul[b] * G[b][m];
+    }
+    Gu[m] = s1;
+    uG[m] = s2;
+  }
+  double uGu = 0, trg = 0;
+  for (int a = 0; a < 4; a++) {
+    uGu += ul[a] * Gu[a];
+    trg += g[a] * G[a][a];
+  }
+  const double tr = trg - uGu;
+  for (int m = 0; m < 4; m++) H[m] = Gu[m] + uG[m];
+  const double c3 = uGu + (1.0 / 3.0) * tr;
+  double PD[4];
+  for (int m = 0; m < 4; m++) {
+    double s = 0;
+    for (int a = 0; a < 4; a++) s += pi10[pidx(m, a)] * Dul[a];
+    PD[m] = s;
+  }
+  // ---- reverse sweep
+  double ulb[4] = {0, 0, 0, 0}, Hb[4] = {0, 0, 0, 0}, PDb[4] = {0, 0, 0, 0}, Dulb[4] = {0, 0, 0, 0};
+  double Gb[4][4], Dcb[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) Gb[a][b] = Dcb[a][b] = 0;
+  double iu0b = 0, divvb = 0, thetab = 0, c3b = 0, trb = 0;
+  for (int m = 0; m < 4; m++)
+    for (int n = m; n < 4; n++) {
+      const int q = pidx(m, n);
+      const double o = ob[q];
+      const double pmn = pi10[q];
+      double sigma = 0.5 * (G[m][n] + G[n][m]) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
+      if (m == n) sigma = sigma - (1.0 / 3.0) * gi[m] * tr;
+      const double pu = u[m] * PD[n] + u[n] * PD[m];
+      const double rel = -(pmn - 2.0 * SH * sigma) / tpi;
+      const double rhs = rel - (4.0 / 3.0) * pmn * theta - pu;
+      double chr = 0;
+      if (milne) {
+        double cm = 0, cn = 0;
+        if (m == 0) cm = u[3] * tau * pi10[pidx(3, n)];
+        if (m == 3) cm = u[0] * itau * pi10[pidx(3, n)] + u[3] * itau * pi10[pidx(0, n)];
+        if (n == 0) cn = u[3] * tau * pi10[pidx(m, 3)];
+        if (n == 3) cn = u[0] * itau * pi10[pidx(m, 3)] + u[3] * itau * pi10[pidx(m, 0)];
+        chr = cm + cn;
+      }
+      // out = (rhs - chr) iu0 + pmn divv
+      const double rhsb = o * iu0, chrb = -o * iu0;
+      iu0b += o * (rhs - chr);
+      pib[q] += o * divv;
+      divvb += o * pmn;
+      if (milne) {
+        if (m == 0) { ub[3] += chrb * tau * pi10[pidx(3, n)]; pib[pidx(3, n)] += chrb * u[3] * tau; }
+        if (m == 3) {
+          ub[0] += chrb * itau * pi10[pidx(3, n)]; pib[pidx(3, n)] += chrb * u[0] * itau;
+          ub[3] += chrb * itau * pi10[pidx(0, n)]; pib[pidx(0, n)] += chrb * u[3] * itau;
+        }
+        if (n == 0) { ub[3] += chrb * tau * pi10[pidx(m, 3)]; pib[pidx(m, 3)] += chrb * u[3] * tau; }
+        if (n == 3) {
+          ub[0] += chrb * itau * pi10[pidx(m, 3)]; pib[pidx(m, 3)] += chrb * u[0] * itau;
+          ub[3] += chrb * itau * pi10[pidx(m, 0)]; pib[pidx(m, 0)] += chrb * u[3] * itau;
+        }
+      }
+      // rhs = rel - 4/3 pmn theta - pu
+      const double relb = rhsb;
+      pib[q] += -(4.0 / 3.0) * theta * rhsb;
+      thetab += -(4.0 / 3.0) * pmn * rhsb;
+      const double pub = -rhsb;
+      // rel = -(pmn - 2 SH sigma) / tpi
+      pib[q] += -relb / tpi;
+      SHb += 2.0 * sigma * relb / tpi;
+      const double sigb = 2.0 * SH * relb / tpi;
+      tpib += (pmn - 2.0 * SH * sigma) / (tpi * tpi) * relb;
+      // pu = u^m PD^n + u^n PD^m
+      ub[m] += pub * PD[n];
+      PDb[n] += pub * u[m];
+      ub[n] += pub * PD[m];
+      PDb[m] += pub * u[n];
+      // sigma
+      Gb[m][n] += 0.5 * sigb;
+      Gb[n][m] += 0.5 * sigb;
+      ub[n] -= 0.5 * sigb * H[m];
+      Hb[m] -= 0.5 * sigb * u[n];
+      ub[m] -= 0.5 * sigb * H[n];
+      Hb[n] -= 0.5 * sigb * u[m];
+      ub[m] += sigb * u[n] * c3;
+      ub[n] += sigb * u[m] * c3;
+      c3b += sigb * u[m] * u[n];
+      if (m == n) trb -= (1.0 / 3.0) * gi[m] * sigb;
+    }
+  // PD^m = pi^{ma} Dul_a
+  for (int m = 0; m < 4; m++)
+    for (int a = 0; a < 4; a++) {
+      pib[pidx(m, a)] += PDb[m] * Dul[a];
+      Dulb[a] += PDb[m] * pi10[pidx(m, a)];
+    }
+  double uGub = c3b;
+  trb += c3b * (1.0 / 3.0);
+  double Gub[4], uGb[4];
+  for (int m = 0; m < 4; m++) Gub[m] = uGb[m] = Hb[m];
+  const double trgb = trb;
+  uGub -= trb;
+  for (int a = 0; a < 4; a++) Gb[a][a] += g[a] * trgb;
+  for (int a = 0; a < 4; a++) {
+    ulb[a] += uGub * Gu[a];
+    Gub[a] += uGub * ul[a];
+  }
+  for (int m = 0; m < 4; m++)
+    for (int b = 0; b < 4; b++) {
+      Gb[m][b] += Gub[m] * ul[b];
+      ulb[b] += Gub[m] * G[m][b];
+      ulb[b] += uGb[m] * G[b][m];
+      Gb[b][m] += uGb[m] * ul[b];
+    }
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) Dcb[b][a] += gi[a] * Gb[a][b];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) {
+      ub[b] += g[a] * Dulb[a] * Dc[a][b];
+      Dcb[a][b] += g[a] * Dulb[a] * u[b];
+    }
+  for (int a = 0; a < 4; a++) Dcb[a][a] += thetab;
+  if (milne) {
+    ub[3] += Dcb[0][3] * tau + Dcb[3][0] * itau;
+    ub[0] += Dcb[3][3] * itau;
+  }
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) dub[a][b] += Dcb[a][b];
+  for (int i = 1; i < 4; i++) {
+    dub[i][i] += divvb * iu0;
+    dub[0][i] -= divvb * u[i] * iu0 * iu0;
+    ub[i] -= divvb * du[0][i] * iu0 * iu0;
+    iu0b += divvb * (du[i][i] - 2.0 * u[i] * du[0][i] * iu0);
+  }
+  for (int m = 0; m < 4; m++) ub[m] += g[m] * ulb[m];
+  ub[0] -= iu0b * iu0 * iu0;
+}
+
+// Sources of one cell (NEWEST10/Source.h:208-221 geometric, :347-517 shear)
+// given the velocity gradients du and the transport coefficients: adds the
+// adjoint of (e, P, PI, u, pi, du) and of the viscosity / relaxation
+// coefficients (eta_coef, tpi_coef) for the source adjoint Sb (15 rows).
+// Conformal EOS (s = (e+P)/T, T = (e/FACTOR)^(1/4)).
+VHD void source_vjp(const Params& p, double tau, double e, double P, double PI, const double u[4], const double pi10[10],
+                    const double du[4][4], const double Sb[15], double& eb, double& Pb, double& PIb, double ub[4],
+                    double pib[10], double dub[4][4], double& cetab, double& ctpib)
+{
+  const double w = e + P - PI;
+  double wb = 0;
+  // S0 = -(P - PI) - tau^2 (pi^{ee} + w u3 u3)
+  Pb -= Sb[0];
+  PIb += Sb[0];
+  pib[9] -= tau * tau * Sb[0];
+  wb -= tau * tau * u[3] * u[3] * Sb[0];
+  ub[3] -= 2.0 * tau * tau * w * u[3] * Sb[0];
+  // S3 = -2 (pi^{te} + w u0 u3)
+  pib[3] -= 2.0 * Sb[3];
+  wb -= 2.0 * u[0] * u[3] * Sb[3];
+  ub[0] -= 2.0 * w * u[3] * Sb[3];
+  ub[3] -= 2.0 * w * u[0] * Sb[3];
+  eb += wb;
+  Pb += wb;
+  PIb -= wb;
+  // shear
+  const double T = eos_T(p, e);
+  const double s = eos_s(p, e, P, T);
+  const double SH = eta_of(p, s);
+  const double tpi = tpi_of(p, SH, P, e);
+  double SHb = 0, tpib = 0;
+  shear_source_vjp(1, tau, u, pi10, du, SH, tpi, Sb + 4, ub, pib, dub, SHb, tpib);
+  if (p.tpi_law == 1) {  // c eta / P
+    SHb += tpib * p.tpi_coef / P;
+    Pb -= tpib * p.tpi_coef * SH / (P * P);
+    ctpib += tpib * SH / P;
+  } else if (p.tpi_law == 2) {
+    ctpib += tpib;
+  } else {  // c eta / (e + P)
+    const double ep = e + P;
+    SHb += tpib * p.tpi_coef / ep;
+    eb -= tpib * p.tpi_coef * SH / (ep * ep);
+    Pb -= tpib * p.tpi_coef * SH / (ep * ep);
+    ctpib += tpib * SH / ep;
+  }
+  double sb;
+  if (p.eta_law == 1) {
+    sb = SHb * p.eta_coef / (4.0 * PIE);
+    cetab += SHb * s / (4.0 * PIE);
+  } else {
+    sb = SHb * p.eta_coef;
+    cetab += SHb * s;
+  }
+  // s = (e + P) / T(e)
+  eb += sb / T - sb * (e + P) / (T * T) * eos_dTde(p, e, T);
+  Pb += sb / T;
+}
+
+// Velocities and root of one cell (NEWEST10/root.h:212-424) in reverse.
+// Forward: T0 = V[0..3]/tn, pi = V[4..13], PI = V[14]; e solves
+//   f(e) = e - A + M2/(A + P(e) - PI) = 0,  A = T0_0 - pi_0,
+//   M2 = sum_i g_i (T0_i - pi_i)^2 (g = 1, 1, tn^2);
+// v_i = (T0_i - pi_i)/D, D = A + P(e) - PI; u = gamma (1, v); P, Temp of e.
+// Given the adjoint of (e, P, Temp, v, u) and of the stored T0, pi, PI,
+// writes Vb (15).  The root enters through the implicit-function theorem at
+// the converged e: de = -(f_A dA + f_M2 dM2 + f_PI dPI)/f_e.
+VHD void update_vjp(const Params& p, double tn, const double T0[4], const double pi10[10], double PI, double e,
+                    double ebo, double Pbo, double Tempbo, const double vbo[3], const double ubo[4], const double T0bo[4],
+                    const double pibo[10], double PIbo, double Vb[15])
+{
+  const double t2 = tn * tn;
+  const double P = eos_p(p, e), a = eos_dpde(p, e);
+  const double N[3] = {T0[1] - pi10[1], T0[2] - pi10[2], T0[3] - pi10[3]};
+  const double D = T0[0] - pi10[0] + P - PI;
+  const double v[3] = {N[0] / D, N[1] / D, N[2] / D};
+  double vb[3] = {vbo[0], vbo[1], vbo[2]};
+  gamma_vjp(v, t2, ubo, vb);
+  double T0b[4] = {T0bo[0], T0bo[1], T0bo[2], T0bo[3]};
+  double pib[10];
+  for (int l = 0; l < 10; l++) pib[l] = pibo[l];
+  double PIb = PIbo;
+  double eb = ebo + Pbo * a + Tempbo * eos_dTde(p, e, eos_T(p, e));
+  double Db = 0;
+  for (int i = 0; i < 3; i++) {
+    const double Nb = vb[i] / D;
+    Db -= vb[i] * v[i] / D;
+    T0b[i + 1] += Nb;
+    pib[i + 1] -= Nb;
+  }
+  T0b[0] += Db;
+  pib[0] -= Db;
+  eb += Db * a;
+  PIb -= Db;
+  // implicit-function theorem at the root
+  const double A = T0[0] - pi10[0];
+  const double M2 = N[0] * N[0] + N[1] * N[1] + t2 * N[2] * N[2];
+  const double Q = A + P - PI;
+  const double fe = 1.0 - M2 * a / (Q * Q);
+  const double lam = -eb / fe;
+  const double Ab = lam * (-1.0 - M2 / (Q * Q));
+  const double M2b = lam / Q;
+  PIb += lam * M2 / (Q * Q);
+  T0b[0] += Ab;
+  pib[0] -= Ab;
+  const double gg[3] = {1.0, 1.0, t2};
+  for (int i = 0; i < 3; i++) {
+    const double nb = M2b * 2.0 * gg[i] * N[i];
+    T0b[i + 1] += nb;
+    pib[i + 1] -= nb;
+  }
+  for (int l = 0; l < 4; l++) Vb[l] = T0b[l] / tn;
+  for (int l = 0; l < 10; l++) Vb[4 + l] = pib[l];
+  Vb[14] = PIb;
+}
+
+#undef VHD
+
+}  // namespace viral

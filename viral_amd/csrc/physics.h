@@ -187,13 +187,13 @@ __host__ __device__ __forceinline__ double s95p_T(double e)
 
 // EOS(e), DPDE(e), FT(e), FS(e,P,T): NEWEST10/maindef.h:68-72 (conformal,
 // with x^(1/4) as sqrt(sqrt(x))) and :274-278 (s95p)
-__device__ __forceinline__ double eos_p(const Params& p, double e) { return p.eos ? s95p_p(e) : e / 3.0; }
-__device__ __forceinline__ double eos_dpde(const Params& p, double e) { return p.eos ? s95p_a(e) : 1.0 / 3.0; }
-__device__ __forceinline__ double eos_T(const Params& p, double e)
+__host__ __device__ __forceinline__ double eos_p(const Params& p, double e) { return p.eos ? s95p_p(e) : e / 3.0; }
+__host__ __device__ __forceinline__ double eos_dpde(const Params& p, double e) { return p.eos ? s95p_a(e) : 1.0 / 3.0; }
+__host__ __device__ __forceinline__ double eos_T(const Params& p, double e)
 {
   return p.eos ? s95p_T(e) : sqrt(sqrt(e / p.factor));
 }
-__device__ __forceinline__ double eos_s(const Params& p, double e, double P, double T)
+__host__ __device__ __forceinline__ double eos_s(const Params& p, double e, double P, double T)
 {
   return p.eos ? s95p_s(e) : (e + P) * (1.0 / T);
 }
@@ -232,11 +232,11 @@ __host__ __device__ __forceinline__ void s95p_all(double e, double out[4])
 }
 
 // Feta / Ftaupi laws, NEWEST10/maindef.h:76-77,187-188,282-283,328-329
-__device__ __forceinline__ double eta_of(const Params& p, double s)
+__host__ __device__ __forceinline__ double eta_of(const Params& p, double s)
 {
   return p.eta_law == 1 ? p.eta_coef * (s / (4.0 * PIE)) : p.eta_coef * s;
 }
-__device__ __forceinline__ double tpi_of(const Params& p, double eta, double P, double e)
+__host__ __device__ __forceinline__ double tpi_of(const Params& p, double eta, double P, double e)
 {
   if (p.tpi_law == 1) return p.tpi_coef * (eta / P);
   if (p.tpi_law == 2) return p.tpi_coef;
@@ -246,7 +246,7 @@ __device__ __forceinline__ double tpi_of(const Params& p, double eta, double P, 
 // ---------------------------------------------------------------- WENO5-JS
 // NEWEST10/utils.h:102-160: L is the value at the right face of the centre
 // cell, R at its left face; d = (0.3, 0.6, 0.1), eps = 1e-6, p = 2.
-__device__ __forceinline__ void weno_beta(double qm2, double qm1, double qc, double qp1, double qp2, double& b0,
+__host__ __device__ __forceinline__ void weno_beta(double qm2, double qm1, double qc, double qp1, double qp2, double& b0,
                                           double& b1, double& b2)
 {
   double a0 = qc - 2 * qp1 + qp2, c0 = 3 * qc - 4 * qp1 + qp2;
@@ -257,7 +257,7 @@ __device__ __forceinline__ void weno_beta(double qm2, double qm1, double qc, dou
   b2 = (13. / 12.) * (a2 * a2) + 0.25 * (c2 * c2);
 }
 
-__device__ __forceinline__ double weno_L(double qm2, double qm1, double qc, double qp1, double qp2)
+__host__ __device__ __forceinline__ double weno_L(double qm2, double qm1, double qc, double qp1, double qp2)
 {
   double b0, b1, b2;
   weno_beta(qm2, qm1, qc, qp1, qp2, b0, b1, b2);
@@ -277,7 +277,7 @@ __device__ __forceinline__ double weno_L(double qm2, double qm1, double qc, doub
   return left;
 }
 
-__device__ __forceinline__ double weno_R(double qm2, double qm1, double qc, double qp1, double qp2)
+__host__ __device__ __forceinline__ double weno_R(double qm2, double qm1, double qc, double qp1, double qp2)
 {
   double b0, b1, b2;
   weno_beta(qm2, qm1, qc, qp1, qp2, b0, b1, b2);
@@ -298,7 +298,7 @@ __device__ __forceinline__ double weno_R(double qm2, double qm1, double qc, doub
 }
 
 // NEWEST10/utils.h:57-98
-__device__ __forceinline__ double weno_der(double qm2, double qm1, double qc, double qp1, double qp2, double dx)
+__host__ __device__ __forceinline__ double weno_der(double qm2, double qm1, double qc, double qp1, double qp2, double dx)
 {
   double left = weno_L(qm2, qm1, qc, qp1, qp2);
   double right = weno_R(qm2, qm1, qc, qp1, qp2);
@@ -312,7 +312,7 @@ __device__ __forceinline__ double weno_der(double qm2, double qm1, double qc, do
 //   sigma^{mn} = Delta^{mn}_{ab} d^a u^b (double projector, du[a][b] = d_b u^a
 //   taken as independent), S = (u.d pi - Christoffel part)/u^0 + pi d_i(u^i/u^0).
 // Operation order matches the CPU restatement exactly.
-__device__ __forceinline__ int pidx(int m, int n)
+__host__ __device__ __forceinline__ int pidx(int m, int n)
 {
   // {{0,1,2,3},{1,4,5,6},{2,5,7,8},{3,6,8,9}}
   const int a = m < n ? m : n, b = m < n ? n : m;
@@ -320,14 +320,14 @@ __device__ __forceinline__ int pidx(int m, int n)
 }
 
 template <bool FAST = false>
-__device__ __forceinline__ void shear_source(int milne, double tau, const double u[4], const double pi10[10],
+__host__ __device__ __forceinline__ void shear_source(int milne, double tau, const double u[4], const double pi10[10],
                                              const double du[4][4], double SH, double tpi, double out[10]);
 
 // ---------------------------------------------------------------- root solve
 // f(e), f'(e) of NEWEST10/root.h:212-285 and the GSL Newton iteration
 // (x_{n+1} = x_n - f/f', stop on two equal iterates, error on f' == 0 or a
 // non-finite f or f'), NEWEST10/root.h:301-364.
-__device__ __forceinline__ double newton_cell(const Params& p, double M2, double Ttt, double pi1, double PI,
+__host__ __device__ __forceinline__ double newton_cell(const Params& p, double M2, double Ttt, double pi1, double PI,
                                               double guess, int& iters, unsigned& flags)
 {
   double x = guess, fv, dfv;
@@ -391,13 +391,17 @@ __device__ __forceinline__ double newton_cell(const Params& p, double M2, double
 // 1/x from the hardware estimate (v_rcp_f64) refined by two Newton-Raphson
 // steps r <- r + r(1 - x r): within an ulp or two of the IEEE quotient for
 // the normal, finite denominators the fast path feeds it.
-__device__ __forceinline__ double frcp(double x)
+__host__ __device__ __forceinline__ double frcp(double x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(x);
   double e = fma(-x, r, 1.0);
   r = fma(r, e, r);
   e = fma(-x, r, 1.0);
   return fma(r, e, r);
+#else
+  return 1.0 / x;  // host builds (the adjoint checks) use the IEEE quotient
+#endif
 }
 
 // Same WENO5-JS weights written with one division per reconstruction:
@@ -595,7 +599,7 @@ __device__ __forceinline__ void eos_Ts_fast(const Params& p, double e, double P,
 // Exact mode follows the CPU restatement's operation order; FAST replaces
 // the divisions by reciprocals.
 template <bool FAST>
-__device__ __forceinline__ void shear_source(int milne, double tau, const double u[4], const double pi10[10],
+__host__ __device__ __forceinline__ void shear_source(int milne, double tau, const double u[4], const double pi10[10],
                                              const double du[4][4], double SH, double tpi, double out[10])
 {
   const double g[4] = {1.0, -1.0, -1.0, milne ? -(tau * tau) : -1.0};
