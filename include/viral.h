@@ -242,6 +242,26 @@ int viral_set_tile(viral_ctx* ctx, int ev, int dir, int tp, int tk);
 int viral_get_tile(viral_ctx* ctx, int ev, int dir, int* tp, int* tk);
 int viral_autotune(viral_ctx* ctx, int ev, int reps);
 
+/* ---- Adjoint (reverse mode) of a recorded run.  No counterpart in the
+ * reference (it has no derivatives).
+ *
+ * viral_adjoint_begin(ctx, every) starts recording: it keeps a device copy of
+ * the current state and, from then on, the (tau, dtau) of every step taken by
+ * viral_step / viral_steps and a copy of the state every `every` steps.
+ * viral_adjoint(ctx, seed, grad, pgrad) then runs the recorded steps in
+ * reverse: between checkpoints it recomputes the stage states forward and
+ * applies hand-written adjoint kernels (KT flux and WENO reconstruction,
+ * sources, EOS, velocities; the root by the implicit-function theorem at the
+ * converged root).  seed[f] holds dJ/d(final state field f) (VIRAL_NFIELDS
+ * ghosted arrays), grad[f] receives dJ/d(recorded initial state field f), each
+ * field taken as an independent input, and pgrad[0], pgrad[1] receive dJ/d
+ * eta_coef and dJ/d tpi_coef.  The state is left as it was.  Single-event
+ * Milne KT contexts with the conformal EOS.  viral_adjoint_end frees the
+ * recording. */
+int viral_adjoint_begin(viral_ctx* ctx, int every);
+int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad, double* pgrad);
+int viral_adjoint_end(viral_ctx* ctx);
+
 /* Per-kernel timing with HIP events recorded on the context's stream around
  * every launch (instrumentation; the reference only times whole steps with
  * clock(), NEWEST10/main.cpp:107,170-171).  viral_timers(ctx, 1) clears and

@@ -1,0 +1,165 @@
+"""GPU: the hand-written adjoint against central finite differences.
+
+J = <seed, final state> after a few steps; the adjoint's dJ/d(initial state)
+and dJ/d(eta_coef, tpi_coef) are checked along random directions against
+central differences of the GPU forward (h = 1e-6 relative) and of the CPU
+oracle's forward (the reference gradient); the torch.autograd harness is
+checked on the momentum anisotropy of a Glauber-like event.
+"""
+import numpy as np
+import pytest
+import torch
+
+from viral_amd import capi, ic, autograd
+from viral_amd.desc import (make_desc, NFIELDS, F_T00, F_PI0, F_BULK, F_EN, F_P, F_U0, F_PU0, F_TEMP, F_VX,
+                            ETA_S_OVER_4PI, TPI_OVER_P, OK)
+import oracle_ctypes
+
+pytestmark = pytest.mark.gpu
+
+TAU, TS = 1.0, 0.005
+# fields the step reads from its input state (Temp and the 3-velocity are outputs only)
+READ = [f for f in range(NFIELDS) if f != F_TEMP and not (F_VX <= f < F_VX + 3)]
+
+
+def run_gpu(d, st, nsteps, every=None, seed=None):
+    h = capi.Hydro(d)
+    h.upload(st)
+    if every:
+        h.adjoint_begin(every)
+    r, done = h.steps(TAU, TS, nsteps)
+    assert (r, done) == (OK, nsteps)
+    out = h.download()
+    if seed is None:
+        return out
+    grad, pg = h.adjoint(seed)
+    after = h.download()
+    assert np.array_equal(after, out), "the adjoint must leave the state as it was"
+    return out, grad, pg
+
+
+def run_cpu(d, st, nsteps):
+    o = oracle_ctypes.Oracle(d)
+    o.set_state(st)
+    t = TAU
+    for _ in range(nsteps):
+        assert o.step(t, TS) == OK
+        t = t + TS
+    return o.state()
+
+
+def direction(st, rng):
+    dx = np.zeros_like(st)
+    for f in READ:
+        scale = np.abs(st[f]).max()
+        dx[f] = rng.standard_normal(st.shape[1]) * (scale if scale > 0 else 1e-3)
+    return dx
+
+
+def small_case(rk=2, fast=0, **kw):
+    d = make_desc(10, 9, 7, rk=rk, fast_math=fast, **kw)
+    st = ic.random_smooth(d, seed=3, tau=TAU, pi_frac=0.02)
+    return d, st
+
+
+@pytest.mark.parametrize("rk,every", [(1, 1), (2, 2), (3, 2)])
+def test_adjoint_state_gradient_matches_fd(rk, every):
+    d, st = small_case(rk=rk)
+    rng = np.random.default_rng(rk)
+    seed = rng.standard_normal(st.shape)
+    print("case adjoint rk=%d every=%d" % (rk, every), flush=True)
+    out, grad, pg = run_gpu(d, st, 3, every, seed)
+    for trial in range(3):
+        dx = direction(st, rng)
+        h = 1e-6
+        Jp = np.sum(seed * run_gpu(d, st + h * dx, 3))
+        Jm = np.sum(seed * run_gpu(d, st - h * dx, 3))
+        fd = (Jp - Jm) / (2 * h)
+        ad = np.sum(grad * dx)
+        print(trial, ad, fd, flush=True)
+        assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+
+
+def test_adjoint_parameter_gradients_match_fd_and_cpu_reference():
+    d, st = small_case(eta_law=ETA_S_OVER_4PI, eta_coef=2.0, tpi_law=TPI_OVER_P, tpi_coef=1.5)
+    rng = np.random.default_rng(9)
+    seed = rng.standard_normal(st.shape)
+    print("case adjoint parameters", flush=True)
+    out, grad, pg = run_gpu(d, st, 3, 2, seed)
+    for q, name in enumerate(["eta_coef", "tpi_coef"]):
+        c0 = getattr(d, name)
+        h = 1e-6 * c0
+        vals = []
+        for sgn in (1, -1):
+            dd = d.copy()
+            setattr(dd, name, c0 + sgn * h)
+            vals.append((np.sum(seed * run_gpu(dd, st, 3)), np.sum(seed * run_cpu(dd, st, 3))))
+        fd_gpu = (vals[0][0] - vals[1][0]) / (2 * h)
+        fd_cpu = (vals[0][1] - vals[1][1]) / (2 * h)
+        print(name, pg[q], fd_gpu, fd_cpu, flush=True)
+        assert abs(pg[q] - fd_gpu) <= 1e-5 * abs(fd_gpu), (name, pg[q], fd_gpu)
+        assert abs(pg[q] - fd_cpu) <= 1e-5 * abs(fd_cpu), (name, pg[q], fd_cpu)
+    # the state gradient against the CPU oracle's finite differences
+    dx = direction(st, rng)
+    h = 1e-6
+    fd = (np.sum(seed * run_cpu(d, st + h * dx, 3)) - np.sum(seed * run_cpu(d, st - h * dx, 3))) / (2 * h)
+    ad = np.sum(grad * dx)
+    assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    d, st = small_case()
+    seed = np.random.default_rng(4).standard_normal(st.shape)
+    print("case adjoint checkpoints", flush=True)
+    _, g1, p1 = run_gpu(d, st, 4, 1, seed)
+    _, g3, p3 = run_gpu(d, st, 4, 3, seed)
+    assert np.array_equal(g1, g3) and np.array_equal(p1, p3)
+
+
+def test_fast_mode_adjoint_matches_fd():
+    d, st = small_case(fast=1)
+    rng = np.random.default_rng(5)
+    seed = rng.standard_normal(st.shape)
+    print("case adjoint fast", flush=True)
+    _, grad, pg = run_gpu(d, st, 2, 2, seed)
+    dx = direction(st, rng)
+    h = 1e-6
+    fd = (np.sum(seed * run_gpu(d, st + h * dx, 2)) - np.sum(seed * run_gpu(d, st - h * dx, 2))) / (2 * h)
+    ad = np.sum(grad * dx)
+    assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+
+
+def test_torch_harness_momentum_anisotropy():
+    """d eps_2 / d(initial e-field, eta/s, tau_pi coefficient) through
+    torch.autograd against central differences of the same pipeline."""
+    d = make_desc(16, 14, 6, eta_coef=0.08, tpi_coef=5.0)
+    base = ic.glauber(d, TAU, TS, b=8.0, e_peak_gev=10.0)
+    e0 = torch.tensor(base[F_EN], dtype=torch.float64)
+    rng = np.random.default_rng(6)
+    e0 = e0 * (1 + 0.05 * torch.from_numpy(rng.standard_normal(e0.shape[0])))
+    print("case torch harness", flush=True)
+
+    def objective(e, c_eta, c_tpi):
+        st = autograd.state_from_energy(d, e, TAU)
+        fin = autograd.run(st, c_eta, c_tpi, d, TAU, TS, 4, every=2)
+        return autograd.momentum_anisotropy(d, fin, TAU + 4 * TS)
+
+    e = e0.clone().requires_grad_(True)
+    ce = torch.tensor(0.08, dtype=torch.float64, requires_grad=True)
+    ct = torch.tensor(5.0, dtype=torch.float64, requires_grad=True)
+    J = objective(e, ce, ct)
+    J.backward()
+    with torch.no_grad():
+        de = torch.from_numpy(rng.standard_normal(e0.shape[0])) * e0
+        h = 1e-6
+        fd = (objective(e0 + h * de, ce, ct) - objective(e0 - h * de, ce, ct)) / (2 * h)
+        ad = float((e.grad * de).sum())
+        print("e-field", ad, float(fd), flush=True)
+        assert abs(ad - fd) <= 1e-5 * abs(fd) + 1e-12
+        for t, g in ((ce, ce.grad), (ct, ct.grad)):
+            hh = 1e-6 * float(t)
+            p = t.detach().clone()
+            fd = (objective(e0, p + hh if t is ce else ce, p + hh if t is ct else ct) -
+                  objective(e0, p - hh if t is ce else ce, p - hh if t is ct else ct)) / (2 * hh)
+            print("param", float(g), float(fd), flush=True)
+            assert abs(float(g) - float(fd)) <= 1e-5 * abs(float(fd)) + 1e-12

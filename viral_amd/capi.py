@@ -21,7 +21,8 @@ EXPORTS = [
     "viral_flags", "viral_iterations", "viral_first_bad_cell", "viral_max_temp_gev",
     "viral_eos_eval", "viral_set_tile", "viral_get_tile", "viral_autotune", "viral_timers", "viral_timer_read",
     "viral_kernel_count", "viral_kernel_name", "viral_last_error", "viral_steps", "viral_get_launch",
-    "viral_set_launch", "viral_tile_count", "viral_tile_table", "viral_xcd_tile",
+    "viral_set_launch", "viral_tile_count", "viral_tile_table", "viral_xcd_tile", "viral_adjoint_begin",
+    "viral_adjoint", "viral_adjoint_end",
 ]
 
 # kernel phases of the tile table (VIRAL_PHASE_*)
@@ -109,6 +110,12 @@ def load():
     L.viral_tile_table.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong)]
     L.viral_xcd_tile.restype = C.c_longlong
     L.viral_xcd_tile.argtypes = [C.c_longlong, C.c_longlong]
+    L.viral_adjoint_begin.restype = C.c_int
+    L.viral_adjoint_begin.argtypes = [vp, C.c_int]
+    L.viral_adjoint.restype = C.c_int
+    L.viral_adjoint.argtypes = [vp, C.POINTER(dp), C.POINTER(dp), dp]
+    L.viral_adjoint_end.restype = C.c_int
+    L.viral_adjoint_end.argtypes = [vp]
     _lib = L
     return L
 
@@ -260,6 +267,26 @@ class Hydro:
             else:
                 setattr(c, k, int(v))
         self._chk(self.L.viral_set_launch(self.h, C.byref(c)))
+
+    def adjoint_begin(self, every):
+        """Start recording steps for the adjoint, with a state checkpoint every
+        `every` steps."""
+        self._chk(self.L.viral_adjoint_begin(self.h, every))
+
+    def adjoint(self, seed):
+        """seed: (NFIELDS, cells) dJ/d(final state).  Returns (dJ/d(initial
+        state) as (NFIELDS, cells), [dJ/d eta_coef, dJ/d tpi_coef])."""
+        seed = np.ascontiguousarray(seed, dtype=np.float64)
+        assert seed.shape == (NFIELDS, self.cells(0)), seed.shape
+        grad = np.empty_like(seed)
+        pg = np.zeros(2)
+        self._chk(self.L.viral_adjoint(self.h, _ptr_array([seed[f] for f in range(NFIELDS)]),
+                                       _ptr_array([grad[f] for f in range(NFIELDS)]),
+                                       pg.ctypes.data_as(C.POINTER(C.c_double))))
+        return grad, pg
+
+    def adjoint_end(self):
+        self._chk(self.L.viral_adjoint_end(self.h))
 
     def tile_table(self, phase):
         out = (C.c_longlong * (len(self.descs) + 1))()

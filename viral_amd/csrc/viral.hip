@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/viral.h"
+#include "adjoint.h"
 #include "physics.h"
 
 using namespace viral;
@@ -1069,6 +1070,400 @@ __global__ void k_max_temp(Geom g, Params p, const double* __restrict__ st, unsi
   if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
 }
 
+// ---------------------------------------------------------------- adjoint kernels
+// Reverse of one Milne KT stage for one event (conformal EOS).  lam_out is
+// the adjoint of the stage's output state (29 fields), lam_in receives the
+// adjoint of its input state.  See adjoint.h for the per-cell pieces.
+
+// Ghost fill in reverse: every interior cell takes the adjoint of the ghosts
+// that copied from it (NEWEST10/boundary.h, including the q7 mirror).
+__global__ void __launch_bounds__(BLK) k_adj_ghost(Geom g, Params p, double* __restrict__ lam)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  const long n = g.n, x = cidx(g, i, j, k);
+  for (int q = 0; q < 28; q++) {
+    const int f = c_bfields[q];
+    const bool mirror = (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && p.milne && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
+    double acc = lam[f * n + x];
+    for (int h = 0; h < VIRAL_BORDER; h++) {
+      if ((mirror ? g.il + VIRAL_BORDER - h - 1 : g.il) == i) acc += lam[f * n + cidx(g, h, j, k)];
+      if ((mirror ? g.ir - h - 1 : g.ir - 1) == i) acc += lam[f * n + cidx(g, g.ir + h, j, k)];
+      if ((mirror ? g.jl + VIRAL_BORDER - h - 1 : g.jl) == j) acc += lam[f * n + cidx(g, i, h, k)];
+      if ((mirror ? g.jr - h - 1 : g.jr - 1) == j) acc += lam[f * n + cidx(g, i, g.jr + h, k)];
+      if (!p.lbi) {
+        if (k == g.kl) acc += lam[f * n + cidx(g, i, j, h)];
+        if (k == g.kr - 1) acc += lam[f * n + cidx(g, i, j, g.kr + h)];
+      }
+    }
+    lam[f * n + x] = acc;
+  }
+}
+
+// Cells a stage never writes keep their input values: corner ghosts (all
+// fields; the fill covers only the x, y and eta slabs of interior rows) and
+// Temp in the ghost slabs.  Their adjoint passes straight through.
+__global__ void __launch_bounds__(BLK) k_adj_pass(Geom g, const double* __restrict__ lo, double* __restrict__ li)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n) return;
+  const int k = (int)(t % g.gz), j = (int)((t / g.gz) % g.gy), i = (int)(t / ((long)g.gz * g.gy));
+  const int out = (i < g.il || i >= g.ir) + (j < g.jl || j >= g.jr) + (k < g.kl || k >= g.kr);
+  const long n = g.n;
+  if (out == 1) {
+    li[VIRAL_F_TEMP * n + t] += lo[VIRAL_F_TEMP * n + t];
+  } else if (out > 1) {
+    for (int f = 0; f < NF; f++) li[f * n + t] += lo[f * n + t];
+  }
+}
+
+// k0 broadcast in reverse (NEWEST10/root.h:426-449): one thread per (i,j)
+// line collects what the other slices took from slice k0.
+__global__ void __launch_bounds__(BLK) k_adj_k0(Geom g, Params p, double tn, const double* __restrict__ so,
+                                                double* __restrict__ lam)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ni = g.ir - g.il, nj = g.jr - g.jl;
+  if (t >= (long)ni * nj) return;
+  const int i = g.il + (int)(t / nj), j = g.jl + (int)(t % nj);
+  const long n = g.n, x0 = cidx(g, i, j, g.k0);
+  const double v0[3] = {so[VIRAL_F_VX * n + x0], so[VIRAL_F_VY * n + x0], so[VIRAL_F_VE * n + x0]};
+  const double t2 = tn * tn;
+  double vb[3] = {0, 0, 0};
+  for (int k = g.kl; k < g.kr; k++) {
+    if (k == g.k0) continue;
+    const long x = cidx(g, i, j, k);
+    const double ub[4] = {lam[VIRAL_F_U0 * n + x], lam[VIRAL_F_U1 * n + x], lam[VIRAL_F_U2 * n + x],
+                          lam[VIRAL_F_U3 * n + x]};
+    for (int c = 0; c < 3; c++) vb[c] += lam[(VIRAL_F_VX + c) * n + x];
+    gamma_vjp(v0, t2, ub, vb);
+    for (int c = 0; c < 3; c++) lam[(VIRAL_F_VX + c) * n + x] = 0.0;
+    for (int c = 0; c < 4; c++) lam[(VIRAL_F_U0 + c) * n + x] = 0.0;
+  }
+  for (int c = 0; c < 3; c++) lam[(VIRAL_F_VX + c) * n + x0] += vb[c];
+}
+
+// Update, root and sources of one interior cell in reverse.  si / so: the
+// stage's input and output states; vb0..vb2: adjoints of the RK registers
+// Var0..Var2; resb: adjoint of Result (= of L), 15 rows; dubuf: adjoint of the
+// spatial velocity gradients du[l][1..3] (12 rows) for k_adj_du; partial:
+// per-block sums of the adjoints of eta_coef and tpi_coef.
+__global__ void __launch_bounds__(BLK) k_adj_update(Geom g, Params p, StageArgs a, const double* __restrict__ si,
+                                                    const double* __restrict__ so, const double* __restrict__ lo,
+                                                    double* __restrict__ li, double* __restrict__ vb0,
+                                                    double* __restrict__ vb1, double* __restrict__ vb2,
+                                                    double* __restrict__ resb, double* __restrict__ dubuf,
+                                                    double* __restrict__ partial)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  double ceb = 0, ctb = 0;
+  if (box_cell(interior_box(g), t, i, j, k)) {
+    const long n = g.n, x = cidx(g, i, j, k);
+    // prevu = stage 0 ? u : prevu
+    for (int l = 0; l < 4; l++) {
+      const double pb = lo[(VIRAL_F_PU0 + l) * n + x];
+      if (a.stage == 0) li[(VIRAL_F_U0 + l) * n + x] += pb;
+      else li[(VIRAL_F_PU0 + l) * n + x] += pb;
+    }
+    // velocities and root
+    double T0[4], pio[10], T0b[4], pibo[10], vbo[3], ubo[4];
+    for (int l = 0; l < 4; l++) {
+      T0[l] = so[(VIRAL_F_T00 + l) * n + x];
+      T0b[l] = lo[(VIRAL_F_T00 + l) * n + x];
+      ubo[l] = lo[(VIRAL_F_U0 + l) * n + x];
+    }
+    for (int l = 0; l < 10; l++) {
+      pio[l] = so[(VIRAL_F_PI0 + l) * n + x];
+      pibo[l] = lo[(VIRAL_F_PI0 + l) * n + x];
+    }
+    for (int c = 0; c < 3; c++) vbo[c] = lo[(VIRAL_F_VX + c) * n + x];
+    double Vb[SV];
+    update_vjp(p, a.tau_new, T0, pio, so[VIRAL_F_BULK * n + x], so[VIRAL_F_EN * n + x], lo[VIRAL_F_EN * n + x],
+               lo[VIRAL_F_P * n + x], lo[VIRAL_F_TEMP * n + x], vbo, ubo, T0b, pibo, lo[VIRAL_F_BULK * n + x], Vb);
+    // RK combination (NEWEST10/time.h:2-98) in reverse
+    double Lb[SV];
+    for (int l = 0; l < SV; l++) {
+      const long y = l * n + x;
+      if (a.kind == 0) {
+        const double Vt = Vb[l] + vb1[y];
+        const double v0b = Vt + vb0[y];
+        Lb[l] = a.ts * Vt;
+        const double sc = l < 4 ? a.tau_src : 1.0;
+        li[(VIRAL_F_T00 + l) * n + x] += sc * v0b;  // Var row l is state field l
+        vb0[y] = 0.0;
+        vb1[y] = 0.0;
+      } else if (a.kind == 1) {
+        Lb[l] = 0.5 * a.ts * Vb[l];
+        vb0[y] += 0.5 * Vb[l];
+        vb1[y] += 0.5 * Vb[l];
+      } else if (a.kind == 2) {
+        const double Vt = Vb[l] + vb2[y];
+        Lb[l] = 0.25 * a.ts * Vt;
+        vb0[y] += 0.75 * Vt;
+        vb1[y] += 0.25 * Vt;
+        vb2[y] = 0.0;
+      } else {
+        Lb[l] = (2.0 / 3.0) * a.ts * Vb[l];
+        vb0[y] += (1.0 / 3.0) * Vb[l];
+        vb2[y] += (2.0 / 3.0) * Vb[l];
+      }
+      resb[y] = Lb[l];
+    }
+    // sources (inputs from si)
+    const double e = si[VIRAL_F_EN * n + x], P = si[VIRAL_F_P * n + x], PI = si[VIRAL_F_BULK * n + x];
+    double u[4], pu[4], pi[10], du[4][4];
+    for (int l = 0; l < 4; l++) {
+      u[l] = si[(VIRAL_F_U0 + l) * n + x];
+      pu[l] = si[(VIRAL_F_PU0 + l) * n + x];
+    }
+    for (int l = 0; l < 10; l++) pi[l] = si[(VIRAL_F_PI0 + l) * n + x];
+    const double hx[3] = {p.dx, p.dy, p.deta};
+    const long sd[3] = {g.sx, g.sy, 1};
+    for (int l = 0; l < 4; l++) {
+      const double* ul = si + (VIRAL_F_U0 + l) * n;
+      du[l][0] = (u[l] - pu[l]) / a.ts_src;
+      for (int d = 0; d < 3; d++) {
+        const long s = sd[d];
+        du[l][d + 1] = (d == 2 && p.lbi) ? 0.0
+                                         : weno_der(ul[x - 2 * s], ul[x - s], ul[x], ul[x + s], ul[x + 2 * s], hx[d]);
+      }
+    }
+    double eb = 0, Pb = 0, PIb = 0, ub[4] = {0, 0, 0, 0}, pib[10] = {0}, dub[4][4];
+    for (int q = 0; q < 16; q++) dub[q / 4][q % 4] = 0;
+    source_vjp(p, a.tau_src, e, P, PI, u, pi, du, Lb, eb, Pb, PIb, ub, pib, dub, ceb, ctb);
+    li[VIRAL_F_EN * n + x] += eb;
+    li[VIRAL_F_P * n + x] += Pb;
+    li[VIRAL_F_BULK * n + x] += PIb;
+    for (int l = 0; l < 10; l++) li[(VIRAL_F_PI0 + l) * n + x] += pib[l];
+    for (int l = 0; l < 4; l++) {
+      li[(VIRAL_F_U0 + l) * n + x] += ub[l] + dub[l][0] / a.ts_src;
+      li[(VIRAL_F_PU0 + l) * n + x] -= dub[l][0] / a.ts_src;
+      for (int d = 0; d < 3; d++) dubuf[(l * 3 + d) * n + x] = (d == 2 && p.lbi) ? 0.0 : dub[l][d + 1];
+    }
+  }
+  // fixed-order block sums of the coefficient adjoints
+  __shared__ double red[2][BLK / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    ceb += __shfl_down(ceb, o);
+    ctb += __shfl_down(ctb, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = ceb;
+    red[1][threadIdx.x >> 6] = ctb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0, s1 = 0;
+    for (int w = 0; w < BLK / 64; w++) {
+      s0 += red[0][w];
+      s1 += red[1][w];
+    }
+    partial[2 * blockIdx.x] += s0;
+    partial[2 * blockIdx.x + 1] += s1;
+  }
+}
+
+// Spatial velocity gradients in reverse: each cell gathers from the interior
+// centres whose WENO-derivative stencil holds it (NEWEST10/Source.h:3-101).
+__global__ void __launch_bounds__(BLK) k_adj_du(Geom g, Params p, const double* __restrict__ si,
+                                                const double* __restrict__ dubuf, double* __restrict__ li)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n) return;
+  const int k = (int)(t % g.gz), j = (int)((t / g.gz) % g.gy), i = (int)(t / ((long)g.gz * g.gy));
+  const long n = g.n, x = t;
+  const bool iin = i >= g.il && i < g.ir, jin = j >= g.jl && j < g.jr, kin = k >= g.kl && k < g.kr;
+  const double hx[3] = {p.dx, p.dy, p.deta};
+  const long sd[3] = {g.sx, g.sy, 1};
+  double acc[4] = {0, 0, 0, 0};
+  for (int d = 0; d < 3; d++) {
+    if (d == 2 && p.lbi) continue;
+    if (d == 0 && !(jin && kin)) continue;
+    if (d == 1 && !(iin && kin)) continue;
+    if (d == 2 && !(iin && jin)) continue;
+    const int pos = d == 0 ? i : d == 1 ? j : k;
+    const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
+    const long s = sd[d];
+    for (int o = -2; o <= 2; o++) {
+      const int c = pos - o;  // centre whose stencil holds this cell at offset o
+      if (c < lo || c >= hi) continue;
+      const long xc = x - (long)o * s;
+      for (int l = 0; l < 4; l++) {
+        const double* ul = si + (VIRAL_F_U0 + l) * n;
+        const double q[5] = {ul[xc - 2 * s], ul[xc - s], ul[xc], ul[xc + s], ul[xc + 2 * s]};
+        double gL[5], gR[5];
+        weno_grad(q, false, gL);
+        weno_grad(q, true, gR);
+        acc[l] += dubuf[(l * 3 + d) * n + xc] * (gL[o + 2] - gR[o + 2]) / hx[d];
+      }
+    }
+  }
+  for (int l = 0; l < 4; l++) li[(VIRAL_F_U0 + l) * n + x] += acc[l];
+}
+
+// centred flux F_l, Var V_l (row l) and eigenvalue of cell y, direction d
+__device__ __forceinline__ void cell_FVA(const Geom& g, const Params& p, int d, double tau,
+                                         const double* __restrict__ st, long y, int l, double& F, double& V)
+{
+  const long n = g.n;
+  const int row = d + 1;
+  const double u0 = st[VIRAL_F_U0 * n + y], ud = st[(VIRAL_F_U1 + d) * n + y];
+  if (l < 4) {
+    const double e = st[VIRAL_F_EN * n + y], P = st[VIRAL_F_P * n + y], PI = st[VIRAL_F_BULK * n + y];
+    const double um = st[(VIRAL_F_U0 + l) * n + y];
+    const double piv = st[(VIRAL_F_PI0 + pidx(row, l)) * n + y];
+    const double pp = d == 2 ? (P - PI) / (tau * tau) : (P - PI);
+    const double v = l == row ? piv + pp + ud * um * (e + P - PI) : piv + ud * um * (e + P - PI);
+    F = tau * v;
+    V = tau * st[(long)l * n + y];
+  } else {
+    const double f = st[(long)l * n + y];  // pi[l-4] or PI: Var row l is state field l
+    F = (f * ud) / u0;
+    V = f;
+  }
+}
+
+__device__ __forceinline__ double cell_A(const Geom& g, const Params& p, int d, const double* __restrict__ st, long y,
+                                         bool live)
+{
+  return live ? st[(VIRAL_F_U1 + d) * g.n + y] / st[VIRAL_F_U0 * g.n + y] : 0.0;
+}
+
+// Face pass of the KT flux in reverse (NEWEST10/hydroKT.h:444-506): for face
+// c+1/2, fbar = d Result/d f from the adjacent cells' Result adjoints, the
+// face speed a = max(|A_L|, |A_R|) and its adjoint split onto A_L or A_R.
+// fbuf rows: 0..14 fbar, 15 a, 16 d/dA_L, 17 d/dA_R.
+__global__ void __launch_bounds__(BLK) k_adj_face(Geom g, Params p, int d, double tau, const double* __restrict__ si,
+                                                  const double* __restrict__ resb, double* __restrict__ fbuf)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(face_box(g, d), t, i, j, k)) return;
+  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+  const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
+  const double h = d == 0 ? p.dx : d == 1 ? p.dy : p.deta;
+  const int pos = d == 0 ? i : d == 1 ? j : k;
+  const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
+  const bool inc = pos >= lo && pos < hi, inc1 = pos + 1 >= lo && pos + 1 < hi;
+  double A[6];
+  for (int o = 0; o < 6; o++) {
+    const int q = pos + o - 2;
+    const int ii = d == 0 ? q : i, jj = d == 1 ? q : j, kk = d == 2 ? q : k;
+    A[o] = cell_A(g, p, d, si, x + (o - 2) * s, interior(g, ii, jj, kk) || !q1);
+  }
+  const double aL = weno_L(A[0], A[1], A[2], A[3], A[4]);
+  const double aR = weno_R(A[1], A[2], A[3], A[4], A[5]);
+  double ml = 0, mr = 0;
+  const bool live = !(q1 && pos < lo);
+  if (live) {
+    if (fabs(aL) > ml) ml = fabs(aL);
+    if (fabs(aR) > mr) mr = fabs(aR);
+  }
+  const double amax = fmax(ml, mr);
+  double abar = 0;
+  for (int l = 0; l < SV; l++) {
+    const double fb = (inc ? -resb[l * n + x] / h : 0.0) + (inc1 ? resb[l * n + x + s] / h : 0.0);
+    fbuf[l * n + x] = fb;
+    double Vs[6];
+    for (int o = 0; o < 6; o++) {
+      double F;
+      cell_FVA(g, p, d, tau, si, x + (o - 2) * s, l, F, Vs[o]);
+    }
+    const double VL = weno_L(Vs[0], Vs[1], Vs[2], Vs[3], Vs[4]);
+    const double VR = weno_R(Vs[1], Vs[2], Vs[3], Vs[4], Vs[5]);
+    abar += -0.5 * fb * (VR - VL);
+  }
+  double AbL = 0, AbR = 0;
+  if (live) {
+    if (ml >= mr) AbL = abar * (aL > 0 ? 1.0 : aL < 0 ? -1.0 : 0.0);
+    else AbR = abar * (aR > 0 ? 1.0 : aR < 0 ? -1.0 : 0.0);
+  }
+  fbuf[15 * n + x] = amax;
+  fbuf[16 * n + x] = AbL;
+  fbuf[17 * n + x] = AbR;
+}
+
+// Cell pass of the KT flux in reverse: each cell of a line gathers the WENO
+// adjoints of the (up to six) faces whose stencils hold it, then maps the
+// adjoints of its F_l, V_l and eigenvalue onto its state fields.
+__global__ void __launch_bounds__(BLK) k_adj_flux(Geom g, Params p, int d, double tau, const double* __restrict__ si,
+                                                  const double* __restrict__ fbuf, double* __restrict__ li)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // the lines of direction d: other two coordinates interior, all G cells along d
+  Box b{g.il, g.jl, g.kl, g.ir - g.il, g.jr - g.jl, g.kr - g.kl};
+  if (d == 0) { b.i0 = 0; b.ni = g.gx; }
+  if (d == 1) { b.j0 = 0; b.nj = g.gy; }
+  if (d == 2) { b.k0 = 0; b.nk = g.gz; }
+  int i, j, k;
+  if (!box_cell(b, t, i, j, k)) return;
+  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+  const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
+  const int pos = d == 0 ? i : d == 1 ? j : k;
+  const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
+  const int G = d == 0 ? g.gx : d == 1 ? g.gy : g.gz;
+  const int p0 = max(pos - 3, lo - 1), p1 = min(pos + 2, hi - 1);  // faces p+1/2 touching this cell
+  double Fb[SV], Vb[SV], Ab = 0;
+  for (int l = 0; l < SV; l++) Fb[l] = Vb[l] = 0;
+  for (int pf = p0; pf <= p1; pf++) {
+    const long xf = x + (long)(pf - pos) * s;
+    const double a = fbuf[15 * n + xf];
+    // eigenvalue stencil of face pf
+    {
+      double A[6];
+      for (int o = 0; o < 6; o++) {
+        const int q = pf + o - 2;
+        const int ii = d == 0 ? q : i, jj = d == 1 ? q : j, kk = d == 2 ? q : k;
+        A[o] = cell_A(g, p, d, si, xf + (o - 2) * s, interior(g, ii, jj, kk) || !q1);
+      }
+      double gL[5], gR[5];
+      weno_grad(A, false, gL);
+      weno_grad(A + 1, true, gR);
+      const int oL = pos - pf + 2, oR = pos - pf + 1;
+      if (oL >= 0 && oL < 5) Ab += fbuf[16 * n + xf] * gL[oL];
+      if (oR >= 0 && oR < 5) Ab += fbuf[17 * n + xf] * gR[oR];
+    }
+    for (int l = 0; l < SV; l++) {
+      const double fb = fbuf[l * n + xf];
+      if (fb == 0.0) continue;
+      double F[6], V[6];
+      for (int o = 0; o < 6; o++) cell_FVA(g, p, d, tau, si, xf + (o - 2) * s, l, F[o], V[o]);
+      double gFL[5], gFR[5], gVL[5], gVR[5];
+      weno_grad(F, false, gFL);
+      weno_grad(F + 1, true, gFR);
+      weno_grad(V, false, gVL);
+      weno_grad(V + 1, true, gVR);
+      const int oL = pos - pf + 2, oR = pos - pf + 1;
+      if (oL >= 0 && oL < 5) {
+        Fb[l] += 0.5 * fb * gFL[oL];
+        Vb[l] += 0.5 * a * fb * gVL[oL];
+      }
+      if (oR >= 0 && oR < 5) {
+        Fb[l] += 0.5 * fb * gFR[oR];
+        Vb[l] -= 0.5 * a * fb * gVR[oR];
+      }
+    }
+  }
+  (void)G;
+  const bool live = interior(g, i, j, k) || !q1;
+  double u[4], pi[10];
+  for (int l = 0; l < 4; l++) u[l] = si[(VIRAL_F_U0 + l) * n + x];
+  for (int l = 0; l < 10; l++) pi[l] = si[(VIRAL_F_PI0 + l) * n + x];
+  double eb = 0, Pb = 0, PIb = 0, ub[4] = {0, 0, 0, 0}, pib[10] = {0}, Tb[4] = {0, 0, 0, 0};
+  flux_cell_vjp(d, tau, si[VIRAL_F_EN * n + x], si[VIRAL_F_P * n + x], si[VIRAL_F_BULK * n + x], u, pi, Fb, Vb,
+                live ? Ab : 0.0, eb, Pb, PIb, ub, pib, Tb);
+  li[VIRAL_F_EN * n + x] += eb;
+  li[VIRAL_F_P * n + x] += Pb;
+  li[VIRAL_F_BULK * n + x] += PIb;
+  for (int l = 0; l < 4; l++) {
+    li[(VIRAL_F_U0 + l) * n + x] += ub[l];
+    li[(VIRAL_F_T00 + l) * n + x] += Tb[l];
+  }
+  for (int l = 0; l < 10; l++) li[(VIRAL_F_PI0 + l) * n + x] += pib[l];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -1129,6 +1524,14 @@ struct viral_ctx {
   bool timing = false;
   std::vector<hipEvent_t> tev[K_COUNT];  // start/stop pairs
   std::vector<hipEvent_t> pool;
+  // adjoint recording: the (tau, dtau) of every step since viral_adjoint_begin
+  // and device copies of the state every `every` steps
+  struct {
+    bool on = false;
+    int every = 0;
+    std::vector<double> taus, tss;
+    std::vector<double*> ckpt;
+  } adj;
 };
 
 static hipEvent_t pool_event(viral_ctx* c)
@@ -1720,6 +2123,7 @@ void viral_destroy(viral_ctx* ctx)
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   drop_graphs(ctx);
+  for (double* c : ctx->adj.ckpt) hipFree(c);
   for (Event& E : ctx->ev) {
     hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1);
     if (E.var2) hipFree(E.var2);
@@ -1806,6 +2210,7 @@ int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
   HIPCHK(hipMemcpyAsync(ctx->d_clk, ctx->h_clk, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_clear_bad, dim3((nev + 63) / 64), dim3(64), 0, ctx->stream, ctx->d_bad, nev);
   const bool graph = ctx->cfg.graph && !ctx->timing;
+  double trec = tau;
   for (int s = 0; s < nsteps; s++) {
     if (graph) {
       if (int r = capture_step(ctx, ctx->cur)) return r;
@@ -1814,6 +2219,19 @@ int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
       launch_step(ctx, ctx->cur);
     }
     ctx->cur ^= (ctx->rk & 1);
+    if (ctx->adj.on) {
+      ctx->adj.taus.push_back(trec);
+      ctx->adj.tss.push_back(ts);
+      if (ctx->adj.taus.size() % ctx->adj.every == 0) {
+        Event& E = ctx->ev[0];
+        const size_t bytes = (size_t)NF * E.g.n * sizeof(double);
+        double* c = nullptr;
+        HIPCHK(hipMalloc(&c, bytes));
+        ctx->adj.ckpt.push_back(c);
+        HIPCHK(hipMemcpyAsync(c, E.st[ctx->cur], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      }
+    }
+    trec = trec + ts;
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(ctx->h_bad, ctx->d_bad, nev * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -2110,6 +2528,164 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps)
   ctx->dirty = true;
   if (int r = refresh_table(ctx)) return r;
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+
+int viral_adjoint_end(viral_ctx* ctx)
+{
+  if (!ctx) return VIRAL_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (double* c : ctx->adj.ckpt) hipFree(c);
+  ctx->adj.ckpt.clear();
+  ctx->adj.taus.clear();
+  ctx->adj.tss.clear();
+  ctx->adj.on = false;
+  return VIRAL_OK;
+}
+
+int viral_adjoint_begin(viral_ctx* ctx, int every)
+{
+  if (!ctx || every < 1) return fail(ctx, VIRAL_ERR_ARG, "bad adjoint recording request");
+  if (ctx->ev.size() != 1) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint runs on single-event contexts");
+  const Event& E = ctx->ev[0];
+  if (E.d.coords != VIRAL_MILNE || E.d.scheme != VIRAL_KT || E.d.eos != VIRAL_EOS_CONFORMAL)
+    return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint covers the Milne KT step with the conformal EOS");
+  if ((E.d.quirks & VIRAL_Q_K0_BROADCAST) && !E.d.lbi && (E.g.k0 < E.g.kl || E.g.k0 >= E.g.kr))
+    return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint needs k0 inside the interior");
+  if (int r = viral_adjoint_end(ctx)) return r;
+  const size_t bytes = (size_t)NF * E.g.n * sizeof(double);
+  double* c = nullptr;
+  HIPCHK(hipMalloc(&c, bytes));
+  ctx->adj.ckpt.push_back(c);
+  HIPCHK(hipMemcpyAsync(c, E.st[ctx->cur], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->adj.every = every;
+  ctx->adj.on = true;
+  return VIRAL_OK;
+}
+
+int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad, double* pgrad)
+{
+  if (!ctx || !seed || !grad || !pgrad) return fail(ctx, VIRAL_ERR_ARG, "bad adjoint arguments");
+  if (!ctx->adj.on) return fail(ctx, VIRAL_ERR_ARG, "no recorded steps: call viral_adjoint_begin before stepping");
+  if (int r = refresh_table(ctx)) return r;
+  Event& E = ctx->ev[0];
+  const Geom& g = E.g;
+  const Params& p = E.p;
+  const size_t n = (size_t)g.n;
+  const size_t fbytes = (size_t)NF * n * sizeof(double);
+  const int M = (int)ctx->adj.taus.size();
+  const int every = ctx->adj.every;
+  const int rk = ctx->rk;
+  const long ni = box_count(interior_box(g));
+  const unsigned nbi = nblocks(ni);
+  std::vector<double*> bufs;
+  auto dalloc = [&](size_t count) -> double* {
+    double* q = nullptr;
+    if (hipMalloc(&q, count * sizeof(double)) != hipSuccess) return nullptr;
+    bufs.push_back(q);
+    return q;
+  };
+  auto release = [&]() {
+    for (double* q : bufs) hipFree(q);
+    bufs.clear();
+  };
+  const int nsnap = std::min(every, std::max(M, 1)) * rk + 1;
+  double* lamA = dalloc(NF * n);
+  double* lamB = dalloc(NF * n);
+  double* vb0 = dalloc(SV * n);
+  double* vb1 = dalloc(SV * n);
+  double* vb2 = dalloc(SV * n);
+  double* resb = dalloc(SV * n);
+  double* dubuf = dalloc(12 * n);
+  double* fbuf = dalloc(18 * n);
+  double* partial = dalloc(2 * (size_t)nbi);
+  double* saved = dalloc(NF * n);
+  double* snaps = dalloc((size_t)nsnap * NF * n);
+  if (!lamA || !lamB || !vb0 || !vb1 || !vb2 || !resb || !dubuf || !fbuf || !partial || !saved || !snaps) {
+    release();
+    return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the adjoint");
+  }
+  for (int f = 0; f < NF; f++) {
+    if (!seed[f] || !grad[f]) {
+      release();
+      return fail(ctx, VIRAL_ERR_ARG, "null field pointer");
+    }
+    HIPCHK(hipMemcpyAsync(lamA + f * n, seed[f], n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  HIPCHK(hipMemsetAsync(partial, 0, 2 * (size_t)nbi * sizeof(double), ctx->stream));
+  HIPCHK(hipMemcpyAsync(saved, E.st[ctx->cur], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  const bool k0b = (p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi;
+  const int ndir = p.lbi ? 2 : 3;
+  for (int seg = (M - 1) / every; seg >= 0 && M > 0; seg--) {
+    const int m0 = seg * every, m1 = std::min(m0 + every, M);
+    // recompute the stage states of the segment from its checkpoint
+    HIPCHK(hipMemcpyAsync(snaps, ctx->adj.ckpt[seg], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+    int idx = 0;
+    for (int m = m0; m < m1; m++) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      ctx->h_clk->tau = ctx->adj.taus[m];
+      ctx->h_clk->ts = ctx->adj.tss[m];
+      ctx->h_clk->step = 0;
+      HIPCHK(hipMemcpyAsync(ctx->d_clk, ctx->h_clk, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+      for (int st = 0; st < rk; st++, idx++) {
+        HIPCHK(hipMemcpyAsync(E.st[0], snaps + (size_t)idx * NF * n, fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+        launch_stage(ctx, st, 0);
+        HIPCHK(hipMemcpyAsync(snaps + (size_t)(idx + 1) * NF * n, E.st[1], fbytes, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+      }
+    }
+    // reverse sweep over the segment
+    for (int m = m1 - 1; m >= m0; m--) {
+      HIPCHK(hipMemsetAsync(vb0, 0, SV * n * sizeof(double), ctx->stream));
+      HIPCHK(hipMemsetAsync(vb1, 0, SV * n * sizeof(double), ctx->stream));
+      HIPCHK(hipMemsetAsync(vb2, 0, SV * n * sizeof(double), ctx->stream));
+      for (int st = rk - 1; st >= 0; st--) {
+        const size_t base = (size_t)((m - m0) * rk + st);
+        const double* si = snaps + base * NF * n;
+        const double* so = snaps + (base + 1) * NF * n;
+        const StageArgs a = stage_args(ctx->adj.taus[m], ctx->adj.tss[m], rk, st);
+        hipLaunchKernelGGL(k_adj_ghost, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, lamA);
+        if (k0b) {
+          const long nl = (long)(g.ir - g.il) * (g.jr - g.jl);
+          hipLaunchKernelGGL(k_adj_k0, dim3(nblocks(nl)), dim3(BLK), 0, ctx->stream, g, p, a.tau_new, so, lamA);
+        }
+        HIPCHK(hipMemsetAsync(lamB, 0, fbytes, ctx->stream));
+        hipLaunchKernelGGL(k_adj_pass, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, lamA, lamB);
+        hipLaunchKernelGGL(k_adj_update, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, so, lamA, lamB, vb0, vb1,
+                           vb2, resb, dubuf, partial);
+        hipLaunchKernelGGL(k_adj_du, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, p, si, dubuf, lamB);
+        for (int d = 0; d < ndir; d++) {
+          hipLaunchKernelGGL(k_adj_face, dim3(nblocks(box_count(face_box(g, d)))), dim3(BLK), 0, ctx->stream, g, p, d,
+                             a.tau_src, si, resb, fbuf);
+          Box b = interior_box(g);
+          long nline = (long)b.ni * b.nj * b.nk / (d == 0 ? b.ni : d == 1 ? b.nj : b.nk) *
+                       (d == 0 ? g.gx : d == 1 ? g.gy : g.gz);
+          hipLaunchKernelGGL(k_adj_flux, dim3(nblocks(nline)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si, fbuf,
+                             lamB);
+        }
+        std::swap(lamA, lamB);
+      }
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(E.st[ctx->cur], saved, fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(E.st[ctx->cur ^ 1], saved, fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  for (int f = 0; f < NF; f++)
+    HIPCHK(hipMemcpyAsync(grad[f], lamA + f * n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<double> part(2 * (size_t)nbi);
+  HIPCHK(hipMemcpyAsync(part.data(), partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double s0 = 0, s1 = 0;
+  for (unsigned b = 0; b < nbi; b++) {
+    s0 += part[2 * b];
+    s1 += part[2 * b + 1];
+  }
+  pgrad[0] = s0;
+  pgrad[1] = s1;
+  release();
   return VIRAL_OK;
 }
 
