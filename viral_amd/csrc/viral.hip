@@ -1265,17 +1265,43 @@ __global__ void __launch_bounds__(BLK) k_adj_update(Geom g, Params p, StageArgs 
   }
 }
 
-// Spatial velocity gradients in reverse: each cell gathers from the interior
-// centres whose WENO-derivative stencil holds it (NEWEST10/Source.h:3-101).
-__global__ void __launch_bounds__(BLK) k_adj_du(Geom g, Params p, const double* __restrict__ si,
-                                                const double* __restrict__ dubuf, double* __restrict__ li)
+// Spatial velocity gradients in reverse (NEWEST10/Source.h:3-101), in two
+// passes.  k_adj_du_src: every interior centre turns the adjoint of its WENO
+// derivatives du[l][d] into contributions to its five stencil cells,
+// cbuf row (l*3 + d)*5 + o for the cell at offset o-2.  k_adj_du: every cell
+// sums what the centres whose stencils hold it contributed.
+__global__ void __launch_bounds__(BLK) k_adj_du_src(Geom g, Params p, const double* __restrict__ si,
+                                                    const double* __restrict__ dubuf, double* __restrict__ cbuf)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  const long n = g.n, x = cidx(g, i, j, k);
+  const double hx[3] = {p.dx, p.dy, p.deta};
+  const long sd[3] = {g.sx, g.sy, 1};
+  for (int d = 0; d < 3; d++) {
+    if (d == 2 && p.lbi) continue;
+    const long s = sd[d];
+    for (int l = 0; l < 4; l++) {
+      const double* ul = si + (VIRAL_F_U0 + l) * n;
+      const double q[5] = {ul[x - 2 * s], ul[x - s], ul[x], ul[x + s], ul[x + 2 * s]};
+      double gL[5], gR[5];
+      weno_grad(q, false, gL);
+      weno_grad(q, true, gR);
+      const double w = dubuf[(l * 3 + d) * n + x] / hx[d];
+      for (int o = 0; o < 5; o++) cbuf[((long)(l * 3 + d) * 5 + o) * n + x] = w * (gL[o] - gR[o]);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_adj_du(Geom g, Params p, const double* __restrict__ cbuf,
+                                                double* __restrict__ li)
 {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.n) return;
   const int k = (int)(t % g.gz), j = (int)((t / g.gz) % g.gy), i = (int)(t / ((long)g.gz * g.gy));
   const long n = g.n, x = t;
   const bool iin = i >= g.il && i < g.ir, jin = j >= g.jl && j < g.jr, kin = k >= g.kl && k < g.kr;
-  const double hx[3] = {p.dx, p.dy, p.deta};
   const long sd[3] = {g.sx, g.sy, 1};
   double acc[4] = {0, 0, 0, 0};
   for (int d = 0; d < 3; d++) {
@@ -1290,14 +1316,7 @@ __global__ void __launch_bounds__(BLK) k_adj_du(Geom g, Params p, const double* 
       const int c = pos - o;  // centre whose stencil holds this cell at offset o
       if (c < lo || c >= hi) continue;
       const long xc = x - (long)o * s;
-      for (int l = 0; l < 4; l++) {
-        const double* ul = si + (VIRAL_F_U0 + l) * n;
-        const double q[5] = {ul[xc - 2 * s], ul[xc - s], ul[xc], ul[xc + s], ul[xc + 2 * s]};
-        double gL[5], gR[5];
-        weno_grad(q, false, gL);
-        weno_grad(q, true, gR);
-        acc[l] += dubuf[(l * 3 + d) * n + xc] * (gL[o + 2] - gR[o + 2]) / hx[d];
-      }
+      for (int l = 0; l < 4; l++) acc[l] += cbuf[((long)(l * 3 + d) * 5 + o + 2) * n + xc];
     }
   }
   for (int l = 0; l < 4; l++) li[(VIRAL_F_U0 + l) * n + x] += acc[l];
@@ -1331,12 +1350,18 @@ __device__ __forceinline__ double cell_A(const Geom& g, const Params& p, int d, 
   return live ? st[(VIRAL_F_U1 + d) * g.n + y] / st[VIRAL_F_U0 * g.n + y] : 0.0;
 }
 
-// Face pass of the KT flux in reverse (NEWEST10/hydroKT.h:444-506): for face
-// c+1/2, fbar = d Result/d f from the adjacent cells' Result adjoints, the
-// face speed a = max(|A_L|, |A_R|) and its adjoint split onto A_L or A_R.
-// fbuf rows: 0..14 fbar, 15 a, 16 d/dA_L, 17 d/dA_R.
+// Face pass of the KT flux in reverse (NEWEST10/hydroKT.h:444-506).  For face
+// c+1/2: fbar = d Result / d f from the adjacent cells' Result adjoints; the
+// face speed a = max(|A_L|, |A_R|) with its adjoint on A_L or A_R; then for
+// every row the WENO gradients of F_L, F_R, V_L, V_R turned into the
+// contributions to the six stencil cells c-2..c+3:
+//   cF[o] = fbar/2 (dF_L/dF_o + dF_R/dF_o),  cV[o] = a fbar/2 (dV_L/dV_o - dV_R/dV_o),
+//   cA[o] = Abar_L dA_L/dA_o + Abar_R dA_R/dA_o.
+// cbuf rows: (l*2 + 0|1)*6 + o for F|V of row l, 180 + o for A.
+constexpr int ADJ_CROWS = 186;
+
 __global__ void __launch_bounds__(BLK) k_adj_face(Geom g, Params p, int d, double tau, const double* __restrict__ si,
-                                                  const double* __restrict__ resb, double* __restrict__ fbuf)
+                                                  const double* __restrict__ resb, double* __restrict__ cbuf)
 {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
@@ -1362,34 +1387,59 @@ __global__ void __launch_bounds__(BLK) k_adj_face(Geom g, Params p, int d, doubl
     if (fabs(aR) > mr) mr = fabs(aR);
   }
   const double amax = fmax(ml, mr);
+  // pass 1: the adjoint of the face speed needs V_R - V_L of every row
   double abar = 0;
   for (int l = 0; l < SV; l++) {
     const double fb = (inc ? -resb[l * n + x] / h : 0.0) + (inc1 ? resb[l * n + x + s] / h : 0.0);
-    fbuf[l * n + x] = fb;
+    if (fb == 0.0) continue;
     double Vs[6];
     for (int o = 0; o < 6; o++) {
       double F;
       cell_FVA(g, p, d, tau, si, x + (o - 2) * s, l, F, Vs[o]);
     }
-    const double VL = weno_L(Vs[0], Vs[1], Vs[2], Vs[3], Vs[4]);
-    const double VR = weno_R(Vs[1], Vs[2], Vs[3], Vs[4], Vs[5]);
-    abar += -0.5 * fb * (VR - VL);
+    abar += -0.5 * fb * (weno_R(Vs[1], Vs[2], Vs[3], Vs[4], Vs[5]) - weno_L(Vs[0], Vs[1], Vs[2], Vs[3], Vs[4]));
   }
   double AbL = 0, AbR = 0;
   if (live) {
     if (ml >= mr) AbL = abar * (aL > 0 ? 1.0 : aL < 0 ? -1.0 : 0.0);
     else AbR = abar * (aR > 0 ? 1.0 : aR < 0 ? -1.0 : 0.0);
   }
-  fbuf[15 * n + x] = amax;
-  fbuf[16 * n + x] = AbL;
-  fbuf[17 * n + x] = AbR;
+  {
+    double gL[5], gR[5];
+    weno_grad(A, false, gL);
+    weno_grad(A + 1, true, gR);
+    for (int o = 0; o < 6; o++)
+      cbuf[(long)(180 + o) * n + x] = (o < 5 ? AbL * gL[o] : 0.0) + (o > 0 ? AbR * gR[o - 1] : 0.0);
+  }
+  // pass 2: stencil-cell contributions of every row
+  for (int l = 0; l < SV; l++) {
+    const double fb = (inc ? -resb[l * n + x] / h : 0.0) + (inc1 ? resb[l * n + x + s] / h : 0.0);
+    double* cF = cbuf + (long)(l * 2) * 6 * n + x;
+    double* cV = cbuf + (long)(l * 2 + 1) * 6 * n + x;
+    if (fb == 0.0) {
+      for (int o = 0; o < 6; o++) cF[o * n] = cV[o * n] = 0.0;
+      continue;
+    }
+    double F[6], V[6];
+    for (int o = 0; o < 6; o++) cell_FVA(g, p, d, tau, si, x + (o - 2) * s, l, F[o], V[o]);
+    double gFL[5], gFR[5], gVL[5], gVR[5];
+    weno_grad(F, false, gFL);
+    weno_grad(F + 1, true, gFR);
+    weno_grad(V, false, gVL);
+    weno_grad(V + 1, true, gVR);
+    const double wf = 0.5 * fb, wv = 0.5 * amax * fb;
+    for (int o = 0; o < 6; o++) {
+      cF[o * n] = wf * ((o < 5 ? gFL[o] : 0.0) + (o > 0 ? gFR[o - 1] : 0.0));
+      cV[o * n] = wv * ((o < 5 ? gVL[o] : 0.0) - (o > 0 ? gVR[o - 1] : 0.0));
+    }
+  }
 }
 
-// Cell pass of the KT flux in reverse: each cell of a line gathers the WENO
-// adjoints of the (up to six) faces whose stencils hold it, then maps the
-// adjoints of its F_l, V_l and eigenvalue onto its state fields.
+// Cell pass of the KT flux in reverse: each cell of a line sums what the (up
+// to six) faces whose stencils hold it contributed, then maps the adjoints of
+// its F_l, V_l and eigenvalue onto its state fields.
 __global__ void __launch_bounds__(BLK) k_adj_flux(Geom g, Params p, int d, double tau, const double* __restrict__ si,
-                                                  const double* __restrict__ fbuf, double* __restrict__ li)
+                                                  const double* __restrict__ cbuf, double* __restrict__ li)
 {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   // the lines of direction d: other two coordinates interior, all G cells along d
@@ -1403,50 +1453,18 @@ __global__ void __launch_bounds__(BLK) k_adj_flux(Geom g, Params p, int d, doubl
   const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
   const int pos = d == 0 ? i : d == 1 ? j : k;
   const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
-  const int G = d == 0 ? g.gx : d == 1 ? g.gy : g.gz;
   const int p0 = max(pos - 3, lo - 1), p1 = min(pos + 2, hi - 1);  // faces p+1/2 touching this cell
   double Fb[SV], Vb[SV], Ab = 0;
   for (int l = 0; l < SV; l++) Fb[l] = Vb[l] = 0;
   for (int pf = p0; pf <= p1; pf++) {
     const long xf = x + (long)(pf - pos) * s;
-    const double a = fbuf[15 * n + xf];
-    // eigenvalue stencil of face pf
-    {
-      double A[6];
-      for (int o = 0; o < 6; o++) {
-        const int q = pf + o - 2;
-        const int ii = d == 0 ? q : i, jj = d == 1 ? q : j, kk = d == 2 ? q : k;
-        A[o] = cell_A(g, p, d, si, xf + (o - 2) * s, interior(g, ii, jj, kk) || !q1);
-      }
-      double gL[5], gR[5];
-      weno_grad(A, false, gL);
-      weno_grad(A + 1, true, gR);
-      const int oL = pos - pf + 2, oR = pos - pf + 1;
-      if (oL >= 0 && oL < 5) Ab += fbuf[16 * n + xf] * gL[oL];
-      if (oR >= 0 && oR < 5) Ab += fbuf[17 * n + xf] * gR[oR];
-    }
+    const int o = pos - pf + 2;  // this cell's slot in the face's stencil c-2..c+3
+    Ab += cbuf[(long)(180 + o) * n + xf];
     for (int l = 0; l < SV; l++) {
-      const double fb = fbuf[l * n + xf];
-      if (fb == 0.0) continue;
-      double F[6], V[6];
-      for (int o = 0; o < 6; o++) cell_FVA(g, p, d, tau, si, xf + (o - 2) * s, l, F[o], V[o]);
-      double gFL[5], gFR[5], gVL[5], gVR[5];
-      weno_grad(F, false, gFL);
-      weno_grad(F + 1, true, gFR);
-      weno_grad(V, false, gVL);
-      weno_grad(V + 1, true, gVR);
-      const int oL = pos - pf + 2, oR = pos - pf + 1;
-      if (oL >= 0 && oL < 5) {
-        Fb[l] += 0.5 * fb * gFL[oL];
-        Vb[l] += 0.5 * a * fb * gVL[oL];
-      }
-      if (oR >= 0 && oR < 5) {
-        Fb[l] += 0.5 * fb * gFR[oR];
-        Vb[l] -= 0.5 * a * fb * gVR[oR];
-      }
+      Fb[l] += cbuf[((long)(l * 2) * 6 + o) * n + xf];
+      Vb[l] += cbuf[((long)(l * 2 + 1) * 6 + o) * n + xf];
     }
   }
-  (void)G;
   const bool live = interior(g, i, j, k) || !q1;
   double u[4], pi[10];
   for (int l = 0; l < 4; l++) u[l] = si[(VIRAL_F_U0 + l) * n + x];
@@ -2600,7 +2618,7 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   double* vb2 = dalloc(SV * n);
   double* resb = dalloc(SV * n);
   double* dubuf = dalloc(12 * n);
-  double* fbuf = dalloc(18 * n);
+  double* fbuf = dalloc((size_t)ADJ_CROWS * n);
   double* partial = dalloc(2 * (size_t)nbi);
   double* saved = dalloc(NF * n);
   double* snaps = dalloc((size_t)nsnap * NF * n);
@@ -2656,7 +2674,8 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         hipLaunchKernelGGL(k_adj_pass, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, lamA, lamB);
         hipLaunchKernelGGL(k_adj_update, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, so, lamA, lamB, vb0, vb1,
                            vb2, resb, dubuf, partial);
-        hipLaunchKernelGGL(k_adj_du, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, p, si, dubuf, lamB);
+        hipLaunchKernelGGL(k_adj_du_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, si, dubuf, fbuf);
+        hipLaunchKernelGGL(k_adj_du, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, p, fbuf, lamB);
         for (int d = 0; d < ndir; d++) {
           hipLaunchKernelGGL(k_adj_face, dim3(nblocks(box_count(face_box(g, d)))), dim3(BLK), 0, ctx->stream, g, p, d,
                              a.tau_src, si, resb, fbuf);
