@@ -75,6 +75,25 @@ def kernel_bytes(name, d):
     return 0.0
 
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+# on the 200x200x64 RK3 fast-mode workload (tabulated s95p), each counter
+# scaled by what it reads for a 1 GiB device copy (read x2.00, write x1.00);
+# profiles/counters_kt_s95p_glauber_200x200x64_rk3.md lists the passes.
+MEASURED_TRAFFIC = {
+    "update": 3.701e9,
+    "flux_x": 1.458e9,
+    "flux_y": 1.457e9,
+    "flux_eta": 1.371e9,
+    "source": 1.273e9,
+}
+
+
+def measured_traffic(name, d):
+    if (d.nx, d.ny, d.nz, d.rk, d.fast_math, d.eos_table) == (200, 200, 64, 3, 1, 1):
+        return MEASURED_TRAFFIC.get(name)
+    return None
+
+
 def cpu_baseline(seconds_target=20.0):
     """The CPU oracle (a one-thread restatement of the reference step) on a
     bounded sample of the same workload: the central 16-slice eta slab of the
@@ -307,7 +326,7 @@ def main():
                    "flux_tiles": tiles, "launch": launch,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, d),
                      "avg_launch_ms": avg_s * 1000.0, "launches": launches,
                      "algorithmic_bytes_per_launch": bytes_launch},
         "step_hbm": {"algorithmic_bytes_per_cell_update": bytes_per_update,
