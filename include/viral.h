@@ -173,9 +173,58 @@ int viral_iterations(viral_ctx* ctx, int ev, int stage, int32_t* out);
  * that stage. */
 int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* stage, unsigned* flags);
 
-/* Device reductions the reference's driver runs every step.
- * MaxTempGev (NEWEST10/utils.h:263-283): max over interior of GEVFM*T(e). */
+/* Device reductions the reference's driver runs every step.  Each reads the
+ * event's current state on the device (no state download).  Cells are
+ * reported as ghosted (i, j, k) triples, -1 -1 -1 where there is none.
+ * Sums are combined in a fixed order, so repeated calls give equal results.
+ *
+ * MaxTempGev (NEWEST10/utils.h:263-283, called at main.cpp:163): max over
+ * the interior of GEVFM*T(e). */
 int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out);
+/* CheckRoot (NEWEST10/root.h:1-199, called at main.cpp:158 with tau after
+ * the step; Cartesian TXYZ5/multiroot.h:7-150).  Milne: 6 statistics,
+ *   0 |e - (T^tt - pi^tt) + M^2 / ((T^tt - pi^tt) + P(e) - PI)|,
+ *   1-4 |pi^{mu nu} u_nu| for mu = t, x, y, eta, 5 |pi^mu_mu|;
+ * Cartesian: 9, 0-3 the residuals of the four conservation equations at
+ * (e, u), 4-7 transversality, 8 the trace as the reference keeps it (its
+ * `fabs(trace) > trace` test: |trace| of the last cell in loop order with
+ * a negative trace, 0 if none).  max[s]: maximum over the interior, with
+ * ijk[3s..3s+2] the first cell in loop order reaching it; avg[s]: sum over
+ * the interior / number of interior cells (the reference's Avg for one
+ * rank).  Arrays hold 9 entries (27 for ijk, which may be NULL); *nstat
+ * receives the count.  P(e) is the closed-form EOS. */
+int viral_check_root(viral_ctx* ctx, int ev, double tau, double* max, double* avg, int* ijk, int* nstat);
+/* DebugMSG (NEWEST10/debug.h:3-100, called at the end of every step,
+ * time.h:334,391,477; Cartesian TXYZ5/debug.h with |v| free of tau):
+ * out[0] the largest |v| = sqrt(vx^2 + vy^2 + tau^2 ve^2) (the reference's
+ * BMax; the last cell in loop order reaching it), out[1..3] that cell's
+ * (vx, vy, ve), out[4] the largest e (first cell), out[5] the smallest e
+ * below 0 (first cell; 0 when none).  ijk: 9 ints (vmax, emax, emin cells),
+ * may be NULL. */
+int viral_debug_extrema(viral_ctx* ctx, int ev, double tau, double* out, int* ijk);
+/* CheckPhysics (NEWEST10/utils.h:231-258, called at the start of every step,
+ * time.h:303,341,399): count[0] interior cells with |v| >= 1, count[1] with
+ * e <= 0; ijk: the first of each in loop order (6 ints, may be NULL). */
+int viral_check_physics(viral_ctx* ctx, int ev, double tau, long long* count, int* ijk);
+
+/* Output records of NEWEST10/write.h (WriteResults :286-365, WriteResultsXY
+ * :52-119): 17 doubles per output cell, {e, T, P, vx, vy, ve, pi[0..9], PI}
+ * (write.h:87-103), for every freq-th cell in x and y and every freqz-th in
+ * eta, in (i, j, k) order -- exactly the bytes of the reference's
+ * res/tau<tau>fm.bin for one rank.  xy (or an LBI grid): the k0 slice only.
+ * The strides must divide the grid (VIRAL_ERR_ARG otherwise).
+ * viral_output_size gives the count of doubles.  viral_output packs the
+ * records on the device and copies them to out; with async != 0 it returns
+ * once the copy is queued on the context's stream (out must then be pinned
+ * memory from viral_host_alloc, and stay untouched until viral_sync), so the
+ * caller can write the previous file while the next step runs. */
+int viral_output_size(const viral_desc* d, int freq, int freqz, int xy, size_t* ndoubles);
+int viral_output(viral_ctx* ctx, int ev, int freq, int freqz, int xy, double* out, int async);
+/* Waits for the context's queued work (steps, async output copies). */
+int viral_sync(viral_ctx* ctx);
+/* Pinned host memory for asynchronous output copies. */
+int viral_host_alloc(size_t bytes, void** p);
+int viral_host_free(void* p);
 
 /* Host evaluation of the equation of state a descriptor selects: pressure,
  * dp/de, temperature and entropy density at n energy densities (1/fm^4).

@@ -1320,3 +1320,183 @@ int or_step(or_ctx* c, double tau, double ts)
   stage_root(c, tau + ts, 2);
   return c->bad_cell >= 0 ? VIRAL_ERR_PHYSICS : VIRAL_OK;
 }
+
+/* ------------------------------------------------------------------ per-step diagnostics */
+
+/* Running maximum of the reference's `if (x > m) m = x` from m = 0: the
+ * first cell in loop order that reaches the maximum keeps its index. */
+static inline void keep_first_max(double x, size_t at, double* m, long long* where)
+{
+  if (x > *m) { *m = x; *where = (long long)at; }
+}
+
+/* Root check after a step.
+ * Milne, NEWEST10/root.h:1-199 -- six statistics:
+ *   0  |e - (T^tt - pi^tt) + M^2 / ((T^tt - pi^tt) + P(e) - PI)|, M^2 as in
+ *      the root solve (root.h:20-44)
+ *   1-4  |pi^{mu nu} u_nu| for mu = t, x, y, eta (transversality, root.h:
+ *        72-75), 5  |pi^mu_mu| (root.h:90)
+ * Cartesian, TXYZ5/multiroot.h:7-150 -- nine statistics: 0-3 the residuals
+ * of the four conservation equations at (e, u) (multiroot.h:45-48, with
+ * pow(x, 0.5) as sqrt and pow(u, 2) as u*u), 4-7 transversality, 8 the
+ * trace, whose maximum follows the reference's `fabs(trace) > trace` test
+ * (multiroot.h:109): the |trace| of the last cell in loop order with a
+ * negative trace, 0 if none.
+ * mx: maxima (at: first cell in loop order reaching each, -1 when it is 0;
+ * for Cartesian statistic 8 the cell the value came from); avg: sum of the
+ * absolute values over the interior / number of interior cells, summed in
+ * loop order.  Returns the number of statistics. */
+int or_check_root(or_ctx* c, double tau, double* mx, double* avg, long long* at)
+{
+  const viral_desc* d = &c->d;
+  double** f = c->f;
+  const int ns = d->coords == VIRAL_MILNE ? 6 : 9;
+  double sum[9];
+  long count = 0;
+  for (int s = 0; s < ns; s++) { mx[s] = 0; sum[s] = 0; at[s] = -1; }
+  FORI {
+    size_t x = IX(c, i, j, k);
+    count++;
+    double e = f[VIRAL_F_EN][x], PI = f[VIRAL_F_BULK][x], P = or_eos_p(d, e);
+    double W = f[VIRAL_F_T00][x], X = f[VIRAL_F_T00 + 1][x], Y = f[VIRAL_F_T00 + 2][x], Z = f[VIRAL_F_T00 + 3][x];
+    if (d->coords == VIRAL_MILNE) {
+      double Ttt = W, Ttx = X, Tty = Y, Tte = Z;
+      double pi1 = f[VIRAL_F_PI0][x], pi2 = f[VIRAL_F_PI0 + 1][x], pi3 = f[VIRAL_F_PI0 + 2][x],
+             pi4 = f[VIRAL_F_PI0 + 3][x];
+      double M2 = (Ttx - pi2) * (Ttx - pi2) + (Tty - pi3) * (Tty - pi3) + tau * tau * (Tte - pi4) * (Tte - pi4);
+      double res = e - (Ttt - pi1) + (M2 / ((Ttt - pi1) + P - PI));
+      keep_first_max(fabs(res), x, &mx[0], &at[0]);
+      sum[0] += fabs(res);
+    } else {
+      double u1 = f[VIRAL_F_U0 + 1][x], u2 = f[VIRAL_F_U0 + 2][x], u3 = f[VIRAL_F_U0 + 3][x];
+      double p1 = f[VIRAL_F_PI0][x], p2 = f[VIRAL_F_PI0 + 1][x], p3 = f[VIRAL_F_PI0 + 2][x],
+             p4 = f[VIRAL_F_PI0 + 3][x], p5 = f[VIRAL_F_PI0 + 4][x];
+      double Q = 1 + u1 * u1 + u2 * u2 + u3 * u3, S3 = u1 * u1 + u2 * u2 + u3 * u3, rq = sqrt(Q);
+      double y[4];
+      y[0] = -(u1 * X) - u2 * Y - u3 * Z + (-e + W) * rq;
+      y[1] = -(p3 * u2) - p4 * u3 - P * u1 * Q - u1 * (p1 + e * Q - PI * Q) + X * rq;
+      y[2] = -(p3 * u1) - (e + p2 - PI) * u2 - p5 * u3 - (e - PI) * u2 * S3 - P * u2 * Q + Y * rq;
+      y[3] = -(p4 * u1) - p5 * u2 - P * u3 + u3 * (p1 + p2 + PI - W) + Z * rq;
+      for (int l = 0; l < 4; l++) {
+        keep_first_max(fabs(y[l]), x, &mx[l], &at[l]);
+        sum[l] += fabs(y[l]);
+      }
+    }
+  }
+  const int r0 = d->coords == VIRAL_MILNE ? 1 : 4;
+  FORI {
+    size_t x = IX(c, i, j, k);
+    double u[4], p[10];
+    for (int m = 0; m < 4; m++) u[m] = f[VIRAL_F_U0 + m][x];
+    if (d->coords == VIRAL_MILNE)
+      for (int l = 0; l < 10; l++) p[l] = f[VIRAL_F_PI0 + l][x];
+    else
+      cell_pi10(c, x, u, p);
+    double u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
+    double rows[4], trace;
+    if (d->coords == VIRAL_MILNE) {
+      rows[0] = p[0] * u0 - p[1] * u1 - p[2] * u2 - p[3] * u3 * tau * tau;
+      rows[1] = p[1] * u0 - p[4] * u1 - p[5] * u2 - p[6] * u3 * tau * tau;
+      rows[2] = p[2] * u0 - p[5] * u1 - p[7] * u2 - p[8] * u3 * tau * tau;
+      rows[3] = p[3] * u0 - p[6] * u1 - p[8] * u2 - p[9] * u3 * tau * tau;
+      trace = p[0] - p[4] - p[7] - p[9] * tau * tau;
+    } else {
+      rows[0] = p[0] * u0 - p[1] * u1 - p[2] * u2 - p[3] * u3;
+      rows[1] = p[1] * u0 - p[4] * u1 - p[5] * u2 - p[6] * u3;
+      rows[2] = p[2] * u0 - p[5] * u1 - p[7] * u2 - p[8] * u3;
+      rows[3] = p[3] * u0 - p[6] * u1 - p[8] * u2 - p[9] * u3;
+      trace = p[0] - p[4] - p[7] - p[9];
+    }
+    for (int l = 0; l < 4; l++) {
+      keep_first_max(fabs(rows[l]), x, &mx[r0 + l], &at[r0 + l]);
+      sum[r0 + l] += fabs(rows[l]);
+    }
+    if (d->coords == VIRAL_MILNE) {
+      keep_first_max(fabs(trace), x, &mx[5], &at[5]);
+    } else if (fabs(trace) > trace) {
+      mx[8] = fabs(trace);
+      at[8] = (long long)x;
+    }
+    sum[r0 + 4] += fabs(trace);
+  }
+  for (int s = 0; s < ns; s++) avg[s] = sum[s] / count;
+  return ns;
+}
+
+/* |v| of a cell: sqrt(vx^2 + vy^2 + tau^2 ve^2) (NEWEST10/debug.h:35,
+ * utils.h:242); without tau in Cartesian coordinates (TXYZ5/debug.h:54) */
+static inline double speed(const or_ctx* c, size_t x, double tau)
+{
+  double vx = c->f[VIRAL_F_VX][x], vy = c->f[VIRAL_F_VX + 1][x], ve = c->f[VIRAL_F_VX + 2][x];
+  if (c->d.coords == VIRAL_MILNE) return sqrt(vx * vx + vy * vy + tau * tau * ve * ve);
+  return sqrt(vx * vx + vy * vy + ve * ve);
+}
+
+/* End-of-step extrema, NEWEST10/debug.h:3-100 (TXYZ5/debug.h): the largest
+ * |v| (`b >= vmax` from 0: the last cell in loop order reaching it), the
+ * largest e (`>` from 0: the first cell) and the smallest e below 0 (`<`
+ * from 0: the first cell; 0 and no cell when none is negative).
+ * out = {vmax, vx, vy, ve of that cell, emax, emin}; at = {vmax cell, emax
+ * cell, emin cell}, ghosted linear indices, -1 where none.  vmax is the
+ * reference's BMax. */
+void or_debug_msg(or_ctx* c, double tau, double out[6], long long at[3])
+{
+  double vmax = 0, emax = 0, emin = 0;
+  at[0] = at[1] = at[2] = -1;
+  FORI {
+    size_t x = IX(c, i, j, k);
+    double b = speed(c, x, tau), e = c->f[VIRAL_F_EN][x];
+    if (e > emax) { emax = e; at[1] = (long long)x; }
+    if (e < emin) { emin = e; at[2] = (long long)x; }
+    if (b >= vmax) { vmax = b; at[0] = (long long)x; }
+  }
+  out[0] = vmax;
+  for (int m = 0; m < 3; m++) out[1 + m] = at[0] >= 0 ? c->f[VIRAL_F_VX + m][at[0]] : 0.0;
+  out[4] = emax;
+  out[5] = emin;
+}
+
+/* Start-of-step physics check, NEWEST10/utils.h:231-258 (TXYZ5/utils.h:
+ * 223-248): interior cells with |v| >= 1 (n[0]) and with e <= 0 (n[1]), and
+ * the first of each in loop order (-1: none). */
+void or_check_physics(or_ctx* c, double tau, long long n[2], long long first[2])
+{
+  n[0] = n[1] = 0;
+  first[0] = first[1] = -1;
+  FORI {
+    size_t x = IX(c, i, j, k);
+    if (speed(c, x, tau) >= 1) { if (!n[0]++) first[0] = (long long)x; }
+    if (c->f[VIRAL_F_EN][x] <= 0) { if (!n[1]++) first[1] = (long long)x; }
+  }
+}
+
+/* Output records of NEWEST10/write.h: PRINTVAR = 17 doubles per cell,
+ * {e, T, P, vx, vy, ve, pi[0..9], PI} (write.h:1,87-103).  Full grid
+ * (WriteResults, write.h:286-365): every freq-th cell in x and y and every
+ * freqz-th in eta, in (i, j, k) order, which is the order of the file's
+ * records (OFFSET, write.h:267-284, when freq divides nx and ny and freqz
+ * divides nz).  xy != 0 (WriteResultsXY, write.h:52-119): the k0 slice in
+ * (i, j) order.  Writes the records to out (may be NULL) and returns their
+ * doubles count; 0 when the strides do not divide the grid. */
+size_t or_output(or_ctx* c, int freq, int freqz, int xy, double* out)
+{
+  const viral_desc* d = &c->d;
+  if (freq < 1 || freqz < 1 || d->nx % freq || d->ny % freq || (!xy && d->nz % freqz)) return 0;
+  size_t q = 0;
+  for (int i = c->il; i < c->ir; i += freq)
+    for (int j = c->jl; j < c->jr; j += freq)
+      for (int k = xy ? c->k0 : c->kl; xy ? k == c->k0 : k < c->kr; k += xy ? 1 : freqz) {
+        size_t x = IX(c, i, j, k);
+        if (out) {
+          double* r = out + q;
+          r[0] = c->f[VIRAL_F_EN][x];
+          r[1] = c->f[VIRAL_F_TEMP][x];
+          r[2] = c->f[VIRAL_F_P][x];
+          for (int m = 0; m < 3; m++) r[3 + m] = c->f[VIRAL_F_VX + m][x];
+          for (int l = 0; l < 10; l++) r[6 + l] = c->f[VIRAL_F_PI0 + l][x];
+          r[16] = c->f[VIRAL_F_BULK][x];
+        }
+        q += 17;
+      }
+  return q;
+}

@@ -80,6 +80,20 @@ double or_weno_L(double qm2, double qm1, double qc, double qp1, double qp2);
 double or_weno_R(double qm2, double qm1, double qc, double qp1, double qp2);
 double or_weno_der(double qm2, double qm1, double qc, double qp1, double qp2, double dx);
 
+/* Per-step diagnostics of the driver loop.  Cell positions are ghosted
+ * linear indices ((i*gy + j)*gz + k), -1 where there is none.
+ * or_check_root: CheckRoot (NEWEST10/root.h:1-199: 6 statistics; TXYZ5/
+ * multiroot.h:7-150: 9); returns the count.  or_debug_msg: DebugMSG extrema
+ * (NEWEST10/debug.h:3-100), out = {vmax, vx, vy, ve, emax, emin}, at =
+ * {vmax, emax, emin cells}.  or_check_physics: CheckPhysics (utils.h:
+ * 231-258), counts and first cells of |v| >= 1 and e <= 0. */
+int or_check_root(or_ctx* c, double tau, double* mx, double* avg, long long* at);
+void or_debug_msg(or_ctx* c, double tau, double out[6], long long at[3]);
+void or_check_physics(or_ctx* c, double tau, long long n[2], long long first[2]);
+/* write.h output records (17 doubles per output cell); returns the count of
+ * doubles, 0 when the strides do not divide the grid; out may be NULL */
+size_t or_output(or_ctx* c, int freq, int freqz, int xy, double* out);
+
 /* deterministic math (oracle/vmath.h) exported for tests */
 double or_vm_exp(double x);
 double or_vm_log(double x);

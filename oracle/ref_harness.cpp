@@ -21,6 +21,15 @@
  * Usage: ref_<profile> <out.bin> <seed> <tau> <ts> <mode>
  *   mode 0: source + flux + boundary on a random state
  *   mode 1: s95p samples
+ *   mode 2: the driver-loop diagnostics and output on a random state with a
+ *           few planted cells (|v| >= 1, e <= 0, and a repeated largest e and
+ *           |v|): DebugMSG (debug.h:3) and CheckPhysics (utils.h:231) print
+ *           to stdout, BMax and MaxTempGev (utils.h:263) go to the file, and
+ *           WriteResults / WriteResultsXY (write.h:52,286) write
+ *           res/tau<tau>fm.bin under the working directory.
+ *   mode 3: the initial state of the profile's own init (initvar,
+ *           init.h:150-214, for the default profile), after the cell
+ *           positions init() sets (init.h:558-566).
  */
 #include <mpi.h>
 #include <iostream>
@@ -100,6 +109,9 @@ int main(int argc, char** argv)
   MPI_Init(&argc, &argv);
   rank = 0;
   size = 1;
+  /* init() makes mpi_grid a Cartesian communicator over the ranks
+   * (NEWEST10/init.h:121-144); with one process that is the world */
+  mpi_grid = MPI_COMM_WORLD;
   /* boundary() asks MYXLEFT/MYXRIGHT (NEWEST10/init.h:98-103), which are
    * BOUNDARY on both sides only for a one-rank profile (NPX == NPY == 1);
    * other profiles skip the boundary call below. */
@@ -150,7 +162,41 @@ int main(int argc, char** argv)
         c.T30 = (e + c.P) * c.u[0] * u3 + c.pi[3];
         c.Vx = u1 / c.u[0]; c.Vy = u2 / c.u[0]; c.Ve = u3 / c.u[0];
         c.Temp = FT(e);
+        /* cell positions as init() sets them (NEWEST10/init.h:44-81) */
+        c.X = XCORDWB(i);
+        c.Y = YCORDWB(j);
+        c.eta = ZCORDWB(k);
       }
+  if (mode == 3) {
+#if !defined(GINIT) && !defined(BJORKEN) && !defined(GUBSER) && !defined(FLUCT)
+    for (int i = 0; i < XCM; i++)
+      for (int j = 0; j < YCM; j++)
+        for (int k = 0; k < ZCM; k++) {
+          cell& c = HydroGrid[i][j][k];
+          c.X = XCORDWB(i);
+          c.Y = YCORDWB(j);
+          c.eta = ZCORDWB(k);
+          c.r = sqrt(c.X * c.X + c.Y * c.Y);
+        }
+    initvar(HydroGrid, tau, ts);
+#else
+    fclose(f);
+    MPI_Finalize();
+    return 4;
+#endif
+  }
+  if (mode == 2) {
+    /* planted cells, in interior loop order: two with the same largest
+     * |v| >= 1, two with the same largest e, two with e <= 0 */
+    int ci[6][3] = {{BORDER + 1, BORDER + 2, kl}, {ir - 2, jr - 1, kr - 1}, {BORDER + 3, BORDER, kl},
+                    {ir - 1, BORDER + 1, kl}, {BORDER + 2, BORDER + 4, kr - 1}, {ir - 3, jr - 2, kl}};
+    for (int q = 0; q < 6; q++) {
+      cell& c = HydroGrid[ci[q][0]][ci[q][1]][ci[q][2]];
+      if (q < 2) { c.Vx = 0.8; c.Vy = -0.7; c.Ve = 0.0; }
+      else if (q < 4) c.En = 90.0;
+      else c.En = q == 4 ? -0.5 : 0.0;
+    }
+  }
   int hdr[8] = {0, XCM, YCM, ZCM, BORDER, ZCMA, one_rank, 0};
 #ifdef LBI
   hdr[7] = 1;
@@ -175,6 +221,25 @@ int main(int argc, char** argv)
   for (int l = 0; l < 4; l++) DUMP(c.u[l]);
   for (int l = 0; l < 4; l++) DUMP(c.prevu[l]);
 
+  if (mode == 3) {
+    fclose(f);
+    MPI_Finalize();
+    return 0;
+  }
+  if (mode == 2) {
+    DebugMSG(HydroGrid, tau);
+    CheckPhysics(HydroGrid, tau, 0);
+    double extra[6] = {BMax, MaxTempGev(HydroGrid), (double)NPX, (double)NPY, (double)FREQ, (double)FREQZ};
+    put(f, extra, 6);
+    fclose(f);
+#ifdef LBI
+    WriteResultsXY(tau, HydroGrid);
+#else
+    WriteResults(tau, HydroGrid);
+#endif
+    MPI_Finalize();
+    return 0;
+  }
   CalcSource(HydroGrid, tau, ts);
   for (int l = 0; l < SVAR; l++) DUMP(c.Source[l]);
 #ifdef KT

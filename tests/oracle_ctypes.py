@@ -69,6 +69,13 @@ def lib():
         L.or_cart_df.argtypes = [C.POINTER(ViralDesc), dp, dp, dp]
         L.or_centre_flux.argtypes = [vp, C.c_int, C.c_double, dp]
         L.or_shear_source.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_double, C.c_double, dp]
+        lp = C.POINTER(C.c_longlong)
+        L.or_check_root.restype = C.c_int
+        L.or_check_root.argtypes = [vp, C.c_double, dp, dp, lp]
+        L.or_debug_msg.argtypes = [vp, C.c_double, dp, lp]
+        L.or_check_physics.argtypes = [vp, C.c_double, lp, lp]
+        L.or_output.restype = C.c_size_t
+        L.or_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -191,3 +198,33 @@ class Oracle:
 
     def max_temp_gev(self):
         return self.L.or_max_temp_gev(self.h)
+
+    def check_root(self, tau):
+        """(max, avg, at) of CheckRoot's statistics (6 Milne, 9 Cartesian)"""
+        mx, avg = np.zeros(9), np.zeros(9)
+        at = np.zeros(9, dtype=np.int64)
+        ns = self.L.or_check_root(self.h, tau, _dp(mx), _dp(avg), at.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return mx[:ns], avg[:ns], at[:ns]
+
+    def debug_msg(self, tau):
+        """({vmax, vx, vy, ve, emax, emin}, cells {vmax, emax, emin})"""
+        out = np.zeros(6)
+        at = np.zeros(3, dtype=np.int64)
+        self.L.or_debug_msg(self.h, tau, _dp(out), at.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return out, at
+
+    def check_physics(self, tau):
+        """(counts, first cells) of |v| >= 1 and e <= 0"""
+        n = np.zeros(2, dtype=np.int64)
+        first = np.zeros(2, dtype=np.int64)
+        self.L.or_check_physics(self.h, tau, n.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                first.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return n, first
+
+    def output(self, freq=1, freqz=1, xy=0):
+        m = self.L.or_output(self.h, freq, freqz, xy, None)
+        if m == 0:
+            return None
+        out = np.zeros(m)
+        self.L.or_output(self.h, freq, freqz, xy, _dp(out))
+        return out

@@ -32,12 +32,37 @@ def available(name):
 
 
 def run(name, seed, tau, ts, mode=0):
+    """mode 0: source, KT/SHASTA result and ghost fill of a random state;
+    mode 1: s95p samples; mode 2: DebugMSG/CheckPhysics printout, BMax,
+    MaxTempGev and the WriteResults[XY] file of a random state with planted
+    cells (see oracle/ref_harness.cpp)."""
     exe = os.path.join(REF_DIR, name)
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, "out.bin")
-        subprocess.run([exe, out, str(seed), repr(tau), repr(ts), str(mode)], check=True,
-                       stdout=subprocess.DEVNULL)
+        os.mkdir(os.path.join(td, "res"))
+        proc = subprocess.run([exe, out, str(seed), repr(tau), repr(ts), str(mode)], check=True, cwd=td,
+                              stdout=subprocess.PIPE if mode == 2 else subprocess.DEVNULL)
         raw = open(out, "rb").read()
+        written = {}
+        for fn in os.listdir(os.path.join(td, "res")):
+            written[fn] = open(os.path.join(td, "res", fn), "rb").read()
+    if mode in (2, 3):
+        hdr = np.frombuffer(raw[:32], dtype=np.int32)
+        _, gx, gy, gz, border, nz, one_rank, lbi = [int(v) for v in hdr]
+        th = np.frombuffer(raw[32:48], dtype=np.float64)
+        n = gx * gy * gz
+        arr = np.frombuffer(raw[48:], dtype=np.float64)
+        state = arr[:NFIELDS * n].reshape(NFIELDS, gx, gy, gz)
+        desc = make_desc(**PROFILES[name])
+        if lbi and gz != 1:
+            state = state[:, :, :, :1]
+        if mode == 3:
+            return dict(desc=desc, tau=th[0], ts=th[1], state=state.reshape(NFIELDS, -1).copy())
+        return dict(desc=desc, tau=th[0], ts=th[1], state=state.reshape(NFIELDS, -1).copy(),
+                    bmax=float(arr[NFIELDS * n]), tmax=float(arr[NFIELDS * n + 1]),
+                    npx=int(arr[NFIELDS * n + 2]), npy=int(arr[NFIELDS * n + 3]),
+                    freq=int(arr[NFIELDS * n + 4]), freqz=int(arr[NFIELDS * n + 5]), lbi=bool(lbi),
+                    stdout=proc.stdout.decode(), files=written)
     if mode == 1:
         hdr = np.frombuffer(raw[:8], dtype=np.int32)
         return np.frombuffer(raw[8:], dtype=np.float64).reshape(hdr[1], 5)

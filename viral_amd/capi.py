@@ -22,7 +22,8 @@ EXPORTS = [
     "viral_eos_eval", "viral_set_tile", "viral_get_tile", "viral_autotune", "viral_timers", "viral_timer_read",
     "viral_kernel_count", "viral_kernel_name", "viral_last_error", "viral_steps", "viral_get_launch",
     "viral_set_launch", "viral_tile_count", "viral_tile_table", "viral_xcd_tile", "viral_adjoint_begin",
-    "viral_adjoint", "viral_adjoint_end",
+    "viral_adjoint", "viral_adjoint_end", "viral_check_root", "viral_debug_extrema", "viral_check_physics",
+    "viral_output_size", "viral_output", "viral_sync", "viral_host_alloc", "viral_host_free",
 ]
 
 # kernel phases of the tile table (VIRAL_PHASE_*)
@@ -116,6 +117,23 @@ def load():
     L.viral_adjoint.argtypes = [vp, C.POINTER(dp), C.POINTER(dp), dp]
     L.viral_adjoint_end.restype = C.c_int
     L.viral_adjoint_end.argtypes = [vp]
+    ip = C.POINTER(C.c_int)
+    L.viral_check_root.restype = C.c_int
+    L.viral_check_root.argtypes = [vp, C.c_int, C.c_double, dp, dp, ip, ip]
+    L.viral_debug_extrema.restype = C.c_int
+    L.viral_debug_extrema.argtypes = [vp, C.c_int, C.c_double, dp, ip]
+    L.viral_check_physics.restype = C.c_int
+    L.viral_check_physics.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_longlong), ip]
+    L.viral_output_size.restype = C.c_int
+    L.viral_output_size.argtypes = [pdesc, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]
+    L.viral_output.restype = C.c_int
+    L.viral_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int]
+    L.viral_sync.restype = C.c_int
+    L.viral_sync.argtypes = [vp]
+    L.viral_host_alloc.restype = C.c_int
+    L.viral_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    L.viral_host_free.restype = C.c_int
+    L.viral_host_free.argtypes = [vp]
     _lib = L
     return L
 
@@ -309,3 +327,46 @@ class Hydro:
         v = C.c_double()
         self._chk(self.L.viral_max_temp_gev(self.h, ev, C.byref(v)))
         return v.value
+
+    def _cells(self, ijk, ev):
+        """ghosted linear indices of (i, j, k) triples, -1 where none"""
+        gx, gy, gz = self.descs[ev].ghost_dims()
+        t = np.asarray(ijk, dtype=np.int64).reshape(-1, 3)
+        return np.where(t[:, 0] < 0, -1, (t[:, 0] * gy + t[:, 1]) * gz + t[:, 2])
+
+    def check_root(self, tau, ev=0):
+        """CheckRoot statistics: (max, avg, cells), 6 (Milne) or 9 (Cartesian)"""
+        mx, avg = np.zeros(9), np.zeros(9)
+        ijk = np.zeros(27, dtype=np.int32)
+        ns = C.c_int()
+        self._chk(self.L.viral_check_root(self.h, ev, tau, mx.ctypes.data_as(C.POINTER(C.c_double)),
+                                          avg.ctypes.data_as(C.POINTER(C.c_double)),
+                                          ijk.ctypes.data_as(C.POINTER(C.c_int)), C.byref(ns)))
+        n = ns.value
+        return mx[:n], avg[:n], self._cells(ijk[:3 * n], ev)
+
+    def debug_extrema(self, tau, ev=0):
+        """DebugMSG extrema: ({vmax, vx, vy, ve, emax, emin}, cells {vmax, emax, emin})"""
+        out = np.zeros(6)
+        ijk = np.zeros(9, dtype=np.int32)
+        self._chk(self.L.viral_debug_extrema(self.h, ev, tau, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                             ijk.ctypes.data_as(C.POINTER(C.c_int))))
+        return out, self._cells(ijk, ev)
+
+    def check_physics(self, tau, ev=0):
+        """CheckPhysics: (counts, first cells) of |v| >= 1 and e <= 0"""
+        n = np.zeros(2, dtype=np.int64)
+        ijk = np.zeros(6, dtype=np.int32)
+        self._chk(self.L.viral_check_physics(self.h, ev, tau, n.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                             ijk.ctypes.data_as(C.POINTER(C.c_int))))
+        return n, self._cells(ijk, ev)
+
+    def output(self, freq=1, freqz=1, xy=0, ev=0):
+        """write.h output records of the current state (17 doubles per cell)"""
+        m = C.c_size_t()
+        r = self.L.viral_output_size(C.byref(self.descs[ev]), freq, freqz, xy, C.byref(m))
+        if r != OK:
+            raise ViralError(r, "output strides must divide the grid")
+        out = np.empty(m.value)
+        self._chk(self.L.viral_output(self.h, ev, freq, freqz, xy, out.ctypes.data_as(C.POINTER(C.c_double)), 0))
+        return out

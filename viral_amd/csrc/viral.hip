@@ -1070,6 +1070,185 @@ __global__ void k_max_temp(Geom g, Params p, const double* __restrict__ st, unsi
   if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
 }
 
+// ---------------------------------------------------------------- per-step diagnostics
+// The driver loop's reductions over an event's interior: CheckRoot (NEWEST10/
+// root.h:1-199; TXYZ5/multiroot.h:7-150), the DebugMSG extrema (debug.h:
+// 3-100) and CheckPhysics (utils.h:231-258).  A fixed grid of DIAG_BLOCKS
+// blocks walks the interior in reference loop order (grid stride); each
+// block reduces its cells to one DiagPart per statistic and the host combines
+// the blocks in block order, so sums are the same from run to run.
+// Statistic rules (which cell a statistic reports):
+//   MAX_FIRST  largest value, first cell in loop order reaching it; cells
+//              with value <= 0 never count (the reference's `x > m` from 0)
+//   MAX_LAST   largest value >= 0, last cell reaching it (`x >= m` from 0)
+//   KEY_FIRST / KEY_LAST  first / last cell where a predicate holds, with
+//              the value of that cell
+// NaN values never count, as with the reference's comparisons.
+constexpr int DIAG_BLOCKS = 512;
+constexpr int DIAG_NSTAT = 9;
+enum DiagRule { R_MAX_FIRST, R_MAX_LAST, R_KEY_FIRST, R_KEY_LAST };
+enum DiagKind { D_ROOT, D_EXTREMA, D_PHYSICS };
+
+struct DiagPart {
+  double v;
+  long long at;  // ghosted linear index, -1: no cell
+  double sum;
+};
+
+__host__ __device__ inline DiagPart diag_combine(DiagPart a, DiagPart b, int rule)
+{
+  DiagPart r;
+  r.sum = a.sum + b.sum;
+  bool take_b;
+  if (b.at < 0) take_b = false;
+  else if (a.at < 0) take_b = true;
+  else if ((rule == R_MAX_FIRST || rule == R_MAX_LAST) && b.v != a.v) take_b = b.v > a.v;
+  else take_b = (rule == R_MAX_FIRST || rule == R_KEY_FIRST) ? b.at < a.at : b.at > a.at;
+  r.v = take_b ? b.v : a.v;
+  r.at = take_b ? b.at : a.at;
+  return r;
+}
+
+// rules and count of the statistics of each kind
+__host__ __device__ inline int diag_nstat(int kind, int milne)
+{
+  return kind == D_ROOT ? (milne ? 6 : 9) : kind == D_EXTREMA ? 3 : 2;
+}
+__host__ __device__ inline int diag_rule(int kind, int milne, int s)
+{
+  if (kind == D_ROOT) return (!milne && s == 8) ? R_KEY_LAST : R_MAX_FIRST;
+  if (kind == D_EXTREMA) return s == 0 ? R_MAX_LAST : R_MAX_FIRST;
+  return R_KEY_FIRST;
+}
+
+// |v|, NEWEST10/debug.h:35 and utils.h:242 (TXYZ5: no tau)
+__device__ __forceinline__ double diag_speed(const Params& p, const double* st, long n, long x, double tau)
+{
+  const double vx = st[VIRAL_F_VX * n + x], vy = st[(VIRAL_F_VX + 1) * n + x], ve = st[(VIRAL_F_VX + 2) * n + x];
+  if (p.milne) return sqrt(vx * vx + vy * vy + tau * tau * ve * ve);
+  return sqrt(vx * vx + vy * vy + ve * ve);
+}
+
+// Per-cell values of the statistics (value, counts toward the statistic,
+// added to the sum).  Exact arithmetic in the reference's operation order
+// (this file is compiled without FP contraction), closed-form EOS.
+template <int KIND>
+__device__ __forceinline__ void diag_cell(const Params& p, const double* __restrict__ st, long n, long x, double tau,
+                                          double val[DIAG_NSTAT], bool cand[DIAG_NSTAT], double add[DIAG_NSTAT])
+{
+  auto F = [&](int f) { return st[(long)f * n + x]; };
+  if (KIND == D_EXTREMA) {
+    const double b = diag_speed(p, st, n, x, tau), e = F(VIRAL_F_EN);
+    val[0] = b; cand[0] = b >= 0;           // b >= vmax from 0: the last cell reaching the maximum
+    val[1] = e; cand[1] = e > 0;            // e > emax from 0
+    val[2] = -e; cand[2] = -e > 0;          // e < emin from 0
+    add[0] = add[1] = add[2] = 0;
+    return;
+  }
+  if (KIND == D_PHYSICS) {
+    const double b = diag_speed(p, st, n, x, tau), e = F(VIRAL_F_EN);
+    cand[0] = b >= 1; val[0] = b; add[0] = cand[0] ? 1.0 : 0.0;
+    cand[1] = e <= 0; val[1] = e; add[1] = cand[1] ? 1.0 : 0.0;
+    return;
+  }
+  const double e = F(VIRAL_F_EN), PI = F(VIRAL_F_BULK), P = eos_p(p, e);
+  const double W = F(VIRAL_F_T00), X = F(VIRAL_F_T00 + 1), Y = F(VIRAL_F_T00 + 2), Z = F(VIRAL_F_T00 + 3);
+  double u[4], pi[10], rows[4], trace;
+  for (int m = 0; m < 4; m++) u[m] = F(VIRAL_F_U0 + m);
+  int r0;
+  if (p.milne) {
+    // root.h:29-44
+    const double pi1 = F(VIRAL_F_PI0), pi2 = F(VIRAL_F_PI0 + 1), pi3 = F(VIRAL_F_PI0 + 2), pi4 = F(VIRAL_F_PI0 + 3);
+    const double M2 = (X - pi2) * (X - pi2) + (Y - pi3) * (Y - pi3) + tau * tau * (Z - pi4) * (Z - pi4);
+    const double res = e - (W - pi1) + (M2 / ((W - pi1) + P - PI));
+    val[0] = fabs(res);
+    r0 = 1;
+    for (int l = 0; l < 10; l++) pi[l] = F(VIRAL_F_PI0 + l);
+    // root.h:72-90
+    rows[0] = pi[0] * u[0] - pi[1] * u[1] - pi[2] * u[2] - pi[3] * u[3] * tau * tau;
+    rows[1] = pi[1] * u[0] - pi[4] * u[1] - pi[5] * u[2] - pi[6] * u[3] * tau * tau;
+    rows[2] = pi[2] * u[0] - pi[5] * u[1] - pi[7] * u[2] - pi[8] * u[3] * tau * tau;
+    rows[3] = pi[3] * u[0] - pi[6] * u[1] - pi[8] * u[2] - pi[9] * u[3] * tau * tau;
+    trace = pi[0] - pi[4] - pi[7] - pi[9] * tau * tau;
+  } else {
+    // multiroot.h:45-48 with pow(x, 0.5) as sqrt and pow(u, 2) as u*u
+    const double u1 = u[1], u2 = u[2], u3 = u[3];
+    const double p1 = F(VIRAL_F_PI0), p2 = F(VIRAL_F_PI0 + 1), p3 = F(VIRAL_F_PI0 + 2), p4 = F(VIRAL_F_PI0 + 3),
+                 p5 = F(VIRAL_F_PI0 + 4);
+    const double Q = 1 + u1 * u1 + u2 * u2 + u3 * u3, S3 = u1 * u1 + u2 * u2 + u3 * u3, rq = sqrt(Q);
+    val[0] = fabs(-(u1 * X) - u2 * Y - u3 * Z + (-e + W) * rq);
+    val[1] = fabs(-(p3 * u2) - p4 * u3 - P * u1 * Q - u1 * (p1 + e * Q - PI * Q) + X * rq);
+    val[2] = fabs(-(p3 * u1) - (e + p2 - PI) * u2 - p5 * u3 - (e - PI) * u2 * S3 - P * u2 * Q + Y * rq);
+    val[3] = fabs(-(p4 * u1) - p5 * u2 - P * u3 + u3 * (p1 + p2 + PI - W) + Z * rq);
+    r0 = 4;
+    cart_pi10(u, p1, p2, p3, p4, p5, pi);
+    // multiroot.h:90-104
+    rows[0] = pi[0] * u[0] - pi[1] * u[1] - pi[2] * u[2] - pi[3] * u[3];
+    rows[1] = pi[1] * u[0] - pi[4] * u[1] - pi[5] * u[2] - pi[6] * u[3];
+    rows[2] = pi[2] * u[0] - pi[5] * u[1] - pi[7] * u[2] - pi[8] * u[3];
+    rows[3] = pi[3] * u[0] - pi[6] * u[1] - pi[8] * u[2] - pi[9] * u[3];
+    trace = pi[0] - pi[4] - pi[7] - pi[9];
+  }
+  for (int l = 0; l < 4; l++) val[r0 + l] = fabs(rows[l]);
+  val[r0 + 4] = fabs(trace);
+  for (int s = 0; s < r0 + 4; s++) { cand[s] = val[s] > 0; add[s] = val[s]; }
+  // Cartesian trace: `fabs(trace) > trace` keeps the last negative trace
+  cand[r0 + 4] = p.milne ? val[r0 + 4] > 0 : fabs(trace) > trace;
+  add[r0 + 4] = val[r0 + 4];
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(BLK) k_diag(Geom g, Params p, const double* __restrict__ st, double tau,
+                                              DiagPart* __restrict__ out)
+{
+  __shared__ DiagPart lds[BLK];
+  const int ns = diag_nstat(KIND, p.milne);
+  DiagPart acc[DIAG_NSTAT];
+  for (int s = 0; s < DIAG_NSTAT; s++) acc[s] = DiagPart{0.0, -1, 0.0};
+  const Box b = interior_box(g);
+  const long total = (long)b.ni * b.nj * b.nk;
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < total; t += (long)gridDim.x * BLK) {
+    int i, j, k;
+    box_cell(b, t, i, j, k);
+    const long x = cidx(g, i, j, k);
+    double val[DIAG_NSTAT], add[DIAG_NSTAT];
+    bool cand[DIAG_NSTAT];
+    diag_cell<KIND>(p, st, g.n, x, tau, val, cand, add);
+    for (int s = 0; s < ns; s++)
+      acc[s] = diag_combine(acc[s], DiagPart{val[s], cand[s] ? (long long)x : -1, add[s]}, diag_rule(KIND, p.milne, s));
+  }
+  for (int s = 0; s < ns; s++) {
+    lds[threadIdx.x] = acc[s];
+    __syncthreads();
+    for (int h = BLK / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h)
+        lds[threadIdx.x] = diag_combine(lds[threadIdx.x], lds[threadIdx.x + h], diag_rule(KIND, p.milne, s));
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(long)blockIdx.x * DIAG_NSTAT + s] = lds[0];
+    __syncthreads();
+  }
+}
+
+// write.h output records (NEWEST10/write.h:87-103): one thread per double of
+// the record array; record r = ((oi * nyo) + oj) * nzo + ok holds
+// {e, T, P, vx, vy, ve, pi[0..9], PI} of cell (il + oi*freq, jl + oj*freq,
+// kl + ok*freqz), or of the k0 slice when xy.
+__global__ void __launch_bounds__(BLK) k_output(Geom g, const double* __restrict__ st, int freq, int freqz, int xy,
+                                                long nyo, long nzo, long total, double* __restrict__ out)
+{
+  const long t = (long)blockIdx.x * BLK + threadIdx.x;
+  if (t >= total) return;
+  const long r = t / 17;
+  const int f = (int)(t - r * 17);
+  const long ok = r % nzo, oj = (r / nzo) % nyo, oi = r / (nzo * nyo);
+  const int i = g.il + (int)oi * freq, j = g.jl + (int)oj * freq;
+  const int k = xy ? g.k0 : g.kl + (int)ok * freqz;
+  const int field = f == 0 ? VIRAL_F_EN : f == 1 ? VIRAL_F_TEMP : f == 2 ? VIRAL_F_P : f < 6 ? VIRAL_F_VX + (f - 3)
+                    : f < 16 ? VIRAL_F_PI0 + (f - 6) : VIRAL_F_BULK;
+  out[t] = st[(long)field * g.n + cidx(g, i, j, k)];
+}
+
 // ---------------------------------------------------------------- adjoint kernels
 // Reverse of one Milne KT stage for one event (conformal EOS).  lam_out is
 // the adjoint of the stage's output state (29 fields), lam_in receives the
@@ -1542,6 +1721,10 @@ struct viral_ctx {
   bool timing = false;
   std::vector<hipEvent_t> tev[K_COUNT];  // start/stop pairs
   std::vector<hipEvent_t> pool;
+  // per-step diagnostics: block partials; output records: device staging
+  DiagPart* d_diag = nullptr;
+  double* d_out = nullptr;
+  size_t out_cap = 0;
   // adjoint recording: the (tau, dtau) of every step since viral_adjoint_begin
   // and device copies of the state every `every` steps
   struct {
@@ -2159,6 +2342,8 @@ void viral_destroy(viral_ctx* ctx)
   if (ctx->d_bad) hipFree(ctx->d_bad);
   if (ctx->h_bad) hipHostFree(ctx->h_bad);
   if (ctx->dtmp) hipFree(ctx->dtmp);
+  if (ctx->d_diag) hipFree(ctx->d_diag);
+  if (ctx->d_out) hipFree(ctx->d_out);
   for (int k = 0; k < K_COUNT; k++)
     for (hipEvent_t e : ctx->tev[k]) hipEventDestroy(e);
   for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
@@ -2323,6 +2508,149 @@ int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)
   double v;
   memcpy(&v, &bits, sizeof(v));
   *out = v;
+  return VIRAL_OK;
+}
+
+// Runs one diagnostic reduction over event E's current state and combines
+// the block partials in block order into res[0..nstat).
+static int run_diag(viral_ctx* ctx, Event* E, int kind, double tau, DiagPart* res, int* nstat)
+{
+  if (!ctx->d_diag) HIPCHK(hipMalloc(&ctx->d_diag, sizeof(DiagPart) * DIAG_BLOCKS * DIAG_NSTAT));
+  const long ni = box_count(interior_box(E->g));
+  const unsigned nb = (unsigned)std::min<long>(DIAG_BLOCKS, std::max<long>(1, (ni + BLK - 1) / BLK));
+  const double* st = E->st[ctx->cur];
+  if (kind == D_ROOT)
+    hipLaunchKernelGGL(k_diag<D_ROOT>, dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, ctx->d_diag);
+  else if (kind == D_EXTREMA)
+    hipLaunchKernelGGL(k_diag<D_EXTREMA>, dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, ctx->d_diag);
+  else
+    hipLaunchKernelGGL(k_diag<D_PHYSICS>, dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, ctx->d_diag);
+  HIPCHK(hipGetLastError());
+  std::vector<DiagPart> part((size_t)nb * DIAG_NSTAT);
+  HIPCHK(hipMemcpyAsync(part.data(), ctx->d_diag, part.size() * sizeof(DiagPart), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int ns = diag_nstat(kind, E->p.milne);
+  for (int q = 0; q < ns; q++) {
+    DiagPart acc{0.0, -1, 0.0};
+    for (unsigned b = 0; b < nb; b++) acc = diag_combine(acc, part[(size_t)b * DIAG_NSTAT + q], diag_rule(kind, E->p.milne, q));
+    res[q] = acc;
+  }
+  if (nstat) *nstat = ns;
+  return VIRAL_OK;
+}
+
+static void cell_ijk(const Geom& g, long long x, int* ijk)
+{
+  if (x < 0) { ijk[0] = ijk[1] = ijk[2] = -1; return; }
+  ijk[2] = (int)(x % g.gz);
+  ijk[1] = (int)((x / g.gz) % g.gy);
+  ijk[0] = (int)(x / ((long long)g.gz * g.gy));
+}
+
+int viral_check_root(viral_ctx* ctx, int ev, double tau, double* max, double* avg, int* ijk, int* nstat)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !max || !avg) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
+  DiagPart r[DIAG_NSTAT];
+  int ns = 0, rc = run_diag(ctx, E, D_ROOT, tau, r, &ns);
+  if (rc != VIRAL_OK) return rc;
+  const double count = (double)box_count(interior_box(E->g));
+  for (int q = 0; q < ns; q++) {
+    max[q] = r[q].v;
+    avg[q] = r[q].sum / count;
+    if (ijk) cell_ijk(E->g, r[q].at, ijk + 3 * q);
+  }
+  if (nstat) *nstat = ns;
+  return VIRAL_OK;
+}
+
+int viral_debug_extrema(viral_ctx* ctx, int ev, double tau, double* out, int* ijk)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !out) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
+  DiagPart r[DIAG_NSTAT];
+  int rc = run_diag(ctx, E, D_EXTREMA, tau, r, nullptr);
+  if (rc != VIRAL_OK) return rc;
+  out[0] = r[0].at >= 0 ? r[0].v : 0.0;
+  for (int m = 0; m < 3; m++) {
+    out[1 + m] = 0.0;
+    if (r[0].at >= 0)
+      HIPCHK(hipMemcpy(&out[1 + m], E->st[ctx->cur] + (long)(VIRAL_F_VX + m) * E->g.n + r[0].at, sizeof(double),
+                       hipMemcpyDeviceToHost));
+  }
+  out[4] = r[1].at >= 0 ? r[1].v : 0.0;
+  out[5] = r[2].at >= 0 ? -r[2].v : 0.0;
+  if (ijk)
+    for (int q = 0; q < 3; q++) cell_ijk(E->g, r[q].at, ijk + 3 * q);
+  return VIRAL_OK;
+}
+
+int viral_check_physics(viral_ctx* ctx, int ev, double tau, long long* count, int* ijk)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !count) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
+  DiagPart r[DIAG_NSTAT];
+  int rc = run_diag(ctx, E, D_PHYSICS, tau, r, nullptr);
+  if (rc != VIRAL_OK) return rc;
+  for (int q = 0; q < 2; q++) {
+    count[q] = (long long)r[q].sum;
+    if (ijk) cell_ijk(E->g, r[q].at, ijk + 3 * q);
+  }
+  return VIRAL_OK;
+}
+
+int viral_output_size(const viral_desc* d, int freq, int freqz, int xy, size_t* ndoubles)
+{
+  if (!d || !ndoubles) return VIRAL_ERR_ARG;
+  if (freq < 1 || freqz < 1 || d->nx % freq || d->ny % freq || (!xy && !d->lbi && d->nz % freqz)) return VIRAL_ERR_ARG;
+  const size_t nzo = (xy || d->lbi) ? 1 : (size_t)(d->nz / freqz);
+  *ndoubles = (size_t)(d->nx / freq) * (size_t)(d->ny / freq) * nzo * 17;
+  return VIRAL_OK;
+}
+
+int viral_output(viral_ctx* ctx, int ev, int freq, int freqz, int xy, double* out, int async)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !out) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
+  size_t m = 0;
+  if (viral_output_size(&E->d, freq, freqz, xy, &m) != VIRAL_OK)
+    return fail(ctx, VIRAL_ERR_ARG, "output strides must divide the grid");
+  if (m > ctx->out_cap) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_out) HIPCHK(hipFree(ctx->d_out));
+    ctx->d_out = nullptr;
+    ctx->out_cap = 0;
+    HIPCHK(hipMalloc(&ctx->d_out, m * sizeof(double)));
+    ctx->out_cap = m;
+  }
+  const long nyo = E->d.ny / freq, nzo = (xy || E->d.lbi) ? 1 : E->d.nz / freqz;
+  hipLaunchKernelGGL(k_output, dim3(nblocks((long)m)), dim3(BLK), 0, ctx->stream, E->g, E->st[ctx->cur], freq, freqz,
+                     xy || E->d.lbi, nyo, nzo, (long)m, ctx->d_out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, ctx->d_out, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (!async) HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
+}
+
+int viral_sync(viral_ctx* ctx)
+{
+  if (!ctx) return VIRAL_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
+}
+
+int viral_host_alloc(size_t bytes, void** p)
+{
+  if (!p) return VIRAL_ERR_ARG;
+  *p = nullptr;
+  if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) return VIRAL_ERR_HIP;
+  return VIRAL_OK;
+}
+
+int viral_host_free(void* p)
+{
+  if (p && hipHostFree(p) != hipSuccess) return VIRAL_ERR_HIP;
   return VIRAL_OK;
 }
 
