@@ -32,13 +32,12 @@ def workload_desc(rk=3, fast=1):
 
 
 def eos_p_vec(desc):
-    """s95p pressure on the host for the initial state (the oracle's closed
-    form is the reference's formula; used only to build inputs)."""
-    import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_ctypes
-    L = oracle_ctypes.lib()
-    return lambda e: np.vectorize(lambda v: L.or_eos_p(desc, float(v)))(e)
+    """s95p pressure on the host for the initial state: the library's host
+    EOS (viral_eos_eval) in its closed form."""
+    from viral_amd import capi
+    dref = desc.copy()
+    dref.fast_math = dref.eos_table = 0
+    return lambda e: capi.eos_eval(dref, e)[0].reshape(e.shape)
 
 
 # Compulsory HBM bytes per launch of each kernel: every input field read
