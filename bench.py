@@ -189,20 +189,23 @@ def bench_ensemble(n_events, steps, tau0=0.6, ts=0.002):
     return out
 
 
-def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002):
+def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
     """Reverse mode through `steps` viscous KT steps of a smooth Glauber event
-    (b=8 fm, peak 30 GeV/fm^3) on a 128x128x32 Milne grid, conformal EOS,
-    eta/s=0.08, tau_pi=5eta/(e+p), RK2, checkpoints every `every` steps: the
-    gradient of the final momentum anisotropy eps_2 with respect to the
-    initial state, eta/s and the tau_pi coefficient.  Reports the time of the
-    recorded forward run and of the adjoint call (which recomputes the stage
-    states between checkpoints)."""
+    (b=8 fm, peak 30 GeV/fm^3) on a 128x128x32 Milne grid, eta/s=0.08,
+    tau_pi=5eta/(e+p), RK2, checkpoints every `every` steps: the gradient of
+    the final momentum anisotropy eps_2 with respect to the initial state,
+    eta/s and the tau_pi coefficient.  eos "s95p" with fast=1 reads the s95p
+    table (the adjoint differentiates its interpolant); "conformal" is e = 3p.
+    Reports the time of the recorded forward run and of the adjoint call
+    (which recomputes the stage states between checkpoints)."""
     import numpy as np
     import torch
     from viral_amd import capi, ic, autograd
-    from viral_amd.desc import make_desc, F_EN
-    d = make_desc(128, 128, 32, rk=2, eta_coef=0.08, tpi_coef=5.0)
-    st = ic.glauber(d, tau0, ts, b=8.0)
+    from viral_amd.desc import make_desc, F_EN, EOS_S95P, EOS_CONFORMAL
+    s95 = eos == "s95p"
+    d = make_desc(128, 128, 32, rk=2, eta_coef=0.08, tpi_coef=5.0, fast_math=fast,
+                  eos=EOS_S95P if s95 else EOS_CONFORMAL, eos_table=int(s95 and fast))
+    st = ic.glauber(d, tau0, ts, b=8.0, eos_p=eos_p_vec(d) if s95 else None)
     h = capi.Hydro(d)
     h.upload(st)
     r = h.step(tau0, ts)   # warm-up (kernel load), then restart from the initial state
@@ -226,9 +229,10 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002):
     grad, pg = h.adjoint(seed)
     t_adj = time.perf_counter() - t0
     h.close()
-    return {"workload": "adjoint of %d viscous KT steps, smooth Glauber b=8 fm, Milne 128x128x32, conformal EOS, "
-                        "eta/s=0.08, tau_pi=5eta/(e+p), RK2, dtau=%g, checkpoints every %d steps; "
-                        "d eps_2 / d(initial state, eta/s, tau_pi coefficient)" % (steps, ts, every),
+    return {"workload": "adjoint of %d viscous KT steps, smooth Glauber b=8 fm, Milne 128x128x32, %s EOS, "
+                        "eta/s=0.08, tau_pi=5eta/(e+p), RK2, dtau=%g, checkpoints every %d steps, %s arithmetic; "
+                        "d eps_2 / d(initial state, eta/s, tau_pi coefficient)"
+                        % (steps, "s95p (table)" if s95 and fast else eos, ts, every, "fast" if fast else "exact"),
             "steps": steps, "forward_ms": t_plain * 1e3, "forward_recorded_ms": t_fwd * 1e3,
             "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_plain,
             "eps2": float(eps2), "d_eps2_d_eta_coef": float(pg[0]), "d_eps2_d_tpi_coef": float(pg[1]),
@@ -245,6 +249,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the adjoint leg (0 skips it)")
+    ap.add_argument("--adjoint-eos", choices=["s95p", "conformal"], default="s95p")
+    ap.add_argument("--adjoint-exact", action="store_true", help="run the adjoint leg in exact arithmetic")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy so a counter run can scale "
@@ -337,7 +343,7 @@ def main():
     if args.ensemble_events > 0:
         out["ensemble"] = bench_ensemble(args.ensemble_events, args.ensemble_steps)
     if args.adjoint_steps > 0:
-        out["adjoint"] = bench_adjoint(args.adjoint_steps)
+        out["adjoint"] = bench_adjoint(args.adjoint_steps, eos=args.adjoint_eos, fast=0 if args.adjoint_exact else 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

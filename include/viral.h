@@ -300,12 +300,13 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps);
  * viral_adjoint(ctx, seed, grad, pgrad) then runs the recorded steps in
  * reverse: between checkpoints it recomputes the stage states forward and
  * applies hand-written adjoint kernels (KT flux and WENO reconstruction,
- * sources, EOS, velocities; the root by the implicit-function theorem at the
- * converged root).  seed[f] holds dJ/d(final state field f) (VIRAL_NFIELDS
+ * sources, EOS -- the closed forms or the derivative of the table's
+ * interpolant --, velocities; the root by the implicit-function theorem at
+ * the converged root).  seed[f] holds dJ/d(final state field f) (VIRAL_NFIELDS
  * ghosted arrays), grad[f] receives dJ/d(recorded initial state field f), each
  * field taken as an independent input, and pgrad[0], pgrad[1] receive dJ/d
  * eta_coef and dJ/d tpi_coef.  The state is left as it was.  Single-event
- * Milne KT contexts with the conformal EOS.  viral_adjoint_end frees the
+ * Milne KT contexts, conformal or s95p EOS.  viral_adjoint_end frees the
  * recording. */
 int viral_adjoint_begin(viral_ctx* ctx, int every);
 int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad, double* pgrad);

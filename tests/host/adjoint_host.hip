@@ -5,15 +5,25 @@
 #include <hip/hip_runtime.h>
 
 #include "../../viral_amd/csrc/adjoint.h"
+#include "../../viral_amd/csrc/eos_table.h"
 
 using namespace viral;
 
-static Params make_params(int eta_law, double eta_coef, int tpi_law, double tpi_coef)
+// the library's s95p table, built on the host
+static EosTab host_table()
+{
+  static std::vector<double> v;
+  static std::vector<unsigned char> c;
+  if (v.empty()) build_eos_table(v, c);
+  return EosTab{v.data(), c.data(), kTabEmin, kTabNoct, kTabK};
+}
+
+static Params make_params(int eta_law, double eta_coef, int tpi_law, double tpi_coef, int eos = 0)
 {
   Params p{};
   p.milne = 1;
   p.lbi = 0;
-  p.eos = 0;
+  p.eos = eos;
   p.eta_law = eta_law;
   p.tpi_law = tpi_law;
   p.max_iter = 5000;
@@ -30,6 +40,30 @@ static Params make_params(int eta_law, double eta_coef, int tpi_law, double tpi_
 }
 
 extern "C" {
+
+// closed-form P, T, s of the EOS and their e-derivatives (eos_d)
+void ah_eos(int eos, double e, double* out /* P, T, s */)
+{
+  Params p = make_params(0, 0.2, 0, 5.0, eos);
+  out[0] = eos_p(p, e);
+  out[1] = eos_T(p, e);
+  out[2] = eos ? s95p_s(e) : 0.0;
+}
+void ah_eos_d(int eos, double e, double* out /* P, dP, T, dT, s, ds */)
+{
+  Params p = make_params(0, 0.2, 0, 5.0, eos);
+  eos_d(p, e, out[0], out[1], out[2], out[3], out[4], out[5]);
+}
+// the table's interpolant and its e-derivative; returns 0 where the table
+// does not apply
+int ah_tab(double e, double* out)
+{
+  return eos_tab_lookup(host_table(), e, out) ? 1 : 0;
+}
+int ah_tab_d(double e, double* out, double* dout)
+{
+  return eos_tab_lookup_d(host_table(), e, out, dout) ? 1 : 0;
+}
 
 double ah_weno(const double* x, int right)
 {
@@ -91,10 +125,10 @@ void ah_flux_cell_vjp(int d, double tau, double e, double P, double PI, const do
 }
 
 // forward of source_vjp: the 15 source rows of one Milne KT cell
-void ah_source(int eta_law, double eta_coef, int tpi_law, double tpi_coef, double tau, double e, double P, double PI,
-               const double* u, const double* pi10, const double* du16, double* S)
+void ah_source(int eos, int eta_law, double eta_coef, int tpi_law, double tpi_coef, double tau, double e, double P,
+               double PI, const double* u, const double* pi10, const double* du16, double* S)
 {
-  Params p = make_params(eta_law, eta_coef, tpi_law, tpi_coef);
+  Params p = make_params(eta_law, eta_coef, tpi_law, tpi_coef, eos);
   double du[4][4];
   for (int a = 0; a < 16; a++) du[a / 4][a % 4] = du16[a];
   const double tau2 = tau * tau;
@@ -107,11 +141,11 @@ void ah_source(int eta_law, double eta_coef, int tpi_law, double tpi_coef, doubl
   const double tpi = tpi_of(p, SH, P, e);
   shear_source<false>(1, tau, u, pi10, du, SH, tpi, S + 4);
 }
-void ah_source_vjp(int eta_law, double eta_coef, int tpi_law, double tpi_coef, double tau, double e, double P,
-                   double PI, const double* u, const double* pi10, const double* du16, const double* Sb,
+void ah_source_vjp(int eos, int eta_law, double eta_coef, int tpi_law, double tpi_coef, double tau, double e,
+                   double P, double PI, const double* u, const double* pi10, const double* du16, const double* Sb,
                    double* inb /* e,P,PI,u4,pi10,du16,ceta,ctpi = 35 */)
 {
-  Params p = make_params(eta_law, eta_coef, tpi_law, tpi_coef);
+  Params p = make_params(eta_law, eta_coef, tpi_law, tpi_coef, eos);
   double du[4][4], dub[4][4];
   for (int a = 0; a < 16; a++) {
     du[a / 4][a % 4] = du16[a];
@@ -131,9 +165,9 @@ void ah_source_vjp(int eta_law, double eta_coef, int tpi_law, double tpi_coef, d
 
 // forward of update_vjp: from V (15) at tn, the root (from guess) and
 // outputs e, P, Temp, v(3), u(4), T0(4), pi(10), PI = 25 values
-int ah_update(double tn, const double* V, double guess, double* out)
+int ah_update(int eos, double tn, const double* V, double guess, double* out)
 {
-  Params p = make_params(0, 0.2, 0, 5.0);
+  Params p = make_params(0, 0.2, 0, 5.0, eos);
   double T0[4];
   for (int l = 0; l < 4; l++) T0[l] = V[l] / tn;
   double pi[10];
@@ -164,9 +198,9 @@ int ah_update(double tn, const double* V, double guess, double* out)
   out[24] = PIn;
   return (int)fl;
 }
-void ah_update_vjp(double tn, const double* V, double e, const double* ob /* 25 */, double* Vb)
+void ah_update_vjp(int eos, double tn, const double* V, double e, const double* ob /* 25 */, double* Vb)
 {
-  Params p = make_params(0, 0.2, 0, 5.0);
+  Params p = make_params(0, 0.2, 0, 5.0, eos);
   double T0[4], pi[10];
   for (int l = 0; l < 4; l++) T0[l] = V[l] / tn;
   for (int l = 0; l < 10; l++) pi[l] = V[4 + l];

@@ -27,13 +27,19 @@ def lib():
     L.ah_shear_vjp.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp]
     L.ah_flux_cell.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
     L.ah_flux_cell_vjp.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
-    L.ah_source.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
-                            dp, dp, dp, dp]
-    L.ah_source_vjp.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
-                                C.c_double, dp, dp, dp, dp, dp]
+    L.ah_source.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                            C.c_double, dp, dp, dp, dp]
+    L.ah_source_vjp.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_double, dp, dp, dp, dp, dp]
     L.ah_update.restype = C.c_int
-    L.ah_update.argtypes = [C.c_double, dp, C.c_double, dp]
-    L.ah_update_vjp.argtypes = [C.c_double, dp, C.c_double, dp, dp]
+    L.ah_update.argtypes = [C.c_int, C.c_double, dp, C.c_double, dp]
+    L.ah_update_vjp.argtypes = [C.c_int, C.c_double, dp, C.c_double, dp, dp]
+    L.ah_eos.argtypes = [C.c_int, C.c_double, dp]
+    L.ah_eos_d.argtypes = [C.c_int, C.c_double, dp]
+    L.ah_tab.restype = C.c_int
+    L.ah_tab.argtypes = [C.c_double, dp]
+    L.ah_tab_d.restype = C.c_int
+    L.ah_tab_d.argtypes = [C.c_double, dp, dp]
     return L
 
 
@@ -129,8 +135,67 @@ def test_flux_cell_vjp(d):
         dot_check(f, x, inb, ob, rng)
 
 
-@pytest.mark.parametrize("laws", [(0, 0.2, 0, 5.0), (1, 1.0, 1, 1.5), (0, 0.08, 2, 0.4)])
-def test_source_vjp(laws):
+# s95p branch points of p, s and T, in 1/fm^4 (GeV/fm^3 / GEVFM)
+S95P_BREAKS = np.array([1e-5, 0.5028563305441270, 1.62, 1.86, 9.9878355786273545, 0.1270769021427449,
+                        0.446707952467404, 1.9402832534193788, 3.7292474570977285, 0.5143939846236409]) / 0.1973
+S95P_BREAKS[0] = 1e-5
+
+
+def test_s95p_closed_form_derivatives():
+    """dP/de, dT/de, ds/de of the s95p fits on every branch against central
+    differences of the fits themselves"""
+    L = lib()
+    rng = np.random.default_rng(50)
+    es = np.exp(rng.uniform(np.log(2e-6), np.log(2000.0), 400))
+    es = np.concatenate([es, S95P_BREAKS * 1.01, S95P_BREAKS * 0.99])
+    for e in es:
+        # the fits vary on the scale of e; a relative step of 1e-4 keeps the
+        # truncation error near 1e-8 and the cancellation of exp(x) - 1 at
+        # small e out of the difference
+        h = 1e-4 * e
+        if np.any(np.abs(S95P_BREAKS - e) < 2 * h):
+            continue
+        d = np.zeros(6)
+        L.ah_eos_d(1, e, P(d))
+        fp, fm = np.zeros(3), np.zeros(3)
+        L.ah_eos(1, e + h, P(fp))
+        L.ah_eos(1, e - h, P(fm))
+        fd = (fp - fm) / (2 * h)
+        for got, want in zip(d[[1, 3, 5]], fd):
+            assert abs(got - want) <= 1e-6 * abs(want) + 1e-12, (e, d, fd)
+
+
+def test_table_interpolant_derivative():
+    """the derivative of the table's cubic interpolant (what the fast-mode
+    adjoint differentiates) against central differences of the interpolant,
+    at interval midpoints over the table's range"""
+    L = lib()
+    rng = np.random.default_rng(51)
+    K = 1024
+    done = 0
+    for _ in range(600):
+        E = rng.integers(-13, 13)
+        q = rng.integers(0, K)
+        e = 2.0 ** E * (1.0 + (q + 0.5) / K)
+        out, dout = np.zeros(4), np.zeros(4)
+        if not L.ah_tab_d(e, P(out), P(dout)):
+            continue
+        h = 2.0 ** E / K * 1e-2  # stays inside the interval, where the interpolant is one cubic
+        fp, fm = np.zeros(4), np.zeros(4)
+        assert L.ah_tab(e + h, P(fp)) and L.ah_tab(e - h, P(fm))
+        fd = (fp - fm) / (2 * h)
+        assert np.all(np.abs(dout - fd) <= 1e-6 * np.abs(fd) + 1e-9 * np.abs(out) / e), (e, dout, fd)
+        # and it follows the closed-form derivatives it interpolates
+        d = np.zeros(6)
+        L.ah_eos_d(1, e, P(d))
+        assert abs(dout[0] - d[1]) <= 1e-6 * abs(d[1]) and abs(dout[2] - d[3]) <= 1e-6 * abs(d[3])
+        done += 1
+    assert done > 400
+
+
+@pytest.mark.parametrize("eos,laws", [(0, (0, 0.2, 0, 5.0)), (0, (1, 1.0, 1, 1.5)), (0, (0, 0.08, 2, 0.4)),
+                                      (1, (1, 1.0, 1, 1.5)), (1, (0, 0.08, 0, 5.0))])
+def test_source_vjp(eos, laws):
     L = lib()
     el, ec, tl, tc = laws
     rng = np.random.default_rng(30 + el + tl)
@@ -143,35 +208,39 @@ def test_source_vjp(laws):
 
         def f(z):
             S = np.zeros(15)
-            L.ah_source(el, z[33], tl, z[34], tau, z[0], z[1], z[2], P(z[3:7]), P(z[7:17]), P(z[17:33]), P(S))
+            L.ah_source(eos, el, z[33], tl, z[34], tau, z[0], z[1], z[2], P(z[3:7]), P(z[7:17]), P(z[17:33]), P(S))
             return S
         ob = rng.standard_normal(15)
         ob[[1, 2, 14]] = 0.0
         inb = np.zeros(35)
-        L.ah_source_vjp(el, ec, tl, tc, tau, x[0], x[1], x[2], P(x[3:7]), P(x[7:17]), P(x[17:33]), P(ob), P(inb))
+        L.ah_source_vjp(eos, el, ec, tl, tc, tau, x[0], x[1], x[2], P(x[3:7]), P(x[7:17]), P(x[17:33]), P(ob),
+                        P(inb))
         dot_check(f, x, inb, ob, rng)
 
 
-def test_update_vjp_root_by_implicit_function_theorem():
+@pytest.mark.parametrize("eos", [0, 1])
+def test_update_vjp_root_by_implicit_function_theorem(eos):
     L = lib()
-    rng = np.random.default_rng(40)
+    rng = np.random.default_rng(40 + eos)
     tn = 1.2
     for _ in range(8):
-        e = rng.uniform(1, 10)
+        e = rng.uniform(1, 10) if eos == 0 else np.exp(rng.uniform(np.log(0.05), np.log(300.0)))
         u = rand_u(rng, tn)
         pi = 0.02 * e * rng.standard_normal(10)
-        Pr = e / 3.0
+        ev = np.zeros(3)
+        L.ah_eos(eos, e, P(ev))
+        Pr = ev[0]
         w = e + Pr
         T = w * u[0] * u - Pr * np.array([1.0, 0, 0, 0]) + pi[[0, 1, 2, 3]]
         V = np.concatenate([T * tn, pi, [0.0]])
 
         def f(z):
             out = np.zeros(25)
-            L.ah_update(tn, P(z), e, P(out))
+            L.ah_update(eos, tn, P(z), e, P(out))
             return out
         out = f(V)
         assert abs(out[0] - e) < 1e-9 * e
         ob = rng.standard_normal(25)
         Vb = np.zeros(15)
-        L.ah_update_vjp(tn, P(V), out[0], P(ob), P(Vb))
+        L.ah_update_vjp(eos, tn, P(V), out[0], P(ob), P(Vb))
         dot_check(f, V, Vb, ob, rng, h=1e-7, tol=1e-5)

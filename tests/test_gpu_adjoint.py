@@ -12,7 +12,7 @@ import torch
 
 from viral_amd import capi, ic, autograd
 from viral_amd.desc import (make_desc, NFIELDS, F_T00, F_PI0, F_BULK, F_EN, F_P, F_U0, F_PU0, F_TEMP, F_VX,
-                            ETA_S_OVER_4PI, TPI_OVER_P, OK)
+                            ETA_S_OVER_4PI, TPI_OVER_P, EOS_S95P, OK)
 import oracle_ctypes
 
 pytestmark = pytest.mark.gpu
@@ -58,7 +58,13 @@ def direction(st, rng):
 
 def small_case(rk=2, fast=0, **kw):
     d = make_desc(10, 9, 7, rk=rk, fast_math=fast, **kw)
-    st = ic.random_smooth(d, seed=3, tau=TAU, pi_frac=0.02)
+    if d.eos == EOS_S95P:
+        dref = d.copy()
+        dref.fast_math = dref.eos_table = 0
+        st = ic.random_smooth(d, seed=3, tau=TAU, pi_frac=0.02, e_scale=30.0,
+                              eos_p=lambda e: capi.eos_eval(dref, e)[0].reshape(np.shape(e)))
+    else:
+        st = ic.random_smooth(d, seed=3, tau=TAU, pi_frac=0.02)
     return d, st
 
 
@@ -127,6 +133,41 @@ def test_fast_mode_adjoint_matches_fd():
     fd = (np.sum(seed * run_gpu(d, st + h * dx, 2)) - np.sum(seed * run_gpu(d, st - h * dx, 2))) / (2 * h)
     ad = np.sum(grad * dx)
     assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+
+
+@pytest.mark.parametrize("fast", [0, 1], ids=["closed_form", "table"])
+def test_s95p_adjoint_matches_fd(fast):
+    """s95p EOS: the closed forms' derivatives (exact mode) and the table
+    interpolant's (fast mode), through the root, the velocities, T(e) and
+    the shear source's s(e); state and parameter gradients against central
+    differences of the same GPU forward"""
+    d, st = small_case(fast=fast, eos=EOS_S95P, eos_table=fast, eta_law=ETA_S_OVER_4PI, eta_coef=1.0,
+                       tpi_law=TPI_OVER_P, tpi_coef=1.5)
+    rng = np.random.default_rng(12 + fast)
+    seed = rng.standard_normal(st.shape)
+    print("case adjoint s95p fast=%d" % fast, flush=True)
+    _, grad, pg = run_gpu(d, st, 3, 2, seed)
+    for trial in range(2):
+        dx = direction(st, rng)
+        # the s95p step is nonlinear enough that central differences at
+        # h = 1e-6 carry ~1e-6 truncation error (it falls as h^2 to a ~1e-8
+        # rounding floor at h = 1e-7)
+        h = 1e-7
+        fd = (np.sum(seed * run_gpu(d, st + h * dx, 3)) - np.sum(seed * run_gpu(d, st - h * dx, 3))) / (2 * h)
+        ad = np.sum(grad * dx)
+        print(trial, ad, fd, flush=True)
+        assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+    for q, name in enumerate(["eta_coef", "tpi_coef"]):
+        c0 = getattr(d, name)
+        hh = 1e-6 * c0
+        vals = []
+        for sgn in (1, -1):
+            dd = d.copy()
+            setattr(dd, name, c0 + sgn * hh)
+            vals.append(np.sum(seed * run_gpu(dd, st, 3)))
+        fd = (vals[0] - vals[1]) / (2 * hh)
+        print(name, pg[q], fd, flush=True)
+        assert abs(pg[q] - fd) <= 1e-5 * abs(fd), (name, pg[q], fd)
 
 
 def test_torch_harness_momentum_anisotropy():

@@ -63,8 +63,37 @@ VHD void weno_grad(const double x[5], bool right, double g[5])
   }
 }
 
-// conformal dT/de of T = (e/FACTOR)^(1/4)
-VHD double eos_dTde(const Params& p, double e, double T) { return T / (4.0 * e); }
+// P(e), T(e), s(e) and their e-derivatives as the forward step evaluates
+// them: the table's interpolant where fast mode reads the table (use_tab is
+// set only in fast mode), the closed forms elsewhere.  The conformal s =
+// (e + P)/T depends on the state's P and is left to the caller (s = ds = 0).
+VHD void eos_d(const Params& p, double e, double& P, double& dP, double& T, double& dT, double& s, double& ds)
+{
+  if (!p.eos) {
+    P = e / 3.0;
+    dP = 1.0 / 3.0;
+    T = eos_T(p, e);
+    dT = T / (4.0 * e);  // T = (e/FACTOR)^(1/4)
+    s = ds = 0.0;
+    return;
+  }
+  double v[4], dv[4];
+  if (p.use_tab && eos_tab_lookup_d(p.tab, e, v, dv)) {
+    P = v[0];
+    dP = dv[0];
+    T = v[2];
+    dT = dv[2];
+    s = v[3];
+    ds = dv[3];
+    return;
+  }
+  P = s95p_p(e);
+  dP = s95p_a(e);
+  T = s95p_T(e);
+  dT = s95p_dT(e);
+  s = s95p_s(e);
+  ds = s95p_ds(e);
+}
 
 // u = gamma (1, v), gamma = (1 - vx^2 - vy^2 - t2 ve^2)^(-1/2):
 // adds d<ub, u>/dv to vb
@@ -305,7 +334,8 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
 // given the velocity gradients du and the transport coefficients: adds the
 // adjoint of (e, P, PI, u, pi, du) and of the viscosity / relaxation
 // coefficients (eta_coef, tpi_coef) for the source adjoint Sb (15 rows).
-// Conformal EOS (s = (e+P)/T, T = (e/FACTOR)^(1/4)).
+// The entropy density is (e + P)/T(e) for the conformal EOS and s(e) for
+// s95p, as the forward evaluates it (eos_d).
 VHD void source_vjp(const Params& p, double tau, double e, double P, double PI, const double u[4], const double pi10[10],
                     const double du[4][4], const double Sb[15], double& eb, double& Pb, double& PIb, double ub[4],
                     double pib[10], double dub[4][4], double& cetab, double& ctpib)
@@ -327,8 +357,9 @@ VHD void source_vjp(const Params& p, double tau, double e, double P, double PI, 
   Pb += wb;
   PIb -= wb;
   // shear
-  const double T = eos_T(p, e);
-  const double s = eos_s(p, e, P, T);
+  double Pe, dPe, T, dT, s, ds;
+  eos_d(p, e, Pe, dPe, T, dT, s, ds);
+  if (!p.eos) s = (e + P) / T;
   const double SH = eta_of(p, s);
   const double tpi = tpi_of(p, SH, P, e);
   double SHb = 0, tpib = 0;
@@ -354,9 +385,12 @@ VHD void source_vjp(const Params& p, double tau, double e, double P, double PI, 
     sb = SHb * p.eta_coef;
     cetab += SHb * s;
   }
-  // s = (e + P) / T(e)
-  eb += sb / T - sb * (e + P) / (T * T) * eos_dTde(p, e, T);
-  Pb += sb / T;
+  if (!p.eos) {  // s = (e + P) / T(e)
+    eb += sb / T - sb * (e + P) / (T * T) * dT;
+    Pb += sb / T;
+  } else {
+    eb += sb * ds;
+  }
 }
 
 // Velocities and root of one cell (NEWEST10/root.h:212-424) in reverse.
@@ -372,7 +406,10 @@ VHD void update_vjp(const Params& p, double tn, const double T0[4], const double
                     const double pibo[10], double PIbo, double Vb[15])
 {
   const double t2 = tn * tn;
-  const double P = eos_p(p, e), a = eos_dpde(p, e);
+  // P(e) and Temp(e) as the forward stores them; a = dP/de of that P (the
+  // interpolant's derivative with the table), which is also f_e's
+  double P, a, Te, dT, se, dse;
+  eos_d(p, e, P, a, Te, dT, se, dse);
   const double N[3] = {T0[1] - pi10[1], T0[2] - pi10[2], T0[3] - pi10[3]};
   const double D = T0[0] - pi10[0] + P - PI;
   const double v[3] = {N[0] / D, N[1] / D, N[2] / D};
@@ -382,7 +419,7 @@ VHD void update_vjp(const Params& p, double tn, const double T0[4], const double
   double pib[10];
   for (int l = 0; l < 10; l++) pib[l] = pibo[l];
   double PIb = PIbo;
-  double eb = ebo + Pbo * a + Tempbo * eos_dTde(p, e, eos_T(p, e));
+  double eb = ebo + Pbo * a + Tempbo * dT;
   double Db = 0;
   for (int i = 0; i < 3; i++) {
     const double Nb = vb[i] / D;

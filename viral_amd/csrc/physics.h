@@ -185,6 +185,46 @@ __host__ __device__ __forceinline__ double s95p_T(double e)
   return r / GEVFM;
 }
 
+// e-derivatives of the s95p entropy-density and temperature fits, taken
+// branch by branch (the derivative of the branch s95p_s / s95p_T evaluates
+// at e; s95p_a is already that of s95p_p).  Used by the adjoint.
+__host__ __device__ __forceinline__ double s95p_ds(double e)
+{
+  const double x = e * GEVFM;
+  double r, dr;
+  if (x < S95_ES1) {
+    r = 12.2304 * dpow(x, 1.16849);
+    dr = 12.2304 * 1.16849 * dpow(x, 0.16849);
+  } else if (x > S95_ES1 && x < S95_ES2) {
+    r = 11.9279 * dpow(x, 1.15635);
+    dr = 11.9279 * 1.15635 * dpow(x, 0.15635);
+  } else if (x > S95_ES2 && x < S95_ES3) {
+    r = 0.0580578 + 11.833 * dpow(x, 1.16187);
+    dr = 11.833 * 1.16187 * dpow(x, 0.16187);
+  } else {
+    const double a4 = dpow(x, 4.54886), g = dexp(-2.72407e-11 * a4);
+    const double b1 = dpow(x, 1.18735), h = dexp(-0.00369624 * b1);
+    const double c0 = dpow(x, -0.128012);
+    const bool mid = x > S95_ES3 && x < S95_ES4;
+    r = 18.202 * x - (mid ? 62.021814 : 63.0218) - 4.85479 * g + 65.1272 * c0 * h -
+        (mid ? 4.75253 * dpow(x, -1.18423) : 0.0);
+    dr = 18.202 + 4.85479 * 2.72407e-11 * 4.54886 * g * a4 / x +
+         65.1272 * h * c0 * (-0.128012 / x - 0.00369624 * 1.18735 * b1 / x) +
+         (mid ? 4.75253 * 1.18423 * dpow(x, -2.18423) : 0.0);
+  }
+  // s = r^(3/4)
+  return 0.75 * dpow(r, -0.25) * dr * GEVFM;
+}
+
+__host__ __device__ __forceinline__ double s95p_dT(double e)
+{
+  const double x = e * GEVFM;
+  if (x < 0.5143939846236409) return 0.203054 * 0.30679 * dpow(x, 0.30679 - 1.0);
+  // T = (e + p(e)) / s(e)
+  const double P = s95p_p(e), a = s95p_a(e), s = s95p_s(e), ds = s95p_ds(e);
+  return ((1.0 + a) * s - (e + P) * ds) / (s * s);
+}
+
 // EOS(e), DPDE(e), FT(e), FS(e,P,T): NEWEST10/maindef.h:68-72 (conformal,
 // with x^(1/4) as sqrt(sqrt(x))) and :274-278 (s95p)
 __host__ __device__ __forceinline__ double eos_p(const Params& p, double e) { return p.eos ? s95p_p(e) : e / 3.0; }
@@ -219,6 +259,29 @@ __host__ __device__ __forceinline__ bool eos_tab_lookup(const EosTab& t, double 
   const double* b = t.v + ((long)oct * (t.K + 3) + q) * 4;
 #pragma unroll
   for (int c = 0; c < 4; c++) out[c] = fma(w0, b[c], fma(w1, b[4 + c], fma(w2, b[8 + c], w3 * b[12 + c])));
+  return true;
+}
+
+// e-derivatives of the interpolated (p, dp/de, T, s), for the adjoint: the
+// derivatives of the four cubic Lagrange weights times du/de = 2K 2^-E.
+__host__ __device__ __forceinline__ bool eos_tab_lookup_d(const EosTab& t, double e, double out[4], double dout[4])
+{
+  if (!eos_tab_lookup(t, e, out)) return false;
+  int E;
+  const double m = frexp(e, &E);
+  const int oct = E - 1 - t.emin;
+  const double u = (2.0 * m - 1.0) * t.K;
+  int q = (int)u;
+  if (q > t.K - 1) q = t.K - 1;
+  const double x = u - q, x2 = x * x;
+  const double d0 = -(3.0 * x2 - 6.0 * x + 2.0) * (1.0 / 6.0);
+  const double d1 = (3.0 * x2 - 4.0 * x - 1.0) * 0.5;
+  const double d2 = -(3.0 * x2 - 2.0 * x - 2.0) * 0.5;
+  const double d3 = (3.0 * x2 - 1.0) * (1.0 / 6.0);
+  const double dude = ldexp(2.0 * t.K, -E);
+  const double* b = t.v + ((long)oct * (t.K + 3) + q) * 4;
+#pragma unroll
+  for (int c = 0; c < 4; c++) dout[c] = (d0 * b[c] + d1 * b[4 + c] + d2 * b[8 + c] + d3 * b[12 + c]) * dude;
   return true;
 }
 

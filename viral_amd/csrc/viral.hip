@@ -24,6 +24,7 @@
 
 #include "../../include/viral.h"
 #include "adjoint.h"
+#include "eos_table.h"
 #include "physics.h"
 
 using namespace viral;
@@ -1250,7 +1251,8 @@ __global__ void __launch_bounds__(BLK) k_output(Geom g, const double* __restrict
 }
 
 // ---------------------------------------------------------------- adjoint kernels
-// Reverse of one Milne KT stage for one event (conformal EOS).  lam_out is
+// Reverse of one Milne KT stage for one event (conformal or s95p EOS, the
+// latter in closed form or, in fast mode, from the table).  lam_out is
 // the adjoint of the stage's output state (29 fields), lam_in receives the
 // adjoint of its input state.  See adjoint.h for the per-cell pieces.
 
@@ -2059,40 +2061,6 @@ static int capture_step(viral_ctx* ctx, int cur0)
   return VIRAL_OK;
 }
 
-// Host build of the tabulated s95p EOS (see EosTab in physics.h): octaves
-// 2^-13 .. 2^13 fm^-4 (1.2e-4 .. 8192 fm^-4, i.e. up to ~1600 GeV/fm^3;
-// below, the closed form's own exp(x)-1 cancellation is already ~1e-10, so
-// those energies keep it), 1024 intervals per octave, 4-point Lagrange
-// stencils (interpolation error ~1e-11 on the steepest branch); intervals whose
-// stencil [e_{q-1}, e_{q+2}] contains a branch point of s95p_p/a/s/T (or the
-// 1e-5 fm^-4 low-density switch) are flagged to use the closed form.
-static const int kTabEmin = -13, kTabNoct = 26, kTabK = 1024;
-
-static void build_eos_table(std::vector<double>& v, std::vector<unsigned char>& closed)
-{
-  const int K = kTabK, W = K + 3;
-  v.assign((size_t)kTabNoct * W * 4, 0.0);
-  closed.assign((size_t)kTabNoct * K, 0);
-  const double brk[] = {1e-5,
-                        S95_E1 / GEVFM, S95_E2 / GEVFM, S95_E3 / GEVFM, S95_E4 / GEVFM,
-                        S95_ES1 / GEVFM, S95_ES2 / GEVFM, S95_ES3 / GEVFM, S95_ES4 / GEVFM,
-                        0.5143939846236409 / GEVFM};
-  for (int o = 0; o < kTabNoct; o++) {
-    const double base = ldexp(1.0, kTabEmin + o);
-    for (int q = -1; q <= K + 1; q++) {
-      const double e = base * (1.0 + (double)q / K);
-      double out[4];
-      s95p_all(e, out);
-      for (int c = 0; c < 4; c++) v[((size_t)o * W + (q + 1)) * 4 + c] = out[c];
-    }
-    for (int q = 0; q < K; q++) {
-      const double lo = base * (1.0 + (double)(q - 1) / K), hi = base * (1.0 + (double)(q + 2) / K);
-      for (double b : brk)
-        if (b >= lo && b <= hi) closed[(size_t)o * K + q] = 1;
-    }
-  }
-}
-
 static EosTab host_tab()
 {
   static std::vector<double> v;
@@ -2896,8 +2864,8 @@ int viral_adjoint_begin(viral_ctx* ctx, int every)
   if (!ctx || every < 1) return fail(ctx, VIRAL_ERR_ARG, "bad adjoint recording request");
   if (ctx->ev.size() != 1) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint runs on single-event contexts");
   const Event& E = ctx->ev[0];
-  if (E.d.coords != VIRAL_MILNE || E.d.scheme != VIRAL_KT || E.d.eos != VIRAL_EOS_CONFORMAL)
-    return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint covers the Milne KT step with the conformal EOS");
+  if (E.d.coords != VIRAL_MILNE || E.d.scheme != VIRAL_KT)
+    return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint covers the Milne KT step");
   if ((E.d.quirks & VIRAL_Q_K0_BROADCAST) && !E.d.lbi && (E.g.k0 < E.g.kl || E.g.k0 >= E.g.kr))
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint needs k0 inside the interior");
   if (int r = viral_adjoint_end(ctx)) return r;
