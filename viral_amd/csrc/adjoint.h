@@ -22,7 +22,10 @@ namespace viral {
 //   dOut/dx_i = sum_c w_c dq_c/dx_i + (1/sum a) sum_c (q_c - Out) da_c/dx_i,
 //   da_c/dx_i = -2 a_c / (eps + b_c) db_c/dx_i,
 //   b_c = 13/12 A_c^2 + 1/4 C_c^2 (A_c, C_c linear in x).
-VHD void weno_grad(const double x[5], bool right, double g[5])
+// Returns the reconstruction itself (to rounding).  One reciprocal per
+// candidate and one for the weight sum: the adjoint needs accuracy, not the
+// forward's operation order.
+VHD double weno_grad(const double x[5], bool right, double g[5])
 {
   // linear forms of A_c and C_c (smoothness indicators), candidates q_c
   const double Ac[3][5] = {{0, 0, 1, -2, 1}, {0, 1, -2, 1, 0}, {1, -2, 1, 0, 0}};
@@ -30,7 +33,7 @@ VHD void weno_grad(const double x[5], bool right, double g[5])
   const double QL[3][5] = {{0, 0, 2, 5, -1}, {0, -1, 5, 2, 0}, {2, -7, 11, 0, 0}};
   const double QR[3][5] = {{0, 0, 11, -7, 2}, {0, 2, 5, -1, 0}, {-1, 5, 2, 0, 0}};
   const double dL[3] = {0.3, 0.6, 0.1}, dR[3] = {0.1, 0.6, 0.3};
-  double A[3], C[3], b[3], al[3], q[3], sum = 0;
+  double A[3], C[3], al[3], k2[3], q[3], sum = 0;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     A[c] = C[c] = q[c] = 0;
@@ -40,27 +43,33 @@ VHD void weno_grad(const double x[5], bool right, double g[5])
       C[c] += Cc[c][i] * x[i];
       q[c] += (right ? QR[c][i] : QL[c][i]) * x[i];
     }
-    q[c] = q[c] / 6.0;
-    b[c] = (13. / 12.) * (A[c] * A[c]) + 0.25 * (C[c] * C[c]);
-    const double t = WENO_EPS + b[c];
-    al[c] = (right ? dR[c] : dL[c]) / (t * t);
+    q[c] = q[c] * (1.0 / 6.0);
+    const double it = 1.0 / (WENO_EPS + (13. / 12.) * (A[c] * A[c]) + 0.25 * (C[c] * C[c]));
+    al[c] = (right ? dR[c] : dL[c]) * (it * it);
+    k2[c] = -2.0 * al[c] * it;  // da_c/db_c
     sum += al[c];
   }
-  double out = 0;
+  const double rs = 1.0 / sum;
+  double w[3], out = 0;
 #pragma unroll
-  for (int c = 0; c < 3; c++) out += (al[c] / sum) * q[c];
+  for (int c = 0; c < 3; c++) {
+    w[c] = al[c] * rs;
+    out += w[c] * q[c];
+  }
+  double r[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) r[c] = (q[c] - out) * k2[c] * rs;
 #pragma unroll
   for (int i = 0; i < 5; i++) {
-    double s1 = 0, s2 = 0;
+    double gi = 0;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const double w = al[c] / sum;
-      s1 += w * (right ? QR[c][i] : QL[c][i]) / 6.0;
       const double db = (13. / 6.) * A[c] * Ac[c][i] + 0.5 * C[c] * Cc[c][i];
-      s2 += (q[c] - out) * (-2.0 * al[c] / (WENO_EPS + b[c])) * db;
+      gi += w[c] * ((right ? QR[c][i] : QL[c][i]) * (1.0 / 6.0)) + r[c] * db;
     }
-    g[i] = s1 + s2 / sum;
+    g[i] = gi;
   }
+  return out;
 }
 
 // P(e), T(e), s(e) and their e-derivatives as the forward step evaluates
@@ -168,6 +177,7 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
   double ul[4];
   for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
   const double iu0 = 1.0 / u[0];
+  const double itpi = 1.0 / tpi;
   double divv = 0;
   for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
   double Dc[4][4];
@@ -227,7 +237,7 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
       double sigma = 0.5 * (G[m][n] + G[n][m]) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
       if (m == n) sigma = sigma - (1.0 / 3.0) * gi[m] * tr;
       const double pu = u[m] * PD[n] + u[n] * PD[m];
-      const double rel = -(pmn - 2.0 * SH * sigma) / tpi;
+      const double rel = -(pmn - 2.0 * SH * sigma) * itpi;
       const double rhs = rel - (4.0 / 3.0) * pmn * theta - pu;
       double chr = 0;
       if (milne) {
@@ -261,10 +271,10 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
       thetab += -(4.0 / 3.0) * pmn * rhsb;
       const double pub = -rhsb;
       // rel = -(pmn - 2 SH sigma) / tpi
-      pib[q] += -relb / tpi;
-      SHb += 2.0 * sigma * relb / tpi;
-      const double sigb = 2.0 * SH * relb / tpi;
-      tpib += (pmn - 2.0 * SH * sigma) / (tpi * tpi) * relb;
+      pib[q] += -relb * itpi;
+      SHb += 2.0 * sigma * relb * itpi;
+      const double sigb = 2.0 * SH * relb * itpi;
+      tpib += (pmn - 2.0 * SH * sigma) * (itpi * itpi) * relb;
       // pu = u^m PD^n + u^n PD^m
       ub[m] += pub * PD[n];
       PDb[n] += pub * u[m];
@@ -411,8 +421,8 @@ VHD void update_vjp(const Params& p, double tn, const double T0[4], const double
   double P, a, Te, dT, se, dse;
   eos_d(p, e, P, a, Te, dT, se, dse);
   const double N[3] = {T0[1] - pi10[1], T0[2] - pi10[2], T0[3] - pi10[3]};
-  const double D = T0[0] - pi10[0] + P - PI;
-  const double v[3] = {N[0] / D, N[1] / D, N[2] / D};
+  const double D = T0[0] - pi10[0] + P - PI, iD = 1.0 / D;
+  const double v[3] = {N[0] * iD, N[1] * iD, N[2] * iD};
   double vb[3] = {vbo[0], vbo[1], vbo[2]};
   gamma_vjp(v, t2, ubo, vb);
   double T0b[4] = {T0bo[0], T0bo[1], T0bo[2], T0bo[3]};
@@ -422,8 +432,8 @@ VHD void update_vjp(const Params& p, double tn, const double T0[4], const double
   double eb = ebo + Pbo * a + Tempbo * dT;
   double Db = 0;
   for (int i = 0; i < 3; i++) {
-    const double Nb = vb[i] / D;
-    Db -= vb[i] * v[i] / D;
+    const double Nb = vb[i] * iD;
+    Db -= vb[i] * v[i] * iD;
     T0b[i + 1] += Nb;
     pib[i + 1] -= Nb;
   }
@@ -434,12 +444,12 @@ VHD void update_vjp(const Params& p, double tn, const double T0[4], const double
   // implicit-function theorem at the root
   const double A = T0[0] - pi10[0];
   const double M2 = N[0] * N[0] + N[1] * N[1] + t2 * N[2] * N[2];
-  const double Q = A + P - PI;
-  const double fe = 1.0 - M2 * a / (Q * Q);
+  const double Q = A + P - PI, iQ = 1.0 / Q, iQ2 = iQ * iQ;
+  const double fe = 1.0 - M2 * a * iQ2;
   const double lam = -eb / fe;
-  const double Ab = lam * (-1.0 - M2 / (Q * Q));
-  const double M2b = lam / Q;
-  PIb += lam * M2 / (Q * Q);
+  const double Ab = lam * (-1.0 - M2 * iQ2);
+  const double M2b = lam * iQ;
+  PIb += lam * M2 * iQ2;
   T0b[0] += Ab;
   pib[0] -= Ab;
   const double gg[3] = {1.0, 1.0, t2};
@@ -448,7 +458,8 @@ VHD void update_vjp(const Params& p, double tn, const double T0[4], const double
     T0b[i + 1] += nb;
     pib[i + 1] -= nb;
   }
-  for (int l = 0; l < 4; l++) Vb[l] = T0b[l] / tn;
+  const double itn = 1.0 / tn;
+  for (int l = 0; l < 4; l++) Vb[l] = T0b[l] * itn;
   for (int l = 0; l < 10; l++) Vb[4 + l] = pib[l];
   Vb[14] = PIb;
 }

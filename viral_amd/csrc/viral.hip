@@ -1263,6 +1263,11 @@ __global__ void __launch_bounds__(BLK) k_adj_ghost(Geom g, Params p, double* __r
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!box_cell(interior_box(g), t, i, j, k)) return;
+  // only cells within BORDER of a face of the interior feed ghosts
+  const bool nx = i < g.il + VIRAL_BORDER || i >= g.ir - VIRAL_BORDER;
+  const bool ny = j < g.jl + VIRAL_BORDER || j >= g.jr - VIRAL_BORDER;
+  const bool nz = !p.lbi && (k == g.kl || k == g.kr - 1);
+  if (!(nx || ny || nz)) return;
   const long n = g.n, x = cidx(g, i, j, k);
   for (int q = 0; q < 28; q++) {
     const int f = c_bfields[q];
@@ -1299,20 +1304,22 @@ __global__ void __launch_bounds__(BLK) k_adj_pass(Geom g, const double* __restri
   }
 }
 
-// k0 broadcast in reverse (NEWEST10/root.h:426-449): one thread per (i,j)
-// line collects what the other slices took from slice k0.
+// k0 broadcast in reverse (NEWEST10/root.h:426-449): one wavefront per (i,j)
+// line; its lanes take the eta slices, each collects what its slices took
+// from slice k0, and a fixed-order wave reduction adds the sum to slice k0.
 __global__ void __launch_bounds__(BLK) k_adj_k0(Geom g, Params p, double tn, const double* __restrict__ so,
                                                 double* __restrict__ lam)
 {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long wv = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   const int ni = g.ir - g.il, nj = g.jr - g.jl;
-  if (t >= (long)ni * nj) return;
-  const int i = g.il + (int)(t / nj), j = g.jl + (int)(t % nj);
+  if (wv >= (long)ni * nj) return;  // whole wavefronts leave together
+  const int i = g.il + (int)(wv / nj), j = g.jl + (int)(wv % nj);
   const long n = g.n, x0 = cidx(g, i, j, g.k0);
   const double v0[3] = {so[VIRAL_F_VX * n + x0], so[VIRAL_F_VY * n + x0], so[VIRAL_F_VE * n + x0]};
   const double t2 = tn * tn;
   double vb[3] = {0, 0, 0};
-  for (int k = g.kl; k < g.kr; k++) {
+  for (int k = g.kl + lane; k < g.kr; k += 64) {
     if (k == g.k0) continue;
     const long x = cidx(g, i, j, k);
     const double ub[4] = {lam[VIRAL_F_U0 * n + x], lam[VIRAL_F_U1 * n + x], lam[VIRAL_F_U2 * n + x],
@@ -1322,7 +1329,10 @@ __global__ void __launch_bounds__(BLK) k_adj_k0(Geom g, Params p, double tn, con
     for (int c = 0; c < 3; c++) lam[(VIRAL_F_VX + c) * n + x] = 0.0;
     for (int c = 0; c < 4; c++) lam[(VIRAL_F_U0 + c) * n + x] = 0.0;
   }
-  for (int c = 0; c < 3; c++) lam[(VIRAL_F_VX + c) * n + x0] += vb[c];
+  for (int o = 32; o > 0; o >>= 1)
+    for (int c = 0; c < 3; c++) vb[c] += __shfl_down(vb[c], o);
+  if (lane == 0)
+    for (int c = 0; c < 3; c++) lam[(VIRAL_F_VX + c) * n + x0] += vb[c];
 }
 
 // Update, root and sources of one interior cell in reverse.  si / so: the
@@ -1451,6 +1461,8 @@ __global__ void __launch_bounds__(BLK) k_adj_update(Geom g, Params p, StageArgs 
 // derivatives du[l][d] into contributions to its five stencil cells,
 // cbuf row (l*3 + d)*5 + o for the cell at offset o-2.  k_adj_du: every cell
 // sums what the centres whose stencils hold it contributed.
+constexpr int ADJ_DROWS = 4 * 3 * 5;
+
 __global__ void __launch_bounds__(BLK) k_adj_du_src(Geom g, Params p, const double* __restrict__ si,
                                                     const double* __restrict__ dubuf, double* __restrict__ cbuf)
 {
@@ -1503,164 +1515,183 @@ __global__ void __launch_bounds__(BLK) k_adj_du(Geom g, Params p, const double* 
   for (int l = 0; l < 4; l++) li[(VIRAL_F_U0 + l) * n + x] += acc[l];
 }
 
-// centred flux F_l, Var V_l (row l) and eigenvalue of cell y, direction d
-__device__ __forceinline__ void cell_FVA(const Geom& g, const Params& p, int d, double tau,
-                                         const double* __restrict__ st, long y, int l, double& F, double& V)
-{
-  const long n = g.n;
-  const int row = d + 1;
-  const double u0 = st[VIRAL_F_U0 * n + y], ud = st[(VIRAL_F_U1 + d) * n + y];
-  if (l < 4) {
-    const double e = st[VIRAL_F_EN * n + y], P = st[VIRAL_F_P * n + y], PI = st[VIRAL_F_BULK * n + y];
-    const double um = st[(VIRAL_F_U0 + l) * n + y];
-    const double piv = st[(VIRAL_F_PI0 + pidx(row, l)) * n + y];
-    const double pp = d == 2 ? (P - PI) / (tau * tau) : (P - PI);
-    const double v = l == row ? piv + pp + ud * um * (e + P - PI) : piv + ud * um * (e + P - PI);
-    F = tau * v;
-    V = tau * st[(long)l * n + y];
-  } else {
-    const double f = st[(long)l * n + y];  // pi[l-4] or PI: Var row l is state field l
-    F = (f * ud) / u0;
-    V = f;
-  }
-}
+// KT flux of direction D in reverse on an LDS line tile: the face pass (each
+// face's WENO gradients turned into contributions to its six stencil cells)
+// and the cell pass (each cell's sum of them, mapped onto its state fields)
+// fused, with no global intermediate.  A block takes TKA lines along D and a window of WA
+// consecutive cells of each.  Per row, every window cell computes its centred
+// flux and Var once into LDS; every face of the window (WA - 5 of them)
+// writes its six stencil-cell contributions to LDS; every own cell (the
+// WA - 10 cells whose six faces all lie in the window) sums them in a fixed
+// face order.  Own cells cover the whole line, ghosts included, and tiles own
+// disjoint cells, so each cell's state adjoint is updated by one thread.
+// Slots map to cells with the contiguous (eta) index fastest.
+constexpr int WA = 64, TKA = 4, NTA = WA * TKA;
+constexpr int OWNA = WA - 10, NFA = WA - 5;
 
-__device__ __forceinline__ double cell_A(const Geom& g, const Params& p, int d, const double* __restrict__ st, long y,
-                                         bool live)
+template <int D>
+__global__ void __launch_bounds__(NTA) k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si,
+                                                  const double* __restrict__ resb, double* __restrict__ li)
 {
-  return live ? st[(VIRAL_F_U1 + d) * g.n + y] / st[VIRAL_F_U0 * g.n + y] : 0.0;
-}
-
-// Face pass of the KT flux in reverse (NEWEST10/hydroKT.h:444-506).  For face
-// c+1/2: fbar = d Result / d f from the adjacent cells' Result adjoints; the
-// face speed a = max(|A_L|, |A_R|) with its adjoint on A_L or A_R; then for
-// every row the WENO gradients of F_L, F_R, V_L, V_R turned into the
-// contributions to the six stencil cells c-2..c+3:
-//   cF[o] = fbar/2 (dF_L/dF_o + dF_R/dF_o),  cV[o] = a fbar/2 (dV_L/dV_o - dV_R/dV_o),
-//   cA[o] = Abar_L dA_L/dA_o + Abar_R dA_R/dA_o.
-// cbuf rows: (l*2 + 0|1)*6 + o for F|V of row l, 180 + o for A.
-constexpr int ADJ_CROWS = 186;
-
-__global__ void __launch_bounds__(BLK) k_adj_face(Geom g, Params p, int d, double tau, const double* __restrict__ si,
-                                                  const double* __restrict__ resb, double* __restrict__ cbuf)
-{
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!box_cell(face_box(g, d), t, i, j, k)) return;
-  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
+  __shared__ double sA[TKA][WA];
+  __shared__ double sF[TKA][WA];
+  __shared__ double sV[TKA][WA];
+  __shared__ double cF[TKA][NFA][6];
+  __shared__ double cV[TKA][NFA][6];
+  const int t = threadIdx.x;
+  const int kk = D == 2 ? t / WA : t % TKA;  // line of the tile
+  const int wi = D == 2 ? t % WA : t / TKA;  // window slot; face slot wi; own cell slot wi
+  const int GD = D == 0 ? g.gx : D == 1 ? g.gy : g.gz;
+  const int lo = D == 0 ? g.il : D == 1 ? g.jl : g.kl;
+  const int hi = D == 0 ? g.ir : D == 1 ? g.jr : g.kr;
+  const int nseg = (GD + OWNA - 1) / OWNA;
+  const int seg = (int)(blockIdx.x % nseg);
+  const long lgrp = blockIdx.x / nseg;
+  const int c0 = seg * OWNA;
+  const int nb = D == 2 ? g.jr - g.jl : g.kr - g.kl;
+  const int na = D == 0 ? g.jr - g.jl : g.ir - g.il;
+  const long ln = lgrp * TKA + kk;
+  const bool line_ok = ln < (long)na * nb;
+  const int la = line_ok ? (int)(ln / nb) : 0, lb = line_ok ? (int)(ln % nb) : 0;
+  auto at = [&](int cd) -> long {
+    if (D == 0) return cidx(g, cd, g.jl + la, g.kl + lb);
+    if (D == 1) return cidx(g, g.il + la, cd, g.kl + lb);
+    return cidx(g, g.il + la, g.jl + lb, cd);
+  };
+  const long n = g.n, sd = dstride(g, D);
   const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
-  const double h = d == 0 ? p.dx : d == 1 ? p.dy : p.deta;
-  const int pos = d == 0 ? i : d == 1 ? j : k;
-  const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
-  const bool inc = pos >= lo && pos < hi, inc1 = pos + 1 >= lo && pos + 1 < hi;
-  double A[6];
-  for (int o = 0; o < 6; o++) {
-    const int q = pos + o - 2;
-    const int ii = d == 0 ? q : i, jj = d == 1 ? q : j, kk = d == 2 ? q : k;
-    A[o] = cell_A(g, p, d, si, x + (o - 2) * s, interior(g, ii, jj, kk) || !q1);
-  }
-  const double aL = weno_L(A[0], A[1], A[2], A[3], A[4]);
-  const double aR = weno_R(A[1], A[2], A[3], A[4], A[5]);
-  double ml = 0, mr = 0;
-  const bool live = !(q1 && pos < lo);
-  if (live) {
-    if (fabs(aL) > ml) ml = fabs(aL);
-    if (fabs(aR) > mr) mr = fabs(aR);
-  }
-  const double amax = fmax(ml, mr);
-  // pass 1: the adjoint of the face speed needs V_R - V_L of every row
-  double abar = 0;
-  for (int l = 0; l < SV; l++) {
-    const double fb = (inc ? -resb[l * n + x] / h : 0.0) + (inc1 ? resb[l * n + x + s] / h : 0.0);
-    if (fb == 0.0) continue;
-    double Vs[6];
-    for (int o = 0; o < 6; o++) {
-      double F;
-      cell_FVA(g, p, d, tau, si, x + (o - 2) * s, l, F, Vs[o]);
-    }
-    abar += -0.5 * fb * (weno_R(Vs[1], Vs[2], Vs[3], Vs[4], Vs[5]) - weno_L(Vs[0], Vs[1], Vs[2], Vs[3], Vs[4]));
-  }
-  double AbL = 0, AbR = 0;
-  if (live) {
-    if (ml >= mr) AbL = abar * (aL > 0 ? 1.0 : aL < 0 ? -1.0 : 0.0);
-    else AbR = abar * (aR > 0 ? 1.0 : aR < 0 ? -1.0 : 0.0);
-  }
-  {
-    double gL[5], gR[5];
-    weno_grad(A, false, gL);
-    weno_grad(A + 1, true, gR);
-    for (int o = 0; o < 6; o++)
-      cbuf[(long)(180 + o) * n + x] = (o < 5 ? AbL * gL[o] : 0.0) + (o > 0 ? AbR * gR[o - 1] : 0.0);
-  }
-  // pass 2: stencil-cell contributions of every row
-  for (int l = 0; l < SV; l++) {
-    const double fb = (inc ? -resb[l * n + x] / h : 0.0) + (inc1 ? resb[l * n + x + s] / h : 0.0);
-    double* cF = cbuf + (long)(l * 2) * 6 * n + x;
-    double* cV = cbuf + (long)(l * 2 + 1) * 6 * n + x;
-    if (fb == 0.0) {
-      for (int o = 0; o < 6; o++) cF[o * n] = cV[o * n] = 0.0;
-      continue;
-    }
-    double F[6], V[6];
-    for (int o = 0; o < 6; o++) cell_FVA(g, p, d, tau, si, x + (o - 2) * s, l, F[o], V[o]);
-    double gFL[5], gFR[5], gVL[5], gVR[5];
-    weno_grad(F, false, gFL);
-    weno_grad(F + 1, true, gFR);
-    weno_grad(V, false, gVL);
-    weno_grad(V + 1, true, gVR);
-    const double wf = 0.5 * fb, wv = 0.5 * amax * fb;
-    for (int o = 0; o < 6; o++) {
-      cF[o * n] = wf * ((o < 5 ? gFL[o] : 0.0) + (o > 0 ? gFR[o - 1] : 0.0));
-      cV[o * n] = wv * ((o < 5 ? gVL[o] : 0.0) - (o > 0 ? gVR[o - 1] : 0.0));
-    }
-  }
-}
+  const int row = D + 1;
+  const double h = D == 0 ? p.dx : D == 1 ? p.dy : p.deta, ih = 1.0 / h;
 
-// Cell pass of the KT flux in reverse: each cell of a line sums what the (up
-// to six) faces whose stencils hold it contributed, then maps the adjoints of
-// its F_l, V_l and eigenvalue onto its state fields.
-__global__ void __launch_bounds__(BLK) k_adj_flux(Geom g, Params p, int d, double tau, const double* __restrict__ si,
-                                                  const double* __restrict__ cbuf, double* __restrict__ li)
-{
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  // the lines of direction d: other two coordinates interior, all G cells along d
-  Box b{g.il, g.jl, g.kl, g.ir - g.il, g.jr - g.jl, g.kr - g.kl};
-  if (d == 0) { b.i0 = 0; b.ni = g.gx; }
-  if (d == 1) { b.j0 = 0; b.nj = g.gy; }
-  if (d == 2) { b.k0 = 0; b.nk = g.gz; }
-  int i, j, k;
-  if (!box_cell(b, t, i, j, k)) return;
-  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
-  const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
-  const int pos = d == 0 ? i : d == 1 ? j : k;
-  const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
-  const int p0 = max(pos - 3, lo - 1), p1 = min(pos + 2, hi - 1);  // faces p+1/2 touching this cell
-  double Fb[SV], Vb[SV], Ab = 0;
-  for (int l = 0; l < SV; l++) Fb[l] = Vb[l] = 0;
-  for (int pf = p0; pf <= p1; pf++) {
-    const long xf = x + (long)(pf - pos) * s;
-    const int o = pos - pf + 2;  // this cell's slot in the face's stencil c-2..c+3
-    Ab += cbuf[(long)(180 + o) * n + xf];
-    for (int l = 0; l < SV; l++) {
-      Fb[l] += cbuf[((long)(l * 2) * 6 + o) * n + xf];
-      Vb[l] += cbuf[((long)(l * 2 + 1) * 6 + o) * n + xf];
+  // window cell of this thread and its common terms
+  const int cw = c0 - 5 + wi;
+  const bool w_ok = line_ok && cw >= 0 && cw < GD;
+  const long xw = w_ok ? at(cw) : 0;
+  double u0 = 1.0, ud = 0.0, ww = 0.0, pp = 0.0;
+  if (w_ok) {
+    const double e = si[VIRAL_F_EN * n + xw], P = si[VIRAL_F_P * n + xw], PI = si[VIRAL_F_BULK * n + xw];
+    u0 = si[VIRAL_F_U0 * n + xw];
+    ud = si[(VIRAL_F_U1 + D) * n + xw];
+    ww = e + P - PI;
+    pp = D == 2 ? (P - PI) / (tau * tau) : (P - PI);
+  }
+  sA[kk][wi] = w_ok && ((cw >= lo && cw < hi) || !q1) ? ud / u0 : 0.0;
+  // face f + 1/2 of this thread: stencil cells f-2..f+3 = window slots wi..wi+5
+  const int f = c0 - 3 + wi;
+  const bool f_ok = line_ok && wi < NFA && f >= lo - 1 && f <= hi - 1;
+  const long xf = f_ok ? at(f) : 0;
+  const bool inc = f >= lo && f < hi, inc1 = f + 1 >= lo && f + 1 < hi;
+  __syncthreads();
+  double A[6], aL = 0, aR = 0, ml = 0, mr = 0, amax = 0;
+  if (f_ok) {
+    for (int o = 0; o < 6; o++) A[o] = sA[kk][wi + o];
+    aL = weno_L(A[0], A[1], A[2], A[3], A[4]);
+    aR = weno_R(A[1], A[2], A[3], A[4], A[5]);
+    if (!(q1 && f < lo)) {
+      ml = fabs(aL);
+      mr = fabs(aR);
+    }
+    amax = fmax(ml, mr);
+  }
+  // own cell of this thread
+  const int c = c0 + wi;
+  const bool o_ok = line_ok && wi < OWNA && c < GD;
+  double Fb[SV], Vb[SV];
+  for (int q = 0; q < SV; q++) Fb[q] = Vb[q] = 0.0;
+  double abar = 0.0;
+  for (int l = 0; l < SV; l++) {
+    // centred flux and Var of row l (cell_FVA)
+    double Fw = 0.0, Vw = 0.0;
+    if (w_ok) {
+      if (l < 4) {
+        const double um = si[(VIRAL_F_U0 + l) * n + xw];
+        const double piv = si[(VIRAL_F_PI0 + pidx(row, l)) * n + xw];
+        const double v = l == row ? piv + pp + ud * um * ww : piv + ud * um * ww;
+        Fw = tau * v;
+        Vw = tau * si[(long)l * n + xw];
+      } else {
+        const double fv = si[(long)l * n + xw];
+        Fw = (fv * ud) / u0;
+        Vw = fv;
+      }
+    }
+    sF[kk][wi] = Fw;
+    sV[kk][wi] = Vw;
+    __syncthreads();
+    if (wi < NFA) {
+      double cf[6] = {0, 0, 0, 0, 0, 0}, cv[6] = {0, 0, 0, 0, 0, 0};
+      const double fb = f_ok ? (inc ? -resb[l * n + xf] * ih : 0.0) + (inc1 ? resb[l * n + xf + sd] * ih : 0.0) : 0.0;
+      if (fb != 0.0) {
+        double Vs[6], Fs[6], gL[5], gR[5];
+        for (int o = 0; o < 6; o++) {
+          Vs[o] = sV[kk][wi + o];
+          Fs[o] = sF[kk][wi + o];
+        }
+        const double VL = weno_grad(Vs, false, gL);
+        const double VR = weno_grad(Vs + 1, true, gR);
+        abar += -0.5 * fb * (VR - VL);
+        const double wv = 0.5 * amax * fb;
+        for (int o = 0; o < 6; o++) cv[o] = wv * ((o < 5 ? gL[o] : 0.0) - (o > 0 ? gR[o - 1] : 0.0));
+        weno_grad(Fs, false, gL);
+        weno_grad(Fs + 1, true, gR);
+        const double wf = 0.5 * fb;
+        for (int o = 0; o < 6; o++) cf[o] = wf * ((o < 5 ? gL[o] : 0.0) + (o > 0 ? gR[o - 1] : 0.0));
+      }
+      for (int o = 0; o < 6; o++) {
+        cF[kk][wi][o] = cf[o];
+        cV[kk][wi][o] = cv[o];
+      }
+    }
+    __syncthreads();
+    if (o_ok) {
+      double sf = 0.0, sv = 0.0;
+      for (int fi = wi; fi <= wi + 5; fi++) {  // faces c-3 .. c+2, the cell at offset wi + 5 - fi
+        sf += cF[kk][fi][wi + 5 - fi];
+        sv += cV[kk][fi][wi + 5 - fi];
+      }
+#pragma unroll
+      for (int q = 0; q < SV; q++)
+        if (q == l) {
+          Fb[q] = sf;
+          Vb[q] = sv;
+        }
     }
   }
-  const bool live = interior(g, i, j, k) || !q1;
+  // the face speed: a = max(|A_L|, |A_R|), adjoint on the branch taken
+  __syncthreads();
+  if (wi < NFA) {
+    double cA[6] = {0, 0, 0, 0, 0, 0};
+    if (f_ok && !(q1 && f < lo)) {
+      double AbL = 0, AbR = 0;
+      if (ml >= mr) AbL = abar * (aL > 0 ? 1.0 : aL < 0 ? -1.0 : 0.0);
+      else AbR = abar * (aR > 0 ? 1.0 : aR < 0 ? -1.0 : 0.0);
+      double gL[5], gR[5];
+      weno_grad(A, false, gL);
+      weno_grad(A + 1, true, gR);
+      for (int o = 0; o < 6; o++) cA[o] = (o < 5 ? AbL * gL[o] : 0.0) + (o > 0 ? AbR * gR[o - 1] : 0.0);
+    }
+    for (int o = 0; o < 6; o++) cF[kk][wi][o] = cA[o];
+  }
+  __syncthreads();
+  if (!o_ok) return;
+  double Ab = 0.0;
+  for (int fi = wi; fi <= wi + 5; fi++) Ab += cF[kk][fi][wi + 5 - fi];
+  const long x = at(c);
+  const bool live = (c >= lo && c < hi) || !q1;
   double u[4], pi[10];
-  for (int l = 0; l < 4; l++) u[l] = si[(VIRAL_F_U0 + l) * n + x];
-  for (int l = 0; l < 10; l++) pi[l] = si[(VIRAL_F_PI0 + l) * n + x];
+  for (int m = 0; m < 4; m++) u[m] = si[(VIRAL_F_U0 + m) * n + x];
+  for (int m = 0; m < 10; m++) pi[m] = si[(VIRAL_F_PI0 + m) * n + x];
   double eb = 0, Pb = 0, PIb = 0, ub[4] = {0, 0, 0, 0}, pib[10] = {0}, Tb[4] = {0, 0, 0, 0};
-  flux_cell_vjp(d, tau, si[VIRAL_F_EN * n + x], si[VIRAL_F_P * n + x], si[VIRAL_F_BULK * n + x], u, pi, Fb, Vb,
+  flux_cell_vjp(D, tau, si[VIRAL_F_EN * n + x], si[VIRAL_F_P * n + x], si[VIRAL_F_BULK * n + x], u, pi, Fb, Vb,
                 live ? Ab : 0.0, eb, Pb, PIb, ub, pib, Tb);
   li[VIRAL_F_EN * n + x] += eb;
   li[VIRAL_F_P * n + x] += Pb;
   li[VIRAL_F_BULK * n + x] += PIb;
-  for (int l = 0; l < 4; l++) {
-    li[(VIRAL_F_U0 + l) * n + x] += ub[l];
-    li[(VIRAL_F_T00 + l) * n + x] += Tb[l];
+  for (int m = 0; m < 4; m++) {
+    li[(VIRAL_F_U0 + m) * n + x] += ub[m];
+    li[(VIRAL_F_T00 + m) * n + x] += Tb[m];
   }
-  for (int l = 0; l < 10; l++) li[(VIRAL_F_PI0 + l) * n + x] += pib[l];
+  for (int m = 0; m < 10; m++) li[(VIRAL_F_PI0 + m) * n + x] += pib[m];
 }
 
 }  // namespace
@@ -2914,7 +2945,7 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   double* vb2 = dalloc(SV * n);
   double* resb = dalloc(SV * n);
   double* dubuf = dalloc(12 * n);
-  double* fbuf = dalloc((size_t)ADJ_CROWS * n);
+  double* fbuf = dalloc((size_t)ADJ_DROWS * n);
   double* partial = dalloc(2 * (size_t)nbi);
   double* saved = dalloc(NF * n);
   double* snaps = dalloc((size_t)nsnap * NF * n);
@@ -2964,7 +2995,7 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         hipLaunchKernelGGL(k_adj_ghost, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, lamA);
         if (k0b) {
           const long nl = (long)(g.ir - g.il) * (g.jr - g.jl);
-          hipLaunchKernelGGL(k_adj_k0, dim3(nblocks(nl)), dim3(BLK), 0, ctx->stream, g, p, a.tau_new, so, lamA);
+          hipLaunchKernelGGL(k_adj_k0, dim3(nblocks(nl * 64)), dim3(BLK), 0, ctx->stream, g, p, a.tau_new, so, lamA);
         }
         HIPCHK(hipMemsetAsync(lamB, 0, fbytes, ctx->stream));
         hipLaunchKernelGGL(k_adj_pass, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, lamA, lamB);
@@ -2973,13 +3004,15 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         hipLaunchKernelGGL(k_adj_du_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, si, dubuf, fbuf);
         hipLaunchKernelGGL(k_adj_du, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, p, fbuf, lamB);
         for (int d = 0; d < ndir; d++) {
-          hipLaunchKernelGGL(k_adj_face, dim3(nblocks(box_count(face_box(g, d)))), dim3(BLK), 0, ctx->stream, g, p, d,
-                             a.tau_src, si, resb, fbuf);
-          Box b = interior_box(g);
-          long nline = (long)b.ni * b.nj * b.nk / (d == 0 ? b.ni : d == 1 ? b.nj : b.nk) *
-                       (d == 0 ? g.gx : d == 1 ? g.gy : g.gz);
-          hipLaunchKernelGGL(k_adj_flux, dim3(nblocks(nline)), dim3(BLK), 0, ctx->stream, g, p, d, a.tau_src, si, fbuf,
-                             lamB);
+          // lines of direction d (interior in the other two), TKA per block, cut into OWNA-cell segments
+          const long nlines = d == 0 ? (long)(g.jr - g.jl) * (g.kr - g.kl)
+                              : d == 1 ? (long)(g.ir - g.il) * (g.kr - g.kl)
+                                       : (long)(g.ir - g.il) * (g.jr - g.jl);
+          const int GD = d == 0 ? g.gx : d == 1 ? g.gy : g.gz;
+          const unsigned nt = (unsigned)(((nlines + TKA - 1) / TKA) * ((GD + OWNA - 1) / OWNA));
+          if (d == 0) hipLaunchKernelGGL(k_adj_tile<0>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, lamB);
+          if (d == 1) hipLaunchKernelGGL(k_adj_tile<1>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, lamB);
+          if (d == 2) hipLaunchKernelGGL(k_adj_tile<2>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, lamB);
         }
         std::swap(lamA, lamB);
       }
