@@ -180,39 +180,45 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
   const double itpi = 1.0 / tpi;
   double divv = 0;
   for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
-  double Dc[4][4];
-  for (int a = 0; a < 4; a++)
-    for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
   const double itau = milne ? 1.0 / tau : 0.0;
-  if (milne) {
-    Dc[0][3] = Dc[0][3] + tau * u[3];
-    Dc[3][0] = Dc[3][0] + itau * u[3];
-    Dc[3][3] = Dc[3][3] + itau * u[0];
-  }
-  const double theta = Dc[0][0] + Dc[1][1] + Dc[2][2] + Dc[3][3];
+  // Dc = du with the Christoffel part, G^{ab} = g^{aa} Dc[b][a]; evaluated on
+  // the fly (the loops are unrolled, so the index tests fold away)
+  auto Dc = [&](int a, int b) -> double {
+    double v = du[a][b];
+    if (milne) {
+      if (a == 0 && b == 3) v = v + tau * u[3];
+      if (a == 3 && b == 0) v = v + itau * u[3];
+      if (a == 3 && b == 3) v = v + itau * u[0];
+    }
+    return v;
+  };
+  auto G = [&](int a, int b) -> double { return gi[a] * Dc(b, a); };
+  const double theta = Dc(0, 0) + Dc(1, 1) + Dc(2, 2) + Dc(3, 3);
   double Dul[4];
+#pragma unroll
   for (int a = 0; a < 4; a++) {
     double s = 0;
-    for (int b = 0; b < 4; b++) s += u[b] * Dc[a][b];
+#pragma unroll
+    for (int b = 0; b < 4; b++) s += u[b] * Dc(a, b);
     Dul[a] = g[a] * s;
   }
-  double G[4][4];
-  for (int a = 0; a < 4; a++)
-    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
   double Gu[4], uG[4], H[4];
+#pragma unroll
   for (int m = 0; m < 4; m++) {
     double s1 = 0, s2 = 0;
+#pragma unroll
     for (int b = 0; b < 4; b++) {
-      s1 += G[m][b] * ul[b];
-      s2 += ul[b] * G[b][m];
+      s1 += G(m, b) * ul[b];
+      s2 += ul[b] * G(b, m);
     }
     Gu[m] = s1;
     uG[m] = s2;
   }
   double uGu = 0, trg = 0;
+#pragma unroll
   for (int a = 0; a < 4; a++) {
     uGu += ul[a] * Gu[a];
-    trg += g[a] * G[a][a];
+    trg += g[a] * G(a, a);
   }
   const double tr = trg - uGu;
   for (int m = 0; m < 4; m++) H[m] = Gu[m] + uG[m];
@@ -225,16 +231,21 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
   }
   // ---- reverse sweep
   double ulb[4] = {0, 0, 0, 0}, Hb[4] = {0, 0, 0, 0}, PDb[4] = {0, 0, 0, 0}, Dulb[4] = {0, 0, 0, 0};
-  double Gb[4][4], Dcb[4][4];
+  // the adjoint of Dc; that of G^{ab} = g^{aa} Dc[b][a] lands on Dc[b][a]
+  double Dcb[4][4];
+#pragma unroll
   for (int a = 0; a < 4; a++)
-    for (int b = 0; b < 4; b++) Gb[a][b] = Dcb[a][b] = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) Dcb[a][b] = 0;
   double iu0b = 0, divvb = 0, thetab = 0, c3b = 0, trb = 0;
+#pragma unroll
   for (int m = 0; m < 4; m++)
+#pragma unroll
     for (int n = m; n < 4; n++) {
       const int q = pidx(m, n);
       const double o = ob[q];
       const double pmn = pi10[q];
-      double sigma = 0.5 * (G[m][n] + G[n][m]) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
+      double sigma = 0.5 * (G(m, n) + G(n, m)) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
       if (m == n) sigma = sigma - (1.0 / 3.0) * gi[m] * tr;
       const double pu = u[m] * PD[n] + u[n] * PD[m];
       const double rel = -(pmn - 2.0 * SH * sigma) * itpi;
@@ -281,8 +292,8 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
       ub[n] += pub * PD[m];
       PDb[m] += pub * u[n];
       // sigma
-      Gb[m][n] += 0.5 * sigb;
-      Gb[n][m] += 0.5 * sigb;
+      Dcb[n][m] += gi[m] * (0.5 * sigb);
+      Dcb[m][n] += gi[n] * (0.5 * sigb);
       ub[n] -= 0.5 * sigb * H[m];
       Hb[m] -= 0.5 * sigb * u[n];
       ub[m] -= 0.5 * sigb * H[n];
@@ -304,23 +315,25 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
   for (int m = 0; m < 4; m++) Gub[m] = uGb[m] = Hb[m];
   const double trgb = trb;
   uGub -= trb;
-  for (int a = 0; a < 4; a++) Gb[a][a] += g[a] * trgb;
+  for (int a = 0; a < 4; a++) Dcb[a][a] += gi[a] * (g[a] * trgb);
   for (int a = 0; a < 4; a++) {
     ulb[a] += uGub * Gu[a];
     Gub[a] += uGub * ul[a];
   }
+#pragma unroll
   for (int m = 0; m < 4; m++)
+#pragma unroll
     for (int b = 0; b < 4; b++) {
-      Gb[m][b] += Gub[m] * ul[b];
-      ulb[b] += Gub[m] * G[m][b];
-      ulb[b] += uGb[m] * G[b][m];
-      Gb[b][m] += uGb[m] * ul[b];
+      Dcb[b][m] += gi[m] * (Gub[m] * ul[b]);
+      ulb[b] += Gub[m] * G(m, b);
+      ulb[b] += uGb[m] * G(b, m);
+      Dcb[m][b] += gi[b] * (uGb[m] * ul[b]);
     }
+#pragma unroll
   for (int a = 0; a < 4; a++)
-    for (int b = 0; b < 4; b++) Dcb[b][a] += gi[a] * Gb[a][b];
-  for (int a = 0; a < 4; a++)
+#pragma unroll
     for (int b = 0; b < 4; b++) {
-      ub[b] += g[a] * Dulb[a] * Dc[a][b];
+      ub[b] += g[a] * Dulb[a] * Dc(a, b);
       Dcb[a][b] += g[a] * Dulb[a] * u[b];
     }
   for (int a = 0; a < 4; a++) Dcb[a][a] += thetab;

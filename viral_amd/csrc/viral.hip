@@ -1335,74 +1335,87 @@ __global__ void __launch_bounds__(BLK) k_adj_k0(Geom g, Params p, double tn, con
     for (int c = 0; c < 3; c++) lam[(VIRAL_F_VX + c) * n + x0] += vb[c];
 }
 
-// Update, root and sources of one interior cell in reverse.  si / so: the
-// stage's input and output states; vb0..vb2: adjoints of the RK registers
-// Var0..Var2; resb: adjoint of Result (= of L), 15 rows; dubuf: adjoint of the
-// spatial velocity gradients du[l][1..3] (12 rows) for k_adj_du; partial:
-// per-block sums of the adjoints of eta_coef and tpi_coef.
-__global__ void __launch_bounds__(BLK) k_adj_update(Geom g, Params p, StageArgs a, const double* __restrict__ si,
-                                                    const double* __restrict__ so, const double* __restrict__ lo,
-                                                    double* __restrict__ li, double* __restrict__ vb0,
-                                                    double* __restrict__ vb1, double* __restrict__ vb2,
-                                                    double* __restrict__ resb, double* __restrict__ dubuf,
-                                                    double* __restrict__ partial)
+// Update and root of one interior cell in reverse.  so / lo: the stage's
+// output state and its adjoint; vb0..vb2: adjoints of the RK registers
+// Var0..Var2; writes resb, the adjoint of L = Result + Source (15 rows), for
+// k_adj_src and the flux pass.
+__global__ void __launch_bounds__(BLK) k_adj_root(Geom g, Params p, StageArgs a, const double* __restrict__ so,
+                                                  const double* __restrict__ lo, double* __restrict__ li,
+                                                  double* __restrict__ vb0, double* __restrict__ vb1,
+                                                  double* __restrict__ vb2, double* __restrict__ resb)
+{
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  const long n = g.n, x = cidx(g, i, j, k);
+  // prevu = stage 0 ? u : prevu
+  for (int l = 0; l < 4; l++) {
+    const double pb = lo[(VIRAL_F_PU0 + l) * n + x];
+    if (a.stage == 0) li[(VIRAL_F_U0 + l) * n + x] += pb;
+    else li[(VIRAL_F_PU0 + l) * n + x] += pb;
+  }
+  // velocities and root
+  double T0[4], pio[10], T0b[4], pibo[10], vbo[3], ubo[4];
+  for (int l = 0; l < 4; l++) {
+    T0[l] = so[(VIRAL_F_T00 + l) * n + x];
+    T0b[l] = lo[(VIRAL_F_T00 + l) * n + x];
+    ubo[l] = lo[(VIRAL_F_U0 + l) * n + x];
+  }
+  for (int l = 0; l < 10; l++) {
+    pio[l] = so[(VIRAL_F_PI0 + l) * n + x];
+    pibo[l] = lo[(VIRAL_F_PI0 + l) * n + x];
+  }
+  for (int c = 0; c < 3; c++) vbo[c] = lo[(VIRAL_F_VX + c) * n + x];
+  double Vb[SV];
+  update_vjp(p, a.tau_new, T0, pio, so[VIRAL_F_BULK * n + x], so[VIRAL_F_EN * n + x], lo[VIRAL_F_EN * n + x],
+             lo[VIRAL_F_P * n + x], lo[VIRAL_F_TEMP * n + x], vbo, ubo, T0b, pibo, lo[VIRAL_F_BULK * n + x], Vb);
+  // RK combination (NEWEST10/time.h:2-98) in reverse
+  for (int l = 0; l < SV; l++) {
+    const long y = l * n + x;
+    double Lb;
+    if (a.kind == 0) {
+      const double Vt = Vb[l] + vb1[y];
+      const double v0b = Vt + vb0[y];
+      Lb = a.ts * Vt;
+      const double sc = l < 4 ? a.tau_src : 1.0;
+      li[(VIRAL_F_T00 + l) * n + x] += sc * v0b;  // Var row l is state field l
+      vb0[y] = 0.0;
+      vb1[y] = 0.0;
+    } else if (a.kind == 1) {
+      Lb = 0.5 * a.ts * Vb[l];
+      vb0[y] += 0.5 * Vb[l];
+      vb1[y] += 0.5 * Vb[l];
+    } else if (a.kind == 2) {
+      const double Vt = Vb[l] + vb2[y];
+      Lb = 0.25 * a.ts * Vt;
+      vb0[y] += 0.75 * Vt;
+      vb1[y] += 0.25 * Vt;
+      vb2[y] = 0.0;
+    } else {
+      Lb = (2.0 / 3.0) * a.ts * Vb[l];
+      vb0[y] += (1.0 / 3.0) * Vb[l];
+      vb2[y] += (2.0 / 3.0) * Vb[l];
+    }
+    resb[y] = Lb;
+  }
+}
+
+// Sources of one interior cell in reverse (inputs from the stage's input
+// state si; the source adjoint is resb, as L = Result + Source): adds to li,
+// writes dubuf, the adjoint of the spatial velocity gradients du[l][1..3]
+// (12 rows) for k_adj_du_src, and adds per-block sums of the adjoints of
+// eta_coef and tpi_coef to partial.
+__global__ void __launch_bounds__(BLK) k_adj_src(Geom g, Params p, StageArgs a, const double* __restrict__ si,
+                                                 const double* __restrict__ resb, double* __restrict__ li,
+                                                 double* __restrict__ dubuf, double* __restrict__ partial)
 {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   double ceb = 0, ctb = 0;
   if (box_cell(interior_box(g), t, i, j, k)) {
     const long n = g.n, x = cidx(g, i, j, k);
-    // prevu = stage 0 ? u : prevu
-    for (int l = 0; l < 4; l++) {
-      const double pb = lo[(VIRAL_F_PU0 + l) * n + x];
-      if (a.stage == 0) li[(VIRAL_F_U0 + l) * n + x] += pb;
-      else li[(VIRAL_F_PU0 + l) * n + x] += pb;
-    }
-    // velocities and root
-    double T0[4], pio[10], T0b[4], pibo[10], vbo[3], ubo[4];
-    for (int l = 0; l < 4; l++) {
-      T0[l] = so[(VIRAL_F_T00 + l) * n + x];
-      T0b[l] = lo[(VIRAL_F_T00 + l) * n + x];
-      ubo[l] = lo[(VIRAL_F_U0 + l) * n + x];
-    }
-    for (int l = 0; l < 10; l++) {
-      pio[l] = so[(VIRAL_F_PI0 + l) * n + x];
-      pibo[l] = lo[(VIRAL_F_PI0 + l) * n + x];
-    }
-    for (int c = 0; c < 3; c++) vbo[c] = lo[(VIRAL_F_VX + c) * n + x];
-    double Vb[SV];
-    update_vjp(p, a.tau_new, T0, pio, so[VIRAL_F_BULK * n + x], so[VIRAL_F_EN * n + x], lo[VIRAL_F_EN * n + x],
-               lo[VIRAL_F_P * n + x], lo[VIRAL_F_TEMP * n + x], vbo, ubo, T0b, pibo, lo[VIRAL_F_BULK * n + x], Vb);
-    // RK combination (NEWEST10/time.h:2-98) in reverse
     double Lb[SV];
-    for (int l = 0; l < SV; l++) {
-      const long y = l * n + x;
-      if (a.kind == 0) {
-        const double Vt = Vb[l] + vb1[y];
-        const double v0b = Vt + vb0[y];
-        Lb[l] = a.ts * Vt;
-        const double sc = l < 4 ? a.tau_src : 1.0;
-        li[(VIRAL_F_T00 + l) * n + x] += sc * v0b;  // Var row l is state field l
-        vb0[y] = 0.0;
-        vb1[y] = 0.0;
-      } else if (a.kind == 1) {
-        Lb[l] = 0.5 * a.ts * Vb[l];
-        vb0[y] += 0.5 * Vb[l];
-        vb1[y] += 0.5 * Vb[l];
-      } else if (a.kind == 2) {
-        const double Vt = Vb[l] + vb2[y];
-        Lb[l] = 0.25 * a.ts * Vt;
-        vb0[y] += 0.75 * Vt;
-        vb1[y] += 0.25 * Vt;
-        vb2[y] = 0.0;
-      } else {
-        Lb[l] = (2.0 / 3.0) * a.ts * Vb[l];
-        vb0[y] += (1.0 / 3.0) * Vb[l];
-        vb2[y] += (2.0 / 3.0) * Vb[l];
-      }
-      resb[y] = Lb[l];
-    }
-    // sources (inputs from si)
+    for (int l = 0; l < SV; l++) Lb[l] = resb[l * n + x];
     const double e = si[VIRAL_F_EN * n + x], P = si[VIRAL_F_P * n + x], PI = si[VIRAL_F_BULK * n + x];
     double u[4], pu[4], pi[10], du[4][4];
     for (int l = 0; l < 4; l++) {
@@ -2999,8 +3012,9 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         }
         HIPCHK(hipMemsetAsync(lamB, 0, fbytes, ctx->stream));
         hipLaunchKernelGGL(k_adj_pass, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, lamA, lamB);
-        hipLaunchKernelGGL(k_adj_update, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, so, lamA, lamB, vb0, vb1,
-                           vb2, resb, dubuf, partial);
+        hipLaunchKernelGGL(k_adj_root, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, so, lamA, lamB, vb0, vb1, vb2,
+                           resb);
+        hipLaunchKernelGGL(k_adj_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB, dubuf, partial);
         hipLaunchKernelGGL(k_adj_du_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, si, dubuf, fbuf);
         hipLaunchKernelGGL(k_adj_du, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, p, fbuf, lamB);
         for (int d = 0; d < ndir; d++) {
