@@ -113,6 +113,25 @@ def test_adjoint_parameter_gradients_match_fd_and_cpu_reference():
     assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
 
 
+def test_gubser_profile_adjoint_matches_fd():
+    """the boost-invariant grid with the GUBSER profile's mirror boundary"""
+    from viral_amd.desc import Q_DEFAULT, Q_GUBSER_MIRROR, ETA_TIMES_S, TPI_OVER_EP
+    d = make_desc(12, 11, 1, lbi=1, quirks=Q_DEFAULT | Q_GUBSER_MIRROR, eta_law=ETA_TIMES_S, eta_coef=0.2,
+                  tpi_law=TPI_OVER_EP, tpi_coef=5.0)
+    st = ic.random_smooth(d, seed=13, tau=TAU, pi_frac=0.02)
+    rng = np.random.default_rng(14)
+    seed = rng.standard_normal(st.shape)
+    print("case adjoint gubser mirror", flush=True)
+    _, grad, pg = run_gpu(d, st, 3, 2, seed)
+    for trial in range(2):
+        dx = direction(st, rng)
+        h = 1e-6
+        fd = (np.sum(seed * run_gpu(d, st + h * dx, 3)) - np.sum(seed * run_gpu(d, st - h * dx, 3))) / (2 * h)
+        ad = np.sum(grad * dx)
+        print(trial, ad, fd, flush=True)
+        assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     d, st = small_case()
     seed = np.random.default_rng(4).standard_normal(st.shape)

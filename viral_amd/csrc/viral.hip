@@ -1258,32 +1258,42 @@ __global__ void __launch_bounds__(BLK) k_output(Geom g, const double* __restrict
 
 // Ghost fill in reverse: every interior cell takes the adjoint of the ghosts
 // that copied from it (NEWEST10/boundary.h, including the q7 mirror).
+template <int D>
 __global__ void __launch_bounds__(BLK) k_adj_ghost(Geom g, Params p, double* __restrict__ lam)
 {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!box_cell(interior_box(g), t, i, j, k)) return;
-  // only cells within BORDER of a face of the interior feed ghosts
-  const bool nx = i < g.il + VIRAL_BORDER || i >= g.ir - VIRAL_BORDER;
-  const bool ny = j < g.jl + VIRAL_BORDER || j >= g.jr - VIRAL_BORDER;
-  const bool nz = !p.lbi && (k == g.kl || k == g.kr - 1);
-  if (!(nx || ny || nz)) return;
-  const long n = g.n, x = cidx(g, i, j, k);
-  for (int q = 0; q < 28; q++) {
-    const int f = c_bfields[q];
-    const bool mirror = (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && p.milne && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
-    double acc = lam[f * n + x];
+  // one thread per (field, line across the D faces, side); lines are interior
+  // in the other two directions, the fast one (eta, or y for D = 2) fastest
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int na = D == 0 ? g.jr - g.jl : g.ir - g.il;
+  const int nb = D == 2 ? g.jr - g.jl : g.kr - g.kl;
+  const long per = (long)na * nb;
+  if (t >= per * 2 * 28) return;
+  const int b = (int)(t % nb);
+  const int a = (int)((t / nb) % na);
+  const int side = (int)((t / per) % 2);
+  const int q = (int)(t / (per * 2));
+  const int f = c_bfields[q];
+  const bool mirror = D < 2 && (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && p.milne && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
+  const int lo = D == 0 ? g.il : D == 1 ? g.jl : g.kl, hi = D == 0 ? g.ir : D == 1 ? g.jr : g.kr;
+  auto at = [&](int cd) -> long {
+    if (D == 0) return cidx(g, cd, g.jl + a, g.kl + b);
+    if (D == 1) return cidx(g, g.il + a, cd, g.kl + b);
+    return cidx(g, g.il + a, g.jl + b, cd);
+  };
+  const long n = g.n;
+  double* lf = lam + (long)f * n;
+  if (mirror) {
+    // ghost h takes its mirror cell (NEWEST10/boundary.h:32-37): one ghost per source cell
     for (int h = 0; h < VIRAL_BORDER; h++) {
-      if ((mirror ? g.il + VIRAL_BORDER - h - 1 : g.il) == i) acc += lam[f * n + cidx(g, h, j, k)];
-      if ((mirror ? g.ir - h - 1 : g.ir - 1) == i) acc += lam[f * n + cidx(g, g.ir + h, j, k)];
-      if ((mirror ? g.jl + VIRAL_BORDER - h - 1 : g.jl) == j) acc += lam[f * n + cidx(g, i, h, k)];
-      if ((mirror ? g.jr - h - 1 : g.jr - 1) == j) acc += lam[f * n + cidx(g, i, g.jr + h, k)];
-      if (!p.lbi) {
-        if (k == g.kl) acc += lam[f * n + cidx(g, i, j, h)];
-        if (k == g.kr - 1) acc += lam[f * n + cidx(g, i, j, g.kr + h)];
-      }
+      const long src = side == 0 ? at(lo + VIRAL_BORDER - h - 1) : at(hi - h - 1);
+      const long gh = side == 0 ? at(h) : at(hi + h);
+      lf[src] += lf[gh];
     }
-    lam[f * n + x] = acc;
+  } else {
+    const long src = side == 0 ? at(lo) : at(hi - 1);
+    double acc = lf[src];
+    for (int h = 0; h < VIRAL_BORDER; h++) acc += lf[side == 0 ? at(h) : at(hi + h)];
+    lf[src] = acc;
   }
 }
 
@@ -3005,7 +3015,16 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         const double* si = snaps + base * NF * n;
         const double* so = snaps + (base + 1) * NF * n;
         const StageArgs a = stage_args(ctx->adj.taus[m], ctx->adj.tss[m], rk, st);
-        hipLaunchKernelGGL(k_adj_ghost, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, lamA);
+        // ghost fill in reverse, one face pair at a time (x, y, eta) so that cells
+        // next to two faces sum in a fixed order
+        {
+          const long nx_ = g.ir - g.il, ny_ = g.jr - g.jl, nz_ = g.kr - g.kl;
+          hipLaunchKernelGGL(k_adj_ghost<0>, dim3(nblocks(ny_ * nz_ * 2 * 28)), dim3(BLK), 0, ctx->stream, g, p, lamA);
+          hipLaunchKernelGGL(k_adj_ghost<1>, dim3(nblocks(nx_ * nz_ * 2 * 28)), dim3(BLK), 0, ctx->stream, g, p, lamA);
+          if (!p.lbi)
+            hipLaunchKernelGGL(k_adj_ghost<2>, dim3(nblocks(nx_ * ny_ * 2 * 28)), dim3(BLK), 0, ctx->stream, g, p,
+                               lamA);
+        }
         if (k0b) {
           const long nl = (long)(g.ir - g.il) * (g.jr - g.jl);
           hipLaunchKernelGGL(k_adj_k0, dim3(nblocks(nl * 64)), dim3(BLK), 0, ctx->stream, g, p, a.tau_new, so, lamA);
