@@ -57,6 +57,11 @@ enum { VIRAL_TPI_OVER_EP = 0,      /* tau_pi = (c*eta)/(e+p)   */
 #define VIRAL_Q_SHEAR_SLICES  (1u << 4) /* q9, Cartesian only: shear sources only in ghosted eta slices k < nz,
                                            as the reference's loop k = 0..ZCMA-1 (TXYZ5/Source.h:272); the top 3
                                            interior slices keep the zero sources of the freshly allocated grid */
+#define VIRAL_Q_ROOT_SAFEGUARD (1u << 5) /* not a reference behaviour (the reference aborts): a Milne cell whose
+                                           Newton solve fails (VIRAL_FLAG_SOLVER, or a root e <= 0) is solved again
+                                           by bracketed Newton-bisection on [0, A], A = T^tt - pi^tt, where a root
+                                           exists (M^2 < A (A - PI)); such cells carry VIRAL_FLAG_RESCUED.  Off by
+                                           default; cells whose Newton solve succeeds are unchanged either way */
 #define VIRAL_Q_DEFAULT (VIRAL_Q_GHOST_EIGEN | VIRAL_Q_K0_BROADCAST | VIRAL_Q_DPDE_OF_P | VIRAL_Q_SHEAR_SLICES)
 
 /* State fields, each one ghosted array of doubles.  Names follow the
@@ -87,6 +92,7 @@ enum {
                                            (the GSL errors that abort the reference)           */
 #define VIRAL_FLAG_MAXITER    (1u << 3) /* Newton hit max_iter without two equal iterates      */
 #define VIRAL_FLAG_FREEZEOUT  (1u << 4) /* Temp < t_freeze                                     */
+#define VIRAL_FLAG_RESCUED    (1u << 5) /* root found by the VIRAL_Q_ROOT_SAFEGUARD bracketed solve */
 
 /* status codes */
 #define VIRAL_OK            0
@@ -172,6 +178,16 @@ int viral_iterations(viral_ctx* ctx, int ev, int stage, int32_t* out);
  * Writes the ghosted (i,j,k), the 0-based stage and that cell's flags in
  * that stage. */
 int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* stage, unsigned* flags);
+
+/* RootSearchForEnVelUsingDerivatives (NEWEST10/root.h:212-450) alone, on the
+ * event's current state in place: every interior cell's e (Newton from its
+ * current e, with the VIRAL_Q_ROOT_SAFEGUARD re-solve when that quirk is set),
+ * P, Temp, v and u from its T^{tau mu}, pi and PI at tau, then the k0
+ * broadcast.  Iteration counts and flags land where stage 0 of a step puts
+ * them (viral_iterations(ctx, ev, 0), viral_flags); a cell raising ACAUSAL,
+ * ENEG or SOLVER gives VIRAL_ERR_PHYSICS and viral_first_bad_cell.  Milne
+ * events only (VIRAL_ERR_UNSUPPORTED otherwise); ghosts are not filled. */
+int viral_root(viral_ctx* ctx, int ev, double tau);
 
 /* Device reductions the reference's driver runs every step.  Each reads the
  * event's current state on the device (no state download).  Cells are

@@ -859,6 +859,71 @@ double or_newton_cell(const viral_desc* d, double tau, double T00, double T10, d
   return x;
 }
 
+/* Safeguard of the root solve, VIRAL_Q_ROOT_SAFEGUARD (not a reference
+ * behaviour: the reference aborts): the root of f(e) = e - A + M2/(A + P(e)
+ * - PI) on [0, A] by the safeguarded Newton method -- a Newton step while it
+ * stays inside the bracket and shrinks |f| fast enough, otherwise a
+ * bisection; the bracket follows the sign of f.  A bracket exists exactly
+ * when f(0) < 0 <= f(A), i.e. M2 < A (A - PI) with P(0) = 0.  Stops when the
+ * step or the bracket falls below 4 ulp of e, or after 200 iterations.  The
+ * device version (viral_amd/csrc/physics.h, root_bracketed) evaluates the
+ * same expressions in the same order. */
+static void brk_f(const viral_desc* d, double M2, double A, double PI, double x, double* fx, double* dfx)
+{
+  const double Q = A + or_eos_p(d, x) - PI;
+  *fx = x - A + M2 / Q;
+  *dfx = 1.0 - M2 * or_eos_dpde(d, x) / (Q * Q);
+}
+
+static int root_bracketed(const viral_desc* d, double M2, double A, double PI, double* root, int32_t* iters)
+{
+  double lo = 0.0, hi = A, fx, dfx;
+  if (!(A > 0.0)) return 0;
+  brk_f(d, M2, A, PI, lo, &fx, &dfx);
+  if (!(fx < 0.0)) return 0;
+  brk_f(d, M2, A, PI, hi, &fx, &dfx);
+  if (!(fx >= 0.0)) return 0;
+  if (fx == 0.0) {
+    *root = hi;
+    *iters = 0;
+    return 1;
+  }
+  double x = 0.5 * (lo + hi), dx = hi - lo, dxold = dx;
+  brk_f(d, M2, A, PI, x, &fx, &dfx);
+  for (int it = 1; it <= 200; it++) {
+    const int leaves = ((x - hi) * dfx - fx) * ((x - lo) * dfx - fx) > 0.0;
+    if (leaves || !(fabs(2.0 * fx) <= fabs(dxold * dfx))) {
+      dxold = dx;
+      dx = 0.5 * (hi - lo);
+      x = lo + dx;
+    } else {
+      dxold = dx;
+      dx = fx / dfx;
+      x = x - dx;
+    }
+    brk_f(d, M2, A, PI, x, &fx, &dfx);
+    if (fx < 0.0) lo = x;
+    else hi = x;
+    if (fx == 0.0 || fabs(dx) <= 4e-16 * fabs(x) || hi - lo <= 4e-16 * hi) {
+      *root = x;
+      *iters = it;
+      return 1;
+    }
+  }
+  *root = x;
+  *iters = 200;
+  return 1;
+}
+
+double or_root_safeguard(const viral_desc* d, double M2, double A, double PI, int32_t* iters)
+{
+  double r;
+  int32_t it = 0;
+  if (!root_bracketed(d, M2, A, PI, &r, &it)) r = NAN;
+  if (iters) *iters = it;
+  return r;
+}
+
 void or_set_newton_shortcut(int on) { newton_shortcut = on; }
 long or_newton_cycle_count(int period) { return (period >= 0 && period < 5) ? newton_cycles[period] : -1; }
 
@@ -1082,6 +1147,18 @@ int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)
         double r = or_newton_cell(d, tau, f[VIRAL_F_T00][x], f[VIRAL_F_T10][x], f[VIRAL_F_T20][x], f[VIRAL_F_T30][x],
                                   f[VIRAL_F_PI0][x], f[VIRAL_F_PI0 + 1][x], f[VIRAL_F_PI0 + 2][x],
                                   f[VIRAL_F_PI0 + 3][x], f[VIRAL_F_BULK][x], f[VIRAL_F_EN][x], &it, &fl);
+        if ((d->quirks & VIRAL_Q_ROOT_SAFEGUARD) && ((fl & VIRAL_FLAG_SOLVER) || !(r > 0))) {
+          const double a1 = f[VIRAL_F_T10][x] - f[VIRAL_F_PI0 + 1][x], a2 = f[VIRAL_F_T20][x] - f[VIRAL_F_PI0 + 2][x],
+                       a3 = f[VIRAL_F_T30][x] - f[VIRAL_F_PI0 + 3][x];
+          const double M2 = a1 * a1 + a2 * a2 + tau * tau * a3 * a3;
+          double er;
+          int32_t ir;
+          if (root_bracketed(d, M2, f[VIRAL_F_T00][x] - f[VIRAL_F_PI0][x], f[VIRAL_F_BULK][x], &er, &ir)) {
+            r = er;
+            it = it + ir;
+            fl = (fl & ~(VIRAL_FLAG_SOLVER | VIRAL_FLAG_MAXITER)) | VIRAL_FLAG_RESCUED;
+          }
+        }
         f[VIRAL_F_EN][x] = r;
         f[VIRAL_F_TEMP][x] = or_eos_T(d, r);
         f[VIRAL_F_P][x] = or_eos_p(d, r);

@@ -54,6 +54,10 @@ double or_newton_cell(const viral_desc* d, double tau, double T00, double T10, d
                       double pi0, double pi1, double pi2, double pi3, double PI, double guess,
                       int32_t* iters, unsigned* flags);
 
+/* The VIRAL_Q_ROOT_SAFEGUARD solve of one cell: the root of f on [0, A] by
+ * bracketed Newton-bisection; NaN when no bracket exists (M2 >= A (A - PI)). */
+double or_root_safeguard(const viral_desc* d, double M2, double A, double PI, int32_t* iters);
+
 /* 1 (default): detect period-2..4 orbits of the Newton iteration and jump to
  * the iterate the loop would end on; 0: run the loop to max_iter. */
 void or_set_newton_shortcut(int on);

@@ -69,6 +69,8 @@ def lib():
         L.or_cart_df.argtypes = [C.POINTER(ViralDesc), dp, dp, dp]
         L.or_centre_flux.argtypes = [vp, C.c_int, C.c_double, dp]
         L.or_shear_source.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_double, C.c_double, dp]
+        L.or_root_safeguard.restype = C.c_double
+        L.or_root_safeguard.argtypes = [C.POINTER(ViralDesc), C.c_double, C.c_double, C.c_double, ip]
         lp = C.POINTER(C.c_longlong)
         L.or_check_root.restype = C.c_int
         L.or_check_root.argtypes = [vp, C.c_double, dp, dp, lp]

@@ -6,7 +6,11 @@ reference harness cannot run (root.h needs GSL):
   positive root the Newton result must equal to a few ulps;
 * round trip (e, u, pi) -> T^{tau mu} -> root -> (e, u);
 * GSL newton semantics: iteration counts, the f' == 0 error, max_iter;
-* the exact period-orbit shortcut: identical results to running the loop.
+* the exact period-orbit shortcut: identical results to running the loop;
+* the opt-in safeguard (VIRAL_Q_ROOT_SAFEGUARD, no counterpart in the
+  reference): its bracketed solve equals the quadratic's positive root, and
+  cells whose Newton solve lands on the unphysical branch are rescued while
+  every other cell is untouched.
 """
 import ctypes
 
@@ -14,7 +18,8 @@ import numpy as np
 
 import oracle_ctypes
 from viral_amd import ic
-from viral_amd.desc import (make_desc, EOS_S95P, FLAG_SOLVER, FLAG_MAXITER, F_EN, F_U0, NFIELDS)
+from viral_amd.desc import (make_desc, EOS_S95P, FLAG_SOLVER, FLAG_MAXITER, FLAG_ENEG, FLAG_RESCUED, F_EN, F_U0,
+                            F_T00, F_PI0, F_BULK, NFIELDS, Q_DEFAULT, Q_ROOT_SAFEGUARD)
 
 L = oracle_ctypes.lib()
 L.or_set_newton_shortcut.argtypes = [ctypes.c_int]
@@ -149,3 +154,62 @@ def test_period_shortcut_is_exact():
         r0 = newton(d, 1.0, T, np.zeros(4), 0.0, g)
         L.or_set_newton_shortcut(1)
         assert r1 == r0 or (np.isnan(r1[0]) and np.isnan(r0[0]) and r1[1:] == r0[1:])
+
+
+def test_safeguard_solve_is_the_quadratic_root():
+    d = make_desc(4, 4, 4)
+    ds = make_desc(4, 4, 4, eos=EOS_S95P)
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(2000):
+        A = rng.uniform(0.05, 80.0)
+        PI = rng.uniform(-0.05, 0.05) * A
+        M2 = (rng.uniform(0, 0.95) * A) ** 2 * (A - PI) / A
+        b = 2 * A / 3 - PI
+        exact = (-b + np.sqrt(b * b - 4 * (M2 - A * (A - PI)) / 3)) / (2.0 / 3)
+        it = ctypes.c_int32()
+        e = L.or_root_safeguard(d, M2, A, PI, ctypes.byref(it))
+        worst = max(worst, abs(e - exact) / exact)
+        assert 0 < it.value <= 200
+        # s95p: a root of f inside the bracket
+        e = L.or_root_safeguard(ds, M2, A, PI, ctypes.byref(it))
+        Pe = L.or_eos_p(ds, e)
+        assert 0 < e <= A and abs(e - A + M2 / (A + Pe - PI)) <= 1e-12 * A
+    assert worst < 1e-14
+    # no bracket for an acausal cell
+    assert np.isnan(L.or_root_safeguard(d, 1.01 * 4.0, 2.0, 0.0, None))
+
+
+def test_safeguard_rescues_cells_on_the_unphysical_branch():
+    """A Newton guess just above the pole of f at e = -3 (A - PI) runs to the
+    quadratic's negative root (ENEG: the reference would abort); the
+    safeguard re-solves those cells to the positive root and leaves every
+    other cell bit for bit as it was."""
+    tau = 1.1
+    base = make_desc(9, 8, 6)
+    st = ic.random_smooth(base, seed=8, tau=tau)
+    gx, gy, gz = base.ghost_dims()
+    planted = [((4 * gy + 5) * gz + 4), ((8 * gy + 3) * gz + 6)]
+    for x in planted:
+        st[F_EN, x] = -2.95 * (st[F_T00, x] - st[F_PI0, x] - st[F_BULK, x])
+    runs = {}
+    for q in (Q_DEFAULT, Q_DEFAULT | Q_ROOT_SAFEGUARD):
+        d = make_desc(9, 8, 6, quirks=q)
+        o = oracle_ctypes.Oracle(d)
+        o.set_state(st)
+        bad, it, fl = o.root(tau)
+        runs[q] = (bad, fl, o.state())
+    bad, fl, s0 = runs[Q_DEFAULT]
+    assert bad == planted[0] and fl[planted[0]] & (FLAG_ENEG | FLAG_SOLVER)
+    bad, fl, s1 = runs[Q_DEFAULT | Q_ROOT_SAFEGUARD]
+    assert bad == -1
+    for x in planted:
+        assert fl[x] & FLAG_RESCUED and not fl[x] & (FLAG_ENEG | FLAG_SOLVER)
+        A, PI = st[F_T00, x] - st[F_PI0, x], st[F_BULK, x]
+        M2 = sum((st[F_T00 + m, x] - st[F_PI0 + m, x]) ** 2 * (tau * tau if m == 3 else 1.0) for m in (1, 2, 3))
+        b = 2 * A / 3 - PI
+        exact = (-b + np.sqrt(b * b - 4 * (M2 - A * (A - PI)) / 3)) / (2.0 / 3)
+        assert abs(s1[F_EN, x] - exact) <= 1e-14 * exact
+    keep = np.ones(s0.shape[1], bool)
+    keep[planted] = False
+    assert np.array_equal(s0[:, keep], s1[:, keep])

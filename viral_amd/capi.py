@@ -23,7 +23,7 @@ EXPORTS = [
     "viral_kernel_count", "viral_kernel_name", "viral_last_error", "viral_steps", "viral_get_launch",
     "viral_set_launch", "viral_tile_count", "viral_tile_table", "viral_xcd_tile", "viral_adjoint_begin",
     "viral_adjoint", "viral_adjoint_end", "viral_check_root", "viral_debug_extrema", "viral_check_physics",
-    "viral_output_size", "viral_output", "viral_sync", "viral_host_alloc", "viral_host_free",
+    "viral_output_size", "viral_output", "viral_sync", "viral_host_alloc", "viral_host_free", "viral_root",
 ]
 
 # kernel phases of the tile table (VIRAL_PHASE_*)
@@ -128,6 +128,8 @@ def load():
     L.viral_output_size.argtypes = [pdesc, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]
     L.viral_output.restype = C.c_int
     L.viral_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int]
+    L.viral_root.restype = C.c_int
+    L.viral_root.argtypes = [vp, C.c_int, C.c_double]
     L.viral_sync.restype = C.c_int
     L.viral_sync.argtypes = [vp]
     L.viral_host_alloc.restype = C.c_int
@@ -327,6 +329,13 @@ class Hydro:
         v = C.c_double()
         self._chk(self.L.viral_max_temp_gev(self.h, ev, C.byref(v)))
         return v.value
+
+    def root(self, tau, ev=0):
+        """the root and velocity solve alone on the current state (status)"""
+        r = self.L.viral_root(self.h, ev, tau)
+        if r not in (OK, ERR_PHYSICS):
+            self._chk(r)
+        return r
 
     def _cells(self, ijk, ev):
         """ghosted linear indices of (i, j, k) triples, -1 where none"""

@@ -580,6 +580,60 @@ __device__ __forceinline__ void eos_pa_fast(const Params& p, double e, double& P
   a = s95p_a(e);
 }
 
+// Safeguard of the root solve (no counterpart in the reference, which aborts
+// instead): the root of f(e) = e - A + M2/(A + P(e) - PI) on [0, A] by the
+// safeguarded Newton method (a Newton step while it stays inside the bracket
+// and shrinks |f| fast enough, a bisection of the bracket otherwise; the
+// bracket follows the sign of f).  f(0) < 0 <= f(A) holds exactly when
+// M2 < A (A - PI) for P(0) = 0, i.e. for a causal cell; false when no bracket
+// exists.  Stops when the step or the bracket falls below 4 ulp of e, or
+// after 200 iterations.
+__host__ __device__ inline bool root_bracketed(const Params& p, double M2, double A, double PI, double& root,
+                                               int& iters)
+{
+  double lo = 0.0, hi = A, fx, dfx;
+  auto F = [&](double x) {
+    const double Q = A + eos_p(p, x) - PI;
+    fx = x - A + M2 / Q;
+    dfx = 1.0 - M2 * eos_dpde(p, x) / (Q * Q);
+  };
+  if (!(A > 0.0)) return false;
+  F(lo);
+  if (!(fx < 0.0)) return false;
+  F(hi);
+  if (!(fx >= 0.0)) return false;
+  if (fx == 0.0) {
+    root = hi;
+    iters = 0;
+    return true;
+  }
+  double x = 0.5 * (lo + hi), dx = hi - lo, dxold = dx;
+  F(x);
+  for (int it = 1; it <= 200; it++) {
+    const bool leaves = ((x - hi) * dfx - fx) * ((x - lo) * dfx - fx) > 0.0;
+    if (leaves || !(fabs(2.0 * fx) <= fabs(dxold * dfx))) {
+      dxold = dx;
+      dx = 0.5 * (hi - lo);
+      x = lo + dx;
+    } else {
+      dxold = dx;
+      dx = fx / dfx;
+      x = x - dx;
+    }
+    F(x);
+    if (fx < 0.0) lo = x;
+    else hi = x;
+    if (fx == 0.0 || fabs(dx) <= 4e-16 * fabs(x) || hi - lo <= 4e-16 * hi) {
+      root = x;
+      iters = it;
+      return true;
+    }
+  }
+  root = x;
+  iters = 200;
+  return true;
+}
+
 __device__ __forceinline__ double newton_cell_fast(const Params& p, double M2, double Ttt, double pi1, double PI,
                                                    double guess, int& iters, unsigned& flags)
 {

@@ -719,6 +719,57 @@ __global__ void __launch_bounds__(BLK) k_source(const EvDev* __restrict__ tab, S
 // CART: the Cartesian update (TXYZ5/time.h:100-165, TXYZ5/multiroot.h:322-479):
 // T^{t mu} = Var, five pi rows, the 4-D gnewton root from (e, u^x, u^y, u^z),
 // u^t = sqrt(1 + u.u), no k0 broadcast.
+// Root and velocities of one Milne cell (NEWEST10/root.h:301-424) from its
+// T^{tau mu}, pi, PI at tn, starting Newton from guess: e (with the
+// VIRAL_Q_ROOT_SAFEGUARD re-solve), P, Temp, v, u^0 = gamma and the cell's
+// iteration count and flags.  Shared by the stage update and viral_root.
+template <bool FAST>
+__device__ __forceinline__ void solve_cell(const Params& p, int milne, double tn, const double T0[4], const double pi[10],
+                                           double PIn, double e, double& en, double& Pn, double& Temp, double& vx,
+                                           double& vy, double& ve, double& gm, int& it, unsigned& fl)
+{
+  // ---- root (NEWEST10/root.h:301-364)
+  double M2 = milne ? (T0[1] - pi[1]) * (T0[1] - pi[1]) + (T0[2] - pi[2]) * (T0[2] - pi[2]) +
+                          tn * tn * (T0[3] - pi[3]) * (T0[3] - pi[3])
+                    : (T0[1] - pi[1]) * (T0[1] - pi[1]) + (T0[2] - pi[2]) * (T0[2] - pi[2]) +
+                          (T0[3] - pi[3]) * (T0[3] - pi[3]);
+  if (FAST) en = newton_cell_fast(p, M2, T0[0], pi[0], PIn, e, it, fl);
+  else en = newton_cell(p, M2, T0[0], pi[0], PIn, e, it, fl);
+  if ((p.quirks & VIRAL_Q_ROOT_SAFEGUARD) && ((fl & VIRAL_FLAG_SOLVER) || !(en > 0))) {
+    double er;
+    int ir;
+    if (root_bracketed(p, M2, T0[0] - pi[0], PIn, er, ir)) {
+      en = er;
+      it = it + ir;
+      fl = (fl & ~(VIRAL_FLAG_SOLVER | VIRAL_FLAG_MAXITER)) | VIRAL_FLAG_RESCUED;
+    }
+  }
+  if (FAST) {
+    double a_, s_;
+    eos_pa_fast(p, en, Pn, a_);
+    eos_Ts_fast(p, en, Pn, Temp, s_);
+  } else {
+    Temp = eos_T(p, en);
+    Pn = eos_p(p, en);
+  }
+  // ---- velocities (NEWEST10/root.h:372-424)
+  double t2 = milne ? tn * tn : 1.0;
+  if (FAST) {
+    const double r = 1.0 / (T0[0] - pi[0] + Pn - PIn);
+    vx = (T0[1] - pi[1]) * r;
+    vy = (T0[2] - pi[2]) * r;
+    ve = (T0[3] - pi[3]) * r;
+  } else {
+    vx = (T0[1] - pi[1]) / (T0[0] - pi[0] + Pn - PIn);
+    vy = (T0[2] - pi[2]) / (T0[0] - pi[0] + Pn - PIn);
+    ve = (T0[3] - pi[3]) / (T0[0] - pi[0] + Pn - PIn);
+  }
+  if (vx * vx + vy * vy + t2 * ve * ve >= 1) fl |= VIRAL_FLAG_ACAUSAL;
+  if (en <= 0) fl |= VIRAL_FLAG_ENEG;
+  gm = 1.0 / sqrt(1 - vx * vx - vy * vy - t2 * ve * ve);
+  if (p.t_freeze > 0 && Temp < p.t_freeze) fl |= VIRAL_FLAG_FREEZEOUT;
+}
+
 template <bool FAST, bool FUSED, bool CART>
 __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, const StageArgs& a, long long step,
                                             const double* __restrict__ si, double* __restrict__ so,
@@ -846,41 +897,11 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
 #pragma unroll
   for (int l = 0; l < 10; l++) pi[l] = (V[4 + l]);
   const double PIn = (V[14]);
-  // ---- root (NEWEST10/root.h:301-364)
-  double M2 = milne ? (T0[1] - pi[1]) * (T0[1] - pi[1]) + (T0[2] - pi[2]) * (T0[2] - pi[2]) +
-                          tn * tn * (T0[3] - pi[3]) * (T0[3] - pi[3])
-                    : (T0[1] - pi[1]) * (T0[1] - pi[1]) + (T0[2] - pi[2]) * (T0[2] - pi[2]) +
-                          (T0[3] - pi[3]) * (T0[3] - pi[3]);
+  // ---- root and velocities (NEWEST10/root.h:301-424)
   int it;
   unsigned fl;
-  double en, Temp, Pn;
-  if (FAST) {
-    en = newton_cell_fast(p, M2, T0[0], pi[0], PIn, e, it, fl);
-    double a_, s_;
-    eos_pa_fast(p, en, Pn, a_);
-    eos_Ts_fast(p, en, Pn, Temp, s_);
-  } else {
-    en = newton_cell(p, M2, T0[0], pi[0], PIn, e, it, fl);
-    Temp = eos_T(p, en);
-    Pn = eos_p(p, en);
-  }
-  // ---- velocities (NEWEST10/root.h:372-424)
-  double t2 = milne ? tn * tn : 1.0;
-  double vx, vy, ve;
-  if (FAST) {
-    const double r = 1.0 / (T0[0] - pi[0] + Pn - PIn);
-    vx = (T0[1] - pi[1]) * r;
-    vy = (T0[2] - pi[2]) * r;
-    ve = (T0[3] - pi[3]) * r;
-  } else {
-    vx = (T0[1] - pi[1]) / (T0[0] - pi[0] + Pn - PIn);
-    vy = (T0[2] - pi[2]) / (T0[0] - pi[0] + Pn - PIn);
-    ve = (T0[3] - pi[3]) / (T0[0] - pi[0] + Pn - PIn);
-  }
-  if (vx * vx + vy * vy + t2 * ve * ve >= 1) fl |= VIRAL_FLAG_ACAUSAL;
-  if (en <= 0) fl |= VIRAL_FLAG_ENEG;
-  double gm = 1.0 / sqrt(1 - vx * vx - vy * vy - t2 * ve * ve);
-  if (p.t_freeze > 0 && Temp < p.t_freeze) fl |= VIRAL_FLAG_FREEZEOUT;
+  double en, Temp, Pn, vx, vy, ve, gm;
+  solve_cell<FAST>(p, milne, tn, T0, pi, PIn, e, en, Pn, Temp, vx, vy, ve, gm, it, fl);
   // ---- store
 #pragma unroll
   for (int l = 0; l < 4; l++) so[(VIRAL_F_T00 + l) * n + x] = T0[l];
@@ -1069,6 +1090,48 @@ __global__ void k_max_temp(Geom g, Params p, const double* __restrict__ st, unsi
     if (w > v) v = w;
   }
   if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+// RootSearchForEnVelUsingDerivatives (NEWEST10/root.h:212-424) on a state in
+// place: every interior cell's e (Newton from its current e), P, Temp, v, u
+// from its T^{tau mu}, pi, PI at tau; iteration counts, flags and the first
+// bad cell as stage 0 of a step.  The k0 broadcast follows in k_root_k0.
+template <bool FAST>
+__global__ void __launch_bounds__(BLK) k_root(Geom g, Params p, double tau, double* __restrict__ st,
+                                              int32_t* __restrict__ iters, uint8_t* __restrict__ flags,
+                                              unsigned long long* __restrict__ bad)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  const long n = g.n, x = cidx(g, i, j, k);
+  double T0[4], pi[10];
+  for (int l = 0; l < 4; l++) T0[l] = st[(VIRAL_F_T00 + l) * n + x];
+  for (int l = 0; l < 10; l++) pi[l] = st[(VIRAL_F_PI0 + l) * n + x];
+  int it;
+  unsigned fl;
+  double en, Pn, Temp, vx, vy, ve, gm;
+  solve_cell<FAST>(p, 1, tau, T0, pi, st[VIRAL_F_BULK * n + x], st[VIRAL_F_EN * n + x], en, Pn, Temp, vx, vy, ve, gm,
+                   it, fl);
+  st[VIRAL_F_EN * n + x] = en;
+  st[VIRAL_F_P * n + x] = Pn;
+  st[VIRAL_F_TEMP * n + x] = Temp;
+  st[VIRAL_F_VX * n + x] = vx;
+  st[VIRAL_F_VY * n + x] = vy;
+  st[VIRAL_F_VE * n + x] = ve;
+  st[VIRAL_F_U0 * n + x] = gm;
+  st[VIRAL_F_U1 * n + x] = gm * vx;
+  st[VIRAL_F_U2 * n + x] = gm * vy;
+  st[VIRAL_F_U3 * n + x] = gm * ve;
+  iters[x] = it;
+  flags[x] = (uint8_t)fl;
+  if (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER))
+    atomicMin(bad, ((unsigned long long)x << 8) | (unsigned long long)fl);
+}
+
+__global__ void __launch_bounds__(BLK) k_root_k0(Geom g, Params p, double tau, double* __restrict__ st)
+{
+  k0_cell(g, p, tau, st, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- per-step diagnostics
@@ -2515,6 +2578,30 @@ int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* st
   *i = (int)(r / E->g.gy);
   *flags = (unsigned)(E->bad_host & 0xff);
   return 0;
+}
+
+int viral_root(viral_ctx* ctx, int ev, double tau)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E) return fail(ctx, VIRAL_ERR_ARG, "bad event");
+  if (!E->p.milne) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "viral_root covers the Milne (1-D) root solve");
+  const long ni = box_count(interior_box(E->g));
+  hipLaunchKernelGGL(k_clear_bad, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bad + ev, 1);
+  double* st = E->st[ctx->cur];
+  if (E->d.fast_math)
+    hipLaunchKernelGGL(k_root<true>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, st, E->iters,
+                       E->flags, ctx->d_bad + ev);
+  else
+    hipLaunchKernelGGL(k_root<false>, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, st, E->iters,
+                       E->flags, ctx->d_bad + ev);
+  if ((E->p.quirks & VIRAL_Q_K0_BROADCAST) && !E->p.lbi)
+    hipLaunchKernelGGL(k_root_k0, dim3(nblocks(ni)), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, st);
+  HIPCHK(hipGetLastError());
+  unsigned long long w = ~0ull;
+  HIPCHK(hipMemcpyAsync(&w, ctx->d_bad + ev, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  E->bad_host = w;
+  return w == ~0ull ? VIRAL_OK : VIRAL_ERR_PHYSICS;
 }
 
 int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)

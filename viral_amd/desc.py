@@ -17,6 +17,7 @@ Q_K0_BROADCAST = 1 << 1
 Q_DPDE_OF_P = 1 << 2
 Q_GUBSER_MIRROR = 1 << 3
 Q_SHEAR_SLICES = 1 << 4
+Q_ROOT_SAFEGUARD = 1 << 5   # opt-in: bracketed re-solve of cells whose Newton solve fails
 Q_DEFAULT = Q_GHOST_EIGEN | Q_K0_BROADCAST | Q_DPDE_OF_P | Q_SHEAR_SLICES
 
 (F_T00, F_T10, F_T20, F_T30) = range(4)
@@ -37,6 +38,7 @@ FLAG_ENEG = 1 << 1
 FLAG_SOLVER = 1 << 2
 FLAG_MAXITER = 1 << 3
 FLAG_FREEZEOUT = 1 << 4
+FLAG_RESCUED = 1 << 5
 
 OK, ERR_ARG, ERR_HIP, ERR_PHYSICS, ERR_UNSUPPORTED = 0, -1, -2, -3, -4
 
