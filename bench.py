@@ -120,31 +120,31 @@ def cpu_baseline(seconds_target=20.0):
                       "CPU oracle (C, -O2, one thread)" % steps}
 
 
-def ensemble_descs(n_events, seed=4):
-    """Uneven ensemble: square transverse side ~U{96..256} cells, eta cells
-    ~U{32..64}, b ~U(0,12) fm; viscous KT, s95p EOS (table), eta/s=0.08,
+def ensemble_descs(n_events, seed=4, side=(96, 256), eta=(32, 64)):
+    """Uneven ensemble: square transverse side ~U{side} cells, eta cells
+    ~U{eta}, b ~U(0,12) fm; viscous KT, s95p EOS (table), eta/s=0.08,
     tau_pi=5eta/(e+p), RK3, dx=deta=0.1."""
     import numpy as np
     from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP
     rng = np.random.default_rng(seed)
     descs, bs = [], []
     for _ in range(n_events):
-        side = int(rng.integers(96, 257))
-        neta = int(rng.integers(32, 65))
-        descs.append(make_desc(side, side, neta, rk=3, eos=EOS_S95P, eta_law=ETA_TIMES_S, eta_coef=0.08,
+        side_ = int(rng.integers(side[0], side[1] + 1))
+        neta = int(rng.integers(eta[0], eta[1] + 1))
+        descs.append(make_desc(side_, side_, neta, rk=3, eos=EOS_S95P, eta_law=ETA_TIMES_S, eta_coef=0.08,
                                tpi_law=TPI_OVER_EP, tpi_coef=5.0, fast_math=1, eos_table=1))
         bs.append(float(rng.uniform(0.0, 12.0)))
     return descs, bs
 
 
-def bench_ensemble(n_events, steps, tau0=0.6, ts=0.002):
+def bench_ensemble(n_events, steps, tau0=0.6, ts=0.002, side=(96, 256), eta=(32, 64)):
     """Plain (one launch per event and kernel phase) vs batched (one launch per
     phase over all events, hipGraph replay, autotuned launch configuration)
     on the same uneven hotspot ensemble in one context; both synchronise once
     per step."""
     from concurrent.futures import ThreadPoolExecutor
     from viral_amd import capi, ic
-    descs, bs = ensemble_descs(n_events)
+    descs, bs = ensemble_descs(n_events, side=side, eta=eta)
     cells = sum(d.nx * d.ny * d.nz for d in descs)
 
     def make_state(e):
@@ -173,8 +173,9 @@ def bench_ensemble(n_events, steps, tau0=0.6, ts=0.002):
             t[0] += ts
         return time.perf_counter() - t0
 
-    out = {"workload": "%d hotspot events (200 spots, width 0.4 fm, b~U(0,12) fm), transverse side ~U{96..256}, "
-                       "eta cells ~U{32..64}, viscous KT, s95p table, eta/s=0.08, RK3, dtau=%g" % (n_events, ts),
+    out = {"workload": "%d hotspot events (200 spots, width 0.4 fm, b~U(0,12) fm), transverse side ~U{%d..%d}, "
+                       "eta cells ~U{%d..%d}, viscous KT, s95p table, eta/s=0.08, RK3, dtau=%g"
+                       % (n_events, side[0], side[1], eta[0], eta[1], ts),
            "events": n_events, "interior_cells": cells, "ic_seconds": round(t_ic, 1)}
     h.set_launch(batched=0, graph=0, blocks_per_cu=0, fusion=0, xcd=0)
     dt = timed(steps)
@@ -247,6 +248,8 @@ def main():
     ap.add_argument("--ensemble-events", type=int, default=64,
                     help="events of the batched-ensemble leg (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
+    ap.add_argument("--small-events", type=int, default=128,
+                    help="events of the small-event ensemble leg (transverse side 24..64, eta 8..24; 0 skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the adjoint leg (0 skips it)")
     ap.add_argument("--adjoint-eos", choices=["s95p", "conformal"], default="s95p")
@@ -342,6 +345,9 @@ def main():
     h.close()
     if args.ensemble_events > 0:
         out["ensemble"] = bench_ensemble(args.ensemble_events, args.ensemble_steps)
+    if args.small_events > 0:
+        # many small events: the launch-bound regime the batched launch is for
+        out["ensemble_small"] = bench_ensemble(args.small_events, 4 * args.ensemble_steps, side=(24, 64), eta=(8, 24))
     if args.adjoint_steps > 0:
         out["adjoint"] = bench_adjoint(args.adjoint_steps, eos=args.adjoint_eos, fast=0 if args.adjoint_exact else 1)
     if not args.no_cpu:
