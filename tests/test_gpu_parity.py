@@ -231,6 +231,8 @@ def test_fast_flux_tile_shapes_agree(tiles):
         h = capi.Hydro(d)
         h.upload(st)
         if tl:
+            if max(tp * tk for tp, tk in tl) > 512:
+                h.set_launch(flux_nt=[512, 512, 512])  # 1024-slot tiles need 512-thread blocks
             for dd, (tp, tk) in enumerate(tl):
                 h.set_tile(dd, tp, tk)
         assert h.step(tau, ts) == OK

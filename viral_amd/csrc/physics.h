@@ -728,39 +728,37 @@ __host__ __device__ __forceinline__ void shear_source(int milne, double tau, con
   double divv = 0;
 #pragma unroll
   for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
-  double Dc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
+  // Dc = du with the Christoffel part and G^{ab} = g^{aa} Dc[b][a], evaluated
+  // where used (the same values each time; the loops are unrolled, so the
+  // index tests fold away and no 4x4 copies stay live)
   const double itau = milne ? 1.0 / tau : 0.0;
-  if (milne) {
-    Dc[0][3] = Dc[0][3] + tau * u[3];
-    Dc[3][0] = Dc[3][0] + itau * u[3];
-    Dc[3][3] = Dc[3][3] + itau * u[0];
-  }
-  const double theta = Dc[0][0] + Dc[1][1] + Dc[2][2] + Dc[3][3];
+  auto Dc = [&](int a, int b) -> double {
+    double v = du[a][b];
+    if (milne) {
+      if (a == 0 && b == 3) v = v + tau * u[3];
+      if (a == 3 && b == 0) v = v + itau * u[3];
+      if (a == 3 && b == 3) v = v + itau * u[0];
+    }
+    return v;
+  };
+  auto G = [&](int a, int b) -> double { return gi[a] * Dc(b, a); };
+  const double theta = Dc(0, 0) + Dc(1, 1) + Dc(2, 2) + Dc(3, 3);
   double Dul[4];
 #pragma unroll
   for (int a = 0; a < 4; a++) {
     double s = 0;
 #pragma unroll
-    for (int b = 0; b < 4; b++) s += u[b] * Dc[a][b];
+    for (int b = 0; b < 4; b++) s += u[b] * Dc(a, b);
     Dul[a] = g[a] * s;
   }
-  double G[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
   double Gu[4], uG[4], H[4];
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     double s1 = 0, s2 = 0;
 #pragma unroll
     for (int b = 0; b < 4; b++) {
-      s1 += G[m][b] * ul[b];
-      s2 += ul[b] * G[b][m];
+      s1 += G(m, b) * ul[b];
+      s2 += ul[b] * G(b, m);
     }
     Gu[m] = s1;
     uG[m] = s2;
@@ -769,7 +767,7 @@ __host__ __device__ __forceinline__ void shear_source(int milne, double tau, con
 #pragma unroll
   for (int a = 0; a < 4; a++) {
     uGu += ul[a] * Gu[a];
-    trg += g[a] * G[a][a];
+    trg += g[a] * G(a, a);
   }
   const double tr = trg - uGu;
 #pragma unroll
@@ -789,7 +787,7 @@ __host__ __device__ __forceinline__ void shear_source(int milne, double tau, con
 #pragma unroll
     for (int n = m; n < 4; n++) {
       const double pmn = pi10[pidx(m, n)];
-      double sigma = 0.5 * (G[m][n] + G[n][m]) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
+      double sigma = 0.5 * (G(m, n) + G(n, m)) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
       if (m == n) sigma = sigma - (1.0 / 3.0) * gi[m] * tr;
       const double pu = u[m] * PD[n] + u[n] * PD[m];
       const double rel = FAST ? -(pmn - 2.0 * SH * sigma) * itpi : -(pmn - 2.0 * SH * sigma) / tpi;

@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(const EvDev* __restrict__ tab,
 // at most 1024 slots = 48 KiB of LDS).
 template <int NT>
 struct TileSlots {
-  static constexpr int MAXR = NT >= 512 ? 2 : 4;
+  static constexpr int MAXR = NT >= 256 ? 2 : 4;
   static constexpr int MAXS = MAXR * NT < 1024 ? MAXR * NT : 1024;
 };
 
@@ -687,7 +687,8 @@ __device__ __forceinline__ void source_terms(const Geom& g, const Params& p, con
 }
 
 template <bool FAST, bool PERSIST, bool CART>
-__global__ void __launch_bounds__(BLK) k_source(const EvDev* __restrict__ tab, Sched s, const Clock* __restrict__ clk,
+// fast mode: registers capped for three waves per SIMD (a few bytes spill)
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(FAST ? 3 : 1))) k_source(const EvDev* __restrict__ tab, Sched s, const Clock* __restrict__ clk,
                                                 int stage, int cur)
 {
   TILE_LOOP(s)
@@ -1801,7 +1802,7 @@ struct Event {
   int32_t* iters = nullptr;  // 3*n
   uint8_t* flags = nullptr;
   unsigned long long bad_host = ~0ull;
-  int tile_tp[3] = {64, 64, 64};  // LDS tile of the fast flux kernel: cells per line
+  int tile_tp[3] = {32, 32, 64};  // LDS tile of the fast flux kernel: cells per line
   int tile_tk[3] = {16, 16, 4};   //   and lines per tile, per direction
   double* tab_v = nullptr;        // tabulated s95p EOS (fast mode)
   unsigned char* tab_closed = nullptr;
@@ -2366,11 +2367,12 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     ok = ok && hipMalloc(&E.flags, n) == hipSuccess;
     if (!ok) break;
     {
-      // default LDS tiles: 64-cell lines x 16 lines across x and y; along eta
-      // the whole ghosted line when it fits (one tile per line)
+      // default LDS tiles: 32-cell lines x 16 lines across x and y; along eta
+      // the whole ghosted line when it fits (one tile per line), as many lines
+      // as the default block size holds
       const int gz = g.gz;
       E.tile_tp[2] = gz <= 128 ? std::max(gz, 8) : 64;  // lbi (gz = 1): the eta phase is empty anyway
-      E.tile_tk[2] = std::max(1, 1024 / E.tile_tp[2]);
+      E.tile_tk[2] = std::max(1, TileSlots<256>::MAXS / E.tile_tp[2]);
     }
     if (d->eos_table && d->eos == VIRAL_EOS_S95P) {
       std::vector<double> tv;
