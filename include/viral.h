@@ -295,8 +295,9 @@ int viral_tile_table(viral_ctx* ctx, int phase, long long* start);
 long long viral_xcd_tile(long long w, long long total);
 
 /* LDS tile shape of the fast KT flux kernel of direction dir (0 x, 1 y,
- * 2 eta): tp cells along the direction (tp-5 faces per tile) times tk lines,
- * tp >= 8, tp*tk at most 4*flux_nt[dir] and at most 1024.
+ * 2 eta): tp cells along the direction (tp-5 faces, tp-6 written cells per
+ * tile) times tk lines, tp >= 8, tp*tk at most 4*flux_nt[dir] for flux_nt 64
+ * and 128, 2*flux_nt[dir] for 256 and 512, and at most 1024.
  * viral_autotune times candidates on the current state at the clock of the
  * last step (reps launches each) and keeps the fastest: with ev >= 0 the tile
  * shapes of event ev; with ev < 0 the whole launch configuration (per

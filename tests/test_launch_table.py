@@ -42,20 +42,26 @@ def test_exact_and_shasta_flux_phases():
 
 
 @pytest.mark.parametrize("tp,tk", [(64, 16), (32, 8), (70, 14), (8, 1), (256, 4)])
-def test_fast_flux_tiles_cover_every_face_once(tp, tk):
-    """A tile holds tk lines and tp cells of each, of which the tp-5 middle
-    positions own a face; tiles along a line step by tp-5 faces."""
+def test_fast_flux_tiles_cover_every_cell_once(tp, tk):
+    """A tile holds tk lines and tp cells of each.  The tp-5 middle positions
+    hold a face; the tp-6 cells with both faces in the tile write their
+    partial result, so tiles along a line step by tp-6 cells and consecutive
+    tiles share one face."""
     d = make_desc(200, 200, 64, fast_math=1)
     n = [200, 200, 64]
     for q in range(3):
         other = [n[a] for a in range(3) if a != q]
         lines = other[0] * other[1]
-        faces = n[q] + 1
-        segs = math.ceil(faces / (tp - 5))
+        cpt = tp - 6
+        segs = math.ceil(n[q] / cpt)
         assert capi.tile_count(d, capi.PHASE_FLUX_X + q, tp, tk) == math.ceil(lines / tk) * segs
-        # the faces the segments own cover [lo-1, hi-1] exactly once
-        owned = np.concatenate([np.arange(s * (tp - 5), min((s + 1) * (tp - 5), faces)) for s in range(segs)])
-        assert np.array_equal(owned, np.arange(faces))
+        # tile s starts its faces at cell lo-1+s*cpt (position 2) and owns the
+        # cells at positions 3 .. tp-4 of the line that lie in [lo, hi)
+        owned = []
+        for s in range(segs):
+            c0 = s * cpt - 1
+            owned += [c0 + ps - 2 for ps in range(3, tp - 3) if 0 <= c0 + ps - 2 < n[q]]
+        assert owned == list(range(n[q]))
     assert capi.tile_count(d, capi.PHASE_FLUX_X, 7, 4) == -1
 
 

@@ -300,11 +300,19 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(const EvDev* __restrict__ tab,
 // cell reconstructs its left and right face values from one set of WENO
 // smoothness indicators, and a face c+1/2 takes L from cell c and R from cell
 // c+1 through a second, double-buffered LDS array.  TP-5 faces per line per
-// tile; the tile shape is a launch parameter.  Slots map to cells with the
-// contiguous (eta) index fastest so loads coalesce in every direction.
+// tile, staged in LDS; each of the TP-6 cells whose two faces both lie in the
+// tile writes its partial result -(1/h)(f(c+1/2) - f(c-1/2)) (the update
+// reads it as it is, with no re-read of a neighbour's face), so consecutive
+// tiles of a line overlap by one face.  The tile shape is a launch
+// parameter.  Slots map to cells with the contiguous (eta) index fastest so
+// loads coalesce in every direction.
 //
 // NT threads per block; each thread owns up to MAXR slots (TP*TK <= MAXR*NT,
-// at most 1024 slots = 48 KiB of LDS).
+// at most 1024 slots = 56 KiB of LDS).
+// LDS arrays of a flux tile, one double per slot each: cell F and V, the
+// double-buffered right face values of F and V, the face fluxes of the row
+constexpr int kTileArrays = 7;
+
 template <int NT>
 struct TileSlots {
   static constexpr int MAXR = NT >= 256 ? 2 : 4;
@@ -325,18 +333,19 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
   double* rV0 = lds + 3 * S;
   double* rF1 = lds + 4 * S;
   double* rV1 = lds + 5 * S;
+  double* Bf = lds + 6 * S;  // face fluxes of the current row
   const int na = D == 0 ? g.jr - g.jl : g.ir - g.il;  // slow line coordinate
   const int nb = D == 2 ? g.jr - g.jl : g.kr - g.kl;  // fast line coordinate
   const long nlines = (long)na * nb;
   const int lo = D == 0 ? g.il : D == 1 ? g.jl : g.kl;
   const int hi = D == 0 ? g.ir : D == 1 ? g.jr : g.kr;
   const int GD = D == 0 ? g.gx : D == 1 ? g.gy : g.gz;
-  const int fpt = TP - 5;
-  const int nface = hi - lo + 1;
-  const int nseg = (nface + fpt - 1) / fpt;
+  const int fpt = TP - 5;            // faces per tile
+  const int cpt = fpt - 1;           // cells per tile: both of their faces are in the tile
+  const int nseg = (hi - lo + cpt - 1) / cpt;
   const long lgrp = tile / nseg;
   const int sg = (int)(tile % nseg);
-  const int c0 = lo - 1 + sg * fpt;  // first face cell of this tile
+  const int c0 = lo - 1 + sg * cpt;  // first face cell of this tile
   const int p0 = c0 - 2;             // cell at tile position 0
   const long n = g.n;
   const int milne = CART ? 0 : p.milne;
@@ -344,10 +353,11 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
   const long sd = D == 0 ? g.sx : D == 1 ? g.sy : 1;
   const int row = D + 1;
   const double tf = milne ? tau : 1.0;
+  const double hD = D == 0 ? p.dx : D == 1 ? p.dy : p.deta;
 
   long xs[MAXR];
   int pos[MAXR];
-  bool cell_ok[MAXR], face_ok[MAXR], in_[MAXR];
+  bool cell_ok[MAXR], face_ok[MAXR], own_ok[MAXR], in_[MAXR];
   double u0v[MAXR], udv[MAXR], vdv[MAXR], wv[MAXR], ppv[MAXR], amax[MAXR], LF[MAXR], LV[MAXR];
   double pT[MAXR][4];  // Cartesian pi^{row,m} of each owned cell
 #pragma unroll
@@ -355,6 +365,7 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     const int sl = threadIdx.x + r * BLK;
     cell_ok[r] = false;
     face_ok[r] = false;
+    own_ok[r] = false;
     in_[r] = false;
     xs[r] = 0;
     pos[r] = -100;
@@ -374,6 +385,7 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     cell_ok[r] = true;
     in_[r] = cd >= lo && cd < hi;
     face_ok[r] = ps >= 2 && ps < 2 + fpt && cd <= hi - 1;
+    own_ok[r] = ps >= 3 && ps < 2 + fpt && cd >= lo && cd < hi;
   }
   // common terms of each owned cell
 #pragma unroll
@@ -461,8 +473,20 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
         amax[r] = fmax(ml, mr);
       } else {
         const double FR = rF[sl + st1];
-        flux[(long)q * n + xs[r]] = fma(-0.5 * amax[r], VR - LV[r], 0.5 * (LF[r] + FR));
+        Bf[sl] = fma(-0.5 * amax[r], VR - LV[r], 0.5 * (LF[r] + FR));
       }
+    }
+    if (q < 0) continue;  // block-uniform
+    __syncthreads();
+    // the cell's partial result -(1/h)(f(c+1/2) - f(c-1/2)), as the update
+    // would form it from the face fluxes
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) {
+      if (!own_ok[r]) continue;
+      const int sl = threadIdx.x + r * BLK;
+      const int st1 = D < 2 ? TK : 1;
+      const double df = Bf[sl] - Bf[sl - st1];
+      flux[(long)q * n + xs[r]] = -df * (1.0 / hD);
     }
   }
 }
@@ -802,11 +826,11 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
       for (int l = 0; l < SV; l++) {
         if (CART && l >= 9 && l < 14) continue;
         double pr;
-        if (p.scheme == VIRAL_SHASTA) {
-          pr = f[l * n + x];  // SHASTA kernels store the partial result itself
+        if (FAST || p.scheme == VIRAL_SHASTA) {
+          pr = f[l * n + x];  // the fast KT tiles and SHASTA store the partial result itself
         } else {
           const double df = f[l * n + x] - f[l * n + x - sd[dd]];
-          pr = FAST ? -df * (1.0 / hh[dd]) : -(1.0 / hh[dd]) * (df);
+          pr = -(1.0 / hh[dd]) * (df);
         }
         Res[l] = Res[l] + pr;
       }
@@ -1948,7 +1972,8 @@ static long tile_count(const viral_desc* d, int ph, int tp, int tk)
       const int nb = dd == 2 ? g.jr - g.jl : g.kr - g.kl;
       const int lo = dd == 0 ? g.il : dd == 1 ? g.jl : g.kl, hi = dd == 0 ? g.ir : dd == 1 ? g.jr : g.kr;
       const long nlines = (long)na * nb;
-      const long nseg = (hi - lo + 1 + tp - 6) / (tp - 5);
+      const long cpt = tp - 6;  // cells per tile (tp - 5 faces)
+      const long nseg = (hi - lo + cpt - 1) / cpt;
       return ((nlines + tk - 1) / tk) * nseg;
     }
     case PH_CELLS: return nblocks(ni);
@@ -1994,7 +2019,7 @@ static int refresh_table(viral_ctx* ctx)
   }
   for (int d = 0; d < 3; d++) {
     size_t m = 0;
-    for (const Event& E : ctx->ev) m = std::max(m, 6 * (size_t)E.tile_tp[d] * E.tile_tk[d] * sizeof(double));
+    for (const Event& E : ctx->ev) m = std::max(m, kTileArrays * (size_t)E.tile_tp[d] * E.tile_tk[d] * sizeof(double));
     ctx->shm[d] = m;
   }
   for (int e = 0; e < nev; e++) {
@@ -2094,7 +2119,7 @@ static void launch_flux(viral_ctx* c, int d, int stage, int cur)
                                stage, cur, d);
         } else if (c->fast) {
           const size_t shm =
-              e < 0 ? c->shm[d] : 6 * (size_t)c->ev[e].tile_tp[d] * c->ev[e].tile_tk[d] * sizeof(double);
+              e < 0 ? c->shm[d] : kTileArrays * (size_t)c->ev[e].tile_tp[d] * c->ev[e].tile_tk[d] * sizeof(double);
           launch_tile<P, CART>(c, d, c->cfg.flux_nt[d], s, grid, shm, stage, cur);
         } else {
           hipLaunchKernelGGL((k_flux_dir<false, P, CART>), dim3(grid), dim3(BLK), 0, c->stream, c->d_tab, s,
@@ -2917,7 +2942,7 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps)
           int r;
           if (ev >= 0) {
             Sched s{nullptr, ctx->cnt[PH_FLUX_X + d][ev], ev, 1, ctx->cfg.xcd};
-            const size_t shm = 6 * (size_t)ctx->ev[ev].tile_tp[d] * ctx->ev[ev].tile_tk[d] * sizeof(double);
+            const size_t shm = kTileArrays * (size_t)ctx->ev[ev].tile_tp[d] * ctx->ev[ev].tile_tk[d] * sizeof(double);
             const unsigned grid = grid_for(ctx, s.total);
             r = VIRAL_OK;
             with_bool(ctx->cfg.blocks_per_cu > 0, [&](auto PT) {
