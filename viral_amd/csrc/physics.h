@@ -89,6 +89,19 @@ __host__ __device__ __forceinline__ double dpow(double x, double y) { return dex
 // of noct octaves, holding (p, dp/de, T, s); 4-point Lagrange interpolation
 // inside an octave, closed form where a node stencil straddles a branch point
 // of the piecewise EOS (flag array, one byte per interval).
+// Element i of a device array reached through a parameter struct: a global
+// load in device code (a pointer read from memory is generic to the compiler,
+// which would emit a flat load), a plain load on the host.
+template <class T>
+__host__ __device__ __forceinline__ T gload(const T* p, long i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+#else
+  return p[i];
+#endif
+}
+
 struct EosTab {
   const double* v;             // [noct][K+3][4]
   const unsigned char* closed; // [noct][K]
@@ -249,7 +262,7 @@ __host__ __device__ __forceinline__ bool eos_tab_lookup(const EosTab& t, double 
   const double u = (2.0 * m - 1.0) * t.K;
   int q = (int)u;
   if (q > t.K - 1) q = t.K - 1;
-  if (t.closed[oct * t.K + q]) return false;
+  if (gload(t.closed, (long)oct * t.K + q)) return false;
   const double x = u - q;
   const double xm = x - 1.0, xp = x + 1.0, x2 = x - 2.0;
   const double w0 = -(x * xm * x2) * (1.0 / 6.0);
@@ -258,7 +271,8 @@ __host__ __device__ __forceinline__ bool eos_tab_lookup(const EosTab& t, double 
   const double w3 = (xp * x * xm) * (1.0 / 6.0);
   const double* b = t.v + ((long)oct * (t.K + 3) + q) * 4;
 #pragma unroll
-  for (int c = 0; c < 4; c++) out[c] = fma(w0, b[c], fma(w1, b[4 + c], fma(w2, b[8 + c], w3 * b[12 + c])));
+  for (int c = 0; c < 4; c++)
+    out[c] = fma(w0, gload(b, c), fma(w1, gload(b, 4 + c), fma(w2, gload(b, 8 + c), w3 * gload(b, 12 + c))));
   return true;
 }
 
@@ -281,7 +295,8 @@ __host__ __device__ __forceinline__ bool eos_tab_lookup_d(const EosTab& t, doubl
   const double dude = ldexp(2.0 * t.K, -E);
   const double* b = t.v + ((long)oct * (t.K + 3) + q) * 4;
 #pragma unroll
-  for (int c = 0; c < 4; c++) dout[c] = (d0 * b[c] + d1 * b[4 + c] + d2 * b[8 + c] + d3 * b[12 + c]) * dude;
+  for (int c = 0; c < 4; c++)
+    dout[c] = (d0 * gload(b, c) + d1 * gload(b, 4 + c) + d2 * gload(b, 8 + c) + d3 * gload(b, 12 + c)) * dude;
   return true;
 }
 

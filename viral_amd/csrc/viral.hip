@@ -44,6 +44,18 @@ struct Geom {
 
 __device__ __forceinline__ long cidx(const Geom& g, int i, int j, int k) { return ((long)i * g.gy + j) * g.gz + k; }
 
+// The device arrays a kernel finds through the event table are global memory,
+// but a pointer loaded from memory is generic to the compiler, which then
+// emits flat loads and stores: those count against the LDS wait counter as
+// well, so every LDS wait also waits for them, and they take 64-bit vector
+// addresses.  A global-address-space pointer gives global loads and stores
+// (scalar base plus vector offset) whose waits are separate from LDS waits.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p)
+{
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ bool interior(const Geom& g, int i, int j, int k)
 {
   return i >= g.il && i < g.ir && j >= g.jl && j < g.jr && k >= g.kl && k < g.kr;
@@ -321,11 +333,13 @@ struct TileSlots {
 
 template <int D, int NT, bool CART>
 __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, double tau,
-                                               const double* __restrict__ st, double* __restrict__ flux, int TP,
+                                               const double* __restrict__ st_, double* __restrict__ flux_, int TP,
                                                int TK, long tile, double* lds)
 {
   constexpr int MAXR = TileSlots<NT>::MAXR;
   constexpr int BLK = NT;
+  const auto* __restrict__ st = as_global(st_);
+  auto* __restrict__ flux = as_global(flux_);
   const int S = TP * TK;
   double* sF = lds;
   double* sV = lds + S;
@@ -410,29 +424,45 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
       for (int m = 0; m < 4; m++) pT[r][m] = pt[pidx(row, m)];
     }
   }
+  // The state values row q reads: field q (T^{tau q}, pi[q-4] or PI) and, for
+  // q < 4, pi^{row q} and u^q.  (Loading them one row ahead costs a wave per
+  // SIMD in registers and is slower.)
+  double ldA[MAXR], ldB[MAXR], ldC[MAXR];
+  auto row_loads = [&](int q) {
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) {
+      ldA[r] = ldB[r] = ldC[r] = 0.0;
+      if (!cell_ok[r]) continue;
+      const long y = xs[r];
+      ldA[r] = st[(long)q * n + y];
+      if (q < 4) {
+        if (!CART) ldB[r] = st[(long)(VIRAL_F_PI0 + pidx(row, q)) * n + y];
+        ldC[r] = st[(long)(VIRAL_F_U0 + q) * n + y];
+      }
+    }
+  };
   // quantity -1: eigenvalue -> face speeds; 0..14: flux rows
   for (int q = -1; q < SV; q++) {
     if (CART && q >= 9 && q < 14) continue;  // block-uniform: no barrier is skipped by part of the block
     double* rF = (q & 1) ? rF1 : rF0;
     double* rV = (q & 1) ? rV1 : rV0;
+    if (q >= 0) row_loads(q);
 #pragma unroll
     for (int r = 0; r < MAXR; r++) {
       const int sl = threadIdx.x + r * BLK;
       if (sl >= S) continue;
       double F = 0.0, V = 0.0;
       if (cell_ok[r]) {
-        const long y = xs[r];
         if (q < 0) {
           V = (in_[r] || !q1) ? vdv[r] : 0.0;
         } else if (q < 4) {
-          const double piv = CART ? (q == 0 ? pT[r][0] : q == 1 ? pT[r][1] : q == 2 ? pT[r][2] : pT[r][3])
-                                  : st[(long)(VIRAL_F_PI0 + pidx(row, q)) * n + y];
-          const double um = st[(long)(VIRAL_F_U0 + q) * n + y];
+          const double piv = CART ? (q == 0 ? pT[r][0] : q == 1 ? pT[r][1] : q == 2 ? pT[r][2] : pT[r][3]) : ldB[r];
+          const double um = ldC[r];
           const double v = (q == row) ? piv + ppv[r] + udv[r] * um * wv[r] : piv + udv[r] * um * wv[r];
           F = milne ? tau * v : v;
-          V = milne ? tf * st[(long)q * n + y] : st[(long)q * n + y];
+          V = milne ? tf * ldA[r] : ldA[r];
         } else {
-          V = st[(long)q * n + y];  // pi[q-4] or PI: Var row q is state field q
+          V = ldA[r];  // pi[q-4] or PI: Var row q is state field q
           F = V * vdv[r];
         }
       }
@@ -514,8 +544,10 @@ __global__ void __launch_bounds__(NT) k_flux_tile(const EvDev* __restrict__ tab,
 // The Q+- factors of the rows that share u^d/u^0 are computed once.
 template <bool FAST>
 __device__ __forceinline__ void shasta_cell(const Geom& g, const Params& p, int d, double tau, double taustep,
-                                            const double* __restrict__ st, double* __restrict__ pr, long t)
+                                            const double* __restrict__ st_, double* __restrict__ pr_, long t)
 {
+  auto* __restrict__ st = as_global(st_);
+  auto* __restrict__ pr = as_global(pr_);
   int i, j, k;
   if (!box_cell(interior_box(g), t, i, j, k)) return;
   const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
@@ -972,8 +1004,9 @@ __global__ void __launch_bounds__(BLK) k_update(const EvDev* __restrict__ tab, S
 }
 
 // NEWEST10/root.h:426-449: every eta slice takes the velocity of slice k0
-__device__ __forceinline__ void k0_cell(const Geom& g, const Params& p, double tau, double* __restrict__ st, long t)
+__device__ __forceinline__ void k0_cell(const Geom& g, const Params& p, double tau, double* __restrict__ st_, long t)
 {
+  auto* __restrict__ st = as_global(st_);
   int i, j, k;
   if (!box_cell(interior_box(g), t, i, j, k)) return;
   if (k == g.k0) return;
@@ -1019,8 +1052,9 @@ __host__ __device__ inline long ghost_count(const Geom& g, int lbi)
   return 2L * VIRAL_BORDER * (ny * nz + nx * nz + (lbi ? 0 : nx * ny));
 }
 
-__device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, double* __restrict__ st, int q, long t)
+__device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, double* __restrict__ st_, int q, long t)
 {
+  auto* __restrict__ st = as_global(st_);
   const int f = c_bfields[q];
   const long n = g.n;
   const int nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
