@@ -841,6 +841,15 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
   const long n = g.n, x = cidx(g, i, j, k);
   const int milne = CART ? 0 : p.milne;
   const double tau = a.tau_src;
+  // ---- sources first: the fused source evaluation's registers are free again
+  // before the 15 sums of the flux divergence are live
+  double S[SV];
+  if (FUSED) {
+    source_terms<FAST, CART>(g, p, a, si, x, S);
+  } else {
+#pragma unroll
+    for (int l = 0; l < SV; l++) S[l] = (CART && l >= 9 && l < 14) ? 0.0 : src[l * n + x];
+  }
   // ---- Result = sum over directions of -(1/h)(f(c+1/2) - f(c-1/2)), x then y then eta
   double Res[SV];
   {
@@ -867,13 +876,6 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
         Res[l] = Res[l] + pr;
       }
     }
-  }
-  double S[SV];
-  if (FUSED) {
-    source_terms<FAST, CART>(g, p, a, si, x, S);
-  } else {
-#pragma unroll
-    for (int l = 0; l < SV; l++) S[l] = (CART && l >= 9 && l < 14) ? 0.0 : src[l * n + x];
   }
   const double e = si[VIRAL_F_EN * n + x];
   double u[4], pu[4], pi[10];
@@ -954,17 +956,19 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
 #pragma unroll
   for (int l = 0; l < 10; l++) pi[l] = (V[4 + l]);
   const double PIn = (V[14]);
+  // ---- store T/pi/PI before the root: pi[4..9] need no register across the
+  // Newton loop
+#pragma unroll
+  for (int l = 0; l < 4; l++) so[(VIRAL_F_T00 + l) * n + x] = T0[l];
+#pragma unroll
+  for (int l = 0; l < 10; l++) so[(VIRAL_F_PI0 + l) * n + x] = pi[l];
+  so[VIRAL_F_BULK * n + x] = PIn;
   // ---- root and velocities (NEWEST10/root.h:301-424)
   int it;
   unsigned fl;
   double en, Temp, Pn, vx, vy, ve, gm;
   solve_cell<FAST>(p, milne, tn, T0, pi, PIn, e, en, Pn, Temp, vx, vy, ve, gm, it, fl);
   // ---- store
-#pragma unroll
-  for (int l = 0; l < 4; l++) so[(VIRAL_F_T00 + l) * n + x] = T0[l];
-#pragma unroll
-  for (int l = 0; l < 10; l++) so[(VIRAL_F_PI0 + l) * n + x] = pi[l];
-  so[VIRAL_F_BULK * n + x] = PIn;
   so[VIRAL_F_EN * n + x] = en;
   so[VIRAL_F_P * n + x] = Pn;
   so[VIRAL_F_TEMP * n + x] = Temp;
