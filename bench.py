@@ -42,31 +42,38 @@ def eos_p_vec(desc):
 
 # Compulsory HBM bytes per launch of each kernel: every input field read
 # once and every output written once over the cells/faces of one launch.
-#   flux_x/_y/_eta (per face): 21 state fields read (e, P, PI, u^mu,
-#     pi^{mu nu}, T^{tau mu}) + 15 face fluxes written = 36 doubles = 288 B
+#   flux_x/_y/_eta, fast LDS tiles (per interior cell): 21 state fields read
+#     (e, P, PI, u^mu, pi^{mu nu}, T^{tau mu}) + the cell's 15 partial results
+#     written = 36 doubles = 288 B; exact per-face kernel: the same 36 doubles
+#     per face (15 face fluxes written)
 #   source (per interior cell): e, P, PI, u (4), prevu (4), pi (10) read,
 #     15 sources written = 36 doubles = 288 B
-#   update (per interior cell): 45 face fluxes + 15 sources + e, u, prevu (9)
-#     read, 29 state fields written, plus per stage kind the RK registers:
-#     first stage reads the 15 evolved fields and writes Var0, Var1 (30);
-#     RK2 final reads Var0, Var1 (30); RK3 middle reads Var0, Var1 and writes
-#     Var2 (30 + 15); RK3 final reads Var0, Var2 (30); + 4 B iteration count,
-#     1 B flags (+1 B read after the first stage).  Averaged over the stages.
-def update_doubles(rk):
-    base = 45 + 15 + 9 + 29
-    kinds = {1: [15 + 30], 2: [15 + 30, 30], 3: [15 + 30, 30 + 15, 30]}[rk]
+#   update (per interior cell): 45 partial results, 29 state fields written,
+#     plus per stage kind the RK registers: first stage writes Var0, Var1
+#     (30); RK2 final reads Var0, Var1 (30); RK3 middle reads Var0, Var1 and
+#     writes Var2 (30 + 15); RK3 final reads Var0, Var2 (30); + 4 B
+#     iteration count, 1 B flags (+1 B read after the first stage).
+#     Sources in their own kernel: 15 sources + e, u, prevu (9) read, and the
+#     first stage reads the 15 evolved fields.  Sources fused in: the
+#     source's 21 inputs read, and the first stage reads T^{tau mu} (4; pi
+#     and PI are among the 21).  Averaged over the stages.
+def update_doubles(rk, fused):
+    base = 45 + 29 + (21 if fused else 15 + 9)
+    first = (4 if fused else 15) + 30
+    kinds = {1: [first], 2: [first, 30], 3: [first, 30 + 15, 30]}[rk]
     return [base + k for k in kinds]
 
 
-def kernel_bytes(name, d):
+def kernel_bytes(name, d, fused):
     faces = [(d.nx + 1) * d.ny * d.nz, d.nx * (d.ny + 1) * d.nz, d.nx * d.ny * (d.nz + 1)]
     cells = d.nx * d.ny * d.nz
     if name.startswith("flux_"):
-        return faces[{"flux_x": 0, "flux_y": 1, "flux_eta": 2}[name]] * 288.0
+        q = {"flux_x": 0, "flux_y": 1, "flux_eta": 2}[name]
+        return (cells if d.fast_math else faces[q]) * 288.0
     if name == "source":
         return cells * 288.0
     if name == "update":
-        per = update_doubles(d.rk)
+        per = update_doubles(d.rk, fused)
         avg = sum(8.0 * x + 5.0 + (1.0 if q else 0.0) for q, x in enumerate(per)) / len(per)
         return cells * avg
     if name == "k0_broadcast":
@@ -78,17 +85,20 @@ def kernel_bytes(name, d):
 # on the 200x200x64 RK3 fast-mode workload (tabulated s95p), each counter
 # scaled by what it reads for a 1 GiB device copy (read x2.00, write x1.00);
 # profiles/counters_kt_s95p_glauber_200x200x64_rk3.md lists the passes.
+# The update kernel's figure is for its fused form (sources evaluated inside).
 MEASURED_TRAFFIC = {
-    "update": 3.701e9,
-    "flux_x": 1.458e9,
-    "flux_y": 1.457e9,
-    "flux_eta": 1.371e9,
-    "source": 1.273e9,
+    "update_fused": 3.786e9,
+    "flux_x": 1.110e9,
+    "flux_y": 1.101e9,
+    "flux_eta": 0.966e9,
+    "k0_broadcast": 0.168e9,
 }
 
 
-def measured_traffic(name, d):
+def measured_traffic(name, d, fused):
     if (d.nx, d.ny, d.nz, d.rk, d.fast_math, d.eos_table) == (200, 200, 64, 3, 1, 1):
+        if name == "update":
+            return MEASURED_TRAFFIC.get("update_fused") if fused else None
         return MEASURED_TRAFFIC.get(name)
     return None
 
@@ -311,7 +321,8 @@ def main():
     dom = max(rep, key=lambda k: rep[k][0])
     tot_ms, launches = rep[dom]
     avg_s = tot_ms / 1000.0 / max(launches, 1)
-    bytes_launch = kernel_bytes(dom, d)
+    fused = launch["fusion"] > 0
+    bytes_launch = kernel_bytes(dom, d, fused)
     achieved = bytes_launch / avg_s / 1e9
     kernel_share = {k: round(v[0] / max(sum(x[0] for x in rep.values()), 1e-12), 4) for k, v in rep.items()}
     bytes_per_update = {1: 336, 2: 784, 3: 1232}[d.rk]
@@ -334,7 +345,7 @@ def main():
                    "flux_tiles": tiles, "launch": launch,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, d),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, d, fused),
                      "avg_launch_ms": avg_s * 1000.0, "launches": launches,
                      "algorithmic_bytes_per_launch": bytes_launch},
         "step_hbm": {"algorithmic_bytes_per_cell_update": bytes_per_update,

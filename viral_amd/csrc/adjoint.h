@@ -14,6 +14,11 @@
 
 namespace viral {
 
+// The reverse pass matches finite differences, not a bitwise operation order,
+// so its expressions may contract into FMAs (the forward kernels are compiled
+// without contraction to keep the reference's rounding).
+#pragma clang fp contract(fast)
+
 #define VHD __host__ __device__ __forceinline__
 
 // d(L or R)/d(x0..x4) of the WENO5-JS reconstruction of NEWEST10/utils.h:
@@ -44,12 +49,12 @@ VHD double weno_grad(const double x[5], bool right, double g[5])
       q[c] += (right ? QR[c][i] : QL[c][i]) * x[i];
     }
     q[c] = q[c] * (1.0 / 6.0);
-    const double it = 1.0 / (WENO_EPS + (13. / 12.) * (A[c] * A[c]) + 0.25 * (C[c] * C[c]));
+    const double it = frcp(WENO_EPS + (13. / 12.) * (A[c] * A[c]) + 0.25 * (C[c] * C[c]));
     al[c] = (right ? dR[c] : dL[c]) * (it * it);
     k2[c] = -2.0 * al[c] * it;  // da_c/db_c
     sum += al[c];
   }
-  const double rs = 1.0 / sum;
+  const double rs = frcp(sum);
   double w[3], out = 0;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -139,7 +144,7 @@ VHD void flux_cell_vjp(int d, double tau, double e, double P, double PI, const d
     ub[l] += vb * ud * w;
     wb += vb * ud * u[l];
   }
-  const double iu0 = 1.0 / u0;
+  const double iu0 = frcp(u0);
 #pragma unroll
   for (int l = 0; l < 11; l++) {
     const double f = l < 10 ? pi10[l] : PI;
@@ -176,8 +181,8 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
   const double gi[4] = {1.0, -1.0, -1.0, milne ? -(1.0 / (tau * tau)) : -1.0};
   double ul[4];
   for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
-  const double iu0 = 1.0 / u[0];
-  const double itpi = 1.0 / tpi;
+  const double iu0 = frcp(u[0]);
+  const double itpi = frcp(tpi);
   double divv = 0;
   for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
   const double itau = milne ? 1.0 / tau : 0.0;
@@ -478,5 +483,7 @@ VHD void update_vjp(const Params& p, double tn, const double T0[4], const double
 }
 
 #undef VHD
+
+#pragma clang fp contract(off)
 
 }  // namespace viral

@@ -1377,6 +1377,9 @@ __global__ void __launch_bounds__(BLK) k_output(Geom g, const double* __restrict
 }
 
 // ---------------------------------------------------------------- adjoint kernels
+// (FMA contraction on: the reverse pass is checked against finite differences,
+// not against a bitwise operation order)
+#pragma clang fp contract(fast)
 // Reverse of one Milne KT stage for one event (conformal or s95p EOS, the
 // latter in closed form or, in fast mode, from the table).  lam_out is
 // the adjoint of the stage's output state (29 fields), lam_in receives the
@@ -1842,6 +1845,8 @@ __global__ void __launch_bounds__(NTA) k_adj_tile(Geom g, Params p, double tau, 
   }
   for (int m = 0; m < 10; m++) li[(VIRAL_F_PI0 + m) * n + x] += pib[m];
 }
+
+#pragma clang fp contract(off)
 
 }  // namespace
 
