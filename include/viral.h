@@ -189,6 +189,24 @@ int viral_first_bad_cell(viral_ctx* ctx, int ev, int* i, int* j, int* k, int* st
  * events only (VIRAL_ERR_UNSUPPORTED otherwise); ghosts are not filled. */
 int viral_root(viral_ctx* ctx, int ev, double tau);
 
+/* Navier-Stokes initialisation of the event's current state at tau, the time
+ * derivative of u taken from prevu over ts (replaces CalcNS, NSInit and
+ * ZeroInit, NEWEST10/Source.h:542-776, called from init() at
+ * NEWEST10/init.h:658-686).  Every ghosted cell, with the velocity
+ * gradients CalcDer4Vel (Source.h:3-101) leaves on a freshly zeroed grid:
+ *   VIRAL_NS_OUTPUT  the Navier-Stokes pi = 2 eta sigma^{mu nu} (CalcNS's nspi)
+ *                    into nspi[0..9], host arrays of viral_cells() doubles in
+ *                    the pi index map; the state is unchanged
+ *   VIRAL_NS_INIT    NSInit: pi = that, PI = 0, T^{tau mu} recomputed from
+ *                    e, P, u and pi
+ *   VIRAL_ZERO_INIT  ZeroInit: pi = 0, PI = 0, T^{tau mu} recomputed
+ * Exact arithmetic and the closed-form EOS in every mode.  Milne events only
+ * (VIRAL_ERR_UNSUPPORTED otherwise: TXYZ5 declares but never defines them).
+ * VIRAL_NS_OUTPUT allocates 10 x cells doubles of staging on first use (kept
+ * until viral_destroy, shared with viral_output). */
+enum { VIRAL_NS_OUTPUT = 0, VIRAL_NS_INIT = 1, VIRAL_ZERO_INIT = 2 };
+int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, double* const* nspi);
+
 /* Device reductions the reference's driver runs every step.  Each reads the
  * event's current state on the device (no state download).  Cells are
  * reported as ghosted (i, j, k) triples, -1 -1 -1 where there is none.

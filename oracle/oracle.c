@@ -465,6 +465,134 @@ void or_shear_source(int milne, double tau, const double u[4], const double pi10
   shear_source(milne, tau, u, pi10, du, SH, tpi, out);
 }
 
+/* 2 eta sigma^{mn}, the Navier-Stokes shear stress of CalcNS
+ * (NEWEST10/Source.h:643-751, a computer-algebra expansion): sigma as in
+ * shear_source above, sigma^{mn} = Delta^{mn}_{ab} D^a u^b with the Milne
+ * Christoffel terms. */
+static void ns_shear(int milne, double tau, const double u[4], double du[4][4], double SH, double out[10])
+{
+  double g[4] = {1.0, -1.0, -1.0, milne ? -(tau * tau) : -1.0};
+  double gi[4] = {1.0, -1.0, -1.0, milne ? -(1.0 / (tau * tau)) : -1.0};
+  double ul[4];
+  for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
+  double Dc[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
+  double itau = milne ? 1.0 / tau : 0.0;
+  if (milne) {
+    Dc[0][3] = Dc[0][3] + tau * u[3];
+    Dc[3][0] = Dc[3][0] + itau * u[3];
+    Dc[3][3] = Dc[3][3] + itau * u[0];
+  }
+  double G[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
+  double Gu[4], uG[4], H[4];
+  for (int m = 0; m < 4; m++) {
+    double s1 = 0, s2 = 0;
+    for (int b = 0; b < 4; b++) {
+      s1 += G[m][b] * ul[b];
+      s2 += ul[b] * G[b][m];
+    }
+    Gu[m] = s1;
+    uG[m] = s2;
+  }
+  double uGu = 0, trg = 0;
+  for (int a = 0; a < 4; a++) {
+    uGu += ul[a] * Gu[a];
+    trg += g[a] * G[a][a];
+  }
+  double tr = trg - uGu;
+  for (int m = 0; m < 4; m++) H[m] = Gu[m] + uG[m];
+  double c3 = uGu + (1.0 / 3.0) * tr;
+  for (int m = 0; m < 4; m++)
+    for (int n = m; n < 4; n++) {
+      double sigma = 0.5 * (G[m][n] + G[n][m]) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
+      if (m == n) sigma = sigma - (1.0 / 3.0) * gi[m] * tr;
+      out[PIDX[m][n]] = 2.0 * SH * sigma;
+    }
+}
+
+/* The velocity gradients CalcDer4Vel (NEWEST10/Source.h:3-101) leaves on a
+ * freshly allocated (zeroed) grid, at cell (i,j,k): the time derivative on the
+ * interior; the x derivative on interior y-z rows for 2 <= i < gx-2, the y
+ * derivative on interior x-z rows for 2 <= j < gy-2; the eta derivative on
+ * interior x-y columns for every k, its stencil clamped at both ends
+ * (qm2 = qm1 = qc at k = 0, qm2 = qm1 = q(k-1) at k = 1, and mirrored at the
+ * top); zero everywhere else. */
+static void ns_du(const or_ctx* c, double ts, int i, int j, int k, double du[4][4])
+{
+  const viral_desc* d = &c->d;
+  const int iin = i >= c->il && i < c->ir, jin = j >= c->jl && j < c->jr, kin = k >= c->kl && k < c->kr;
+  const size_t x = IX(c, i, j, k), sx = (size_t)c->gy * c->gz, sy = (size_t)c->gz;
+  for (int l = 0; l < 4; l++) {
+    const double* ul = c->f[VIRAL_F_U0 + l];
+    du[l][0] = (iin && jin && kin) ? (ul[x] - c->f[VIRAL_F_PU0 + l][x]) / ts : 0.0;
+    du[l][1] = (jin && kin && i >= 2 && i < c->gx - 2)
+                   ? or_weno_der(ul[x - 2 * sx], ul[x - sx], ul[x], ul[x + sx], ul[x + 2 * sx], d->dx)
+                   : 0.0;
+    du[l][2] = (iin && kin && j >= 2 && j < c->gy - 2)
+                   ? or_weno_der(ul[x - 2 * sy], ul[x - sy], ul[x], ul[x + sy], ul[x + 2 * sy], d->dy)
+                   : 0.0;
+    du[l][3] = 0.0;
+    if (!d->lbi && iin && jin) {
+      const int Z = c->gz;
+      double qm2, qm1, qc = ul[x], qp1, qp2;
+      if (k > 1 && k < Z - 2) {
+        qm1 = ul[x - 1]; qm2 = ul[x - 2]; qp1 = ul[x + 1]; qp2 = ul[x + 2];
+      } else if (k == 0) {
+        qm2 = qm1 = qc; qp1 = ul[x + 1]; qp2 = ul[x + 2];
+      } else if (k == 1) {
+        qm2 = qm1 = ul[x - 1]; qp1 = ul[x + 1]; qp2 = ul[x + 2];
+      } else if (k == Z - 2) {
+        qm1 = ul[x - 1]; qm2 = ul[x - 2]; qp2 = qp1 = ul[x + 1];
+      } else {
+        qm1 = ul[x - 1]; qm2 = ul[x - 2]; qp2 = qp1 = qc;
+      }
+      du[l][3] = or_weno_der(qm2, qm1, qc, qp1, qp2, d->deta);
+    }
+  }
+}
+
+/* CalcNS / NSInit / ZeroInit (NEWEST10/Source.h:542-562, 643-776) over every
+ * ghosted cell: mode 0 writes the Navier-Stokes pi to nspi (10 x cells);
+ * mode 1 sets pi to it, mode 2 to zero; both then set PI = 0 (no BULK) and
+ * T^{tau mu} = (e + P - PI) u^0 u^mu - (P - PI) g^{tau mu} + pi^{tau mu}.
+ * Milne only (TXYZ5 declares NSInit but does not define it). */
+int or_ns_init(or_ctx* c, double tau, double ts, int mode, double* nspi)
+{
+  const viral_desc* d = &c->d;
+  if (d->coords != VIRAL_MILNE || mode < 0 || mode > 2 || (mode == 0 && !nspi)) return -1;
+  double** f = c->f;
+  const size_t n = c->n;
+  for (int i = 0; i < c->gx; i++)
+    for (int j = 0; j < c->gy; j++)
+      for (int k = 0; k < c->gz; k++) {
+        const size_t x = IX(c, i, j, k);
+        double pn[10] = {0};
+        if (mode != 2) {
+          double du[4][4], u[4];
+          ns_du(c, ts, i, j, k, du);
+          for (int l = 0; l < 4; l++) u[l] = f[VIRAL_F_U0 + l][x];
+          const double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x];
+          const double T = or_eos_T(d, e);
+          const double s = or_eos_s(d, e, P, T);
+          ns_shear(1, tau, u, du, eta_of(d, s), pn);
+        }
+        if (mode == 0) {
+          for (int l = 0; l < 10; l++) nspi[l * n + x] = pn[l];
+          continue;
+        }
+        for (int l = 0; l < 10; l++) f[VIRAL_F_PI0 + l][x] = pn[l];
+        f[VIRAL_F_BULK][x] = 0.0;
+        const double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x], PI = 0.0;
+        const double u0 = f[VIRAL_F_U0][x];
+        f[VIRAL_F_T00][x] = -P + PI + (e + P - PI) * (u0 * u0) + pn[0];
+        for (int l = 1; l < 4; l++) f[VIRAL_F_T00 + l][x] = (e + P - PI) * u0 * f[VIRAL_F_U0 + l][x] + pn[l];
+      }
+  return 0;
+}
+
 /* geometric + shear sources on the interior, NEWEST10/Source.h:195-536
  * (Cartesian: TXYZ5/Source.h:136-157 zero geometric terms, 270-330 shear for
  * the five evolved components; under q9 only ghosted k < nz) */

@@ -98,6 +98,11 @@ void or_check_physics(or_ctx* c, double tau, long long n[2], long long first[2])
  * doubles, 0 when the strides do not divide the grid; out may be NULL */
 size_t or_output(or_ctx* c, int freq, int freqz, int xy, double* out);
 
+/* CalcNS (mode 0: Navier-Stokes pi into nspi, 10 x cells), NSInit (1) and
+ * ZeroInit (2) over every ghosted cell (NEWEST10/Source.h:542-776); -1 on a
+ * Cartesian context or bad mode. */
+int or_ns_init(or_ctx* c, double tau, double ts, int mode, double* nspi);
+
 /* deterministic math (oracle/vmath.h) exported for tests */
 double or_vm_exp(double x);
 double or_vm_log(double x);

@@ -30,6 +30,9 @@
  *   mode 3: the initial state of the profile's own init (initvar,
  *           init.h:150-214, for the default profile), after the cell
  *           positions init() sets (init.h:558-566).
+ *   mode 4: CalcNS (Source.h:643-751) on a random state: the Navier-Stokes
+ *           pi (nspi, nsPI), then the state after NSInit (Source.h:755-776).
+ *   mode 5: the state after ZeroInit (Source.h:542-562) of a random state.
  */
 #include <mpi.h>
 #include <iostream>
@@ -222,6 +225,26 @@ int main(int argc, char** argv)
   for (int l = 0; l < 4; l++) DUMP(c.prevu[l]);
 
   if (mode == 3) {
+    fclose(f);
+    MPI_Finalize();
+    return 0;
+  }
+  if (mode == 4 || mode == 5) {
+    /* on the freshly allocated grid, as at init: du is zero wherever
+     * CalcDer4Vel does not write it */
+    if (mode == 4) {
+      CalcNS(HydroGrid, tau, ts);
+      for (int l = 0; l < 10; l++) DUMP(c.nspi[l]);
+      DUMP(c.nsPI);
+      NSInit(HydroGrid, tau, ts);
+    } else {
+      ZeroInit(HydroGrid, tau, ts);
+    }
+    DUMP(c.T00); DUMP(c.T10); DUMP(c.T20); DUMP(c.T30);
+    for (int l = 0; l < 10; l++) DUMP(c.pi[l]);
+    DUMP(c.PI); DUMP(c.En); DUMP(c.P); DUMP(c.Temp); DUMP(c.Vx); DUMP(c.Vy); DUMP(c.Ve);
+    for (int l = 0; l < 4; l++) DUMP(c.u[l]);
+    for (int l = 0; l < 4; l++) DUMP(c.prevu[l]);
     fclose(f);
     MPI_Finalize();
     return 0;

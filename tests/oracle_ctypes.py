@@ -78,6 +78,7 @@ def lib():
         L.or_check_physics.argtypes = [vp, C.c_double, lp, lp]
         L.or_output.restype = C.c_size_t
         L.or_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+        L.or_ns_init.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -230,3 +231,13 @@ class Oracle:
         out = np.zeros(m)
         self.L.or_output(self.h, freq, freqz, xy, _dp(out))
         return out
+
+    def ns_pi(self, tau, ts):
+        """CalcNS's Navier-Stokes pi of the current state, (10, cells)"""
+        out = np.zeros((10, self.n))
+        assert self.L.or_ns_init(self.h, tau, ts, 0, _dp(out)) == 0
+        return out
+
+    def ns_init(self, tau, ts, zero=False):
+        """NSInit (or ZeroInit) in place"""
+        assert self.L.or_ns_init(self.h, tau, ts, 2 if zero else 1, None) == 0

@@ -35,7 +35,8 @@ def run(name, seed, tau, ts, mode=0):
     """mode 0: source, KT/SHASTA result and ghost fill of a random state;
     mode 1: s95p samples; mode 2: DebugMSG/CheckPhysics printout, BMax,
     MaxTempGev and the WriteResults[XY] file of a random state with planted
-    cells (see oracle/ref_harness.cpp)."""
+    cells; mode 4: CalcNS's Navier-Stokes pi and the state after NSInit;
+    mode 5: the state after ZeroInit (see oracle/ref_harness.cpp)."""
     exe = os.path.join(REF_DIR, name)
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, "out.bin")
@@ -46,6 +47,24 @@ def run(name, seed, tau, ts, mode=0):
         written = {}
         for fn in os.listdir(os.path.join(td, "res")):
             written[fn] = open(os.path.join(td, "res", fn), "rb").read()
+    if mode in (4, 5):
+        hdr = np.frombuffer(raw[:32], dtype=np.int32)
+        _, gx, gy, gz, border, nz, one_rank, lbi = [int(v) for v in hdr]
+        th = np.frombuffer(raw[32:48], dtype=np.float64)
+        n = gx * gy * gz
+        blocks = np.frombuffer(raw[48:], dtype=np.float64).reshape(-1, gx, gy, gz)
+        if lbi and gz != 1:
+            blocks = blocks[:, :, :, :1]
+        blocks = blocks.reshape(blocks.shape[0], -1)
+        out = dict(desc=make_desc(**PROFILES[name]), tau=th[0], ts=th[1], state=blocks[:NFIELDS].copy())
+        if mode == 4:
+            out["nspi"] = blocks[NFIELDS:NFIELDS + 10].copy()
+            out["nsPI"] = blocks[NFIELDS + 10].copy()
+            out["after"] = blocks[NFIELDS + 11:].copy()
+        else:
+            out["after"] = blocks[NFIELDS:].copy()
+        assert out["after"].shape[0] == NFIELDS
+        return out
     if mode in (2, 3):
         hdr = np.frombuffer(raw[:32], dtype=np.int32)
         _, gx, gy, gz, border, nz, one_rank, lbi = [int(v) for v in hdr]

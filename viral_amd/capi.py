@@ -24,7 +24,11 @@ EXPORTS = [
     "viral_set_launch", "viral_tile_count", "viral_tile_table", "viral_xcd_tile", "viral_adjoint_begin",
     "viral_adjoint", "viral_adjoint_end", "viral_check_root", "viral_debug_extrema", "viral_check_physics",
     "viral_output_size", "viral_output", "viral_sync", "viral_host_alloc", "viral_host_free", "viral_root",
+    "viral_ns_init",
 ]
+
+# modes of viral_ns_init
+NS_OUTPUT, NS_INIT, ZERO_INIT = 0, 1, 2
 
 # kernel phases of the tile table (VIRAL_PHASE_*)
 PHASE_FLUX_X, PHASE_FLUX_Y, PHASE_FLUX_ETA, PHASE_CELLS, PHASE_K0, PHASE_GHOST = range(6)
@@ -130,6 +134,8 @@ def load():
     L.viral_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int]
     L.viral_root.restype = C.c_int
     L.viral_root.argtypes = [vp, C.c_int, C.c_double]
+    L.viral_ns_init.restype = C.c_int
+    L.viral_ns_init.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.POINTER(C.c_double))]
     L.viral_sync.restype = C.c_int
     L.viral_sync.argtypes = [vp]
     L.viral_host_alloc.restype = C.c_int
@@ -336,6 +342,18 @@ class Hydro:
         if r not in (OK, ERR_PHYSICS):
             self._chk(r)
         return r
+
+    def ns_pi(self, tau, ts, ev=0):
+        """CalcNS: the Navier-Stokes pi of every ghosted cell, (10, cells)"""
+        n = self.descs[ev].cells()
+        out = np.zeros((10, n))
+        ptrs = (C.POINTER(C.c_double) * 10)(*[out[l].ctypes.data_as(C.POINTER(C.c_double)) for l in range(10)])
+        self._chk(self.L.viral_ns_init(self.h, ev, tau, ts, NS_OUTPUT, ptrs))
+        return out
+
+    def ns_init(self, tau, ts, zero=False, ev=0):
+        """NSInit (or ZeroInit) on the current state in place"""
+        self._chk(self.L.viral_ns_init(self.h, ev, tau, ts, ZERO_INIT if zero else NS_INIT, None))
 
     def _cells(self, ijk, ev):
         """ghosted linear indices of (i, j, k) triples, -1 where none"""

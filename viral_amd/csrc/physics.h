@@ -1017,4 +1017,52 @@ __device__ __forceinline__ int cart_gnewton(const Params& p, const double prm[10
   return iter;
 }
 
+// ---------------------------------------------------------------- initial state
+// 2 eta sigma^{mn}, the Navier-Stokes shear stress of CalcNS
+// (NEWEST10/Source.h:643-751, written there as a computer-algebra expansion):
+// sigma^{mn} = Delta^{mn}_{ab} D^a u^b with the Milne Christoffel terms, in
+// the operation order of the CPU restatement (shear_source's sigma).
+__host__ __device__ __forceinline__ void ns_shear(double tau, const double u[4], const double du[4][4], double SH,
+                                                  double out[10])
+{
+  const double g[4] = {1.0, -1.0, -1.0, -(tau * tau)};
+  const double gi[4] = {1.0, -1.0, -1.0, -(1.0 / (tau * tau))};
+  double ul[4];
+  for (int m = 0; m < 4; m++) ul[m] = g[m] * u[m];
+  double Dc[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) Dc[a][b] = du[a][b];
+  const double itau = 1.0 / tau;
+  Dc[0][3] = Dc[0][3] + tau * u[3];
+  Dc[3][0] = Dc[3][0] + itau * u[3];
+  Dc[3][3] = Dc[3][3] + itau * u[0];
+  double G[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) G[a][b] = gi[a] * Dc[b][a];
+  double Gu[4], uG[4], H[4];
+  for (int m = 0; m < 4; m++) {
+    double s1 = 0, s2 = 0;
+    for (int b = 0; b < 4; b++) {
+      s1 += G[m][b] * ul[b];
+      s2 += ul[b] * G[b][m];
+    }
+    Gu[m] = s1;
+    uG[m] = s2;
+  }
+  double uGu = 0, trg = 0;
+  for (int a = 0; a < 4; a++) {
+    uGu += ul[a] * Gu[a];
+    trg += g[a] * G[a][a];
+  }
+  const double tr = trg - uGu;
+  for (int m = 0; m < 4; m++) H[m] = Gu[m] + uG[m];
+  const double c3 = uGu + (1.0 / 3.0) * tr;
+  for (int m = 0; m < 4; m++)
+    for (int n = m; n < 4; n++) {
+      double sigma = 0.5 * (G[m][n] + G[n][m]) - 0.5 * (u[n] * H[m] + u[m] * H[n]) + u[m] * u[n] * c3;
+      if (m == n) sigma = sigma - (1.0 / 3.0) * gi[m] * tr;
+      out[pidx(m, n)] = 2.0 * SH * sigma;
+    }
+}
+
 }  // namespace viral

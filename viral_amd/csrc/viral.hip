@@ -1197,6 +1197,86 @@ __global__ void __launch_bounds__(BLK) k_root_k0(Geom g, Params p, double tau, d
   k0_cell(g, p, tau, st, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// ---------------------------------------------------------------- Navier-Stokes initialisation
+// The velocity gradients CalcDer4Vel (NEWEST10/Source.h:3-101) leaves on a
+// freshly allocated (zeroed) grid, at ghosted cell (i,j,k): the time
+// derivative (u - prevu)/ts on the interior; the x derivative on interior y-z
+// rows for 2 <= i < gx-2 and the y derivative on interior x-z rows for
+// 2 <= j < gy-2; the eta derivative on interior x-y columns for every k, the
+// stencil clamped at both ends; zero everywhere else.
+__device__ __forceinline__ void ns_du(const Geom& g, const Params& p, const double* __restrict__ st, int i, int j,
+                                      int k, double ts, double du[4][4])
+{
+  const bool iin = i >= g.il && i < g.ir, jin = j >= g.jl && j < g.jr, kin = k >= g.kl && k < g.kr;
+  const long n = g.n, x = cidx(g, i, j, k);
+  for (int l = 0; l < 4; l++) {
+    const double* ul = st + (VIRAL_F_U0 + l) * n;
+    du[l][0] = (iin && jin && kin) ? (ul[x] - st[(VIRAL_F_PU0 + l) * n + x]) / ts : 0.0;
+    du[l][1] = (jin && kin && i >= 2 && i < g.gx - 2)
+                   ? weno_der(ul[x - 2 * g.sx], ul[x - g.sx], ul[x], ul[x + g.sx], ul[x + 2 * g.sx], p.dx)
+                   : 0.0;
+    du[l][2] = (iin && kin && j >= 2 && j < g.gy - 2)
+                   ? weno_der(ul[x - 2 * g.sy], ul[x - g.sy], ul[x], ul[x + g.sy], ul[x + 2 * g.sy], p.dy)
+                   : 0.0;
+    du[l][3] = 0.0;
+    if (!p.lbi && iin && jin) {
+      const int Z = g.gz;
+      const double qc = ul[x];
+      double qm2, qm1, qp1, qp2;
+      if (k > 1 && k < Z - 2) {
+        qm1 = ul[x - 1]; qm2 = ul[x - 2]; qp1 = ul[x + 1]; qp2 = ul[x + 2];
+      } else if (k == 0) {
+        qm2 = qm1 = qc; qp1 = ul[x + 1]; qp2 = ul[x + 2];
+      } else if (k == 1) {
+        qm2 = qm1 = ul[x - 1]; qp1 = ul[x + 1]; qp2 = ul[x + 2];
+      } else if (k == Z - 2) {
+        qm1 = ul[x - 1]; qm2 = ul[x - 2]; qp2 = qp1 = ul[x + 1];
+      } else {
+        qm1 = ul[x - 1]; qm2 = ul[x - 2]; qp2 = qp1 = qc;
+      }
+      du[l][3] = weno_der(qm2, qm1, qc, qp1, qp2, p.deta);
+    }
+  }
+}
+
+// CalcNS / NSInit / ZeroInit (NEWEST10/Source.h:542-776) over every ghosted
+// cell, one thread per cell.  MODE 0 writes the Navier-Stokes pi to
+// out[10][n] and leaves the state alone; MODE 1 sets pi to it, MODE 2 to
+// zero, and both set PI = 0 (no BULK) and T^{tau mu} = (e + P - PI) u^tau
+// u^mu - (P - PI) g^{tau mu} + pi^{tau mu} in place (a cell writes only its
+// own pi, PI and T; the gradients read u and prevu).  Exact arithmetic and the
+// closed-form EOS in every mode, as the CPU restatement.
+template <int MODE>
+__global__ void __launch_bounds__(BLK) k_ns_init(Geom g, Params p, double tau, double ts, double* __restrict__ st,
+                                                 double* __restrict__ out)
+{
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= g.n) return;
+  const long n = g.n;
+  const int i = (int)(x / g.sx), j = (int)((x / g.sy) % g.gy), k = (int)(x % g.gz);
+  double pn[10];
+  for (int l = 0; l < 10; l++) pn[l] = 0.0;
+  if (MODE != 2) {
+    double du[4][4], u[4];
+    ns_du(g, p, st, i, j, k, ts, du);
+    for (int l = 0; l < 4; l++) u[l] = st[(VIRAL_F_U0 + l) * n + x];
+    const double e = st[VIRAL_F_EN * n + x], P = st[VIRAL_F_P * n + x];
+    const double T = eos_T(p, e);
+    const double s = eos_s(p, e, P, T);
+    ns_shear(tau, u, du, eta_of(p, s), pn);
+  }
+  if (MODE == 0) {
+    for (int l = 0; l < 10; l++) out[l * n + x] = pn[l];
+    return;
+  }
+  for (int l = 0; l < 10; l++) st[(VIRAL_F_PI0 + l) * n + x] = pn[l];
+  st[VIRAL_F_BULK * n + x] = 0.0;
+  const double e = st[VIRAL_F_EN * n + x], P = st[VIRAL_F_P * n + x], PI = 0.0;
+  const double u0 = st[VIRAL_F_U0 * n + x];
+  st[VIRAL_F_T00 * n + x] = -P + PI + (e + P - PI) * (u0 * u0) + pn[0];
+  for (int l = 1; l < 4; l++) st[(VIRAL_F_T00 + l) * n + x] = (e + P - PI) * u0 * st[(VIRAL_F_U0 + l) * n + x] + pn[l];
+}
+
 // ---------------------------------------------------------------- per-step diagnostics
 // The driver loop's reductions over an event's interior: CheckRoot (NEWEST10/
 // root.h:1-199; TXYZ5/multiroot.h:7-150), the DebugMSG extrema (debug.h:
@@ -2672,6 +2752,46 @@ int viral_root(viral_ctx* ctx, int ev, double tau)
   HIPCHK(hipStreamSynchronize(ctx->stream));
   E->bad_host = w;
   return w == ~0ull ? VIRAL_OK : VIRAL_ERR_PHYSICS;
+}
+
+int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, double* const* nspi)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E) return fail(ctx, VIRAL_ERR_ARG, "bad event");
+  if (!E->p.milne) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the Navier-Stokes initialisation is Milne only");
+  if (mode < VIRAL_NS_OUTPUT || mode > VIRAL_ZERO_INIT || !(ts > 0) || !(tau > 0))
+    return fail(ctx, VIRAL_ERR_ARG, "bad mode, tau or ts");
+  if (mode == VIRAL_NS_OUTPUT) {
+    if (!nspi) return fail(ctx, VIRAL_ERR_ARG, "VIRAL_NS_OUTPUT needs 10 output arrays");
+    for (int l = 0; l < 10; l++)
+      if (!nspi[l]) return fail(ctx, VIRAL_ERR_ARG, "VIRAL_NS_OUTPUT needs 10 output arrays");
+  }
+  const long n = E->g.n;
+  double* st = E->st[ctx->cur];
+  const unsigned grid = nblocks(n);
+  if (mode == VIRAL_NS_OUTPUT) {
+    const size_t m = 10 * (size_t)n;
+    if (m > ctx->out_cap) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      if (ctx->d_out) HIPCHK(hipFree(ctx->d_out));
+      ctx->d_out = nullptr;
+      ctx->out_cap = 0;
+      HIPCHK(hipMalloc(&ctx->d_out, m * sizeof(double)));
+      ctx->out_cap = m;
+    }
+    hipLaunchKernelGGL(k_ns_init<0>, dim3(grid), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, ts, st, ctx->d_out);
+    HIPCHK(hipGetLastError());
+    for (int l = 0; l < 10; l++)
+      HIPCHK(hipMemcpyAsync(nspi[l], ctx->d_out + (size_t)l * n, n * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+  } else if (mode == VIRAL_NS_INIT) {
+    hipLaunchKernelGGL(k_ns_init<1>, dim3(grid), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, ts, st, nullptr);
+  } else {
+    hipLaunchKernelGGL(k_ns_init<2>, dim3(grid), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, ts, st, nullptr);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return VIRAL_OK;
 }
 
 int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)
