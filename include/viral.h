@@ -207,6 +207,25 @@ int viral_root(viral_ctx* ctx, int ev, double tau);
 enum { VIRAL_NS_OUTPUT = 0, VIRAL_NS_INIT = 1, VIRAL_ZERO_INIT = 2 };
 int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, double* const* nspi);
 
+/* Landau matching of a T^{mu nu} field onto the event's current state
+ * (replaces FindVar and the per-cell part of ginit, NEWEST10/ginit.h:521-629
+ * and 876-990, which init() calls at tau - ts and at tau, init.h:658-659).
+ * tmunu[0..9]: host arrays of viral_cells() doubles, the contravariant
+ * components {tt, tx, ty, teta, xx, xy, xeta, yy, yeta, etaeta} at tau, for
+ * every ghosted cell.  Each cell takes the eigenvector of T^mu_nu that
+ * FindVar would select (real, with |v_i| < 1 and |v| < 1 for v = x^i/x^0;
+ * exactly one such) and sets e = |lambda|, Temp, P, v and u; at_start = 0
+ * (the call at tau - ts) also sets prevu = u and nothing else; at_start = 1
+ * sets pi = T - T_ideal - PI (g^{-1} - u u) with PI = P - e/3, then PI = 0
+ * and T^{tau mu} from e, P, u and pi.  Where no single eigenvector passes
+ * (the reference prints EIGEN VALUE SEARCH FAILED and exits) the cell keeps
+ * its state and gets VIRAL_FLAG_SOLVER in viral_flags, and the call returns
+ * VIRAL_ERR_PHYSICS with viral_first_bad_cell naming the first such cell in
+ * loop order.  Overwrites the flags of every cell.  Exact arithmetic and the
+ * closed-form EOS.  Milne events only.  Allocates 10 x cells doubles of
+ * staging on first use (shared with viral_output). */
+int viral_landau(viral_ctx* ctx, int ev, double tau, const double* const* tmunu, int at_start);
+
 /* Device reductions the reference's driver runs every step.  Each reads the
  * event's current state on the device (no state download).  Cells are
  * reported as ghosted (i, j, k) triples, -1 -1 -1 where there is none.

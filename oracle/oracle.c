@@ -593,6 +593,266 @@ int or_ns_init(or_ctx* c, double tau, double ts, int mode, double* nspi)
   return 0;
 }
 
+/* ---------------------------------------------------------------- Landau matching */
+
+/* the reference's velocity test on an eigenvector x (NEWEST10/ginit.h:
+ * 567-585): x^0 != 0, v = x^i/x^0 with |v_x|, |v_y|, |v_eta| < 1 and
+ * sqrt(v_x^2 + v_y^2 + v_eta^2) < 1 */
+static int landau_passes(const double x[4], double v[3])
+{
+  if (x[0] == 0.0) return 0;
+  v[0] = x[1] / x[0];
+  v[1] = x[2] / x[0];
+  v[2] = x[3] / x[0];
+  return fabs(v[0]) < 1 && fabs(v[1]) < 1 && fabs(v[2]) < 1 && sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) < 1;
+}
+
+/* 3x3 inverse by cofactors; 0 when singular or not finite */
+static int inv3(const double a[3][3], double r[3][3])
+{
+  const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1], c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2],
+               c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+  const double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+  if (det == 0.0 || !isfinite(det)) return 0;
+  const double id = 1.0 / det;
+  r[0][0] = c00 * id;
+  r[1][0] = c01 * id;
+  r[2][0] = c02 * id;
+  r[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+  r[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+  r[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+  r[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+  r[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+  r[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+  return 1;
+}
+
+/* Landau matching of one cell, the eigenvalue problem of FindVar
+ * (NEWEST10/ginit.h:521-629): M x = lambda x with M^a_b = T^{ab} g_bb,
+ * g = diag(1, -1, -1, -tau^2).  The reference takes all four eigenpairs from
+ * GSL's nonsymmetric solver and keeps those whose eigenvector is real and
+ * passes landau_passes; exactly one must, and then E = |lambda|.
+ *
+ * Here (own method): the timelike eigenvector by Newton's method on v with
+ * x = (1, v): r^i = (Mx)^i - (Mx)^0 v^i = 0, J^i_k = M^i_k - M^0_k v^i -
+ * (Mx)^0 delta^i_k, from v^i = T^{0i} / (4/3 T^{00}), steps halved while the
+ * new v is not timelike.  The other three eigenvectors lie in the
+ * g-orthogonal complement of u, which M maps to itself (M is g-self-adjoint):
+ * with the basis b_k = e_k - u_k u (k = x, y, eta), M b = b C, and C is
+ * symmetric after the change of basis by the Cholesky factor of -b^T g b, so
+ * cyclic Jacobi rotations give its eigenvectors.  They are spacelike, so
+ * they can pass the test only when tau > 1 (the test ignores the tau^2 of
+ * g_etaeta); for tau <= 1 they are not formed.  Returns the number of
+ * eigenvectors passing (1 on success), with E and v of the passing one. */
+static int landau_cell(double tau, const double T[10], double* E, double vout[3])
+{
+  const double g[4] = {1.0, -1.0, -1.0, -(tau * tau)};
+  double M[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) M[a][b] = T[PIDX[a][b]] * g[b];
+  double v[3];
+  for (int i = 0; i < 3; i++) v[i] = M[i + 1][0] / ((4.0 / 3.0) * M[0][0]);
+  if (!(v[0] * v[0] + v[1] * v[1] + (tau * tau) * (v[2] * v[2]) < 1.0)) v[0] = v[1] = v[2] = 0.0;
+  double e = 0.0;
+  int ok = 0;
+  for (int it = 0; it < 64; it++) {
+    double Mx[4];
+    for (int a = 0; a < 4; a++) Mx[a] = M[a][0] + M[a][1] * v[0] + M[a][2] * v[1] + M[a][3] * v[2];
+    e = Mx[0];
+    double r[3], J[3][3], Ji[3][3];
+    for (int i = 0; i < 3; i++) {
+      r[i] = Mx[i + 1] - e * v[i];
+      for (int k = 0; k < 3; k++) J[i][k] = M[i + 1][k + 1] - M[0][k + 1] * v[i] - (i == k ? e : 0.0);
+    }
+    if (!inv3(J, Ji)) break;
+    double dv[3];
+    for (int i = 0; i < 3; i++) dv[i] = -(Ji[i][0] * r[0] + Ji[i][1] * r[1] + Ji[i][2] * r[2]);
+    double t = 1.0, vn[3];
+    for (int h = 0; h < 60; h++) {
+      for (int i = 0; i < 3; i++) vn[i] = v[i] + t * dv[i];
+      if (vn[0] * vn[0] + vn[1] * vn[1] + (tau * tau) * (vn[2] * vn[2]) < 1.0) break;
+      t = 0.5 * t;
+    }
+    if (!(vn[0] * vn[0] + vn[1] * vn[1] + (tau * tau) * (vn[2] * vn[2]) < 1.0)) break;
+    const double step = fmax(fabs(vn[0] - v[0]), fmax(fabs(vn[1] - v[1]), fabs(vn[2] - v[2])));
+    v[0] = vn[0];
+    v[1] = vn[1];
+    v[2] = vn[2];
+    if (!isfinite(step)) break;
+    if (step <= 1e-15) {
+      ok = 1;
+      break;
+    }
+  }
+  if (!ok) return 0;
+  {
+    /* an eigenvector to working accuracy */
+    double Mx[4], sc = 0.0, res = 0.0;
+    for (int a = 0; a < 4; a++) {
+      Mx[a] = M[a][0] + M[a][1] * v[0] + M[a][2] * v[1] + M[a][3] * v[2];
+      for (int b = 0; b < 4; b++) sc = fmax(sc, fabs(M[a][b]));
+    }
+    e = Mx[0];
+    for (int i = 0; i < 3; i++) res = fmax(res, fabs(Mx[i + 1] - e * v[i]));
+    if (!(res <= 1e-11 * sc)) return 0;
+  }
+  int count = 0;
+  {
+    const double x[4] = {1.0, v[0], v[1], v[2]};
+    double vv[3];
+    if (landau_passes(x, vv)) {
+      count++;
+      *E = fabs(e);
+      vout[0] = vv[0];
+      vout[1] = vv[1];
+      vout[2] = vv[2];
+    }
+  }
+  if (tau > 1.0) {
+    const double u0 = 1.0 / sqrt(1.0 - v[0] * v[0] - v[1] * v[1] - (tau * tau) * (v[2] * v[2]));
+    const double u[4] = {u0, u0 * v[0], u0 * v[1], u0 * v[2]};
+    double Bs[4][3], MB[4][3], S[3][3], Si[3][3], Cm[3][3], H[3][3];
+    for (int a = 0; a < 4; a++)
+      for (int k = 0; k < 3; k++) Bs[a][k] = (a == k + 1 ? 1.0 : 0.0) - g[k + 1] * u[k + 1] * u[a];
+    for (int a = 0; a < 4; a++)
+      for (int k = 0; k < 3; k++) MB[a][k] = M[a][0] * Bs[0][k] + M[a][1] * Bs[1][k] + M[a][2] * Bs[2][k] + M[a][3] * Bs[3][k];
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) S[i][k] = Bs[i + 1][k];
+    if (!inv3(S, Si)) return 0;
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) Cm[i][k] = Si[i][0] * MB[1][k] + Si[i][1] * MB[2][k] + Si[i][2] * MB[3][k];
+    for (int k = 0; k < 3; k++)
+      for (int l = 0; l < 3; l++)
+        H[k][l] = -(g[0] * Bs[0][k] * Bs[0][l] + g[1] * Bs[1][k] * Bs[1][l] + g[2] * Bs[2][k] * Bs[2][l] +
+                    g[3] * Bs[3][k] * Bs[3][l]);
+    /* H = L L^T */
+    double L[3][3] = {{0}};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j <= i; j++) {
+        double s = H[i][j];
+        for (int q = 0; q < j; q++) s -= L[i][q] * L[j][q];
+        if (i == j) {
+          if (!(s > 0.0)) return 0;
+          L[i][i] = sqrt(s);
+        } else {
+          L[i][j] = s / L[j][j];
+        }
+      }
+    double Li[3][3], LiT[3][3], A[3][3], tmp[3][3], V[3][3];
+    if (!inv3(L, Li)) return 0;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) LiT[i][j] = Li[j][i];
+    /* A = L^T C L^{-T}, symmetrised against rounding */
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) tmp[i][j] = Cm[i][0] * LiT[0][j] + Cm[i][1] * LiT[1][j] + Cm[i][2] * LiT[2][j];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) A[i][j] = L[0][i] * tmp[0][j] + L[1][i] * tmp[1][j] + L[2][i] * tmp[2][j];
+    for (int i = 0; i < 3; i++)
+      for (int j = i + 1; j < 3; j++) A[i][j] = A[j][i] = 0.5 * (A[i][j] + A[j][i]);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) V[i][j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 12; sweep++) {
+      const double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+      const double dia = fabs(A[0][0]) + fabs(A[1][1]) + fabs(A[2][2]);
+      if (!(off > 1e-17 * dia)) break;
+      for (int pq = 0; pq < 3; pq++) {
+        const int P = pq == 2 ? 1 : 0, Q = pq == 0 ? 1 : 2;
+        if (A[P][Q] == 0.0) continue;
+        const double th = (A[Q][Q] - A[P][P]) / (2.0 * A[P][Q]);
+        const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double cc = 1.0 / sqrt(tt * tt + 1.0), ss = tt * cc;
+        for (int q = 0; q < 3; q++) {
+          const double aP = A[q][P], aQ = A[q][Q];
+          A[q][P] = cc * aP - ss * aQ;
+          A[q][Q] = ss * aP + cc * aQ;
+        }
+        for (int q = 0; q < 3; q++) {
+          const double aP = A[P][q], aQ = A[Q][q];
+          A[P][q] = cc * aP - ss * aQ;
+          A[Q][q] = ss * aP + cc * aQ;
+        }
+        for (int q = 0; q < 3; q++) {
+          const double vP = V[q][P], vQ = V[q][Q];
+          V[q][P] = cc * vP - ss * vQ;
+          V[q][Q] = ss * vP + cc * vQ;
+        }
+      }
+    }
+    for (int c = 0; c < 3; c++) {
+      double y[3], x[4], vv[3];
+      for (int k = 0; k < 3; k++) y[k] = LiT[k][0] * V[0][c] + LiT[k][1] * V[1][c] + LiT[k][2] * V[2][c];
+      for (int a = 0; a < 4; a++) x[a] = Bs[a][0] * y[0] + Bs[a][1] * y[1] + Bs[a][2] * y[2];
+      if (landau_passes(x, vv)) {
+        count++;
+        *E = fabs(A[c][c]);
+        vout[0] = vv[0];
+        vout[1] = vv[1];
+        vout[2] = vv[2];
+      }
+    }
+  }
+  return count;
+}
+
+/* The per-cell part of ginit (NEWEST10/ginit.h:876-990) on a T^{mu nu}
+ * field (tmunu: 10 x cells, pi index map order, contravariant at tau):
+ * Landau matching, then e, Temp = T(e), P = P(e), v and u; at_start 0 (the
+ * call at tau - ts) stores prevu = u; at_start 1 sets PI = P - e/3 (minus a
+ * third of the ideal part's trace), pi = T - T_ideal - PI (g^{-1} - u u),
+ * then PI = 0 (no BULK) and T^{tau mu} from e, P, u and pi.  Every ghosted
+ * cell; a cell without exactly one passing eigenvector ("EIGEN VALUE SEARCH
+ * FAILED", where the reference exits) is left alone and counted.  Returns the
+ * number of such cells and the first in loop order in *first (-1 if none). */
+long or_landau(or_ctx* c, double tau, const double* tmunu, int at_start, long long* first)
+{
+  const viral_desc* d = &c->d;
+  double** f = c->f;
+  const size_t n = c->n;
+  long nbad = 0;
+  *first = -1;
+  const double tau2 = tau * tau;
+  const double guu[4] = {1.0, -1.0, -1.0, -1.0 / tau2};
+  for (size_t x = 0; x < n; x++) {
+    double T[10], E = 0.0, v[3] = {0, 0, 0};
+    for (int l = 0; l < 10; l++) T[l] = tmunu[l * n + x];
+    if (landau_cell(tau, T, &E, v) != 1) {
+      if (nbad++ == 0) *first = (long long)x;
+      continue;
+    }
+    const double e = E, P = or_eos_p(d, E);
+    f[VIRAL_F_EN][x] = e;
+    f[VIRAL_F_TEMP][x] = or_eos_T(d, e);
+    f[VIRAL_F_P][x] = P;
+    f[VIRAL_F_VX][x] = v[0];
+    f[VIRAL_F_VY][x] = v[1];
+    f[VIRAL_F_VE][x] = v[2];
+    const double u0 = 1.0 / sqrt(1.0 - v[0] * v[0] - v[1] * v[1] - tau2 * v[2] * v[2]);
+    const double u[4] = {u0, u0 * v[0], u0 * v[1], u0 * v[2]};
+    for (int l = 0; l < 4; l++) f[VIRAL_F_U0 + l][x] = u[l];
+    if (!at_start) {
+      for (int l = 0; l < 4; l++) f[VIRAL_F_PU0 + l][x] = u[l];
+      continue;
+    }
+    double tid[4][4];
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) tid[a][b] = (e + P) * u[a] * u[b] - P * (a == b ? guu[a] : 0.0);
+    const double temp2 = tid[0][0] - tid[1][1] - tid[2][2] - tau * tau * tid[3][3];
+    const double PIt = -temp2 / 3.0;
+    double pn[10];
+    for (int a = 0; a < 4; a++)
+      for (int b = a; b < 4; b++) {
+        const double PImat = PIt * ((a == b ? guu[a] : 0.0) - u[a] * u[b]);
+        pn[PIDX[a][b]] = T[PIDX[a][b]] - tid[a][b] - PImat;
+      }
+    for (int l = 0; l < 10; l++) f[VIRAL_F_PI0 + l][x] = pn[l];
+    f[VIRAL_F_BULK][x] = 0.0;
+    const double PI = 0.0;
+    f[VIRAL_F_T00][x] = -P + PI + (e + P - PI) * (u0 * u0) + pn[0];
+    for (int l = 1; l < 4; l++) f[VIRAL_F_T00 + l][x] = (e + P - PI) * u0 * u[l] + pn[l];
+  }
+  return nbad;
+}
+
 /* geometric + shear sources on the interior, NEWEST10/Source.h:195-536
  * (Cartesian: TXYZ5/Source.h:136-157 zero geometric terms, 270-330 shear for
  * the five evolved components; under q9 only ghosted k < nz) */

@@ -103,6 +103,11 @@ size_t or_output(or_ctx* c, int freq, int freqz, int xy, double* out);
  * Cartesian context or bad mode. */
 int or_ns_init(or_ctx* c, double tau, double ts, int mode, double* nspi);
 
+/* Landau matching of a T^{mu nu} field (10 x cells) and the per-cell part of
+ * ginit (NEWEST10/ginit.h:521-629, 876-990); returns the number of cells
+ * without exactly one passing eigenvector, the first of them in *first. */
+long or_landau(or_ctx* c, double tau, const double* tmunu, int at_start, long long* first);
+
 /* deterministic math (oracle/vmath.h) exported for tests */
 double or_vm_exp(double x);
 double or_vm_log(double x);

@@ -79,6 +79,8 @@ def lib():
         L.or_output.restype = C.c_size_t
         L.or_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.or_ns_init.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp]
+        L.or_landau.restype = C.c_long
+        L.or_landau.argtypes = [vp, C.c_double, dp, C.c_int, lp]
         _lib = L
     return _lib
 
@@ -241,3 +243,11 @@ class Oracle:
     def ns_init(self, tau, ts, zero=False):
         """NSInit (or ZeroInit) in place"""
         assert self.L.or_ns_init(self.h, tau, ts, 2 if zero else 1, None) == 0
+
+    def landau(self, tau, tmunu, at_start=True):
+        """Landau matching of a (10, cells) T^{mu nu} field in place; returns
+        (number of failing cells, first failing cell or -1)"""
+        t = np.ascontiguousarray(tmunu, dtype=np.float64)
+        first = np.zeros(1, dtype=np.int64)
+        nbad = self.L.or_landau(self.h, tau, _dp(t), 1 if at_start else 0, first.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return int(nbad), int(first[0])

@@ -24,7 +24,7 @@ EXPORTS = [
     "viral_set_launch", "viral_tile_count", "viral_tile_table", "viral_xcd_tile", "viral_adjoint_begin",
     "viral_adjoint", "viral_adjoint_end", "viral_check_root", "viral_debug_extrema", "viral_check_physics",
     "viral_output_size", "viral_output", "viral_sync", "viral_host_alloc", "viral_host_free", "viral_root",
-    "viral_ns_init",
+    "viral_ns_init", "viral_landau",
 ]
 
 # modes of viral_ns_init
@@ -136,6 +136,8 @@ def load():
     L.viral_root.argtypes = [vp, C.c_int, C.c_double]
     L.viral_ns_init.restype = C.c_int
     L.viral_ns_init.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.POINTER(C.c_double))]
+    L.viral_landau.restype = C.c_int
+    L.viral_landau.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.POINTER(C.c_double)), C.c_int]
     L.viral_sync.restype = C.c_int
     L.viral_sync.argtypes = [vp]
     L.viral_host_alloc.restype = C.c_int
@@ -354,6 +356,17 @@ class Hydro:
     def ns_init(self, tau, ts, zero=False, ev=0):
         """NSInit (or ZeroInit) on the current state in place"""
         self._chk(self.L.viral_ns_init(self.h, ev, tau, ts, ZERO_INIT if zero else NS_INIT, None))
+
+    def landau(self, tau, tmunu, at_start=True, ev=0):
+        """Landau matching of a (10, cells) T^{mu nu} field onto the state
+        (status: OK, or ERR_PHYSICS with first_bad_cell naming a failed cell)"""
+        t = np.ascontiguousarray(tmunu, dtype=np.float64)
+        assert t.shape == (10, self.descs[ev].cells())
+        ptrs = (C.POINTER(C.c_double) * 10)(*[t[l].ctypes.data_as(C.POINTER(C.c_double)) for l in range(10)])
+        r = self.L.viral_landau(self.h, ev, tau, ptrs, 1 if at_start else 0)
+        if r not in (OK, ERR_PHYSICS):
+            self._chk(r)
+        return r
 
     def _cells(self, ijk, ev):
         """ghosted linear indices of (i, j, k) triples, -1 where none"""

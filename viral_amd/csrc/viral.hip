@@ -1277,6 +1277,66 @@ __global__ void __launch_bounds__(BLK) k_ns_init(Geom g, Params p, double tau, d
   for (int l = 1; l < 4; l++) st[(VIRAL_F_T00 + l) * n + x] = (e + P - PI) * u0 * st[(VIRAL_F_U0 + l) * n + x] + pn[l];
 }
 
+// ---------------------------------------------------------------- Landau matching
+// The per-cell part of ginit (NEWEST10/ginit.h:876-990) on a T^{mu nu} field
+// (tmunu[10][n], pi index map order, contravariant at tau), one thread per
+// ghosted cell: the Landau frame from landau_cell (FindVar's selection,
+// ginit.h:521-629), then e, Temp, P, v and u; AT_START false (the call at
+// tau - ts) stores prevu = u; AT_START true sets pi = T - T_ideal - PI
+// (g^{-1} - u u) with PI = P - e/3, then PI = 0 (no BULK) and T^{tau mu}.  A
+// cell without exactly one passing eigenvector ("EIGEN VALUE SEARCH FAILED",
+// where the reference exits) keeps its state, gets VIRAL_FLAG_SOLVER and
+// competes for the first bad cell.  Exact arithmetic and the closed-form EOS.
+template <bool AT_START>
+__global__ void __launch_bounds__(BLK) k_landau(Geom g, Params p, double tau, const double* __restrict__ tmunu,
+                                                double* __restrict__ st, uint8_t* __restrict__ flags,
+                                                unsigned long long* __restrict__ bad)
+{
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= g.n) return;
+  const long n = g.n;
+  double T[10], E = 0.0, v[3] = {0.0, 0.0, 0.0};
+  for (int l = 0; l < 10; l++) T[l] = tmunu[l * n + x];
+  if (landau_cell(tau, T, &E, v) != 1) {
+    flags[x] = (uint8_t)VIRAL_FLAG_SOLVER;
+    atomicMin(bad, ((unsigned long long)x << 8) | (unsigned long long)VIRAL_FLAG_SOLVER);
+    return;
+  }
+  flags[x] = 0;
+  const double tau2 = tau * tau;
+  const double guu[4] = {1.0, -1.0, -1.0, -1.0 / tau2};
+  const double e = E, P = eos_p(p, E);
+  st[VIRAL_F_EN * n + x] = e;
+  st[VIRAL_F_TEMP * n + x] = eos_T(p, e);
+  st[VIRAL_F_P * n + x] = P;
+  st[VIRAL_F_VX * n + x] = v[0];
+  st[VIRAL_F_VY * n + x] = v[1];
+  st[VIRAL_F_VE * n + x] = v[2];
+  const double u0 = 1.0 / sqrt(1.0 - v[0] * v[0] - v[1] * v[1] - tau2 * v[2] * v[2]);
+  const double u[4] = {u0, u0 * v[0], u0 * v[1], u0 * v[2]};
+  for (int l = 0; l < 4; l++) st[(VIRAL_F_U0 + l) * n + x] = u[l];
+  if (!AT_START) {
+    for (int l = 0; l < 4; l++) st[(VIRAL_F_PU0 + l) * n + x] = u[l];
+    return;
+  }
+  double tid[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) tid[a][b] = (e + P) * u[a] * u[b] - P * (a == b ? guu[a] : 0.0);
+  const double temp2 = tid[0][0] - tid[1][1] - tid[2][2] - tau * tau * tid[3][3];
+  const double PIt = -temp2 / 3.0;
+  double pn[10];
+  for (int a = 0; a < 4; a++)
+    for (int b = a; b < 4; b++) {
+      const double PImat = PIt * ((a == b ? guu[a] : 0.0) - u[a] * u[b]);
+      pn[pidx(a, b)] = T[pidx(a, b)] - tid[a][b] - PImat;
+    }
+  for (int l = 0; l < 10; l++) st[(VIRAL_F_PI0 + l) * n + x] = pn[l];
+  st[VIRAL_F_BULK * n + x] = 0.0;
+  const double PI = 0.0;
+  st[VIRAL_F_T00 * n + x] = -P + PI + (e + P - PI) * (u0 * u0) + pn[0];
+  for (int l = 1; l < 4; l++) st[(VIRAL_F_T00 + l) * n + x] = (e + P - PI) * u0 * u[l] + pn[l];
+}
+
 // ---------------------------------------------------------------- per-step diagnostics
 // The driver loop's reductions over an event's interior: CheckRoot (NEWEST10/
 // root.h:1-199; TXYZ5/multiroot.h:7-150), the DebugMSG extrema (debug.h:
@@ -2792,6 +2852,43 @@ int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, doubl
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return VIRAL_OK;
+}
+
+int viral_landau(viral_ctx* ctx, int ev, double tau, const double* const* tmunu, int at_start)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E) return fail(ctx, VIRAL_ERR_ARG, "bad event");
+  if (!E->p.milne) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "Landau matching is Milne only (ginit)");
+  if (!tmunu || !(tau > 0)) return fail(ctx, VIRAL_ERR_ARG, "bad T^{mu nu} arrays or tau");
+  for (int l = 0; l < 10; l++)
+    if (!tmunu[l]) return fail(ctx, VIRAL_ERR_ARG, "bad T^{mu nu} arrays or tau");
+  const long n = E->g.n;
+  const size_t m = 10 * (size_t)n;
+  if (m > ctx->out_cap) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_out) HIPCHK(hipFree(ctx->d_out));
+    ctx->d_out = nullptr;
+    ctx->out_cap = 0;
+    HIPCHK(hipMalloc(&ctx->d_out, m * sizeof(double)));
+    ctx->out_cap = m;
+  }
+  for (int l = 0; l < 10; l++)
+    HIPCHK(hipMemcpyAsync(ctx->d_out + (size_t)l * n, tmunu[l], n * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+  hipLaunchKernelGGL(k_clear_bad, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bad + ev, 1);
+  double* st = E->st[ctx->cur];
+  if (at_start)
+    hipLaunchKernelGGL(k_landau<true>, dim3(nblocks(n)), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, ctx->d_out, st,
+                       E->flags, ctx->d_bad + ev);
+  else
+    hipLaunchKernelGGL(k_landau<false>, dim3(nblocks(n)), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, ctx->d_out,
+                       st, E->flags, ctx->d_bad + ev);
+  HIPCHK(hipGetLastError());
+  unsigned long long w = ~0ull;
+  HIPCHK(hipMemcpyAsync(&w, ctx->d_bad + ev, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  E->bad_host = w;
+  return w == ~0ull ? VIRAL_OK : VIRAL_ERR_PHYSICS;
 }
 
 int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)
