@@ -70,6 +70,10 @@ double ah_weno(const double* x, int right)
   return right ? weno_R(x[0], x[1], x[2], x[3], x[4]) : weno_L(x[0], x[1], x[2], x[3], x[4]);
 }
 void ah_weno_grad(const double* x, int right, double* g) { weno_grad(x, right != 0, g); }
+void ah_weno_lr_vjp(const double* x, double sL, double sR, double* LR, double* g)
+{
+  weno_lr_vjp(x, sL, sR, LR[0], LR[1], g);
+}
 
 void ah_shear(int milne, double tau, const double* u, const double* pi10, const double* du16, double SH, double tpi,
               double* out)

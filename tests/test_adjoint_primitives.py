@@ -23,6 +23,7 @@ def lib():
     L.ah_weno.restype = C.c_double
     L.ah_weno.argtypes = [dp, C.c_int]
     L.ah_weno_grad.argtypes = [dp, C.c_int, dp]
+    L.ah_weno_lr_vjp.argtypes = [dp, C.c_double, C.c_double, dp, dp]
     L.ah_shear.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_double, C.c_double, dp]
     L.ah_shear_vjp.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_double, C.c_double, dp, dp, dp, dp, dp, dp]
     L.ah_flux_cell.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
@@ -82,6 +83,25 @@ def test_weno_gradient(right):
     g = np.zeros(5)
     L.ah_weno_grad(P(x), right, P(g))
     assert abs(g.sum() - 1.0) < 1e-12
+
+
+def test_weno_pair_vjp_matches_single_gradients():
+    """weno_lr_vjp (shared indicators) = sL dL/dx + sR dR/dx from two
+    weno_grad calls, and its values are the reconstructions"""
+    L = lib()
+    rng = np.random.default_rng(4)
+    for scale in (1e-3, 1.0, 50.0):
+        for _ in range(30):
+            x = arr(scale * (1.0 + rng.standard_normal(5)))
+            sL, sR = rng.standard_normal(2)
+            gl, gr, g, LR = np.zeros(5), np.zeros(5), np.zeros(5), np.zeros(2)
+            L.ah_weno_grad(P(x), 0, P(gl))
+            L.ah_weno_grad(P(x), 1, P(gr))
+            L.ah_weno_lr_vjp(P(x), sL, sR, P(LR), P(g))
+            want = sL * gl + sR * gr
+            assert np.abs(g - want).max() <= 1e-12 * (1 + np.abs(want).max()), (g, want)
+            assert abs(LR[0] - L.ah_weno(P(x), 0)) <= 1e-13 * (1 + abs(LR[0]))
+            assert abs(LR[1] - L.ah_weno(P(x), 1)) <= 1e-13 * (1 + abs(LR[1]))
 
 
 def rand_u(rng, tau, milne=1):

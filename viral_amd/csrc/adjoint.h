@@ -77,6 +77,69 @@ VHD double weno_grad(const double x[5], bool right, double g[5])
   return out;
 }
 
+// Both WENO5-JS reconstructions of one cell from its stencil x (L, the value
+// at the cell's right face, and R, at its left face) sharing the smoothness
+// indicators, and the vector-Jacobian product of the pair:
+//   g_i = sL dL/dx_i + sR dR/dx_i
+//       = sum_c (sL wL_c dqL_c/dx_i + sR wR_c dqR_c/dx_i) + sum_c k_c db_c/dx_i,
+//   k_c = -2 / (eps + b_c) (sL (qL_c - L) aL_c / SL + sR (qR_c - R) aR_c / SR),
+// with the notation of weno_grad.  One set of b_c, 1/(eps + b_c) and db/dx
+// serves both sides.
+VHD void weno_lr_vjp(const double x[5], double sL, double sR, double& L, double& R, double g[5])
+{
+  const double Ac[3][5] = {{0, 0, 1, -2, 1}, {0, 1, -2, 1, 0}, {1, -2, 1, 0, 0}};
+  const double Cc[3][5] = {{0, 0, 3, -4, 1}, {0, 1, 0, -1, 0}, {1, -4, 3, 0, 0}};
+  const double QL[3][5] = {{0, 0, 2, 5, -1}, {0, -1, 5, 2, 0}, {2, -7, 11, 0, 0}};
+  const double QR[3][5] = {{0, 0, 11, -7, 2}, {0, 2, 5, -1, 0}, {-1, 5, 2, 0, 0}};
+  const double dL[3] = {0.3, 0.6, 0.1}, dR[3] = {0.1, 0.6, 0.3};
+  double A[3], C[3], it[3], aL[3], aR[3], qL[3], qR[3], SL = 0, SR = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    A[c] = C[c] = qL[c] = qR[c] = 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      A[c] += Ac[c][i] * x[i];
+      C[c] += Cc[c][i] * x[i];
+      qL[c] += QL[c][i] * x[i];
+      qR[c] += QR[c][i] * x[i];
+    }
+    qL[c] = qL[c] * (1.0 / 6.0);
+    qR[c] = qR[c] * (1.0 / 6.0);
+    it[c] = frcp(WENO_EPS + (13. / 12.) * (A[c] * A[c]) + 0.25 * (C[c] * C[c]));
+    const double i2 = it[c] * it[c];
+    aL[c] = dL[c] * i2;
+    aR[c] = dR[c] * i2;
+    SL += aL[c];
+    SR += aR[c];
+  }
+  const double rL = frcp(SL), rR = frcp(SR);
+  double wL[3], wR[3];
+  L = 0;
+  R = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    wL[c] = aL[c] * rL;
+    wR[c] = aR[c] * rR;
+    L += wL[c] * qL[c];
+    R += wR[c] * qR[c];
+  }
+  double k[3], hL[3], hR[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    k[c] = -2.0 * it[c] * (sL * (qL[c] - L) * wL[c] + sR * (qR[c] - R) * wR[c]);
+    hL[c] = sL * wL[c] * (1.0 / 6.0);
+    hR[c] = sR * wR[c] * (1.0 / 6.0);
+  }
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    double gi = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      gi += hL[c] * QL[c][i] + hR[c] * QR[c][i] + k[c] * ((13. / 6.) * A[c] * Ac[c][i] + 0.5 * C[c] * Cc[c][i]);
+    g[i] = gi;
+  }
+}
+
 // P(e), T(e), s(e) and their e-derivatives as the forward step evaluates
 // them: the table's interpolant where fast mode reads the table (use_tab is
 // set only in fast mode), the closed forms elsewhere.  The conformal s =

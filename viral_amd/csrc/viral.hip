@@ -1807,32 +1807,38 @@ __global__ void __launch_bounds__(BLK) k_adj_du(Geom g, Params p, const double* 
   for (int l = 0; l < 4; l++) li[(VIRAL_F_U0 + l) * n + x] += acc[l];
 }
 
-// KT flux of direction D in reverse on an LDS line tile: the face pass (each
-// face's WENO gradients turned into contributions to its six stencil cells)
-// and the cell pass (each cell's sum of them, mapped onto its state fields)
-// fused, with no global intermediate.  A block takes TKA lines along D and a window of WA
-// consecutive cells of each.  Per row, every window cell computes its centred
-// flux and Var once into LDS; every face of the window (WA - 5 of them)
-// writes its six stencil-cell contributions to LDS; every own cell (the
-// WA - 10 cells whose six faces all lie in the window) sums them in a fixed
-// face order.  Own cells cover the whole line, ghosts included, and tiles own
-// disjoint cells, so each cell's state adjoint is updated by one thread.
-// Slots map to cells with the contiguous (eta) index fastest.
+// KT flux of direction D in reverse on an LDS line tile, with no global
+// intermediate.  A block takes TKA lines along D and a window of WA
+// consecutive cells of each (slot wi holds cell c0 - 5 + wi).  Per row, every
+// window cell stages its centred flux and Var in LDS; every cell with a full
+// stencil in the window (slots 2..WA-3) forms both WENO reconstructions of F
+// and of Var (L at its right face, R at its left face) from one set of
+// smoothness indicators and turns the adjoints of its two faces' Rusanov
+// fluxes into contributions to its five stencil cells (weno_lr_vjp); every
+// own cell (slots 5..WA-6: its five contributors, and the eigenvalue
+// reconstructions their face speeds need, lie in the window) sums them in a
+// fixed order.  The face speeds a = max(|A_L|, |A_R|) come from the same
+// per-cell reconstructions of the eigenvalue, and their adjoint (collected
+// per face over the rows) goes back through it at the end.  Own cells cover
+// the whole line, ghosts included, and tiles own disjoint cells, so each
+// cell's state adjoint is updated by one thread.  Slots map to cells with the
+// contiguous (eta) index fastest.
 constexpr int WA = 64, TKA = 4, NTA = WA * TKA;
-constexpr int OWNA = WA - 10, NFA = WA - 5;
+constexpr int OWNA = WA - 10;
 
 template <int D>
-__global__ void __launch_bounds__(NTA) k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si,
+__global__ void __launch_bounds__(NTA) __attribute__((amdgpu_waves_per_eu(3))) k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si,
                                                   const double* __restrict__ resb, double* __restrict__ li)
 {
-  __shared__ double sA[TKA][WA];
   __shared__ double sF[TKA][WA];
   __shared__ double sV[TKA][WA];
-  __shared__ double cF[TKA][NFA][6];
-  __shared__ double cV[TKA][NFA][6];
+  __shared__ double xL[TKA][WA];  // per cell: A_L, then the face-adjoint parts
+  __shared__ double xR[TKA][WA];  // per cell: A_R
+  __shared__ double cF[TKA][WA][5];
+  __shared__ double cV[TKA][WA][5];
   const int t = threadIdx.x;
   const int kk = D == 2 ? t / WA : t % TKA;  // line of the tile
-  const int wi = D == 2 ? t % WA : t / TKA;  // window slot; face slot wi; own cell slot wi
+  const int wi = D == 2 ? t % WA : t / TKA;  // window slot
   const int GD = D == 0 ? g.gx : D == 1 ? g.gy : g.gz;
   const int lo = D == 0 ? g.il : D == 1 ? g.jl : g.kl;
   const int hi = D == 0 ? g.ir : D == 1 ? g.jr : g.kr;
@@ -1850,15 +1856,15 @@ __global__ void __launch_bounds__(NTA) k_adj_tile(Geom g, Params p, double tau, 
     if (D == 1) return cidx(g, g.il + la, cd, g.kl + lb);
     return cidx(g, g.il + la, g.jl + lb, cd);
   };
-  const long n = g.n, sd = dstride(g, D);
+  const long n = g.n;
   const bool q1 = (p.quirks & VIRAL_Q_GHOST_EIGEN) != 0;
   const int row = D + 1;
   const double h = D == 0 ? p.dx : D == 1 ? p.dy : p.deta, ih = 1.0 / h;
 
   // window cell of this thread and its common terms
-  const int cw = c0 - 5 + wi;
-  const bool w_ok = line_ok && cw >= 0 && cw < GD;
-  const long xw = w_ok ? at(cw) : 0;
+  const int c = c0 - 5 + wi;
+  const bool w_ok = line_ok && c >= 0 && c < GD;
+  const long xw = w_ok ? at(c) : 0;
   double u0 = 1.0, ud = 0.0, ww = 0.0, pp = 0.0;
   if (w_ok) {
     const double e = si[VIRAL_F_EN * n + xw], P = si[VIRAL_F_P * n + xw], PI = si[VIRAL_F_BULK * n + xw];
@@ -1867,123 +1873,160 @@ __global__ void __launch_bounds__(NTA) k_adj_tile(Geom g, Params p, double tau, 
     ww = e + P - PI;
     pp = D == 2 ? (P - PI) / (tau * tau) : (P - PI);
   }
-  sA[kk][wi] = w_ok && ((cw >= lo && cw < hi) || !q1) ? ud / u0 : 0.0;
-  // face f + 1/2 of this thread: stencil cells f-2..f+3 = window slots wi..wi+5
-  const int f = c0 - 3 + wi;
-  const bool f_ok = line_ok && wi < NFA && f >= lo - 1 && f <= hi - 1;
-  const long xf = f_ok ? at(f) : 0;
-  const bool inc = f >= lo && f < hi, inc1 = f + 1 >= lo && f + 1 < hi;
+  const bool rc = wi >= 2 && wi < WA - 2;  // full stencil in the window
+  // faces of this cell: right c+1/2 (face index c), left c-1/2 (face index c-1)
+  const bool fR = line_ok && rc && c >= lo - 1 && c <= hi - 1;
+  const bool fL = line_ok && rc && c - 1 >= lo - 1 && c - 1 <= hi - 1;
+  // the eigenvalue's reconstructions: A_L(c), A_R(c)
+  sF[kk][wi] = w_ok && ((c >= lo && c < hi) || !q1) ? ud / u0 : 0.0;
   __syncthreads();
-  double A[6], aL = 0, aR = 0, ml = 0, mr = 0, amax = 0;
-  if (f_ok) {
-    for (int o = 0; o < 6; o++) A[o] = sA[kk][wi + o];
-    aL = weno_L(A[0], A[1], A[2], A[3], A[4]);
-    aR = weno_R(A[1], A[2], A[3], A[4], A[5]);
-    if (!(q1 && f < lo)) {
-      ml = fabs(aL);
-      mr = fabs(aR);
-    }
-    amax = fmax(ml, mr);
+  double As[5], AL = 0.0, AR = 0.0;
+  if (rc) {
+    for (int o = 0; o < 5; o++) As[o] = sF[kk][wi - 2 + o];
+    AL = weno_L(As[0], As[1], As[2], As[3], As[4]);
+    AR = weno_R(As[0], As[1], As[2], As[3], As[4]);
+  }
+  xL[kk][wi] = AL;
+  xR[kk][wi] = AR;
+  __syncthreads();
+  // face speeds and branches: face c+1/2 takes |A_L(c)| and |A_R(c+1)|, zero
+  // under q1 before the first interior cell
+  double aFR = 0.0, aFL = 0.0;
+  bool brR = true, brL = true;  // true: the A_L side is the maximum
+  if (fR && !(q1 && c < lo)) {
+    const double ml = fabs(AL), mr = fabs(xR[kk][wi + 1]);
+    aFR = fmax(ml, mr);
+    brR = ml >= mr;
+  }
+  if (fL && !(q1 && c - 1 < lo)) {
+    const double ml = fabs(xL[kk][wi - 1]), mr = fabs(AR);
+    aFL = fmax(ml, mr);
+    brL = ml >= mr;
   }
   // own cell of this thread
-  const int c = c0 + wi;
-  const bool o_ok = line_ok && wi < OWNA && c < GD;
-  double Fb[SV], Vb[SV];
-  for (int q = 0; q < SV; q++) Fb[q] = Vb[q] = 0.0;
-  double abar = 0.0;
+  const bool o_ok = line_ok && wi >= 5 && wi < WA - 5 && c < GD;
+  // The own cell's centred flux and Var in reverse (flux_cell_vjp, one row at
+  // a time): each row's field adjoints go straight to li (only this thread
+  // touches the cell); the adjoints of w = e + P - PI, the pressure term, u^d
+  // and u^mu accumulate here.
+  double wb = 0.0, ppb = 0.0, udb = 0.0, ub0 = 0.0, ub1 = 0.0, ub2 = 0.0, ub3 = 0.0;
+  const double iu0 = frcp(u0);
+  double abR = 0.0, abL = 0.0;  // this cell's parts of the speed adjoints of its two faces
+  const long xr = w_ok ? xw : 0;
   for (int l = 0; l < SV; l++) {
     // centred flux and Var of row l (cell_FVA)
-    double Fw = 0.0, Vw = 0.0;
+    double Fw = 0.0, Vw = 0.0, um = 0.0, fv = 0.0;
     if (w_ok) {
       if (l < 4) {
-        const double um = si[(VIRAL_F_U0 + l) * n + xw];
+        um = si[(VIRAL_F_U0 + l) * n + xw];
         const double piv = si[(VIRAL_F_PI0 + pidx(row, l)) * n + xw];
         const double v = l == row ? piv + pp + ud * um * ww : piv + ud * um * ww;
         Fw = tau * v;
         Vw = tau * si[(long)l * n + xw];
       } else {
-        const double fv = si[(long)l * n + xw];
+        fv = si[(long)l * n + xw];
         Fw = (fv * ud) / u0;
         Vw = fv;
       }
     }
+    // (the previous row's sF, sV and cF reads all precede the barrier below)
     sF[kk][wi] = Fw;
     sV[kk][wi] = Vw;
     __syncthreads();
-    if (wi < NFA) {
-      double cf[6] = {0, 0, 0, 0, 0, 0}, cv[6] = {0, 0, 0, 0, 0, 0};
-      const double fb = f_ok ? (inc ? -resb[l * n + xf] * ih : 0.0) + (inc1 ? resb[l * n + xf + sd] * ih : 0.0) : 0.0;
-      if (fb != 0.0) {
-        double Vs[6], Fs[6], gL[5], gR[5];
-        for (int o = 0; o < 6; o++) {
-          Vs[o] = sV[kk][wi + o];
-          Fs[o] = sF[kk][wi + o];
-        }
-        const double VL = weno_grad(Vs, false, gL);
-        const double VR = weno_grad(Vs + 1, true, gR);
-        abar += -0.5 * fb * (VR - VL);
-        const double wv = 0.5 * amax * fb;
-        for (int o = 0; o < 6; o++) cv[o] = wv * ((o < 5 ? gL[o] : 0.0) - (o > 0 ? gR[o - 1] : 0.0));
-        weno_grad(Fs, false, gL);
-        weno_grad(Fs + 1, true, gR);
-        const double wf = 0.5 * fb;
-        for (int o = 0; o < 6; o++) cf[o] = wf * ((o < 5 ? gL[o] : 0.0) + (o > 0 ? gR[o - 1] : 0.0));
+    {
+      double gF[5] = {0, 0, 0, 0, 0}, gV[5] = {0, 0, 0, 0, 0};
+      // adjoints of the face fluxes: partial result of cell x is
+      // -(f(x+1/2) - f(x-1/2))/h, interior cells only
+      double fbR = 0.0, fbL = 0.0;
+      if (fR) {
+        if (c >= lo && c < hi) fbR -= resb[l * n + xr] * ih;
+        if (c + 1 >= lo && c + 1 < hi) fbR += resb[l * n + at(c + 1)] * ih;
       }
-      for (int o = 0; o < 6; o++) {
-        cF[kk][wi][o] = cf[o];
-        cV[kk][wi][o] = cv[o];
+      if (fL) {
+        if (c - 1 >= lo && c - 1 < hi) fbL -= resb[l * n + at(c - 1)] * ih;
+        if (c >= lo && c < hi) fbL += resb[l * n + xr] * ih;
+      }
+      if (fbR != 0.0 || fbL != 0.0) {
+        double Vs[5], Fs[5], VL, VR, FL, FR;
+        for (int o = 0; o < 5; o++) {
+          Vs[o] = sV[kk][wi - 2 + o];
+          Fs[o] = sF[kk][wi - 2 + o];
+        }
+        // f = (F_L + F_R)/2 - a (V_R - V_L)/2
+        weno_lr_vjp(Vs, 0.5 * aFR * fbR, -0.5 * aFL * fbL, VL, VR, gV);
+        weno_lr_vjp(Fs, 0.5 * fbR, 0.5 * fbL, FL, FR, gF);
+        abR += 0.5 * fbR * VL;
+        abL += -0.5 * fbL * VR;
+      }
+      for (int o = 0; o < 5; o++) {
+        cF[kk][wi][o] = gF[o];
+        cV[kk][wi][o] = gV[o];
       }
     }
     __syncthreads();
     if (o_ok) {
       double sf = 0.0, sv = 0.0;
-      for (int fi = wi; fi <= wi + 5; fi++) {  // faces c-3 .. c+2, the cell at offset wi + 5 - fi
-        sf += cF[kk][fi][wi + 5 - fi];
-        sv += cV[kk][fi][wi + 5 - fi];
+      for (int o = 0; o < 5; o++) {  // stencil cells c+2 .. c-2 of this cell's contributors
+        sf += cF[kk][wi + 2 - o][o];
+        sv += cV[kk][wi + 2 - o][o];
       }
-#pragma unroll
-      for (int q = 0; q < SV; q++)
-        if (q == l) {
-          Fb[q] = sf;
-          Vb[q] = sv;
-        }
+      if (l < 4) {
+        // F_l = tau (pi^{d+1,l} + [l = d+1] pp + u^{d+1} u^l w), V_l = tau T^{tau l}
+        const double vb = tau * sf;
+        li[(VIRAL_F_PI0 + pidx(row, l)) * n + xw] += vb;
+        li[(VIRAL_F_T00 + l) * n + xw] += tau * sv;
+        if (l == row) ppb += vb;
+        udb += vb * um * ww;
+        const double ubl = vb * ud * ww;
+        if (l == 0) ub0 += ubl;
+        else if (l == 1) ub1 += ubl;
+        else if (l == 2) ub2 += ubl;
+        else ub3 += ubl;
+        wb += vb * ud * um;
+      } else {
+        // F_l = f u^{d+1}/u^0, V_l = f for the state field f = pi[l-4] or PI
+        li[(long)l * n + xw] += sf * ud * iu0 + sv;
+        udb += sf * fv * iu0;
+        ub0 -= sf * fv * ud * iu0 * iu0;
+      }
     }
   }
-  // the face speed: a = max(|A_L|, |A_R|), adjoint on the branch taken
+  // the face speeds in reverse: a = max(|A_L|, |A_R|) on the branch taken;
+  // abar(c+1/2) = abR(c) + abL(c+1)
   __syncthreads();
-  if (wi < NFA) {
-    double cA[6] = {0, 0, 0, 0, 0, 0};
-    if (f_ok && !(q1 && f < lo)) {
-      double AbL = 0, AbR = 0;
-      if (ml >= mr) AbL = abar * (aL > 0 ? 1.0 : aL < 0 ? -1.0 : 0.0);
-      else AbR = abar * (aR > 0 ? 1.0 : aR < 0 ? -1.0 : 0.0);
-      double gL[5], gR[5];
-      weno_grad(A, false, gL);
-      weno_grad(A + 1, true, gR);
-      for (int o = 0; o < 6; o++) cA[o] = (o < 5 ? AbL * gL[o] : 0.0) + (o > 0 ? AbR * gR[o - 1] : 0.0);
+  xL[kk][wi] = abL;
+  xR[kk][wi] = abR;
+  __syncthreads();
+  {
+    double gA[5] = {0, 0, 0, 0, 0};
+    double sAL = 0.0, sAR = 0.0;
+    if (fR && !(q1 && c < lo) && brR) sAL = (abR + xL[kk][wi + 1]) * (AL > 0 ? 1.0 : AL < 0 ? -1.0 : 0.0);
+    if (fL && !(q1 && c - 1 < lo) && !brL) sAR = (xR[kk][wi - 1] + abL) * (AR > 0 ? 1.0 : AR < 0 ? -1.0 : 0.0);
+    if (sAL != 0.0 || sAR != 0.0) {
+      double dl, dr;
+      weno_lr_vjp(As, sAL, sAR, dl, dr, gA);
     }
-    for (int o = 0; o < 6; o++) cF[kk][wi][o] = cA[o];
+    for (int o = 0; o < 5; o++) cF[kk][wi][o] = gA[o];
   }
   __syncthreads();
   if (!o_ok) return;
   double Ab = 0.0;
-  for (int fi = wi; fi <= wi + 5; fi++) Ab += cF[kk][fi][wi + 5 - fi];
-  const long x = at(c);
-  const bool live = (c >= lo && c < hi) || !q1;
-  double u[4], pi[10];
-  for (int m = 0; m < 4; m++) u[m] = si[(VIRAL_F_U0 + m) * n + x];
-  for (int m = 0; m < 10; m++) pi[m] = si[(VIRAL_F_PI0 + m) * n + x];
-  double eb = 0, Pb = 0, PIb = 0, ub[4] = {0, 0, 0, 0}, pib[10] = {0}, Tb[4] = {0, 0, 0, 0};
-  flux_cell_vjp(D, tau, si[VIRAL_F_EN * n + x], si[VIRAL_F_P * n + x], si[VIRAL_F_BULK * n + x], u, pi, Fb, Vb,
-                live ? Ab : 0.0, eb, Pb, PIb, ub, pib, Tb);
-  li[VIRAL_F_EN * n + x] += eb;
-  li[VIRAL_F_P * n + x] += Pb;
-  li[VIRAL_F_BULK * n + x] += PIb;
-  for (int m = 0; m < 4; m++) {
-    li[(VIRAL_F_U0 + m) * n + x] += ub[m];
-    li[(VIRAL_F_T00 + m) * n + x] += Tb[m];
-  }
-  for (int m = 0; m < 10; m++) li[(VIRAL_F_PI0 + m) * n + x] += pib[m];
+  for (int o = 0; o < 5; o++) Ab += cF[kk][wi + 2 - o][o];
+  if (!((c >= lo && c < hi) || !q1)) Ab = 0.0;
+  // A = u^{d+1}/u^0
+  udb += Ab * iu0;
+  ub0 -= Ab * ud * iu0 * iu0;
+  if (row == 1) ub1 += udb;
+  else if (row == 2) ub2 += udb;
+  else ub3 += udb;
+  const double k = D == 2 ? 1.0 / (tau * tau) : 1.0;
+  li[VIRAL_F_EN * n + xw] += wb;
+  li[VIRAL_F_P * n + xw] += ppb * k + wb;
+  li[VIRAL_F_BULK * n + xw] += -(ppb * k) - wb;
+  li[VIRAL_F_U0 * n + xw] += ub0;
+  li[(VIRAL_F_U0 + 1) * n + xw] += ub1;
+  li[(VIRAL_F_U0 + 2) * n + xw] += ub2;
+  li[(VIRAL_F_U0 + 3) * n + xw] += ub3;
 }
 
 #pragma clang fp contract(off)
