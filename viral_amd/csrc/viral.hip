@@ -1682,8 +1682,9 @@ __global__ void __launch_bounds__(BLK) k_adj_root(Geom g, Params p, StageArgs a,
 // Sources of one interior cell in reverse (inputs from the stage's input
 // state si; the source adjoint is resb, as L = Result + Source): adds to li,
 // writes dubuf, the adjoint of the spatial velocity gradients du[l][1..3]
-// (12 rows) for k_adj_du_src, and adds per-block sums of the adjoints of
-// eta_coef and tpi_coef to partial.
+// (12 rows, taken back through their WENO derivatives by the flux tiles of
+// each direction), and adds per-block sums of the adjoints of eta_coef and
+// tpi_coef to partial.
 __global__ void __launch_bounds__(BLK) k_adj_src(Geom g, Params p, StageArgs a, const double* __restrict__ si,
                                                  const double* __restrict__ resb, double* __restrict__ li,
                                                  double* __restrict__ dubuf, double* __restrict__ partial)
@@ -1748,65 +1749,6 @@ __global__ void __launch_bounds__(BLK) k_adj_src(Geom g, Params p, StageArgs a, 
   }
 }
 
-// Spatial velocity gradients in reverse (NEWEST10/Source.h:3-101), in two
-// passes.  k_adj_du_src: every interior centre turns the adjoint of its WENO
-// derivatives du[l][d] into contributions to its five stencil cells,
-// cbuf row (l*3 + d)*5 + o for the cell at offset o-2.  k_adj_du: every cell
-// sums what the centres whose stencils hold it contributed.
-constexpr int ADJ_DROWS = 4 * 3 * 5;
-
-__global__ void __launch_bounds__(BLK) k_adj_du_src(Geom g, Params p, const double* __restrict__ si,
-                                                    const double* __restrict__ dubuf, double* __restrict__ cbuf)
-{
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!box_cell(interior_box(g), t, i, j, k)) return;
-  const long n = g.n, x = cidx(g, i, j, k);
-  const double hx[3] = {p.dx, p.dy, p.deta};
-  const long sd[3] = {g.sx, g.sy, 1};
-  for (int d = 0; d < 3; d++) {
-    if (d == 2 && p.lbi) continue;
-    const long s = sd[d];
-    for (int l = 0; l < 4; l++) {
-      const double* ul = si + (VIRAL_F_U0 + l) * n;
-      const double q[5] = {ul[x - 2 * s], ul[x - s], ul[x], ul[x + s], ul[x + 2 * s]};
-      double gL[5], gR[5];
-      weno_grad(q, false, gL);
-      weno_grad(q, true, gR);
-      const double w = dubuf[(l * 3 + d) * n + x] / hx[d];
-      for (int o = 0; o < 5; o++) cbuf[((long)(l * 3 + d) * 5 + o) * n + x] = w * (gL[o] - gR[o]);
-    }
-  }
-}
-
-__global__ void __launch_bounds__(BLK) k_adj_du(Geom g, Params p, const double* __restrict__ cbuf,
-                                                double* __restrict__ li)
-{
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.n) return;
-  const int k = (int)(t % g.gz), j = (int)((t / g.gz) % g.gy), i = (int)(t / ((long)g.gz * g.gy));
-  const long n = g.n, x = t;
-  const bool iin = i >= g.il && i < g.ir, jin = j >= g.jl && j < g.jr, kin = k >= g.kl && k < g.kr;
-  const long sd[3] = {g.sx, g.sy, 1};
-  double acc[4] = {0, 0, 0, 0};
-  for (int d = 0; d < 3; d++) {
-    if (d == 2 && p.lbi) continue;
-    if (d == 0 && !(jin && kin)) continue;
-    if (d == 1 && !(iin && kin)) continue;
-    if (d == 2 && !(iin && jin)) continue;
-    const int pos = d == 0 ? i : d == 1 ? j : k;
-    const int lo = d == 0 ? g.il : d == 1 ? g.jl : g.kl, hi = d == 0 ? g.ir : d == 1 ? g.jr : g.kr;
-    const long s = sd[d];
-    for (int o = -2; o <= 2; o++) {
-      const int c = pos - o;  // centre whose stencil holds this cell at offset o
-      if (c < lo || c >= hi) continue;
-      const long xc = x - (long)o * s;
-      for (int l = 0; l < 4; l++) acc[l] += cbuf[((long)(l * 3 + d) * 5 + o + 2) * n + xc];
-    }
-  }
-  for (int l = 0; l < 4; l++) li[(VIRAL_F_U0 + l) * n + x] += acc[l];
-}
-
 // KT flux of direction D in reverse on an LDS line tile, with no global
 // intermediate.  A block takes TKA lines along D and a window of WA
 // consecutive cells of each (slot wi holds cell c0 - 5 + wi).  Per row, every
@@ -1827,8 +1769,9 @@ constexpr int WA = 64, TKA = 4, NTA = WA * TKA;
 constexpr int OWNA = WA - 10;
 
 template <int D>
-__global__ void __launch_bounds__(NTA) __attribute__((amdgpu_waves_per_eu(3))) k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si,
-                                                  const double* __restrict__ resb, double* __restrict__ li)
+__global__ void __launch_bounds__(NTA) __attribute__((amdgpu_waves_per_eu(3)))
+k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const double* __restrict__ resb,
+           const double* __restrict__ dubuf, double* __restrict__ li)
 {
   __shared__ double sF[TKA][WA];
   __shared__ double sV[TKA][WA];
@@ -1989,6 +1932,31 @@ __global__ void __launch_bounds__(NTA) __attribute__((amdgpu_waves_per_eu(3))) k
         udb += sf * fv * iu0;
         ub0 -= sf * fv * ud * iu0 * iu0;
       }
+    }
+  }
+  // the sources' velocity gradients du[l][D] in reverse: their WENO
+  // derivative along D (NEWEST10/Source.h:3-101, d u/dx = (L - R)/h) at every
+  // interior centre, the adjoint dubuf of k_adj_src turned into contributions
+  // to the five stencil cells, summed per own cell as above
+  const bool cin = rc && line_ok && c >= lo && c < hi;
+  for (int l = 0; l < 4; l++) {
+    sF[kk][wi] = w_ok ? si[(VIRAL_F_U0 + l) * n + xw] : 0.0;
+    __syncthreads();
+    double gq[5] = {0, 0, 0, 0, 0};
+    if (cin) {
+      const double w = dubuf[(long)(l * 3 + D) * n + xw] * ih;
+      if (w != 0.0) {
+        double q[5], dl, dr;
+        for (int o = 0; o < 5; o++) q[o] = sF[kk][wi - 2 + o];
+        weno_lr_vjp(q, w, -w, dl, dr, gq);
+      }
+    }
+    for (int o = 0; o < 5; o++) cF[kk][wi][o] = gq[o];
+    __syncthreads();
+    if (o_ok) {
+      double acc = 0.0;
+      for (int o = 0; o < 5; o++) acc += cF[kk][wi + 2 - o][o];
+      li[(VIRAL_F_U0 + l) * n + xw] += acc;
     }
   }
   // the face speeds in reverse: a = max(|A_L|, |A_R|) on the branch taken;
@@ -3385,11 +3353,10 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   double* vb2 = dalloc(SV * n);
   double* resb = dalloc(SV * n);
   double* dubuf = dalloc(12 * n);
-  double* fbuf = dalloc((size_t)ADJ_DROWS * n);
   double* partial = dalloc(2 * (size_t)nbi);
   double* saved = dalloc(NF * n);
   double* snaps = dalloc((size_t)nsnap * NF * n);
-  if (!lamA || !lamB || !vb0 || !vb1 || !vb2 || !resb || !dubuf || !fbuf || !partial || !saved || !snaps) {
+  if (!lamA || !lamB || !vb0 || !vb1 || !vb2 || !resb || !dubuf || !partial || !saved || !snaps) {
     release();
     return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the adjoint");
   }
@@ -3451,8 +3418,6 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         hipLaunchKernelGGL(k_adj_root, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, so, lamA, lamB, vb0, vb1, vb2,
                            resb);
         hipLaunchKernelGGL(k_adj_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB, dubuf, partial);
-        hipLaunchKernelGGL(k_adj_du_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, si, dubuf, fbuf);
-        hipLaunchKernelGGL(k_adj_du, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, p, fbuf, lamB);
         for (int d = 0; d < ndir; d++) {
           // lines of direction d (interior in the other two), TKA per block, cut into OWNA-cell segments
           const long nlines = d == 0 ? (long)(g.jr - g.jl) * (g.kr - g.kl)
@@ -3460,9 +3425,9 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
                                        : (long)(g.ir - g.il) * (g.jr - g.jl);
           const int GD = d == 0 ? g.gx : d == 1 ? g.gy : g.gz;
           const unsigned nt = (unsigned)(((nlines + TKA - 1) / TKA) * ((GD + OWNA - 1) / OWNA));
-          if (d == 0) hipLaunchKernelGGL(k_adj_tile<0>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, lamB);
-          if (d == 1) hipLaunchKernelGGL(k_adj_tile<1>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, lamB);
-          if (d == 2) hipLaunchKernelGGL(k_adj_tile<2>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, lamB);
+          if (d == 0) hipLaunchKernelGGL(k_adj_tile<0>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB);
+          if (d == 1) hipLaunchKernelGGL(k_adj_tile<1>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB);
+          if (d == 2) hipLaunchKernelGGL(k_adj_tile<2>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB);
         }
         std::swap(lamA, lamB);
       }
