@@ -1711,7 +1711,8 @@ __global__ void __launch_bounds__(BLK) k_adj_src(Geom g, Params p, StageArgs a, 
       for (int d = 0; d < 3; d++) {
         const long s = sd[d];
         du[l][d + 1] = (d == 2 && p.lbi) ? 0.0
-                                         : weno_der(ul[x - 2 * s], ul[x - s], ul[x], ul[x + s], ul[x + 2 * s], hx[d]);
+                                         : wder<true>(ul[x - 2 * s], ul[x - s], ul[x], ul[x + s], ul[x + 2 * s], hx[d],
+                                                      1.0 / hx[d]);
       }
     }
     double eb = 0, Pb = 0, PIb = 0, ub[4] = {0, 0, 0, 0}, pib[10] = {0}, dub[4][4];
