@@ -495,9 +495,9 @@ __device__ __forceinline__ void weno_LR_fast(double qm2, double qm1, double qc, 
   const double a1 = fma(-2.0, qc, qm1) + qp1, c1 = qm1 - qp1;
   const double a2 = fma(-2.0, qm1, qm2) + qc, c2 = fma(3.0, qc, fma(-4.0, qm1, qm2));
   const double k = 13.0 / 12.0;
-  double t0 = WENO_EPS + fma(k * a0, a0, 0.25 * (c0 * c0));
-  double t1 = WENO_EPS + fma(k * a1, a1, 0.25 * (c1 * c1));
-  double t2 = WENO_EPS + fma(k * a2, a2, 0.25 * (c2 * c2));
+  double t0 = fma(k * a0, a0, fma(0.25 * c0, c0, WENO_EPS));
+  double t1 = fma(k * a1, a1, fma(0.25 * c1, c1, WENO_EPS));
+  double t2 = fma(k * a2, a2, fma(0.25 * c2, c2, WENO_EPS));
   t0 = t0 * t0;
   t1 = t1 * t1;
   t2 = t2 * t2;
@@ -510,9 +510,12 @@ __device__ __forceinline__ void weno_LR_fast(double qm2, double qm1, double qc, 
   const double r1 = fma(2.0, qm1, fma(5.0, qc, -qp1));
   const double r2 = fma(5.0, qm1, fma(2.0, qc, -qm2));
   const double wl0 = 0.3 * P0, wl1 = 0.6 * P1, wl2 = 0.1 * P2;
-  const double wr0 = 0.1 * P0, wr1 = 0.6 * P1, wr2 = 0.3 * P2;
-  L = fma(wl0, l0, fma(wl1, l1, wl2 * l2)) * frcp(6.0 * (wl0 + wl1 + wl2));
-  R = fma(wr0, r0, fma(wr1, r1, wr2 * r2)) * frcp(6.0 * (wr0 + wr1 + wr2));
+  const double wr0 = 0.1 * P0, wr2 = 0.3 * P2;
+  // one reciprocal for both normalisations: 1/Dl = Dr/(Dl Dr), 1/Dr = Dl/(Dl Dr)
+  const double Dl = 6.0 * (wl0 + wl1 + wl2), Dr = 6.0 * (wr0 + wl1 + wr2);
+  const double iDD = frcp(Dl * Dr);
+  L = fma(wl0, l0, fma(wl1, l1, wl2 * l2)) * (Dr * iDD);
+  R = fma(wr0, r0, fma(wl1, r1, wr2 * r2)) * (Dl * iDD);
 }
 
 __device__ __forceinline__ double weno_L_fast(double qm2, double qm1, double qc, double qp1, double qp2)
