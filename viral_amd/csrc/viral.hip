@@ -1751,14 +1751,16 @@ __global__ void __launch_bounds__(BLK) k_adj_src(Geom g, Params p, StageArgs a, 
 }
 
 // KT flux of direction D in reverse on an LDS line tile, with no global
-// intermediate.  A block takes TKA lines along D and a window of WA
-// consecutive cells of each (slot wi holds cell c0 - 5 + wi).  Per row, every
+// intermediate.  A block takes TK lines along D and a window of W
+// consecutive cells of each (slot wi holds cell c0 - 5 + wi), W and TK chosen
+// per direction on the host so the segments of W - 10 own cells cover a line
+// with little waste.  Per row, every
 // window cell stages its centred flux and Var in LDS; every cell with a full
-// stencil in the window (slots 2..WA-3) forms both WENO reconstructions of F
+// stencil in the window (slots 2..W-3) forms both WENO reconstructions of F
 // and of Var (L at its right face, R at its left face) from one set of
 // smoothness indicators and turns the adjoints of its two faces' Rusanov
 // fluxes into contributions to its five stencil cells (weno_lr_vjp); every
-// own cell (slots 5..WA-6: its five contributors, and the eigenvalue
+// own cell (slots 5..W-6: its five contributors, and the eigenvalue
 // reconstructions their face speeds need, lie in the window) sums them in a
 // fixed order.  The face speeds a = max(|A_L|, |A_R|) come from the same
 // per-cell reconstructions of the eigenvalue, and their adjoint (collected
@@ -1766,34 +1768,38 @@ __global__ void __launch_bounds__(BLK) k_adj_src(Geom g, Params p, StageArgs a, 
 // the whole line, ghosts included, and tiles own disjoint cells, so each
 // cell's state adjoint is updated by one thread.  Slots map to cells with the
 // contiguous (eta) index fastest.
-constexpr int WA = 64, TKA = 4, NTA = WA * TKA;
-constexpr int OWNA = WA - 10;
+constexpr int NTA = 256;  // threads per block; the window shape is chosen per direction on the host
 
 template <int D>
 __global__ void __launch_bounds__(NTA) __attribute__((amdgpu_waves_per_eu(3)))
 k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const double* __restrict__ resb,
-           const double* __restrict__ dubuf, double* __restrict__ li)
+           const double* __restrict__ dubuf, double* __restrict__ li, int W, int TK, int nseg)
 {
-  __shared__ double sF[TKA][WA];
-  __shared__ double sV[TKA][WA];
-  __shared__ double xL[TKA][WA];  // per cell: A_L, then the face-adjoint parts
-  __shared__ double xR[TKA][WA];  // per cell: A_R
-  __shared__ double cF[TKA][WA][5];
-  __shared__ double cV[TKA][WA][5];
+  // one LDS slot per thread: thread t holds window slot wi of tile line kk,
+  // the window index fastest along eta and the line index fastest otherwise,
+  // so neighbouring window slots are st1 threads apart
+  __shared__ double sF[NTA];
+  __shared__ double sV[NTA];
+  __shared__ double xL[NTA];  // per cell: A_L, then the face-adjoint parts
+  __shared__ double xR[NTA];  // per cell: A_R
+  __shared__ double cF[NTA][5];
+  __shared__ double cV[NTA][5];
   const int t = threadIdx.x;
-  const int kk = D == 2 ? t / WA : t % TKA;  // line of the tile
-  const int wi = D == 2 ? t % WA : t / TKA;  // window slot
+  const int st1 = D == 2 ? 1 : TK;
+  const bool act = t < TK * W;               // threads past the last line idle (barriers only)
+  const int kk = D == 2 ? t / W : t % TK;    // line of the tile
+  const int wi = D == 2 ? t % W : t / TK;    // window slot
   const int GD = D == 0 ? g.gx : D == 1 ? g.gy : g.gz;
   const int lo = D == 0 ? g.il : D == 1 ? g.jl : g.kl;
   const int hi = D == 0 ? g.ir : D == 1 ? g.jr : g.kr;
-  const int nseg = (GD + OWNA - 1) / OWNA;
+  const int own = W - 10;                    // own cells per line and segment
   const int seg = (int)(blockIdx.x % nseg);
   const long lgrp = blockIdx.x / nseg;
-  const int c0 = seg * OWNA;
+  const int c0 = seg * own;
   const int nb = D == 2 ? g.jr - g.jl : g.kr - g.kl;
   const int na = D == 0 ? g.jr - g.jl : g.ir - g.il;
-  const long ln = lgrp * TKA + kk;
-  const bool line_ok = ln < (long)na * nb;
+  const long ln = lgrp * TK + kk;
+  const bool line_ok = act && ln < (long)na * nb;
   const int la = line_ok ? (int)(ln / nb) : 0, lb = line_ok ? (int)(ln % nb) : 0;
   auto at = [&](int cd) -> long {
     if (D == 0) return cidx(g, cd, g.jl + la, g.kl + lb);
@@ -1817,38 +1823,38 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
     ww = e + P - PI;
     pp = D == 2 ? (P - PI) / (tau * tau) : (P - PI);
   }
-  const bool rc = wi >= 2 && wi < WA - 2;  // full stencil in the window
+  const bool rc = act && wi >= 2 && wi < W - 2;  // full stencil in the window
   // faces of this cell: right c+1/2 (face index c), left c-1/2 (face index c-1)
   const bool fR = line_ok && rc && c >= lo - 1 && c <= hi - 1;
   const bool fL = line_ok && rc && c - 1 >= lo - 1 && c - 1 <= hi - 1;
   // the eigenvalue's reconstructions: A_L(c), A_R(c)
-  sF[kk][wi] = w_ok && ((c >= lo && c < hi) || !q1) ? ud / u0 : 0.0;
+  sF[t] = w_ok && ((c >= lo && c < hi) || !q1) ? ud / u0 : 0.0;
   __syncthreads();
   double As[5], AL = 0.0, AR = 0.0;
   if (rc) {
-    for (int o = 0; o < 5; o++) As[o] = sF[kk][wi - 2 + o];
+    for (int o = 0; o < 5; o++) As[o] = sF[t + (o - 2) * st1];
     AL = weno_L(As[0], As[1], As[2], As[3], As[4]);
     AR = weno_R(As[0], As[1], As[2], As[3], As[4]);
   }
-  xL[kk][wi] = AL;
-  xR[kk][wi] = AR;
+  xL[t] = AL;
+  xR[t] = AR;
   __syncthreads();
   // face speeds and branches: face c+1/2 takes |A_L(c)| and |A_R(c+1)|, zero
   // under q1 before the first interior cell
   double aFR = 0.0, aFL = 0.0;
   bool brR = true, brL = true;  // true: the A_L side is the maximum
   if (fR && !(q1 && c < lo)) {
-    const double ml = fabs(AL), mr = fabs(xR[kk][wi + 1]);
+    const double ml = fabs(AL), mr = fabs(xR[t + st1]);
     aFR = fmax(ml, mr);
     brR = ml >= mr;
   }
   if (fL && !(q1 && c - 1 < lo)) {
-    const double ml = fabs(xL[kk][wi - 1]), mr = fabs(AR);
+    const double ml = fabs(xL[t - st1]), mr = fabs(AR);
     aFL = fmax(ml, mr);
     brL = ml >= mr;
   }
   // own cell of this thread
-  const bool o_ok = line_ok && wi >= 5 && wi < WA - 5 && c < GD;
+  const bool o_ok = line_ok && wi >= 5 && wi < W - 5 && c < GD;
   // The own cell's centred flux and Var in reverse (flux_cell_vjp, one row at
   // a time): each row's field adjoints go straight to li (only this thread
   // touches the cell); the adjoints of w = e + P - PI, the pressure term, u^d
@@ -1874,8 +1880,8 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
       }
     }
     // (the previous row's sF, sV and cF reads all precede the barrier below)
-    sF[kk][wi] = Fw;
-    sV[kk][wi] = Vw;
+    sF[t] = Fw;
+    sV[t] = Vw;
     __syncthreads();
     {
       double gF[5] = {0, 0, 0, 0, 0}, gV[5] = {0, 0, 0, 0, 0};
@@ -1893,8 +1899,8 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
       if (fbR != 0.0 || fbL != 0.0) {
         double Vs[5], Fs[5], VL, VR, FL, FR;
         for (int o = 0; o < 5; o++) {
-          Vs[o] = sV[kk][wi - 2 + o];
-          Fs[o] = sF[kk][wi - 2 + o];
+          Vs[o] = sV[t + (o - 2) * st1];
+          Fs[o] = sF[t + (o - 2) * st1];
         }
         // f = (F_L + F_R)/2 - a (V_R - V_L)/2
         weno_lr_vjp(Vs, 0.5 * aFR * fbR, -0.5 * aFL * fbL, VL, VR, gV);
@@ -1903,16 +1909,16 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
         abL += -0.5 * fbL * VR;
       }
       for (int o = 0; o < 5; o++) {
-        cF[kk][wi][o] = gF[o];
-        cV[kk][wi][o] = gV[o];
+        cF[t][o] = gF[o];
+        cV[t][o] = gV[o];
       }
     }
     __syncthreads();
     if (o_ok) {
       double sf = 0.0, sv = 0.0;
       for (int o = 0; o < 5; o++) {  // stencil cells c+2 .. c-2 of this cell's contributors
-        sf += cF[kk][wi + 2 - o][o];
-        sv += cV[kk][wi + 2 - o][o];
+        sf += cF[t + (2 - o) * st1][o];
+        sv += cV[t + (2 - o) * st1][o];
       }
       if (l < 4) {
         // F_l = tau (pi^{d+1,l} + [l = d+1] pp + u^{d+1} u^l w), V_l = tau T^{tau l}
@@ -1941,46 +1947,46 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
   // to the five stencil cells, summed per own cell as above
   const bool cin = rc && line_ok && c >= lo && c < hi;
   for (int l = 0; l < 4; l++) {
-    sF[kk][wi] = w_ok ? si[(VIRAL_F_U0 + l) * n + xw] : 0.0;
+    sF[t] = w_ok ? si[(VIRAL_F_U0 + l) * n + xw] : 0.0;
     __syncthreads();
     double gq[5] = {0, 0, 0, 0, 0};
     if (cin) {
       const double w = dubuf[(long)(l * 3 + D) * n + xw] * ih;
       if (w != 0.0) {
         double q[5], dl, dr;
-        for (int o = 0; o < 5; o++) q[o] = sF[kk][wi - 2 + o];
+        for (int o = 0; o < 5; o++) q[o] = sF[t + (o - 2) * st1];
         weno_lr_vjp(q, w, -w, dl, dr, gq);
       }
     }
-    for (int o = 0; o < 5; o++) cF[kk][wi][o] = gq[o];
+    for (int o = 0; o < 5; o++) cF[t][o] = gq[o];
     __syncthreads();
     if (o_ok) {
       double acc = 0.0;
-      for (int o = 0; o < 5; o++) acc += cF[kk][wi + 2 - o][o];
+      for (int o = 0; o < 5; o++) acc += cF[t + (2 - o) * st1][o];
       li[(VIRAL_F_U0 + l) * n + xw] += acc;
     }
   }
   // the face speeds in reverse: a = max(|A_L|, |A_R|) on the branch taken;
   // abar(c+1/2) = abR(c) + abL(c+1)
   __syncthreads();
-  xL[kk][wi] = abL;
-  xR[kk][wi] = abR;
+  xL[t] = abL;
+  xR[t] = abR;
   __syncthreads();
   {
     double gA[5] = {0, 0, 0, 0, 0};
     double sAL = 0.0, sAR = 0.0;
-    if (fR && !(q1 && c < lo) && brR) sAL = (abR + xL[kk][wi + 1]) * (AL > 0 ? 1.0 : AL < 0 ? -1.0 : 0.0);
-    if (fL && !(q1 && c - 1 < lo) && !brL) sAR = (xR[kk][wi - 1] + abL) * (AR > 0 ? 1.0 : AR < 0 ? -1.0 : 0.0);
+    if (fR && !(q1 && c < lo) && brR) sAL = (abR + xL[t + st1]) * (AL > 0 ? 1.0 : AL < 0 ? -1.0 : 0.0);
+    if (fL && !(q1 && c - 1 < lo) && !brL) sAR = (xR[t - st1] + abL) * (AR > 0 ? 1.0 : AR < 0 ? -1.0 : 0.0);
     if (sAL != 0.0 || sAR != 0.0) {
       double dl, dr;
       weno_lr_vjp(As, sAL, sAR, dl, dr, gA);
     }
-    for (int o = 0; o < 5; o++) cF[kk][wi][o] = gA[o];
+    for (int o = 0; o < 5; o++) cF[t][o] = gA[o];
   }
   __syncthreads();
   if (!o_ok) return;
   double Ab = 0.0;
-  for (int o = 0; o < 5; o++) Ab += cF[kk][wi + 2 - o][o];
+  for (int o = 0; o < 5; o++) Ab += cF[t + (2 - o) * st1][o];
   if (!((c >= lo && c < hi) || !q1)) Ab = 0.0;
   // A = u^{d+1}/u^0
   udb += Ab * iu0;
@@ -3420,15 +3426,34 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
                            resb);
         hipLaunchKernelGGL(k_adj_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB, dubuf, partial);
         for (int d = 0; d < ndir; d++) {
-          // lines of direction d (interior in the other two), TKA per block, cut into OWNA-cell segments
+          // lines of direction d (interior in the other two), TK per block, each cut into nseg segments
+          // of W - 10 own cells; (nseg, W, TK) minimises blocks per line, nseg / TK
           const long nlines = d == 0 ? (long)(g.jr - g.jl) * (g.kr - g.kl)
                               : d == 1 ? (long)(g.ir - g.il) * (g.kr - g.kl)
                                        : (long)(g.ir - g.il) * (g.jr - g.jl);
           const int GD = d == 0 ? g.gx : d == 1 ? g.gy : g.gz;
-          const unsigned nt = (unsigned)(((nlines + TKA - 1) / TKA) * ((GD + OWNA - 1) / OWNA));
-          if (d == 0) hipLaunchKernelGGL(k_adj_tile<0>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB);
-          if (d == 1) hipLaunchKernelGGL(k_adj_tile<1>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB);
-          if (d == 2) hipLaunchKernelGGL(k_adj_tile<2>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB);
+          int bs = 0, bW = 0, bT = 0;
+          for (int ns = 1; ns <= GD; ns++) {
+            const int W = (GD + ns - 1) / ns + 10;
+            if (W > NTA) continue;
+            const int TK = NTA / W;
+            if (!bs || (long)ns * bT < (long)bs * TK) {
+              bs = ns;
+              bW = W;
+              bT = TK;
+            }
+            if (W <= 16) break;
+          }
+          const unsigned nt = (unsigned)(((nlines + bT - 1) / bT) * bs);
+          if (d == 0)
+            hipLaunchKernelGGL(k_adj_tile<0>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB,
+                               bW, bT, bs);
+          if (d == 1)
+            hipLaunchKernelGGL(k_adj_tile<1>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB,
+                               bW, bT, bs);
+          if (d == 2)
+            hipLaunchKernelGGL(k_adj_tile<2>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB,
+                               bW, bT, bs);
         }
         std::swap(lamA, lamB);
       }
