@@ -1130,6 +1130,25 @@ __global__ void k_clear_bad(unsigned long long* __restrict__ bad, int nev)
 }
 
 // last kernel of a step: tau += dtau (NEWEST10/main.cpp:156)
+// stream-ordered resets used by the adjoint's recompute: the clock, and the
+// two state buffers an event's table row points at
+__global__ void k_set_clock(Clock* __restrict__ clk, double tau, double ts)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    clk->tau = tau;
+    clk->ts = ts;
+    clk->step = 0;
+  }
+}
+
+__global__ void k_set_state_ptrs(EvDev* __restrict__ tab, int ev, double* a, double* b)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    tab[ev].st[0] = a;
+    tab[ev].st[1] = b;
+  }
+}
+
 __global__ void k_clock_advance(Clock* __restrict__ clk)
 {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -3382,20 +3401,21 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
     const int m0 = seg * every, m1 = std::min(m0 + every, M);
     // recompute the stage states of the segment from its checkpoint
     HIPCHK(hipMemcpyAsync(snaps, ctx->adj.ckpt[seg], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+    // every stage reads snapshot idx and writes snapshot idx + 1 in place:
+    // the table row points the stage's two state copies there (cells no
+    // stage writes, the corner ghosts, are never read)
     int idx = 0;
     for (int m = m0; m < m1; m++) {
-      HIPCHK(hipStreamSynchronize(ctx->stream));
-      ctx->h_clk->tau = ctx->adj.taus[m];
-      ctx->h_clk->ts = ctx->adj.tss[m];
-      ctx->h_clk->step = 0;
-      HIPCHK(hipMemcpyAsync(ctx->d_clk, ctx->h_clk, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, ctx->stream, ctx->d_clk, ctx->adj.taus[m],
+                         ctx->adj.tss[m]);
       for (int st = 0; st < rk; st++, idx++) {
-        HIPCHK(hipMemcpyAsync(E.st[0], snaps + (size_t)idx * NF * n, fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0,
+                           snaps + (size_t)idx * NF * n, snaps + (size_t)(idx + 1) * NF * n);
         launch_stage(ctx, st, 0);
-        HIPCHK(hipMemcpyAsync(snaps + (size_t)(idx + 1) * NF * n, E.st[1], fbytes, hipMemcpyDeviceToDevice,
-                              ctx->stream));
       }
     }
+    hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0, E.st[0], E.st[1]);
+    HIPCHK(hipGetLastError());
     // reverse sweep over the segment
     for (int m = m1 - 1; m >= m0; m--) {
       HIPCHK(hipMemsetAsync(vb0, 0, SV * n * sizeof(double), ctx->stream));
