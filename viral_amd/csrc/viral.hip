@@ -1803,6 +1803,7 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
   __shared__ double xR[NTA];  // per cell: A_R
   __shared__ double cF[NTA][5];
   __shared__ double cV[NTA][5];
+  __shared__ double sR[NTA];  // the row's partial-result adjoint, zero outside the interior
   const int t = threadIdx.x;
   const int st1 = D == 2 ? 1 : TK;
   const bool act = t < TK * W;               // threads past the last line idle (barriers only)
@@ -1881,40 +1882,48 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
   double wb = 0.0, ppb = 0.0, udb = 0.0, ub0 = 0.0, ub1 = 0.0, ub2 = 0.0, ub3 = 0.0;
   const double iu0 = frcp(u0);
   double abR = 0.0, abL = 0.0;  // this cell's parts of the speed adjoints of its two faces
-  const long xr = w_ok ? xw : 0;
+  // a row's global loads, issued one row ahead so they overlap the previous
+  // row's reconstructions: state field l, pi^{d+1,l} and u^l (l < 4), and the
+  // partial-result adjoint of row l (zero outside the interior)
+  double nA = 0.0, nB = 0.0, nM = 0.0, nR = 0.0;
+  auto row_load = [&](int l) {
+    if (!w_ok) return;
+    nA = si[(long)l * n + xw];
+    if (l < 4) {
+      nB = si[(VIRAL_F_PI0 + pidx(row, l)) * n + xw];
+      nM = si[(VIRAL_F_U0 + l) * n + xw];
+    }
+    nR = (c >= lo && c < hi) ? resb[l * n + xw] : 0.0;
+  };
+  row_load(0);
   for (int l = 0; l < SV; l++) {
     // centred flux and Var of row l (cell_FVA)
+    const double ldA = nA, ldB = nB, ldR = nR;
     double Fw = 0.0, Vw = 0.0, um = 0.0, fv = 0.0;
     if (w_ok) {
       if (l < 4) {
-        um = si[(VIRAL_F_U0 + l) * n + xw];
-        const double piv = si[(VIRAL_F_PI0 + pidx(row, l)) * n + xw];
-        const double v = l == row ? piv + pp + ud * um * ww : piv + ud * um * ww;
+        um = nM;
+        const double v = l == row ? ldB + pp + ud * um * ww : ldB + ud * um * ww;
         Fw = tau * v;
-        Vw = tau * si[(long)l * n + xw];
+        Vw = tau * ldA;
       } else {
-        fv = si[(long)l * n + xw];
+        fv = ldA;
         Fw = (fv * ud) / u0;
         Vw = fv;
       }
     }
-    // (the previous row's sF, sV and cF reads all precede the barrier below)
+    // (the previous row's sF, sV, sR and cF reads all precede the barrier below)
     sF[t] = Fw;
     sV[t] = Vw;
+    sR[t] = ldR;
+    if (l + 1 < SV) row_load(l + 1);
     __syncthreads();
     {
       double gF[5] = {0, 0, 0, 0, 0}, gV[5] = {0, 0, 0, 0, 0};
       // adjoints of the face fluxes: partial result of cell x is
       // -(f(x+1/2) - f(x-1/2))/h, interior cells only
-      double fbR = 0.0, fbL = 0.0;
-      if (fR) {
-        if (c >= lo && c < hi) fbR -= resb[l * n + xr] * ih;
-        if (c + 1 >= lo && c + 1 < hi) fbR += resb[l * n + at(c + 1)] * ih;
-      }
-      if (fL) {
-        if (c - 1 >= lo && c - 1 < hi) fbL -= resb[l * n + at(c - 1)] * ih;
-        if (c >= lo && c < hi) fbL += resb[l * n + xr] * ih;
-      }
+      const double fbR = fR ? sR[t + st1] * ih - sR[t] * ih : 0.0;
+      const double fbL = fL ? sR[t] * ih - sR[t - st1] * ih : 0.0;
       if (fbR != 0.0 || fbL != 0.0) {
         double Vs[5], Fs[5], VL, VR, FL, FR;
         for (int o = 0; o < 5; o++) {
