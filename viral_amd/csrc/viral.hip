@@ -3456,7 +3456,8 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         hipLaunchKernelGGL(k_adj_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB, dubuf, partial);
         for (int d = 0; d < ndir; d++) {
           // lines of direction d (interior in the other two), TK per block, each cut into nseg segments
-          // of W - 10 own cells; (nseg, W, TK) minimises blocks per line, nseg / TK
+          // of W - 10 own cells; (nseg, W, TK) minimises blocks per line, nseg / TK, with at least
+          // 8 lines per block across eta (the lines' loads are TK consecutive eta cells)
           const long nlines = d == 0 ? (long)(g.jr - g.jl) * (g.kr - g.kl)
                               : d == 1 ? (long)(g.ir - g.il) * (g.kr - g.kl)
                                        : (long)(g.ir - g.il) * (g.jr - g.jl);
@@ -3466,6 +3467,7 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
             const int W = (GD + ns - 1) / ns + 10;
             if (W > NTA) continue;
             const int TK = NTA / W;
+            if (d < 2 && TK < 8 && W > 18) continue;
             if (!bs || (long)ns * bT < (long)bs * TK) {
               bs = ns;
               bW = W;
