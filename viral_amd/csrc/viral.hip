@@ -3403,6 +3403,9 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
     HIPCHK(hipMemcpyAsync(lamA + f * n, seed[f], n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   }
   HIPCHK(hipMemsetAsync(partial, 0, 2 * (size_t)nbi * sizeof(double), ctx->stream));
+  HIPCHK(hipMemsetAsync(vb0, 0, SV * n * sizeof(double), ctx->stream));
+  HIPCHK(hipMemsetAsync(vb1, 0, SV * n * sizeof(double), ctx->stream));
+  HIPCHK(hipMemsetAsync(vb2, 0, SV * n * sizeof(double), ctx->stream));
   HIPCHK(hipMemcpyAsync(saved, E.st[ctx->cur], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
   const bool k0b = (p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi;
   const int ndir = p.lbi ? 2 : 3;
@@ -3427,9 +3430,9 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
     HIPCHK(hipGetLastError());
     // reverse sweep over the segment
     for (int m = m1 - 1; m >= m0; m--) {
-      HIPCHK(hipMemsetAsync(vb0, 0, SV * n * sizeof(double), ctx->stream));
-      HIPCHK(hipMemsetAsync(vb1, 0, SV * n * sizeof(double), ctx->stream));
-      HIPCHK(hipMemsetAsync(vb2, 0, SV * n * sizeof(double), ctx->stream));
+      // vb0..vb2 are zero here: cleared once below before the first segment,
+      // and k_adj_root's first-stage (and RK3 middle-stage) reverse zeroes
+      // what it consumes, so each reversed step leaves them zero
       for (int st = rk - 1; st >= 0; st--) {
         const size_t base = (size_t)((m - m0) * rk + st);
         const double* si = snaps + base * NF * n;
