@@ -200,6 +200,67 @@ def bench_ensemble(n_events, steps, tau0=0.6, ts=0.002, side=(96, 256), eta=(32,
     return out
 
 
+def bench_c3(steps, tau0=0.6, ts=0.002):
+    """The 256x256x96 grid of the fluctuating-event configuration (s95p EOS,
+    table in fast mode, eta/s = 0.12, tau_pi = 5 eta/(e+p), RK2): the viscous
+    KT step (autotuned) on 200 Gaussian hot spots of width 0.4 fm from two
+    Woods-Saxon nuclei (b ~ U(0,10) fm, seed 3), with the update kernel's HBM
+    figure; and the SHASTA step on the smooth Glauber state (b = 7 fm) of the
+    same grid.  SHASTA's step breaks down in the dilute periphery of both
+    states (e <= 0 after 4 steps on the hot spots, 10 on the Glauber state,
+    exact arithmetic included), so its leg runs at most 2 + 5 steps."""
+    import torch
+    from viral_amd import capi, ic
+    from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP, SHASTA
+    out = {"workload": "256x256x96 Milne grid, s95p table, eta/s=0.12, tau_pi=5eta/(e+p), RK2, dtau=%g, fast "
+                       "arithmetic; KT on 200 hot spots (width 0.4 fm, b~U(0,10) fm, seed 3), SHASTA on the smooth "
+                       "Glauber state (b=7 fm)" % ts}
+    for scheme in ("kt", "shasta"):
+        d = make_desc(256, 256, 96, dx=0.1, dy=0.1, deta=0.1, rk=2, eos=EOS_S95P, eta_law=ETA_TIMES_S, eta_coef=0.12,
+                      tpi_law=TPI_OVER_EP, tpi_coef=5.0, fast_math=1, eos_table=1,
+                      scheme=SHASTA if scheme == "shasta" else 0)
+        dref = d.copy()
+        dref.fast_math = dref.eos_table = 0
+        eosp = (lambda x: capi.eos_eval(dref, x)[0].reshape(x.shape))
+        if scheme == "kt":
+            st = ic.hotspots(d, tau0, ts, seed=3, eos_p=eosp)
+        else:
+            st = ic.glauber(d, tau0, ts, b=7.0, eos_p=eosp)
+            steps = min(steps, 5)
+        h = capi.Hydro(d)
+        h.upload(st)
+        tau = tau0
+        for _ in range(2):
+            r = h.step(tau, ts)
+            assert r == 0, r
+            tau += ts
+        if scheme == "kt":
+            h.autotune(reps=1)
+        h.timers(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = h.step(tau, ts)
+            assert r == 0, r
+            tau += ts
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rep = h.timer_report()
+        h.timers(False)
+        cells = d.nx * d.ny * d.nz
+        leg = {"value": cells * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt * 1e3 / steps, "steps": steps,
+               "launch": h.launch().as_dict(),
+               "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in rep.items() if v[1]}}
+        if scheme == "kt":
+            fused = leg["launch"]["fusion"] > 0
+            ms, nl = rep["update"]
+            leg["update_hbm"] = {"algorithmic_GBs": kernel_bytes("update", d, fused) / (ms / nl * 1e-3) / 1e9,
+                                 "frac": kernel_bytes("update", d, fused) / (ms / nl * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out[scheme] = leg
+        h.close()
+    return out
+
+
 def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
     """Reverse mode through `steps` viscous KT steps of a smooth Glauber event
     (b=8 fm, peak 30 GeV/fm^3) on a 128x128x32 Milne grid, eta/s=0.08,
@@ -264,6 +325,8 @@ def main():
                     help="steps of the adjoint leg (0 skips it)")
     ap.add_argument("--adjoint-eos", choices=["s95p", "conformal"], default="s95p")
     ap.add_argument("--adjoint-exact", action="store_true", help="run the adjoint leg in exact arithmetic")
+    ap.add_argument("--c3-steps", type=int, default=10,
+                    help="timed steps of the 256x256x96 hot-spot grid legs (KT and SHASTA); 0 skips them")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy so a counter run can scale "
@@ -359,6 +422,8 @@ def main():
     if args.small_events > 0:
         # many small events: the launch-bound regime the batched launch is for
         out["ensemble_small"] = bench_ensemble(args.small_events, 4 * args.ensemble_steps, side=(24, 64), eta=(8, 24))
+    if args.c3_steps > 0:
+        out["grid_256x256x96"] = bench_c3(args.c3_steps)
     if args.adjoint_steps > 0:
         out["adjoint"] = bench_adjoint(args.adjoint_steps, eos=args.adjoint_eos, fast=0 if args.adjoint_exact else 1)
     if not args.no_cpu:

@@ -191,11 +191,11 @@ def glauber(desc, tau, ts, b=7.0, e_peak_gev=30.0, R=6.62, a=0.546, plateau=1.5,
 
 
 def hotspots(desc, tau, ts, seed, n_spots=200, width=0.4, b=None, b_max=10.0, R=6.62, a=0.546, gamma_k=1.0,
-             e_scale_gev=10.0, plateau=1.5, falloff=1.0, eos_p=None):
+             e_scale_gev=10.0, plateau=1.5, falloff=1.0, eos_p=None, e_floor=1e-7):
     """Fluctuating event: n_spots Gaussian hot spots (width fm) whose centres
     are drawn from two Woods-Saxon nuclei at +-b/2 (b ~ U(0, b_max) unless
-    given) with Gamma(k)-distributed weights, times the plateau eta profile.
-    Seeded; fluid at rest, pi = 0."""
+    given) with Gamma(k)-distributed weights, times the plateau eta profile,
+    floored at e_floor times the maximum.  Seeded; fluid at rest, pi = 0."""
     rng = np.random.default_rng(seed)
     if b is None:
         b = rng.uniform(0.0, b_max)
@@ -219,7 +219,7 @@ def hotspots(desc, tau, ts, seed, n_spots=200, width=0.4, b=None, b_max=10.0, R=
         tr += w[q] * np.exp(-((x1 - cx[q]) ** 2 + (y1 - cy[q]) ** 2) / (2 * width * width))
     tr = tr / max(tr.max(), 1e-300)
     e = (e_scale_gev / GEVFM) * tr[:, :, None] * eta_profile(E[0, 0, :], plateau, falloff)[None, None, :]
-    e = np.maximum(e, 1e-7 * e.max())
+    e = np.maximum(e, e_floor * e.max())
     P = e / 3.0 if eos_p is None else eos_p(e)
     z = np.zeros_like(e)
     o = np.ones_like(e)
