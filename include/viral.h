@@ -62,6 +62,13 @@ enum { VIRAL_TPI_OVER_EP = 0,      /* tau_pi = (c*eta)/(e+p)   */
                                            by bracketed Newton-bisection on [0, A], A = T^tt - pi^tt, where a root
                                            exists (M^2 < A (A - PI)); such cells carry VIRAL_FLAG_RESCUED.  Off by
                                            default; cells whose Newton solve succeeds are unchanged either way */
+#define VIRAL_Q_FIX_VE        (1u << 6) /* the FIX build (NEWEST10/time.h:280-290, 328, 385, 471; FixTwo): at
+                                           the end of every step, after the ghost fill, zero the stored v_eta of
+                                           every interior cell when BMax > 0.98, BMax being DebugMSG's maximum of
+                                           sqrt(vx^2 + vy^2 + tau^2 ve^2) over the interior (debug.h:30-74) at the
+                                           end of the previous step (or of the uploaded state, at the first
+                                           step's tau), then take BMax of the new state.  Milne only; off in every
+                                           reference profile */
 #define VIRAL_Q_DEFAULT (VIRAL_Q_GHOST_EIGEN | VIRAL_Q_K0_BROADCAST | VIRAL_Q_DPDE_OF_P | VIRAL_Q_SHEAR_SLICES)
 
 /* State fields, each one ghosted array of doubles.  Names follow the

@@ -72,6 +72,8 @@ struct or_ctx {
   uint8_t* flags;
   int32_t* iters[3];
   int bad_cell, bad_stage; unsigned bad_flags;
+  double bmax;             /* DebugMSG's BMax of the last step (VIRAL_Q_FIX_VE) */
+  int bmax_valid;          /* 0 after the state was set from outside */
 };
 
 static inline size_t IX(const or_ctx* c, int i, int j, int k) { return ((size_t)i * c->gy + j) * c->gz + k; }
@@ -563,6 +565,7 @@ int or_ns_init(or_ctx* c, double tau, double ts, int mode, double* nspi)
 {
   const viral_desc* d = &c->d;
   if (d->coords != VIRAL_MILNE || mode < 0 || mode > 2 || (mode == 0 && !nspi)) return -1;
+  if (mode != 0) c->bmax_valid = 0;
   double** f = c->f;
   const size_t n = c->n;
   for (int i = 0; i < c->gx; i++)
@@ -810,6 +813,7 @@ long or_landau(or_ctx* c, double tau, const double* tmunu, int at_start, long lo
   const size_t n = c->n;
   long nbad = 0;
   *first = -1;
+  c->bmax_valid = 0;
   const double tau2 = tau * tau;
   const double guu[4] = {1.0, -1.0, -1.0, -1.0 / tau2};
   for (size_t x = 0; x < n; x++) {
@@ -1524,6 +1528,7 @@ int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)
   const int milne = d->coords == VIRAL_MILNE;
   double** f = c->f;
   if (!milne) return cart_root(c, iters, flags);
+  c->bmax_valid = 0;
   double t2 = milne ? tau * tau : 1.0;
   int bad = -1;
   for (int i = c->il; i < c->ir; i++)
@@ -1738,7 +1743,48 @@ static int stage_root(or_ctx* c, double tau_r, int stage)
   return bad;
 }
 
+/* BMax of DebugMSG (NEWEST10/debug.h:30-74): the largest
+ * sqrt(vx^2 + vy^2 + tau^2 ve^2) over the interior, from 0 (NaN never counts) */
+static double debug_bmax(const or_ctx* c, double tau)
+{
+  double m = 0.0;
+  for (int i = c->il; i < c->ir; i++)
+    for (int j = c->jl; j < c->jr; j++)
+      for (int k = c->kl; k < c->kr; k++) {
+        const size_t x = IX(c, i, j, k);
+        const double vx = c->f[VIRAL_F_VX][x], vy = c->f[VIRAL_F_VY][x], ve = c->f[VIRAL_F_VE][x];
+        const double b = sqrt(vx * vx + vy * vy + tau * tau * ve * ve);
+        if (b > m) m = b;
+      }
+  return m;
+}
+
+void or_state_changed(or_ctx* c) { c->bmax_valid = 0; }
+
+static int step_core(or_ctx* c, double tau, double ts);
+
+/* One step; with VIRAL_Q_FIX_VE the FIX build's end of step: FixTwo with the
+ * previous BMax, then DebugMSG's BMax at the step's start time
+ * (NEWEST10/time.h:385-392) */
 int or_step(or_ctx* c, double tau, double ts)
+{
+  const int fix = (c->d.quirks & VIRAL_Q_FIX_VE) && c->d.coords == VIRAL_MILNE;
+  if (fix && !c->bmax_valid) {
+    c->bmax = debug_bmax(c, tau);
+    c->bmax_valid = 1;
+  }
+  const int r = step_core(c, tau, ts);
+  if (fix) {
+    if (c->bmax > 0.98)
+      for (int i = c->il; i < c->ir; i++)
+        for (int j = c->jl; j < c->jr; j++)
+          for (int k = c->kl; k < c->kr; k++) c->f[VIRAL_F_VE][IX(c, i, j, k)] = 0;
+    c->bmax = debug_bmax(c, tau);
+  }
+  return r;
+}
+
+static int step_core(or_ctx* c, double tau, double ts)
 {
   const viral_desc* d = &c->d;
   size_t n = c->n;

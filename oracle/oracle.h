@@ -29,6 +29,8 @@ size_t or_cells(const or_ctx* c);
 
 /* one full RK step (NEWEST10/time.h:300-478); returns VIRAL_OK or VIRAL_ERR_PHYSICS */
 int or_step(or_ctx* c, double tau, double dtau);
+/* the state was written from outside (VIRAL_Q_FIX_VE takes BMax anew) */
+void or_state_changed(or_ctx* c);
 const uint8_t* or_flags(const or_ctx* c);
 const int32_t* or_iters(const or_ctx* c, int stage);
 int or_first_bad(const or_ctx* c, int* i, int* j, int* k, int* stage, unsigned* flags);

@@ -79,6 +79,7 @@ def lib():
         L.or_output.restype = C.c_size_t
         L.or_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.or_ns_init.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp]
+        L.or_state_changed.argtypes = [vp]
         L.or_landau.restype = C.c_long
         L.or_landau.argtypes = [vp, C.c_double, dp, C.c_int, lp]
         _lib = L
@@ -155,6 +156,7 @@ class Oracle:
         """state: array (NFIELDS, n)"""
         for f in range(NFIELDS):
             self.field(f)[:] = state[f]
+        self.L.or_state_changed(self.h)
 
     def state(self):
         return np.stack([self.field(f).copy() for f in range(NFIELDS)])

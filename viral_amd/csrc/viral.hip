@@ -1149,6 +1149,48 @@ __global__ void k_set_state_ptrs(EvDev* __restrict__ tab, int ev, double* a, dou
   }
 }
 
+// The FIX build's end of step (VIRAL_Q_FIX_VE; NEWEST10/time.h:280-290,
+// FixTwo): the stored v_eta of every interior cell is zeroed when the last
+// BMax exceeds 0.98; then DebugMSG's BMax (debug.h:30-74), the largest
+// sqrt(vx^2 + vy^2 + tau^2 ve^2) over the interior at the step's start tau,
+// is taken anew.  BMax is kept as the bits of a non-negative double, whose
+// integer order is the value order, so one atomicMax per block combines it.
+__global__ void __launch_bounds__(BLK) k_fix_ve(Geom g, double* __restrict__ st,
+                                                const unsigned long long* __restrict__ bmax)
+{
+  int i, j, k;
+  if (!box_cell(interior_box(g), (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!(__longlong_as_double((long long)*bmax) > 0.98)) return;
+  st[VIRAL_F_VE * g.n + cidx(g, i, j, k)] = 0.0;
+}
+
+__global__ void k_bmax_clear(unsigned long long* __restrict__ bmax)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *bmax = 0ull;
+}
+
+__global__ void __launch_bounds__(BLK) k_bmax(Geom g, const double* __restrict__ st, const Clock* __restrict__ clk,
+                                              unsigned long long* __restrict__ bmax)
+{
+  int i, j, k;
+  double m = 0.0;
+  if (box_cell(interior_box(g), (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) {
+    const long n = g.n, x = cidx(g, i, j, k);
+    const double tau = clk->tau;
+    const double vx = st[VIRAL_F_VX * n + x], vy = st[VIRAL_F_VY * n + x], ve = st[VIRAL_F_VE * n + x];
+    const double b = sqrt(vx * vx + vy * vy + tau * tau * ve * ve);
+    if (b > m) m = b;  // NaN never counts
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o));
+  __shared__ double red[BLK / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLK / 64; w++) m = fmax(m, red[w]);
+    if (m > 0.0) atomicMax(bmax, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
 __global__ void k_clock_advance(Clock* __restrict__ clk)
 {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -2086,6 +2128,8 @@ struct viral_ctx {
   Clock* h_clk = nullptr;
   unsigned long long* d_bad = nullptr;
   unsigned long long* h_bad = nullptr;
+  unsigned long long* d_bmax = nullptr;  // per event: BMax of the FIX build (VIRAL_Q_FIX_VE), as double bits
+  std::vector<char> bmax_ok;             // per event: d_bmax belongs to the current state
   double last_tau = 1.0, last_ts = 0.01;  // clock the autotuner times with
   // captured RK step per starting parity
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -2411,12 +2455,30 @@ static void launch_stage(viral_ctx* c, int stage, int cur)
 }
 
 // every stage of one step starting from copy cur0, then the clock advance
+// FIX build: BMax of event e's state copy `cur` at the clock's tau
+static void launch_bmax(viral_ctx* c, int e, int cur)
+{
+  const Event& E = c->ev[e];
+  hipLaunchKernelGGL(k_bmax_clear, dim3(1), dim3(64), 0, c->stream, c->d_bmax + e);
+  hipLaunchKernelGGL(k_bmax, dim3(nblocks(box_count(interior_box(E.g)))), dim3(BLK), 0, c->stream, E.g, E.st[cur],
+                     c->d_clk, c->d_bmax + e);
+}
+
+static bool fix_ve(const Event& E) { return (E.p.quirks & VIRAL_Q_FIX_VE) && E.p.milne; }
+
 static void launch_step(viral_ctx* c, int cur0)
 {
   int cur = cur0;
   for (int stage = 0; stage < c->rk; stage++) {
     launch_stage(c, stage, cur);
     cur ^= 1;
+  }
+  for (int e = 0; e < (int)c->ev.size(); e++) {
+    const Event& E = c->ev[e];
+    if (!fix_ve(E)) continue;
+    hipLaunchKernelGGL(k_fix_ve, dim3(nblocks(box_count(interior_box(E.g)))), dim3(BLK), 0, c->stream, E.g,
+                       E.st[cur], c->d_bmax + e);
+    launch_bmax(c, e, cur);
   }
   hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, c->stream, c->d_clk);
 }
@@ -2583,6 +2645,8 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
   ok = ok && hipMalloc(&ctx->d_clk, sizeof(Clock)) == hipSuccess;
   ok = ok && hipHostMalloc(&ctx->h_clk, sizeof(Clock), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipMalloc(&ctx->d_bad, nev * sizeof(unsigned long long)) == hipSuccess;
+  ok = ok && hipMalloc(&ctx->d_bmax, nev * sizeof(unsigned long long)) == hipSuccess;
+  ctx->bmax_ok.assign(nev, 0);
   ok = ok && hipHostMalloc(&ctx->h_bad, nev * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipMalloc(&ctx->dtmp, sizeof(unsigned long long)) == hipSuccess;
   ctx->ev.resize(nev);
@@ -2681,6 +2745,7 @@ void viral_destroy(viral_ctx* ctx)
   if (ctx->d_clk) hipFree(ctx->d_clk);
   if (ctx->h_clk) hipHostFree(ctx->h_clk);
   if (ctx->d_bad) hipFree(ctx->d_bad);
+  if (ctx->d_bmax) hipFree(ctx->d_bmax);
   if (ctx->h_bad) hipHostFree(ctx->h_bad);
   if (ctx->dtmp) hipFree(ctx->dtmp);
   if (ctx->d_diag) hipFree(ctx->d_diag);
@@ -2713,6 +2778,7 @@ int viral_upload_state(viral_ctx* ctx, int ev, const double* const* fields)
     HIPCHK(hipMemcpyAsync(E->st[1] + f * n, fields[f], n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->bmax_ok[ev] = 0;
   return VIRAL_OK;
 }
 
@@ -2753,6 +2819,11 @@ int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
   ctx->h_clk->step = 0;
   HIPCHK(hipMemcpyAsync(ctx->d_clk, ctx->h_clk, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_clear_bad, dim3((nev + 63) / 64), dim3(64), 0, ctx->stream, ctx->d_bad, nev);
+  for (int e = 0; e < nev; e++)
+    if (fix_ve(ctx->ev[e]) && !ctx->bmax_ok[e]) {  // init()'s DebugMSG of the uploaded state
+      launch_bmax(ctx, e, ctx->cur);
+      ctx->bmax_ok[e] = 1;
+    }
   const bool graph = ctx->cfg.graph && !ctx->timing;
   double trec = tau;
   for (int s = 0; s < nsteps; s++) {
@@ -2841,6 +2912,7 @@ int viral_root(viral_ctx* ctx, int ev, double tau)
   Event* E = get_ev(ctx, ev);
   if (!E) return fail(ctx, VIRAL_ERR_ARG, "bad event");
   if (!E->p.milne) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "viral_root covers the Milne (1-D) root solve");
+  ctx->bmax_ok[ev] = 0;
   const long ni = box_count(interior_box(E->g));
   hipLaunchKernelGGL(k_clear_bad, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bad + ev, 1);
   double* st = E->st[ctx->cur];
@@ -2875,6 +2947,7 @@ int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, doubl
   const long n = E->g.n;
   double* st = E->st[ctx->cur];
   const unsigned grid = nblocks(n);
+  if (mode != VIRAL_NS_OUTPUT) ctx->bmax_ok[ev] = 0;
   if (mode == VIRAL_NS_OUTPUT) {
     const size_t m = 10 * (size_t)n;
     if (m > ctx->out_cap) {
@@ -2910,6 +2983,7 @@ int viral_landau(viral_ctx* ctx, int ev, double tau, const double* const* tmunu,
     if (!tmunu[l]) return fail(ctx, VIRAL_ERR_ARG, "bad T^{mu nu} arrays or tau");
   const long n = E->g.n;
   const size_t m = 10 * (size_t)n;
+  ctx->bmax_ok[ev] = 0;
   if (m > ctx->out_cap) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->d_out) HIPCHK(hipFree(ctx->d_out));
@@ -3340,6 +3414,7 @@ int viral_adjoint_begin(viral_ctx* ctx, int every)
   const Event& E = ctx->ev[0];
   if (E.d.coords != VIRAL_MILNE || E.d.scheme != VIRAL_KT)
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint covers the Milne KT step");
+  if (E.d.quirks & VIRAL_Q_FIX_VE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint does not cover the FIX build");
   if ((E.d.quirks & VIRAL_Q_K0_BROADCAST) && !E.d.lbi && (E.g.k0 < E.g.kl || E.g.k0 >= E.g.kr))
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint needs k0 inside the interior");
   if (int r = viral_adjoint_end(ctx)) return r;

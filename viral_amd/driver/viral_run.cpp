@@ -71,7 +71,7 @@ void usage()
       "  grid:      --nx N --ny N --nz N --lbi 0|1 --dx F --dy F --deta F --coords milne|cartesian\n"
       "  step:      --rk 1|2|3 --scheme kt|shasta --fast 0|1 --eos conformal|s95p --eos-table 0|1\n"
       "             --eta-law times_s|s_over_4pi --eta-coef F --tpi-law over_ep|over_p|const --tpi-coef F\n"
-      "             --quirks BITS --max-iter N\n"
+      "             --quirks BITS --fix 0|1 --max-iter N\n"
       "  time:      --tau0 F --dtau F --max-steps N --tau-stop F --tmin-mev F\n"
       "  init:      --init initvar|gubser|glauber|bjorken --gubser-file PATH --gubser-rho0 F --gubser-that0 F\n"
       "             --b F --e-peak F --plateau F --falloff F\n"
@@ -158,6 +158,7 @@ bool parse(int argc, char** argv, Options& o)
     else if (k == "--tpi-law") d.tpi_law = v == "over_ep" ? VIRAL_TPI_OVER_EP : v == "const" ? VIRAL_TPI_CONST : VIRAL_TPI_OVER_P;
     else if (k == "--tpi-coef") d.tpi_coef = f;
     else if (k == "--quirks") d.quirks = (unsigned)std::strtoul(v.c_str(), nullptr, 0);
+    else if (k == "--fix") d.quirks = n ? (d.quirks | VIRAL_Q_FIX_VE) : (d.quirks & ~VIRAL_Q_FIX_VE);  // -DFIX
     else if (k == "--max-iter") d.max_iter = n;
     else if (k == "--tau0") o.tau0 = f;
     else if (k == "--dtau") o.ts = f;
