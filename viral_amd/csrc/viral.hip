@@ -2454,7 +2454,6 @@ static void launch_stage(viral_ctx* c, int stage, int cur)
   });
 }
 
-// every stage of one step starting from copy cur0, then the clock advance
 // FIX build: BMax of event e's state copy `cur` at the clock's tau
 static void launch_bmax(viral_ctx* c, int e, int cur)
 {
@@ -2466,6 +2465,8 @@ static void launch_bmax(viral_ctx* c, int e, int cur)
 
 static bool fix_ve(const Event& E) { return (E.p.quirks & VIRAL_Q_FIX_VE) && E.p.milne; }
 
+// every stage of one step starting from copy cur0, then the FIX build's end
+// of step and the clock advance
 static void launch_step(viral_ctx* c, int cur0)
 {
   int cur = cur0;
