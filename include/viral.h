@@ -368,7 +368,13 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps);
  * field taken as an independent input, and pgrad[0], pgrad[1] receive dJ/d
  * eta_coef and dJ/d tpi_coef.  The state is left as it was.  Single-event
  * Milne KT contexts, conformal or s95p EOS.  viral_adjoint_end frees the
- * recording. */
+ * recording.
+ *
+ * every == 0 keeps every stage's output state instead of checkpoints (the
+ * stages write straight into the recording; no recompute in viral_adjoint):
+ * NFIELDS * cells * 8 bytes per RK stage of every recorded step, allocated by
+ * each viral_steps call for its steps (63 GB for 200 RK2 steps of a
+ * 128x128x32 grid).  The recorded steps do not replay a hipGraph. */
 int viral_adjoint_begin(viral_ctx* ctx, int every);
 int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad, double* pgrad);
 int viral_adjoint_end(viral_ctx* ctx);

@@ -25,7 +25,7 @@ READ = [f for f in range(NFIELDS) if f != F_TEMP and not (F_VX <= f < F_VX + 3)]
 def run_gpu(d, st, nsteps, every=None, seed=None):
     h = capi.Hydro(d)
     h.upload(st)
-    if every:
+    if every is not None:
         h.adjoint_begin(every)
     r, done = h.steps(TAU, TS, nsteps)
     assert (r, done) == (OK, nsteps)
@@ -139,6 +139,29 @@ def test_checkpoint_interval_does_not_change_the_gradient():
     _, g1, p1 = run_gpu(d, st, 4, 1, seed)
     _, g3, p3 = run_gpu(d, st, 4, 3, seed)
     assert np.array_equal(g1, g3) and np.array_equal(p1, p3)
+
+
+@pytest.mark.parametrize("rk,fast", [(2, 0), (3, 1), (1, 0)])
+def test_stored_stage_states_match_recompute(rk, fast):
+    """every == 0 keeps each stage's state instead of recomputing it: the
+    forward state (every cell, two viral_steps calls) equals the unrecorded
+    run's, and the gradient equals the checkpointed adjoint's, bit for bit"""
+    d, st = small_case(rk=rk, fast=fast)
+    seed = np.random.default_rng(5).standard_normal(st.shape)
+    print("case adjoint stored rk=%d fast=%d" % (rk, fast), flush=True)
+    plain = run_gpu(d, st, 4)
+    _, g3, p3 = run_gpu(d, st, 4, 3, seed)
+    h = capi.Hydro(d)
+    h.upload(st)
+    h.adjoint_begin(0)
+    assert h.steps(TAU, TS, 1) == (OK, 1)
+    assert h.steps(TAU + TS, TS, 3) == (OK, 3)
+    out = h.download()
+    assert np.array_equal(out.view(np.int64), plain.view(np.int64))
+    g0, p0 = h.adjoint(seed)
+    assert np.array_equal(h.download().view(np.int64), plain.view(np.int64))
+    assert np.array_equal(g0, g3) and np.array_equal(p0, p3)
+    h.adjoint_end()
 
 
 def test_fast_mode_adjoint_matches_fd():

@@ -298,7 +298,7 @@ class Hydro:
 
     def adjoint_begin(self, every):
         """Start recording steps for the adjoint, with a state checkpoint every
-        `every` steps."""
+        `every` steps (every = 0: keep every stage state, no recompute)."""
         self._chk(self.L.viral_adjoint_begin(self.h, every))
 
     def adjoint(self, seed):
