@@ -1149,6 +1149,22 @@ __global__ void k_set_state_ptrs(EvDev* __restrict__ tab, int ev, double* a, dou
   }
 }
 
+// Copies what a stage writes from one state copy to another: every field of
+// the interior cells, and the ghost-filled fields (all but Temp) of the face
+// ghost cells (one index outside the interior).  The edge and corner ghosts,
+// which no stage writes, keep the destination's values.
+__global__ void __launch_bounds__(BLK) k_copy_written(Geom g, int lbi, const double* __restrict__ src,
+                                                      double* __restrict__ dst)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)NF * g.n) return;
+  const int f = (int)(t / g.n);
+  const long x = t - (long)f * g.n;
+  const int k = (int)(x % g.gz), j = (int)((x / g.gz) % g.gy), i = (int)(x / ((long)g.gz * g.gy));
+  const int out = (i < g.il || i >= g.ir) + (j < g.jl || j >= g.jr) + (!lbi && (k < g.kl || k >= g.kr));
+  if (out == 0 || (out == 1 && f != VIRAL_F_TEMP)) dst[t] = src[t];
+}
+
 // The FIX build's end of step (VIRAL_Q_FIX_VE; NEWEST10/time.h:280-290,
 // FixTwo): the stored v_eta of every interior cell is zeroed when the last
 // BMax exceeds 0.98; then DebugMSG's BMax (debug.h:30-74), the largest
@@ -2144,11 +2160,14 @@ struct viral_ctx {
   size_t out_cap = 0;
   // adjoint recording: the (tau, dtau) of every step since viral_adjoint_begin
   // and device copies of the state every `every` steps
+  // (every == 0: no checkpoints past the first; each stage's output state is
+  // kept instead, in one slab per viral_steps call)
   struct {
     bool on = false;
     int every = 0;
     std::vector<double> taus, tss;
     std::vector<double*> ckpt;
+    std::vector<double*> slabs, stages;
   } adj;
 };
 
@@ -2825,10 +2844,32 @@ int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
       launch_bmax(ctx, e, ctx->cur);
       ctx->bmax_ok[e] = 1;
     }
-  const bool graph = ctx->cfg.graph && !ctx->timing;
+  // adjoint recording that keeps every stage state: the stages write straight
+  // into a slab of stage states (the event's table row points there), and the
+  // last one is copied back into the state at the end
+  const bool store = ctx->adj.on && ctx->adj.every == 0;
+  const size_t sbytes = store ? (size_t)NF * ctx->ev[0].g.n * sizeof(double) : 0;
+  double* slab = nullptr;
+  if (store) {
+    if (hipMalloc(&slab, (size_t)nsteps * ctx->rk * sbytes) != hipSuccess)
+      return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the recorded stage states");
+    ctx->adj.slabs.push_back(slab);
+  }
+  const bool graph = ctx->cfg.graph && !ctx->timing && !store;
   double trec = tau;
   for (int s = 0; s < nsteps; s++) {
-    if (graph) {
+    if (store) {
+      for (int st = 0; st < ctx->rk; st++) {
+        // the call's first stage reads the state itself (equal, on every cell a
+        // stage reads, to the last recorded stage state or the first checkpoint)
+        double* in = (s == 0 && st == 0) ? ctx->ev[0].st[ctx->cur] : ctx->adj.stages.back();
+        double* out = slab + ((size_t)s * ctx->rk + st) * (sbytes / sizeof(double));
+        hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0, in, out);
+        launch_stage(ctx, st, 0);
+        ctx->adj.stages.push_back(out);
+      }
+      hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_clk);
+    } else if (graph) {
       if (int r = capture_step(ctx, ctx->cur)) return r;
       HIPCHK(hipGraphLaunch(ctx->gexec[ctx->cur], ctx->stream));
     } else {
@@ -2838,7 +2879,7 @@ int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
     if (ctx->adj.on) {
       ctx->adj.taus.push_back(trec);
       ctx->adj.tss.push_back(ts);
-      if (ctx->adj.taus.size() % ctx->adj.every == 0) {
+      if (!store && ctx->adj.taus.size() % ctx->adj.every == 0) {
         Event& E = ctx->ev[0];
         const size_t bytes = (size_t)NF * E.g.n * sizeof(double);
         double* c = nullptr;
@@ -2848,6 +2889,12 @@ int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
       }
     }
     trec = trec + ts;
+  }
+  if (store) {
+    Event& E = ctx->ev[0];
+    hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0, E.st[0], E.st[1]);
+    hipLaunchKernelGGL(k_copy_written, dim3(nblocks((long)NF * E.g.n)), dim3(BLK), 0, ctx->stream, E.g, E.p.lbi,
+                       ctx->adj.stages.back(), E.st[ctx->cur]);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(ctx->h_bad, ctx->d_bad, nev * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -3401,7 +3448,10 @@ int viral_adjoint_end(viral_ctx* ctx)
   if (!ctx) return VIRAL_ERR_ARG;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   for (double* c : ctx->adj.ckpt) hipFree(c);
+  for (double* c : ctx->adj.slabs) hipFree(c);
   ctx->adj.ckpt.clear();
+  ctx->adj.slabs.clear();
+  ctx->adj.stages.clear();
   ctx->adj.taus.clear();
   ctx->adj.tss.clear();
   ctx->adj.on = false;
@@ -3410,7 +3460,7 @@ int viral_adjoint_end(viral_ctx* ctx)
 
 int viral_adjoint_begin(viral_ctx* ctx, int every)
 {
-  if (!ctx || every < 1) return fail(ctx, VIRAL_ERR_ARG, "bad adjoint recording request");
+  if (!ctx || every < 0) return fail(ctx, VIRAL_ERR_ARG, "bad adjoint recording request");
   if (ctx->ev.size() != 1) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint runs on single-event contexts");
   const Event& E = ctx->ev[0];
   if (E.d.coords != VIRAL_MILNE || E.d.scheme != VIRAL_KT)
@@ -3456,7 +3506,9 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
     for (double* q : bufs) hipFree(q);
     bufs.clear();
   };
-  const int nsnap = std::min(every, std::max(M, 1)) * rk + 1;
+  const bool stored = every == 0;  // every stage state kept: one segment, no recompute
+  const int seglen = stored ? std::max(M, 1) : every;
+  const int nsnap = stored ? 1 : std::min(every, std::max(M, 1)) * rk + 1;
   double* lamA = dalloc(NF * n);
   double* lamB = dalloc(NF * n);
   double* vb0 = dalloc(SV * n);
@@ -3485,15 +3537,23 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   HIPCHK(hipMemcpyAsync(saved, E.st[ctx->cur], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
   const bool k0b = (p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi;
   const int ndir = p.lbi ? 2 : 3;
-  for (int seg = (M - 1) / every; seg >= 0 && M > 0; seg--) {
-    const int m0 = seg * every, m1 = std::min(m0 + every, M);
-    // recompute the stage states of the segment from its checkpoint
-    HIPCHK(hipMemcpyAsync(snaps, ctx->adj.ckpt[seg], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  std::vector<const double*> sp;  // the segment's stage states: sp[idx] feeds stage idx
+  for (int seg = (M - 1) / seglen; seg >= 0 && M > 0; seg--) {
+    const int m0 = seg * seglen, m1 = std::min(m0 + seglen, M);
+    sp.clear();
+    if (stored) {
+      sp.push_back(ctx->adj.ckpt[0]);
+      sp.insert(sp.end(), ctx->adj.stages.begin(), ctx->adj.stages.end());
+    } else {
+      for (int q = 0; q <= (m1 - m0) * rk; q++) sp.push_back(snaps + (size_t)q * NF * n);
+      // recompute the stage states of the segment from its checkpoint
+      HIPCHK(hipMemcpyAsync(snaps, ctx->adj.ckpt[seg], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     // every stage reads snapshot idx and writes snapshot idx + 1 in place:
     // the table row points the stage's two state copies there (cells no
     // stage writes, the corner ghosts, are never read)
     int idx = 0;
-    for (int m = m0; m < m1; m++) {
+    for (int m = m0; m < m1 && !stored; m++) {
       hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, ctx->stream, ctx->d_clk, ctx->adj.taus[m],
                          ctx->adj.tss[m]);
       for (int st = 0; st < rk; st++, idx++) {
@@ -3511,8 +3571,8 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
       // what it consumes, so each reversed step leaves them zero
       for (int st = rk - 1; st >= 0; st--) {
         const size_t base = (size_t)((m - m0) * rk + st);
-        const double* si = snaps + base * NF * n;
-        const double* so = snaps + (base + 1) * NF * n;
+        const double* si = sp[base];
+        const double* so = sp[base + 1];
         const StageArgs a = stage_args(ctx->adj.taus[m], ctx->adj.tss[m], rk, st);
         // ghost fill in reverse, one face pair at a time (x, y, eta) so that cells
         // next to two faces sum in a fixed order
