@@ -268,7 +268,10 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
     the final momentum anisotropy eps_2 with respect to the initial state,
     eta/s and the tau_pi coefficient.  eos "s95p" with fast=1 reads the s95p
     table (the adjoint differentiates its interpolant); "conformal" is e = 3p.
-    Reports the time of the recorded forward run and of the adjoint call,
+    Reports the time of the recorded forward run and of the adjoint call
+    (adjoint_ms: the C call, with the seed's upload and the gradient's
+    download, 2 x 29 fields over PCIe; adjoint_device_ms: its device part,
+    HIP events around the recompute and reverse kernels),
     first with every stage state kept during the forward run (every = 0, no
     recompute: 63 GB of HBM for 200 RK2 steps), then with checkpoints every
     `every` steps (the adjoint recomputes the stage states between them)."""
@@ -304,14 +307,17 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
             eps2 = autograd.momentum_anisotropy(d, fin, tau0 + steps * ts)
             eps2.backward()
             seed = fin.grad.numpy()
+        h.timers(True)
         t0 = time.perf_counter()
         grad, pg = h.adjoint(seed)
         t_adj = time.perf_counter() - t0
+        dev_ms = h.timer_report()["adjoint_device"][0]
+        h.timers(False)
         h.adjoint_end()
-        runs[ev] = (t_fwd, t_adj, grad, pg)
+        runs[ev] = (t_fwd, t_adj, grad, pg, dev_ms)
     h.close()
-    t_fwd, t_adj, grad, pg = runs[0]
-    c_fwd, c_adj, c_grad, c_pg = runs[every]
+    t_fwd, t_adj, grad, pg, dev = runs[0]
+    c_fwd, c_adj, c_grad, c_pg, c_dev = runs[every]
     assert np.array_equal(grad, c_grad) and np.array_equal(pg, c_pg), "stored and recomputed stage states disagree"
     return {"workload": "adjoint of %d viscous KT steps, smooth Glauber b=8 fm, Milne 128x128x32, %s EOS, "
                         "eta/s=0.08, tau_pi=5eta/(e+p), RK2, dtau=%g, every stage state kept in HBM, %s arithmetic; "
@@ -319,8 +325,10 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
                         % (steps, "s95p (table)" if s95 and fast else eos, ts, "fast" if fast else "exact"),
             "steps": steps, "forward_ms": t_plain * 1e3, "forward_recorded_ms": t_fwd * 1e3,
             "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_plain,
+            "adjoint_device_ms": dev, "adjoint_device_over_forward": dev / (t_plain * 1e3),
             "checkpointed": {"every": every, "forward_recorded_ms": c_fwd * 1e3, "adjoint_ms": c_adj * 1e3,
-                             "adjoint_over_forward": c_adj / t_plain, "same_gradient_bits": True},
+                             "adjoint_over_forward": c_adj / t_plain, "adjoint_device_ms": c_dev,
+                             "adjoint_device_over_forward": c_dev / (t_plain * 1e3), "same_gradient_bits": True},
             "eps2": float(eps2), "d_eps2_d_eta_coef": float(pg[0]), "d_eps2_d_tpi_coef": float(pg[1]),
             "d_eps2_d_e_norm": float(np.linalg.norm(grad[F_EN]))}
 

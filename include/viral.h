@@ -383,7 +383,10 @@ int viral_adjoint_end(viral_ctx* ctx);
  * every launch (instrumentation; the reference only times whole steps with
  * clock(), NEWEST10/main.cpp:107,170-171).  viral_timers(ctx, 1) clears and
  * starts recording; viral_timer_read returns the summed duration and launch
- * count of kernel id k in [0, viral_kernel_count()). */
+ * count of kernel id k in [0, viral_kernel_count()).  The last id,
+ * "adjoint_device", times the device part of each viral_adjoint call (its
+ * recompute and reverse kernels, without the seed's upload and the
+ * gradient's download). */
 int viral_timers(viral_ctx* ctx, int enable);
 int viral_timer_read(viral_ctx* ctx, int kernel, double* total_ms, long long* launches);
 int viral_kernel_count(void);

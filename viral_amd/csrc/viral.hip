@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <algorithm>
 #include <type_traits>
 #include <string>
@@ -2119,9 +2120,11 @@ struct Event {
   unsigned char* tab_closed = nullptr;
 };
 
-enum KernelId { K_FLUX_X = 0, K_FLUX_Y, K_FLUX_Z, K_SOURCE, K_UPDATE, K_K0, K_GHOST, K_COUNT };
+// K_ADJ: the device part of viral_adjoint (recompute and reverse kernels,
+// between the seed's upload and the gradient's download)
+enum KernelId { K_FLUX_X = 0, K_FLUX_Y, K_FLUX_Z, K_SOURCE, K_UPDATE, K_K0, K_GHOST, K_ADJ, K_COUNT };
 static const char* const kKernelNames[K_COUNT] = {"flux_x", "flux_y", "flux_eta", "source", "update",
-                                                  "k0_broadcast", "ghost_fill"};
+                                                  "k0_broadcast", "ghost_fill", "adjoint_device"};
 
 struct viral_ctx {
   std::vector<Event> ev;
@@ -3538,6 +3541,7 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   const bool k0b = (p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi;
   const int ndir = p.lbi ? 2 : 3;
   std::vector<const double*> sp;  // the segment's stage states: sp[idx] feeds stage idx
+  auto adj_timer = std::make_unique<KTimer>(ctx, K_ADJ);  // stops before the gradient's download
   for (int seg = (M - 1) / seglen; seg >= 0 && M > 0; seg--) {
     const int m0 = seg * seglen, m1 = std::min(m0 + seglen, M);
     sp.clear();
@@ -3629,6 +3633,7 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
       }
     }
   }
+  adj_timer.reset();
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(E.st[ctx->cur], saved, fbytes, hipMemcpyDeviceToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(E.st[ctx->cur ^ 1], saved, fbytes, hipMemcpyDeviceToDevice, ctx->stream));
