@@ -3,7 +3,8 @@ viral_adjoint_*): a recorded multi-step run as a differentiable function of
 the initial state and of the viscosity / relaxation coefficients.
 
 The forward runs on the device through the C ABI; backward calls
-viral_adjoint, which recomputes the stage states between checkpoints and
+viral_adjoint, which recomputes the stage states between checkpoints (every
+> 0) or reads the stage states the forward kept in HBM (every = 0), and
 applies the adjoint kernels.  torch only carries the host-side arrays.
 """
 import math
@@ -42,6 +43,8 @@ class HydroRun(torch.autograd.Function):
 
 
 def run(state, eta_coef, tpi_coef, desc, tau0, ts, nsteps, every=20):
+    """every: checkpoint interval of the recording; 0 keeps every stage state
+    (29 doubles per ghosted cell per RK stage) and skips the recompute."""
     return HydroRun.apply(state, eta_coef, tpi_coef, desc, tau0, ts, nsteps, every)
 
 
