@@ -409,7 +409,7 @@ def main():
     fused = launch["fusion"] > 0
     bytes_launch = kernel_bytes(dom, d, fused)
     achieved = bytes_launch / avg_s / 1e9
-    kernel_share = {k: round(v[0] / max(sum(x[0] for x in rep.values()), 1e-12), 4) for k, v in rep.items()}
+    kernel_share = {k: round(v[0] / max(sum(x[0] for x in rep.values()), 1e-12), 4) for k, v in rep.items() if v[1]}
     bytes_per_update = {1: 336, 2: 784, 3: 1232}[d.rk]
     out = {
         "metric": METRIC,
