@@ -49,6 +49,16 @@
 #define GEVFM 0.1973
 #define PIE 3.141592653589793
 
+/* Planted mistakes, for the tests that show each pin of the RK
+ * orchestration, the k0 broadcast and the GUBSER mirror fails on a wrong
+ * oracle (tests/test_oracle_pins.py builds one library per value into a
+ * temporary directory).  0, the only value any product or check library is
+ * built with, plants nothing. */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+static const int MUT = OR_MUTANT;
+
 struct or_ctx {
   viral_desc d;
   int gx, gy, gz;          /* ghosted dims */
@@ -1589,12 +1599,12 @@ int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)
       for (int j = c->jl; j < c->jr; j++)
         for (int k = c->kl; k < c->kr; k++) {
           if (k == c->k0) continue;
-          size_t x = IX(c, i, j, k), x0 = IX(c, i, j, c->k0);
+          size_t x = IX(c, i, j, k), x0 = IX(c, i, j, MUT == 7 ? c->k0 + 1 : c->k0);
           double vx = f[VIRAL_F_VX][x0], vy = f[VIRAL_F_VY][x0], ve = f[VIRAL_F_VE][x0];
           f[VIRAL_F_VE][x] = ve;
           f[VIRAL_F_VX][x] = vx;
           f[VIRAL_F_VY][x] = vy;
-          double u0 = 1.0 / sqrt(1.0 - vx * vx - vy * vy - t2 * ve * ve);
+          double u0 = 1.0 / sqrt(1.0 - vx * vx - vy * vy - (MUT == 8 ? 1.0 : t2) * ve * ve);
           f[VIRAL_F_U0][x] = u0;
           f[VIRAL_F_U1][x] = u0 * vx;
           f[VIRAL_F_U2][x] = u0 * vy;
@@ -1630,9 +1640,9 @@ void or_boundary(or_ctx* c)
           f[BFIELDS[q]][xr] = f[BFIELDS[q]][sr];
         }
         if (mirror) {
-          size_t ml = IX(c, c->il + B - g - 1, j, k), mr = IX(c, c->ir - g - 1, j, k);
+          size_t ml = IX(c, MUT == 9 ? c->il + g : c->il + B - g - 1, j, k), mr = IX(c, c->ir - g - 1, j, k);
           f[VIRAL_F_U1][xl] = f[VIRAL_F_U1][ml];
-          f[VIRAL_F_U2][xl] = f[VIRAL_F_U2][ml];
+          if (MUT != 10) f[VIRAL_F_U2][xl] = f[VIRAL_F_U2][ml];
           f[VIRAL_F_U1][xr] = f[VIRAL_F_U1][mr];
           f[VIRAL_F_U2][xr] = f[VIRAL_F_U2][mr];
         }
@@ -1807,25 +1817,31 @@ static int step_core(or_ctx* c, double tau, double ts)
   /* stage 2 */
   stage_rhs(c, tau + ts, ts, sub);
   if (d->rk == 2) {
+    const double w2 = MUT == 5 ? 1.0 : 0.5;
     for (int l = 0; l < SVAR; l++) FORI {
         size_t x = l * n + IX(c, i, j, k);
-        c->var2[x] = 0.5 * c->var0[x] + 0.5 * c->var1[x] + 0.5 * ts * L[x];
+        c->var2[x] = 0.5 * c->var0[x] + 0.5 * c->var1[x] + w2 * ts * L[x];
       }
     update_primary(c, c->var2, tau + ts);
+    if (MUT == 6) update_prevu(c);
     stage_root(c, tau + ts, 1);
     return c->bad_cell >= 0 ? VIRAL_ERR_PHYSICS : VIRAL_OK;
   }
+  const double a0 = MUT == 1 ? 0.5 : 0.75, a1 = MUT == 1 ? 0.5 : 0.25;
   for (int l = 0; l < SVAR; l++) FORI {
       size_t x = l * n + IX(c, i, j, k);
-      c->var2[x] = 0.75 * c->var0[x] + 0.25 * c->var1[x] + 0.25 * ts * L[x];
+      c->var2[x] = a0 * c->var0[x] + a1 * c->var1[x] + a1 * ts * L[x];
     }
-  update_primary(c, c->var2, tau + ts / 2.0);
-  stage_root(c, tau + ts / 2.0, 1);
+  const double tmid = MUT == 2 ? tau + ts : tau + ts / 2.0;
+  update_primary(c, c->var2, tmid);
+  if (MUT == 6) update_prevu(c);
+  stage_root(c, tmid, 1);
   /* stage 3 */
-  stage_rhs(c, tau + ts / 2.0, ts / 2.0, sub);
+  stage_rhs(c, tmid, MUT == 3 ? ts : ts / 2.0, sub);
+  const double b0 = MUT == 4 ? 0.25 : 1.0 / 3.0, b1 = MUT == 4 ? 0.75 : 2.0 / 3.0;
   for (int l = 0; l < SVAR; l++) FORI {
       size_t x = l * n + IX(c, i, j, k);
-      c->var1[x] = (1.0 / 3.0) * c->var0[x] + (2.0 / 3.0) * c->var2[x] + (2.0 / 3.0) * ts * L[x];
+      c->var1[x] = b0 * c->var0[x] + b1 * c->var2[x] + b1 * ts * L[x];
     }
   update_primary(c, c->var1, tau + ts);
   stage_root(c, tau + ts, 2);

@@ -33,6 +33,10 @@
  *   mode 4: CalcNS (Source.h:643-751) on a random state: the Navier-Stokes
  *           pi (nspi, nsPI), then the state after NSInit (Source.h:755-776).
  *   mode 5: the state after ZeroInit (Source.h:542-562) of a random state.
+ *   mode 6: under mpiexec with NPX*NPY ranks (the GUBSER profile's 6x6):
+ *           mpi_init (init.h:118-141), then boundary() on the same random
+ *           state in every rank; the two corner ranks write the state before
+ *           and after it to <out>.<xcord>_<ycord>.
  */
 #include <mpi.h>
 #include <iostream>
@@ -92,8 +96,9 @@ int main(int argc, char** argv)
   double tau_ = atof(argv[3]);
   double ts_ = atof(argv[4]);
   int mode = atoi(argv[5]);
-  FILE* f = fopen(out, "wb");
-  if (!f) return 3;
+  /* mode 6 runs on several ranks: each corner rank opens its own file below */
+  FILE* f = mode == 6 ? NULL : fopen(out, "wb");
+  if (!f && mode != 6) return 3;
 
   if (mode == 1) {
     int n = 4000;
@@ -121,6 +126,15 @@ int main(int argc, char** argv)
   xcord = 0;
   ycord = 0;
   zcord = 0;
+  if (mode == 6) {
+    /* run under mpiexec with NPX*NPY ranks: the reference's own mpi_init
+     * (NEWEST10/init.h:118-141) builds the Cartesian communicator and sets
+     * this rank's coordinates, so boundary() fills exactly the faces that lie
+     * on the domain edge (MYXLEFT/MYXRIGHT/MYYLEFT/MYYRIGHT == BOUNDARY) */
+    MPI_Comm_size(MPI_COMM_WORLD, &size);
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    mpi_init();
+  }
   const int one_rank = (NPX == 1 && NPY == 1);
   StoreInHeap();
   AllocatePack();
@@ -204,7 +218,51 @@ int main(int argc, char** argv)
 #ifdef LBI
   hdr[7] = 1;
 #endif
-  fwrite(hdr, sizeof(int), 8, f);
+  if (mode == 6) {
+    /* every rank holds the same seeded state; the ranks at the two corners of
+     * the rank grid write it before and after boundary() to
+     * <out>.<xcord>_<ycord> (the (0,0) rank fills the left x and y ghosts,
+     * the (NPX-1,NPY-1) rank the right ones) */
+    const int corner = (xcord == 0 && ycord == 0) || (xcord == NPX - 1 && ycord == NPY - 1);
+    if (corner) {
+      char name[4096];
+      snprintf(name, sizeof(name), "%s.%d_%d", out, xcord, ycord);
+      f = fopen(name, "wb");
+      if (!f) MPI_Abort(MPI_COMM_WORLD, 3);
+      hdr[6] = size;
+      fwrite(hdr, sizeof(int), 8, f);
+      double th6[2] = {tau, ts};
+      fwrite(th6, sizeof(double), 2, f);
+    }
+  } else {
+    fwrite(hdr, sizeof(int), 8, f);
+  }
+  if (mode == 6) {
+    size_t n6 = (size_t)XCM * YCM * ZCM;
+    std::vector<double> b6(n6);
+#define DUMP6(EXPR)                                                    \
+  do {                                                                  \
+    size_t q = 0;                                                       \
+    for (int i = 0; i < XCM; i++)                                       \
+      for (int j = 0; j < YCM; j++)                                     \
+        for (int k = 0; k < ZCM; k++) { cell& c = HydroGrid[i][j][k]; b6[q++] = (EXPR); } \
+    if (f) put(f, &b6[0], n6);                                          \
+  } while (0)
+#define DUMPSTATE6()                                                    \
+  do {                                                                  \
+    DUMP6(c.T00); DUMP6(c.T10); DUMP6(c.T20); DUMP6(c.T30);             \
+    for (int l = 0; l < 10; l++) DUMP6(c.pi[l]);                        \
+    DUMP6(c.PI); DUMP6(c.En); DUMP6(c.P); DUMP6(c.Temp); DUMP6(c.Vx); DUMP6(c.Vy); DUMP6(c.Ve); \
+    for (int l = 0; l < 4; l++) DUMP6(c.u[l]);                          \
+    for (int l = 0; l < 4; l++) DUMP6(c.prevu[l]);                      \
+  } while (0)
+    DUMPSTATE6();
+    boundary(HydroGrid);
+    DUMPSTATE6();
+    if (f) fclose(f);
+    MPI_Finalize();
+    return 0;
+  }
   double th[2] = {tau, ts};
   put(f, th, 2);
   size_t n = (size_t)XCM * YCM * ZCM;

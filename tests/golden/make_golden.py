@@ -8,6 +8,12 @@ profile) together with the reference's CalcSource and KT Result on the
 stencil reaches three cells, the velocity gradients two), so the oracle can
 recompute them on a 3x3x3-interior grid whose ghosts are the box halo.
 
+It also writes tests/golden/ref_mirror.npz: the GUBSER profile's boundary()
+run by the reference's own code on its 6x6 rank grid (the harness under
+MPICH's mpiexec, mode 6), keeping u^1 and u^2 on the four edge bands (three
+ghost rows and the three interior rows they mirror) before and after the
+fill, the left bands from rank (0,0) and the right ones from rank (5,5).
+
 Run from the repository root:  python tests/golden/make_golden.py
 """
 import os
@@ -50,6 +56,35 @@ def main():
         out[p + "result"] = R[(slice(None),) + core].copy()
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "ref_boxes.npz"), **out)
     print("wrote", len(CASES), "cases")
+    mirror_fixture()
+
+
+def mirror_bands(a, d):
+    """the four 6-row edge bands of a (gx, gy) field over the interior
+    columns: x-left, x-right (rows i), y-left, y-right (columns j)"""
+    gx, gy, _ = d.ghost_dims()
+    a = a.reshape(gx, gy)
+    B = 3
+    return dict(xl=a[0:2 * B, B:gy - B].copy(), xr=a[gx - 2 * B:gx, B:gy - B].copy(),
+                yl=a[B:gx - B, 0:2 * B].copy(), yr=a[B:gx - B, gy - 2 * B:gy].copy())
+
+
+def mirror_fixture():
+    from viral_amd.desc import F_U1, F_U2
+    ranks = refdata.run_ranks("ref_kt_gubser", 31, 1.0, 0.005, 36)
+    lo, hi = ranks[(0, 0)], ranks[(5, 5)]
+    d = lo["desc"]
+    out = {"tau_ts": np.array([lo["tau"], lo["ts"]])}
+    for name, f in (("u1", F_U1), ("u2", F_U2)):
+        before = mirror_bands(lo["state"][f], d)
+        after_lo = mirror_bands(lo["after_boundary"][f], d)
+        after_hi = mirror_bands(hi["after_boundary"][f], d)
+        for side in ("xl", "xr", "yl", "yr"):
+            out["%s_%s_before" % (name, side)] = before[side]
+            src = after_lo if side.endswith("l") else after_hi
+            out["%s_%s_after" % (name, side)] = src[side]
+    np.savez_compressed(os.path.join(ROOT, "tests", "golden", "ref_mirror.npz"), **out)
+    print("wrote the GUBSER mirror bands")
 
 
 if __name__ == "__main__":

@@ -21,69 +21,75 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("oracle library missing: %s (run __graft_entry__.build())" % LIB_PATH)
-        L = C.CDLL(LIB_PATH)
-        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
-        L.or_create.restype = vp
-        L.or_create.argtypes = [C.POINTER(ViralDesc)]
-        L.or_destroy.argtypes = [vp]
-        L.or_field.restype = dp
-        L.or_field.argtypes = [vp, C.c_int]
-        L.or_cells.restype = C.c_size_t
-        L.or_cells.argtypes = [vp]
-        L.or_step.restype = C.c_int
-        L.or_step.argtypes = [vp, C.c_double, C.c_double]
-        L.or_flags.restype = C.POINTER(C.c_uint8)
-        L.or_flags.argtypes = [vp]
-        L.or_iters.restype = ip
-        L.or_iters.argtypes = [vp, C.c_int]
-        L.or_first_bad.restype = C.c_int
-        L.or_first_bad.argtypes = [vp] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_uint)]
-        L.or_calc_source.argtypes = [vp, C.c_double, C.c_double, dp]
-        L.or_hydro_explicit.argtypes = [vp, C.c_double, C.c_double, dp]
-        L.or_root.restype = C.c_int
-        L.or_root.argtypes = [vp, C.c_double, ip, C.POINTER(C.c_uint8)]
-        L.or_boundary.argtypes = [vp]
-        L.or_max_temp_gev.restype = C.c_double
-        L.or_max_temp_gev.argtypes = [vp]
-        for name in ("or_eos_p", "or_eos_dpde", "or_eos_T"):
-            getattr(L, name).restype = C.c_double
-            getattr(L, name).argtypes = [C.POINTER(ViralDesc), C.c_double]
-        L.or_eos_s.restype = C.c_double
-        L.or_eos_s.argtypes = [C.POINTER(ViralDesc), C.c_double, C.c_double, C.c_double]
-        L.or_newton_cell.restype = C.c_double
-        L.or_newton_cell.argtypes = [C.POINTER(ViralDesc)] + [C.c_double] * 11 + [ip, C.POINTER(C.c_uint)]
-        for name in ("or_weno_L", "or_weno_R"):
-            getattr(L, name).restype = C.c_double
-            getattr(L, name).argtypes = [C.c_double] * 5
-        L.or_weno_der.restype = C.c_double
-        L.or_weno_der.argtypes = [C.c_double] * 6
-        for name in ("or_vm_exp", "or_vm_log"):
-            getattr(L, name).restype = C.c_double
-            getattr(L, name).argtypes = [C.c_double]
-        L.or_vm_pow.restype = C.c_double
-        L.or_vm_pow.argtypes = [C.c_double, C.c_double]
-        L.or_cart_pi10.argtypes = [dp, dp, dp]
-        L.or_cart_gnewton.restype = C.c_int
-        L.or_cart_gnewton.argtypes = [C.POINTER(ViralDesc), dp, dp, C.POINTER(C.c_uint)]
-        L.or_cart_f.argtypes = [C.POINTER(ViralDesc), dp, dp, dp]
-        L.or_cart_df.argtypes = [C.POINTER(ViralDesc), dp, dp, dp]
-        L.or_centre_flux.argtypes = [vp, C.c_int, C.c_double, dp]
-        L.or_shear_source.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_double, C.c_double, dp]
-        L.or_root_safeguard.restype = C.c_double
-        L.or_root_safeguard.argtypes = [C.POINTER(ViralDesc), C.c_double, C.c_double, C.c_double, ip]
-        lp = C.POINTER(C.c_longlong)
-        L.or_check_root.restype = C.c_int
-        L.or_check_root.argtypes = [vp, C.c_double, dp, dp, lp]
-        L.or_debug_msg.argtypes = [vp, C.c_double, dp, lp]
-        L.or_check_physics.argtypes = [vp, C.c_double, lp, lp]
-        L.or_output.restype = C.c_size_t
-        L.or_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
-        L.or_ns_init.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp]
-        L.or_state_changed.argtypes = [vp]
-        L.or_landau.restype = C.c_long
-        L.or_landau.argtypes = [vp, C.c_double, dp, C.c_int, lp]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def load(path):
+    """an oracle library at `path` with its prototypes set (the default one,
+    or a variant the tests built, such as one with a planted mistake)"""
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    L.or_create.restype = vp
+    L.or_create.argtypes = [C.POINTER(ViralDesc)]
+    L.or_destroy.argtypes = [vp]
+    L.or_field.restype = dp
+    L.or_field.argtypes = [vp, C.c_int]
+    L.or_cells.restype = C.c_size_t
+    L.or_cells.argtypes = [vp]
+    L.or_step.restype = C.c_int
+    L.or_step.argtypes = [vp, C.c_double, C.c_double]
+    L.or_flags.restype = C.POINTER(C.c_uint8)
+    L.or_flags.argtypes = [vp]
+    L.or_iters.restype = ip
+    L.or_iters.argtypes = [vp, C.c_int]
+    L.or_first_bad.restype = C.c_int
+    L.or_first_bad.argtypes = [vp] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_uint)]
+    L.or_calc_source.argtypes = [vp, C.c_double, C.c_double, dp]
+    L.or_hydro_explicit.argtypes = [vp, C.c_double, C.c_double, dp]
+    L.or_root.restype = C.c_int
+    L.or_root.argtypes = [vp, C.c_double, ip, C.POINTER(C.c_uint8)]
+    L.or_boundary.argtypes = [vp]
+    L.or_max_temp_gev.restype = C.c_double
+    L.or_max_temp_gev.argtypes = [vp]
+    for name in ("or_eos_p", "or_eos_dpde", "or_eos_T"):
+        getattr(L, name).restype = C.c_double
+        getattr(L, name).argtypes = [C.POINTER(ViralDesc), C.c_double]
+    L.or_eos_s.restype = C.c_double
+    L.or_eos_s.argtypes = [C.POINTER(ViralDesc), C.c_double, C.c_double, C.c_double]
+    L.or_newton_cell.restype = C.c_double
+    L.or_newton_cell.argtypes = [C.POINTER(ViralDesc)] + [C.c_double] * 11 + [ip, C.POINTER(C.c_uint)]
+    for name in ("or_weno_L", "or_weno_R"):
+        getattr(L, name).restype = C.c_double
+        getattr(L, name).argtypes = [C.c_double] * 5
+    L.or_weno_der.restype = C.c_double
+    L.or_weno_der.argtypes = [C.c_double] * 6
+    for name in ("or_vm_exp", "or_vm_log"):
+        getattr(L, name).restype = C.c_double
+        getattr(L, name).argtypes = [C.c_double]
+    L.or_vm_pow.restype = C.c_double
+    L.or_vm_pow.argtypes = [C.c_double, C.c_double]
+    L.or_cart_pi10.argtypes = [dp, dp, dp]
+    L.or_cart_gnewton.restype = C.c_int
+    L.or_cart_gnewton.argtypes = [C.POINTER(ViralDesc), dp, dp, C.POINTER(C.c_uint)]
+    L.or_cart_f.argtypes = [C.POINTER(ViralDesc), dp, dp, dp]
+    L.or_cart_df.argtypes = [C.POINTER(ViralDesc), dp, dp, dp]
+    L.or_centre_flux.argtypes = [vp, C.c_int, C.c_double, dp]
+    L.or_shear_source.argtypes = [C.c_int, C.c_double, dp, dp, dp, C.c_double, C.c_double, dp]
+    L.or_root_safeguard.restype = C.c_double
+    L.or_root_safeguard.argtypes = [C.POINTER(ViralDesc), C.c_double, C.c_double, C.c_double, ip]
+    lp = C.POINTER(C.c_longlong)
+    L.or_check_root.restype = C.c_int
+    L.or_check_root.argtypes = [vp, C.c_double, dp, dp, lp]
+    L.or_debug_msg.argtypes = [vp, C.c_double, dp, lp]
+    L.or_check_physics.argtypes = [vp, C.c_double, lp, lp]
+    L.or_output.restype = C.c_size_t
+    L.or_output.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+    L.or_ns_init.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp]
+    L.or_state_changed.argtypes = [vp]
+    L.or_landau.restype = C.c_long
+    L.or_landau.argtypes = [vp, C.c_double, dp, C.c_int, lp]
+    return L
 
 
 def _dp(a):
@@ -136,9 +142,9 @@ def shear_source(milne, tau, u, pi10, du, SH, tpi):
 class Oracle:
     """One event on the CPU oracle; fields are numpy views of its buffers."""
 
-    def __init__(self, desc):
+    def __init__(self, desc, L=None):
         self.desc = desc
-        self.L = lib()
+        self.L = L if L is not None else lib()
         self.h = self.L.or_create(C.byref(desc))
         self.n = self.L.or_cells(self.h)
         self.dims = desc.ghost_dims()

@@ -104,3 +104,40 @@ def run(name, seed, tau, ts, mode=0):
                 source=blocks[NFIELDS:NFIELDS + 15].copy(),
                 result=blocks[NFIELDS + 15:NFIELDS + 30].copy(),
                 after_boundary=blocks[NFIELDS + 30:].copy(), one_rank=bool(one_rank))
+
+
+def run_ranks(name, seed, tau, ts, nranks):
+    """mode 6: the harness under MPICH's mpiexec with nranks ranks (the
+    profile's NPX*NPY); returns {(xcord, ycord): dict(state, after_boundary)}
+    for the two corner ranks, whose boundary() fills the left (0,0) and the
+    right (NPX-1,NPY-1) x and y ghost layers."""
+    exe = os.path.join(REF_DIR, name)
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        base = os.path.join(td, "out.bin")
+        subprocess.run([MPIEXEC, "-n", str(nranks), exe, base, str(seed), repr(tau), repr(ts), "6"], check=True,
+                       cwd=td, stdout=subprocess.DEVNULL, timeout=300)
+        for fn in sorted(os.listdir(td)):
+            if not fn.startswith("out.bin."):
+                continue
+            xc, yc = (int(v) for v in fn.split(".")[-1].split("_"))
+            raw = open(os.path.join(td, fn), "rb").read()
+            hdr = np.frombuffer(raw[:32], dtype=np.int32)
+            _, gx, gy, gz, border, nz, size, lbi = [int(v) for v in hdr]
+            assert size == nranks
+            th = np.frombuffer(raw[32:48], dtype=np.float64)
+            blocks = np.frombuffer(raw[48:], dtype=np.float64).reshape(-1, gx, gy, gz)
+            if lbi and gz != 1:
+                blocks = blocks[:, :, :, :1]
+            blocks = blocks.reshape(blocks.shape[0], -1)
+            assert blocks.shape[0] == 2 * NFIELDS
+            out[(xc, yc)] = dict(desc=make_desc(**PROFILES[name]), tau=th[0], ts=th[1],
+                                 state=blocks[:NFIELDS].copy(), after_boundary=blocks[NFIELDS:].copy())
+    return out
+
+
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def mpi_available():
+    return os.path.exists(MPIEXEC)

@@ -141,14 +141,25 @@ def test_checkpoint_interval_does_not_change_the_gradient():
     assert np.array_equal(g1, g3) and np.array_equal(p1, p3)
 
 
-@pytest.mark.parametrize("rk,fast", [(2, 0), (3, 1), (1, 0)])
-def test_stored_stage_states_match_recompute(rk, fast):
+@pytest.mark.parametrize("rk,fast,variant", [(2, 0, "default"), (3, 1, "default"), (1, 0, "default"),
+                                             (2, 0, "gubser_lbi"), (3, 0, "no_k0")])
+def test_stored_stage_states_match_recompute(rk, fast, variant):
     """every == 0 keeps each stage's state instead of recomputing it: the
     forward state (every cell, two viral_steps calls) equals the unrecorded
-    run's, and the gradient equals the checkpointed adjoint's, bit for bit"""
-    d, st = small_case(rk=rk, fast=fast)
+    run's, and the gradient equals the checkpointed adjoint's, bit for bit;
+    also on the boost-invariant grid with the GUBSER mirror (no eta ghosts in
+    the stage slabs) and with the k0 broadcast off"""
+    from viral_amd.desc import Q_DEFAULT, Q_GUBSER_MIRROR, Q_K0_BROADCAST, ETA_TIMES_S, TPI_OVER_EP
+    if variant == "gubser_lbi":
+        d = make_desc(12, 11, 1, lbi=1, rk=rk, fast_math=fast, quirks=Q_DEFAULT | Q_GUBSER_MIRROR,
+                      eta_law=ETA_TIMES_S, eta_coef=0.2, tpi_law=TPI_OVER_EP, tpi_coef=5.0)
+        st = ic.random_smooth(d, seed=13, tau=TAU, pi_frac=0.02)
+    elif variant == "no_k0":
+        d, st = small_case(rk=rk, fast=fast, quirks=Q_DEFAULT & ~Q_K0_BROADCAST)
+    else:
+        d, st = small_case(rk=rk, fast=fast)
     seed = np.random.default_rng(5).standard_normal(st.shape)
-    print("case adjoint stored rk=%d fast=%d" % (rk, fast), flush=True)
+    print("case adjoint stored rk=%d fast=%d %s" % (rk, fast, variant), flush=True)
     plain = run_gpu(d, st, 4)
     _, g3, p3 = run_gpu(d, st, 4, 3, seed)
     h = capi.Hydro(d)
@@ -246,3 +257,33 @@ def test_torch_harness_momentum_anisotropy():
                   objective(e0, p - hh if t is ce else ce, p - hh if t is ct else ct)) / (2 * hh)
             print("param", float(g), float(fd), flush=True)
             assert abs(float(g) - float(fd)) <= 1e-5 * abs(float(fd)) + 1e-12
+
+
+def test_closing_a_recorded_context_frees_its_stage_states():
+    """every == 0 keeps the stage states in slabs of device memory: closing
+    the context (the autograd backward does) must free them, so repeated
+    recorded runs leave the device's free memory where it was"""
+    d = make_desc(24, 22, 16)
+    st = ic.random_smooth(d, seed=3, tau=TAU, pi_frac=0.02)
+    seed = np.random.default_rng(8).standard_normal(st.shape)
+    print("case adjoint memory release", flush=True)
+
+    def once():
+        x = torch.tensor(st, requires_grad=True)
+        fin = autograd.run(x, torch.tensor(d.eta_coef, dtype=torch.float64),
+                           torch.tensor(d.tpi_coef, dtype=torch.float64), d, TAU, TS, 6, every=0)
+        (fin * torch.from_numpy(seed)).sum().backward()
+        return x.grad
+
+    torch.cuda.synchronize()
+    once()
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    for _ in range(3):
+        g = once()
+        assert torch.isfinite(g).all()
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    # one recording holds 6 steps x 2 stages x 29 doubles per ghosted cell
+    slab = 6 * 2 * 29 * d.cells() * 8
+    assert free0 - free1 < slab / 2, (free0, free1, slab)

@@ -2755,7 +2755,12 @@ void viral_destroy(viral_ctx* ctx)
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   drop_graphs(ctx);
+  // the adjoint recording: checkpoints and, when every == 0, the stage-state slabs
   for (double* c : ctx->adj.ckpt) hipFree(c);
+  for (double* c : ctx->adj.slabs) hipFree(c);
+  ctx->adj.ckpt.clear();
+  ctx->adj.slabs.clear();
+  ctx->adj.stages.clear();
   for (Event& E : ctx->ev) {
     hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1);
     if (E.var2) hipFree(E.var2);
@@ -3501,16 +3506,23 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   const int rk = ctx->rk;
   const long ni = box_count(interior_box(g));
   const unsigned nbi = nblocks(ni);
-  std::vector<double*> bufs;
+  // device buffers of this call; the guard frees them on every return path
+  // (after draining the stream, so no kernel still reads them)
+  struct Bufs {
+    hipStream_t s;
+    std::vector<double*> v;
+    ~Bufs()
+    {
+      if (v.empty()) return;
+      hipStreamSynchronize(s);
+      for (double* q : v) hipFree(q);
+    }
+  } bufs{ctx->stream, {}};
   auto dalloc = [&](size_t count) -> double* {
     double* q = nullptr;
     if (hipMalloc(&q, count * sizeof(double)) != hipSuccess) return nullptr;
-    bufs.push_back(q);
+    bufs.v.push_back(q);
     return q;
-  };
-  auto release = [&]() {
-    for (double* q : bufs) hipFree(q);
-    bufs.clear();
   };
   const bool stored = every == 0;  // every stage state kept: one segment, no recompute
   const int seglen = stored ? std::max(M, 1) : every;
@@ -3525,15 +3537,10 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   double* partial = dalloc(2 * (size_t)nbi);
   double* saved = dalloc(NF * n);
   double* snaps = dalloc((size_t)nsnap * NF * n);
-  if (!lamA || !lamB || !vb0 || !vb1 || !vb2 || !resb || !dubuf || !partial || !saved || !snaps) {
-    release();
+  if (!lamA || !lamB || !vb0 || !vb1 || !vb2 || !resb || !dubuf || !partial || !saved || !snaps)
     return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the adjoint");
-  }
   for (int f = 0; f < NF; f++) {
-    if (!seed[f] || !grad[f]) {
-      release();
-      return fail(ctx, VIRAL_ERR_ARG, "null field pointer");
-    }
+    if (!seed[f] || !grad[f]) return fail(ctx, VIRAL_ERR_ARG, "null field pointer");
     HIPCHK(hipMemcpyAsync(lamA + f * n, seed[f], n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   }
   HIPCHK(hipMemsetAsync(partial, 0, 2 * (size_t)nbi * sizeof(double), ctx->stream));
@@ -3652,7 +3659,6 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   }
   pgrad[0] = s0;
   pgrad[1] = s1;
-  release();
   return VIRAL_OK;
 }
 
