@@ -45,6 +45,12 @@ enum { VIRAL_ETA_TIMES_S = 0,      /* eta = c * s              */
 enum { VIRAL_TPI_OVER_EP = 0,      /* tau_pi = (c*eta)/(e+p)   */
        VIRAL_TPI_OVER_P = 1,       /* tau_pi = c*(eta/p)       */
        VIRAL_TPI_CONST = 2 };      /* tau_pi = c               */
+/* bulk viscosity law FZeta, NEWEST10/maindef.h:191 (const) and :82,140,289 (c s/(4 pi)) */
+enum { VIRAL_ZETA_CONST = 0,       /* zeta = c                 */
+       VIRAL_ZETA_S_OVER_4PI = 1 };/* zeta = c * (s / (4 pi))  */
+/* bulk relaxation time law FtauPI, NEWEST10/maindef.h:192 (const) and :83,141,290 (c zeta/p) */
+enum { VIRAL_TPIB_CONST = 0,       /* tau_PI = c               */
+       VIRAL_TPIB_OVER_P = 1 };    /* tau_PI = c * zeta / p    */
 
 /* Reference behaviours kept as switchable modes (all on = "as reference"). */
 #define VIRAL_Q_GHOST_EIGEN   (1u << 0) /* q1: eigenvalues only in the interior; first face of each row gets zero
@@ -131,6 +137,14 @@ typedef struct viral_desc {
                                interpolation, 1024 nodes per octave of e, closed form next
                                to its branch points and below 1.2e-4 fm^-4) instead of
                                the closed form                                            */
+  int bulk;                 /* 1: the -DBULK build (BULKTEST profile, NEWEST10/maindef.h:
+                               157-158): Israel-Stewart bulk source of PI
+                               (NEWEST10/Source.h:519-527; TXYZ5/Source.h:327-333), the
+                               Navier-Stokes PI = -zeta theta of NSInit (Source.h:740-745)
+                               and the kept PI of the Landau matching (ginit.h:996-999).
+                               0: PI's source is zero and the initial states set PI = 0  */
+  int zeta_law;  double zeta_coef;  /* VIRAL_ZETA_*: bulk viscosity (FZeta)            */
+  int tpib_law;  double tpib_coef;  /* VIRAL_TPIB_*: bulk relaxation time (FtauPI)     */
 } viral_desc;
 
 /* Fill *d with the reference's defaults for an nx*ny*nz Milne KT RK2 grid. */
@@ -203,14 +217,16 @@ int viral_root(viral_ctx* ctx, int ev, double tau);
  * gradients CalcDer4Vel (Source.h:3-101) leaves on a freshly zeroed grid:
  *   VIRAL_NS_OUTPUT  the Navier-Stokes pi = 2 eta sigma^{mu nu} (CalcNS's nspi)
  *                    into nspi[0..9], host arrays of viral_cells() doubles in
- *                    the pi index map; the state is unchanged
- *   VIRAL_NS_INIT    NSInit: pi = that, PI = 0, T^{tau mu} recomputed from
- *                    e, P, u and pi
+ *                    the pi index map, and with desc.bulk the Navier-Stokes
+ *                    PI = -zeta theta (nsPI, Source.h:740-745) into nspi[10];
+ *                    the state is unchanged
+ *   VIRAL_NS_INIT    NSInit: pi = that, PI = nsPI (0 without bulk),
+ *                    T^{tau mu} recomputed from e, P, u, pi and PI
  *   VIRAL_ZERO_INIT  ZeroInit: pi = 0, PI = 0, T^{tau mu} recomputed
  * Exact arithmetic and the closed-form EOS in every mode.  Milne events only
  * (VIRAL_ERR_UNSUPPORTED otherwise: TXYZ5 declares but never defines them).
- * VIRAL_NS_OUTPUT allocates 10 x cells doubles of staging on first use (kept
- * until viral_destroy, shared with viral_output). */
+ * VIRAL_NS_OUTPUT allocates 10 (11) x cells doubles of staging on first use
+ * (kept until viral_destroy, shared with viral_output). */
 enum { VIRAL_NS_OUTPUT = 0, VIRAL_NS_INIT = 1, VIRAL_ZERO_INIT = 2 };
 int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, double* const* nspi);
 
@@ -224,7 +240,7 @@ int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, doubl
  * exactly one such) and sets e = |lambda|, Temp, P, v and u; at_start = 0
  * (the call at tau - ts) also sets prevu = u and nothing else; at_start = 1
  * sets pi = T - T_ideal - PI (g^{-1} - u u) with PI = P - e/3, then PI = 0
- * and T^{tau mu} from e, P, u and pi.  Where no single eigenvector passes
+ * (kept when desc.bulk, as ginit.h:996-999) and T^{tau mu} from e, P, u and pi.  Where no single eigenvector passes
  * (the reference prints EIGEN VALUE SEARCH FAILED and exits) the cell keeps
  * its state and gets VIRAL_FLAG_SOLVER in viral_flags, and the call returns
  * VIRAL_ERR_PHYSICS with viral_first_bad_cell naming the first such cell in

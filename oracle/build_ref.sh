@@ -29,4 +29,5 @@ build ref_kt_gubser -DKT -DGUBSER
 build ref_kt_ginit -DKT -DGINIT
 build ref_shas -DSHAS
 build ref_shas_ginit -DSHAS -DGINIT
+build ref_kt_bulktest -DKT -DBULKTEST
 echo "built $(ls "$OUT" | tr '\n' ' ')"

@@ -242,6 +242,16 @@ static double tpi_of(const viral_desc* d, double eta, double p, double e)
     default: return d->tpi_coef * eta / (e + p);
   }
 }
+/* FZeta / FtauPI laws of the -DBULK build, NEWEST10/maindef.h:82-83,140-141,
+ * 191-192,289-290 */
+static double zeta_of(const viral_desc* d, double s)
+{
+  return d->zeta_law == VIRAL_ZETA_S_OVER_4PI ? d->zeta_coef * (s / (4.0 * PIE)) : d->zeta_coef;
+}
+static double tpib_of(const viral_desc* d, double zeta, double p)
+{
+  return d->tpib_law == VIRAL_TPIB_OVER_P ? d->tpib_coef * (zeta / p) : d->tpib_coef;
+}
 
 /* ------------------------------------------------------------------ WENO */
 /* NEWEST10/utils.h:102-160: WENO5-JS, d = (0.3, 0.6, 0.1), eps = 1e-6, p = 2.
@@ -367,6 +377,26 @@ int or_first_bad(const or_ctx* c, int* i, int* j, int* k, int* stage, unsigned* 
 }
 
 /* ------------------------------------------------------------------ sources */
+
+/* Israel-Stewart bulk relaxation source of PI (conservative form
+ * d_tau PI + d_i(v^i PI) = S), the physics of NEWEST10/Source.h:519-527
+ * (Milne) and TXYZ5/Source.h:327-333 (Cartesian):
+ *   u.d PI = -(PI + zeta theta)/tau_PI - 4/3 PI theta,
+ *   theta  = d_mu u^mu, plus u^tau/tau in Milne coordinates (the Christoffel
+ *            G^eta_{eta tau} = 1/tau),
+ *   S      = u.d PI / u^0 + PI d_i(u^i/u^0).
+ * du[a][b] = d_b u^a. */
+static double bulk_source(int milne, double tau, const double u[4], double PI, double du[4][4], double zeta,
+                          double tPI)
+{
+  double iu0 = 1.0 / u[0];
+  double divv = 0;
+  for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
+  double theta = du[0][0] + du[1][1] + du[2][2] + du[3][3];
+  if (milne) theta = theta + u[0] / tau;
+  double rhs = -(PI + zeta * theta) / tPI - (MUT == 11 ? 0.0 : 4.0 / 3.0) * PI * theta;
+  return rhs * iu0 + PI * divv;
+}
 
 /* Israel-Stewart shear relaxation source for the ten pi^{mu nu} (conservative
  * form: d_tau pi + d_i(v^i pi) = S), the physics of NEWEST10/Source.h:448-516:
@@ -567,8 +597,9 @@ static void ns_du(const or_ctx* c, double ts, int i, int j, int k, double du[4][
 }
 
 /* CalcNS / NSInit / ZeroInit (NEWEST10/Source.h:542-562, 643-776) over every
- * ghosted cell: mode 0 writes the Navier-Stokes pi to nspi (10 x cells);
- * mode 1 sets pi to it, mode 2 to zero; both then set PI = 0 (no BULK) and
+ * ghosted cell: mode 0 writes the Navier-Stokes pi to nspi (10 x cells, and
+ * with bulk PI = -zeta theta as an 11th row, Source.h:740-745); mode 1 sets
+ * pi and PI to them (PI = 0 without bulk), mode 2 both to zero; both then set
  * T^{tau mu} = (e + P - PI) u^0 u^mu - (P - PI) g^{tau mu} + pi^{tau mu}.
  * Milne only (TXYZ5 declares NSInit but does not define it). */
 int or_ns_init(or_ctx* c, double tau, double ts, int mode, double* nspi)
@@ -582,7 +613,7 @@ int or_ns_init(or_ctx* c, double tau, double ts, int mode, double* nspi)
     for (int j = 0; j < c->gy; j++)
       for (int k = 0; k < c->gz; k++) {
         const size_t x = IX(c, i, j, k);
-        double pn[10] = {0};
+        double pn[10] = {0}, nsPI = 0.0;
         if (mode != 2) {
           double du[4][4], u[4];
           ns_du(c, ts, i, j, k, du);
@@ -591,14 +622,20 @@ int or_ns_init(or_ctx* c, double tau, double ts, int mode, double* nspi)
           const double T = or_eos_T(d, e);
           const double s = or_eos_s(d, e, P, T);
           ns_shear(1, tau, u, du, eta_of(d, s), pn);
+          if (d->bulk) {
+            /* theta = d_mu u^mu + u^tau/tau */
+            const double theta = du[0][0] + du[1][1] + du[2][2] + du[3][3] + u[0] / tau;
+            nsPI = -(zeta_of(d, s) * theta);
+          }
         }
         if (mode == 0) {
           for (int l = 0; l < 10; l++) nspi[l * n + x] = pn[l];
+          if (d->bulk) nspi[10 * n + x] = nsPI;
           continue;
         }
         for (int l = 0; l < 10; l++) f[VIRAL_F_PI0 + l][x] = pn[l];
-        f[VIRAL_F_BULK][x] = 0.0;
-        const double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x], PI = 0.0;
+        f[VIRAL_F_BULK][x] = nsPI;
+        const double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x], PI = nsPI;
         const double u0 = f[VIRAL_F_U0][x];
         f[VIRAL_F_T00][x] = -P + PI + (e + P - PI) * (u0 * u0) + pn[0];
         for (int l = 1; l < 4; l++) f[VIRAL_F_T00 + l][x] = (e + P - PI) * u0 * f[VIRAL_F_U0 + l][x] + pn[l];
@@ -812,7 +849,7 @@ static int landau_cell(double tau, const double T[10], double* E, double vout[3]
  * Landau matching, then e, Temp = T(e), P = P(e), v and u; at_start 0 (the
  * call at tau - ts) stores prevu = u; at_start 1 sets PI = P - e/3 (minus a
  * third of the ideal part's trace), pi = T - T_ideal - PI (g^{-1} - u u),
- * then PI = 0 (no BULK) and T^{tau mu} from e, P, u and pi.  Every ghosted
+ * then PI = 0 (kept with bulk) and T^{tau mu} from e, P, u and pi.  Every ghosted
  * cell; a cell without exactly one passing eigenvector ("EIGEN VALUE SEARCH
  * FAILED", where the reference exits) is left alone and counted.  Returns the
  * number of such cells and the first in loop order in *first (-1 if none). */
@@ -859,8 +896,8 @@ long or_landau(or_ctx* c, double tau, const double* tmunu, int at_start, long lo
         pn[PIDX[a][b]] = T[PIDX[a][b]] - tid[a][b] - PImat;
       }
     for (int l = 0; l < 10; l++) f[VIRAL_F_PI0 + l][x] = pn[l];
-    f[VIRAL_F_BULK][x] = 0.0;
-    const double PI = 0.0;
+    const double PI = d->bulk ? PIt : 0.0; /* ginit.h:996-999 zeroes it without BULK */
+    f[VIRAL_F_BULK][x] = PI;
     f[VIRAL_F_T00][x] = -P + PI + (e + P - PI) * (u0 * u0) + pn[0];
     for (int l = 1; l < 4; l++) f[VIRAL_F_T00 + l][x] = (e + P - PI) * u0 * u[l] + pn[l];
   }
@@ -965,12 +1002,20 @@ void or_calc_source(or_ctx* c, double tau, double ts, double* S)
         double tpi = tpi_of(d, SH, P, e);
         double out[10];
         shear_source(milne, tau, u, pi, du, SH, tpi, out);
+        double sb = 0.0;
+        if (d->bulk) {
+          const double BU = zeta_of(d, s);
+          sb = bulk_source(milne, tau, u, f[VIRAL_F_BULK][x], du, BU, tpib_of(d, BU, P));
+        }
         if (milne) {
           for (int l = 0; l < 10; l++) S[(4 + l) * n + x] = out[l];
+          S[14 * n + x] = sb;
         } else {
+          /* TXYZ5 evaluates the bulk row in the same slice loop (Source.h:272,327) */
           const int live = !(d->quirks & VIRAL_Q_SHEAR_SLICES) || k < d->nz;
           for (int l = 0; l < 5; l++) S[(4 + l) * n + x] = live ? out[CART5[l]] : 0.0;
           for (int l = 5; l < 10; l++) S[(4 + l) * n + x] = 0.0;
+          S[14 * n + x] = live ? sb : 0.0;
         }
       }
 }

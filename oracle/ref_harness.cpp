@@ -165,6 +165,10 @@ int main(int argc, char** argv)
         c.En = e;
         c.P = EOS(e);
         c.PI = 0;
+#ifdef BULK
+        /* the -DBULK build (BULKTEST profile) evolves PI: give it a value */
+        c.PI = 0.05 * e * (urand() - 0.5);
+#endif
         double u1 = 0.3 * sin(0.7 * x + ph[3]) + 0.05 * urand();
         double u2 = 0.3 * cos(0.6 * y + ph[4]) + 0.05 * urand();
         double u3 = 0.1 * sin(0.9 * z + ph[5]) / tau + 0.01 * urand();
@@ -173,10 +177,10 @@ int main(int argc, char** argv)
         for (int l = 0; l < 4; l++) c.prevu[l] = c.u[l] * (1.0 + 0.02 * (urand() - 0.5));
         for (int l = 0; l < 10; l++) c.pi[l] = 0.2 * e * (urand() - 0.5) * (l == 3 || l == 6 || l == 8 ? 1.0 / tau : 1.0) *
                                                 (l == 9 ? 1.0 / tau : 1.0);
-        c.T00 = (e + c.P) * c.u[0] * c.u[0] - c.P + c.pi[0];
-        c.T10 = (e + c.P) * c.u[0] * u1 + c.pi[1];
-        c.T20 = (e + c.P) * c.u[0] * u2 + c.pi[2];
-        c.T30 = (e + c.P) * c.u[0] * u3 + c.pi[3];
+        c.T00 = (e + c.P - c.PI) * c.u[0] * c.u[0] - (c.P - c.PI) + c.pi[0];
+        c.T10 = (e + c.P - c.PI) * c.u[0] * u1 + c.pi[1];
+        c.T20 = (e + c.P - c.PI) * c.u[0] * u2 + c.pi[2];
+        c.T30 = (e + c.P - c.PI) * c.u[0] * u3 + c.pi[3];
         c.Vx = u1 / c.u[0]; c.Vy = u2 / c.u[0]; c.Ve = u3 / c.u[0];
         c.Temp = FT(e);
         /* cell positions as init() sets them (NEWEST10/init.h:44-81) */

@@ -243,8 +243,9 @@ class Oracle:
         return out
 
     def ns_pi(self, tau, ts):
-        """CalcNS's Navier-Stokes pi of the current state, (10, cells)"""
-        out = np.zeros((10, self.n))
+        """CalcNS's Navier-Stokes pi of the current state, (10, cells); with
+        bulk (11, cells), row 10 the Navier-Stokes PI"""
+        out = np.zeros((11 if self.desc.bulk else 10, self.n))
         assert self.L.or_ns_init(self.h, tau, ts, 0, _dp(out)) == 0
         return out
 

@@ -7,13 +7,14 @@ import tempfile
 
 import numpy as np
 
-from viral_amd.desc import NFIELDS, make_desc, ETA_S_OVER_4PI, TPI_OVER_P, EOS_S95P, SHASTA
+from viral_amd.desc import (NFIELDS, make_desc, ETA_S_OVER_4PI, ETA_TIMES_S, TPI_OVER_P, TPI_CONST, EOS_S95P, SHASTA,
+                            ZETA_CONST, TPIB_CONST)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
 
 # compile-time profiles of the harness binaries, as runtime descriptors
-# (NEWEST10/maindef.h:36-87 default, :236-294 GINIT, :298-339 GUBSER)
+# (NEWEST10/maindef.h:36-87 default, :236-294 GINIT, :298-339 GUBSER, :157-196 BULKTEST)
 PROFILES = {
     "ref_kt": dict(nx=40, ny=40, nz=101, dx=0.1, dy=0.1, deta=0.1,
                    eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5),
@@ -24,6 +25,10 @@ PROFILES = {
                      eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5),
     "ref_shas_ginit": dict(nx=30, ny=30, nz=101, dx=0.1, dy=0.1, deta=0.1, eos=EOS_S95P, scheme=SHASTA,
                            eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5),
+    # BULKTEST (NEWEST10/maindef.h:157-196): -DBULK, zeta = 1, tau_PI = 0.1
+    "ref_kt_bulktest": dict(nx=10, ny=10, nz=41, dx=0.1, dy=0.1, deta=0.1, eta_law=ETA_TIMES_S, eta_coef=0.2,
+                            tpi_law=TPI_CONST, tpi_coef=0.005, bulk=1, zeta_law=ZETA_CONST, zeta_coef=1.0,
+                            tpib_law=TPIB_CONST, tpib_coef=0.1),
 }
 
 

@@ -73,3 +73,43 @@ def test_driver_files_and_reports_match_the_binding(tmp_path, name):
     assert seen == set(files), (seen, files)
     # the first step's output and at least one print-interval output
     assert "tau%2.3ffm.bin" % (tau0 + ts) in seen and len(seen) >= 3
+
+
+@pytest.mark.parametrize("name", ["bjorken", "bulktest"])
+def test_driver_temp_file_matches_the_oracle(tmp_path, name):
+    """The BJORKEN and BULKTEST profiles' temp.txt (NEWEST10/main.cpp:124-141):
+    one "tau  value" line before every step, value the centre cell's T in GeV
+    (BJORKEN) or PI (BULKTEST), printed as C++ streams print doubles (6
+    significant digits).  The CPU oracle stepped from the same initial state
+    must give the same lines; the state at the last line is the step count's."""
+    import oracle_ctypes
+    from viral_amd.desc import F_EN, F_BULK, GEVFM, ZETA_CONST, TPIB_CONST, TPI_CONST
+    assert os.path.exists(EXE), "viral_amd/viral_run not built"
+    print("case driver temp.txt %s" % name, flush=True)
+    nsteps = 40
+    dump = tmp_path / "init.bin"
+    temp = tmp_path / "temp.txt"
+    r = subprocess.run([EXE, "--profile", name, "--max-steps", str(nsteps), "--write", "0", "--checks", "0",
+                        "--dump-state", str(dump), "--temp-file", str(temp)], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-3000:]
+    lines = temp.read_text().splitlines()
+    assert len(lines) == nsteps
+    kw = dict(eta_law=ETA_TIMES_S, eta_coef=0.2, tpi_law=TPI_CONST, tpi_coef=0.005)
+    if name == "bulktest":
+        kw.update(bulk=1, zeta_law=ZETA_CONST, zeta_coef=1.0, tpib_law=TPIB_CONST, tpib_coef=0.1)
+    d = make_desc(10, 10, 41, **kw)
+    st = np.fromfile(dump, dtype=np.float64).reshape(NFIELDS, -1)
+    o = oracle_ctypes.Oracle(d)
+    o.set_state(st)
+    centre = np.ravel_multi_index((8, 8, 20), d.ghost_dims())
+    L = oracle_ctypes.lib()
+    t, ts = 0.6, 0.01
+    for s in range(nsteps):
+        if name == "bjorken":
+            v = (1000 * (L.or_eos_T(d, float(o.state()[F_EN][centre])) * GEVFM)) / 1000
+        else:
+            v = float(o.state()[F_BULK][centre])
+        assert lines[s] == "%g  %g" % (t, v), (s, lines[s], t, v)
+        assert o.step(t, ts) == OK
+        t = t + ts

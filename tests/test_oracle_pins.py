@@ -2,7 +2,8 @@
 the RK1/2/3 orchestration (NEWEST10/time.h:300-478; time.h needs root.h,
 which needs GSL), the k0 velocity broadcast (NEWEST10/root.h:426-449) and
 the GUBSER mirror of u^1, u^2 in the ghosts (NEWEST10/boundary.h:32-37,
-77-81, 118-122, 160-165; the GUBSER profile runs on 6x6 ranks).
+77-81, 118-122, 160-165; the GUBSER profile runs on 6x6 ranks), and the bulk
+source's time evolution (the BULKTEST profile integrates it, main.cpp:137-141).
 
 - RK orchestration: Bjorken flow (the reference's BJORKEN profile,
   NEWEST10/maindef.h:200-232: conformal EOS, e0 = 30 GeV/fm^3 at tau0 = 0.6,
@@ -16,6 +17,9 @@ the GUBSER mirror of u^1, u^2 in the ghosts (NEWEST10/boundary.h:32-37,
   stage time steps of CalcSource and the prevu bookkeeping, time.h:358,418)
   is pinned by the viscous Gubser flow at RK1/2/3: its error must shrink
   under refinement.
+- Bulk: the BULKTEST profile (BJORKEN with -DBULK, zeta = 1, tau_PI = 0.1)
+  against the same equations with dPI/dtau = -(PI + zeta/tau)/tau_PI
+  - (4/3) PI/tau and the energy equation's e + P - PI, at orders 1, 2, 3.
 - k0 broadcast: after the root solve every eta slice carries slice k0's
   (vx, vy, ve), with u rebuilt from them at the solve's tau; e, P, T and the
   ghost cells are the unbroadcast solve's.
@@ -121,6 +125,68 @@ def check_bjorken(L, rk, viscous):
 @pytest.mark.parametrize("rk", [1, 2, 3])
 def test_bjorken_converges_at_the_rk_order(rk, viscous):
     check_bjorken(oracle_ctypes.lib(), rk, viscous)
+
+
+# the BULKTEST profile (NEWEST10/maindef.h:157-196): the BJORKEN flow with the
+# bulk source on, zeta = 1, tau_PI = 0.1 (initBulktest, init.h:356-413)
+ZETA, TPIB = 1.0, 0.1
+
+
+def bulk_exact():
+    """(e, Phi, PI) at TAU1 of
+        de/dtau   = -(e + P - PI)/tau + Phi/tau,
+        dPhi/dtau = as above,
+        dPI/dtau  = -(PI + zeta/tau)/tau_PI - (4/3) PI/tau"""
+    def rhs(t, y):
+        e, phi, PI = y
+        T = (e / FACTOR) ** 0.25
+        eta = ETA_C * (4.0 / 3.0) * e / T
+        return [-(e + e / 3.0 - PI) / t + phi / t,
+                -phi / TPI + (4.0 / 3.0) * eta / (t * TPI) - (4.0 / 3.0) * phi / t,
+                -(PI + ZETA / t) / TPIB - (4.0 / 3.0) * PI / t]
+
+    sol = solve_ivp(rhs, (TAU0, TAU1), [E0, 0.0, 0.0], method="Radau", rtol=1e-13, atol=1e-14)
+    return sol.y[:, -1]
+
+
+def bulk_run(L, rk, nsteps):
+    from viral_amd.desc import ZETA_CONST, TPIB_CONST, F_BULK
+    d = make_desc(4, 4, 3, rk=rk, eta_law=ETA_TIMES_S, eta_coef=ETA_C, tpi_law=TPI_CONST, tpi_coef=TPI, bulk=1,
+                  zeta_law=ZETA_CONST, zeta_coef=ZETA, tpib_law=TPIB_CONST, tpib_coef=TPIB)
+    n = d.cells()
+    e = np.full(n, E0)
+    u = np.zeros((4, n))
+    u[0] = 1.0
+    st = ic.assemble(d, e, e / 3.0, u, np.zeros((10, n)), u.copy(), TAU0, Temp=ic.conformal_T(d, e))
+    o = oracle_ctypes.Oracle(d, L)
+    o.set_state(st)
+    ts = (TAU1 - TAU0) / nsteps
+    for q in range(nsteps):
+        assert o.step(TAU0 + q * ts, ts) == OK
+    s = o.state().reshape((NFIELDS,) + d.ghost_dims())[(slice(None),) + d.interior_slices()]
+    for f in (F_EN, F_PI0 + 9, F_BULK):
+        assert np.ptp(s[f]) == 0.0, f
+    return s[F_EN].flat[0], -TAU1 * TAU1 * s[F_PI0 + 9].flat[0], s[F_BULK].flat[0]
+
+
+def check_bulk(L, rk):
+    ref = bulk_exact()
+    errs = []
+    for n in NSTEPS[1]:
+        got = bulk_run(L, rk, n)
+        errs.append(sum(abs(g - r) for g, r in zip(got, ref)) / ref[0])
+    orders = [np.log2(errs[q] / errs[q + 1]) for q in range(len(errs) - 1)]
+    print("bulk bjorken rk=%d errors %s orders %s (PI = %g)" % (rk, errs, orders, ref[2]))
+    for p in orders:
+        assert rk - 0.2 <= p <= rk + 0.4, (rk, errs, orders)
+    assert errs[-1] < {1: 3e-3, 2: 1e-5, 3: 5e-8}[rk], errs
+
+
+@pytest.mark.parametrize("rk", [1, 2, 3])
+def test_bulk_bjorken_converges_at_the_rk_order(rk):
+    """the bulk row (NEWEST10/Source.h:519-527) with the shear rows, the
+    energy equation's -(P - PI) and the root solve's A + P - PI"""
+    check_bulk(oracle_ctypes.lib(), rk)
 
 
 # ------------------------------------------------------------------ Gubser, all RK orders
@@ -316,6 +382,7 @@ MUTANTS = {
     8: ("k0 broadcast's u rebuilt without tau^2", check_k0_broadcast),
     9: ("GUBSER mirror without the reflection", check_mirror_fixture),
     10: ("GUBSER mirror of u^1 only", check_mirror_fixture),
+    11: ("bulk source without the 4/3 PI theta term", lambda L: check_bulk(L, 2)),
 }
 
 

@@ -4,7 +4,8 @@ harness binaries were not built (no reference tree); the committed golden
 fixtures (tests/test_golden.py) carry the same pins everywhere.
 
 Pinned here, per compile-time profile of NEWEST10/maindef.h:
-  CalcSource (geometric + velocity gradients + shear), NEWEST10/Source.h:195
+  CalcSource (geometric + velocity gradients + shear, and the bulk row of
+  the BULKTEST profile), NEWEST10/Source.h:195
   CalcCentreFlux + hydroExplicit (KT/WENO Result), NEWEST10/hydroKT.h:16,406
   boundary (ghost fill), NEWEST10/boundary.h:266
   s95p_p/a/s/T, NEWEST10/s95p.h
@@ -15,7 +16,7 @@ import pytest
 import oracle_ctypes
 import refdata
 
-PROFILES = ["ref_kt", "ref_kt_gubser", "ref_kt_ginit", "ref_shas", "ref_shas_ginit"]
+PROFILES = ["ref_kt", "ref_kt_gubser", "ref_kt_ginit", "ref_shas", "ref_shas_ginit", "ref_kt_bulktest"]
 
 
 def need(name):
@@ -42,12 +43,13 @@ def test_source_matches_reference(case):
     I = d.interior_slices()
     for l in range(15):
         a, b = S[l][I], Sr[l][I]
-        if l < 4 or l == 14:
-            # geometric rows are the reference's expressions verbatim
+        if l < 4 or (l == 14 and not d.bulk):
+            # geometric rows are the reference's expressions verbatim (and
+            # the bulk row is zero without -DBULK)
             assert np.array_equal(a, b), l
         else:
-            # shear rows: covariant tensor form vs the reference's expanded
-            # algebra, equal up to rounding
+            # shear rows and the bulk row: covariant form vs the reference's
+            # expanded algebra, equal up to rounding
             scale = np.abs(b).max()
             assert np.abs(a - b).max() <= 1e-14 * scale, (l, np.abs(a - b).max() / scale)
 

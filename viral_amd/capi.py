@@ -346,10 +346,12 @@ class Hydro:
         return r
 
     def ns_pi(self, tau, ts, ev=0):
-        """CalcNS: the Navier-Stokes pi of every ghosted cell, (10, cells)"""
+        """CalcNS: the Navier-Stokes pi of every ghosted cell, (10, cells);
+        (11, cells) with bulk, row 10 the Navier-Stokes PI"""
         n = self.descs[ev].cells()
-        out = np.zeros((10, n))
-        ptrs = (C.POINTER(C.c_double) * 10)(*[out[l].ctypes.data_as(C.POINTER(C.c_double)) for l in range(10)])
+        rows = 11 if self.descs[ev].bulk else 10
+        out = np.zeros((rows, n))
+        ptrs = (C.POINTER(C.c_double) * rows)(*[out[l].ctypes.data_as(C.POINTER(C.c_double)) for l in range(rows)])
         self._chk(self.L.viral_ns_init(self.h, ev, tau, ts, NS_OUTPUT, ptrs))
         return out
 

@@ -113,6 +113,8 @@ struct Params {
   unsigned quirks;
   double dx, dy, deta;
   double eta_coef, tpi_coef;
+  int bulk, zeta_law, tpib_law;  // bulk viscosity (viral_desc.bulk and its laws)
+  double zeta_coef, tpib_coef;
   double factor;    // conformal FACTOR = 3*DOF*pi^2/90 (NEWEST10/maindef.h:10)
   double t_freeze;
   int use_tab;      // fast mode with the tabulated s95p EOS
@@ -319,6 +321,15 @@ __host__ __device__ __forceinline__ double tpi_of(const Params& p, double eta, d
   if (p.tpi_law == 1) return p.tpi_coef * (eta / P);
   if (p.tpi_law == 2) return p.tpi_coef;
   return p.tpi_coef * eta / (e + P);
+}
+// FZeta / FtauPI laws of the -DBULK build, NEWEST10/maindef.h:82-83,140-141,191-192,289-290
+__host__ __device__ __forceinline__ double zeta_of(const Params& p, double s)
+{
+  return p.zeta_law == 1 ? p.zeta_coef * (s / (4.0 * PIE)) : p.zeta_coef;
+}
+__host__ __device__ __forceinline__ double tpib_of(const Params& p, double zeta, double P)
+{
+  return p.tpib_law == 1 ? p.tpib_coef * (zeta / P) : p.tpib_coef;
 }
 
 // ---------------------------------------------------------------- WENO5-JS
@@ -1020,7 +1031,35 @@ __device__ __forceinline__ int cart_gnewton(const Params& p, const double prm[10
   return iter;
 }
 
+// Israel-Stewart bulk relaxation source of PI in conservative form, the
+// physics of NEWEST10/Source.h:519-527 (Milne) and TXYZ5/Source.h:327-333:
+//   u.d PI = -(PI + zeta theta)/tau_PI - 4/3 PI theta,
+//   theta = d_mu u^mu (+ u^tau/tau in Milne),  S = u.d PI / u^0 + PI d_i(u^i/u^0).
+// du[a][b] = d_b u^a.  Exact mode follows the CPU restatement's operation order.
+template <bool FAST>
+__host__ __device__ __forceinline__ double bulk_source(int milne, double tau, const double u[4], double PI,
+                                                       const double du[4][4], double zeta, double tPI)
+{
+  const double iu0 = FAST ? frcp(u[0]) : 1.0 / u[0];
+  double divv = 0;
+#pragma unroll
+  for (int i = 1; i < 4; i++) divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
+  double theta = du[0][0] + du[1][1] + du[2][2] + du[3][3];
+  if (milne) theta = theta + u[0] / tau;
+  const double rhs = FAST ? -(PI + zeta * theta) * frcp(tPI) - (4.0 / 3.0) * PI * theta
+                          : -(PI + zeta * theta) / tPI - (4.0 / 3.0) * PI * theta;
+  return rhs * iu0 + PI * divv;
+}
+
 // ---------------------------------------------------------------- initial state
+// -zeta theta, the Navier-Stokes bulk pressure of CalcNS under BULK
+// (NEWEST10/Source.h:740-745), theta = d_mu u^mu + u^tau/tau (Milne).
+__host__ __device__ __forceinline__ double ns_bulk(double tau, const double u[4], const double du[4][4], double zeta)
+{
+  const double theta = du[0][0] + du[1][1] + du[2][2] + du[3][3] + u[0] / tau;
+  return -(zeta * theta);
+}
+
 // 2 eta sigma^{mn}, the Navier-Stokes shear stress of CalcNS
 // (NEWEST10/Source.h:643-751, written there as a computer-algebra expansion):
 // sigma^{mn} = Delta^{mn}_{ab} D^a u^b with the Milne Christoffel terms, in

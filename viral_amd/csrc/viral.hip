@@ -730,15 +730,23 @@ __device__ __forceinline__ void source_terms(const Geom& g, const Params& p, con
     double tpi = tpi_of(p, SH, P, e);
     double out[10];
     shear_source<FAST>(milne, tau, u, pi, du, SH, tpi, out);
+    double sb = 0.0;
+    if (p.bulk) {
+      const double BU = zeta_of(p, s);
+      sb = bulk_source<FAST>(milne, tau, u, PI, du, BU, tpib_of(p, BU, P));
+    }
     if (CART) {
+      // TXYZ5 evaluates the bulk row in the same slice loop (Source.h:272,327)
       const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)(x % g.gz) < g.kr - g.kl;
 #pragma unroll
       for (int l = 0; l < 5; l++) S[4 + l] = live ? out[cart5(l)] : 0.0;
 #pragma unroll
       for (int l = 5; l < 10; l++) S[4 + l] = 0.0;
+      S[14] = live ? sb : 0.0;
     } else {
 #pragma unroll
       for (int l = 0; l < 10; l++) S[4 + l] = out[l];
+      S[14] = sb;
     }
   }
 }
@@ -1319,11 +1327,12 @@ __device__ __forceinline__ void ns_du(const Geom& g, const Params& p, const doub
 
 // CalcNS / NSInit / ZeroInit (NEWEST10/Source.h:542-776) over every ghosted
 // cell, one thread per cell.  MODE 0 writes the Navier-Stokes pi to
-// out[10][n] and leaves the state alone; MODE 1 sets pi to it, MODE 2 to
-// zero, and both set PI = 0 (no BULK) and T^{tau mu} = (e + P - PI) u^tau
-// u^mu - (P - PI) g^{tau mu} + pi^{tau mu} in place (a cell writes only its
-// own pi, PI and T; the gradients read u and prevu).  Exact arithmetic and the
-// closed-form EOS in every mode, as the CPU restatement.
+// out[10][n] (and, with bulk, PI = -zeta theta to out[10]) and leaves the
+// state alone; MODE 1 sets pi and PI to them, MODE 2 both to zero, and both
+// set T^{tau mu} = (e + P - PI) u^tau u^mu - (P - PI) g^{tau mu} + pi^{tau mu}
+// in place (a cell writes only its own pi, PI and T; the gradients read u and
+// prevu).  Exact arithmetic and the closed-form EOS in every mode, as the CPU
+// restatement.
 template <int MODE>
 __global__ void __launch_bounds__(BLK) k_ns_init(Geom g, Params p, double tau, double ts, double* __restrict__ st,
                                                  double* __restrict__ out)
@@ -1332,7 +1341,7 @@ __global__ void __launch_bounds__(BLK) k_ns_init(Geom g, Params p, double tau, d
   if (x >= g.n) return;
   const long n = g.n;
   const int i = (int)(x / g.sx), j = (int)((x / g.sy) % g.gy), k = (int)(x % g.gz);
-  double pn[10];
+  double pn[10], nsPI = 0.0;
   for (int l = 0; l < 10; l++) pn[l] = 0.0;
   if (MODE != 2) {
     double du[4][4], u[4];
@@ -1342,14 +1351,16 @@ __global__ void __launch_bounds__(BLK) k_ns_init(Geom g, Params p, double tau, d
     const double T = eos_T(p, e);
     const double s = eos_s(p, e, P, T);
     ns_shear(tau, u, du, eta_of(p, s), pn);
+    if (p.bulk) nsPI = ns_bulk(tau, u, du, zeta_of(p, s));
   }
   if (MODE == 0) {
     for (int l = 0; l < 10; l++) out[l * n + x] = pn[l];
+    if (p.bulk) out[10 * n + x] = nsPI;
     return;
   }
   for (int l = 0; l < 10; l++) st[(VIRAL_F_PI0 + l) * n + x] = pn[l];
-  st[VIRAL_F_BULK * n + x] = 0.0;
-  const double e = st[VIRAL_F_EN * n + x], P = st[VIRAL_F_P * n + x], PI = 0.0;
+  st[VIRAL_F_BULK * n + x] = nsPI;
+  const double e = st[VIRAL_F_EN * n + x], P = st[VIRAL_F_P * n + x], PI = nsPI;
   const double u0 = st[VIRAL_F_U0 * n + x];
   st[VIRAL_F_T00 * n + x] = -P + PI + (e + P - PI) * (u0 * u0) + pn[0];
   for (int l = 1; l < 4; l++) st[(VIRAL_F_T00 + l) * n + x] = (e + P - PI) * u0 * st[(VIRAL_F_U0 + l) * n + x] + pn[l];
@@ -1361,7 +1372,7 @@ __global__ void __launch_bounds__(BLK) k_ns_init(Geom g, Params p, double tau, d
 // ghosted cell: the Landau frame from landau_cell (FindVar's selection,
 // ginit.h:521-629), then e, Temp, P, v and u; AT_START false (the call at
 // tau - ts) stores prevu = u; AT_START true sets pi = T - T_ideal - PI
-// (g^{-1} - u u) with PI = P - e/3, then PI = 0 (no BULK) and T^{tau mu}.  A
+// (g^{-1} - u u) with PI = P - e/3, then PI = 0 (kept with bulk) and T^{tau mu}.  A
 // cell without exactly one passing eigenvector ("EIGEN VALUE SEARCH FAILED",
 // where the reference exits) keeps its state, gets VIRAL_FLAG_SOLVER and
 // competes for the first bad cell.  Exact arithmetic and the closed-form EOS.
@@ -1409,8 +1420,8 @@ __global__ void __launch_bounds__(BLK) k_landau(Geom g, Params p, double tau, co
       pn[pidx(a, b)] = T[pidx(a, b)] - tid[a][b] - PImat;
     }
   for (int l = 0; l < 10; l++) st[(VIRAL_F_PI0 + l) * n + x] = pn[l];
-  st[VIRAL_F_BULK * n + x] = 0.0;
-  const double PI = 0.0;
+  const double PI = p.bulk ? PIt : 0.0;  // ginit.h:996-999 zeroes it without BULK
+  st[VIRAL_F_BULK * n + x] = PI;
   st[VIRAL_F_T00 * n + x] = -P + PI + (e + P - PI) * (u0 * u0) + pn[0];
   for (int l = 1; l < 4; l++) st[(VIRAL_F_T00 + l) * n + x] = (e + P - PI) * u0 * u[l] + pn[l];
 }
@@ -2577,6 +2588,11 @@ void viral_desc_default(viral_desc* d, int nx, int ny, int nz)
   d->t_freeze = 0.0;
   d->fast_math = 0;
   d->eos_table = 0;
+  d->bulk = 0;
+  d->zeta_law = VIRAL_ZETA_CONST;
+  d->zeta_coef = 1.0;
+  d->tpib_law = VIRAL_TPIB_CONST;
+  d->tpib_coef = 0.1;
 }
 
 void viral_ghost_dims(const viral_desc* d, int* gx, int* gy, int* gz)
@@ -2620,6 +2636,9 @@ static int check_desc(viral_ctx* ctx, const viral_desc* d)
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "boost invariance needs Milne coordinates");
   if (d->eos != VIRAL_EOS_CONFORMAL && d->eos != VIRAL_EOS_S95P) return fail(ctx, VIRAL_ERR_ARG, "unknown eos");
   if (d->max_iter < 1) return fail(ctx, VIRAL_ERR_ARG, "max_iter must be >= 1");
+  if (d->bulk && (d->zeta_law < VIRAL_ZETA_CONST || d->zeta_law > VIRAL_ZETA_S_OVER_4PI ||
+                  d->tpib_law < VIRAL_TPIB_CONST || d->tpib_law > VIRAL_TPIB_OVER_P))
+    return fail(ctx, VIRAL_ERR_ARG, "unknown bulk viscosity or bulk relaxation law");
   if (d->eos_table && !d->fast_math)
     return fail(ctx, VIRAL_ERR_ARG, "the tabulated EOS is a fast-mode option (exact mode evaluates the closed form)");
   int gx, gy, gz;
@@ -2695,6 +2714,11 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     p.dx = d->dx; p.dy = d->dy; p.deta = d->deta;
     p.eta_coef = d->eta_coef;
     p.tpi_coef = d->tpi_coef;
+    p.bulk = d->bulk;
+    p.zeta_law = d->zeta_law;
+    p.tpib_law = d->tpib_law;
+    p.zeta_coef = d->zeta_coef;
+    p.tpib_coef = d->tpib_coef;
     p.factor = 3.0 * d->dof * PIE * PIE / 90.0;
     p.t_freeze = d->t_freeze;
     p.use_tab = 0;
@@ -2998,17 +3022,18 @@ int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, doubl
   if (!E->p.milne) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the Navier-Stokes initialisation is Milne only");
   if (mode < VIRAL_NS_OUTPUT || mode > VIRAL_ZERO_INIT || !(ts > 0) || !(tau > 0))
     return fail(ctx, VIRAL_ERR_ARG, "bad mode, tau or ts");
+  const int nout = E->p.bulk ? 11 : 10;  // with bulk, nspi[10] receives the Navier-Stokes PI
   if (mode == VIRAL_NS_OUTPUT) {
-    if (!nspi) return fail(ctx, VIRAL_ERR_ARG, "VIRAL_NS_OUTPUT needs 10 output arrays");
-    for (int l = 0; l < 10; l++)
-      if (!nspi[l]) return fail(ctx, VIRAL_ERR_ARG, "VIRAL_NS_OUTPUT needs 10 output arrays");
+    if (!nspi) return fail(ctx, VIRAL_ERR_ARG, "VIRAL_NS_OUTPUT needs 10 output arrays (11 with bulk)");
+    for (int l = 0; l < nout; l++)
+      if (!nspi[l]) return fail(ctx, VIRAL_ERR_ARG, "VIRAL_NS_OUTPUT needs 10 output arrays (11 with bulk)");
   }
   const long n = E->g.n;
   double* st = E->st[ctx->cur];
   const unsigned grid = nblocks(n);
   if (mode != VIRAL_NS_OUTPUT) ctx->bmax_ok[ev] = 0;
   if (mode == VIRAL_NS_OUTPUT) {
-    const size_t m = 10 * (size_t)n;
+    const size_t m = (size_t)nout * n;
     if (m > ctx->out_cap) {
       HIPCHK(hipStreamSynchronize(ctx->stream));
       if (ctx->d_out) HIPCHK(hipFree(ctx->d_out));
@@ -3019,7 +3044,7 @@ int viral_ns_init(viral_ctx* ctx, int ev, double tau, double ts, int mode, doubl
     }
     hipLaunchKernelGGL(k_ns_init<0>, dim3(grid), dim3(BLK), 0, ctx->stream, E->g, E->p, tau, ts, st, ctx->d_out);
     HIPCHK(hipGetLastError());
-    for (int l = 0; l < 10; l++)
+    for (int l = 0; l < nout; l++)
       HIPCHK(hipMemcpyAsync(nspi[l], ctx->d_out + (size_t)l * n, n * sizeof(double), hipMemcpyDeviceToHost,
                             ctx->stream));
   } else if (mode == VIRAL_NS_INIT) {
@@ -3477,6 +3502,7 @@ int viral_adjoint_begin(viral_ctx* ctx, int every)
   if (E.d.coords != VIRAL_MILNE || E.d.scheme != VIRAL_KT)
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint covers the Milne KT step");
   if (E.d.quirks & VIRAL_Q_FIX_VE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint does not cover the FIX build");
+  if (E.d.bulk) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint does not cover the bulk source");
   if ((E.d.quirks & VIRAL_Q_K0_BROADCAST) && !E.d.lbi && (E.g.k0 < E.g.kl || E.g.k0 >= E.g.kr))
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint needs k0 inside the interior");
   if (int r = viral_adjoint_end(ctx)) return r;

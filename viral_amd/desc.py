@@ -11,6 +11,8 @@ KT, SHASTA = 0, 1
 EOS_CONFORMAL, EOS_S95P = 0, 1
 ETA_TIMES_S, ETA_S_OVER_4PI = 0, 1
 TPI_OVER_EP, TPI_OVER_P, TPI_CONST = 0, 1, 2
+ZETA_CONST, ZETA_S_OVER_4PI = 0, 1
+TPIB_CONST, TPIB_OVER_P = 0, 1
 
 Q_GHOST_EIGEN = 1 << 0
 Q_K0_BROADCAST = 1 << 1
@@ -64,6 +66,9 @@ class ViralDesc(C.Structure):
         ("t_freeze", C.c_double),
         ("fast_math", C.c_int),
         ("eos_table", C.c_int),
+        ("bulk", C.c_int),
+        ("zeta_law", C.c_int), ("zeta_coef", C.c_double),
+        ("tpib_law", C.c_int), ("tpib_coef", C.c_double),
     ]
 
     def ghost_dims(self):
@@ -113,6 +118,10 @@ def make_desc(nx, ny, nz, **kw):
     d.t_freeze = 0.0
     d.fast_math = 0
     d.eos_table = 0
+    # bulk off; when on, the BULKTEST profile's zeta = 1, tau_PI = 0.1 (NEWEST10/maindef.h:190-193)
+    d.bulk = 0
+    d.zeta_law, d.zeta_coef = ZETA_CONST, 1.0
+    d.tpib_law, d.tpib_coef = TPIB_CONST, 0.1
     for k, v in kw.items():
         if not hasattr(d, k):
             raise AttributeError(k)

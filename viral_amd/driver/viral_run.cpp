@@ -22,9 +22,14 @@
  *            and prevu from the Gubser flow as init.h:470-530 sets them
  *   glauber  optical Glauber overlap with the plateau eta profile (the shape
  *            of NEWEST10/ginit.h's input; ginit itself needs GSL)
- *   bjorken  uniform e = 30 GeV/fm^3 at rest (NEWEST10/init.h:293-353)
+ *   bjorken  uniform e = 30 GeV/fm^3 at rest (NEWEST10/init.h:293-353; the
+ *            BULKTEST profile's initBulktest, init.h:356-413, is the same state)
  *
- * Output files: <out>/tau%2.3ffm.bin, the reference's records (write.h).
+ * Output files: <out>/tau%2.3ffm.bin, the reference's records (write.h).  The
+ * bjorken and bulktest profiles also write temp.txt (main.cpp:124-141): one
+ * line "tau  value" before every step, value the centre cell's
+ * (HydroGrid[XCM/2][YCM/2][ZCMA/2]) temperature in GeV (BJORKEN) or bulk
+ * pressure PI (BULKTEST), in the stream's default 6-digit format.
  */
 #include <cmath>
 #include <cstdio>
@@ -62,21 +67,24 @@ struct Options {
   int write = 1;
   std::string dump_state;    // write the initial 29-field state here
   int dry_run = 0;           // initial state and first output only, no device
+  int temp_kind = 0;         // temp.txt: 0 none, 1 centre T in GeV (BJORKEN), 2 centre PI (BULKTEST)
+  std::string temp_file = "temp.txt";
 };
 
 void usage()
 {
   std::printf(
-      "usage: viral_run [--profile default|gubser|ginit|bjorken] [options]\n"
+      "usage: viral_run [--profile default|gubser|ginit|bjorken|bulktest] [options]\n"
       "  grid:      --nx N --ny N --nz N --lbi 0|1 --dx F --dy F --deta F --coords milne|cartesian\n"
       "  step:      --rk 1|2|3 --scheme kt|shasta --fast 0|1 --eos conformal|s95p --eos-table 0|1\n"
       "             --eta-law times_s|s_over_4pi --eta-coef F --tpi-law over_ep|over_p|const --tpi-coef F\n"
       "             --quirks BITS --fix 0|1 --max-iter N\n"
+      "             --bulk 0|1 --zeta-law const|s_over_4pi --zeta-coef F --tpib-law const|over_p --tpib-coef F\n"
       "  time:      --tau0 F --dtau F --max-steps N --tau-stop F --tmin-mev F\n"
       "  init:      --init initvar|gubser|glauber|bjorken --gubser-file PATH --gubser-rho0 F --gubser-that0 F\n"
       "             --b F --e-peak F --plateau F --falloff F\n"
       "  output:    --out DIR --pfreq F --freq N --freqz N --write 0|1 --checks 0|1\n"
-      "             --dump-state PATH --dry-run\n");
+      "             --dump-state PATH --dry-run --temp-file PATH --temp none|t|pi\n");
 }
 
 // maindef.h profiles as presets
@@ -118,6 +126,15 @@ void set_profile(Options& o, const std::string& name)
     o.tau0 = 0.6; o.ts = 0.01; o.pfreq = 0.2; o.freq = 1; o.freqz = 1;
     o.init = "bjorken";
     o.tau_stop_profile = 10.0;
+    o.temp_kind = 1;
+  } else if (name == "bulktest") {  // :157-196, the bjorken grid with -DBULK (zeta = 1, tau_PI = 0.1)
+    set_profile(o, "bjorken");
+    o.profile = name;
+    d.bulk = 1;
+    d.zeta_law = VIRAL_ZETA_CONST; d.zeta_coef = 1.0;
+    d.tpib_law = VIRAL_TPIB_CONST; d.tpib_coef = 0.1;
+    o.pfreq = 0.1; o.freq = 2; o.freqz = 1;
+    o.temp_kind = 2;
   } else {
     std::fprintf(stderr, "unknown profile %s\n", name.c_str());
     std::exit(2);
@@ -160,6 +177,13 @@ bool parse(int argc, char** argv, Options& o)
     else if (k == "--quirks") d.quirks = (unsigned)std::strtoul(v.c_str(), nullptr, 0);
     else if (k == "--fix") d.quirks = n ? (d.quirks | VIRAL_Q_FIX_VE) : (d.quirks & ~VIRAL_Q_FIX_VE);  // -DFIX
     else if (k == "--max-iter") d.max_iter = n;
+    else if (k == "--bulk") d.bulk = n;
+    else if (k == "--zeta-law") d.zeta_law = v == "const" ? VIRAL_ZETA_CONST : VIRAL_ZETA_S_OVER_4PI;
+    else if (k == "--zeta-coef") d.zeta_coef = f;
+    else if (k == "--tpib-law") d.tpib_law = v == "const" ? VIRAL_TPIB_CONST : VIRAL_TPIB_OVER_P;
+    else if (k == "--tpib-coef") d.tpib_coef = f;
+    else if (k == "--temp-file") o.temp_file = v;
+    else if (k == "--temp") o.temp_kind = v == "t" ? 1 : v == "pi" ? 2 : 0;
     else if (k == "--tau0") o.tau0 = f;
     else if (k == "--dtau") o.ts = f;
     else if (k == "--max-steps") o.max_steps = n;
@@ -649,10 +673,37 @@ int main(int argc, char** argv)
   if (o.checks) report_checkroot(ctx, d, tau);
   if (!output(tau)) return 1;
 
+  // temp.txt of the BJORKEN / BULKTEST builds (main.cpp:124-141)
+  FILE* temp = nullptr;
+  std::vector<double> fbuf;
+  const size_t centre = g.at(g.gx / 2, g.gy / 2, d.lbi ? 0 : d.nz / 2);
+  if (o.temp_kind) {
+    temp = std::fopen(o.temp_file.c_str(), "w");
+    if (!temp) {
+      std::fprintf(stderr, "cannot open %s\n", o.temp_file.c_str());
+      return 1;
+    }
+    fbuf.resize(g.n);
+  }
+  auto temp_line = [&](double t) -> bool {
+    if (!temp) return true;
+    const int f = o.temp_kind == 1 ? VIRAL_F_EN : VIRAL_F_BULK;
+    if (viral_download_field(ctx, 0, f, fbuf.data()) != VIRAL_OK) return false;
+    double v = fbuf[centre];
+    if (o.temp_kind == 1) {  // fmtoMev(FT(En))/1000: T in GeV (maindef.h:481-485)
+      double p, a, T, s_;
+      if (viral_eos_eval(&d, 1, &v, &p, &a, &T, &s_) != VIRAL_OK) return false;
+      v = (1000 * (T * GEVFM)) / 1000;
+    }
+    std::fprintf(temp, "%g  %g\n", t, v);
+    return true;
+  };
+
   const clock_t start = std::clock();
   double tauPrint = o.tau0;
   int rc = 0;
   for (long l = 1;; l++) {
+    if (!temp_line(tau)) { rc = 1; break; }
     if (o.checks) report_physics(ctx, g, tau);
     const int r = viral_step(ctx, tau, ts);
     if (r != VIRAL_OK) {
@@ -688,6 +739,7 @@ int main(int argc, char** argv)
     if (tau > o.tau_stop) break;
     if (o.max_steps >= 0 && l >= o.max_steps) break;
   }
+  if (temp) std::fclose(temp);
   viral_host_free(rec);
   viral_destroy(ctx);
   return rc;

@@ -79,11 +79,11 @@ def cart_pi10(u, p):
     return [p1 + p2 + zz, tx, ty, tz, p1, p3, p4, p2, p5, zz]
 
 
-def cartesian_state(desc, e, P, u, p5, prevu, Temp=None):
+def cartesian_state(desc, e, P, u, p5, prevu, Temp=None, PI=None):
     """29-field Cartesian state: T^{t mu} from the full pi, the five evolved
     components in pi slots 0..4, slots 5..9 zero."""
     full = cart_pi10(u, p5)
-    st = assemble(desc, e, P, u, full, prevu, 1.0, Temp=Temp)
+    st = assemble(desc, e, P, u, full, prevu, 1.0, Temp=Temp, PI=PI)
     for l in range(5):
         st[F_PI0 + l] = np.asarray(p5[l]).ravel()
     for l in range(5, 10):
@@ -121,10 +121,12 @@ def gubser(desc, tau, ts, q=1.0, eps0=1.0):
     return assemble(desc, e, P, u, pi, prevu, tau, Temp=conformal_T(desc, e))
 
 
-def random_smooth(desc, seed, tau, e_scale=4.0, v_scale=0.3, pi_frac=0.05, prevu_jitter=0.01, eos_p=None):
+def random_smooth(desc, seed, tau, e_scale=4.0, v_scale=0.3, pi_frac=0.05, prevu_jitter=0.01, eos_p=None,
+                  bulk_frac=0.0):
     """A seeded smooth random state with non-zero shear stress, used by the
     parity tests (values of the kind the step sees: e of a few fm^-4,
-    |v| below ~0.5, |pi| a few percent of e)."""
+    |v| below ~0.5, |pi| a few percent of e; bulk_frac > 0 adds a bulk
+    pressure PI of that fraction of e)."""
     rng = np.random.default_rng(seed)
     gx, gy, gz = desc.ghost_dims()
     i, j, k = np.meshgrid(np.arange(gx), np.arange(gy), np.arange(gz), indexing="ij")
@@ -143,12 +145,15 @@ def random_smooth(desc, seed, tau, e_scale=4.0, v_scale=0.3, pi_frac=0.05, prevu
     u = [u0, u1, u2, u3]
     prevu = [c * (1.0 + prevu_jitter * rng.standard_normal(e.shape)) for c in u]
     pi = [pi_frac * e * rng.standard_normal(e.shape) for _ in range(10)]
+    PI = None
+    if bulk_frac:
+        PI = bulk_frac * e * (np.sin(0.19 * i + ph[6]) * np.cos(0.23 * j + ph[7]) + 0.1 * rng.standard_normal(e.shape))
     if desc.coords != MILNE:
-        return cartesian_state(desc, e, P, u, pi[:5], prevu)
+        return cartesian_state(desc, e, P, u, pi[:5], prevu, PI=PI)
     for l in (3, 6, 8):
         pi[l] = pi[l] / t3
     pi[9] = pi[9] / (t3 * t3)
-    return assemble(desc, e, P, u, pi, prevu, tau)
+    return assemble(desc, e, P, u, pi, prevu, tau, PI=PI)
 
 
 def _thickness(xs, ys, R=6.62, a=0.546, nz=801, zmax=20.0):
