@@ -40,6 +40,7 @@
 #include "oracle.h"
 #include "vmath.h"
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -58,6 +59,14 @@
 #define OR_MUTANT 0
 #endif
 static const int MUT = OR_MUTANT;
+
+/* Work-sharing of the per-cell loops for the all-core CPU baseline (built
+ * with -fopenmp; the test build has no -fopenmp and runs them in order).
+ * Every loop marked OMP_FOR writes only its own cells from inputs no other
+ * iteration writes, so its results do not depend on the thread count; the
+ * first failing cell is a min-reduction over cell indices, which grow in the
+ * reference's loop order. */
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
 
 struct or_ctx {
   viral_desc d;
@@ -914,6 +923,7 @@ void or_calc_source(or_ctx* c, double tau, double ts, double* S)
   double tau2 = tau * tau;
   size_t n = c->n;
   double** f = c->f;
+  OMP_FOR
   for (int i = c->il; i < c->ir; i++)
     for (int j = c->jl; j < c->jr; j++)
       for (int k = c->kl; k < c->kr; k++) {
@@ -939,6 +949,7 @@ void or_calc_source(or_ctx* c, double tau, double ts, double* S)
     const double hh[3] = {d->dx, d->dy, d->deta};
     for (int dir = 0; dir < ndir; dir++) {
       size_t st = dir == 0 ? IX(c, 1, 0, 0) : dir == 1 ? IX(c, 0, 1, 0) : 1;
+      OMP_FOR
       for (int i = c->il; i < c->ir; i++)
         for (int j = c->jl; j < c->jr; j++)
           for (int k = c->kl; k < c->kr; k++) {
@@ -976,6 +987,7 @@ void or_calc_source(or_ctx* c, double tau, double ts, double* S)
   }
   /* velocity gradients, NEWEST10/Source.h:3-101 */
   const double hx[3] = {d->dx, d->dy, d->deta};
+  OMP_FOR
   for (int i = c->il; i < c->ir; i++)
     for (int j = c->jl; j < c->jr; j++)
       for (int k = c->kl; k < c->kr; k++) {
@@ -1028,6 +1040,7 @@ static void centre_flux(or_ctx* c, int dir, double tau, double* F)
   const int milne = c->d.coords == VIRAL_MILNE;
   double** f = c->f;
   size_t n = c->n;
+  OMP_FOR
   for (size_t x = 0; x < n; x++) {
     double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x], PI = f[VIRAL_F_BULK][x];
     double u[4] = {f[VIRAL_F_U0][x], f[VIRAL_F_U1][x], f[VIRAL_F_U2][x], f[VIRAL_F_U3][x]};
@@ -1071,6 +1084,7 @@ static void weno_rows(or_ctx* c, int dir, const double* q, double* L, double* R)
   if (dir == 0) { a0 = c->jl; a1 = c->jr; b0 = c->kl; b1 = c->kr; }
   else if (dir == 1) { a0 = c->il; a1 = c->ir; b0 = c->kl; b1 = c->kr; }
   else { a0 = c->il; a1 = c->ir; b0 = c->jl; b1 = c->jr; }
+  OMP_FOR
   for (int a = a0; a < a1; a++)
     for (int b = b0; b < b1; b++) {
       size_t base = dir == 0 ? IX(c, 0, a, b) : dir == 1 ? IX(c, a, 0, b) : IX(c, a, b, 0);
@@ -1100,6 +1114,7 @@ static void shasta_explicit(or_ctx* c, double tau, double taustep, double* Resul
   size_t n = c->n;
   double** f = c->f;
   memset(Result, 0, SVAR * n * sizeof(double));
+  OMP_FOR
   for (size_t x = 0; x < n; x++) {
     double tf = milne ? tau : 1.0;
     for (int mu = 0; mu < 4; mu++) c->var[mu * n + x] = milne ? tf * f[VIRAL_F_T00 + mu][x] : f[VIRAL_F_T00 + mu][x];
@@ -1113,6 +1128,7 @@ static void shasta_explicit(or_ctx* c, double tau, double taustep, double* Resul
   double* AD = c->FR;   /* [n] */
   for (int dir = 0; dir < ndir; dir++) {
     /* CalcNumVel: row 0 moves with T^{dir,tau}/T^{tau tau}, the rest with u^dir/u^0 */
+    OMP_FOR
     for (size_t x = 0; x < n; x++) {
       double e = f[VIRAL_F_EN][x], P = f[VIRAL_F_P][x], PI = f[VIRAL_F_BULK][x];
       double u0 = f[VIRAL_F_U0][x], ud = f[VIRAL_F_U1 + dir][x];
@@ -1132,6 +1148,7 @@ static void shasta_explicit(or_ctx* c, double tau, double taustep, double* Resul
     for (int l = 0; l < SVAR; l++) {
       const double* U = c->var + l * n;
       const double* V = NV + l * n;
+      OMP_FOR
       for (int a = a0; a < a1; a++)
         for (int b = b0; b < b1; b++) {
           size_t base = dir == 0 ? IX(c, 0, a, b) : dir == 1 ? IX(c, a, 0, b) : IX(c, a, b, 0);
@@ -1175,6 +1192,7 @@ void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* Result)
   double** f = c->f;
   /* ClearResultVariable + CopyPrimaryVariablesToVar (all cells) */
   memset(Result, 0, SVAR * n * sizeof(double));
+  OMP_FOR
   for (size_t x = 0; x < n; x++) {
     double tf = milne ? tau : 1.0;
     for (int mu = 0; mu < 4; mu++) c->var[mu * n + x] = milne ? tf * f[VIRAL_F_T00 + mu][x] : f[VIRAL_F_T00 + mu][x];
@@ -1186,6 +1204,7 @@ void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* Result)
   for (int dir = 0; dir < ndir; dir++) {
     /* eigenvalue u^dir/u^0: interior only under q1 (ghost entries stay 0) */
     memset(c->A, 0, n * sizeof(double));
+    OMP_FOR
     for (int i = 0; i < c->gx; i++)
       for (int j = 0; j < c->gy; j++)
         for (int k = 0; k < c->gz; k++) {
@@ -1212,6 +1231,7 @@ void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* Result)
     memset(c->AmL, 0, n * sizeof(double));
     memset(c->AmR, 0, n * sizeof(double));
     int lo_ext = (d->quirks & VIRAL_Q_GHOST_EIGEN) ? 0 : 1;
+    OMP_FOR
     for (int i = c->il - (dir == 0 ? lo_ext : 0); i < c->ir; i++)
       for (int j = c->jl - (dir == 1 ? lo_ext : 0); j < c->jr; j++)
         for (int k = c->kl - (dir == 2 ? lo_ext : 0); k < c->kr; k++) {
@@ -1225,6 +1245,7 @@ void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* Result)
     size_t st = dir == 0 ? IX(c, 1, 0, 0) : dir == 1 ? IX(c, 0, 1, 0) : 1;
     /* face fluxes at faces p+1/2 for p = lo-1 .. hi-1 */
     for (int l = 0; l < SVAR; l++)
+      OMP_FOR
       for (int i = c->il - (dir == 0); i < c->ir; i++)
         for (int j = c->jl - (dir == 1); j < c->jr; j++)
           for (int k = c->kl - (dir == 2); k < c->kr; k++) {
@@ -1233,6 +1254,7 @@ void or_hydro_explicit(or_ctx* c, double tau, double taustep, double* Result)
                                  0.5 * (fmax(c->AmL[x], c->AmR[x])) * (c->VR[l * n + x] - c->VL[l * n + x]);
           }
     for (int l = 0; l < SVAR; l++)
+      OMP_FOR
       for (int i = c->il; i < c->ir; i++)
         for (int j = c->jl; j < c->jr; j++)
           for (int k = c->kl; k < c->kr; k++) {
@@ -1537,6 +1559,7 @@ static int cart_root(or_ctx* c, int32_t* iters, uint8_t* flags)
   const viral_desc* d = &c->d;
   double** f = c->f;
   int bad = -1;
+  OMP_FOR
   for (int i = c->il; i < c->ir; i++)
     for (int j = c->jl; j < c->jr; j++)
       for (int k = c->kl; k < c->kr; k++) {
@@ -1556,6 +1579,8 @@ static int cart_root(or_ctx* c, int32_t* iters, uint8_t* flags)
         if (iters) iters[x] = it;
         if (flags) flags[x] = (uint8_t)fl;
       }
+  int badx = INT_MAX;
+  _Pragma("omp parallel for schedule(static) reduction(min : badx)")
   for (int i = c->il; i < c->ir; i++)
     for (int j = c->jl; j < c->jr; j++)
       for (int k = c->kl; k < c->kr; k++) {
@@ -1571,8 +1596,9 @@ static int cart_root(or_ctx* c, int32_t* iters, uint8_t* flags)
         f[VIRAL_F_P][x] = or_eos_p(d, En);
         if (d->t_freeze > 0 && f[VIRAL_F_TEMP][x] < d->t_freeze) fl |= VIRAL_FLAG_FREEZEOUT;
         if (flags) flags[x] = (uint8_t)fl;
-        if (bad < 0 && (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER))) bad = (int)x;
+        if ((fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER)) && (int)x < badx) badx = (int)x;
       }
+  bad = badx == INT_MAX ? -1 : badx;
   return bad;
 }
 
@@ -1586,6 +1612,7 @@ int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)
   c->bmax_valid = 0;
   double t2 = milne ? tau * tau : 1.0;
   int bad = -1;
+  OMP_FOR
   for (int i = c->il; i < c->ir; i++)
     for (int j = c->jl; j < c->jr; j++)
       for (int k = c->kl; k < c->kr; k++) {
@@ -1613,6 +1640,8 @@ int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)
         if (iters) iters[x] = it;
         if (flags) flags[x] = (uint8_t)fl;
       }
+  int badx = INT_MAX;
+  _Pragma("omp parallel for schedule(static) reduction(min : badx)")
   for (int i = c->il; i < c->ir; i++)
     for (int j = c->jl; j < c->jr; j++)
       for (int k = c->kl; k < c->kr; k++) {
@@ -1637,9 +1666,11 @@ int or_root(or_ctx* c, double tau, int32_t* iters, uint8_t* flags)
         f[VIRAL_F_VE][x] = ve;
         if (d->t_freeze > 0 && f[VIRAL_F_TEMP][x] < d->t_freeze) fl |= VIRAL_FLAG_FREEZEOUT;
         if (flags) flags[x] = (uint8_t)fl;
-        if (bad < 0 && (fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER))) bad = (int)x;
+        if ((fl & (VIRAL_FLAG_ACAUSAL | VIRAL_FLAG_ENEG | VIRAL_FLAG_SOLVER)) && (int)x < badx) badx = (int)x;
       }
+  bad = badx == INT_MAX ? -1 : badx;
   if ((d->quirks & VIRAL_Q_K0_BROADCAST) && !d->lbi && milne) {
+    OMP_FOR
     for (int i = c->il; i < c->ir; i++)
       for (int j = c->jl; j < c->jr; j++)
         for (int k = c->kl; k < c->kr; k++) {
@@ -1675,6 +1706,7 @@ void or_boundary(or_ctx* c)
 {
   double** f = c->f;
   const int mirror = (c->d.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && c->d.coords == VIRAL_MILNE;
+  OMP_FOR
   for (int j = c->jl; j < c->jr; j++)
     for (int k = c->kl; k < c->kr; k++)
       for (int g = 0; g < B; g++) {
@@ -1692,6 +1724,7 @@ void or_boundary(or_ctx* c)
           f[VIRAL_F_U2][xr] = f[VIRAL_F_U2][mr];
         }
       }
+  OMP_FOR
   for (int i = c->il; i < c->ir; i++)
     for (int k = c->kl; k < c->kr; k++)
       for (int g = 0; g < B; g++) {
@@ -1710,6 +1743,7 @@ void or_boundary(or_ctx* c)
         }
       }
   if (!c->d.lbi)
+    OMP_FOR
     for (int i = c->il; i < c->ir; i++)
       for (int j = c->jl; j < c->jr; j++)
         for (int g = 0; g < B; g++) {
@@ -1742,6 +1776,7 @@ static void update_primary(or_ctx* c, const double* V, double tau)
 {
   const int milne = c->d.coords == VIRAL_MILNE;
   size_t n = c->n;
+  OMP_FOR
   for (int i = c->il; i < c->ir; i++)
     for (int j = c->jl; j < c->jr; j++)
       for (int k = c->kl; k < c->kr; k++) {
@@ -1754,6 +1789,7 @@ static void update_primary(or_ctx* c, const double* V, double tau)
 
 static void update_prevu(or_ctx* c)
 {
+  OMP_FOR
   for (int i = c->il; i < c->ir; i++)
     for (int j = c->jl; j < c->jr; j++)
       for (int k = c->kl; k < c->kr; k++) {
@@ -1769,6 +1805,7 @@ static void stage_rhs(or_ctx* c, double tau_s, double ts_src, double taustep)
   or_hydro_explicit(c, tau_s, taustep, c->res);
   size_t n = c->n;
   for (int l = 0; l < SVAR; l++)
+    OMP_FOR
     for (int i = c->il; i < c->ir; i++)
       for (int j = c->jl; j < c->jr; j++)
         for (int k = c->kl; k < c->kr; k++) {
@@ -1778,13 +1815,15 @@ static void stage_rhs(or_ctx* c, double tau_s, double ts_src, double taustep)
 }
 
 #define FORI for (int i = c->il; i < c->ir; i++) for (int j = c->jl; j < c->jr; j++) for (int k = c->kl; k < c->kr; k++)
+/* FORI with the iterations shared among threads: for loops whose cells are independent */
+#define FORI_PAR OMP_FOR FORI
 
 static int stage_root(or_ctx* c, double tau_r, int stage)
 {
   uint8_t* fl = (uint8_t*)malloc(c->n);
   memset(fl, 0, c->n);
   int bad = or_root(c, tau_r, c->iters[stage], fl);
-  FORI {
+  FORI_PAR {
     size_t x = IX(c, i, j, k);
     c->flags[x] = (uint8_t)((c->flags[x] & ~VIRAL_FLAG_FREEZEOUT) | fl[x]);
   }
@@ -1851,7 +1890,7 @@ static int step_core(or_ctx* c, double tau, double ts)
   /* stage 1 */
   stage_rhs(c, tau, ts, sub);
   memcpy(c->var0, c->var, SVAR * n * sizeof(double)); /* CalcVar0 (interior entries used) */
-  for (int l = 0; l < SVAR; l++) FORI {
+  for (int l = 0; l < SVAR; l++) FORI_PAR {
       size_t x = l * n + IX(c, i, j, k);
       c->var1[x] = c->var0[x] + ts * (L[x]);
     }
@@ -1863,7 +1902,7 @@ static int step_core(or_ctx* c, double tau, double ts)
   stage_rhs(c, tau + ts, ts, sub);
   if (d->rk == 2) {
     const double w2 = MUT == 5 ? 1.0 : 0.5;
-    for (int l = 0; l < SVAR; l++) FORI {
+    for (int l = 0; l < SVAR; l++) FORI_PAR {
         size_t x = l * n + IX(c, i, j, k);
         c->var2[x] = 0.5 * c->var0[x] + 0.5 * c->var1[x] + w2 * ts * L[x];
       }
@@ -1873,7 +1912,7 @@ static int step_core(or_ctx* c, double tau, double ts)
     return c->bad_cell >= 0 ? VIRAL_ERR_PHYSICS : VIRAL_OK;
   }
   const double a0 = MUT == 1 ? 0.5 : 0.75, a1 = MUT == 1 ? 0.5 : 0.25;
-  for (int l = 0; l < SVAR; l++) FORI {
+  for (int l = 0; l < SVAR; l++) FORI_PAR {
       size_t x = l * n + IX(c, i, j, k);
       c->var2[x] = a0 * c->var0[x] + a1 * c->var1[x] + a1 * ts * L[x];
     }
@@ -1884,7 +1923,7 @@ static int step_core(or_ctx* c, double tau, double ts)
   /* stage 3 */
   stage_rhs(c, tmid, MUT == 3 ? ts : ts / 2.0, sub);
   const double b0 = MUT == 4 ? 0.25 : 1.0 / 3.0, b1 = MUT == 4 ? 0.75 : 2.0 / 3.0;
-  for (int l = 0; l < SVAR; l++) FORI {
+  for (int l = 0; l < SVAR; l++) FORI_PAR {
       size_t x = l * n + IX(c, i, j, k);
       c->var1[x] = b0 * c->var0[x] + b1 * c->var2[x] + b1 * ts * L[x];
     }
