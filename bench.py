@@ -1,14 +1,24 @@
 """Benchmark of the MI355X viscous hydro step (one GPU).
 
-Workload (N=1): the second benchmark configuration -- viscous Kurganov-Tadmor
-step with the s95p EOS on a smooth optical-Glauber event (Woods-Saxon
-R=6.62 fm, a=0.546 fm, b=7 fm, peak e=30 GeV/fm^3, eta plateau 1.5 with
-Gaussian fall-off 1.0), Milne grid 200x200x64, dx=deta=0.1, tau0=0.6 fm/c,
-eta/s=0.08, tau_pi=5 eta/(e+p), RK3 with dtau=0.002 fm/c as the reference's
-s95p build (NEWEST10/Makeeos:5, maindef.h:255), FP64.  A step is one full RK
-step of the whole grid; inputs are resident in HBM before timing starts.
+Headline workload: the batched-ensemble configuration -- 64 seeded
+fluctuating events (200 Gaussian hot spots of width 0.4 fm from two
+Woods-Saxon nuclei, b ~ U(0,12) fm) on uneven Milne grids (transverse side
+~U{96..256} cells, eta cells ~U{32..64}, dx = deta = 0.1), viscous
+Kurganov-Tadmor step with the s95p EOS (table), eta/s = 0.08,
+tau_pi = 5 eta/(e+p), RK3, dtau = 0.002 fm/c, FP64, all events in one context.
+A step is one full RK step of every event; `value` is interior cell-updates
+per second of the batched launch (one launch per kernel phase over all
+events, hipGraph replay, autotuned launch configuration), and `plain` is the
+same ensemble stepped with per-event launches of the same kernels.  Inputs are
+resident in HBM before timing starts.
 
-Prints one JSON line.  `value` is interior cell-updates per second.
+Further legs (each its own JSON key): the smooth-Glauber 200x200x64 grid in
+fast and exact arithmetic, the Gubser 64x64x8 grid, the 256x256x96 hot-spot
+grid (KT and SHASTA), the adjoint of 200 steps on 128x128x32 with checkpoints
+every 20 steps, a small-event ensemble, and the CPU restatement of the step
+on the host's cores (one thread and all cores).
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -22,6 +32,11 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64 cell-updates/s: plain vs batched ensemble; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
+# Compulsory HBM bytes of an ideal fused step per interior cell-update: the
+# per-cell state T^{tau mu} (4), pi (10), e, u (4), prevu (4) = 23 doubles,
+# read and written once per stage where the stage needs them (P, T and v are
+# recomputed).  RK2 60 reads + 38 writes, RK3 97 + 57, RK1 42 doubles.
+IDEAL_BYTES_PER_UPDATE = {1: 336, 2: 784, 3: 1232}
 
 
 def workload_desc(rk=3, fast=1):
@@ -95,6 +110,27 @@ MEASURED_TRAFFIC = {
 }
 
 
+# The headline ensemble's dominant kernel (the fused update of one stage, one
+# batched launch over all 64 events), from rocprofv3 counter passes on the
+# ensemble leg: HBM bytes per launch (FETCH_SIZE x 2.00 x 1024 + WRITE_SIZE x
+# 1.00 x 1024, the 1 GiB copy's calibration) and FP64 VALU work per launch
+# (64 lanes x (2 SQ_INSTS_VALU_FMA_F64 + SQ_INSTS_VALU_MUL_F64 +
+# SQ_INSTS_VALU_ADD_F64 + SQ_INSTS_VALU_TRANS_F64)), VALU wave-instructions per
+# launch and the VALU issue fraction (SQ_INSTS_VALU / (256 x GRBM_GUI_ACTIVE)).
+# Filled from the profile of the same bench leg; None until measured.
+ENSEMBLE_TRAFFIC = {}
+ENSEMBLE_COUNTERS = {}
+
+
+def ensemble_counters(kernel, avg_s):
+    c = ENSEMBLE_COUNTERS.get(kernel)
+    if not c:
+        return {"valu_frac": None}
+    return {"valu_frac": c["fp64_flops"] / avg_s / 1e12 / FP64_VALU_PEAK_TFLOPS,
+            "fp64_tflops": c["fp64_flops"] / avg_s / 1e12, "valu_issue_frac": c["valu_issue_frac"],
+            "counters": c["source"]}
+
+
 def measured_traffic(name, d, fused):
     if (d.nx, d.ny, d.nz, d.rk, d.fast_math, d.eos_table) == (200, 200, 64, 3, 1, 1):
         if name == "update":
@@ -103,31 +139,77 @@ def measured_traffic(name, d, fused):
     return None
 
 
-def cpu_baseline(seconds_target=20.0):
-    """The CPU oracle (a one-thread restatement of the reference step) on a
-    bounded sample of the same workload: the central 16-slice eta slab of the
-    200x200 Glauber grid, one RK3 step."""
-    import numpy as np
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_all_core():
+    """The oracle built for the host it runs on (-O3 -march=native -fopenmp,
+    -ffp-contract=off so it rounds as the test build does) into a temporary
+    directory; the prebuilt portable OpenMP build (x86-64-v3) where no
+    compiler is at hand."""
+    import subprocess
+    import tempfile
+    src = os.path.join(ROOT, "oracle", "oracle.c")
+    out = os.path.join(tempfile.mkdtemp(prefix="viral_oracle_"), "liboracle_native.so")
+    flags = ["-O3", "-march=native", "-fopenmp", "-ffp-contract=off"]
+    r = subprocess.run(["gcc"] + flags + ["-fPIC", "-shared", "-o", out, src, "-lm"], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT)
+    if r.returncode == 0:
+        return out, " ".join(flags)
+    return os.path.join(ROOT, "oracle", "liboracle_omp.so"), "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"
+
+
+def cpu_baseline(seconds_target=12.0, max_steps=3):
+    """The CPU restatement of the reference step (oracle/oracle.c, the
+    reference's loop structure and operation order) on a bounded sample of
+    the headline workload: the central 16 eta slices of the ensemble's first
+    event (hot spots, s95p closed form, eta/s = 0.08, RK3), timed first on one
+    thread, then with OpenMP over the host cores this process may use."""
+    import ctypes
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes
-    from viral_amd import ic
-    d = workload_desc(fast=0)
+    from viral_amd import capi, ic
+    descs, bs = ensemble_descs(1)
+    d = descs[0].copy()
+    d.fast_math = d.eos_table = 0
     d.nz = 16
-    st = ic.glauber(d, 0.6, 0.002, b=7.0, eos_p=eos_p_vec(d))
-    o = oracle_ctypes.Oracle(d)
-    o.set_state(st)
-    t0 = time.perf_counter()
-    steps = 0
-    while True:
-        o.step(0.6 + steps * 0.002, 0.002)
-        steps += 1
-        if time.perf_counter() - t0 > seconds_target or steps >= 3:
-            break
-    dt = time.perf_counter() - t0
+    st = ic.hotspots(d, 0.6, 0.002, seed=1000, b=bs[0], eos_p=lambda x: capi.eos_eval(d, x)[0].reshape(x.shape))
+    path, flags = _oracle_all_core()
+    L = oracle_ctypes.load(path)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     cells = d.nx * d.ny * d.nz
-    return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "%d RK3 step(s) of the central 200x200x16 eta slab of the same Glauber s95p grid, "
-                      "CPU oracle (C, -O2, one thread)" % steps}
+
+    L.omp_set_num_threads.argtypes = [ctypes.c_int]
+
+    def timed(nthreads):
+        L.omp_set_num_threads(nthreads)
+        o = oracle_ctypes.Oracle(d, L)
+        o.set_state(st)
+        t0 = time.perf_counter()
+        steps = 0
+        while True:
+            assert o.step(0.6 + steps * 0.002, 0.002) == 0
+            steps += 1
+            if time.perf_counter() - t0 > seconds_target or steps >= max_steps:
+                break
+        dt = time.perf_counter() - t0
+        return cells * steps / dt, steps
+
+    one, n1 = timed(1)
+    allc, na = timed(threads)
+    sample = ("%d x %d x 16 cells (the central 16 eta slices of the ensemble's first event, hot spots, s95p closed "
+              "form, RK3, dtau=0.002), CPU oracle built with %s" % (d.nx, d.ny, flags))
+    return {"value": allc, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": "%d RK3 step(s) on %d threads of %s" % (na, threads, sample),
+            "single_thread": {"value": one, "unit": "cell-updates/s", "cores": 1, "steps": n1},
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
 
 
 def ensemble_descs(n_events, seed=4, side=(96, 256), eta=(32, 64)):
@@ -147,11 +229,20 @@ def ensemble_descs(n_events, seed=4, side=(96, 256), eta=(32, 64)):
     return descs, bs
 
 
-def bench_ensemble(n_events, steps, tau0=0.6, ts=0.002, side=(96, 256), eta=(32, 64)):
-    """Plain (one launch per event and kernel phase) vs batched (one launch per
-    phase over all events, hipGraph replay, autotuned launch configuration)
-    on the same uneven hotspot ensemble in one context; both synchronise once
-    per step."""
+def bench_ensemble(n_events, steps, warmup=2, tau0=0.6, ts=0.002, side=(96, 256), eta=(32, 64), kernel_times=False):
+    """The same uneven hot-spot ensemble in one context stepped three ways,
+    each synchronising once per step:
+      plain_default  one launch per event and kernel phase, the library's
+                     default launch options
+      plain          one launch per event and kernel phase with the kernel
+                     options the autotuner picks for the batched launch
+                     (flux block sizes and tile shapes, fused sources)
+      batched        one launch per kernel phase over all events, hipGraph
+                     replay of the whole RK step, autotuned (batched_over_plain
+                     is batched / plain)
+    kernel_times: afterwards, two batched steps without the graph with HIP
+    events around every launch (the kernels the graph replays), for the
+    per-kernel average durations."""
     from concurrent.futures import ThreadPoolExecutor
     from viral_amd import capi, ic
     descs, bs = ensemble_descs(n_events, side=side, eta=eta)
@@ -170,63 +261,111 @@ def bench_ensemble(n_events, steps, tau0=0.6, ts=0.002, side=(96, 256), eta=(32,
             h.upload(st, ev=e)
             del st
     t_ic = time.perf_counter() - t_ic
+    clock = [tau0]
+
+    def run(n):
+        for _ in range(n):
+            r = h.step(clock[0], ts)
+            assert r == 0, r
+            clock[0] += ts
 
     def timed(n):
-        t = [tau0]
-        r = h.step(t[0], ts)
-        assert r == 0, r
-        t[0] += ts
+        run(warmup)
         t0 = time.perf_counter()
-        for _ in range(n):
-            r = h.step(t[0], ts)
-            assert r == 0, r
-            t[0] += ts
+        run(n)
         return time.perf_counter() - t0
 
     out = {"workload": "%d hotspot events (200 spots, width 0.4 fm, b~U(0,12) fm), transverse side ~U{%d..%d}, "
-                       "eta cells ~U{%d..%d}, viscous KT, s95p table, eta/s=0.08, RK3, dtau=%g"
+                       "eta cells ~U{%d..%d}, viscous KT, s95p table, eta/s=0.08, tau_pi=5eta/(e+p), RK3, dtau=%g"
                        % (n_events, side[0], side[1], eta[0], eta[1], ts),
-           "events": n_events, "interior_cells": cells, "ic_seconds": round(t_ic, 1)}
+           "events": n_events, "interior_cells": cells, "steps": steps, "warmup": warmup,
+           "ic_seconds": round(t_ic, 1)}
     h.set_launch(batched=0, graph=0, blocks_per_cu=0, fusion=0, xcd=0)
     dt = timed(steps)
-    out["plain"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps}
+    out["plain_default"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps,
+                            "launch": h.launch().as_dict()}
     h.set_launch(batched=1, graph=1)
     h.autotune(reps=1)
-    out["launch"] = h.launch().as_dict()
+    tuned = h.launch().as_dict()
+    out["launch"] = tuned
     dt = timed(steps)
     out["batched"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps}
+    h.set_launch(batched=0, graph=0, blocks_per_cu=0, xcd=0)
+    out["plain"] = {"value": cells * steps / timed(steps) , "launch": h.launch().as_dict()}
+    out["plain"]["ms_per_step"] = cells / out["plain"]["value"] * 1000.0
     out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    out["batched_over_plain_default"] = out["batched"]["value"] / out["plain_default"]["value"]
+    if kernel_times:
+        h.set_launch(**{k: v for k, v in tuned.items() if k != "graph"}, graph=0)
+        run(1)
+        h.timers(True)
+        run(2)
+        rep = h.timer_report()
+        h.timers(False)
+        out["kernel_times"] = rep
+        out["kernel_launch"] = h.launch().as_dict()
     h.close()
     return out
 
 
+def kernel_roofline(rep, d_rk, cells, fused, traffic=None, counters=None):
+    """Roofline figure of the dominant kernel of a timed run (`rep`: kernel ->
+    (total ms, launches)): algorithmic bytes per launch from the ideal fused
+    step's bytes per cell-update (one stage's share for a per-stage kernel,
+    times the interior cells one launch covers), over the kernel's HIP-event
+    average launch time."""
+    dom = max(rep, key=lambda k: rep[k][0])
+    tot_ms, launches = rep[dom]
+    avg_s = tot_ms / 1000.0 / max(launches, 1)
+    per_stage = IDEAL_BYTES_PER_UPDATE[d_rk] / d_rk
+    algo = per_stage * cells
+    achieved = algo / avg_s / 1e9
+    out = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_ms": avg_s * 1000.0,
+           "launches": launches, "algorithmic_bytes_per_launch": algo,
+           "basis": "ideal fused step, %d B per RK%d cell-update (23 state doubles per cell), one stage's share "
+                    "(%.1f B) per cell per launch" % (IDEAL_BYTES_PER_UPDATE[d_rk], d_rk, d_rk and per_stage)}
+    if traffic:
+        out["traffic_over_algorithmic"] = traffic / algo
+    if counters:
+        out.update(counters(dom, avg_s))
+    return out
+
+
+# Energy floor of the fluctuating-event states, relative to their peak
+# (10 GeV/fm^3): 1e-2, i.e. 0.1 GeV/fm^3.  At 1e-7 the hot spots expand into
+# near-vacuum and a dilute cell turns acausal (KT after 259 steps; SHASTA with
+# the full-step sub-step after 36, at 1e-3 after 338); at 1e-2 both schemes
+# run the configuration's 400 steps.
+E_FLOOR_C3 = 1e-2
+
+
 def bench_c3(steps, tau0=0.6, ts=0.002):
-    """The 256x256x96 grid of the fluctuating-event configuration (s95p EOS,
-    table in fast mode, eta/s = 0.12, tau_pi = 5 eta/(e+p), RK2): the viscous
-    KT step (autotuned) on 200 Gaussian hot spots of width 0.4 fm from two
-    Woods-Saxon nuclei (b ~ U(0,10) fm, seed 3), with the update kernel's HBM
-    figure; and the SHASTA step on the smooth Glauber state (b = 7 fm) of the
-    same grid.  SHASTA's step breaks down in the dilute periphery of both
-    states (e <= 0 after 4 steps on the hot spots, 10 on the Glauber state,
-    exact arithmetic included), so its leg runs at most 2 + 5 steps."""
+    """The fluctuating-event configuration: 200 Gaussian hot spots of width
+    0.4 fm from two Woods-Saxon nuclei (b ~ U(0,10) fm, seed 3) on a Milne
+    256x256x96 grid, s95p EOS (table in fast mode), eta/s = 0.12,
+    tau_pi = 5 eta/(e+p), RK2, `steps` steps (400 as configured), timed from
+    the state after two warm-up steps: the SHASTA step (the configuration's
+    scheme) with the full-step sub-step (VIRAL_Q_SHASTA_FULLSTEP; the
+    reference's dtau/2 makes the grid-scale checkerboard grow 2.5x per step in
+    3-D, so its step fails within 5 steps on this state), and the viscous KT
+    step (autotuned) on the same state, with the update kernel's HBM
+    figure."""
     import torch
     from viral_amd import capi, ic
-    from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP, SHASTA
-    out = {"workload": "256x256x96 Milne grid, s95p table, eta/s=0.12, tau_pi=5eta/(e+p), RK2, dtau=%g, fast "
-                       "arithmetic; KT on 200 hot spots (width 0.4 fm, b~U(0,10) fm, seed 3), SHASTA on the smooth "
-                       "Glauber state (b=7 fm)" % ts}
-    for scheme in ("kt", "shasta"):
+    from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP, SHASTA, Q_DEFAULT, Q_SHASTA_FULLSTEP
+    out = {"workload": "200 hot spots (width 0.4 fm, b~U(0,10) fm, seed 3, energy floor %g of the peak), Milne "
+                       "256x256x96, s95p table, eta/s=0.12, tau_pi=5eta/(e+p), RK2, dtau=%g, fast arithmetic, %d steps; "
+                       "SHASTA with the full-step sub-step, and KT" % (E_FLOOR_C3, ts, steps)}
+    for scheme in ("shasta", "kt"):
         d = make_desc(256, 256, 96, dx=0.1, dy=0.1, deta=0.1, rk=2, eos=EOS_S95P, eta_law=ETA_TIMES_S, eta_coef=0.12,
                       tpi_law=TPI_OVER_EP, tpi_coef=5.0, fast_math=1, eos_table=1,
-                      scheme=SHASTA if scheme == "shasta" else 0)
+                      scheme=SHASTA if scheme == "shasta" else 0,
+                      quirks=Q_DEFAULT | (Q_SHASTA_FULLSTEP if scheme == "shasta" else 0))
         dref = d.copy()
         dref.fast_math = dref.eos_table = 0
-        eosp = (lambda x: capi.eos_eval(dref, x)[0].reshape(x.shape))
-        if scheme == "kt":
-            st = ic.hotspots(d, tau0, ts, seed=3, eos_p=eosp)
-        else:
-            st = ic.glauber(d, tau0, ts, b=7.0, eos_p=eosp)
-            steps = min(steps, 5)
+        st = ic.hotspots(d, tau0, ts, seed=3, e_floor=E_FLOOR_C3,
+                         eos_p=lambda x: capi.eos_eval(dref, x)[0].reshape(x.shape))
         h = capi.Hydro(d)
         h.upload(st)
         tau = tau0
@@ -241,7 +380,7 @@ def bench_c3(steps, tau0=0.6, ts=0.002):
         t0 = time.perf_counter()
         for _ in range(steps):
             r = h.step(tau, ts)
-            assert r == 0, r
+            assert r == 0, (scheme, r, h.first_bad_cell())
             tau += ts
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
@@ -251,11 +390,7 @@ def bench_c3(steps, tau0=0.6, ts=0.002):
         leg = {"value": cells * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt * 1e3 / steps, "steps": steps,
                "launch": h.launch().as_dict(),
                "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in rep.items() if v[1]}}
-        if scheme == "kt":
-            fused = leg["launch"]["fusion"] > 0
-            ms, nl = rep["update"]
-            leg["update_hbm"] = {"algorithmic_GBs": kernel_bytes("update", d, fused) / (ms / nl * 1e-3) / 1e9,
-                                 "frac": kernel_bytes("update", d, fused) / (ms / nl * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        leg["roofline"] = kernel_roofline(rep, d.rk, cells, leg["launch"]["fusion"] > 0)
         out[scheme] = leg
         h.close()
     return out
@@ -271,14 +406,14 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
     Reports the time of the recorded forward run and of the adjoint call
     (adjoint_ms: the C call, with the seed's upload and the gradient's
     download, 2 x 29 fields over PCIe; adjoint_device_ms: its device part,
-    HIP events around the recompute and reverse kernels),
-    first with every stage state kept during the forward run (every = 0, no
-    recompute: 63 GB of HBM for 200 RK2 steps), then with checkpoints every
-    `every` steps (the adjoint recomputes the stage states between them)."""
+    HIP events around the recompute and reverse kernels) with checkpoints
+    every `every` steps (the adjoint recomputes the stage states between
+    them), and as a labelled extra with every stage state kept during the
+    forward run (every = 0, no recompute: 63 GB of HBM for 200 RK2 steps)."""
     import numpy as np
     import torch
     from viral_amd import capi, ic, autograd
-    from viral_amd.desc import make_desc, F_EN, EOS_S95P, EOS_CONFORMAL
+    from viral_amd.desc import make_desc, F_EN, F_T00, F_P, EOS_S95P, EOS_CONFORMAL
     s95 = eos == "s95p"
     d = make_desc(128, 128, 32, rk=2, eta_coef=0.08, tpi_coef=5.0, fast_math=fast,
                   eos=EOS_S95P if s95 else EOS_CONFORMAL, eos_table=int(s95 and fast))
@@ -316,139 +451,197 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
         h.adjoint_end()
         runs[ev] = (t_fwd, t_adj, grad, pg, dev_ms)
     h.close()
+    dref = d.copy()
+    dref.fast_math = dref.eos_table = 0
+    dpde = capi.eos_eval(dref, st[F_EN])[1]
     t_fwd, t_adj, grad, pg, dev = runs[0]
     c_fwd, c_adj, c_grad, c_pg, c_dev = runs[every]
     assert np.array_equal(grad, c_grad) and np.array_equal(pg, c_pg), "stored and recomputed stage states disagree"
+    slab_gb = 29 * 8 * d.cells() * 2 * steps / 1e9
     return {"workload": "adjoint of %d viscous KT steps, smooth Glauber b=8 fm, Milne 128x128x32, %s EOS, "
-                        "eta/s=0.08, tau_pi=5eta/(e+p), RK2, dtau=%g, every stage state kept in HBM, %s arithmetic; "
+                        "eta/s=0.08, tau_pi=5eta/(e+p), RK2, dtau=%g, checkpoints every %d steps, %s arithmetic; "
                         "d eps_2 / d(initial state, eta/s, tau_pi coefficient)"
-                        % (steps, "s95p (table)" if s95 and fast else eos, ts, "fast" if fast else "exact"),
-            "steps": steps, "forward_ms": t_plain * 1e3, "forward_recorded_ms": t_fwd * 1e3,
-            "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_plain,
-            "adjoint_device_ms": dev, "adjoint_device_over_forward": dev / (t_plain * 1e3),
-            "checkpointed": {"every": every, "forward_recorded_ms": c_fwd * 1e3, "adjoint_ms": c_adj * 1e3,
-                             "adjoint_over_forward": c_adj / t_plain, "adjoint_device_ms": c_dev,
-                             "adjoint_device_over_forward": c_dev / (t_plain * 1e3), "same_gradient_bits": True},
-            "eps2": float(eps2), "d_eps2_d_eta_coef": float(pg[0]), "d_eps2_d_tpi_coef": float(pg[1]),
-            "d_eps2_d_e_norm": float(np.linalg.norm(grad[F_EN]))}
+                        % (steps, "s95p (table)" if s95 and fast else eos, ts, every, "fast" if fast else "exact"),
+            "steps": steps, "every": every, "forward_ms": t_plain * 1e3, "forward_recorded_ms": c_fwd * 1e3,
+            "adjoint_ms": c_adj * 1e3, "adjoint_over_forward": c_adj / t_plain,
+            "adjoint_device_ms": c_dev, "adjoint_device_over_forward": c_dev / (t_plain * 1e3),
+            "stored_stages": {"note": "every stage state kept in HBM during the recorded forward (every = 0, no "
+                                      "recompute), %.1f GB" % slab_gb,
+                              "forward_recorded_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
+                              "adjoint_over_forward": t_adj / t_plain, "adjoint_device_ms": dev,
+                              "adjoint_device_over_forward": dev / (t_plain * 1e3), "same_gradient_bits": True},
+            "eps2": float(eps2.detach()), "d_eps2_d_eta_coef": float(pg[0]), "d_eps2_d_tpi_coef": float(pg[1]),
+            "d_eps2_d_initial_e_norm": float(np.linalg.norm(grad[F_EN] + grad[F_T00] + grad[F_P] * dpde)),
+            "d_eps2_d_initial_e_note": "the initial e-field of the fluid at rest enters the state as En, T^{tau tau} "
+                                       "= e and P(e) (s95p closed form): dJ/de = dJ/dEn + dJ/dT^tt + dJ/dP dP/de"}
+
+
+def bench_c2(steps, warmup, exact_steps, copy_probe=False):
+    """The smooth-Glauber grid: viscous KT, s95p EOS, Glauber b=7 fm, Milne
+    200x200x64, dx=deta=0.1, tau0=0.6, dtau=0.002, eta/s=0.08,
+    tau_pi=5eta/(e+p), RK3.  Fast arithmetic (s95p table, autotuned, HIP
+    events around every launch) and exact arithmetic (the oracle's operation
+    order, bitwise equal to it)."""
+    import torch
+    from viral_amd import capi, ic
+    out = {"workload": "viscous KT, s95p EOS, Glauber b=7 fm, Milne 200x200x64, dx=deta=0.1, tau0=0.6, dtau=0.002, "
+                       "eta/s=0.08, tau_pi=5eta/(e+p), RK3"}
+    for mode, n in (("fast", steps), ("exact", exact_steps)):
+        if n <= 0:
+            continue
+        d = workload_desc(fast=1 if mode == "fast" else 0)
+        tau0, ts = 0.6, 0.002
+        st = ic.glauber(d, tau0, ts, b=7.0, eos_p=eos_p_vec(d))
+        h = capi.Hydro(d)
+        h.upload(st)
+        tau = tau0
+        for _ in range(warmup if mode == "fast" else 1):
+            assert h.step(tau, ts) == 0
+            tau += ts
+        tiles = h.autotune(reps=2) if d.fast_math else None
+        launch = h.launch().as_dict()
+        h.timers(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            assert h.step(tau, ts) == 0
+            tau += ts
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rep = h.timer_report()
+        h.timers(False)
+        if copy_probe and mode == "fast":
+            src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+            dst = torch.empty_like(src)
+            torch.cuda.synchronize()
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
+        cells = d.nx * d.ny * d.nz
+        fused = launch["fusion"] > 0
+        leg = {"value": cells * n / dt, "unit": "cell-updates/s", "ms_per_step": dt * 1000.0 / n, "steps": n,
+               "launch": launch, "flux_tiles": tiles,
+               "kernel_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in rep.items() if v[1]}}
+        rl = kernel_roofline(rep, d.rk, cells, fused, traffic=measured_traffic(max(rep, key=lambda k: rep[k][0]), d,
+                                                                              fused))
+        dom = rl["kernel"]
+        rl["kernel_model"] = {"algorithmic_bytes_per_launch": kernel_bytes(dom, d, fused),
+                              "note": "every input field of the kernel read once, every output written once"}
+        leg["roofline"] = rl
+        leg["step_hbm"] = {"ideal_bytes_per_cell_update": IDEAL_BYTES_PER_UPDATE[d.rk],
+                           "achieved_GBs": leg["value"] * IDEAL_BYTES_PER_UPDATE[d.rk] / 1e9,
+                           "frac": leg["value"] * IDEAL_BYTES_PER_UPDATE[d.rk] / 1e9 / HBM_PEAK_GBS}
+        out[mode] = leg
+        h.close()
+    return out
+
+
+def bench_c1(steps=100):
+    """The Gubser grid: ideal Gubser initial flow (q = 1 fm^-1) at tau0 = 1,
+    conformal EOS, eta/s = 0.2, tau_pi = 5 eta/(e+p), Milne 64x64x8,
+    dx = deta = 0.1, dtau = 0.01, RK2; all `steps` steps in one viral_steps
+    call (hipGraph replay), fast and exact arithmetic."""
+    import torch
+    from viral_amd import capi, ic
+    from viral_amd.desc import make_desc
+    out = {"workload": "Gubser flow, conformal EOS, eta/s=0.2, tau_pi=5eta/(e+p), Milne 64x64x8, dx=deta=0.1, "
+                       "tau0=1, dtau=0.01, RK2, %d steps" % steps}
+    for mode in ("fast", "exact"):
+        d = make_desc(64, 64, 8, fast_math=1 if mode == "fast" else 0)
+        st = ic.gubser(d, 1.0, 0.01)
+        h = capi.Hydro(d)
+        h.set_launch(graph=1)
+        h.upload(st)
+        r, done = h.steps(1.0, 0.01, 2)
+        assert r == 0, (r, done)
+        h.upload(st)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r, done = h.steps(1.0, 0.01, steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        assert r == 0, (r, done)
+        out[mode] = {"value": d.nx * d.ny * d.nz * steps / dt, "unit": "cell-updates/s",
+                     "ms_per_step": dt * 1000.0 / steps}
+        h.close()
+    return out
+
+
+LEGS = ("c4", "c2", "c1", "c3", "c5", "small", "cpu")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--ensemble-events", type=int, default=64,
-                    help="events of the batched-ensemble leg (0 skips it)")
-    ap.add_argument("--ensemble-steps", type=int, default=5)
-    ap.add_argument("--small-events", type=int, default=128,
-                    help="events of the small-event ensemble leg (transverse side 24..64, eta 8..24; 0 skips it)")
-    ap.add_argument("--adjoint-steps", type=int, default=200,
-                    help="steps of the adjoint leg (0 skips it)")
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of the headline ensemble (each way)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma list of legs to run: c4 (headline ensemble), c2 (200x200x64 Glauber), c1 (Gubser "
+                         "64x64x8), c3 (256x256x96 hot spots, KT and SHASTA), c5 (adjoint), small (small-event "
+                         "ensemble), cpu (CPU baseline)")
+    ap.add_argument("--ensemble-events", type=int, default=64)
+    ap.add_argument("--c2-steps", type=int, default=20)
+    ap.add_argument("--exact-steps", type=int, default=3, help="timed steps of the exact-arithmetic c2 leg")
+    ap.add_argument("--c3-steps", type=int, default=400)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-eos", choices=["s95p", "conformal"], default="s95p")
     ap.add_argument("--adjoint-exact", action="store_true", help="run the adjoint leg in exact arithmetic")
-    ap.add_argument("--c3-steps", type=int, default=10,
-                    help="timed steps of the 256x256x96 hot-spot grid legs (KT and SHASTA); 0 skips them")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--small-events", type=int, default=128)
     ap.add_argument("--copy-probe", action="store_true",
-                    help="after the timed steps, run one plain 1 GiB device copy so a counter run can scale "
-                         "FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
-    ap.add_argument("--exact", action="store_true",
-                    help="time the exact-arithmetic mode (bitwise equal to the CPU oracle) instead of fast mode")
+                    help="after the c2 fast leg's timed steps, run one plain 1 GiB device copy so a counter run can "
+                         "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (events shard by event, a "
               "single event by x-y slabs; see the design notes). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    legs = set(x for x in args.legs.split(",") if x)
+    if args.no_cpu:
+        legs.discard("cpu")
 
-    import numpy as np
     import torch
-    from viral_amd import capi, ic
-
     assert torch.cuda.is_available(), "needs a GPU"
     torch.cuda.set_device(0)
-    d = workload_desc(fast=0 if args.exact else 1)
-    tau0, ts = 0.6, 0.002
-    st = ic.glauber(d, tau0, ts, b=7.0, eos_p=eos_p_vec(d))
-    h = capi.Hydro(d)
-    h.upload(st)
-    tau = tau0
-    for _ in range(args.warmup):
-        r = h.step(tau, ts)
-        assert r == 0, r
-        tau += ts
-    tiles = h.autotune(reps=2) if d.fast_math else None   # launch configuration, on this state
-    launch = h.launch().as_dict()
-    torch.cuda.synchronize()
-    h.timers(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = h.step(tau, ts)
-        assert r == 0, r
-        tau += ts
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    rep = h.timer_report()
-    h.timers(False)
-    if args.copy_probe:
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    cells = d.nx * d.ny * d.nz
-    value = cells * args.steps / dt
-    ms_step = dt * 1000.0 / args.steps
 
-    # dominant kernel and its roofline figure
-    dom = max(rep, key=lambda k: rep[k][0])
-    tot_ms, launches = rep[dom]
-    avg_s = tot_ms / 1000.0 / max(launches, 1)
-    fused = launch["fusion"] > 0
-    bytes_launch = kernel_bytes(dom, d, fused)
-    achieved = bytes_launch / avg_s / 1e9
-    kernel_share = {k: round(v[0] / max(sum(x[0] for x in rep.values()), 1e-12), 4) for k, v in rep.items() if v[1]}
-    bytes_per_update = {1: 336, 2: 784, 3: 1232}[d.rk]
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (optical-Glauber initial state generated on the host)",
-        "config": {"workload": "viscous KT, s95p EOS, Glauber b=7 fm, Milne 200x200x64, dx=deta=0.1, "
-                               "tau0=0.6, dtau=0.002, eta/s=0.08, tau_pi=5eta/(e+p), RK3",
-                   "grid": [d.nx, d.ny, d.nz], "rk": d.rk, "fast_math": d.fast_math, "eos_table": d.eos_table,
-                   "flux_tiles": tiles, "launch": launch,
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, d, fused),
-                     "avg_launch_ms": avg_s * 1000.0, "launches": launches,
-                     "algorithmic_bytes_per_launch": bytes_launch},
-        "step_hbm": {"algorithmic_bytes_per_cell_update": bytes_per_update,
-                     "achieved_GBs": value * bytes_per_update / 1e9,
-                     "frac": value * bytes_per_update / 1e9 / HBM_PEAK_GBS},
-        "kernel_time_share": kernel_share,
-    }
-    h.close()
-    if args.ensemble_events > 0:
-        out["ensemble"] = bench_ensemble(args.ensemble_events, args.ensemble_steps)
-    if args.small_events > 0:
-        # many small events: the launch-bound regime the batched launch is for
-        out["ensemble_small"] = bench_ensemble(args.small_events, 4 * args.ensemble_steps, side=(24, 64), eta=(8, 24))
-    if args.c3_steps > 0:
+    out = {"metric": METRIC, "value": None, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded hot-spot and optical-Glauber initial states generated on the host)"}
+    if "c4" in legs:
+        ens = bench_ensemble(args.ensemble_events, args.steps, warmup=args.warmup, kernel_times=True)
+        out["value"] = ens["batched"]["value"]
+        out["ms_per_step"] = ens["batched"]["ms_per_step"]
+        out["config"] = {"workload": ens["workload"], "events": ens["events"], "interior_cells": ens["interior_cells"],
+                         "launch": ens["launch"], "parallelism": "single GPU"}
+        out["plain"] = ens["plain"]
+        out["plain_default"] = ens["plain_default"]
+        out["batched_over_plain"] = ens["batched_over_plain"]
+        out["batched_over_plain_default"] = ens["batched_over_plain_default"]
+        rep = ens["kernel_times"]
+        out["roofline"] = kernel_roofline(rep, 3, ens["interior_cells"], ens["launch"]["fusion"] > 0,
+                                          traffic=ENSEMBLE_TRAFFIC.get(max(rep, key=lambda k: rep[k][0])),
+                                          counters=ensemble_counters)
+        out["roofline"]["timing"] = ("HIP events around every launch of two batched steps replayed without the "
+                                     "graph after the timed steps (launch %s)" % ens["kernel_launch"])
+        tot = sum(v[0] for v in rep.values())
+        out["kernel_time_share"] = {k: round(v[0] / tot, 4) for k, v in rep.items() if v[1]}
+        out["ensemble"] = {k: ens[k] for k in ("ic_seconds", "steps", "warmup")}
+    if "c2" in legs:
+        out["glauber_200x200x64"] = bench_c2(args.c2_steps, args.warmup, args.exact_steps, args.copy_probe)
+    if "c1" in legs:
+        out["gubser_64x64x8"] = bench_c1()
+    if "c3" in legs and args.c3_steps > 0:
         out["grid_256x256x96"] = bench_c3(args.c3_steps)
-    if args.adjoint_steps > 0:
-        out["adjoint"] = bench_adjoint(args.adjoint_steps, eos=args.adjoint_eos, fast=0 if args.adjoint_exact else 1)
-    if not args.no_cpu:
+    if "c5" in legs and args.adjoint_steps > 0:
+        adj = bench_adjoint(args.adjoint_steps, eos=args.adjoint_eos, fast=0 if args.adjoint_exact else 1)
+        out["adjoint"] = adj
+        out["adjoint_over_forward"] = adj["adjoint_over_forward"]
+    if "small" in legs and args.small_events > 0:
+        # many small events: the launch-bound regime the batched launch is for
+        out["ensemble_small"] = bench_ensemble(args.small_events, 4 * args.steps, warmup=args.warmup,
+                                               side=(24, 64), eta=(8, 24))
+    if "cpu" in legs:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
 

@@ -75,6 +75,13 @@ enum { VIRAL_TPIB_CONST = 0,       /* tau_PI = c               */
                                            end of the previous step (or of the uploaded state, at the first
                                            step's tau), then take BMax of the new state.  Milne only; off in every
                                            reference profile */
+#define VIRAL_Q_SHASTA_FULLSTEP (1u << 7) /* not a reference behaviour: the SHASTA rate uses the full step dtau as
+                                           its sub-step instead of the reference's dtau/2 (RK2) or dtau/3 (RK3)
+                                           (q6, NEWEST10/time.h:352,411).  The reference's choice divides each
+                                           direction's 1/8 diffusion by the sub-step, which makes the 3-D
+                                           checkerboard mode of a fluid at rest grow by 2.5 per RK2 step and 8.6
+                                           per RK3 step (its antidiffusion limiter is off at extrema); with dtau
+                                           it decays (0.625, 0.0625).  Off by default */
 #define VIRAL_Q_DEFAULT (VIRAL_Q_GHOST_EIGEN | VIRAL_Q_K0_BROADCAST | VIRAL_Q_DPDE_OF_P | VIRAL_Q_SHEAR_SLICES)
 
 /* State fields, each one ghosted array of doubles.  Names follow the

@@ -1887,6 +1887,7 @@ static int step_core(or_ctx* c, double tau, double ts)
   c->bad_cell = -1;
   double* L = c->res;
   double sub = d->rk == 3 ? ts / 3.0 : d->rk == 2 ? ts / 2 : ts;
+  if (d->quirks & VIRAL_Q_SHASTA_FULLSTEP) sub = ts; /* the stable variant (include/viral.h) */
   /* stage 1 */
   stage_rhs(c, tau, ts, sub);
   memcpy(c->var0, c->var, SVAR * n * sizeof(double)); /* CalcVar0 (interior entries used) */

@@ -25,6 +25,20 @@ def lib():
     return _lib
 
 
+_omp = None
+
+
+def fast_lib():
+    """The OpenMP build of the oracle (oracle/liboracle_omp.so, bitwise equal
+    to the serial build: tests/test_oracle_openmp.py) for long reference runs;
+    the serial build where it is missing."""
+    global _omp
+    if _omp is None:
+        path = os.path.join(ROOT, "oracle", "liboracle_omp.so")
+        _omp = load(path) if os.path.exists(path) else lib()
+    return _omp
+
+
 def load(path):
     """an oracle library at `path` with its prototypes set (the default one,
     or a variant the tests built, such as one with a planted mistake)"""

@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import capi
-from .desc import (NFIELDS, F_T00, F_PI0, F_BULK, F_EN, F_P, F_TEMP, F_VX, F_U0, F_PU0, BORDER)
+from .desc import (NFIELDS, F_T00, F_PI0, F_BULK, F_EN, F_P, F_TEMP, F_VX, F_U0, F_PU0, BORDER, EOS_CONFORMAL)
 
 
 class HydroRun(torch.autograd.Function):
@@ -48,14 +48,44 @@ def run(state, eta_coef, tpi_coef, desc, tau0, ts, nsteps, every=20):
     return HydroRun.apply(state, eta_coef, tpi_coef, desc, tau0, ts, nsteps, every)
 
 
+class EosPressure(torch.autograd.Function):
+    """P(e) of a descriptor's EOS evaluated by the library on the host
+    (viral_eos_eval: the closed forms; the descriptor's arithmetic mode does
+    not matter, the initial state is built with the closed form), with dP/de
+    from the same call for the backward pass."""
+
+    @staticmethod
+    def forward(ctx, e, desc):
+        d = desc.copy()
+        d.fast_math = d.eos_table = 0
+        p, dpde, _, _ = capi.eos_eval(d, e.detach().cpu().numpy())
+        ctx.save_for_backward(torch.from_numpy(dpde).reshape(e.shape))
+        return torch.from_numpy(p).reshape(e.shape)
+
+    @staticmethod
+    def backward(ctx, gp):
+        (dpde,) = ctx.saved_tensors
+        return gp * dpde, None
+
+
 def state_from_energy(desc, e, tau):
     """Differentiable 29-field state of a fluid at rest with zero shear stress
-    and energy density e (torch, ghosted cells), conformal EOS:
-    T^{tau tau} = e, T^{tau i} = 0, P = e/3 (NEWEST10/init.h:537-540)."""
+    and energy density e (torch, ghosted cells) under the descriptor's EOS:
+    T^{tau tau} = (e + P) - P = e, T^{tau i} = 0 (NEWEST10/init.h:537-540),
+    P = P(e) -- e/3 for the conformal EOS, the s95p closed form through
+    EosPressure otherwise -- and Temp = T(e) (not read by the step, so it
+    carries no gradient)."""
     n = desc.cells()
     e = e.reshape(n)
-    P = e / 3.0
-    factor = 3.0 * desc.dof * math.pi * math.pi / 90.0
+    if desc.eos == EOS_CONFORMAL:
+        P = e / 3.0
+        factor = 3.0 * desc.dof * math.pi * math.pi / 90.0
+        temp = torch.sqrt(torch.sqrt(e / factor))
+    else:
+        P = EosPressure.apply(e, desc)
+        d = desc.copy()
+        d.fast_math = d.eos_table = 0
+        temp = torch.from_numpy(capi.eos_eval(d, e.detach().cpu().numpy())[2])
     zero = torch.zeros_like(e)
     one = torch.ones_like(e)
     rows = [zero] * NFIELDS
@@ -63,7 +93,7 @@ def state_from_energy(desc, e, tau):
     rows[F_T00] = (e + P) - P
     rows[F_EN] = e
     rows[F_P] = P
-    rows[F_TEMP] = torch.sqrt(torch.sqrt(e / factor))
+    rows[F_TEMP] = temp
     rows[F_U0] = one
     rows[F_PU0] = one
     return torch.stack(rows)

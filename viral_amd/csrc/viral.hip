@@ -618,7 +618,8 @@ __global__ void __launch_bounds__(BLK) k_shasta_dir(const EvDev* __restrict__ ta
     long lb;
     const EvDev& E = tab[sched_find(s, w_, lb)];
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
-    shasta_cell<FAST>(E.g, E.p, d, a.tau_src, a.taustep, E.st[cur], E.flux[d], lb * BLK + threadIdx.x);
+    const double taustep = (E.p.quirks & VIRAL_Q_SHASTA_FULLSTEP) ? a.ts : a.taustep;
+    shasta_cell<FAST>(E.g, E.p, d, a.tau_src, taustep, E.st[cur], E.flux[d], lb * BLK + threadIdx.x);
   }
 }
 

@@ -21,6 +21,7 @@ Q_GUBSER_MIRROR = 1 << 3
 Q_SHEAR_SLICES = 1 << 4
 Q_ROOT_SAFEGUARD = 1 << 5   # opt-in: bracketed re-solve of cells whose Newton solve fails
 Q_FIX_VE = 1 << 6           # opt-in: the FIX build's FixTwo (v_eta zeroed after a step with BMax > 0.98)
+Q_SHASTA_FULLSTEP = 1 << 7  # opt-in: SHASTA sub-step dtau (the reference's dtau/2, dtau/3 is unstable in 3-D)
 Q_DEFAULT = Q_GHOST_EIGEN | Q_K0_BROADCAST | Q_DPDE_OF_P | Q_SHEAR_SLICES
 
 (F_T00, F_T10, F_T20, F_T30) = range(4)
