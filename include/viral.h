@@ -277,6 +277,18 @@ int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out);
  * rank).  Arrays hold 9 entries (27 for ijk, which may be NULL); *nstat
  * receives the count.  P(e) is the closed-form EOS. */
 int viral_check_root(viral_ctx* ctx, int ev, double tau, double* max, double* avg, int* ijk, int* nstat);
+/* viral_check_root without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results, bit for bit.  Its block partials and combined
+ * statistics live in a fixed device array of the library (DIAG_BLOCKS = 512
+ * blocks x 9 statistics + 9, 110.8 KB per GPU, held until the process ends;
+ * every grid fits, since the reduction strides over the interior with at most
+ * 512 blocks, and a layout that would not fit is rejected with
+ * VIRAL_ERR_ARG).  All calls on one GPU share that array and each call
+ * overwrites what it reads: calls that overlap on one GPU (several threads,
+ * or contexts on several streams) race. */
+int viral_check_root_noalloc(viral_ctx* ctx, int ev, double tau, double* max, double* avg, int* ijk, int* nstat);
 /* DebugMSG (NEWEST10/debug.h:3-100, called at the end of every step,
  * time.h:334,391,477; Cartesian TXYZ5/debug.h with |v| free of tau):
  * out[0] the largest |v| = sqrt(vx^2 + vy^2 + tau^2 ve^2) (the reference's

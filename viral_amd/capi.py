@@ -22,7 +22,8 @@ EXPORTS = [
     "viral_eos_eval", "viral_set_tile", "viral_get_tile", "viral_autotune", "viral_timers", "viral_timer_read",
     "viral_kernel_count", "viral_kernel_name", "viral_last_error", "viral_steps", "viral_get_launch",
     "viral_set_launch", "viral_tile_count", "viral_tile_table", "viral_xcd_tile", "viral_adjoint_begin",
-    "viral_adjoint", "viral_adjoint_end", "viral_check_root", "viral_debug_extrema", "viral_check_physics",
+    "viral_adjoint", "viral_adjoint_end", "viral_check_root", "viral_check_root_noalloc", "viral_debug_extrema",
+    "viral_check_physics",
     "viral_output_size", "viral_output", "viral_sync", "viral_host_alloc", "viral_host_free", "viral_root",
     "viral_ns_init", "viral_landau",
 ]
@@ -124,6 +125,8 @@ def load():
     ip = C.POINTER(C.c_int)
     L.viral_check_root.restype = C.c_int
     L.viral_check_root.argtypes = [vp, C.c_int, C.c_double, dp, dp, ip, ip]
+    L.viral_check_root_noalloc.restype = C.c_int
+    L.viral_check_root_noalloc.argtypes = [vp, C.c_int, C.c_double, dp, dp, ip, ip]
     L.viral_debug_extrema.restype = C.c_int
     L.viral_debug_extrema.argtypes = [vp, C.c_int, C.c_double, dp, ip]
     L.viral_check_physics.restype = C.c_int

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <algorithm>
 #include <type_traits>
 #include <string>
@@ -1467,7 +1468,7 @@ __host__ __device__ inline DiagPart diag_combine(DiagPart a, DiagPart b, int rul
 }
 
 // rules and count of the statistics of each kind
-__host__ __device__ inline int diag_nstat(int kind, int milne)
+__host__ __device__ constexpr int diag_nstat(int kind, int milne)
 {
   return kind == D_ROOT ? (milne ? 6 : 9) : kind == D_EXTREMA ? 3 : 2;
 }
@@ -1489,7 +1490,9 @@ __device__ __forceinline__ double diag_speed(const Params& p, const double* st, 
 // Per-cell values of the statistics (value, counts toward the statistic,
 // added to the sum).  Exact arithmetic in the reference's operation order
 // (this file is compiled without FP contraction), closed-form EOS.
-template <int KIND>
+// CART (TXYZ5) fixes the statistic layout at compile time, so the per-cell
+// arrays stay in registers.
+template <int KIND, bool CART>
 __device__ __forceinline__ void diag_cell(const Params& p, const double* __restrict__ st, long n, long x, double tau,
                                           double val[DIAG_NSTAT], bool cand[DIAG_NSTAT], double add[DIAG_NSTAT])
 {
@@ -1512,14 +1515,13 @@ __device__ __forceinline__ void diag_cell(const Params& p, const double* __restr
   const double W = F(VIRAL_F_T00), X = F(VIRAL_F_T00 + 1), Y = F(VIRAL_F_T00 + 2), Z = F(VIRAL_F_T00 + 3);
   double u[4], pi[10], rows[4], trace;
   for (int m = 0; m < 4; m++) u[m] = F(VIRAL_F_U0 + m);
-  int r0;
-  if (p.milne) {
+  constexpr int r0 = CART ? 4 : 1;
+  if (!CART) {
     // root.h:29-44
     const double pi1 = F(VIRAL_F_PI0), pi2 = F(VIRAL_F_PI0 + 1), pi3 = F(VIRAL_F_PI0 + 2), pi4 = F(VIRAL_F_PI0 + 3);
     const double M2 = (X - pi2) * (X - pi2) + (Y - pi3) * (Y - pi3) + tau * tau * (Z - pi4) * (Z - pi4);
     const double res = e - (W - pi1) + (M2 / ((W - pi1) + P - PI));
     val[0] = fabs(res);
-    r0 = 1;
     for (int l = 0; l < 10; l++) pi[l] = F(VIRAL_F_PI0 + l);
     // root.h:72-90
     rows[0] = pi[0] * u[0] - pi[1] * u[1] - pi[2] * u[2] - pi[3] * u[3] * tau * tau;
@@ -1537,7 +1539,6 @@ __device__ __forceinline__ void diag_cell(const Params& p, const double* __restr
     val[1] = fabs(-(p3 * u2) - p4 * u3 - P * u1 * Q - u1 * (p1 + e * Q - PI * Q) + X * rq);
     val[2] = fabs(-(p3 * u1) - (e + p2 - PI) * u2 - p5 * u3 - (e - PI) * u2 * S3 - P * u2 * Q + Y * rq);
     val[3] = fabs(-(p4 * u1) - p5 * u2 - P * u3 + u3 * (p1 + p2 + PI - W) + Z * rq);
-    r0 = 4;
     cart_pi10(u, p1, p2, p3, p4, p5, pi);
     // multiroot.h:90-104
     rows[0] = pi[0] * u[0] - pi[1] * u[1] - pi[2] * u[2] - pi[3] * u[3];
@@ -1546,21 +1547,24 @@ __device__ __forceinline__ void diag_cell(const Params& p, const double* __restr
     rows[3] = pi[3] * u[0] - pi[6] * u[1] - pi[8] * u[2] - pi[9] * u[3];
     trace = pi[0] - pi[4] - pi[7] - pi[9];
   }
+#pragma unroll
   for (int l = 0; l < 4; l++) val[r0 + l] = fabs(rows[l]);
   val[r0 + 4] = fabs(trace);
+#pragma unroll
   for (int s = 0; s < r0 + 4; s++) { cand[s] = val[s] > 0; add[s] = val[s]; }
   // Cartesian trace: `fabs(trace) > trace` keeps the last negative trace
-  cand[r0 + 4] = p.milne ? val[r0 + 4] > 0 : fabs(trace) > trace;
+  cand[r0 + 4] = !CART ? val[r0 + 4] > 0 : fabs(trace) > trace;
   add[r0 + 4] = val[r0 + 4];
 }
 
-template <int KIND>
+template <int KIND, bool CART>
 __global__ void __launch_bounds__(BLK) k_diag(Geom g, Params p, const double* __restrict__ st, double tau,
                                               DiagPart* __restrict__ out)
 {
   __shared__ DiagPart lds[BLK];
-  const int ns = diag_nstat(KIND, p.milne);
+  constexpr int ns = diag_nstat(KIND, !CART);
   DiagPart acc[DIAG_NSTAT];
+#pragma unroll
   for (int s = 0; s < DIAG_NSTAT; s++) acc[s] = DiagPart{0.0, -1, 0.0};
   const Box b = interior_box(g);
   const long total = (long)b.ni * b.nj * b.nk;
@@ -1570,22 +1574,45 @@ __global__ void __launch_bounds__(BLK) k_diag(Geom g, Params p, const double* __
     const long x = cidx(g, i, j, k);
     double val[DIAG_NSTAT], add[DIAG_NSTAT];
     bool cand[DIAG_NSTAT];
-    diag_cell<KIND>(p, st, g.n, x, tau, val, cand, add);
+    diag_cell<KIND, CART>(p, st, g.n, x, tau, val, cand, add);
+#pragma unroll
     for (int s = 0; s < ns; s++)
-      acc[s] = diag_combine(acc[s], DiagPart{val[s], cand[s] ? (long long)x : -1, add[s]}, diag_rule(KIND, p.milne, s));
+      acc[s] = diag_combine(acc[s], DiagPart{val[s], cand[s] ? (long long)x : -1, add[s]}, diag_rule(KIND, !CART, s));
   }
+#pragma unroll
   for (int s = 0; s < ns; s++) {
     lds[threadIdx.x] = acc[s];
     __syncthreads();
     for (int h = BLK / 2; h > 0; h >>= 1) {
       if ((int)threadIdx.x < h)
-        lds[threadIdx.x] = diag_combine(lds[threadIdx.x], lds[threadIdx.x + h], diag_rule(KIND, p.milne, s));
+        lds[threadIdx.x] = diag_combine(lds[threadIdx.x], lds[threadIdx.x + h], diag_rule(KIND, !CART, s));
       __syncthreads();
     }
     if (threadIdx.x == 0) out[(long)blockIdx.x * DIAG_NSTAT + s] = lds[0];
     __syncthreads();
   }
 }
+
+// Combines the block partials of k_diag in block order, one thread per
+// statistic (the same fold as the host side of run_diag, so the results are
+// equal bit for bit), into out[0..ns).
+template <int KIND>
+__global__ void k_diag_combine(const DiagPart* __restrict__ part, unsigned nb, int milne, DiagPart* __restrict__ out)
+{
+  const int s = (int)threadIdx.x;
+  if (s >= diag_nstat(KIND, milne)) return;
+  DiagPart acc{0.0, -1, 0.0};
+  for (unsigned b = 0; b < nb; b++) acc = diag_combine(acc, part[(size_t)b * DIAG_NSTAT + s], diag_rule(KIND, milne, s));
+  out[s] = acc;
+}
+
+// Device scratch of the allocation-free CheckRoot (viral_check_root_noalloc):
+// the block partials of k_diag (at most DIAG_BLOCKS blocks) followed by the
+// DIAG_NSTAT combined statistics, 110.8 KB per GPU for the life of the
+// process.  Host code never passes this name to a kernel: it passes the
+// address hipGetSymbolAddress returned on that device (scratch_ptr below).
+constexpr int SCRATCH_PARTS = DIAG_BLOCKS * DIAG_NSTAT + DIAG_NSTAT;
+static __device__ __align__(256) DiagPart scratch[SCRATCH_PARTS];
 
 // write.h output records (NEWEST10/write.h:87-103): one thread per double of
 // the record array; record r = ((oi * nyo) + oj) * nzo + ok holds
@@ -2144,6 +2171,7 @@ static const char* const kKernelNames[K_COUNT] = {"flux_x", "flux_y", "flux_eta"
 struct viral_ctx {
   std::vector<Event> ev;
   hipStream_t stream = nullptr;
+  int dev = 0;    // the device current at viral_create (the context's device)
   std::string err;
   unsigned long long* dtmp = nullptr;
   int cur = 0;    // which state copy is current (all events step together)
@@ -2227,6 +2255,26 @@ static int fail(viral_ctx* c, int code, const std::string& msg)
 {
   if (c) c->err = msg;
   return code;
+}
+
+// Per device: the address of `scratch` on that device, resolved once with
+// the device current (the first viral_create on it), and whether that
+// succeeded.
+constexpr int MAX_DEVICES = 64;
+static DiagPart* scratch_ptr[MAX_DEVICES] = {};
+static std::mutex scratch_mu;
+
+static hipError_t resolve_scratch(int dev)
+{
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_ptr[dev]) return hipSuccess;
+  DiagPart* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  scratch_ptr[dev] = ptr;
+  return hipSuccess;
 }
 
 #define HIPCHK(call)                                                                          \
@@ -2676,8 +2724,12 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
   for (int d = 0; d < 3; d++) ctx->cfg.flux_nt[d] = 256;
   {
     int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+    if (hipGetDevice(&dev) != hipSuccess || resolve_scratch(dev) != hipSuccess) {
+      delete ctx;
+      return VIRAL_ERR_HIP;
+    }
+    ctx->dev = dev;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
       ctx->ncu = ncu;
   }
   hipError_t he = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -3114,18 +3166,27 @@ int viral_max_temp_gev(viral_ctx* ctx, int ev, double* out)
 
 // Runs one diagnostic reduction over event E's current state and combines
 // the block partials in block order into res[0..nstat).
+static void launch_diag(viral_ctx* ctx, const Event* E, int kind, unsigned nb, const double* st, double tau,
+                        DiagPart* out)
+{
+  const bool cart = !E->p.milne;
+  if (kind == D_ROOT && !cart)
+    hipLaunchKernelGGL((k_diag<D_ROOT, false>), dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, out);
+  else if (kind == D_ROOT)
+    hipLaunchKernelGGL((k_diag<D_ROOT, true>), dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, out);
+  else if (kind == D_EXTREMA)
+    hipLaunchKernelGGL((k_diag<D_EXTREMA, false>), dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, out);
+  else
+    hipLaunchKernelGGL((k_diag<D_PHYSICS, false>), dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, out);
+}
+
 static int run_diag(viral_ctx* ctx, Event* E, int kind, double tau, DiagPart* res, int* nstat)
 {
   if (!ctx->d_diag) HIPCHK(hipMalloc(&ctx->d_diag, sizeof(DiagPart) * DIAG_BLOCKS * DIAG_NSTAT));
   const long ni = box_count(interior_box(E->g));
   const unsigned nb = (unsigned)std::min<long>(DIAG_BLOCKS, std::max<long>(1, (ni + BLK - 1) / BLK));
   const double* st = E->st[ctx->cur];
-  if (kind == D_ROOT)
-    hipLaunchKernelGGL(k_diag<D_ROOT>, dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, ctx->d_diag);
-  else if (kind == D_EXTREMA)
-    hipLaunchKernelGGL(k_diag<D_EXTREMA>, dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, ctx->d_diag);
-  else
-    hipLaunchKernelGGL(k_diag<D_PHYSICS>, dim3(nb), dim3(BLK), 0, ctx->stream, E->g, E->p, st, tau, ctx->d_diag);
+  launch_diag(ctx, E, kind, nb, st, tau, ctx->d_diag);
   HIPCHK(hipGetLastError());
   std::vector<DiagPart> part((size_t)nb * DIAG_NSTAT);
   HIPCHK(hipMemcpyAsync(part.data(), ctx->d_diag, part.size() * sizeof(DiagPart), hipMemcpyDeviceToHost,
@@ -3157,6 +3218,38 @@ int viral_check_root(viral_ctx* ctx, int ev, double tau, double* max, double* av
   int ns = 0, rc = run_diag(ctx, E, D_ROOT, tau, r, &ns);
   if (rc != VIRAL_OK) return rc;
   const double count = (double)box_count(interior_box(E->g));
+  for (int q = 0; q < ns; q++) {
+    max[q] = r[q].v;
+    avg[q] = r[q].sum / count;
+    if (ijk) cell_ijk(E->g, r[q].at, ijk + 3 * q);
+  }
+  if (nstat) *nstat = ns;
+  return VIRAL_OK;
+}
+
+int viral_check_root_noalloc(viral_ctx* ctx, int ev, double tau, double* max, double* avg, int* ijk, int* nstat)
+{
+  Event* E = get_ev(ctx, ev);
+  if (!E || !max || !avg) return fail(ctx, VIRAL_ERR_ARG, "bad event or pointer");
+  DiagPart* sp = (ctx->dev >= 0 && ctx->dev < MAX_DEVICES) ? scratch_ptr[ctx->dev] : nullptr;
+  if (!sp) return fail(ctx, VIRAL_ERR_HIP, "device scratch of viral_check_root_noalloc not resolved");
+  const long ni = box_count(interior_box(E->g));
+  const unsigned nb = (unsigned)std::min<long>(DIAG_BLOCKS, std::max<long>(1, (ni + BLK - 1) / BLK));
+  const int ns = diag_nstat(D_ROOT, E->p.milne);
+  // the partials of nb blocks and ns statistics must fit the scratch (k_diag
+  // strides over the interior with at most DIAG_BLOCKS blocks, so every grid
+  // fits; the check guards the layout)
+  if ((size_t)nb * DIAG_NSTAT + ns > (size_t)SCRATCH_PARTS || ns > DIAG_NSTAT)
+    return fail(ctx, VIRAL_ERR_ARG, "viral_check_root_noalloc: partials exceed the device scratch");
+  DiagPart* part = sp;
+  DiagPart* res = sp + (size_t)DIAG_BLOCKS * DIAG_NSTAT;
+  launch_diag(ctx, E, D_ROOT, nb, E->st[ctx->cur], tau, part);
+  hipLaunchKernelGGL(k_diag_combine<D_ROOT>, dim3(1), dim3(64), 0, ctx->stream, part, nb, E->p.milne, res);
+  HIPCHK(hipGetLastError());
+  DiagPart r[DIAG_NSTAT];
+  HIPCHK(hipMemcpyAsync(r, res, (size_t)ns * sizeof(DiagPart), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const double count = (double)ni;
   for (int q = 0; q < ns; q++) {
     max[q] = r[q].v;
     avg[q] = r[q].sum / count;
