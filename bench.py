@@ -229,7 +229,8 @@ def ensemble_descs(n_events, seed=4, side=(96, 256), eta=(32, 64)):
     return descs, bs
 
 
-def bench_ensemble(n_events, steps, warmup=2, tau0=0.6, ts=0.002, side=(96, 256), eta=(32, 64), kernel_times=False):
+def bench_ensemble(n_events, steps, warmup=2, tau0=0.6, ts=0.002, side=(96, 256), eta=(32, 64), kernel_times=False,
+                   modes=("plain_default", "batched", "plain")):
     """The same uneven hot-spot ensemble in one context stepped three ways,
     each synchronising once per step:
       plain_default  one launch per event and kernel phase, the library's
@@ -280,21 +281,25 @@ def bench_ensemble(n_events, steps, warmup=2, tau0=0.6, ts=0.002, side=(96, 256)
                        % (n_events, side[0], side[1], eta[0], eta[1], ts),
            "events": n_events, "interior_cells": cells, "steps": steps, "warmup": warmup,
            "ic_seconds": round(t_ic, 1)}
-    h.set_launch(batched=0, graph=0, blocks_per_cu=0, fusion=0, xcd=0)
-    dt = timed(steps)
-    out["plain_default"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps,
-                            "launch": h.launch().as_dict()}
+    if "plain_default" in modes:
+        h.set_launch(batched=0, graph=0, blocks_per_cu=0, fusion=0, xcd=0)
+        dt = timed(steps)
+        out["plain_default"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps,
+                                "launch": h.launch().as_dict()}
     h.set_launch(batched=1, graph=1)
     h.autotune(reps=1)
     tuned = h.launch().as_dict()
     out["launch"] = tuned
     dt = timed(steps)
     out["batched"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps}
-    h.set_launch(batched=0, graph=0, blocks_per_cu=0, xcd=0)
-    out["plain"] = {"value": cells * steps / timed(steps) , "launch": h.launch().as_dict()}
-    out["plain"]["ms_per_step"] = cells / out["plain"]["value"] * 1000.0
-    out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
-    out["batched_over_plain_default"] = out["batched"]["value"] / out["plain_default"]["value"]
+    if "plain" in modes:
+        h.set_launch(batched=0, graph=0, blocks_per_cu=0, xcd=0)
+        dt = timed(steps)
+        out["plain"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps,
+                        "launch": h.launch().as_dict()}
+        out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    if "plain_default" in modes:
+        out["batched_over_plain_default"] = out["batched"]["value"] / out["plain_default"]["value"]
     if kernel_times:
         h.set_launch(**{k: v for k, v in tuned.items() if k != "graph"}, graph=0)
         run(1)
@@ -476,7 +481,7 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
                                        "= e and P(e) (s95p closed form): dJ/de = dJ/dEn + dJ/dT^tt + dJ/dP dP/de"}
 
 
-def bench_c2(steps, warmup, exact_steps, copy_probe=False):
+def bench_c2(steps, warmup, exact_steps):
     """The smooth-Glauber grid: viscous KT, s95p EOS, Glauber b=7 fm, Milne
     200x200x64, dx=deta=0.1, tau0=0.6, dtau=0.002, eta/s=0.08,
     tau_pi=5eta/(e+p), RK3.  Fast arithmetic (s95p table, autotuned, HIP
@@ -510,13 +515,6 @@ def bench_c2(steps, warmup, exact_steps, copy_probe=False):
         dt = time.perf_counter() - t0
         rep = h.timer_report()
         h.timers(False)
-        if copy_probe and mode == "fast":
-            src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
-            dst = torch.empty_like(src)
-            torch.cuda.synchronize()
-            dst.copy_(src)
-            torch.cuda.synchronize()
-            del src, dst
         cells = d.nx * d.ny * d.nz
         fused = launch["fusion"] > 0
         leg = {"value": cells * n / dt, "unit": "cell-updates/s", "ms_per_step": dt * 1000.0 / n, "steps": n,
@@ -580,6 +578,8 @@ def main():
                          "64x64x8), c3 (256x256x96 hot spots, KT and SHASTA), c5 (adjoint), small (small-event "
                          "ensemble), cpu (CPU baseline)")
     ap.add_argument("--ensemble-events", type=int, default=64)
+    ap.add_argument("--ensemble-modes", default="plain_default,batched,plain",
+                    help="ways the headline ensemble is stepped (profiling runs take batched alone)")
     ap.add_argument("--c2-steps", type=int, default=20)
     ap.add_argument("--exact-steps", type=int, default=3, help="timed steps of the exact-arithmetic c2 leg")
     ap.add_argument("--c3-steps", type=int, default=400)
@@ -588,8 +588,8 @@ def main():
     ap.add_argument("--adjoint-exact", action="store_true", help="run the adjoint leg in exact arithmetic")
     ap.add_argument("--small-events", type=int, default=128)
     ap.add_argument("--copy-probe", action="store_true",
-                    help="after the c2 fast leg's timed steps, run one plain 1 GiB device copy so a counter run can "
-                         "scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
+                    help="at the end, run one plain 1 GiB device copy so a counter run can scale "
+                         "FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -609,15 +609,15 @@ def main():
            "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded hot-spot and optical-Glauber initial states generated on the host)"}
     if "c4" in legs:
-        ens = bench_ensemble(args.ensemble_events, args.steps, warmup=args.warmup, kernel_times=True)
+        ens = bench_ensemble(args.ensemble_events, args.steps, warmup=args.warmup, kernel_times=True,
+                             modes=tuple(args.ensemble_modes.split(",")))
         out["value"] = ens["batched"]["value"]
         out["ms_per_step"] = ens["batched"]["ms_per_step"]
         out["config"] = {"workload": ens["workload"], "events": ens["events"], "interior_cells": ens["interior_cells"],
                          "launch": ens["launch"], "parallelism": "single GPU"}
-        out["plain"] = ens["plain"]
-        out["plain_default"] = ens["plain_default"]
-        out["batched_over_plain"] = ens["batched_over_plain"]
-        out["batched_over_plain_default"] = ens["batched_over_plain_default"]
+        for k in ("plain", "plain_default", "batched_over_plain", "batched_over_plain_default"):
+            if k in ens:
+                out[k] = ens[k]
         rep = ens["kernel_times"]
         out["roofline"] = kernel_roofline(rep, 3, ens["interior_cells"], ens["launch"]["fusion"] > 0,
                                           traffic=ENSEMBLE_TRAFFIC.get(max(rep, key=lambda k: rep[k][0])),
@@ -628,7 +628,7 @@ def main():
         out["kernel_time_share"] = {k: round(v[0] / tot, 4) for k, v in rep.items() if v[1]}
         out["ensemble"] = {k: ens[k] for k in ("ic_seconds", "steps", "warmup")}
     if "c2" in legs:
-        out["glauber_200x200x64"] = bench_c2(args.c2_steps, args.warmup, args.exact_steps, args.copy_probe)
+        out["glauber_200x200x64"] = bench_c2(args.c2_steps, args.warmup, args.exact_steps)
     if "c1" in legs:
         out["gubser_64x64x8"] = bench_c1()
     if "c3" in legs and args.c3_steps > 0:
@@ -643,6 +643,13 @@ def main():
                                                side=(24, 64), eta=(8, 24))
     if "cpu" in legs:
         out["cpu_baseline"] = cpu_baseline()
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print(json.dumps(out))
 
 

@@ -264,7 +264,11 @@ __host__ __device__ __forceinline__ bool eos_tab_lookup(const EosTab& t, double 
   const double u = (2.0 * m - 1.0) * t.K;
   int q = (int)u;
   if (q > t.K - 1) q = t.K - 1;
-  if (gload(t.closed, (long)oct * t.K + q)) return false;
+  // The interval's flag and its four points are loaded together and the flag
+  // is tested after the interpolation (a closed interval's points are in the
+  // table too), so a lookup costs one memory round trip rather than two --
+  // the Newton iteration chains one lookup per iterate.
+  const bool closed = gload(t.closed, (long)oct * t.K + q) != 0;
   const double x = u - q;
   const double xm = x - 1.0, xp = x + 1.0, x2 = x - 2.0;
   const double w0 = -(x * xm * x2) * (1.0 / 6.0);
@@ -275,7 +279,7 @@ __host__ __device__ __forceinline__ bool eos_tab_lookup(const EosTab& t, double 
 #pragma unroll
   for (int c = 0; c < 4; c++)
     out[c] = fma(w0, gload(b, c), fma(w1, gload(b, 4 + c), fma(w2, gload(b, 8 + c), w3 * gload(b, 12 + c))));
-  return true;
+  return !closed;
 }
 
 // e-derivatives of the interpolated (p, dp/de, T, s), for the adjoint: the
@@ -493,22 +497,38 @@ __host__ __device__ __forceinline__ double frcp(double x)
 #endif
 }
 
+// 1/x with one Newton-Raphson step: v_rcp_f64 is good to about 2^-24 and the
+// step squares that, to within about ten ulp of the quotient (2.2e-15 relative
+// at most over 4M random normal x of both signs and exponents -300..300).  For
+// the WENO weights' normalisation, where a few ulp do not matter.
+__device__ __forceinline__ double frcp1(double x)
+{
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
 // Same WENO5-JS weights written with one division per reconstruction:
 //   w_c = d_c / t_c / sum  ==  d_c P_c / sum_c d_c P_c,  t_c = (eps+b_c)^2,
 //   P_0 = t_1 t_2, P_1 = t_0 t_2, P_2 = t_0 t_1,
-// the 1/6 of the candidate polynomials folded into the denominator, L and R
-// sharing the smoothness indicators, and FMAs.  Agrees with weno_L/weno_R to
-// a few ulp (tests hold fast mode to 1e-10 relative per step).
+// with the weights' common factors taken out (d = (3, 6, 1)/10 for L and
+// (1, 6, 3)/10 for R; the t_c scaled by 4, which leaves the normalised
+// weights unchanged), the 1/6 of the candidate polynomials folded into the
+// denominator, L and R sharing the smoothness indicators and one reciprocal,
+// and the candidates built from the indicators' differences where that saves
+// an operation (r1 = l1 + 3 c1, r0 = 2 c0 + 5 q + q+1, l2 = 2 c2 + 5 q + q-1).
+// 59 FP64 operations.  Agrees with weno_L/weno_R to about ten ulp (tests hold
+// fast mode to 1e-10 relative per step).
 __device__ __forceinline__ void weno_LR_fast(double qm2, double qm1, double qc, double qp1, double qp2, double& L,
                                              double& R)
 {
   const double a0 = fma(-2.0, qp1, qc) + qp2, c0 = fma(3.0, qc, fma(-4.0, qp1, qp2));
   const double a1 = fma(-2.0, qc, qm1) + qp1, c1 = qm1 - qp1;
   const double a2 = fma(-2.0, qm1, qm2) + qc, c2 = fma(3.0, qc, fma(-4.0, qm1, qm2));
-  const double k = 13.0 / 12.0;
-  double t0 = fma(k * a0, a0, fma(0.25 * c0, c0, WENO_EPS));
-  double t1 = fma(k * a1, a1, fma(0.25 * c1, c1, WENO_EPS));
-  double t2 = fma(k * a2, a2, fma(0.25 * c2, c2, WENO_EPS));
+  // 4 (eps + b_c) = 4 eps + (13/3) a_c^2 + c_c^2
+  const double k4 = 13.0 / 3.0, e4 = 4.0 * WENO_EPS;
+  double t0 = fma(k4 * a0, a0, fma(c0, c0, e4));
+  double t1 = fma(k4 * a1, a1, fma(c1, c1, e4));
+  double t2 = fma(k4 * a2, a2, fma(c2, c2, e4));
   t0 = t0 * t0;
   t1 = t1 * t1;
   t2 = t2 * t2;
@@ -516,17 +536,17 @@ __device__ __forceinline__ void weno_LR_fast(double qm2, double qm1, double qc, 
   // candidate values times 6
   const double l0 = fma(2.0, qc, fma(5.0, qp1, -qp2));
   const double l1 = fma(5.0, qc, fma(2.0, qp1, -qm1));
-  const double l2 = fma(2.0, qm2, fma(-7.0, qm1, 11.0 * qc));
-  const double r0 = fma(11.0, qc, fma(-7.0, qp1, 2.0 * qp2));
-  const double r1 = fma(2.0, qm1, fma(5.0, qc, -qp1));
+  const double l2 = fma(2.0, c2, fma(5.0, qc, qm1));
+  const double r0 = fma(2.0, c0, fma(5.0, qc, qp1));
+  const double r1 = fma(3.0, c1, l1);
   const double r2 = fma(5.0, qm1, fma(2.0, qc, -qm2));
-  const double wl0 = 0.3 * P0, wl1 = 0.6 * P1, wl2 = 0.1 * P2;
-  const double wr0 = 0.1 * P0, wr2 = 0.3 * P2;
-  // one reciprocal for both normalisations: 1/Dl = Dr/(Dl Dr), 1/Dr = Dl/(Dl Dr)
-  const double Dl = 6.0 * (wl0 + wl1 + wl2), Dr = 6.0 * (wr0 + wl1 + wr2);
-  const double iDD = frcp(Dl * Dr);
-  L = fma(wl0, l0, fma(wl1, l1, wl2 * l2)) * (Dr * iDD);
-  R = fma(wr0, r0, fma(wl1, r1, wr2 * r2)) * (Dl * iDD);
+  // L = (3 P0 l0 + 6 P1 l1 + P2 l2) / (6 Dl),  R = (P0 r0 + 6 P1 r1 + 3 P2 r2) / (6 Dr)
+  const double A0 = 3.0 * P0, B1 = 6.0 * P1, C2 = 3.0 * P2;
+  const double Dl = A0 + B1 + P2, Dr = P0 + B1 + C2;
+  // one reciprocal for both: 1/(6 Dl) = Dr/(6 Dl Dr), 1/(6 Dr) = Dl/(6 Dl Dr)
+  const double iDD = frcp1(6.0 * Dl * Dr);
+  L = fma(A0, l0, fma(B1, l1, P2 * l2)) * (Dr * iDD);
+  R = fma(P0, r0, fma(B1, r1, C2 * r2)) * (Dl * iDD);
 }
 
 __device__ __forceinline__ double weno_L_fast(double qm2, double qm1, double qc, double qp1, double qp2)
@@ -663,14 +683,33 @@ __host__ __device__ inline bool root_bracketed(const Params& p, double M2, doubl
   return true;
 }
 
+// eos_pa_fast that keeps the table's (p, dp/de, T, s) at e in tv (and sets
+// have) where the table applies.
+__device__ __forceinline__ void eos_pa_keep(const Params& p, double e, double& P, double& a, double tv[4], bool& have)
+{
+  if (p.eos && p.use_tab && eos_tab_lookup(p.tab, e, tv)) {
+    P = tv[0];
+    a = tv[1];
+    have = true;
+    return;
+  }
+  have = false;
+  eos_pa_fast(p, e, P, a);
+}
+
+// The fast Newton iteration (see eos_pa_fast).  tv and have_tv: the table's
+// values at the returned root when the last evaluation was at that root (every
+// exit but the period shortcut), so the caller's P and T there need no second
+// lookup.
 __device__ __forceinline__ double newton_cell_fast(const Params& p, double M2, double Ttt, double pi1, double PI,
-                                                   double guess, int& iters, unsigned& flags)
+                                                   double guess, int& iters, unsigned& flags, double tv[4],
+                                                   bool& have_tv)
 {
   const double A = Ttt - pi1;
   double x = guess, fv, dfv;
   {
     double Pe, a;
-    eos_pa_fast(p, x, Pe, a);
+    eos_pa_keep(p, x, Pe, a, tv, have_tv);
     const double r = frcp(A + Pe - PI);
     fv = x - A + M2 * r;
     dfv = 1 - (M2 * r * r) * a;
@@ -686,7 +725,7 @@ __device__ __forceinline__ double newton_cell_fast(const Params& p, double M2, d
     }
     double xn = x - fv * frcp(dfv);
     double Pe, a;
-    eos_pa_fast(p, xn, Pe, a);
+    eos_pa_keep(p, xn, Pe, a, tv, have_tv);
     const double r = frcp(A + Pe - PI);
     fv = xn - A + M2 * r;
     dfv = 1 - (M2 * r * r) * a;
@@ -712,6 +751,7 @@ __device__ __forceinline__ double newton_cell_fast(const Params& p, double M2, d
       x = q == 1 ? h1 : q == 2 ? h2 : q == 3 ? h3 : h4;
       it = p.max_iter;
       fl |= 8u;
+      have_tv = false;
       break;
     }
   }

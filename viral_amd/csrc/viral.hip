@@ -78,6 +78,28 @@ __device__ __forceinline__ bool box_cell(const Box& b, long t, int& i, int& j, i
   return true;
 }
 
+// The same box in slab order: slabs of kSlabRows j-rows; inside a slab i is
+// slowest, then j, then k.  The cell kernels read u at i +- 1, 2: in plain
+// order those planes are a whole i-plane of traffic apart and leave L2 before
+// their reuse; in slab order consecutive blocks advance through i within a
+// slab of kSlabRows x nk cells per plane, so the five planes a block reads are
+// the ones its in-flight neighbours read.  Only the slab's two edge rows on
+// each side are read twice.
+constexpr int kSlabRows = 16;
+__device__ __forceinline__ bool slab_cell(const Box& b, long t, int& i, int& j, int& k)
+{
+  const long nk = b.nk, per = (long)b.ni * kSlabRows * nk;
+  if (t >= (long)b.ni * b.nj * nk) return false;
+  const int s = (int)(t / per);
+  const long r = t - s * per;
+  const int rows = min(kSlabRows, b.nj - s * kSlabRows);
+  k = b.k0 + (int)(r % nk);
+  const long q = r / nk;
+  j = b.j0 + s * kSlabRows + (int)(q % rows);
+  i = b.i0 + (int)(q / rows);
+  return true;
+}
+
 // faces c+1/2 along d for c in [lo-1, hi-1], interior in the other directions
 __host__ __device__ inline Box face_box(const Geom& g, int d)
 {
@@ -315,9 +337,11 @@ __global__ void __launch_bounds__(BLK) k_flux_dir(const EvDev* __restrict__ tab,
 // smoothness indicators, and a face c+1/2 takes L from cell c and R from cell
 // c+1 through a second, double-buffered LDS array.  TP-5 faces per line per
 // tile, staged in LDS; each of the TP-6 cells whose two faces both lie in the
-// tile writes its partial result -(1/h)(f(c+1/2) - f(c-1/2)) (the update
-// reads it as it is, with no re-read of a neighbour's face), so consecutive
-// tiles of a line overlap by one face.  The tile shape is a launch
+// tile forms its partial result -(1/h)(f(c+1/2) - f(c-1/2)), so consecutive
+// tiles of a line overlap by one face.  The three directions share one
+// 15-row array: x writes its partial results, y and eta add theirs (the
+// phases run in that order on one stream), and the update reads the sum as it
+// is -- 15 doubles per cell instead of 45.  The tile shape is a launch
 // parameter.  Slots map to cells with the contiguous (eta) index fastest so
 // loads coalesce in every direction.
 //
@@ -511,14 +535,17 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     if (q < 0) continue;  // block-uniform
     __syncthreads();
     // the cell's partial result -(1/h)(f(c+1/2) - f(c-1/2)), as the update
-    // would form it from the face fluxes
+    // would form it from the face fluxes, added to the earlier directions'
+    // (x, then y, then eta: the update's order of summation)
 #pragma unroll
     for (int r = 0; r < MAXR; r++) {
       if (!own_ok[r]) continue;
       const int sl = threadIdx.x + r * BLK;
       const int st1 = D < 2 ? TK : 1;
       const double df = Bf[sl] - Bf[sl - st1];
-      flux[(long)q * n + xs[r]] = -df * (1.0 / hD);
+      const double pr = -df * (1.0 / hD);
+      const long o = (long)q * n + xs[r];
+      flux[o] = D == 0 ? pr : flux[o] + pr;
     }
   }
 }
@@ -533,7 +560,7 @@ __global__ void __launch_bounds__(NT) k_flux_tile(const EvDev* __restrict__ tab,
     long lb;
     const EvDev& E = tab[sched_find(s, w_, lb)];
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
-    flux_tile_body<D, NT, CART>(E.g, E.p, a.tau_src, E.st[cur], E.flux[D], E.tp[D], E.tk[D], lb, lds);
+    flux_tile_body<D, NT, CART>(E.g, E.p, a.tau_src, E.st[cur], E.flux[0], E.tp[D], E.tk[D], lb, lds);
     __syncthreads();  // the next tile may lay its LDS arrays out differently
   }
 }
@@ -763,7 +790,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(FAST ?
     long lb;
     const EvDev& E = tab[sched_find(s, w_, lb)];
     int i, j, k;
-    if (!box_cell(interior_box(E.g), lb * BLK + threadIdx.x, i, j, k)) continue;
+    if (!slab_cell(interior_box(E.g), lb * BLK + threadIdx.x, i, j, k)) continue;
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
     const long x = cidx(E.g, i, j, k), n = E.g.n;
     double S[SV];
@@ -801,7 +828,9 @@ __device__ __forceinline__ void solve_cell(const Params& p, int milne, double tn
                           tn * tn * (T0[3] - pi[3]) * (T0[3] - pi[3])
                     : (T0[1] - pi[1]) * (T0[1] - pi[1]) + (T0[2] - pi[2]) * (T0[2] - pi[2]) +
                           (T0[3] - pi[3]) * (T0[3] - pi[3]);
-  if (FAST) en = newton_cell_fast(p, M2, T0[0], pi[0], PIn, e, it, fl);
+  double tv[4];
+  bool have_tv = false;
+  if (FAST) en = newton_cell_fast(p, M2, T0[0], pi[0], PIn, e, it, fl, tv, have_tv);
   else en = newton_cell(p, M2, T0[0], pi[0], PIn, e, it, fl);
   if ((p.quirks & VIRAL_Q_ROOT_SAFEGUARD) && ((fl & VIRAL_FLAG_SOLVER) || !(en > 0))) {
     double er;
@@ -810,9 +839,14 @@ __device__ __forceinline__ void solve_cell(const Params& p, int milne, double tn
       en = er;
       it = it + ir;
       fl = (fl & ~(VIRAL_FLAG_SOLVER | VIRAL_FLAG_MAXITER)) | VIRAL_FLAG_RESCUED;
+      have_tv = false;
     }
   }
-  if (FAST) {
+  if (FAST && have_tv) {
+    // the table values of the root solve's last evaluation, at en
+    Pn = tv[0];
+    Temp = tv[2];
+  } else if (FAST) {
     double a_, s_;
     eos_pa_fast(p, en, Pn, a_);
     eos_Ts_fast(p, en, Pn, Temp, s_);
@@ -848,7 +882,7 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
                                             uint8_t* __restrict__ flags, unsigned long long* __restrict__ bad, long t)
 {
   int i, j, k;
-  if (!box_cell(interior_box(g), t, i, j, k)) return;
+  if (!slab_cell(interior_box(g), t, i, j, k)) return;
   const long n = g.n, x = cidx(g, i, j, k);
   const int milne = CART ? 0 : p.milne;
   const double tau = a.tau_src;
@@ -862,8 +896,12 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
     for (int l = 0; l < SV; l++) S[l] = (CART && l >= 9 && l < 14) ? 0.0 : src[l * n + x];
   }
   // ---- Result = sum over directions of -(1/h)(f(c+1/2) - f(c-1/2)), x then y then eta
+  // (the fast KT tiles have already summed the three directions into fx)
   double Res[SV];
-  {
+  if (FAST && p.scheme != VIRAL_SHASTA) {
+#pragma unroll
+    for (int l = 0; l < SV; l++) Res[l] = (CART && l >= 9 && l < 14) ? 0.0 : fx[l * n + x];
+  } else {
     const double* fl[3] = {fx, fy, fz};
     const long sd[3] = {g.sx, g.sy, 1};
     const double hh[3] = {p.dx, p.dy, p.deta};
@@ -878,8 +916,8 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
       for (int l = 0; l < SV; l++) {
         if (CART && l >= 9 && l < 14) continue;
         double pr;
-        if (FAST || p.scheme == VIRAL_SHASTA) {
-          pr = f[l * n + x];  // the fast KT tiles and SHASTA store the partial result itself
+        if (p.scheme == VIRAL_SHASTA) {
+          pr = f[l * n + x];  // SHASTA stores the partial result itself
         } else {
           const double df = f[l * n + x] - f[l * n + x - sd[dd]];
           pr = -(1.0 / hh[dd]) * (df);
@@ -896,6 +934,12 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
     pu[l] = si[(VIRAL_F_PU0 + l) * n + x];
   }
   // ---- RK combination (NEWEST10/time.h:2-98)
+  // Fast mode keeps only Var0: the Var of the previous stage (Var1 before the
+  // RK2 final and the RK3 middle stage, Var2 before the RK3 final one) is the
+  // stage-start state's own T^{tau mu} tau_src, pi and PI -- the stage before
+  // stored T = Var / tau_new with tau_new = this stage's tau_src -- to within
+  // two roundings of the T rows, instead of a 15-double copy written and read
+  // back.
   double V[SV];
 #pragma unroll
   for (int l = 0; l < SV; l++) {
@@ -904,21 +948,22 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
       continue;
     }
     double L = (Res[l] + S[l]);
+    // the stage-start state's Var (Var0 at stage 0)
+    const double vs = l < 4 ? (milne ? tau * si[(VIRAL_F_T00 + l) * n + x] : si[(VIRAL_F_T00 + l) * n + x])
+                            : si[(VIRAL_F_T00 + l) * n + x];
     if (a.kind == 0) {
-      double v0 = l < 4 ? (milne ? tau * si[(VIRAL_F_T00 + l) * n + x] : si[(VIRAL_F_T00 + l) * n + x])
-                        : si[(VIRAL_F_T00 + l) * n + x];
-      double v1 = FAST ? fma(a.ts, L, v0) : v0 + a.ts * (L);
-      var0[l * n + x] = v0;
-      var1[l * n + x] = v1;
+      double v1 = FAST ? fma(a.ts, L, vs) : vs + a.ts * (L);
+      var0[l * n + x] = vs;
+      if (!FAST) var1[l * n + x] = v1;
       V[l] = v1;
     } else if (a.kind == 1) {
-      V[l] = 0.5 * var0[l * n + x] + 0.5 * var1[l * n + x] + 0.5 * a.ts * L;
+      V[l] = 0.5 * var0[l * n + x] + 0.5 * (FAST ? vs : var1[l * n + x]) + 0.5 * a.ts * L;
     } else if (a.kind == 2) {
-      double v2 = 0.75 * var0[l * n + x] + 0.25 * var1[l * n + x] + 0.25 * a.ts * L;
-      var2[l * n + x] = v2;
+      double v2 = 0.75 * var0[l * n + x] + 0.25 * (FAST ? vs : var1[l * n + x]) + 0.25 * a.ts * L;
+      if (!FAST) var2[l * n + x] = v2;
       V[l] = v2;
     } else {
-      V[l] = (1.0 / 3.0) * var0[l * n + x] + (2.0 / 3.0) * var2[l * n + x] + (2.0 / 3.0) * a.ts * L;
+      V[l] = (1.0 / 3.0) * var0[l * n + x] + (2.0 / 3.0) * (FAST ? vs : var2[l * n + x]) + (2.0 / 3.0) * a.ts * L;
     }
   }
   if (CART) {
@@ -2779,10 +2824,14 @@ int viral_create(const viral_desc* descs, int nev, viral_ctx** out)
     size_t n = (size_t)g.n;
     ok = ok && hipMalloc(&E.st[0], NF * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.st[1], NF * n * sizeof(double)) == hipSuccess;
+    // fast mode keeps Var0 only (the stage-start state stands in for Var1 and
+    // Var2), and its KT tiles sum the three directions into flux[0]
+    const bool fast = descs[0].fast_math != 0;
     ok = ok && hipMalloc(&E.var0, SV * n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&E.var1, SV * n * sizeof(double)) == hipSuccess;
-    if (d->rk == 3) ok = ok && hipMalloc(&E.var2, SV * n * sizeof(double)) == hipSuccess;
-    for (int q = 0; q < (d->lbi ? 2 : 3); q++) ok = ok && hipMalloc(&E.flux[q], SV * n * sizeof(double)) == hipSuccess;
+    if (!fast) ok = ok && hipMalloc(&E.var1, SV * n * sizeof(double)) == hipSuccess;
+    if (!fast && d->rk == 3) ok = ok && hipMalloc(&E.var2, SV * n * sizeof(double)) == hipSuccess;
+    const int nflux = (fast && d->scheme == VIRAL_KT) ? 1 : d->lbi ? 2 : 3;
+    for (int q = 0; q < nflux; q++) ok = ok && hipMalloc(&E.flux[q], SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.src, SV * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&E.iters, 3 * n * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc(&E.flags, n) == hipSuccess;
@@ -2839,7 +2888,8 @@ void viral_destroy(viral_ctx* ctx)
   ctx->adj.slabs.clear();
   ctx->adj.stages.clear();
   for (Event& E : ctx->ev) {
-    hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0); hipFree(E.var1);
+    hipFree(E.st[0]); hipFree(E.st[1]); hipFree(E.var0);
+    if (E.var1) hipFree(E.var1);
     if (E.var2) hipFree(E.var2);
     for (int q = 0; q < 3; q++)
       if (E.flux[q]) hipFree(E.flux[q]);
