@@ -59,7 +59,8 @@ enum { VIRAL_TPIB_CONST = 0,       /* tau_PI = c               */
                                            (NEWEST10/root.h:426-449) */
 #define VIRAL_Q_DPDE_OF_P     (1u << 2) /* q3: a = DPDE(P) in the CON eigenvalues (NEWEST10/maindef.h:490) */
 #define VIRAL_Q_GUBSER_MIRROR (1u << 3) /* q7: mirror u^1,u^2 into x/y ghosts (NEWEST10/boundary.h:32-37,77-81,
-                                           118-122,160-165); off by default: only the GUBSER profile has it */
+                                           118-122,160-165); off by default: only the GUBSER profile has it;
+                                           needs nx, ny >= 3 (viral_create refuses narrower Milne grids) */
 #define VIRAL_Q_SHEAR_SLICES  (1u << 4) /* q9, Cartesian only: shear sources only in ghosted eta slices k < nz,
                                            as the reference's loop k = 0..ZCMA-1 (TXYZ5/Source.h:272); the top 3
                                            interior slices keep the zero sources of the freshly allocated grid */

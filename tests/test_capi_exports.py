@@ -38,3 +38,17 @@ def test_desc_default_and_dims_without_gpu():
     gx, gy, gz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     lib.viral_ghost_dims(ctypes.byref(d), ctypes.byref(gx), ctypes.byref(gy), ctypes.byref(gz))
     assert (gx.value, gy.value, gz.value) == (70, 70, 1)
+
+
+def test_narrow_mirror_grid_refused_without_gpu():
+    """Under the Gubser mirror quirk ghost layer h of U1, U2 copies the
+    (2-h)-th cell from the face; with fewer than 3 interior cells that cell is
+    a ghost the same fill writes, so viral_create refuses the grid (it checks
+    the descriptors before touching the device)."""
+    from viral_amd.desc import make_desc, Q_DEFAULT, Q_GUBSER_MIRROR, ERR_ARG
+    lib = capi.load()
+    h = ctypes.c_void_p()
+    for nx, ny in ((2, 9), (9, 1)):
+        d = make_desc(nx, ny, 1, lbi=1, quirks=Q_DEFAULT | Q_GUBSER_MIRROR)
+        assert lib.viral_create(ctypes.byref(d), 1, ctypes.byref(h)) == ERR_ARG
+        assert not h.value

@@ -2730,6 +2730,12 @@ static int check_desc(viral_ctx* ctx, const viral_desc* d)
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "boost invariance needs Milne coordinates");
   if (d->eos != VIRAL_EOS_CONFORMAL && d->eos != VIRAL_EOS_S95P) return fail(ctx, VIRAL_ERR_ARG, "unknown eos");
   if (d->max_iter < 1) return fail(ctx, VIRAL_ERR_ARG, "max_iter must be >= 1");
+  // the mirrored ghost layer h of U1, U2 copies the (B-1-h)-th cell from the
+  // face: on a grid narrower than the layers that is a ghost cell the same
+  // fill writes, an order the parallel fill does not reproduce
+  if ((d->quirks & VIRAL_Q_GUBSER_MIRROR) && d->coords == VIRAL_MILNE &&
+      (d->nx < VIRAL_BORDER || d->ny < VIRAL_BORDER))
+    return fail(ctx, VIRAL_ERR_ARG, "the mirror quirk needs at least 3 interior cells in x and y");
   if (d->bulk && (d->zeta_law < VIRAL_ZETA_CONST || d->zeta_law > VIRAL_ZETA_S_OVER_4PI ||
                   d->tpib_law < VIRAL_TPIB_CONST || d->tpib_law > VIRAL_TPIB_OVER_P))
     return fail(ctx, VIRAL_ERR_ARG, "unknown bulk viscosity or bulk relaxation law");
@@ -3510,12 +3516,35 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps)
       bool any = false;
       for (int e = 0; e < nev; e++) any = any || !(d == 2 && ctx->ev[e].d.lbi);
       if (!any) continue;
-      float best = 1e30f;
       int bnt = ctx->cfg.flux_nt[d];
       std::vector<int> btp(nev), btk(nev);
       for (int e = 0; e < nev; e++) {
         btp[e] = ctx->ev[e].tile_tp[d];
         btk[e] = ctx->ev[e].tile_tk[d];
+      }
+      // the configuration in place is the first candidate: tuning never
+      // leaves a slower one behind
+      float best = 1e30f;
+      {
+        ctx->dirty = true;
+        if (int r = refresh_table(ctx)) return r;
+        int r;
+        if (ev >= 0) {
+          Sched s{nullptr, ctx->cnt[PH_FLUX_X + d][ev], ev, 1, ctx->cfg.xcd};
+          const size_t shm = kTileArrays * (size_t)ctx->ev[ev].tile_tp[d] * ctx->ev[ev].tile_tk[d] * sizeof(double);
+          const unsigned grid = grid_for(ctx, s.total);
+          r = VIRAL_OK;
+          with_bool(ctx->cfg.blocks_per_cu > 0, [&](auto PT) {
+            with_bool(ctx->cart, [&](auto CT) {
+              r = time_ms(ctx, reps, &best, [&] {
+                launch_tile<decltype(PT)::value, decltype(CT)::value>(ctx, d, bnt, s, grid, shm, 0, cur);
+              });
+            });
+          });
+        } else {
+          r = time_ms(ctx, reps, &best, [&] { launch_flux(ctx, d, 0, cur); });
+        }
+        if (r) return r;
       }
       for (int nt : nts) {
         if (ev >= 0 && nt != ctx->cfg.flux_nt[d]) continue;  // one event: keep the context's block size
@@ -3529,10 +3558,9 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps)
             if (d == 2 && E.d.lbi) continue;  // no eta phase
             const int G = d == 0 ? E.g.gx : d == 1 ? E.g.gy : E.g.gz;
             int tp = sh[0], tk = sh[1];
-            if (tp == 0) {
+            if (tp == 0) {  // as many whole lines as the block's slots hold
               tp = std::max(8, G);
-              tk = 1;
-              while ((long)tp * tk * 2 <= max_slots(nt)) tk *= 2;
+              tk = std::max(1, max_slots(nt) / tp);
             }
             if (!tile_ok(tp, tk, nt) || (tp > G + 8 && sh[0] != 0)) {
               valid = false;
