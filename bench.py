@@ -39,6 +39,13 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 IDEAL_BYTES_PER_UPDATE = {1: 336, 2: 784, 3: 1232}
 
 
+def _progress(msg):
+    """One line on stderr when a leg of the run ends (the JSON line goes to
+    stdout at the very end)."""
+    sys.stderr.write("[bench] %s\n" % msg)
+    sys.stderr.flush()
+
+
 def workload_desc(rk=3, fast=1):
     from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP
     return make_desc(200, 200, 64, dx=0.1, dy=0.1, deta=0.1, rk=rk, eos=EOS_S95P,
@@ -203,6 +210,7 @@ def cpu_baseline(seconds_target=12.0, max_steps=3):
         return cells * steps / dt, steps
 
     one, n1 = timed(1)
+    _progress("CPU one thread: %.3g cell-updates/s" % one)
     allc, na = timed(threads)
     sample = ("%d x %d x 16 cells (the central 16 eta slices of the ensemble's first event, hot spots, s95p closed "
               "form, RK3, dtau=0.002), CPU oracle built with %s" % (d.nx, d.ny, flags))
@@ -286,18 +294,21 @@ def bench_ensemble(n_events, steps, warmup=2, tau0=0.6, ts=0.002, side=(96, 256)
         dt = timed(steps)
         out["plain_default"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps,
                                 "launch": h.launch().as_dict()}
+        _progress("%d events: plain_default %.3g cell-updates/s" % (n_events, out["plain_default"]["value"]))
     h.set_launch(batched=1, graph=1)
     h.autotune(reps=1)
     tuned = h.launch().as_dict()
     out["launch"] = tuned
     dt = timed(steps)
     out["batched"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps}
+    _progress("%d events: batched %.3g cell-updates/s" % (n_events, out["batched"]["value"]))
     if "plain" in modes:
         h.set_launch(batched=0, graph=0, blocks_per_cu=0, xcd=0)
         dt = timed(steps)
         out["plain"] = {"value": cells * steps / dt, "ms_per_step": dt * 1000.0 / steps,
                         "launch": h.launch().as_dict()}
         out["batched_over_plain"] = out["batched"]["value"] / out["plain"]["value"]
+        _progress("%d events: plain %.3g cell-updates/s" % (n_events, out["plain"]["value"]))
     if "plain_default" in modes:
         out["batched_over_plain_default"] = out["batched"]["value"] / out["plain_default"]["value"]
     if kernel_times:
@@ -398,6 +409,7 @@ def bench_c3(steps, tau0=0.6, ts=0.002):
         leg["roofline"] = kernel_roofline(rep, d.rk, cells, leg["launch"]["fusion"] > 0)
         out[scheme] = leg
         h.close()
+        _progress("256x256x96 %s: %.3g cell-updates/s" % (scheme, leg["value"]))
     return out
 
 
@@ -455,6 +467,7 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
         h.timers(False)
         h.adjoint_end()
         runs[ev] = (t_fwd, t_adj, grad, pg, dev_ms)
+        _progress("adjoint every=%d: %.0f ms (forward %.0f ms)" % (ev, t_adj * 1e3, t_plain * 1e3))
     h.close()
     dref = d.copy()
     dref.fast_math = dref.eos_table = 0
@@ -531,6 +544,7 @@ def bench_c2(steps, warmup, exact_steps):
                            "frac": leg["value"] * IDEAL_BYTES_PER_UPDATE[d.rk] / 1e9 / HBM_PEAK_GBS}
         out[mode] = leg
         h.close()
+        _progress("200x200x64 %s: %.3g cell-updates/s" % (mode, leg["value"]))
     return out
 
 
