@@ -66,23 +66,31 @@ def eos_p_vec(desc):
 # once and every output written once over the cells/faces of one launch.
 #   flux_x/_y/_eta, fast LDS tiles (per interior cell): 21 state fields read
 #     (e, P, PI, u^mu, pi^{mu nu}, T^{tau mu}) + the cell's 15 partial results
-#     written = 36 doubles = 288 B; exact per-face kernel: the same 36 doubles
-#     per face (15 face fluxes written)
+#     written (x) or read, added to and written back (y, eta: the tiles sum
+#     the three directions into one array) = 36 or 51 doubles; exact per-face
+#     kernel: 21 state fields read + 15 face fluxes written per face
 #   source (per interior cell): e, P, PI, u (4), prevu (4), pi (10) read,
 #     15 sources written = 36 doubles = 288 B
-#   update (per interior cell): 45 partial results, 29 state fields written,
-#     plus per stage kind the RK registers: first stage writes Var0, Var1
-#     (30); RK2 final reads Var0, Var1 (30); RK3 middle reads Var0, Var1 and
-#     writes Var2 (30 + 15); RK3 final reads Var0, Var2 (30); + 4 B
-#     iteration count, 1 B flags (+1 B read after the first stage).
-#     Sources in their own kernel: 15 sources + e, u, prevu (9) read, and the
-#     first stage reads the 15 evolved fields.  Sources fused in: the
-#     source's 21 inputs read, and the first stage reads T^{tau mu} (4; pi
-#     and PI are among the 21).  Averaged over the stages.
-def update_doubles(rk, fused):
-    base = 45 + 29 + (21 if fused else 15 + 9)
-    first = (4 if fused else 15) + 30
-    kinds = {1: [first], 2: [first, 30], 3: [first, 30 + 15, 30]}[rk]
+#   update (per interior cell): the partial results (fast KT: the 15 summed
+#     ones; exact KT and SHASTA: 15 per direction), 29 state fields written,
+#     plus per stage kind the RK registers.  Exact mode: the first stage
+#     writes Var0, Var1 (30); RK2 final reads Var0, Var1 (30); RK3 middle
+#     reads Var0, Var1 and writes Var2 (30 + 15); RK3 final reads Var0, Var2
+#     (30).  Fast mode keeps Var0 only: written by the first stage (15), read
+#     by the later ones (15), which take the previous stage's Var from their
+#     input state.  Every stage reads its input state's Var: T^{tau mu} (4)
+#     when the sources are fused in (pi and PI are among the source's 21
+#     inputs), the 15 evolved fields otherwise.  Sources in their own kernel:
+#     15 sources + e, u, prevu (9) read.  + 4 B iteration count, 1 B flags
+#     (+1 B read after the first stage).  Averaged over the stages.
+def update_doubles(rk, fused, fast=True, kt=True):
+    partial = 15 if (fast and kt) else 45
+    base = partial + 29 + (21 if fused else 15 + 9)
+    own = 4 if fused else 15
+    if fast:
+        kinds = [own + 15] + [own + 15] * (rk - 1)
+    else:
+        kinds = {1: [own + 30], 2: [own + 30, 30], 3: [own + 30, 30 + 15, 30]}[rk]
     return [base + k for k in kinds]
 
 
@@ -91,11 +99,13 @@ def kernel_bytes(name, d, fused):
     cells = d.nx * d.ny * d.nz
     if name.startswith("flux_"):
         q = {"flux_x": 0, "flux_y": 1, "flux_eta": 2}[name]
+        if d.fast_math and d.scheme == 0:
+            return cells * (288.0 if q == 0 else 408.0)
         return (cells if d.fast_math else faces[q]) * 288.0
     if name == "source":
         return cells * 288.0
     if name == "update":
-        per = update_doubles(d.rk, fused)
+        per = update_doubles(d.rk, fused, bool(d.fast_math), d.scheme == 0)
         avg = sum(8.0 * x + 5.0 + (1.0 if q else 0.0) for q, x in enumerate(per)) / len(per)
         return cells * avg
     if name == "k0_broadcast":
@@ -109,10 +119,10 @@ def kernel_bytes(name, d, fused):
 # profiles/counters_kt_s95p_glauber_200x200x64_rk3.md lists the passes.
 # The update kernel's figure is for its fused form (sources evaluated inside).
 MEASURED_TRAFFIC = {
-    "update_fused": 3.784e9,
-    "flux_x": 1.114e9,
-    "flux_y": 1.098e9,
-    "flux_eta": 0.965e9,
+    "update_fused": 2.035e9,
+    "flux_x": 1.008e9,
+    "flux_y": 1.288e9,
+    "flux_eta": 1.269e9,
     "k0_broadcast": 0.168e9,
 }
 
@@ -123,10 +133,17 @@ MEASURED_TRAFFIC = {
 # 1.00 x 1024, the 1 GiB copy's calibration) and FP64 VALU work per launch
 # (64 lanes x (2 SQ_INSTS_VALU_FMA_F64 + SQ_INSTS_VALU_MUL_F64 +
 # SQ_INSTS_VALU_ADD_F64 + SQ_INSTS_VALU_TRANS_F64)), VALU wave-instructions per
-# launch and the VALU issue fraction (SQ_INSTS_VALU / (256 x GRBM_GUI_ACTIVE)).
-# Filled from the profile of the same bench leg; None until measured.
-ENSEMBLE_TRAFFIC = {}
-ENSEMBLE_COUNTERS = {}
+# launch and the VALU issue fraction (SQ_INSTS_VALU / (256 x GRBM_GUI_ACTIVE
+# per XCD): the chip issues at most one FP64 wave-instruction per CU and
+# cycle).  Averages over the six launches of the two steps the bench times
+# per kernel; profiles/counters_ensemble_64_hotspot_events_rk3.md lists the
+# passes.
+ENSEMBLE_TRAFFIC = {"update": 94.02e9}
+ENSEMBLE_COUNTERS = {
+    "update": {"fp64_flops": 2.554e11, "valu_wave_instructions": 4.337e9, "valu_issue_frac": 0.278,
+               "source": "rocprofv3 --pmc passes on bench.py --legs c4 (profiles/counters_ensemble_64_hotspot_"
+                         "events_rk3.md)"},
+}
 
 
 def ensemble_counters(kernel, avg_s):

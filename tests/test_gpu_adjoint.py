@@ -2,9 +2,11 @@
 
 J = <seed, final state> after a few steps; the adjoint's dJ/d(initial state)
 and dJ/d(eta_coef, tpi_coef) are checked along random directions against
-central differences of the GPU forward (h = 1e-6 relative) and of the CPU
-oracle's forward (the reference gradient); the torch.autograd harness is
-checked on the momentum anisotropy of a Glauber-like event.
+four-point central differences (fourth order, steps h and 2h, h = 1e-5
+relative, so the truncation error sits far below the bar) of the GPU forward
+and of the CPU oracle's forward (the reference gradient), to 1e-6 relative;
+the torch.autograd harness is checked on the momentum anisotropy of a
+Glauber-like event.
 """
 import numpy as np
 import pytest
@@ -20,6 +22,17 @@ pytestmark = pytest.mark.gpu
 TAU, TS = 1.0, 0.005
 # fields the step reads from its input state (Temp and the 3-velocity are outputs only)
 READ = [f for f in range(NFIELDS) if f != F_TEMP and not (F_VX <= f < F_VX + 3)]
+H = 1e-5   # relative step of the differences
+BAR = 1e-6
+
+
+def fd4(J, h=H):
+    """dJ/dt at t = 0 from J(+-h), J(+-2h): (8 (J(h) - J(-h)) - (J(2h) - J(-2h))) / (12 h)"""
+    return (8.0 * (J(h) - J(-h)) - (J(2 * h) - J(-2 * h))) / (12.0 * h)
+
+
+def close(ad, fd, bar=BAR):
+    return abs(ad - fd) <= bar * abs(fd)
 
 
 def run_gpu(d, st, nsteps, every=None, seed=None):
@@ -77,13 +90,10 @@ def test_adjoint_state_gradient_matches_fd(rk, every):
     out, grad, pg = run_gpu(d, st, 3, every, seed)
     for trial in range(3):
         dx = direction(st, rng)
-        h = 1e-6
-        Jp = np.sum(seed * run_gpu(d, st + h * dx, 3))
-        Jm = np.sum(seed * run_gpu(d, st - h * dx, 3))
-        fd = (Jp - Jm) / (2 * h)
+        fd = fd4(lambda h: np.sum(seed * run_gpu(d, st + h * dx, 3)))
         ad = np.sum(grad * dx)
-        print(trial, ad, fd, flush=True)
-        assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+        print(trial, ad, fd, abs(ad - fd) / abs(fd), flush=True)
+        assert close(ad, fd), (ad, fd)
 
 
 def test_adjoint_parameter_gradients_match_fd_and_cpu_reference():
@@ -94,23 +104,23 @@ def test_adjoint_parameter_gradients_match_fd_and_cpu_reference():
     out, grad, pg = run_gpu(d, st, 3, 2, seed)
     for q, name in enumerate(["eta_coef", "tpi_coef"]):
         c0 = getattr(d, name)
-        h = 1e-6 * c0
-        vals = []
-        for sgn in (1, -1):
+
+        def J(h, run):
             dd = d.copy()
-            setattr(dd, name, c0 + sgn * h)
-            vals.append((np.sum(seed * run_gpu(dd, st, 3)), np.sum(seed * run_cpu(dd, st, 3))))
-        fd_gpu = (vals[0][0] - vals[1][0]) / (2 * h)
-        fd_cpu = (vals[0][1] - vals[1][1]) / (2 * h)
+            setattr(dd, name, c0 * (1 + h))
+            return np.sum(seed * run(dd, st, 3))
+
+        fd_gpu = fd4(lambda h: J(h, run_gpu)) / c0
+        fd_cpu = fd4(lambda h: J(h, run_cpu)) / c0
         print(name, pg[q], fd_gpu, fd_cpu, flush=True)
-        assert abs(pg[q] - fd_gpu) <= 1e-5 * abs(fd_gpu), (name, pg[q], fd_gpu)
-        assert abs(pg[q] - fd_cpu) <= 1e-5 * abs(fd_cpu), (name, pg[q], fd_cpu)
+        assert close(pg[q], fd_gpu), (name, pg[q], fd_gpu)
+        assert close(pg[q], fd_cpu), (name, pg[q], fd_cpu)
     # the state gradient against the CPU oracle's finite differences
     dx = direction(st, rng)
-    h = 1e-6
-    fd = (np.sum(seed * run_cpu(d, st + h * dx, 3)) - np.sum(seed * run_cpu(d, st - h * dx, 3))) / (2 * h)
+    fd = fd4(lambda h: np.sum(seed * run_cpu(d, st + h * dx, 3)))
     ad = np.sum(grad * dx)
-    assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+    print("state vs oracle", ad, fd, flush=True)
+    assert close(ad, fd), (ad, fd)
 
 
 def test_gubser_profile_adjoint_matches_fd():
@@ -125,11 +135,10 @@ def test_gubser_profile_adjoint_matches_fd():
     _, grad, pg = run_gpu(d, st, 3, 2, seed)
     for trial in range(2):
         dx = direction(st, rng)
-        h = 1e-6
-        fd = (np.sum(seed * run_gpu(d, st + h * dx, 3)) - np.sum(seed * run_gpu(d, st - h * dx, 3))) / (2 * h)
+        fd = fd4(lambda h: np.sum(seed * run_gpu(d, st + h * dx, 3)))
         ad = np.sum(grad * dx)
         print(trial, ad, fd, flush=True)
-        assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+        assert close(ad, fd), (ad, fd)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
@@ -182,10 +191,10 @@ def test_fast_mode_adjoint_matches_fd():
     print("case adjoint fast", flush=True)
     _, grad, pg = run_gpu(d, st, 2, 2, seed)
     dx = direction(st, rng)
-    h = 1e-6
-    fd = (np.sum(seed * run_gpu(d, st + h * dx, 2)) - np.sum(seed * run_gpu(d, st - h * dx, 2))) / (2 * h)
+    fd = fd4(lambda h: np.sum(seed * run_gpu(d, st + h * dx, 2)))
     ad = np.sum(grad * dx)
-    assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+    print("fast", ad, fd, flush=True)
+    assert close(ad, fd), (ad, fd)
 
 
 @pytest.mark.parametrize("fast", [0, 1], ids=["closed_form", "table"])
@@ -202,25 +211,24 @@ def test_s95p_adjoint_matches_fd(fast):
     _, grad, pg = run_gpu(d, st, 3, 2, seed)
     for trial in range(2):
         dx = direction(st, rng)
-        # the s95p step is nonlinear enough that central differences at
-        # h = 1e-6 carry ~1e-6 truncation error (it falls as h^2 to a ~1e-8
-        # rounding floor at h = 1e-7)
-        h = 1e-7
-        fd = (np.sum(seed * run_gpu(d, st + h * dx, 3)) - np.sum(seed * run_gpu(d, st - h * dx, 3))) / (2 * h)
+        # the s95p step is nonlinear enough that two-point differences at
+        # h = 1e-6 carry ~1e-6 truncation error; the four-point difference's
+        # error falls as h^4
+        fd = fd4(lambda h: np.sum(seed * run_gpu(d, st + h * dx, 3)), h=3e-6)
         ad = np.sum(grad * dx)
-        print(trial, ad, fd, flush=True)
-        assert abs(ad - fd) <= 1e-5 * (abs(fd) + abs(ad)), (ad, fd)
+        print(trial, ad, fd, abs(ad - fd) / abs(fd), flush=True)
+        assert close(ad, fd), (ad, fd)
     for q, name in enumerate(["eta_coef", "tpi_coef"]):
         c0 = getattr(d, name)
-        hh = 1e-6 * c0
-        vals = []
-        for sgn in (1, -1):
+
+        def J(h):
             dd = d.copy()
-            setattr(dd, name, c0 + sgn * hh)
-            vals.append(np.sum(seed * run_gpu(dd, st, 3)))
-        fd = (vals[0] - vals[1]) / (2 * hh)
+            setattr(dd, name, c0 * (1 + h))
+            return np.sum(seed * run_gpu(dd, st, 3))
+
+        fd = fd4(J) / c0
         print(name, pg[q], fd, flush=True)
-        assert abs(pg[q] - fd) <= 1e-5 * abs(fd), (name, pg[q], fd)
+        assert close(pg[q], fd), (name, pg[q], fd)
 
 
 def test_torch_harness_momentum_anisotropy():
@@ -245,18 +253,16 @@ def test_torch_harness_momentum_anisotropy():
     J.backward()
     with torch.no_grad():
         de = torch.from_numpy(rng.standard_normal(e0.shape[0])) * e0
-        h = 1e-6
-        fd = (objective(e0 + h * de, ce, ct) - objective(e0 - h * de, ce, ct)) / (2 * h)
+        fd = float(fd4(lambda h: objective(e0 + h * de, ce, ct)))
         ad = float((e.grad * de).sum())
-        print("e-field", ad, float(fd), flush=True)
-        assert abs(ad - fd) <= 1e-5 * abs(fd) + 1e-12
+        print("e-field", ad, fd, flush=True)
+        assert abs(ad - fd) <= BAR * abs(fd) + 1e-12
         for t, g in ((ce, ce.grad), (ct, ct.grad)):
-            hh = 1e-6 * float(t)
             p = t.detach().clone()
-            fd = (objective(e0, p + hh if t is ce else ce, p + hh if t is ct else ct) -
-                  objective(e0, p - hh if t is ce else ce, p - hh if t is ct else ct)) / (2 * hh)
-            print("param", float(g), float(fd), flush=True)
-            assert abs(float(g) - float(fd)) <= 1e-5 * abs(float(fd)) + 1e-12
+            fd = float(fd4(lambda h: objective(e0, p * (1 + h) if t is ce else ce, p * (1 + h) if t is ct else ct)))
+            fd /= float(p)
+            print("param", float(g), fd, flush=True)
+            assert abs(float(g) - fd) <= BAR * abs(fd) + 1e-12
 
 
 def test_closing_a_recorded_context_frees_its_stage_states():
