@@ -399,12 +399,14 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps);
  * applies hand-written adjoint kernels (KT flux and WENO reconstruction,
  * sources, EOS -- the closed forms or the derivative of the table's
  * interpolant --, velocities; the root by the implicit-function theorem at
- * the converged root).  seed[f] holds dJ/d(final state field f) (VIRAL_NFIELDS
+ * the converged root: the 1-D Newton of the Milne step, the 4-D gnewton of the
+ * Cartesian one).  seed[f] holds dJ/d(final state field f) (VIRAL_NFIELDS
  * ghosted arrays), grad[f] receives dJ/d(recorded initial state field f), each
  * field taken as an independent input, and pgrad[0], pgrad[1] receive dJ/d
- * eta_coef and dJ/d tpi_coef.  The state is left as it was.  Single-event
- * Milne KT contexts, conformal or s95p EOS.  viral_adjoint_end frees the
- * recording.
+ * eta_coef and dJ/d tpi_coef.  The state is left as it was.  Single-event KT
+ * contexts in Milne or Cartesian coordinates, conformal or s95p EOS (not
+ * SHASTA, the bulk source or the FIX build: VIRAL_ERR_UNSUPPORTED).
+ * viral_adjoint_end frees the recording.
  *
  * every == 0 keeps every stage's output state instead of checkpoints (the
  * stages write straight into the recording; no recompute in viral_adjoint):

@@ -211,4 +211,83 @@ void ah_update_vjp(int eos, double tn, const double* V, double e, const double* 
   update_vjp(p, tn, T0, pi, V[14], e, ob[0], ob[1], ob[2], ob + 3, ob + 6, ob + 10, ob + 14, ob[24], Vb);
 }
 
+// ---- Cartesian (TXYZ5)
+void ah_cart_pi10(const double* u, const double* pe, double* out)
+{
+  cart_pi10(u, pe[0], pe[1], pe[2], pe[3], pe[4], out);
+}
+void ah_cart_pi10_vjp(const double* u, const double* pe, const double* ob, double* inb /* u4, pe5 */)
+{
+  double ub[4] = {0, 0, 0, 0}, peb[5] = {0, 0, 0, 0, 0};
+  cart_pi10_vjp(u, pe, ob, ub, peb);
+  for (int l = 0; l < 4; l++) inb[l] = ub[l];
+  for (int l = 0; l < 5; l++) inb[4 + l] = peb[l];
+}
+// forward of cart_update_vjp: from prm (10) the gnewton root started at
+// guess (4); outputs e, P, Temp, v(3), u(4), T(4), pe(5), PI = 20 values;
+// returns the solver flags
+int ah_cart_update(int eos, const double* prm, const double* guess, double* out)
+{
+  Params p = make_params(0, 0.2, 0, 5.0, eos);
+  p.milne = 0;
+  double x[4] = {guess[0], guess[1], guess[2], guess[3]};
+  unsigned fl;
+  cart_gnewton(p, prm, x, fl);
+  const double c0 = sqrt(1 + x[1] * x[1] + x[2] * x[2] + x[3] * x[3]);
+  out[0] = x[0];
+  out[1] = eos_p(p, x[0]);
+  out[2] = eos_T(p, x[0]);
+  for (int l = 0; l < 3; l++) out[3 + l] = x[1 + l] / c0;
+  out[6] = c0;
+  for (int l = 0; l < 3; l++) out[7 + l] = x[1 + l];
+  for (int l = 0; l < 4; l++) out[10 + l] = prm[l];
+  for (int l = 0; l < 5; l++) out[14 + l] = prm[4 + l];
+  out[19] = prm[9];
+  return (int)fl;
+}
+void ah_cart_update_vjp(int eos, const double* prm, const double* x, const double* ob /* 20 */, double* Vb)
+{
+  Params p = make_params(0, 0.2, 0, 5.0, eos);
+  p.milne = 0;
+  cart_update_vjp(p, prm, x, ob + 10, ob + 14, ob[19], ob[0], ob[1], ob[2], ob + 3, ob + 6, Vb);
+}
+// forward of source_vjp_cart: the Cartesian source rows 4..8 of one cell
+void ah_source_cart(int eos, int eta_law, double eta_coef, int tpi_law, double tpi_coef, double e, double P,
+                    const double* u, const double* pe, const double* du16, double* S5)
+{
+  Params p = make_params(eta_law, eta_coef, tpi_law, tpi_coef, eos);
+  p.milne = 0;
+  double du[4][4], pi10[10], out[10];
+  for (int a = 0; a < 16; a++) du[a / 4][a % 4] = du16[a];
+  cart_pi10(u, pe[0], pe[1], pe[2], pe[3], pe[4], pi10);
+  const double T = eos_T(p, e);
+  const double s = eos_s(p, e, P, T);
+  const double SH = eta_of(p, s);
+  const double tpi = tpi_of(p, SH, P, e);
+  shear_source<false>(0, 1.0, u, pi10, du, SH, tpi, out);
+  for (int l = 0; l < 5; l++) S5[l] = out[cart5(l)];
+}
+void ah_source_cart_vjp(int eos, int eta_law, double eta_coef, int tpi_law, double tpi_coef, double e, double P,
+                        const double* u, const double* pe, const double* du16, const double* Sb5,
+                        double* inb /* e, P, u4, pe5, du16, ceta, ctpi = 29 */)
+{
+  Params p = make_params(eta_law, eta_coef, tpi_law, tpi_coef, eos);
+  p.milne = 0;
+  double du[4][4], dub[4][4], Sb[15] = {0};
+  for (int a = 0; a < 16; a++) {
+    du[a / 4][a % 4] = du16[a];
+    dub[a / 4][a % 4] = 0;
+  }
+  for (int l = 0; l < 5; l++) Sb[4 + l] = Sb5[l];
+  double eb = 0, Pb = 0, ub[4] = {0, 0, 0, 0}, peb[5] = {0, 0, 0, 0, 0}, ceb = 0, ctb = 0;
+  source_vjp_cart(p, e, P, u, pe, true, du, Sb, eb, Pb, ub, peb, dub, ceb, ctb);
+  inb[0] = eb;
+  inb[1] = Pb;
+  for (int l = 0; l < 4; l++) inb[2 + l] = ub[l];
+  for (int l = 0; l < 5; l++) inb[6 + l] = peb[l];
+  for (int a = 0; a < 16; a++) inb[11 + a] = dub[a / 4][a % 4];
+  inb[27] = ceb;
+  inb[28] = ctb;
+}
+
 }  // extern "C"

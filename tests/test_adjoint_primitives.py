@@ -41,6 +41,15 @@ def lib():
     L.ah_tab.argtypes = [C.c_double, dp]
     L.ah_tab_d.restype = C.c_int
     L.ah_tab_d.argtypes = [C.c_double, dp, dp]
+    L.ah_cart_pi10.argtypes = [dp, dp, dp]
+    L.ah_cart_pi10_vjp.argtypes = [dp, dp, dp, dp]
+    L.ah_cart_update.restype = C.c_int
+    L.ah_cart_update.argtypes = [C.c_int, dp, dp, dp]
+    L.ah_cart_update_vjp.argtypes = [C.c_int, dp, dp, dp, dp]
+    L.ah_source_cart.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp,
+                                 dp, dp]
+    L.ah_source_cart_vjp.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, dp,
+                                     dp, dp, dp, dp]
     return L
 
 
@@ -264,3 +273,94 @@ def test_update_vjp_root_by_implicit_function_theorem(eos):
         Vb = np.zeros(15)
         L.ah_update_vjp(eos, tn, P(V), out[0], P(ob), P(Vb))
         dot_check(f, V, Vb, ob, rng, h=1e-7, tol=1e-5)
+
+
+# ---------------------------------------------------------------- Cartesian (TXYZ5)
+
+def test_cart_pi10_vjp():
+    """the full Cartesian pi^{mu nu} from u and the five evolved components"""
+    L = lib()
+    rng = np.random.default_rng(50)
+    for _ in range(6):
+        u = rand_u(rng, 1.0, milne=0)
+        pe = 0.1 * rng.standard_normal(5)
+        x = np.concatenate([u, pe])
+
+        def f(z):
+            out = np.zeros(10)
+            L.ah_cart_pi10(P(z[:4]), P(z[4:]), P(out))
+            return out
+        # the tensor is traceless and transverse to u (g = diag(1, -1, -1, -1))
+        pi = f(x)
+        full = np.array([[pi[0], pi[1], pi[2], pi[3]], [pi[1], pi[4], pi[5], pi[6]], [pi[2], pi[5], pi[7], pi[8]],
+                         [pi[3], pi[6], pi[8], pi[9]]])
+        g = np.diag([1.0, -1.0, -1.0, -1.0])
+        assert abs(np.trace(g @ full)) < 1e-12
+        assert np.abs(full @ (g @ u)).max() < 1e-12
+        ob = rng.standard_normal(10)
+        inb = np.zeros(9)
+        L.ah_cart_pi10_vjp(P(u), P(pe), P(ob), P(inb))
+        dot_check(f, x, inb, ob, rng)
+
+
+@pytest.mark.parametrize("eos", [0, 1])
+def test_cart_update_vjp_root_by_implicit_function_theorem(eos):
+    """the Cartesian 4-D gnewton root (e, u^x, u^y, u^z) from T^{t mu}, the
+    five pi and PI, reversed by the implicit-function theorem: dot-product
+    test against central differences of the converged root"""
+    L = lib()
+    rng = np.random.default_rng(60 + eos)
+    for _ in range(8):
+        e = rng.uniform(1, 10) if eos == 0 else np.exp(rng.uniform(np.log(0.05), np.log(300.0)))
+        u = rand_u(rng, 1.0, milne=0)
+        pe = 0.02 * e * rng.standard_normal(5)
+        PI = 0.0
+        ev = np.zeros(3)
+        L.ah_eos(eos, e, P(ev))
+        Pr = ev[0]
+        pi = np.zeros(10)
+        L.ah_cart_pi10(P(u), P(pe), P(pi))
+        w = e + Pr - PI
+        T = w * u[0] * u - (Pr - PI) * np.array([1.0, 0, 0, 0]) + pi[[0, 1, 2, 3]]
+        prm = np.concatenate([T, pe, [PI]])
+        guess = np.array([e * 1.01, *(u[1:] * 0.99)])
+
+        def f(z):
+            out = np.zeros(20)
+            # flag 8: gnewton's test_delta(1e-30) asks for a zero step, so it
+            # often runs to its 100 iterations between neighbouring doubles
+            assert L.ah_cart_update(eos, P(z), P(guess), P(out)) & 4 == 0
+            return out
+        out = f(prm)
+        assert abs(out[0] - e) < 1e-9 * e and np.abs(out[7:10] - u[1:]).max() < 1e-9
+        ob = rng.standard_normal(20)
+        Vb = np.zeros(15)
+        x = np.array([out[0], *out[7:10]])
+        L.ah_cart_update_vjp(eos, P(prm), P(x), P(ob), P(Vb))
+        assert np.all(Vb[9:14] == 0.0)
+        inb = np.concatenate([Vb[:9], [Vb[14]]])
+        dot_check(f, prm, inb, ob, rng, h=1e-7, tol=1e-6)
+
+
+@pytest.mark.parametrize("eos,laws", [(0, (0, 0.2, 0, 5.0)), (0, (1, 1.0, 1, 1.5)), (1, (1, 1.0, 1, 1.5)),
+                                      (1, (0, 0.08, 0, 5.0))])
+def test_source_cart_vjp(eos, laws):
+    """the Cartesian shear source rows (the five evolved components) with the
+    full tensor built from them"""
+    L = lib()
+    el, ec, tl, tc = laws
+    rng = np.random.default_rng(70 + el + tl)
+    for _ in range(5):
+        u = rand_u(rng, 1.0, milne=0)
+        e = rng.uniform(1, 8)
+        x = np.concatenate([[e, e / 3.0 * (1 + 0.05 * rng.standard_normal())], u, 0.05 * e * rng.standard_normal(5),
+                            rng.standard_normal(16), [ec, tc]])
+
+        def f(z):
+            S = np.zeros(5)
+            L.ah_source_cart(eos, el, z[27], tl, z[28], z[0], z[1], P(z[2:6]), P(z[6:11]), P(z[11:27]), P(S))
+            return S
+        ob = rng.standard_normal(5)
+        inb = np.zeros(29)
+        L.ah_source_cart_vjp(eos, el, ec, tl, tc, x[0], x[1], P(x[2:6]), P(x[6:11]), P(x[11:27]), P(ob), P(inb))
+        dot_check(f, x, inb, ob, rng)

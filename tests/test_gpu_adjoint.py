@@ -299,3 +299,86 @@ def test_closing_a_recorded_context_frees_its_stage_states():
     # one recording holds 6 steps x 2 stages x 29 doubles per ghosted cell
     slab = 6 * 2 * 29 * d.cells() * 8
     assert free0 - free1 < slab / 2, (free0, free1, slab)
+
+
+# ------------------------------------------------------------------ Cartesian (TXYZ5)
+
+def cart_case(rk=2, fast=0, **kw):
+    from viral_amd.desc import CARTESIAN
+    d = make_desc(10, 9, 8, coords=CARTESIAN, rk=rk, fast_math=fast, **kw)
+    if d.eos == EOS_S95P:
+        dref = d.copy()
+        dref.fast_math = dref.eos_table = 0
+        st = ic.random_smooth(d, seed=21, tau=TAU, pi_frac=0.02, e_scale=30.0,
+                              eos_p=lambda e: capi.eos_eval(dref, e)[0].reshape(np.shape(e)))
+    else:
+        st = ic.random_smooth(d, seed=21, tau=TAU, pi_frac=0.02)
+    return d, st
+
+
+@pytest.mark.parametrize("rk,every", [(1, 1), (2, 2), (3, 2)])
+def test_cartesian_adjoint_state_gradient_matches_fd(rk, every):
+    """the Cartesian (TXYZ5) step in reverse: the 4-D root by the
+    implicit-function theorem, the flux with the shear tensor built from its
+    five evolved components, no geometric sources, the shear-slice quirk"""
+    d, st = cart_case(rk=rk, eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5)
+    rng = np.random.default_rng(30 + rk)
+    seed = rng.standard_normal(st.shape)
+    print("case cartesian adjoint rk=%d every=%d" % (rk, every), flush=True)
+    _, grad, pg = run_gpu(d, st, 3, every, seed)
+    for trial in range(2):
+        dx = direction(st, rng)
+        fd = fd4(lambda h: np.sum(seed * run_gpu(d, st + h * dx, 3)))
+        ad = np.sum(grad * dx)
+        print(trial, ad, fd, abs(ad - fd) / abs(fd), flush=True)
+        assert close(ad, fd), (ad, fd)
+
+
+@pytest.mark.parametrize("fast", [0, 1], ids=["exact", "fast_table"])
+def test_cartesian_adjoint_s95p_state_and_coefficients(fast):
+    """s95p EOS (fast mode: the table in the sources; the Cartesian root and
+    its outputs keep the closed forms): state and transport-coefficient
+    gradients against differences of the GPU forward, and of the CPU oracle
+    (the reference gradient) in exact mode"""
+    d, st = cart_case(fast=fast, eos=EOS_S95P, eos_table=fast, eta_law=ETA_S_OVER_4PI, eta_coef=1.0,
+                      tpi_law=TPI_OVER_P, tpi_coef=1.5)
+    rng = np.random.default_rng(40 + fast)
+    seed = rng.standard_normal(st.shape)
+    print("case cartesian adjoint s95p fast=%d" % fast, flush=True)
+    _, grad, pg = run_gpu(d, st, 3, 2, seed)
+    runs = [run_gpu] + ([run_cpu] if not fast else [])
+    dx = direction(st, rng)
+    ad = np.sum(grad * dx)
+    for run in runs:
+        fd = fd4(lambda h: np.sum(seed * run(d, st + h * dx, 3)))
+        print("state", run.__name__, ad, fd, abs(ad - fd) / abs(fd), flush=True)
+        assert close(ad, fd), (run.__name__, ad, fd)
+    for q, name in enumerate(["eta_coef", "tpi_coef"]):
+        c0 = getattr(d, name)
+        for run in runs:
+            def J(h):
+                dd = d.copy()
+                setattr(dd, name, c0 * (1 + h))
+                return np.sum(seed * run(dd, st, 3))
+
+            fd = fd4(J, HP) / c0
+            print(name, run.__name__, pg[q], fd, flush=True)
+            assert close(pg[q], fd), (name, run.__name__, pg[q], fd)
+
+
+def test_cartesian_stored_stage_states_match_recompute():
+    d, st = cart_case(rk=3)
+    seed = np.random.default_rng(6).standard_normal(st.shape)
+    print("case cartesian adjoint stored", flush=True)
+    plain = run_gpu(d, st, 4)
+    _, g3, p3 = run_gpu(d, st, 4, 3, seed)
+    _, g1, p1 = run_gpu(d, st, 4, 1, seed)
+    assert np.array_equal(g1, g3) and np.array_equal(p1, p3)
+    h = capi.Hydro(d)
+    h.upload(st)
+    h.adjoint_begin(0)
+    assert h.steps(TAU, TS, 4) == (OK, 4)
+    assert np.array_equal(h.download().view(np.int64), plain.view(np.int64))
+    g0, p0 = h.adjoint(seed)
+    assert np.array_equal(g0, g3) and np.array_equal(p0, p3)
+    h.adjoint_end()

@@ -1,9 +1,11 @@
 // adjoint.h -- hand-written reverse-mode (vector-Jacobian) forms of the
-// pieces of one viscous KT stage in Milne coordinates: WENO5 reconstruction,
-// the centred flux / Var / eigenvalue of a cell, the Israel-Stewart shear
-// source, the geometric source, the EOS and transport coefficients, the
-// velocities and the Newton root (implicit-function theorem at the converged
-// root: de = -(df/dtheta) dtheta / (df/de)).
+// pieces of one viscous KT stage in Milne and Cartesian coordinates: WENO5
+// reconstruction, the centred flux / Var / eigenvalue of a cell, the
+// Israel-Stewart shear source, the geometric source, the EOS and transport
+// coefficients, the velocities and the roots by the implicit-function theorem
+// at the converged root (Milne's 1-D Newton: de = -(df/dtheta) dtheta /
+// (df/de); Cartesian 4-D gnewton: dx = -J^{-1} (dF/dprm) dprm), and the
+// Cartesian shear tensor built from its five evolved components.
 //
 // Each *_vjp function takes the adjoint of its outputs and ADDS the adjoint of
 // its inputs into the caller's accumulators.  All functions are host- and
@@ -421,6 +423,10 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
   ub[0] -= iu0b * iu0 * iu0;
 }
 
+VHD void shear_transport_vjp(const Params& p, int milne, double tau, double e, double P, const double u[4],
+                             const double pi10[10], const double du[4][4], const double Sb10[10], double& eb,
+                             double& Pb, double ub[4], double pib[10], double dub[4][4], double& cetab, double& ctpib);
+
 // Sources of one cell (NEWEST10/Source.h:208-221 geometric, :347-517 shear)
 // given the velocity gradients du and the transport coefficients: adds the
 // adjoint of (e, P, PI, u, pi, du) and of the viscosity / relaxation
@@ -447,14 +453,23 @@ VHD void source_vjp(const Params& p, double tau, double e, double P, double PI, 
   eb += wb;
   Pb += wb;
   PIb -= wb;
-  // shear
+  shear_transport_vjp(p, 1, tau, e, P, u, pi10, du, Sb + 4, eb, Pb, ub, pib, dub, cetab, ctpib);
+}
+
+// The shear source with its transport coefficients eta(s) and tau_pi in
+// reverse (milne 0: Cartesian): given the adjoint Sb10 of its ten outputs,
+// adds the adjoint of (e, P, u, pi10, du) and of eta_coef, tpi_coef.
+VHD void shear_transport_vjp(const Params& p, int milne, double tau, double e, double P, const double u[4],
+                             const double pi10[10], const double du[4][4], const double Sb10[10], double& eb,
+                             double& Pb, double ub[4], double pib[10], double dub[4][4], double& cetab, double& ctpib)
+{
   double Pe, dPe, T, dT, s, ds;
   eos_d(p, e, Pe, dPe, T, dT, s, ds);
   if (!p.eos) s = (e + P) / T;
   const double SH = eta_of(p, s);
   const double tpi = tpi_of(p, SH, P, e);
   double SHb = 0, tpib = 0;
-  shear_source_vjp(1, tau, u, pi10, du, SH, tpi, Sb + 4, ub, pib, dub, SHb, tpib);
+  shear_source_vjp(milne, tau, u, pi10, du, SH, tpi, Sb10, ub, pib, dub, SHb, tpib);
   if (p.tpi_law == 1) {  // c eta / P
     SHb += tpib * p.tpi_coef / P;
     Pb -= tpib * p.tpi_coef * SH / (P * P);
@@ -543,6 +558,167 @@ VHD void update_vjp(const Params& p, double tn, const double T0[4], const double
   for (int l = 0; l < 4; l++) Vb[l] = T0b[l] * itn;
   for (int l = 0; l < 10; l++) Vb[4 + l] = pib[l];
   Vb[14] = PIb;
+}
+
+// ---------------------------------------------------------------- Cartesian (TXYZ5)
+
+// The full pi^{mu nu} of the Cartesian step (cart_pi10, from u and the five
+// evolved components pe = {xx, yy, xy, xz, yz}) in reverse: given the adjoint
+// ob of its ten slots, adds the adjoint of u and pe.
+VHD void cart_pi10_vjp(const double u[4], const double pe[5], const double ob[10], double ub[4], double peb[5])
+{
+  const double u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
+  const double p1 = pe[0], p2 = pe[1], p3 = pe[2], p4 = pe[3], p5 = pe[4];
+  const double num =
+      p1 * u1 * u1 + 2.0 * p3 * u1 * u2 + p2 * u2 * u2 + 2.0 * u3 * (p4 * u1 + p5 * u2) - u0 * u0 * (p1 + p2);
+  const double iden = 1.0 / (u0 * u0 - u3 * u3);
+  const double zz = num * iden;
+  const double iu0 = 1.0 / u0;
+  // slots tx, ty, tz = n_i / u0
+  const double n1 = p1 * u1 + p3 * u2 + p4 * u3, n2 = p3 * u1 + p2 * u2 + p5 * u3, n3 = p4 * u1 + p5 * u2 + zz * u3;
+  const double b1 = ob[1] * iu0, b2 = ob[2] * iu0, b3 = ob[3] * iu0;
+  const double zzb = ob[0] + ob[9] + b3 * u3;
+  double p1b = ob[0] + ob[4] + b1 * u1;
+  double p2b = ob[0] + ob[7] + b2 * u2;
+  double p3b = ob[5] + b1 * u2 + b2 * u1;
+  double p4b = ob[6] + b1 * u3 + b3 * u1;
+  double p5b = ob[8] + b2 * u3 + b3 * u2;
+  double u0b = -(b1 * n1 + b2 * n2 + b3 * n3) * iu0;
+  double u1b = b1 * p1 + b2 * p3 + b3 * p4;
+  double u2b = b1 * p3 + b2 * p2 + b3 * p5;
+  double u3b = b1 * p4 + b2 * p5 + b3 * zz;
+  // zz = num / den, den = u0^2 - u3^2
+  const double numb = zzb * iden, denb = -zzb * zz * iden;
+  p1b += numb * (u1 * u1 - u0 * u0);
+  p2b += numb * (u2 * u2 - u0 * u0);
+  p3b += numb * 2.0 * u1 * u2;
+  p4b += numb * 2.0 * u3 * u1;
+  p5b += numb * 2.0 * u3 * u2;
+  u1b += numb * 2.0 * (p1 * u1 + p3 * u2 + p4 * u3);
+  u2b += numb * 2.0 * (p3 * u1 + p2 * u2 + p5 * u3);
+  u3b += numb * 2.0 * (p4 * u1 + p5 * u2) - denb * 2.0 * u3;
+  u0b += -numb * 2.0 * u0 * (p1 + p2) + denb * 2.0 * u0;
+  ub[0] += u0b;
+  ub[1] += u1b;
+  ub[2] += u2b;
+  ub[3] += u3b;
+  peb[0] += p1b;
+  peb[1] += p2b;
+  peb[2] += p3b;
+  peb[3] += p4b;
+  peb[4] += p5b;
+}
+
+// Sources of one Cartesian cell (TXYZ5/Source.h:152-155, 272-326): no
+// geometric rows; the shear source's xx, yy, xy, xz, yz slots feed Var rows
+// 4..8 (zero where the reference's slice loop stops short, `live` false).
+// Given Sb (15 rows), adds the adjoint of (e, P, u, pe, du) and of the
+// coefficients.
+VHD void source_vjp_cart(const Params& p, double e, double P, const double u[4], const double pe[5], bool live,
+                         const double du[4][4], const double Sb[15], double& eb, double& Pb, double ub[4],
+                         double peb[5], double dub[4][4], double& cetab, double& ctpib)
+{
+  if (!live) return;
+  double pi10[10], Sb10[10], pib[10];
+  cart_pi10(u, pe[0], pe[1], pe[2], pe[3], pe[4], pi10);
+  for (int l = 0; l < 10; l++) Sb10[l] = pib[l] = 0.0;
+  for (int l = 0; l < 5; l++) Sb10[cart5(l)] = Sb[4 + l];
+  shear_transport_vjp(p, 0, 1.0, e, P, u, pi10, du, Sb10, eb, Pb, ub, pib, dub, cetab, ctpib);
+  cart_pi10_vjp(u, pe, pib, ub, peb);
+}
+
+// Solves A^T y = b for a 4x4 A by Gaussian elimination with partial pivoting.
+// Every index is a compile-time constant once the loops unroll (rows are
+// swapped by comparing the pivot row with each candidate), so the matrix
+// stays in registers.
+VHD void solve4t(const double A[4][4], const double b[4], double y[4])
+{
+  double M[4][5];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) M[i][k] = A[k][i];
+    M[i][4] = b[i];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    int ip = j;
+    double mx = fabs(M[j][j]);
+#pragma unroll
+    for (int i = j + 1; i < 4; i++)
+      if (fabs(M[i][j]) > mx) {
+        mx = fabs(M[i][j]);
+        ip = i;
+      }
+#pragma unroll
+    for (int i = j + 1; i < 4; i++)
+      if (ip == i) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+          const double t = M[j][k];
+          M[j][k] = M[i][k];
+          M[i][k] = t;
+        }
+      }
+    const double inv = 1.0 / M[j][j];
+#pragma unroll
+    for (int i = j + 1; i < 4; i++) {
+      const double f = M[i][j] * inv;
+#pragma unroll
+      for (int k = j; k < 5; k++) M[i][k] -= f * M[j][k];
+    }
+  }
+#pragma unroll
+  for (int i = 3; i >= 0; i--) {
+    double t = M[i][4];
+#pragma unroll
+    for (int k = i + 1; k < 4; k++) t -= M[i][k] * y[k];
+    y[i] = t / M[i][i];
+  }
+}
+
+// The Cartesian update and its 4-D root (TXYZ5/time.h:100-165,
+// TXYZ5/multiroot.h:203-479) in reverse.  Forward: prm = (T^{t mu}, p1..p5,
+// PI) are the stage's Var rows {0..8, 14}; x = (e, u^x, u^y, u^z) solves
+// F(x; prm) = 0 (cart_f); the stored outputs are T^{t mu}, the five pi and PI
+// (= prm), e, P(e), Temp(e) (closed forms: the Cartesian root and its outputs
+// do not read the table), v = u / u^t, u^t = sqrt(1 + u.u) and u.  Given the
+// adjoint of those outputs, writes Vb (15 rows; rows 9..13 zero).  The root
+// enters by the implicit-function theorem at the converged x:
+// J^T lam = xbar with J = dF/dx (cart_df), prm_bar = -(dF/dprm)^T lam.
+VHD void cart_update_vjp(const Params& p, const double prm[10], const double x[4], const double T0bo[4],
+                         const double pebo[5], double PIbo, double ebo, double Pbo, double Tempbo, const double vbo[3],
+                         const double ubo[4], double Vb[15])
+{
+  Params pc = p;
+  pc.use_tab = 0;
+  double Pe, a, Te, dT, se, dse;
+  eos_d(pc, x[0], Pe, a, Te, dT, se, dse);
+  const double u1 = x[1], u2 = x[2], u3 = x[3];
+  const double q = 1.0 + u1 * u1 + u2 * u2 + u3 * u3, sq = sqrt(q), isq = 1.0 / sq;
+  // adjoint of x from e, P, Temp, u^t, u, v
+  const double vs = (vbo[0] * u1 + vbo[1] * u2 + vbo[2] * u3) * isq * isq * isq;
+  double xb[4];
+  xb[0] = ebo + Pbo * a + Tempbo * dT;
+  xb[1] = ubo[1] + ubo[0] * u1 * isq + vbo[0] * isq - u1 * vs;
+  xb[2] = ubo[2] + ubo[0] * u2 * isq + vbo[1] * isq - u2 * vs;
+  xb[3] = ubo[3] + ubo[0] * u3 * isq + vbo[2] * isq - u3 * vs;
+  double J[4][4], lam[4];
+  cart_df(pc, prm, x, J);
+  solve4t(J, xb, lam);
+  // dF/dprm of cart_f, prm = (W, X, Y, Z, p1, p2, p3, p4, p5, PI)
+  const double l0 = lam[0], l1 = lam[1], l2 = lam[2], l3 = lam[3];
+  Vb[0] = T0bo[0] - (l0 * sq - l3 * u3);
+  Vb[1] = T0bo[1] - (-l0 * u1 + l1 * sq);
+  Vb[2] = T0bo[2] - (-l0 * u2 + l2 * sq);
+  Vb[3] = T0bo[3] - (-l0 * u3 + l3 * sq);
+  Vb[4] = pebo[0] - (-l1 * u1 + l3 * u3);
+  Vb[5] = pebo[1] - (-l2 * u2 + l3 * u3);
+  Vb[6] = pebo[2] - (-l1 * u2 - l2 * u1);
+  Vb[7] = pebo[3] - (-l1 * u3 - l3 * u1);
+  Vb[8] = pebo[4] - (-l2 * u3 - l3 * u2);
+  for (int l = 9; l < 14; l++) Vb[l] = 0.0;
+  Vb[14] = PIbo - (l1 * u1 * q + l2 * u2 * q + l3 * u3);
 }
 
 #undef VHD

@@ -879,7 +879,7 @@ __host__ __device__ __forceinline__ void shear_source(int milne, double tau, con
 // components {xx, yy, xy, xz, yz} (TXYZ5/maindef.h:240) by u_mu pi^{mu nu} = 0
 // and pi^mu_mu = 0 -- the A1..A5 of NEWEST5/maindef.h:497-501 at tau = 1.
 // Same operation order as the CPU restatement.
-__device__ __forceinline__ void cart_pi10(const double u[4], double p1, double p2, double p3, double p4, double p5,
+__host__ __device__ __forceinline__ void cart_pi10(const double u[4], double p1, double p2, double p3, double p4, double p5,
                                           double pi10[10])
 {
   const double u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
@@ -899,12 +899,12 @@ __device__ __forceinline__ void cart_pi10(const double u[4], double p1, double p
 }
 
 // slot of the full tensor holding evolved Cartesian component l
-__device__ __forceinline__ int cart5(int l) { return l == 0 ? 4 : l == 1 ? 7 : l == 2 ? 5 : l == 3 ? 6 : 8; }
+__host__ __device__ __forceinline__ int cart5(int l) { return l == 0 ? 4 : l == 1 ? 7 : l == 2 ? 5 : l == 3 ? 6 : 8; }
 
 // conservation_f / conservation_df of TXYZ5/multiroot.h:203-319 (residuals of
 // T^{t mu} in the unknowns e, u^x, u^y, u^z), prm = {T^tt, T^tx, T^ty, T^tz,
 // p1..p5, PI}.
-__device__ __forceinline__ void cart_f(const Params& p, const double prm[10], const double x[4], double y[4])
+__host__ __device__ __forceinline__ void cart_f(const Params& p, const double prm[10], const double x[4], double y[4])
 {
   const double W = prm[0], X = prm[1], Y = prm[2], Z = prm[3];
   const double p1 = prm[4], p2 = prm[5], p3 = prm[6], p4 = prm[7], p5 = prm[8], PI = prm[9];
@@ -918,7 +918,7 @@ __device__ __forceinline__ void cart_f(const Params& p, const double prm[10], co
   y[3] = -(p4 * u1) - p5 * u2 - P * u3 + u3 * (p1 + p2 + PI - W) + Z * sq;
 }
 
-__device__ __forceinline__ void cart_df(const Params& p, const double prm[10], const double x[4], double J[4][4])
+__host__ __device__ __forceinline__ void cart_df(const Params& p, const double prm[10], const double x[4], double J[4][4])
 {
   const double W = prm[0], X = prm[1], Y = prm[2], Z = prm[3];
   const double p1 = prm[4], p2 = prm[5], p3 = prm[6], p4 = prm[7], p5 = prm[8], PI = prm[9];
@@ -946,7 +946,7 @@ __device__ __forceinline__ void cart_df(const Params& p, const double prm[10], c
   J[3][3] = -P + p1 + p2 + PI - W + u3 * Z * isq;
 }
 
-__device__ __forceinline__ double enorm4(const double f[4])
+__host__ __device__ __forceinline__ double enorm4(const double f[4])
 {
   double e2 = 0;
 #pragma unroll
@@ -961,7 +961,7 @@ __device__ __forceinline__ double enorm4(const double f[4])
 // step halving rule t *= (sqrt(1+6 theta)-1)/(3 theta) while the residual
 // norm rises and t > DBL_EPSILON.  All indices are compile-time (no scratch).
 // A zero pivot or a non-finite root sets the solver flag (4).
-__device__ __forceinline__ int cart_gnewton(const Params& p, const double prm[10], double x[4], unsigned& flags)
+__host__ __device__ __forceinline__ int cart_gnewton(const Params& p, const double prm[10], double x[4], unsigned& flags)
 {
   double f[4], J[4][4];
   unsigned fl = 0;

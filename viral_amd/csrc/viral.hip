@@ -1779,7 +1779,9 @@ __global__ void __launch_bounds__(BLK) k_adj_k0(Geom g, Params p, double tn, con
 // Update and root of one interior cell in reverse.  so / lo: the stage's
 // output state and its adjoint; vb0..vb2: adjoints of the RK registers
 // Var0..Var2; writes resb, the adjoint of L = Result + Source (15 rows), for
-// k_adj_src and the flux pass.
+// k_adj_src and the flux pass.  CART: the Cartesian update (Var rows
+// {0..8, 14}, T = Var, the 4-D root; cart_update_vjp).
+template <bool CART>
 __global__ void __launch_bounds__(BLK) k_adj_root(Geom g, Params p, StageArgs a, const double* __restrict__ so,
                                                   const double* __restrict__ lo, double* __restrict__ li,
                                                   double* __restrict__ vb0, double* __restrict__ vb1,
@@ -1795,6 +1797,10 @@ __global__ void __launch_bounds__(BLK) k_adj_root(Geom g, Params p, StageArgs a,
     if (a.stage == 0) li[(VIRAL_F_U0 + l) * n + x] += pb;
     else li[(VIRAL_F_PU0 + l) * n + x] += pb;
   }
+  // the Cartesian step never writes pi slots 5..9 (it evolves five
+  // components): they keep their values, and their adjoint passes through
+  if (CART)
+    for (int l = 5; l < 10; l++) li[(VIRAL_F_PI0 + l) * n + x] += lo[(VIRAL_F_PI0 + l) * n + x];
   // velocities and root
   double T0[4], pio[10], T0b[4], pibo[10], vbo[3], ubo[4];
   for (int l = 0; l < 4; l++) {
@@ -1802,23 +1808,36 @@ __global__ void __launch_bounds__(BLK) k_adj_root(Geom g, Params p, StageArgs a,
     T0b[l] = lo[(VIRAL_F_T00 + l) * n + x];
     ubo[l] = lo[(VIRAL_F_U0 + l) * n + x];
   }
-  for (int l = 0; l < 10; l++) {
+  for (int l = 0; l < (CART ? 5 : 10); l++) {
     pio[l] = so[(VIRAL_F_PI0 + l) * n + x];
     pibo[l] = lo[(VIRAL_F_PI0 + l) * n + x];
   }
   for (int c = 0; c < 3; c++) vbo[c] = lo[(VIRAL_F_VX + c) * n + x];
   double Vb[SV];
-  update_vjp(p, a.tau_new, T0, pio, so[VIRAL_F_BULK * n + x], so[VIRAL_F_EN * n + x], lo[VIRAL_F_EN * n + x],
-             lo[VIRAL_F_P * n + x], lo[VIRAL_F_TEMP * n + x], vbo, ubo, T0b, pibo, lo[VIRAL_F_BULK * n + x], Vb);
+  if (CART) {
+    const double prm[10] = {T0[0], T0[1], T0[2], T0[3], pio[0], pio[1], pio[2], pio[3], pio[4],
+                            so[VIRAL_F_BULK * n + x]};
+    const double xr[4] = {so[VIRAL_F_EN * n + x], so[VIRAL_F_U1 * n + x], so[VIRAL_F_U2 * n + x],
+                          so[VIRAL_F_U3 * n + x]};
+    cart_update_vjp(p, prm, xr, T0b, pibo, lo[VIRAL_F_BULK * n + x], lo[VIRAL_F_EN * n + x], lo[VIRAL_F_P * n + x],
+                    lo[VIRAL_F_TEMP * n + x], vbo, ubo, Vb);
+  } else {
+    update_vjp(p, a.tau_new, T0, pio, so[VIRAL_F_BULK * n + x], so[VIRAL_F_EN * n + x], lo[VIRAL_F_EN * n + x],
+               lo[VIRAL_F_P * n + x], lo[VIRAL_F_TEMP * n + x], vbo, ubo, T0b, pibo, lo[VIRAL_F_BULK * n + x], Vb);
+  }
   // RK combination (NEWEST10/time.h:2-98) in reverse
   for (int l = 0; l < SV; l++) {
     const long y = l * n + x;
+    if (CART && l >= 9 && l < 14) {  // rows the Cartesian step does not evolve
+      resb[y] = 0.0;
+      continue;
+    }
     double Lb;
     if (a.kind == 0) {
       const double Vt = Vb[l] + vb1[y];
       const double v0b = Vt + vb0[y];
       Lb = a.ts * Vt;
-      const double sc = l < 4 ? a.tau_src : 1.0;
+      const double sc = (l < 4 && !CART) ? a.tau_src : 1.0;
       li[(VIRAL_F_T00 + l) * n + x] += sc * v0b;  // Var row l is state field l
       vb0[y] = 0.0;
       vb1[y] = 0.0;
@@ -1850,6 +1869,9 @@ __global__ void __launch_bounds__(BLK) k_adj_root(Geom g, Params p, StageArgs a,
 // two waves per SIMD: without the cap the compiler takes 256 VGPRs plus 16
 // AGPRs (one wave); with it, 88 bytes per lane spill and the kernel runs 9 %
 // faster
+// CART: the Cartesian sources (no geometric rows, the shear tensor built from
+// the five evolved components; source_vjp_cart).
+template <bool CART>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2))) k_adj_src(Geom g, Params p, StageArgs a, const double* __restrict__ si,
                                                  const double* __restrict__ resb, double* __restrict__ li,
                                                  double* __restrict__ dubuf, double* __restrict__ partial)
@@ -1882,11 +1904,17 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2))) k
     }
     double eb = 0, Pb = 0, PIb = 0, ub[4] = {0, 0, 0, 0}, pib[10] = {0}, dub[4][4];
     for (int q = 0; q < 16; q++) dub[q / 4][q % 4] = 0;
-    source_vjp(p, a.tau_src, e, P, PI, u, pi, du, Lb, eb, Pb, PIb, ub, pib, dub, ceb, ctb);
+    if (CART) {
+      // TXYZ5's shear-source loop stops short of the top interior slices
+      const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)(x % g.gz) < g.kr - g.kl;
+      source_vjp_cart(p, e, P, u, pi, live, du, Lb, eb, Pb, ub, pib, dub, ceb, ctb);
+    } else {
+      source_vjp(p, a.tau_src, e, P, PI, u, pi, du, Lb, eb, Pb, PIb, ub, pib, dub, ceb, ctb);
+    }
     li[VIRAL_F_EN * n + x] += eb;
     li[VIRAL_F_P * n + x] += Pb;
     li[VIRAL_F_BULK * n + x] += PIb;
-    for (int l = 0; l < 10; l++) li[(VIRAL_F_PI0 + l) * n + x] += pib[l];
+    for (int l = 0; l < (CART ? 5 : 10); l++) li[(VIRAL_F_PI0 + l) * n + x] += pib[l];
     for (int l = 0; l < 4; l++) {
       li[(VIRAL_F_U0 + l) * n + x] += ub[l] + dub[l][0] / a.ts_src;
       li[(VIRAL_F_PU0 + l) * n + x] -= dub[l][0] / a.ts_src;
@@ -1935,7 +1963,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2))) k
 // contiguous (eta) index fastest.
 constexpr int NTA = 256;  // threads per block; the window shape is chosen per direction on the host
 
-template <int D>
+//
+// CART: the Cartesian flux (tau = 1 from the host, no eta pressure factor),
+// rows 9..13 skipped, and the T rows' pi^{d+1, l} built per window cell from
+// u and the five evolved components (cart_pi10); its adjoint is collected per
+// own cell and taken back through cart_pi10_vjp at the end.
+template <int D, bool CART>
 __global__ void __launch_bounds__(NTA) __attribute__((amdgpu_waves_per_eu(3)))
 k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const double* __restrict__ resb,
            const double* __restrict__ dubuf, double* __restrict__ li, int W, int TK, int nseg)
@@ -1982,12 +2015,20 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
   const bool w_ok = line_ok && c >= 0 && c < GD;
   const long xw = w_ok ? at(c) : 0;
   double u0 = 1.0, ud = 0.0, ww = 0.0, pp = 0.0;
+  double pT[4] = {0.0, 0.0, 0.0, 0.0};  // Cartesian pi^{d+1, m} of the window cell
   if (w_ok) {
     const double e = si[VIRAL_F_EN * n + xw], P = si[VIRAL_F_P * n + xw], PI = si[VIRAL_F_BULK * n + xw];
     u0 = si[VIRAL_F_U0 * n + xw];
     ud = si[(VIRAL_F_U1 + D) * n + xw];
     ww = e + P - PI;
-    pp = D == 2 ? (P - PI) / (tau * tau) : (P - PI);
+    pp = (D == 2 && !CART) ? (P - PI) / (tau * tau) : (P - PI);
+    if (CART) {
+      const double uu[4] = {u0, si[VIRAL_F_U1 * n + xw], si[VIRAL_F_U2 * n + xw], si[VIRAL_F_U3 * n + xw]};
+      double pt[10];
+      cart_pi10(uu, si[VIRAL_F_PI0 * n + xw], si[(VIRAL_F_PI0 + 1) * n + xw], si[(VIRAL_F_PI0 + 2) * n + xw],
+                si[(VIRAL_F_PI0 + 3) * n + xw], si[(VIRAL_F_PI0 + 4) * n + xw], pt);
+      for (int m = 0; m < 4; m++) pT[m] = pt[pidx(row, m)];
+    }
   }
   const bool rc = act && wi >= 2 && wi < W - 2;  // full stencil in the window
   // faces of this cell: right c+1/2 (face index c), left c-1/2 (face index c-1)
@@ -2026,6 +2067,7 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
   // touches the cell); the adjoints of w = e + P - PI, the pressure term, u^d
   // and u^mu accumulate here.
   double wb = 0.0, ppb = 0.0, udb = 0.0, ub0 = 0.0, ub1 = 0.0, ub2 = 0.0, ub3 = 0.0;
+  double pTb[4] = {0.0, 0.0, 0.0, 0.0};  // CART: adjoint of pi^{d+1, m}
   const double iu0 = frcp(u0);
   double abR = 0.0, abL = 0.0;  // this cell's parts of the speed adjoints of its two faces
   // a row's global loads, issued one row ahead so they overlap the previous
@@ -2036,13 +2078,18 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
     if (!w_ok) return;
     nA = si[(long)l * n + xw];
     if (l < 4) {
-      nB = si[(VIRAL_F_PI0 + pidx(row, l)) * n + xw];
+      nB = CART ? (l == 0 ? pT[0] : l == 1 ? pT[1] : l == 2 ? pT[2] : pT[3])
+                : si[(VIRAL_F_PI0 + pidx(row, l)) * n + xw];
       nM = si[(VIRAL_F_U0 + l) * n + xw];
     }
     nR = (c >= lo && c < hi) ? resb[l * n + xw] : 0.0;
   };
   row_load(0);
   for (int l = 0; l < SV; l++) {
+    if (CART && l >= 9 && l < 14) {  // block-uniform: rows the Cartesian step does not evolve
+      if (l == 13) row_load(14);
+      continue;
+    }
     // centred flux and Var of row l (cell_FVA)
     const double ldA = nA, ldB = nB, ldR = nR;
     double Fw = 0.0, Vw = 0.0, um = 0.0, fv = 0.0;
@@ -2062,7 +2109,7 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
     sF[t] = Fw;
     sV[t] = Vw;
     sR[t] = ldR;
-    if (l + 1 < SV) row_load(l + 1);
+    if (l + 1 < SV && !(CART && l == 8)) row_load(l + 1);
     __syncthreads();
     {
       double gF[5] = {0, 0, 0, 0, 0}, gV[5] = {0, 0, 0, 0, 0};
@@ -2097,7 +2144,14 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
       if (l < 4) {
         // F_l = tau (pi^{d+1,l} + [l = d+1] pp + u^{d+1} u^l w), V_l = tau T^{tau l}
         const double vb = tau * sf;
-        li[(VIRAL_F_PI0 + pidx(row, l)) * n + xw] += vb;
+        if (CART) {
+          if (l == 0) pTb[0] += vb;
+          else if (l == 1) pTb[1] += vb;
+          else if (l == 2) pTb[2] += vb;
+          else pTb[3] += vb;
+        } else {
+          li[(VIRAL_F_PI0 + pidx(row, l)) * n + xw] += vb;
+        }
         li[(VIRAL_F_T00 + l) * n + xw] += tau * sv;
         if (l == row) ppb += vb;
         udb += vb * um * ww;
@@ -2168,7 +2222,22 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
   if (row == 1) ub1 += udb;
   else if (row == 2) ub2 += udb;
   else ub3 += udb;
-  const double k = D == 2 ? 1.0 / (tau * tau) : 1.0;
+  if (CART) {
+    // pi^{d+1, m} = cart_pi10(u, p1..p5) in reverse
+    const double uu[4] = {u0, si[VIRAL_F_U1 * n + xw], si[VIRAL_F_U2 * n + xw], si[VIRAL_F_U3 * n + xw]};
+    const double pe[5] = {si[VIRAL_F_PI0 * n + xw], si[(VIRAL_F_PI0 + 1) * n + xw], si[(VIRAL_F_PI0 + 2) * n + xw],
+                          si[(VIRAL_F_PI0 + 3) * n + xw], si[(VIRAL_F_PI0 + 4) * n + xw]};
+    double ob[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < 4; m++) ob[pidx(row, m)] = pTb[m];
+    double ubc[4] = {0, 0, 0, 0}, peb[5] = {0, 0, 0, 0, 0};
+    cart_pi10_vjp(uu, pe, ob, ubc, peb);
+    ub0 += ubc[0];
+    ub1 += ubc[1];
+    ub2 += ubc[2];
+    ub3 += ubc[3];
+    for (int m = 0; m < 5; m++) li[(VIRAL_F_PI0 + m) * n + xw] += peb[m];
+  }
+  const double k = (D == 2 && !CART) ? 1.0 / (tau * tau) : 1.0;
   li[VIRAL_F_EN * n + xw] += wb;
   li[VIRAL_F_P * n + xw] += ppb * k + wb;
   li[VIRAL_F_BULK * n + xw] += -(ppb * k) - wb;
@@ -3671,11 +3740,12 @@ int viral_adjoint_begin(viral_ctx* ctx, int every)
   if (!ctx || every < 0) return fail(ctx, VIRAL_ERR_ARG, "bad adjoint recording request");
   if (ctx->ev.size() != 1) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint runs on single-event contexts");
   const Event& E = ctx->ev[0];
-  if (E.d.coords != VIRAL_MILNE || E.d.scheme != VIRAL_KT)
-    return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint covers the Milne KT step");
+  if (E.d.scheme != VIRAL_KT)
+    return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint covers the KT step (Milne and Cartesian)");
   if (E.d.quirks & VIRAL_Q_FIX_VE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint does not cover the FIX build");
   if (E.d.bulk) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint does not cover the bulk source");
-  if ((E.d.quirks & VIRAL_Q_K0_BROADCAST) && !E.d.lbi && (E.g.k0 < E.g.kl || E.g.k0 >= E.g.kr))
+  if (E.d.coords == VIRAL_MILNE && (E.d.quirks & VIRAL_Q_K0_BROADCAST) && !E.d.lbi &&
+      (E.g.k0 < E.g.kl || E.g.k0 >= E.g.kr))
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint needs k0 inside the interior");
   if (int r = viral_adjoint_end(ctx)) return r;
   const size_t bytes = (size_t)NF * E.g.n * sizeof(double);
@@ -3746,7 +3816,8 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   HIPCHK(hipMemsetAsync(vb1, 0, SV * n * sizeof(double), ctx->stream));
   HIPCHK(hipMemsetAsync(vb2, 0, SV * n * sizeof(double), ctx->stream));
   HIPCHK(hipMemcpyAsync(saved, E.st[ctx->cur], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
-  const bool k0b = (p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi;
+  const bool cart = ctx->cart != 0;
+  const bool k0b = !cart && (p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi;  // TXYZ5 has no k0 broadcast
   const int ndir = p.lbi ? 2 : 3;
   std::vector<const double*> sp;  // the segment's stage states: sp[idx] feeds stage idx
   auto adj_timer = std::make_unique<KTimer>(ctx, K_ADJ);  // stops before the gradient's download
@@ -3802,9 +3873,14 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         }
         HIPCHK(hipMemsetAsync(lamB, 0, fbytes, ctx->stream));
         hipLaunchKernelGGL(k_adj_pass, dim3(nblocks(g.n)), dim3(BLK), 0, ctx->stream, g, lamA, lamB);
-        hipLaunchKernelGGL(k_adj_root, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, so, lamA, lamB, vb0, vb1, vb2,
-                           resb);
-        hipLaunchKernelGGL(k_adj_src, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB, dubuf, partial);
+        with_bool(cart, [&](auto CT) {
+          constexpr bool CART = decltype(CT)::value;
+          hipLaunchKernelGGL(k_adj_root<CART>, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, so, lamA, lamB, vb0,
+                             vb1, vb2, resb);
+          hipLaunchKernelGGL(k_adj_src<CART>, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB, dubuf,
+                             partial);
+        });
+        const double ttile = cart ? 1.0 : a.tau_src;  // the Cartesian flux has no tau factors
         for (int d = 0; d < ndir; d++) {
           // lines of direction d (interior in the other two), TK per block, each cut into nseg segments
           // of W - 10 own cells; (nseg, W, TK) minimises blocks per line, nseg / TK, with at least
@@ -3827,15 +3903,18 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
             if (W <= 16) break;
           }
           const unsigned nt = (unsigned)(((nlines + bT - 1) / bT) * bs);
-          if (d == 0)
-            hipLaunchKernelGGL(k_adj_tile<0>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB,
-                               bW, bT, bs);
-          if (d == 1)
-            hipLaunchKernelGGL(k_adj_tile<1>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB,
-                               bW, bT, bs);
-          if (d == 2)
-            hipLaunchKernelGGL(k_adj_tile<2>, dim3(nt), dim3(NTA), 0, ctx->stream, g, p, a.tau_src, si, resb, dubuf, lamB,
-                               bW, bT, bs);
+          with_bool(cart, [&](auto CT) {
+            constexpr bool CART = decltype(CT)::value;
+            if (d == 0)
+              hipLaunchKernelGGL((k_adj_tile<0, CART>), dim3(nt), dim3(NTA), 0, ctx->stream, g, p, ttile, si, resb,
+                                 dubuf, lamB, bW, bT, bs);
+            if (d == 1)
+              hipLaunchKernelGGL((k_adj_tile<1, CART>), dim3(nt), dim3(NTA), 0, ctx->stream, g, p, ttile, si, resb,
+                                 dubuf, lamB, bW, bT, bs);
+            if (d == 2)
+              hipLaunchKernelGGL((k_adj_tile<2, CART>), dim3(nt), dim3(NTA), 0, ctx->stream, g, p, ttile, si, resb,
+                                 dubuf, lamB, bW, bT, bs);
+          });
         }
         std::swap(lamA, lamB);
       }
