@@ -86,60 +86,56 @@ VHD double weno_grad(const double x[5], bool right, double g[5])
 //       = sum_c (sL wL_c dqL_c/dx_i + sR wR_c dqR_c/dx_i) + sum_c k_c db_c/dx_i,
 //   k_c = -2 / (eps + b_c) (sL (qL_c - L) aL_c / SL + sR (qR_c - R) aR_c / SR),
 // with the notation of weno_grad.  One set of b_c, 1/(eps + b_c) and db/dx
-// serves both sides.
+// serves both sides; the three 1/(eps + b_c) come from one reciprocal of
+// their product and the two weight sums from one more, and the sparse
+// stencil coefficients are written out.
 VHD void weno_lr_vjp(const double x[5], double sL, double sR, double& L, double& R, double g[5])
 {
-  const double Ac[3][5] = {{0, 0, 1, -2, 1}, {0, 1, -2, 1, 0}, {1, -2, 1, 0, 0}};
-  const double Cc[3][5] = {{0, 0, 3, -4, 1}, {0, 1, 0, -1, 0}, {1, -4, 3, 0, 0}};
-  const double QL[3][5] = {{0, 0, 2, 5, -1}, {0, -1, 5, 2, 0}, {2, -7, 11, 0, 0}};
-  const double QR[3][5] = {{0, 0, 11, -7, 2}, {0, 2, 5, -1, 0}, {-1, 5, 2, 0, 0}};
-  const double dL[3] = {0.3, 0.6, 0.1}, dR[3] = {0.1, 0.6, 0.3};
-  double A[3], C[3], it[3], aL[3], aR[3], qL[3], qR[3], SL = 0, SR = 0;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    A[c] = C[c] = qL[c] = qR[c] = 0;
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-      A[c] += Ac[c][i] * x[i];
-      C[c] += Cc[c][i] * x[i];
-      qL[c] += QL[c][i] * x[i];
-      qR[c] += QR[c][i] * x[i];
-    }
-    qL[c] = qL[c] * (1.0 / 6.0);
-    qR[c] = qR[c] * (1.0 / 6.0);
-    it[c] = frcp(WENO_EPS + (13. / 12.) * (A[c] * A[c]) + 0.25 * (C[c] * C[c]));
-    const double i2 = it[c] * it[c];
-    aL[c] = dL[c] * i2;
-    aR[c] = dR[c] * i2;
-    SL += aL[c];
-    SR += aR[c];
-  }
-  const double rL = frcp(SL), rR = frcp(SR);
-  double wL[3], wR[3];
-  L = 0;
-  R = 0;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    wL[c] = aL[c] * rL;
-    wR[c] = aR[c] * rR;
-    L += wL[c] * qL[c];
-    R += wR[c] * qR[c];
-  }
-  double k[3], hL[3], hR[3];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    k[c] = -2.0 * it[c] * (sL * (qL[c] - L) * wL[c] + sR * (qR[c] - R) * wR[c]);
-    hL[c] = sL * wL[c] * (1.0 / 6.0);
-    hR[c] = sR * wR[c] * (1.0 / 6.0);
-  }
-#pragma unroll
-  for (int i = 0; i < 5; i++) {
-    double gi = 0;
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-      gi += hL[c] * QL[c][i] + hR[c] * QR[c][i] + k[c] * ((13. / 6.) * A[c] * Ac[c][i] + 0.5 * C[c] * Cc[c][i]);
-    g[i] = gi;
-  }
+  const double qm2 = x[0], qm1 = x[1], qc = x[2], qp1 = x[3], qp2 = x[4];
+  // smoothness indicators b_c = 13/12 a_c^2 + 1/4 c_c^2 and s_c = 4 (eps + b_c)
+  const double a0 = qc - 2.0 * qp1 + qp2, c0 = 3.0 * qc - 4.0 * qp1 + qp2;
+  const double a1 = qm1 - 2.0 * qc + qp1, c1 = qm1 - qp1;
+  const double a2 = qm2 - 2.0 * qm1 + qc, c2 = qm2 - 4.0 * qm1 + 3.0 * qc;
+  const double k4 = 13.0 / 3.0, e4 = 4.0 * WENO_EPS;
+  const double s0 = k4 * a0 * a0 + c0 * c0 + e4;
+  const double s1 = k4 * a1 * a1 + c1 * c1 + e4;
+  const double s2 = k4 * a2 * a2 + c2 * c2 + e4;
+  // 1/s_c from one reciprocal of the product
+  const double s12 = s1 * s2, inv3 = frcp1(s0 * s12);
+  const double i0 = s12 * inv3, i1 = s0 * s2 * inv3, i2 = s0 * s1 * inv3;
+  const double j0 = i0 * i0, j1 = i1 * i1, j2 = i2 * i2;
+  // unnormalised weights d_c / s_c^2 (d = 0.3, 0.6, 0.1 for L; 0.1, 0.6, 0.3 for R)
+  const double uL0 = 0.3 * j0, uL1 = 0.6 * j1, uL2 = 0.1 * j2;
+  const double uR0 = 0.1 * j0, uR1 = 0.6 * j1, uR2 = 0.3 * j2;
+  const double DL = uL0 + uL1 + uL2, DR = uR0 + uR1 + uR2;
+  const double inv2 = frcp1(DL * DR);
+  const double rL = DR * inv2, rR = DL * inv2;
+  const double wL0 = uL0 * rL, wL1 = uL1 * rL, wL2 = uL2 * rL;
+  const double wR0 = uR0 * rR, wR1 = uR1 * rR, wR2 = uR2 * rR;
+  // candidate values
+  const double o6 = 1.0 / 6.0;
+  const double qL0 = (2.0 * qc + 5.0 * qp1 - qp2) * o6, qL1 = (-qm1 + 5.0 * qc + 2.0 * qp1) * o6,
+               qL2 = (2.0 * qm2 - 7.0 * qm1 + 11.0 * qc) * o6;
+  const double qR0 = (11.0 * qc - 7.0 * qp1 + 2.0 * qp2) * o6, qR1 = (2.0 * qm1 + 5.0 * qc - qp1) * o6,
+               qR2 = (-qm2 + 5.0 * qm1 + 2.0 * qc) * o6;
+  L = wL0 * qL0 + wL1 * qL1 + wL2 * qL2;
+  R = wR0 * qR0 + wR1 * qR1 + wR2 * qR2;
+  // k_c = -2/(eps + b_c) (sL (qL_c - L) wL_c + sR (qR_c - R) wR_c), 1/(eps + b_c) = 4/s_c;
+  // db_c = 13/6 a_c da_c + 1/2 c_c dc_c
+  const double m0 = -8.0 * i0 * (sL * (qL0 - L) * wL0 + sR * (qR0 - R) * wR0);
+  const double m1 = -8.0 * i1 * (sL * (qL1 - L) * wL1 + sR * (qR1 - R) * wR1);
+  const double m2 = -8.0 * i2 * (sL * (qL2 - L) * wL2 + sR * (qR2 - R) * wR2);
+  const double ka0 = (13.0 / 6.0) * a0 * m0, ka1 = (13.0 / 6.0) * a1 * m1, ka2 = (13.0 / 6.0) * a2 * m2;
+  const double kc0 = 0.5 * c0 * m0, kc1 = 0.5 * c1 * m1, kc2 = 0.5 * c2 * m2;
+  // the candidates' linear parts, weighted
+  const double hL0 = sL * wL0 * o6, hL1 = sL * wL1 * o6, hL2 = sL * wL2 * o6;
+  const double hR0 = sR * wR0 * o6, hR1 = sR * wR1 * o6, hR2 = sR * wR2 * o6;
+  g[0] = 2.0 * hL2 - hR2 + ka2 + kc2;
+  g[1] = -hL1 - 7.0 * hL2 + 2.0 * hR1 + 5.0 * hR2 + ka1 - 2.0 * ka2 + kc1 - 4.0 * kc2;
+  g[2] = 2.0 * hL0 + 5.0 * hL1 + 11.0 * hL2 + 11.0 * hR0 + 5.0 * hR1 + 2.0 * hR2 + ka0 - 2.0 * ka1 + ka2 +
+         3.0 * kc0 + 3.0 * kc2;
+  g[3] = 5.0 * hL0 + 2.0 * hL1 - 7.0 * hR0 - hR1 - 2.0 * ka0 + ka1 - 4.0 * kc0 - kc1;
+  g[4] = -hL0 + 2.0 * hR0 + ka0 + kc0;
 }
 
 // P(e), T(e), s(e) and their e-derivatives as the forward step evaluates

@@ -501,10 +501,14 @@ __host__ __device__ __forceinline__ double frcp(double x)
 // step squares that, to within about ten ulp of the quotient (2.2e-15 relative
 // at most over 4M random normal x of both signs and exponents -300..300).  For
 // the WENO weights' normalisation, where a few ulp do not matter.
-__device__ __forceinline__ double frcp1(double x)
+__host__ __device__ __forceinline__ double frcp1(double x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
   const double r = __builtin_amdgcn_rcp(x);
   return fma(r, fma(-x, r, 1.0), r);
+#else
+  return 1.0 / x;  // host builds (the adjoint checks) use the IEEE quotient
+#endif
 }
 
 // Same WENO5-JS weights written with one division per reconstruction:
