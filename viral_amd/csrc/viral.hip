@@ -220,8 +220,10 @@ __device__ __forceinline__ int sched_find(const Sched& s, long w, long& local)
 // the full pi^{mu nu} that cart_pi10 derives per stencil cell.
 template <bool FAST, bool CART>
 __device__ __forceinline__ void flux_face(const Geom& g, const Params& p, int d, double tau,
-                                          const double* __restrict__ st, double* __restrict__ flux, long t)
+                                          const double* __restrict__ st_, double* __restrict__ flux_, long t)
 {
+  const auto* __restrict__ st = as_global(st_);
+  auto* __restrict__ flux = as_global(flux_);
   int i, j, k;
   Box b = face_box(g, d);
   if (!box_cell(b, t, i, j, k)) return;
@@ -267,7 +269,7 @@ __device__ __forceinline__ void flux_face(const Geom& g, const Params& p, int d,
     amax = fmax(ml, mr);
   }
   const double tf = milne ? tau : 1.0;
-  const double* __restrict__ base[6];
+  decltype(st) base[6];
 #pragma unroll
   for (int o = 0; o < 6; o++) base[o] = st + x + (o - 2) * s;
   double piT[6][4];  // Cartesian pi^{row,m} of each stencil cell
@@ -658,9 +660,9 @@ __global__ void __launch_bounds__(BLK) k_shasta_dir(const EvDev* __restrict__ ta
 // evolved components (rows 4..8) from the full tensor, and under q9 only in
 // ghosted eta slices k < nz (the reference's loop k = 0..ZCMA-1,
 // TXYZ5/Source.h:272; other slices keep zero sources).
-template <bool FAST, bool CART>
+template <bool FAST, bool CART, class Ptr>
 __device__ __forceinline__ void source_terms(const Geom& g, const Params& p, const StageArgs& a,
-                                             const double* __restrict__ si, long x, double* S)
+                                             Ptr __restrict__ si, long x, double* S)
 {
   const long n = g.n;
   const int milne = CART ? 0 : p.milne;
@@ -738,7 +740,7 @@ __device__ __forceinline__ void source_terms(const Geom& g, const Params& p, con
   const double hx[3] = {p.dx, p.dy, p.deta};
 #pragma unroll
   for (int l = 0; l < 4; l++) {
-    const double* ul = si + (VIRAL_F_U0 + l) * n;
+    const auto ul = si + (VIRAL_F_U0 + l) * n;
     du[l][0] = FAST ? (u[l] - pu[l]) * (1.0 / a.ts_src) : (u[l] - pu[l]) / a.ts_src;
     du[l][1] = wder<FAST>(ul[x - 2 * g.sx], ul[x - g.sx], ul[x], ul[x + g.sx], ul[x + 2 * g.sx], hx[0], 1.0 / hx[0]);
     du[l][2] = wder<FAST>(ul[x - 2 * g.sy], ul[x - g.sy], ul[x], ul[x + g.sy], ul[x + 2 * g.sy], hx[1], 1.0 / hx[1]);
@@ -794,10 +796,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(FAST ?
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
     const long x = cidx(E.g, i, j, k), n = E.g.n;
     double S[SV];
-    source_terms<FAST, CART>(E.g, E.p, a, E.st[cur], x, S);
+    source_terms<FAST, CART>(E.g, E.p, a, as_global((const double*)E.st[cur]), x, S);
+    auto* __restrict__ src = as_global(E.src);
 #pragma unroll
     for (int l = 0; l < SV; l++)
-      if (!CART || l < 9 || l == 14) E.src[l * n + x] = S[l];
+      if (!CART || l < 9 || l == 14) src[l * n + x] = S[l];
   }
 }
 
@@ -874,13 +877,27 @@ __device__ __forceinline__ void solve_cell(const Params& p, int milne, double tn
 
 template <bool FAST, bool FUSED, bool CART>
 __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, const StageArgs& a, long long step,
-                                            const double* __restrict__ si, double* __restrict__ so,
-                                            const double* __restrict__ fx, const double* __restrict__ fy,
-                                            const double* __restrict__ fz, const double* __restrict__ src,
-                                            double* __restrict__ var0, double* __restrict__ var1,
-                                            double* __restrict__ var2, int32_t* __restrict__ iters,
-                                            uint8_t* __restrict__ flags, unsigned long long* __restrict__ bad, long t)
+                                            const double* __restrict__ si_, double* __restrict__ so_,
+                                            const double* __restrict__ fx_, const double* __restrict__ fy_,
+                                            const double* __restrict__ fz_, const double* __restrict__ src_,
+                                            double* __restrict__ var0_, double* __restrict__ var1_,
+                                            double* __restrict__ var2_, int32_t* __restrict__ iters_,
+                                            uint8_t* __restrict__ flags_, unsigned long long* __restrict__ bad, long t)
 {
+  // global-address-space views of the event's arrays (the table hands them
+  // over as generic pointers, whose flat accesses also count against the
+  // scalar-load wait counter)
+  const auto* __restrict__ si = as_global(si_);
+  auto* __restrict__ so = as_global(so_);
+  const auto* __restrict__ fx = as_global(fx_);
+  const auto* __restrict__ fy = as_global(fy_);
+  const auto* __restrict__ fz = as_global(fz_);
+  const auto* __restrict__ src = as_global(src_);
+  auto* __restrict__ var0 = as_global(var0_);
+  auto* __restrict__ var1 = as_global(var1_);
+  auto* __restrict__ var2 = as_global(var2_);
+  auto* __restrict__ iters = as_global(iters_);
+  auto* __restrict__ flags = as_global(flags_);
   int i, j, k;
   if (!slab_cell(interior_box(g), t, i, j, k)) return;
   const long n = g.n, x = cidx(g, i, j, k);
@@ -902,7 +919,7 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
 #pragma unroll
     for (int l = 0; l < SV; l++) Res[l] = (CART && l >= 9 && l < 14) ? 0.0 : fx[l * n + x];
   } else {
-    const double* fl[3] = {fx, fy, fz};
+    decltype(fx) fl[3] = {fx, fy, fz};
     const long sd[3] = {g.sx, g.sy, 1};
     const double hh[3] = {p.dx, p.dy, p.deta};
     const int ndir = p.lbi ? 2 : 3;
@@ -911,7 +928,7 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
 #pragma unroll
     for (int dd = 0; dd < 3; dd++) {
       if (dd >= ndir) break;
-      const double* f = fl[dd];
+      const auto f = fl[dd];
 #pragma unroll
       for (int l = 0; l < SV; l++) {
         if (CART && l >= 9 && l < 14) continue;
