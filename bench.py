@@ -381,9 +381,9 @@ def bench_c3(steps, tau0=0.6, ts=0.002):
     the state after two warm-up steps: the SHASTA step (the configuration's
     scheme) with the full-step sub-step (VIRAL_Q_SHASTA_FULLSTEP; the
     reference's dtau/2 makes the grid-scale checkerboard grow 2.5x per step in
-    3-D, so its step fails within 5 steps on this state), and the viscous KT
-    step (autotuned) on the same state, with the update kernel's HBM
-    figure."""
+    3-D, so its step fails within 5 steps on this state; the library's
+    default launch), and the viscous KT step (autotuned) on the same state,
+    with the update kernel's HBM figure."""
     import torch
     from viral_amd import capi, ic
     from viral_amd.desc import make_desc, EOS_S95P, ETA_TIMES_S, TPI_OVER_EP, SHASTA, Q_DEFAULT, Q_SHASTA_FULLSTEP
@@ -406,7 +406,7 @@ def bench_c3(steps, tau0=0.6, ts=0.002):
             r = h.step(tau, ts)
             assert r == 0, r
             tau += ts
-        if scheme == "kt":
+        if scheme == "kt":  # SHASTA keeps the library's default launch
             h.autotune(reps=1)
         h.timers(True)
         torch.cuda.synchronize()

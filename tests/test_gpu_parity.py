@@ -4,6 +4,8 @@ In exact mode the kernels evaluate every expression in the reference's
 operation order, so the bar is bitwise equality of all 29 state fields, of
 the per-stage Newton iteration counts and of the per-cell flags.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -109,6 +111,26 @@ def test_shasta_bitwise(rk):
     st = ic.random_smooth(d, seed=31 + rk, tau=tau)
     h, o = run_both(d, st, tau, ts, 2)
     assert_bitwise(h, o, rk)
+
+
+def test_shasta_launch_options_other_than_its_tested_configuration_are_refused():
+    """SHASTA runs its LDS line tiles with 256-thread blocks, one tile per
+    block and the sources in their own kernel; persistent blocks, source
+    fusion, other block sizes and the autotuner are refused (a host check:
+    no kernel runs)"""
+    from viral_amd.desc import SHASTA, ERR_UNSUPPORTED
+    d = make_desc(12, 10, 6, scheme=SHASTA, fast_math=1)
+    h = capi.Hydro(d)
+    L = h.launch()
+    for field, value in (("blocks_per_cu", 2), ("fusion", 1)):
+        c = h.launch()
+        setattr(c, field, value)
+        assert h.L.viral_set_launch(h.h, C.byref(c)) == ERR_UNSUPPORTED, field
+    c = h.launch()
+    c.flux_nt[0] = 128
+    assert h.L.viral_set_launch(h.h, C.byref(c)) == ERR_UNSUPPORTED
+    assert h.L.viral_set_launch(h.h, C.byref(L)) == OK
+    assert h.L.viral_autotune(h.h, -1, 1) == ERR_UNSUPPORTED
 
 
 def test_shasta_s95p_lbi_bitwise():

@@ -34,9 +34,15 @@ def test_exact_and_shasta_flux_phases():
     faces = [(18 * 9 * 5), (17 * 10 * 5), (17 * 9 * 6)]
     for q in range(3):
         assert capi.tile_count(d, capi.PHASE_FLUX_X + q) == blocks(faces[q])
+    # SHASTA runs on the same LDS line tiles as fast KT, in both modes
     d.scheme = SHASTA
+    n = [17, 9, 5]
     for q in range(3):
-        assert capi.tile_count(d, capi.PHASE_FLUX_X + q) == blocks(17 * 9 * 5)
+        other = [n[a] for a in range(3) if a != q]
+        for tp, tk in ((64, 16), (8, 2)):
+            assert capi.tile_count(d, capi.PHASE_FLUX_X + q, tp, tk) == \
+                math.ceil(other[0] * other[1] / tk) * math.ceil(n[q] / (tp - 6))
+    assert capi.tile_count(d, capi.PHASE_FLUX_X, 7, 4) == -1
     d2 = make_desc(12, 10, 1, lbi=1)
     assert capi.tile_count(d2, capi.PHASE_FLUX_ETA) == 0
 
