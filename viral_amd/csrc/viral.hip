@@ -568,102 +568,20 @@ __global__ void __launch_bounds__(NT) k_flux_tile(const EvDev* __restrict__ tab,
   }
 }
 
-// SHASTA flux-corrected transport along direction d for one interior cell c
-// (NEWEST10/hydroshasta.h:8-32, 101-284): transport velocity v (row 0:
-// T^{d,tau}/T^{tau tau}, other rows u^d/u^0), low-order transported UTD at
-// c-2..c+2, antidiffusion A at c-1 and c, partial result
-//   (-U + UTD - A_c + A_{c-1}) / taustep.
-// The Q+- factors of the rows that share u^d/u^0 are computed once.
-template <bool FAST>
-__device__ __forceinline__ void shasta_cell(const Geom& g, const Params& p, int d, double tau, double taustep,
-                                            const double* __restrict__ st_, double* __restrict__ pr_, long t)
-{
-  auto* __restrict__ st = as_global(st_);
-  auto* __restrict__ pr = as_global(pr_);
-  int i, j, k;
-  if (!box_cell(interior_box(g), t, i, j, k)) return;
-  const long n = g.n, s = dstride(g, d), x = cidx(g, i, j, k);
-  const int milne = p.milne;
-  const double hh = d == 0 ? p.dx : d == 1 ? p.dy : p.deta;
-  const double lam = taustep / hh;
-  double v0[7], vr[7];
-#pragma unroll
-  for (int o = 0; o < 7; o++) {
-    const long y = x + (o - 3) * s;
-    const double e = st[VIRAL_F_EN * n + y], P = st[VIRAL_F_P * n + y], PI = st[VIRAL_F_BULK * n + y];
-    const double u0 = st[VIRAL_F_U0 * n + y], ud = st[(VIRAL_F_U1 + d) * n + y];
-    const double pd = st[(VIRAL_F_PI0 + 1 + d) * n + y];
-    v0[o] = (pd + ud * u0 * (e + P - PI)) / st[VIRAL_F_T00 * n + y];
-    vr[o] = ud / u0;
-  }
-  // Q+ and Q- at stencil positions 1..5 (cells c-2..c+2)
-  double qp0[5], qm0[5], qpr[5], qmr[5];
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    const int o = m + 1;
-    qp0[m] = (0.5 - v0[o] * lam) / (1.0 + (v0[o + 1] - v0[o]) * lam);
-    qm0[m] = (0.5 + v0[o] * lam) / (1.0 - (v0[o - 1] - v0[o]) * lam);
-    qpr[m] = (0.5 - vr[o] * lam) / (1.0 + (vr[o + 1] - vr[o]) * lam);
-    qmr[m] = (0.5 + vr[o] * lam) / (1.0 - (vr[o - 1] - vr[o]) * lam);
-  }
-  const double tf = milne ? tau : 1.0;
-#pragma unroll 1
-  for (int l = 0; l < SV; l++) {
-    double U[7];
-#pragma unroll
-    for (int o = 0; o < 7; o++) {
-      const long y = x + (o - 3) * s;
-      const double vv = st[(long)l * n + y];
-      U[o] = (l < 4 && milne) ? tf * vv : vv;
-    }
-    double utd[5];
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-      const int o = m + 1;
-      const double Qp = l == 0 ? qp0[m] : qpr[m], Qm = l == 0 ? qm0[m] : qmr[m];
-      utd[m] = 0.5 * ((Qp * Qp * (U[o + 1] - U[o])) + (Qm * Qm * (U[o - 1] - U[o]))) + (Qp + Qm) * U[o];
-    }
-    // antidiffusion at c-1 (uses utd[0..3]) and c (utd[1..4])
-    double A[2];
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      double signa = 1;
-      const double temp = 1 * 0.125 * (utd[q + 2] - utd[q + 1]);
-      if (temp < 0) signa = -1;
-      const double term1 = signa * (utd[q + 3] - utd[q + 2]);
-      const double term3 = signa * (utd[q + 1] - utd[q]);
-      const double mn = fmin(fmin(term1, fabs(temp)), term3);
-      A[q] = signa * fmax(0, mn);
-    }
-    const double num = -U[3] + utd[2] - A[1] + A[0];
-    pr[l * n + x] = FAST ? num * (1.0 / taustep) : num / taustep;
-  }
-}
-
-template <bool FAST, bool PERSIST>
-__global__ void __launch_bounds__(BLK) k_shasta_dir(const EvDev* __restrict__ tab, Sched s,
-                                                    const Clock* __restrict__ clk, int stage, int cur, int d)
-{
-  TILE_LOOP(s)
-  {
-    long lb;
-    const EvDev& E = tab[sched_find(s, w_, lb)];
-    const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
-    const double taustep = (E.p.quirks & VIRAL_Q_SHASTA_FULLSTEP) ? a.ts : a.taustep;
-    shasta_cell<FAST>(E.g, E.p, d, a.tau_src, taustep, E.st[cur], E.flux[d], lb * BLK + threadIdx.x);
-  }
-}
-
-
-// SHASTA of direction D on an LDS line tile (the KT tiles' layout: TK lines x
-// TP cells, slots with the contiguous eta index fastest, so loads coalesce
-// in every direction).  Every slot loads its own cell once per quantity and
-// the stencil neighbours come from LDS: the cell's v0 and vr, its Q+/Q-
-// (positions 1..TP-2), per row its U, its transported-diffused utd
-// (positions 1..TP-2), and the owned cells (positions 3..TP-4, which have
-// the whole 7-cell stencil in the tile) form the antidiffusion and write
-// the partial result.  Each cell evaluates the same expressions, in the same
-// order, as shasta_cell, so the two agree bit for bit.
+// SHASTA flux-corrected transport along direction D (NEWEST10/hydroshasta.h:
+// 8-32, 101-284): transport velocity v (row 0: T^{D,tau}/T^{tau tau}, other
+// rows u^D/u^0), Q+- factors, low-order transported-diffused UTD, limited
+// antidiffusion A, and per cell the partial result
+//   (-U + UTD_c - A_c + A_{c-1}) / taustep,
+// evaluated in the reference's operation order (exact mode is bitwise equal
+// to the CPU restatement).  On an LDS line tile (the KT tiles' layout: TK
+// lines x TP cells, slots with the contiguous eta index fastest, so loads
+// coalesce in every direction) every slot loads its own cell once per
+// quantity and the stencil neighbours come from LDS: the cell's v0 and vr,
+// its Q+/Q- (positions 1..TP-2), per row its U and UTD (positions 1..TP-2),
+// and the owned cells (positions 3..TP-4, which have the whole 7-cell
+// stencil in the tile) form the antidiffusion and write the partial result.
+// The Q factors of the rows that share u^D/u^0 are computed once.
 template <int D, int NT, bool FAST>
 __device__ __forceinline__ void shasta_tile_body(const Geom& g, const Params& p, double tau, double taustep,
                                                  const double* __restrict__ st_, double* __restrict__ pr_, int TP,
