@@ -1130,17 +1130,21 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
   unsigned fl;
   double en, Temp, Pn, vx, vy, ve, gm;
   solve_cell<FAST>(p, milne, tn, T0, pi, PIn, e, en, Pn, Temp, vx, vy, ve, gm, it, fl);
-  // ---- store
+  // ---- store (with the k0 broadcast on, the stage's broadcast kernel writes
+  // v and u of every slice but k0 right after this kernel: those stores
+  // would be dead)
   so[VIRAL_F_EN * n + x] = en;
   so[VIRAL_F_P * n + x] = Pn;
   so[VIRAL_F_TEMP * n + x] = Temp;
-  so[VIRAL_F_VX * n + x] = vx;
-  so[VIRAL_F_VY * n + x] = vy;
-  so[VIRAL_F_VE * n + x] = ve;
-  so[VIRAL_F_U0 * n + x] = gm;
-  so[VIRAL_F_U1 * n + x] = gm * vx;
-  so[VIRAL_F_U2 * n + x] = gm * vy;
-  so[VIRAL_F_U3 * n + x] = gm * ve;
+  if (!((p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi && k != g.k0)) {
+    so[VIRAL_F_VX * n + x] = vx;
+    so[VIRAL_F_VY * n + x] = vy;
+    so[VIRAL_F_VE * n + x] = ve;
+    so[VIRAL_F_U0 * n + x] = gm;
+    so[VIRAL_F_U1 * n + x] = gm * vx;
+    so[VIRAL_F_U2 * n + x] = gm * vy;
+    so[VIRAL_F_U3 * n + x] = gm * ve;
+  }
 #pragma unroll
   for (int l = 0; l < 4; l++) so[(VIRAL_F_PU0 + l) * n + x] = a.stage == 0 ? u[l] : pu[l];
   iters[x] = it;
