@@ -404,8 +404,8 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps);
  * ghosted arrays), grad[f] receives dJ/d(recorded initial state field f), each
  * field taken as an independent input, and pgrad[0], pgrad[1] receive dJ/d
  * eta_coef and dJ/d tpi_coef.  The state is left as it was.  Single-event KT
- * contexts in Milne or Cartesian coordinates, conformal or s95p EOS (not
- * SHASTA, the bulk source or the FIX build: VIRAL_ERR_UNSUPPORTED).
+ * contexts in Milne or Cartesian coordinates, conformal or s95p EOS, with or
+ * without the bulk source (not SHASTA or the FIX build: VIRAL_ERR_UNSUPPORTED).
  * viral_adjoint_end frees the recording.
  *
  * every == 0 keeps every stage's output state instead of checkpoints (the

@@ -280,7 +280,8 @@ void ah_source_cart_vjp(int eos, int eta_law, double eta_coef, int tpi_law, doub
   }
   for (int l = 0; l < 5; l++) Sb[4 + l] = Sb5[l];
   double eb = 0, Pb = 0, ub[4] = {0, 0, 0, 0}, peb[5] = {0, 0, 0, 0, 0}, ceb = 0, ctb = 0;
-  source_vjp_cart(p, e, P, u, pe, true, du, Sb, eb, Pb, ub, peb, dub, ceb, ctb);
+  double PIb = 0;
+  source_vjp_cart(p, e, P, 0.0, u, pe, true, du, Sb, eb, Pb, PIb, ub, peb, dub, ceb, ctb);
   inb[0] = eb;
   inb[1] = Pb;
   for (int l = 0; l < 4; l++) inb[2 + l] = ub[l];
@@ -288,6 +289,48 @@ void ah_source_cart_vjp(int eos, int eta_law, double eta_coef, int tpi_law, doub
   for (int a = 0; a < 16; a++) inb[11 + a] = dub[a / 4][a % 4];
   inb[27] = ceb;
   inb[28] = ctb;
+}
+
+// ---- bulk source: forward (as the step evaluates it) and reverse
+static Params bulk_params(int eos, int zeta_law, double zeta_coef, int tpib_law, double tpib_coef, int milne)
+{
+  Params p = make_params(0, 0.2, 0, 5.0, eos);
+  p.milne = milne;
+  p.bulk = 1;
+  p.zeta_law = zeta_law;
+  p.zeta_coef = zeta_coef;
+  p.tpib_law = tpib_law;
+  p.tpib_coef = tpib_coef;
+  return p;
+}
+double ah_bulk(int eos, int zeta_law, double zeta_coef, int tpib_law, double tpib_coef, int milne, double tau,
+               double e, double P, const double* u, double PI, const double* du16)
+{
+  Params p = bulk_params(eos, zeta_law, zeta_coef, tpib_law, tpib_coef, milne);
+  double du[4][4];
+  for (int a = 0; a < 16; a++) du[a / 4][a % 4] = du16[a];
+  const double T = eos_T(p, e);
+  const double s = eos_s(p, e, P, T);
+  const double BU = zeta_of(p, s);
+  return bulk_source<false>(milne, tau, u, PI, du, BU, tpib_of(p, BU, P));
+}
+void ah_bulk_vjp(int eos, int zeta_law, double zeta_coef, int tpib_law, double tpib_coef, int milne, double tau,
+                 double e, double P, const double* u, double PI, const double* du16, double Sb,
+                 double* inb /* e, P, PI, u4, du16 = 23 */)
+{
+  Params p = bulk_params(eos, zeta_law, zeta_coef, tpib_law, tpib_coef, milne);
+  double du[4][4], dub[4][4];
+  for (int a = 0; a < 16; a++) {
+    du[a / 4][a % 4] = du16[a];
+    dub[a / 4][a % 4] = 0;
+  }
+  double eb = 0, Pb = 0, PIb = 0, ub[4] = {0, 0, 0, 0};
+  bulk_vjp(p, milne, tau, e, P, u, PI, du, Sb, eb, Pb, PIb, ub, dub);
+  inb[0] = eb;
+  inb[1] = Pb;
+  inb[2] = PIb;
+  for (int l = 0; l < 4; l++) inb[3 + l] = ub[l];
+  for (int a = 0; a < 16; a++) inb[7 + a] = dub[a / 4][a % 4];
 }
 
 }  // extern "C"

@@ -50,6 +50,11 @@ def lib():
                                  dp, dp]
     L.ah_source_cart_vjp.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, dp,
                                      dp, dp, dp, dp]
+    L.ah_bulk.restype = C.c_double
+    L.ah_bulk.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                          C.c_double, dp, C.c_double, dp]
+    L.ah_bulk_vjp.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                              C.c_double, dp, C.c_double, dp, C.c_double, dp]
     return L
 
 
@@ -363,4 +368,30 @@ def test_source_cart_vjp(eos, laws):
         ob = rng.standard_normal(5)
         inb = np.zeros(29)
         L.ah_source_cart_vjp(eos, el, ec, tl, tc, x[0], x[1], P(x[2:6]), P(x[6:11]), P(x[11:27]), P(ob), P(inb))
+        dot_check(f, x, inb, ob, rng)
+
+
+# ---------------------------------------------------------------- bulk
+
+@pytest.mark.parametrize("eos,laws,milne", [(0, (0, 1.0, 0, 0.1), 1), (0, (1, 0.5, 1, 1.5), 1), (1, (1, 0.5, 1, 1.5), 1),
+                                            (1, (0, 0.3, 0, 0.2), 0), (0, (1, 0.5, 1, 1.5), 0)])
+def test_bulk_vjp(eos, laws, milne):
+    """the Israel-Stewart bulk source with zeta(s) and tau_Pi(zeta, P)"""
+    L = lib()
+    zl, zc, tl, tc = laws
+    rng = np.random.default_rng(80 + zl + 2 * tl + 4 * milne)
+    tau = 1.3 if milne else 1.0
+    for _ in range(5):
+        u = rand_u(rng, tau, milne)
+        e = rng.uniform(1, 8)
+        ev = np.zeros(3)
+        L.ah_eos(eos, e, P(ev))
+        x = np.concatenate([[e, ev[0] * (1 + 0.05 * rng.standard_normal()), -0.05 * e * rng.uniform(0.1, 1.0)], u,
+                            rng.standard_normal(16)])
+
+        def f(z):
+            return np.array([L.ah_bulk(eos, zl, zc, tl, tc, milne, tau, z[0], z[1], P(z[3:7]), z[2], P(z[7:23]))])
+        ob = rng.standard_normal(1)
+        inb = np.zeros(23)
+        L.ah_bulk_vjp(eos, zl, zc, tl, tc, milne, tau, x[0], x[1], P(x[3:7]), x[2], P(x[7:23]), ob[0], P(inb))
         dot_check(f, x, inb, ob, rng)

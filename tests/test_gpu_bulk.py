@@ -8,7 +8,8 @@ harness profile) and to the 0+1D bulk Bjorken solution at RK1/2/3
 bit in exact mode (state, Newton iteration counts, flags) on random states
 carrying a bulk pressure, in Milne and Cartesian coordinates and in an
 ensemble; fast mode within 1e-10 after one step; the Navier-Stokes PI of
-viral_ns_init and the kept PI of viral_landau bit for bit."""
+viral_ns_init and the kept PI of viral_landau bit for bit; the adjoint with
+the bulk source against finite differences."""
 import numpy as np
 import pytest
 
@@ -151,7 +152,55 @@ def test_bulk_ns_init_and_landau_bitwise():
         assert np.array_equal(gs[f].view(np.int64), cs[f].view(np.int64)), FIELD_NAMES[f]
 
 
-def test_adjoint_refuses_bulk():
-    d = make_desc(8, 8, 6, bulk=1)
-    h = capi.Hydro(d)
-    assert h.L.viral_adjoint_begin(h.h, 2) == ERR_UNSUPPORTED
+@pytest.mark.parametrize("case", ["milne_rk2", "milne_rk3_s_over_4pi", "cartesian", "milne_s95p_table"])
+def test_bulk_adjoint_matches_fd(case):
+    """the adjoint with the bulk source (bulk_vjp: PI's relaxation, zeta(s)
+    and tau_Pi(zeta, P) in reverse) along a random direction of the state,
+    and the shear coefficients, against four-point central differences of
+    the GPU forward (and of the CPU oracle in exact mode) to 1e-6"""
+    from test_gpu_adjoint import run_gpu, run_cpu, direction, fd4, close, HP, TAU
+    kw = dict(bulk=1, eta_law=ETA_S_OVER_4PI, eta_coef=1.0, tpi_law=TPI_OVER_P, tpi_coef=1.5)
+    if case == "milne_rk2":
+        d = make_desc(10, 9, 7, rk=2, **LAWS["bulktest"], **kw)
+    elif case == "milne_rk3_s_over_4pi":
+        d = make_desc(10, 9, 7, rk=3, **LAWS["s_over_4pi"], **kw)
+    elif case == "cartesian":
+        d = make_desc(10, 9, 8, rk=2, coords=CARTESIAN, **LAWS["s_over_4pi"], **kw)
+    else:
+        d = make_desc(10, 9, 7, rk=2, fast_math=1, eos=EOS_S95P, eos_table=1, **LAWS["s_over_4pi"], **kw)
+    if d.eos == EOS_S95P:
+        dref = d.copy()
+        dref.fast_math = dref.eos_table = 0
+        st = ic.random_smooth(d, seed=95, tau=TAU, pi_frac=0.02, bulk_frac=0.03, e_scale=30.0,
+                              eos_p=lambda e: capi.eos_eval(dref, e)[0].reshape(np.shape(e)))
+    else:
+        st = ic.random_smooth(d, seed=95, tau=TAU, pi_frac=0.02, bulk_frac=0.03)
+    rng = np.random.default_rng(96)
+    seed = rng.standard_normal(st.shape)
+    print("case bulk adjoint %s" % case, flush=True)
+    _, grad, pg = run_gpu(d, st, 3, 2, seed)
+    runs = [run_gpu] + ([run_cpu] if not d.fast_math else [])
+    dx = direction(st, rng)
+    ad = np.sum(grad * dx)
+    for run in runs:
+        fd = fd4(lambda h: np.sum(seed * run(d, st + h * dx, 3)))
+        print("state", run.__name__, ad, fd, abs(ad - fd) / abs(fd), flush=True)
+        assert close(ad, fd), (run.__name__, ad, fd)
+    # the bulk pressure's own direction
+    dP = np.zeros_like(st)
+    dP[F_BULK] = rng.standard_normal(st.shape[1]) * np.abs(st[F_BULK]).max()
+    fd = fd4(lambda h: np.sum(seed * run_gpu(d, st + h * dP, 3)))
+    ad = np.sum(grad * dP)
+    print("PI direction", ad, fd, flush=True)
+    assert close(ad, fd), (ad, fd)
+    for q, name in enumerate(["eta_coef", "tpi_coef"]):
+        c0 = getattr(d, name)
+
+        def J(h):
+            dd = d.copy()
+            setattr(dd, name, c0 * (1 + h))
+            return np.sum(seed * run_gpu(dd, st, 3))
+
+        fd = fd4(J, HP) / c0
+        print(name, pg[q], fd, flush=True)
+        assert close(pg[q], fd), (name, pg[q], fd)

@@ -422,6 +422,9 @@ VHD void shear_source_vjp(int milne, double tau, const double u[4], const double
 VHD void shear_transport_vjp(const Params& p, int milne, double tau, double e, double P, const double u[4],
                              const double pi10[10], const double du[4][4], const double Sb10[10], double& eb,
                              double& Pb, double ub[4], double pib[10], double dub[4][4], double& cetab, double& ctpib);
+VHD void bulk_vjp(const Params& p, int milne, double tau, double e, double P, const double u[4], double PI,
+                  const double du[4][4], double Sb14, double& eb, double& Pb, double& PIb, double ub[4],
+                  double dub[4][4]);
 
 // Sources of one cell (NEWEST10/Source.h:208-221 geometric, :347-517 shear)
 // given the velocity gradients du and the transport coefficients: adds the
@@ -450,6 +453,7 @@ VHD void source_vjp(const Params& p, double tau, double e, double P, double PI, 
   Pb += wb;
   PIb -= wb;
   shear_transport_vjp(p, 1, tau, e, P, u, pi10, du, Sb + 4, eb, Pb, ub, pib, dub, cetab, ctpib);
+  if (p.bulk) bulk_vjp(p, 1, tau, e, P, u, PI, du, Sb[14], eb, Pb, PIb, ub, dub);
 }
 
 // The shear source with its transport coefficients eta(s) and tau_pi in
@@ -492,6 +496,70 @@ VHD void shear_transport_vjp(const Params& p, int milne, double tau, double e, d
     Pb += sb / T;
   } else {
     eb += sb * ds;
+  }
+}
+
+// The Israel-Stewart bulk source (bulk_source, physics.h; NEWEST10/Source.h:
+// 519-527, TXYZ5/Source.h:327-333) with zeta(s) and tau_Pi(zeta, P) in
+// reverse (milne 0: Cartesian): given the adjoint Sb14 of the PI row, adds
+// the adjoint of (e, P, PI, u, du).
+//   out = rhs / u^0 + PI divv,  rhs = -(PI + zeta theta)/tau_Pi - 4/3 PI theta,
+//   theta = sum_a du[a][a] (+ u^0/tau in Milne),
+//   divv = sum_i (du[i][i]/u^0 - u^i du[0][i]/(u^0)^2).
+VHD void bulk_vjp(const Params& p, int milne, double tau, double e, double P, const double u[4], double PI,
+                  const double du[4][4], double Sb14, double& eb, double& Pb, double& PIb, double ub[4],
+                  double dub[4][4])
+{
+  if (Sb14 == 0.0) return;
+  double Pe, dPe, T, dT, s, ds;
+  eos_d(p, e, Pe, dPe, T, dT, s, ds);
+  if (!p.eos) s = (e + P) / T;
+  const double zeta = zeta_of(p, s);
+  const double tPI = tpib_of(p, zeta, P);
+  const double iu0 = 1.0 / u[0];
+  double divv = 0, sdd = 0, sud = 0;
+  for (int i = 1; i < 4; i++) {
+    divv += du[i][i] * iu0 - u[i] * du[0][i] * (iu0 * iu0);
+    sdd += du[i][i];
+    sud += u[i] * du[0][i];
+  }
+  double theta = du[0][0] + du[1][1] + du[2][2] + du[3][3];
+  if (milne) theta = theta + u[0] / tau;
+  const double itp = 1.0 / tPI;
+  const double rhs = -(PI + zeta * theta) * itp - (4.0 / 3.0) * PI * theta;
+  // out = rhs iu0 + PI divv
+  const double rhsb = Sb14 * iu0;
+  double iu0b = Sb14 * rhs;
+  PIb += Sb14 * divv;
+  const double divvb = Sb14 * PI;
+  iu0b += divvb * (sdd - 2.0 * iu0 * sud);
+  for (int i = 1; i < 4; i++) {
+    dub[i][i] += divvb * iu0;
+    dub[0][i] -= divvb * u[i] * iu0 * iu0;
+    ub[i] -= divvb * du[0][i] * iu0 * iu0;
+  }
+  // rhs
+  PIb += rhsb * (-itp - (4.0 / 3.0) * theta);
+  const double thetab = rhsb * (-zeta * itp - (4.0 / 3.0) * PI);
+  double zetab = rhsb * (-theta * itp);
+  const double tPIb = rhsb * (PI + zeta * theta) * itp * itp;
+  for (int a = 0; a < 4; a++) dub[a][a] += thetab;
+  if (milne) ub[0] += thetab / tau;
+  ub[0] -= iu0b * iu0 * iu0;
+  // tau_Pi = c zeta / P (law 1) or c
+  if (p.tpib_law == 1) {
+    zetab += tPIb * p.tpib_coef / P;
+    Pb -= tPIb * p.tpib_coef * zeta / (P * P);
+  }
+  // zeta = c s / 4 pi (law 1) or c
+  if (p.zeta_law == 1) {
+    const double sb = zetab * p.zeta_coef / (4.0 * PIE);
+    if (!p.eos) {  // s = (e + P) / T(e)
+      eb += sb / T - sb * (e + P) / (T * T) * dT;
+      Pb += sb / T;
+    } else {
+      eb += sb * ds;
+    }
   }
 }
 
@@ -605,14 +673,14 @@ VHD void cart_pi10_vjp(const double u[4], const double pe[5], const double ob[10
   peb[4] += p5b;
 }
 
-// Sources of one Cartesian cell (TXYZ5/Source.h:152-155, 272-326): no
+// Sources of one Cartesian cell (TXYZ5/Source.h:152-155, 272-333): no
 // geometric rows; the shear source's xx, yy, xy, xz, yz slots feed Var rows
-// 4..8 (zero where the reference's slice loop stops short, `live` false).
-// Given Sb (15 rows), adds the adjoint of (e, P, u, pe, du) and of the
-// coefficients.
-VHD void source_vjp_cart(const Params& p, double e, double P, const double u[4], const double pe[5], bool live,
-                         const double du[4][4], const double Sb[15], double& eb, double& Pb, double ub[4],
-                         double peb[5], double dub[4][4], double& cetab, double& ctpib)
+// 4..8 and the bulk source row 14 (all zero where the reference's slice loop
+// stops short, `live` false).  Given Sb (15 rows), adds the adjoint of
+// (e, P, PI, u, pe, du) and of the shear coefficients.
+VHD void source_vjp_cart(const Params& p, double e, double P, double PI, const double u[4], const double pe[5],
+                         bool live, const double du[4][4], const double Sb[15], double& eb, double& Pb, double& PIb,
+                         double ub[4], double peb[5], double dub[4][4], double& cetab, double& ctpib)
 {
   if (!live) return;
   double pi10[10], Sb10[10], pib[10];
@@ -621,6 +689,7 @@ VHD void source_vjp_cart(const Params& p, double e, double P, const double u[4],
   for (int l = 0; l < 5; l++) Sb10[cart5(l)] = Sb[4 + l];
   shear_transport_vjp(p, 0, 1.0, e, P, u, pi10, du, Sb10, eb, Pb, ub, pib, dub, cetab, ctpib);
   cart_pi10_vjp(u, pe, pib, ub, peb);
+  if (p.bulk) bulk_vjp(p, 0, 1.0, e, P, u, PI, du, Sb[14], eb, Pb, PIb, ub, dub);
 }
 
 // Solves A^T y = b for a 4x4 A by Gaussian elimination with partial pivoting.

@@ -2017,7 +2017,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2))) k
     if (CART) {
       // TXYZ5's shear-source loop stops short of the top interior slices
       const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)(x % g.gz) < g.kr - g.kl;
-      source_vjp_cart(p, e, P, u, pi, live, du, Lb, eb, Pb, ub, pib, dub, ceb, ctb);
+      source_vjp_cart(p, e, P, PI, u, pi, live, du, Lb, eb, Pb, PIb, ub, pib, dub, ceb, ctb);
     } else {
       source_vjp(p, a.tau_src, e, P, PI, u, pi, du, Lb, eb, Pb, PIb, ub, pib, dub, ceb, ctb);
     }
@@ -3894,7 +3894,6 @@ int viral_adjoint_begin(viral_ctx* ctx, int every)
   if (E.d.scheme != VIRAL_KT)
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint covers the KT step (Milne and Cartesian)");
   if (E.d.quirks & VIRAL_Q_FIX_VE) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint does not cover the FIX build");
-  if (E.d.bulk) return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint does not cover the bulk source");
   if (E.d.coords == VIRAL_MILNE && (E.d.quirks & VIRAL_Q_K0_BROADCAST) && !E.d.lbi &&
       (E.g.k0 < E.g.kl || E.g.k0 >= E.g.kr))
     return fail(ctx, VIRAL_ERR_UNSUPPORTED, "the adjoint needs k0 inside the interior");
