@@ -964,6 +964,36 @@ __device__ __forceinline__ void solve_cell(const Params& p, int milne, double tn
   if (p.t_freeze > 0 && Temp < p.t_freeze) fl |= VIRAL_FLAG_FREEZEOUT;
 }
 
+// The eta ghost layers a cell feeds (NEWEST10/boundary.h:1-275, the eta
+// part of the zero-gradient fill): slices 0..2 from kl, kr..kr+2 from kr-1.
+// The kernels that write a boundary cell's field also write its eta ghost
+// copies, while the cache lines are theirs; the ghost-fill kernel keeps the
+// x and y faces.  Not in boost-invariant grids (no eta ghosts).
+struct EtaGhosts {
+  long lo = -1, hi = -1;  // first ghost of each side, or -1
+  __device__ __forceinline__ EtaGhosts(const Geom& g, int lbi, int i, int j, int k)
+  {
+    if (lbi) return;
+    if (k == g.kl) lo = cidx(g, i, j, 0);
+    if (k == g.kr - 1) hi = cidx(g, i, j, g.kr);
+  }
+  __device__ __forceinline__ bool any() const { return lo >= 0 || hi >= 0; }
+  template <class Ptr>
+  __device__ __forceinline__ void put(Ptr st, long n, int f, double v) const
+  {
+    if (lo >= 0) {
+      st[f * n + lo] = v;
+      st[f * n + lo + 1] = v;
+      st[f * n + lo + 2] = v;
+    }
+    if (hi >= 0) {
+      st[f * n + hi] = v;
+      st[f * n + hi + 1] = v;
+      st[f * n + hi + 2] = v;
+    }
+  }
+};
+
 template <bool FAST, bool FUSED, bool CART>
 __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, const StageArgs& a, long long step,
                                             const double* __restrict__ si_, double* __restrict__ so_,
@@ -1102,6 +1132,28 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
     so[VIRAL_F_U3 * n + x] = r[3];
 #pragma unroll
     for (int l = 0; l < 4; l++) so[(VIRAL_F_PU0 + l) * n + x] = a.stage == 0 ? u[l] : pu[l];
+    const EtaGhosts eg(g, p.lbi, i, j, k);
+    if (eg.any()) {
+#pragma unroll
+      for (int l = 0; l < 4; l++) eg.put(so, n, VIRAL_F_T00 + l, V[l]);
+#pragma unroll
+      for (int l = 0; l < 5; l++) eg.put(so, n, VIRAL_F_PI0 + l, V[4 + l]);
+      // pi slots 5..9 are never written: their ghosts copy the cell's kept value
+#pragma unroll
+      for (int l = 5; l < 10; l++) eg.put(so, n, VIRAL_F_PI0 + l, si[(VIRAL_F_PI0 + l) * n + x]);
+      eg.put(so, n, VIRAL_F_BULK, V[14]);
+      eg.put(so, n, VIRAL_F_EN, En);
+      eg.put(so, n, VIRAL_F_P, eos_p(p, En));
+      eg.put(so, n, VIRAL_F_VX, vx);
+      eg.put(so, n, VIRAL_F_VY, vy);
+      eg.put(so, n, VIRAL_F_VE, ve);
+      eg.put(so, n, VIRAL_F_U0, c0);
+      eg.put(so, n, VIRAL_F_U1, r[1]);
+      eg.put(so, n, VIRAL_F_U2, r[2]);
+      eg.put(so, n, VIRAL_F_U3, r[3]);
+#pragma unroll
+      for (int l = 0; l < 4; l++) eg.put(so, n, VIRAL_F_PU0 + l, a.stage == 0 ? u[l] : pu[l]);
+    }
     iters[x] = it;
     const uint8_t prev = a.stage == 0 ? 0 : flags[x];
     flags[x] = (uint8_t)((prev & ~VIRAL_FLAG_FREEZEOUT) | fl);
@@ -1125,6 +1177,14 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
 #pragma unroll
   for (int l = 0; l < 10; l++) so[(VIRAL_F_PI0 + l) * n + x] = pi[l];
   so[VIRAL_F_BULK * n + x] = PIn;
+  const EtaGhosts eg(g, p.lbi, i, j, k);
+  if (eg.any()) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) eg.put(so, n, VIRAL_F_T00 + l, T0[l]);
+#pragma unroll
+    for (int l = 0; l < 10; l++) eg.put(so, n, VIRAL_F_PI0 + l, pi[l]);
+    eg.put(so, n, VIRAL_F_BULK, PIn);
+  }
   // ---- root and velocities (NEWEST10/root.h:301-424)
   int it;
   unsigned fl;
@@ -1136,7 +1196,8 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
   so[VIRAL_F_EN * n + x] = en;
   so[VIRAL_F_P * n + x] = Pn;
   so[VIRAL_F_TEMP * n + x] = Temp;
-  if (!((p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi && k != g.k0)) {
+  const bool vu = !((p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi && k != g.k0);
+  if (vu) {
     so[VIRAL_F_VX * n + x] = vx;
     so[VIRAL_F_VY * n + x] = vy;
     so[VIRAL_F_VE * n + x] = ve;
@@ -1147,6 +1208,21 @@ __device__ __forceinline__ void update_cell(const Geom& g, const Params& p, cons
   }
 #pragma unroll
   for (int l = 0; l < 4; l++) so[(VIRAL_F_PU0 + l) * n + x] = a.stage == 0 ? u[l] : pu[l];
+  if (eg.any()) {
+    eg.put(so, n, VIRAL_F_EN, en);
+    eg.put(so, n, VIRAL_F_P, Pn);
+    if (vu) {
+      eg.put(so, n, VIRAL_F_VX, vx);
+      eg.put(so, n, VIRAL_F_VY, vy);
+      eg.put(so, n, VIRAL_F_VE, ve);
+      eg.put(so, n, VIRAL_F_U0, gm);
+      eg.put(so, n, VIRAL_F_U1, gm * vx);
+      eg.put(so, n, VIRAL_F_U2, gm * vy);
+      eg.put(so, n, VIRAL_F_U3, gm * ve);
+    }
+#pragma unroll
+    for (int l = 0; l < 4; l++) eg.put(so, n, VIRAL_F_PU0 + l, a.stage == 0 ? u[l] : pu[l]);
+  }
   iters[x] = it;
   uint8_t prev = a.stage == 0 ? 0 : flags[x];
   flags[x] = (uint8_t)((prev & ~VIRAL_FLAG_FREEZEOUT) | fl);
@@ -1174,6 +1250,9 @@ __global__ void __launch_bounds__(BLK) k_update(const EvDev* __restrict__ tab, S
 }
 
 // NEWEST10/root.h:426-449: every eta slice takes the velocity of slice k0
+// GHOSTS: also the eta ghost copies of the cells it writes (the stage's
+// broadcast, whose ghost fill covers only the x and y faces)
+template <bool GHOSTS>
 __device__ __forceinline__ void k0_cell(const Geom& g, const Params& p, double tau, double* __restrict__ st_, long t)
 {
   auto* __restrict__ st = as_global(st_);
@@ -1191,6 +1270,16 @@ __device__ __forceinline__ void k0_cell(const Geom& g, const Params& p, double t
   st[VIRAL_F_U1 * n + x] = u0 * vx;
   st[VIRAL_F_U2 * n + x] = u0 * vy;
   st[VIRAL_F_U3 * n + x] = u0 * ve;
+  const EtaGhosts eg(g, p.lbi, i, j, k);
+  if (GHOSTS && eg.any()) {
+    eg.put(st, n, VIRAL_F_VX, vx);
+    eg.put(st, n, VIRAL_F_VY, vy);
+    eg.put(st, n, VIRAL_F_VE, ve);
+    eg.put(st, n, VIRAL_F_U0, u0);
+    eg.put(st, n, VIRAL_F_U1, u0 * vx);
+    eg.put(st, n, VIRAL_F_U2, u0 * vy);
+    eg.put(st, n, VIRAL_F_U3, u0 * ve);
+  }
 }
 
 template <bool PERSIST>
@@ -1202,7 +1291,7 @@ __global__ void __launch_bounds__(BLK) k_k0_broadcast(const EvDev* __restrict__ 
     long lb;
     const EvDev& E = tab[sched_find(s, w_, lb)];
     const StageArgs a = stage_args(clk->tau, clk->ts, E.p.rk, stage);
-    k0_cell(E.g, E.p, a.tau_new, E.st[cur], lb * BLK + threadIdx.x);
+    k0_cell<true>(E.g, E.p, a.tau_new, E.st[cur], lb * BLK + threadIdx.x);
   }
 }
 
@@ -1213,13 +1302,16 @@ __constant__ int c_bfields[28] = {VIRAL_F_T00, VIRAL_F_T10, VIRAL_F_T20, VIRAL_F
                                   VIRAL_F_PI0 + 2, VIRAL_F_PI0 + 3, VIRAL_F_PI0 + 4, VIRAL_F_PI0 + 5,
                                   VIRAL_F_PI0 + 6, VIRAL_F_PI0 + 7, VIRAL_F_PI0 + 8, VIRAL_F_PI0 + 9, VIRAL_F_BULK};
 
-// Zero-gradient ghost fill of the x, y (and eta) layers of interior rows,
-// NEWEST10/boundary.h:1-275.  Thread = (ghost cell, field); the three ghost
-// slabs are disjoint and read only interior cells.
+// Zero-gradient ghost fill of the x and y layers of interior rows,
+// NEWEST10/boundary.h:1-275 (the eta layers: EtaGhosts).  Thread = (ghost
+// cell, field); the ghost slabs are disjoint and read only interior cells.
 __host__ __device__ inline long ghost_count(const Geom& g, int lbi)
 {
+  // x and y faces; the eta faces are written by the update and k0 kernels
+  // (EtaGhosts)
+  (void)lbi;
   const long nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
-  return 2L * VIRAL_BORDER * (ny * nz + nx * nz + (lbi ? 0 : nx * ny));
+  return 2L * VIRAL_BORDER * (ny * nz + nx * nz);
 }
 
 __device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, double* __restrict__ st_, int q, long t)
@@ -1230,7 +1322,6 @@ __device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, doubl
   const int nx = g.ir - g.il, ny = g.jr - g.jl, nz = g.kr - g.kl;
   const long cx = 2L * VIRAL_BORDER * ny * nz;  // x slabs
   const long cy = 2L * VIRAL_BORDER * nx * nz;  // y slabs
-  const long cz = p.lbi ? 0 : 2L * VIRAL_BORDER * nx * ny;
   const bool mirror = (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && p.milne && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
   long dst, src;
   if (t < cx) {
@@ -1256,18 +1347,6 @@ __device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, doubl
     if (mirror) js = side == 0 ? g.jl + VIRAL_BORDER - h - 1 : g.jr - h - 1;
     dst = cidx(g, i, j, k);
     src = cidx(g, i, js, k);
-  } else if (t < cx + cy + cz) {
-    t -= cx + cy;
-    int h = (int)(t % (2 * VIRAL_BORDER));
-    long r = t / (2 * VIRAL_BORDER);
-    int j = g.jl + (int)(r % ny);
-    int i = g.il + (int)(r / ny);
-    int side = h / VIRAL_BORDER;
-    h = h % VIRAL_BORDER;
-    int k = side == 0 ? h : g.kr + h;
-    int ks = side == 0 ? g.kl : g.kr - 1;
-    dst = cidx(g, i, j, k);
-    src = cidx(g, i, j, ks);
   } else {
     return;
   }
@@ -1437,7 +1516,7 @@ __global__ void __launch_bounds__(BLK) k_root(Geom g, Params p, double tau, doub
 
 __global__ void __launch_bounds__(BLK) k_root_k0(Geom g, Params p, double tau, double* __restrict__ st)
 {
-  k0_cell(g, p, tau, st, (long)blockIdx.x * blockDim.x + threadIdx.x);
+  k0_cell<false>(g, p, tau, st, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- Navier-Stokes initialisation
