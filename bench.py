@@ -119,11 +119,11 @@ def kernel_bytes(name, d, fused):
 # profiles/counters_kt_s95p_glauber_200x200x64_rk3.md lists the passes.
 # The update kernel's figure is for its fused form (sources evaluated inside).
 MEASURED_TRAFFIC = {
-    "update_fused": 2.035e9,
-    "flux_x": 1.008e9,
-    "flux_y": 1.288e9,
-    "flux_eta": 1.269e9,
-    "k0_broadcast": 0.168e9,
+    "update_fused": 2.181e9,
+    "flux_x": 1.106e9,
+    "flux_y": 1.421e9,
+    "flux_eta": 1.293e9,
+    "k0_broadcast": 0.175e9,
 }
 
 
@@ -138,9 +138,9 @@ MEASURED_TRAFFIC = {
 # cycle).  Averages over the six launches of the two steps the bench times
 # per kernel; profiles/counters_ensemble_64_hotspot_events_rk3.md lists the
 # passes.
-ENSEMBLE_TRAFFIC = {"update": 94.02e9}
+ENSEMBLE_TRAFFIC = {"update": 87.37e9}
 ENSEMBLE_COUNTERS = {
-    "update": {"fp64_flops": 2.554e11, "valu_wave_instructions": 4.337e9, "valu_issue_frac": 0.278,
+    "update": {"fp64_flops": 2.554e11, "valu_wave_instructions": 4.603e9, "valu_issue_frac": 0.277,
                "source": "rocprofv3 --pmc passes on bench.py --legs c4 (profiles/counters_ensemble_64_hotspot_"
                          "events_rk3.md)"},
 }
