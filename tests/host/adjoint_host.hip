@@ -280,8 +280,7 @@ void ah_source_cart_vjp(int eos, int eta_law, double eta_coef, int tpi_law, doub
   }
   for (int l = 0; l < 5; l++) Sb[4 + l] = Sb5[l];
   double eb = 0, Pb = 0, ub[4] = {0, 0, 0, 0}, peb[5] = {0, 0, 0, 0, 0}, ceb = 0, ctb = 0;
-  double PIb = 0;
-  source_vjp_cart(p, e, P, 0.0, u, pe, true, du, Sb, eb, Pb, PIb, ub, peb, dub, ceb, ctb);
+  source_vjp_cart(p, e, P, u, pe, true, du, Sb, eb, Pb, ub, peb, dub, ceb, ctb);
   inb[0] = eb;
   inb[1] = Pb;
   for (int l = 0; l < 4; l++) inb[2 + l] = ub[l];

@@ -429,7 +429,8 @@ VHD void bulk_vjp(const Params& p, int milne, double tau, double e, double P, co
 // Sources of one cell (NEWEST10/Source.h:208-221 geometric, :347-517 shear)
 // given the velocity gradients du and the transport coefficients: adds the
 // adjoint of (e, P, PI, u, pi, du) and of the viscosity / relaxation
-// coefficients (eta_coef, tpi_coef) for the source adjoint Sb (15 rows).
+// coefficients (eta_coef, tpi_coef) for the source adjoint Sb (rows 0..13;
+// the bulk row 14 is the caller's, bulk_vjp).
 // The entropy density is (e + P)/T(e) for the conformal EOS and s(e) for
 // s95p, as the forward evaluates it (eos_d).
 VHD void source_vjp(const Params& p, double tau, double e, double P, double PI, const double u[4], const double pi10[10],
@@ -453,7 +454,7 @@ VHD void source_vjp(const Params& p, double tau, double e, double P, double PI, 
   Pb += wb;
   PIb -= wb;
   shear_transport_vjp(p, 1, tau, e, P, u, pi10, du, Sb + 4, eb, Pb, ub, pib, dub, cetab, ctpib);
-  if (p.bulk) bulk_vjp(p, 1, tau, e, P, u, PI, du, Sb[14], eb, Pb, PIb, ub, dub);
+  // the bulk row (Sb[14]) is the caller's: bulk_vjp
 }
 
 // The shear source with its transport coefficients eta(s) and tau_pi in
@@ -675,12 +676,12 @@ VHD void cart_pi10_vjp(const double u[4], const double pe[5], const double ob[10
 
 // Sources of one Cartesian cell (TXYZ5/Source.h:152-155, 272-333): no
 // geometric rows; the shear source's xx, yy, xy, xz, yz slots feed Var rows
-// 4..8 and the bulk source row 14 (all zero where the reference's slice loop
-// stops short, `live` false).  Given Sb (15 rows), adds the adjoint of
-// (e, P, PI, u, pe, du) and of the shear coefficients.
-VHD void source_vjp_cart(const Params& p, double e, double P, double PI, const double u[4], const double pe[5],
-                         bool live, const double du[4][4], const double Sb[15], double& eb, double& Pb, double& PIb,
-                         double ub[4], double peb[5], double dub[4][4], double& cetab, double& ctpib)
+// 4..8 (zero where the reference's slice loop stops short, `live` false).
+// Given Sb (15 rows), adds the adjoint of (e, P, u, pe, du) and of the shear
+// coefficients; the bulk row is the caller's (bulk_vjp).
+VHD void source_vjp_cart(const Params& p, double e, double P, const double u[4], const double pe[5], bool live,
+                         const double du[4][4], const double Sb[15], double& eb, double& Pb, double ub[4],
+                         double peb[5], double dub[4][4], double& cetab, double& ctpib)
 {
   if (!live) return;
   double pi10[10], Sb10[10], pib[10];
@@ -689,7 +690,7 @@ VHD void source_vjp_cart(const Params& p, double e, double P, double PI, const d
   for (int l = 0; l < 5; l++) Sb10[cart5(l)] = Sb[4 + l];
   shear_transport_vjp(p, 0, 1.0, e, P, u, pi10, du, Sb10, eb, Pb, ub, pib, dub, cetab, ctpib);
   cart_pi10_vjp(u, pe, pib, ub, peb);
-  if (p.bulk) bulk_vjp(p, 0, 1.0, e, P, u, PI, du, Sb[14], eb, Pb, PIb, ub, dub);
+  // the bulk row (Sb[14], also zero where `live` is false) is the caller's: bulk_vjp
 }
 
 // Solves A^T y = b for a 4x4 A by Gaussian elimination with partial pivoting.

@@ -2059,8 +2059,10 @@ __global__ void __launch_bounds__(BLK) k_adj_root(Geom g, Params p, StageArgs a,
 // AGPRs (one wave); with it, 88 bytes per lane spill and the kernel runs 9 %
 // faster
 // CART: the Cartesian sources (no geometric rows, the shear tensor built from
-// the five evolved components; source_vjp_cart).
-template <bool CART>
+// the five evolved components; source_vjp_cart).  BULK: the bulk source
+// (bulk_vjp), a separate instantiation so contexts without it keep the
+// shear-only kernel's registers.
+template <bool CART, bool BULK>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2))) k_adj_src(Geom g, Params p, StageArgs a, const double* __restrict__ si,
                                                  const double* __restrict__ resb, double* __restrict__ li,
                                                  double* __restrict__ dubuf, double* __restrict__ partial)
@@ -2096,9 +2098,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2))) k
     if (CART) {
       // TXYZ5's shear-source loop stops short of the top interior slices
       const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)(x % g.gz) < g.kr - g.kl;
-      source_vjp_cart(p, e, P, PI, u, pi, live, du, Lb, eb, Pb, PIb, ub, pib, dub, ceb, ctb);
+      source_vjp_cart(p, e, P, u, pi, live, du, Lb, eb, Pb, ub, pib, dub, ceb, ctb);
+      if (BULK && live) bulk_vjp(p, 0, 1.0, e, P, u, PI, du, Lb[14], eb, Pb, PIb, ub, dub);
     } else {
       source_vjp(p, a.tau_src, e, P, PI, u, pi, du, Lb, eb, Pb, PIb, ub, pib, dub, ceb, ctb);
+      if (BULK) bulk_vjp(p, 1, a.tau_src, e, P, u, PI, du, Lb[14], eb, Pb, PIb, ub, dub);
     }
     li[VIRAL_F_EN * n + x] += eb;
     li[VIRAL_F_P * n + x] += Pb;
@@ -4106,8 +4110,12 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
           constexpr bool CART = decltype(CT)::value;
           hipLaunchKernelGGL(k_adj_root<CART>, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, so, lamA, lamB, vb0,
                              vb1, vb2, resb);
-          hipLaunchKernelGGL(k_adj_src<CART>, dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB, dubuf,
-                             partial);
+          if (p.bulk)
+            hipLaunchKernelGGL((k_adj_src<CART, true>), dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB,
+                               dubuf, partial);
+          else
+            hipLaunchKernelGGL((k_adj_src<CART, false>), dim3(nbi), dim3(BLK), 0, ctx->stream, g, p, a, si, resb, lamB,
+                               dubuf, partial);
         });
         const double ttile = cart ? 1.0 : a.tau_src;  // the Cartesian flux has no tau factors
         for (int d = 0; d < ndir; d++) {
