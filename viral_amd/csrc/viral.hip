@@ -2478,6 +2478,7 @@ static const char* const kKernelNames[K_COUNT] = {"flux_x", "flux_y", "flux_eta"
 struct viral_ctx {
   std::vector<Event> ev;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // the adjoint's recompute stream (default priority, made on first use)
   int dev = 0;    // the device current at viral_create (the context's device)
   std::string err;
   unsigned long long* dtmp = nullptr;
@@ -3209,6 +3210,7 @@ void viral_destroy(viral_ctx* ctx)
     for (hipEvent_t e : ctx->tev[k]) hipEventDestroy(e);
   for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   delete ctx;
 }
 
@@ -4009,23 +4011,26 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   const unsigned nbi = nblocks(ni);
   // device buffers of this call; the guard frees them on every return path
   // (after draining the stream, so no kernel still reads them)
+  const bool stored = ctx->adj.every == 0;  // every stage state kept: one segment, no recompute
+  if (!stored && !ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
   struct Bufs {
-    hipStream_t s;
+    hipStream_t s, s2;
     std::vector<double*> v;
+    std::vector<hipEvent_t> e;
     ~Bufs()
     {
-      if (v.empty()) return;
+      if (s2) hipStreamSynchronize(s2);
       hipStreamSynchronize(s);
       for (double* q : v) hipFree(q);
+      for (hipEvent_t x : e) hipEventDestroy(x);
     }
-  } bufs{ctx->stream, {}};
+  } bufs{ctx->stream, stored ? nullptr : ctx->stream2, {}, {}};
   auto dalloc = [&](size_t count) -> double* {
     double* q = nullptr;
     if (hipMalloc(&q, count * sizeof(double)) != hipSuccess) return nullptr;
     bufs.v.push_back(q);
     return q;
   };
-  const bool stored = every == 0;  // every stage state kept: one segment, no recompute
   const int seglen = stored ? std::max(M, 1) : every;
   const int nsnap = stored ? 1 : std::min(every, std::max(M, 1)) * rk + 1;
   double* lamA = dalloc(NF * n);
@@ -4053,33 +4058,73 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   const bool k0b = !cart && (p.quirks & VIRAL_Q_K0_BROADCAST) && !p.lbi;  // TXYZ5 has no k0 broadcast
   const int ndir = p.lbi ? 2 : 3;
   std::vector<const double*> sp;  // the segment's stage states: sp[idx] feeds stage idx
+  const int nseg = M > 0 ? (M - 1) / seglen + 1 : 0;
+  // Checkpointed mode: segment j's stage states are recomputed on the second
+  // stream into snapshot buffer j % 2 while the reverse sweep of segment j + 1
+  // runs on the context stream; ev_rec[b] marks buffer b's recompute done,
+  // ev_rev[b] the reverse sweep that last read it.  The recompute alone uses
+  // the device table and clock (the reverse kernels take their pointers and
+  // clock as arguments), so the two streams share no written memory.
+  double* sbuf[2] = {snaps, snaps};
+  hipEvent_t ev_rec[2] = {}, ev_rev[2] = {}, ev_join = nullptr;
+  if (!stored && nseg > 0) {
+    if (nseg > 1 && !(sbuf[1] = dalloc((size_t)nsnap * NF * n)))
+      return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the adjoint");
+    for (hipEvent_t* x : {&ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1], &ev_join}) {
+      HIPCHK(hipEventCreateWithFlags(x, hipEventDisableTiming));
+      bufs.e.push_back(*x);
+    }
+  }
+  // every stage reads snapshot idx and writes snapshot idx + 1 in place:
+  // the table row points the stage's two state copies there (cells no stage
+  // writes, the corner ghosts, are never read)
+  auto recompute = [&](int seg) -> int {
+    const int m0 = seg * seglen, m1 = std::min(m0 + seglen, M);
+    double* buf = sbuf[seg & 1];
+    struct Swap {
+      viral_ctx* c;
+      hipStream_t keep;
+      ~Swap() { c->stream = keep; }
+    } swap{ctx, ctx->stream};
+    ctx->stream = ctx->stream2;  // launch_stage launches on the context's stream member
+    HIPCHK(hipMemcpyAsync(buf, ctx->adj.ckpt[seg], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+    int idx = 0;
+    for (int m = m0; m < m1; m++) {
+      hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, ctx->stream, ctx->d_clk, ctx->adj.taus[m],
+                         ctx->adj.tss[m]);
+      for (int st = 0; st < rk; st++, idx++) {
+        hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0,
+                           buf + (size_t)idx * NF * n, buf + (size_t)(idx + 1) * NF * n);
+        launch_stage(ctx, st, 0);
+      }
+    }
+    hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0, E.st[0], E.st[1]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev_rec[seg & 1], ctx->stream));
+    return VIRAL_OK;
+  };
   auto adj_timer = std::make_unique<KTimer>(ctx, K_ADJ);  // stops before the gradient's download
-  for (int seg = (M - 1) / seglen; seg >= 0 && M > 0; seg--) {
+  if (!stored && nseg > 0) {
+    // the recompute starts after the seed's upload and the timer's start
+    HIPCHK(hipEventRecord(ev_join, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->stream2, ev_join, 0));
+    if (int r = recompute(nseg - 1)) return r;
+  }
+  for (int seg = nseg - 1; seg >= 0; seg--) {
     const int m0 = seg * seglen, m1 = std::min(m0 + seglen, M);
     sp.clear();
     if (stored) {
       sp.push_back(ctx->adj.ckpt[0]);
       sp.insert(sp.end(), ctx->adj.stages.begin(), ctx->adj.stages.end());
     } else {
-      for (int q = 0; q <= (m1 - m0) * rk; q++) sp.push_back(snaps + (size_t)q * NF * n);
-      // recompute the stage states of the segment from its checkpoint
-      HIPCHK(hipMemcpyAsync(snaps, ctx->adj.ckpt[seg], fbytes, hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    // every stage reads snapshot idx and writes snapshot idx + 1 in place:
-    // the table row points the stage's two state copies there (cells no
-    // stage writes, the corner ghosts, are never read)
-    int idx = 0;
-    for (int m = m0; m < m1 && !stored; m++) {
-      hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, ctx->stream, ctx->d_clk, ctx->adj.taus[m],
-                         ctx->adj.tss[m]);
-      for (int st = 0; st < rk; st++, idx++) {
-        hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0,
-                           snaps + (size_t)idx * NF * n, snaps + (size_t)(idx + 1) * NF * n);
-        launch_stage(ctx, st, 0);
+      for (int q = 0; q <= (m1 - m0) * rk; q++) sp.push_back(sbuf[seg & 1] + (size_t)q * NF * n);
+      HIPCHK(hipStreamWaitEvent(ctx->stream, ev_rec[seg & 1], 0));
+      if (seg > 0) {
+        // buffer (seg - 1) % 2 was last read by the reverse sweep of seg + 1
+        if (seg + 1 < nseg) HIPCHK(hipStreamWaitEvent(ctx->stream2, ev_rev[(seg + 1) & 1], 0));
+        if (int r = recompute(seg - 1)) return r;
       }
     }
-    hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0, E.st[0], E.st[1]);
-    HIPCHK(hipGetLastError());
     // reverse sweep over the segment
     for (int m = m1 - 1; m >= m0; m--) {
       // vb0..vb2 are zero here: cleared once below before the first segment,
@@ -4156,6 +4201,12 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         std::swap(lamA, lamB);
       }
     }
+    if (!stored) HIPCHK(hipEventRecord(ev_rev[seg & 1], ctx->stream));
+  }
+  if (!stored && nseg > 0) {
+    // join the recompute stream (its last work precedes the last reverse sweep already)
+    HIPCHK(hipEventRecord(ev_join, ctx->stream2));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ev_join, 0));
   }
   adj_timer.reset();
   HIPCHK(hipGetLastError());
