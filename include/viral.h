@@ -406,10 +406,14 @@ int viral_autotune(viral_ctx* ctx, int ev, int reps);
  * eta_coef and dJ/d tpi_coef.  The state is left as it was.  Single-event KT
  * contexts in Milne or Cartesian coordinates, conformal or s95p EOS, with or
  * without the bulk source (not SHASTA or the FIX build: VIRAL_ERR_UNSUPPORTED).
- * viral_adjoint_end frees the recording.  With checkpoints, viral_adjoint
- * recomputes segment j on a second stream of the context while it reverses
+ * viral_adjoint_end frees the recording.  With checkpoints the recording
+ * also keeps the stage states of the current segment (the steps since the
+ * last checkpoint) in a ring of every * RK stages state copies, so the last
+ * segment is reversed without a recompute; viral_adjoint recomputes each
+ * earlier segment j on a second stream of the context while it reverses
  * segment j + 1, into two device buffers of (every * RK stages + 1) state
- * copies that it allocates and frees itself.
+ * copies that it allocates and frees itself.  The recorded steps do not
+ * replay a hipGraph.
  *
  * every == 0 keeps every stage's output state instead of checkpoints (the
  * stages write straight into the recording; no recompute in viral_adjoint):

@@ -190,6 +190,36 @@ def test_stored_stage_states_match_recompute(rk, fast, variant):
     h.adjoint_end()
 
 
+@pytest.mark.parametrize("every,calls", [(3, (1, 3, 3)), (2, (4, 3)), (7, (2, 3, 2)), (1, (3, 2)), (4, (4,))])
+def test_checkpoint_recording_across_calls(every, calls):
+    """With checkpoints the recording keeps the current segment's stage
+    states in a ring, and the adjoint reverses the last segment from it: over
+    steps recorded in several viral_steps calls, with segment ends inside and
+    at the end of calls, the forward state (every cell) equals the unrecorded
+    run's, and two adjoint calls on one recording both give the stored-stage
+    gradient bit for bit"""
+    d, st = small_case(rk=2, fast=1)
+    seed = np.random.default_rng(8).standard_normal(st.shape)
+    M = sum(calls)
+    print("case adjoint ring every=%d calls=%s" % (every, calls), flush=True)
+    plain = run_gpu(d, st, M)
+    _, g0, p0 = run_gpu(d, st, M, 0, seed)
+    h = capi.Hydro(d)
+    h.upload(st)
+    h.adjoint_begin(every)
+    t = TAU
+    for c in calls:
+        assert h.steps(t, TS, c) == (OK, c)
+        for _ in range(c):
+            t = t + TS
+    assert np.array_equal(h.download().view(np.int64), plain.view(np.int64))
+    for _ in range(2):
+        g, pg = h.adjoint(seed)
+        assert np.array_equal(h.download().view(np.int64), plain.view(np.int64))
+        assert np.array_equal(g, g0) and np.array_equal(pg, p0)
+    h.adjoint_end()
+
+
 def test_fast_mode_adjoint_matches_fd():
     d, st = small_case(fast=1)
     rng = np.random.default_rng(5)

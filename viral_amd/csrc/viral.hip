@@ -2515,13 +2515,16 @@ struct viral_ctx {
   // adjoint recording: the (tau, dtau) of every step since viral_adjoint_begin
   // and device copies of the state every `every` steps
   // (every == 0: no checkpoints past the first; each stage's output state is
-  // kept instead, in one slab per viral_steps call)
+  // kept instead, in one slab per viral_steps call; every > 0: the stage
+  // states of the current segment, the steps since the last checkpoint, are
+  // kept in a ring of every * rk state copies)
   struct {
     bool on = false;
     int every = 0;
     std::vector<double> taus, tss;
     std::vector<double*> ckpt;
     std::vector<double*> slabs, stages;
+    double* ring = nullptr;
   } adj;
 };
 
@@ -3182,6 +3185,8 @@ void viral_destroy(viral_ctx* ctx)
   // the adjoint recording: checkpoints and, when every == 0, the stage-state slabs
   for (double* c : ctx->adj.ckpt) hipFree(c);
   for (double* c : ctx->adj.slabs) hipFree(c);
+  if (ctx->adj.ring) hipFree(ctx->adj.ring);
+  ctx->adj.ring = nullptr;
   ctx->adj.ckpt.clear();
   ctx->adj.slabs.clear();
   ctx->adj.stages.clear();
@@ -3292,10 +3297,28 @@ int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
       return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the recorded stage states");
     ctx->adj.slabs.push_back(slab);
   }
-  const bool graph = ctx->cfg.graph && !ctx->timing && !store;
+  // adjoint recording with checkpoints: the stages of the current segment
+  // write straight into the ring of stage states (stage q of the segment
+  // reads q - 1, its first stage the segment's checkpoint), so viral_adjoint
+  // reverses the last segment without recomputing it; a checkpoint copies the
+  // segment's last stage state, and the last one is copied back into the
+  // state at the end
+  const bool ringrec = ctx->adj.on && ctx->adj.every > 0;
+  const size_t rslot = ringrec ? (size_t)NF * ctx->ev[0].g.n : 0;
+  const bool graph = ctx->cfg.graph && !ctx->timing && !store && !ringrec;
   double trec = tau;
   for (int s = 0; s < nsteps; s++) {
-    if (store) {
+    if (ringrec) {
+      const size_t q0 = (ctx->adj.taus.size() % ctx->adj.every) * ctx->rk;
+      for (int st = 0; st < ctx->rk; st++) {
+        const size_t q = q0 + st;
+        double* in = q == 0 ? ctx->adj.ckpt.back() : ctx->adj.ring + (q - 1) * rslot;
+        hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0, in,
+                           ctx->adj.ring + q * rslot);
+        launch_stage(ctx, st, 0);
+      }
+      hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, ctx->stream, ctx->d_clk);
+    } else if (store) {
       for (int st = 0; st < ctx->rk; st++) {
         // the call's first stage reads the state itself (equal, on every cell a
         // stage reads, to the last recorded stage state or the first checkpoint)
@@ -3316,16 +3339,23 @@ int viral_steps(viral_ctx* ctx, double tau, double ts, int nsteps, int* done)
     if (ctx->adj.on) {
       ctx->adj.taus.push_back(trec);
       ctx->adj.tss.push_back(ts);
-      if (!store && ctx->adj.taus.size() % ctx->adj.every == 0) {
-        Event& E = ctx->ev[0];
-        const size_t bytes = (size_t)NF * E.g.n * sizeof(double);
+      if (ringrec && ctx->adj.taus.size() % ctx->adj.every == 0) {
+        const size_t bytes = rslot * sizeof(double);
         double* c = nullptr;
         HIPCHK(hipMalloc(&c, bytes));
         ctx->adj.ckpt.push_back(c);
-        HIPCHK(hipMemcpyAsync(c, E.st[ctx->cur], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(c, ctx->adj.ring + ((size_t)ctx->adj.every * ctx->rk - 1) * rslot, bytes,
+                              hipMemcpyDeviceToDevice, ctx->stream));
       }
     }
     trec = trec + ts;
+  }
+  if (ringrec) {
+    Event& E = ctx->ev[0];
+    const size_t last = ((ctx->adj.taus.size() - 1) % ctx->adj.every) * ctx->rk + ctx->rk - 1;
+    hipLaunchKernelGGL(k_set_state_ptrs, dim3(1), dim3(64), 0, ctx->stream, ctx->d_tab, 0, E.st[0], E.st[1]);
+    hipLaunchKernelGGL(k_copy_written, dim3(nblocks((long)NF * E.g.n)), dim3(BLK), 0, ctx->stream, E.g, E.p.lbi,
+                       ctx->adj.ring + last * rslot, E.st[ctx->cur]);
   }
   if (store) {
     Event& E = ctx->ev[0];
@@ -3962,6 +3992,8 @@ int viral_adjoint_end(viral_ctx* ctx)
   HIPCHK(hipStreamSynchronize(ctx->stream));
   for (double* c : ctx->adj.ckpt) hipFree(c);
   for (double* c : ctx->adj.slabs) hipFree(c);
+  if (ctx->adj.ring) hipFree(ctx->adj.ring);
+  ctx->adj.ring = nullptr;
   ctx->adj.ckpt.clear();
   ctx->adj.slabs.clear();
   ctx->adj.stages.clear();
@@ -3989,6 +4021,11 @@ int viral_adjoint_begin(viral_ctx* ctx, int every)
   ctx->adj.ckpt.push_back(c);
   HIPCHK(hipMemcpyAsync(c, E.st[ctx->cur], bytes, hipMemcpyDeviceToDevice, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (every > 0 && hipMalloc(&ctx->adj.ring, (size_t)every * ctx->rk * bytes) != hipSuccess) {
+    ctx->adj.ring = nullptr;
+    viral_adjoint_end(ctx);
+    return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the recorded segment's stage states");
+  }
   ctx->adj.every = every;
   ctx->adj.on = true;
   return VIRAL_OK;
@@ -4042,8 +4079,7 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   double* dubuf = dalloc(12 * n);
   double* partial = dalloc(2 * (size_t)nbi);
   double* saved = dalloc(NF * n);
-  double* snaps = dalloc((size_t)nsnap * NF * n);
-  if (!lamA || !lamB || !vb0 || !vb1 || !vb2 || !resb || !dubuf || !partial || !saved || !snaps)
+  if (!lamA || !lamB || !vb0 || !vb1 || !vb2 || !resb || !dubuf || !partial || !saved)
     return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the adjoint");
   for (int f = 0; f < NF; f++) {
     if (!seed[f] || !grad[f]) return fail(ctx, VIRAL_ERR_ARG, "null field pointer");
@@ -4059,17 +4095,20 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
   const int ndir = p.lbi ? 2 : 3;
   std::vector<const double*> sp;  // the segment's stage states: sp[idx] feeds stage idx
   const int nseg = M > 0 ? (M - 1) / seglen + 1 : 0;
-  // Checkpointed mode: segment j's stage states are recomputed on the second
-  // stream into snapshot buffer j % 2 while the reverse sweep of segment j + 1
-  // runs on the context stream; ev_rec[b] marks buffer b's recompute done,
-  // ev_rev[b] the reverse sweep that last read it.  The recompute alone uses
-  // the device table and clock (the reverse kernels take their pointers and
-  // clock as arguments), so the two streams share no written memory.
-  double* sbuf[2] = {snaps, snaps};
+  // Checkpointed mode: the recording's ring holds the stage states of the
+  // last segment.  Every earlier segment j is recomputed on the second stream
+  // into snapshot buffer j % 2 while the reverse sweep of segment j + 1 runs
+  // on the context stream; ev_rec[b] marks buffer b's recompute done, ev_rev[b]
+  // the reverse sweep that last read it.  The recompute alone uses the device
+  // table and clock (the reverse kernels take their pointers and clock as
+  // arguments), so the two streams share no written memory.
+  const int nrec = stored ? 0 : std::max(nseg - 1, 0);  // segments to recompute
+  double* sbuf[2] = {nullptr, nullptr};
   hipEvent_t ev_rec[2] = {}, ev_rev[2] = {}, ev_join = nullptr;
-  if (!stored && nseg > 0) {
-    if (nseg > 1 && !(sbuf[1] = dalloc((size_t)nsnap * NF * n)))
-      return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the adjoint");
+  if (nrec > 0) {
+    for (int b = 0; b < std::min(nrec, 2); b++)
+      if (!(sbuf[b] = dalloc((size_t)nsnap * NF * n)))
+        return fail(ctx, VIRAL_ERR_HIP, "out of device memory for the adjoint");
     for (hipEvent_t* x : {&ev_rec[0], &ev_rec[1], &ev_rev[0], &ev_rev[1], &ev_join}) {
       HIPCHK(hipEventCreateWithFlags(x, hipEventDisableTiming));
       bufs.e.push_back(*x);
@@ -4104,11 +4143,10 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
     return VIRAL_OK;
   };
   auto adj_timer = std::make_unique<KTimer>(ctx, K_ADJ);  // stops before the gradient's download
-  if (!stored && nseg > 0) {
+  if (nrec > 0) {
     // the recompute starts after the seed's upload and the timer's start
     HIPCHK(hipEventRecord(ev_join, ctx->stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream2, ev_join, 0));
-    if (int r = recompute(nseg - 1)) return r;
   }
   for (int seg = nseg - 1; seg >= 0; seg--) {
     const int m0 = seg * seglen, m1 = std::min(m0 + seglen, M);
@@ -4116,14 +4154,19 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
     if (stored) {
       sp.push_back(ctx->adj.ckpt[0]);
       sp.insert(sp.end(), ctx->adj.stages.begin(), ctx->adj.stages.end());
+    } else if (seg == nseg - 1) {
+      // the last segment: its checkpoint, then the ring's stage outputs
+      sp.push_back(ctx->adj.ckpt[seg]);
+      for (int q = 0; q < (m1 - m0) * rk; q++) sp.push_back(ctx->adj.ring + (size_t)q * NF * n);
     } else {
       for (int q = 0; q <= (m1 - m0) * rk; q++) sp.push_back(sbuf[seg & 1] + (size_t)q * NF * n);
       HIPCHK(hipStreamWaitEvent(ctx->stream, ev_rec[seg & 1], 0));
-      if (seg > 0) {
-        // buffer (seg - 1) % 2 was last read by the reverse sweep of seg + 1
-        if (seg + 1 < nseg) HIPCHK(hipStreamWaitEvent(ctx->stream2, ev_rev[(seg + 1) & 1], 0));
-        if (int r = recompute(seg - 1)) return r;
-      }
+    }
+    if (!stored && seg > 0) {
+      // buffer (seg - 1) % 2 was last read by the reverse sweep of seg + 1,
+      // if that segment was recomputed
+      if (seg + 1 < nseg - 1) HIPCHK(hipStreamWaitEvent(ctx->stream2, ev_rev[(seg + 1) & 1], 0));
+      if (int r = recompute(seg - 1)) return r;
     }
     // reverse sweep over the segment
     for (int m = m1 - 1; m >= m0; m--) {
@@ -4201,9 +4244,9 @@ int viral_adjoint(viral_ctx* ctx, const double* const* seed, double* const* grad
         std::swap(lamA, lamB);
       }
     }
-    if (!stored) HIPCHK(hipEventRecord(ev_rev[seg & 1], ctx->stream));
+    if (nrec > 0) HIPCHK(hipEventRecord(ev_rev[seg & 1], ctx->stream));
   }
-  if (!stored && nseg > 0) {
+  if (nrec > 0) {
     // join the recompute stream (its last work precedes the last reverse sweep already)
     HIPCHK(hipEventRecord(ev_join, ctx->stream2));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ev_join, 0));
