@@ -439,7 +439,8 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
     table (the adjoint differentiates its interpolant); "conformal" is e = 3p.
     Reports the time of the recorded forward run and of the adjoint call
     (adjoint_ms: the C call, with the seed's upload and the gradient's
-    download, 2 x 29 fields over PCIe; adjoint_device_ms: its device part,
+    download, 2 x 29 fields over PCIe from and into page-locked host
+    buffers; adjoint_device_ms: its device part,
     HIP events around the recompute and reverse kernels) with checkpoints
     every `every` steps (the adjoint recomputes the stage states between
     them), and as a labelled extra with every stage state kept during the
@@ -475,10 +476,13 @@ def bench_adjoint(steps, every=20, tau0=0.6, ts=0.002, eos="s95p", fast=1):
             fin = torch.from_numpy(h.download()).requires_grad_(True)
             eps2 = autograd.momentum_anisotropy(d, fin, tau0 + steps * ts)
             eps2.backward()
-            seed = fin.grad.numpy()
+            # seed and gradient in page-locked host buffers (filled before the timing)
+            seed = capi.pinned_empty(fin.grad.shape)
+            seed[...] = fin.grad.numpy()
+        gbuf = capi.pinned_empty(seed.shape)
         h.timers(True)
         t0 = time.perf_counter()
-        grad, pg = h.adjoint(seed)
+        grad, pg = h.adjoint(seed, out=gbuf)
         t_adj = time.perf_counter() - t0
         dev_ms = h.timer_report()["adjoint_device"][0]
         h.timers(False)

@@ -158,6 +158,34 @@ def _ptr_array(arrays):
     return arr
 
 
+class _Pinned:
+    """Owner of one viral_host_alloc block (page-locked host memory), freed
+    with the last array that views it."""
+
+    def __init__(self, nbytes):
+        self.L = load()
+        self.p = C.c_void_p()
+        r = self.L.viral_host_alloc(nbytes, C.byref(self.p))
+        if r != OK:
+            raise ViralError(r, "viral_host_alloc")
+
+    def __del__(self):
+        if self.p:
+            self.L.viral_host_free(self.p)
+            self.p = C.c_void_p()
+
+
+def pinned_empty(shape):
+    """A float64 array in page-locked host memory: host<->device copies of
+    state-sized buffers (the adjoint's seed and gradient) run at full PCIe
+    rate from it instead of through a staging copy."""
+    count = int(np.prod(shape))
+    owner = _Pinned(max(count, 1) * 8)
+    buf = (C.c_double * max(count, 1)).from_address(owner.p.value)
+    buf._owner = owner  # the ctypes buffer keeps the block alive
+    return np.frombuffer(buf, dtype=np.float64, count=count).reshape(shape)
+
+
 def tile_count(desc, phase, tp=64, tk=16):
     """Tiles (blocks' worth of work) of one event in one kernel phase."""
     return load().viral_tile_count(C.byref(desc), phase, tp, tk)
@@ -304,12 +332,18 @@ class Hydro:
         `every` steps (every = 0: keep every stage state, no recompute)."""
         self._chk(self.L.viral_adjoint_begin(self.h, every))
 
-    def adjoint(self, seed):
+    def adjoint(self, seed, out=None):
         """seed: (NFIELDS, cells) dJ/d(final state).  Returns (dJ/d(initial
-        state) as (NFIELDS, cells), [dJ/d eta_coef, dJ/d tpi_coef])."""
+        state) as (NFIELDS, cells), [dJ/d eta_coef, dJ/d tpi_coef]); the
+        state gradient goes into `out` when given (e.g. a pinned_empty array,
+        as the seed may be)."""
         seed = np.ascontiguousarray(seed, dtype=np.float64)
         assert seed.shape == (NFIELDS, self.cells(0)), seed.shape
-        grad = np.empty_like(seed)
+        if out is None:
+            grad = np.empty_like(seed)
+        else:
+            grad = out
+            assert grad.shape == seed.shape and grad.dtype == np.float64 and grad.flags.c_contiguous
         pg = np.zeros(2)
         self._chk(self.L.viral_adjoint(self.h, _ptr_array([seed[f] for f in range(NFIELDS)]),
                                        _ptr_array([grad[f] for f in range(NFIELDS)]),
