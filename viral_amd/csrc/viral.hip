@@ -476,6 +476,14 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     double* rF = (q & 1) ? rF1 : rF0;
     double* rV = (q & 1) ? rV1 : rV0;
     if (q >= 0) row_loads(q);
+    // eta adds to the partial result x and y wrote: its load is issued here,
+    // three barriers ahead of the add (each cell's partial result is written
+    // by one thread of one tile only).  The y tile loads it at the add: the
+    // early load's registers cost the y tile a wave per SIMD, which measured
+    // slower on the ensemble and the 128x128x32 grid.
+    double prev[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) prev[r] = (D == 2 && q >= 0 && own_ok[r]) ? flux[(long)q * n + xs[r]] : 0.0;
 #pragma unroll
     for (int r = 0; r < MAXR; r++) {
       const int sl = threadIdx.x + r * BLK;
@@ -548,7 +556,7 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
       const double df = Bf[sl] - Bf[sl - st1];
       const double pr = -df * (1.0 / hD);
       const long o = (long)q * n + xs[r];
-      flux[o] = D == 0 ? pr : flux[o] + pr;
+      flux[o] = D == 0 ? pr : (D == 2 ? prev[r] : flux[o]) + pr;
     }
   }
 }
