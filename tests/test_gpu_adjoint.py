@@ -301,19 +301,21 @@ def test_torch_harness_momentum_anisotropy():
             assert abs(float(g) - fd) <= BAR * abs(fd) + 1e-12
 
 
-def test_closing_a_recorded_context_frees_its_stage_states():
-    """every == 0 keeps the stage states in slabs of device memory: closing
-    the context (the autograd backward does) must free them, so repeated
+@pytest.mark.parametrize("every", [0, 3])
+def test_closing_a_recorded_context_frees_its_stage_states(every):
+    """every == 0 keeps the stage states in slabs of device memory, every > 0
+    the current segment's in a ring beside the checkpoints: closing the
+    context (the autograd backward does) must free them, so repeated
     recorded runs leave the device's free memory where it was"""
     d = make_desc(24, 22, 16)
     st = ic.random_smooth(d, seed=3, tau=TAU, pi_frac=0.02)
     seed = np.random.default_rng(8).standard_normal(st.shape)
-    print("case adjoint memory release", flush=True)
+    print("case adjoint memory release every=%d" % every, flush=True)
 
     def once():
         x = torch.tensor(st, requires_grad=True)
         fin = autograd.run(x, torch.tensor(d.eta_coef, dtype=torch.float64),
-                           torch.tensor(d.tpi_coef, dtype=torch.float64), d, TAU, TS, 6, every=0)
+                           torch.tensor(d.tpi_coef, dtype=torch.float64), d, TAU, TS, 6, every=every)
         (fin * torch.from_numpy(seed)).sum().backward()
         return x.grad
 
@@ -327,6 +329,7 @@ def test_closing_a_recorded_context_frees_its_stage_states():
     torch.cuda.synchronize()
     free1, _ = torch.cuda.mem_get_info()
     # one recording holds 6 steps x 2 stages x 29 doubles per ghosted cell
+    # (every = 3: a ring of 3 x 2 stage states and 3 checkpoints)
     slab = 6 * 2 * 29 * d.cells() * 8
     assert free0 - free1 < slab / 2, (free0, free1, slab)
 
