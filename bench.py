@@ -119,10 +119,10 @@ def kernel_bytes(name, d, fused):
 # profiles/counters_kt_s95p_glauber_200x200x64_rk3.md lists the passes.
 # The update kernel's figure is for its fused form (sources evaluated inside).
 MEASURED_TRAFFIC = {
-    "update_fused": 2.181e9,
-    "flux_x": 1.106e9,
-    "flux_y": 1.421e9,
-    "flux_eta": 1.293e9,
+    "update_fused": 1.893e9,
+    "flux_x": 1.007e9,
+    "flux_y": 1.285e9,
+    "flux_eta": 1.272e9,
     "k0_broadcast": 0.175e9,
 }
 
