@@ -476,14 +476,12 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     double* rF = (q & 1) ? rF1 : rF0;
     double* rV = (q & 1) ? rV1 : rV0;
     if (q >= 0) row_loads(q);
-    // eta adds to the partial result x and y wrote: its load is issued here,
-    // three barriers ahead of the add (each cell's partial result is written
-    // by one thread of one tile only).  The y tile loads it at the add: the
-    // early load's registers cost the y tile a wave per SIMD, which measured
-    // slower on the ensemble and the 128x128x32 grid.
+    // y and eta add to the partial result the earlier directions wrote: its
+    // load is issued here, three barriers ahead of the add (each cell's
+    // partial result is written by one thread of one tile only).
     double prev[MAXR];
 #pragma unroll
-    for (int r = 0; r < MAXR; r++) prev[r] = (D == 2 && q >= 0 && own_ok[r]) ? flux[(long)q * n + xs[r]] : 0.0;
+    for (int r = 0; r < MAXR; r++) prev[r] = (D > 0 && q >= 0 && own_ok[r]) ? flux[(long)q * n + xs[r]] : 0.0;
 #pragma unroll
     for (int r = 0; r < MAXR; r++) {
       const int sl = threadIdx.x + r * BLK;
@@ -556,13 +554,17 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
       const double df = Bf[sl] - Bf[sl - st1];
       const double pr = -df * (1.0 / hD);
       const long o = (long)q * n + xs[r];
-      flux[o] = D == 0 ? pr : (D == 2 ? prev[r] : flux[o]) + pr;
+      flux[o] = D == 0 ? pr : prev[r] + pr;
     }
   }
 }
 
 template <int D, int NT, bool PERSIST, bool CART>
-__global__ void __launch_bounds__(NT) k_flux_tile(const EvDev* __restrict__ tab, Sched s,
+// Milne y tiles of 256 threads, one tile per block: registers capped for
+// five waves per SIMD, the wave the early partial-result load would take
+// (20 bytes per lane spill; the other directions fit without a cap, and the
+// Cartesian and persistent forms keep their occupancy with the load)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(D == 1 && NT == 256 && !PERSIST && !CART ? 5 : 1))) k_flux_tile(const EvDev* __restrict__ tab, Sched s,
                                                   const Clock* __restrict__ clk, int stage, int cur)
 {
   extern __shared__ double lds[];
