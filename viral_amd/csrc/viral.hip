@@ -454,8 +454,9 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     }
   }
   // The state values row q reads: field q (T^{tau q}, pi[q-4] or PI) and, for
-  // q < 4, pi^{row q} and u^q.  (Loading them one row ahead costs a wave per
-  // SIMD in registers and is slower.)
+  // q < 4, pi^{row q} and u^q.  (Loading them at the top of the previous row,
+  // with its reconstructions still to come, costs a wave per SIMD in
+  // registers and is slower.)
   double ldA[MAXR], ldB[MAXR], ldC[MAXR];
   auto row_loads = [&](int q) {
 #pragma unroll
@@ -475,7 +476,11 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     if (CART && q >= 9 && q < 14) continue;  // block-uniform: no barrier is skipped by part of the block
     double* rF = (q & 1) ? rF1 : rF0;
     double* rV = (q & 1) ? rV1 : rV0;
-    if (q >= 0) row_loads(q);
+    // x and eta load each row's state after the previous row's
+    // reconstructions (below), so the loads wait behind that row's face
+    // fluxes and partial results; y, whose registers hold its early
+    // partial-result load, loads here
+    if (D == 1 ? q >= 0 : q == -1) row_loads(D == 1 ? q : 0);
     // y and eta add to the partial result the earlier directions wrote: its
     // load is issued here, three barriers ahead of the add (each cell's
     // partial result is written by one thread of one tile only).
@@ -522,6 +527,10 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
       }
     }
     __syncthreads();
+    if (D != 1 && q >= 0) {  // the next row's state (this row's F and V are in LDS)
+      const int qn = (CART && q == 8) ? 14 : q + 1;
+      if (qn < SV) row_loads(qn);
+    }
 #pragma unroll
     for (int r = 0; r < MAXR; r++) {
       if (!face_ok[r]) continue;
@@ -560,11 +569,11 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
 }
 
 template <int D, int NT, bool PERSIST, bool CART>
-// Milne y tiles of 256 threads, one tile per block: registers capped for
-// five waves per SIMD, the wave the early partial-result load would take
-// (20 bytes per lane spill; the other directions fit without a cap, and the
-// Cartesian and persistent forms keep their occupancy with the load)
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(D == 1 && NT == 256 && !PERSIST && !CART ? 5 : 1))) k_flux_tile(const EvDev* __restrict__ tab, Sched s,
+// Milne tiles of 256 threads, one tile per block: registers capped for five
+// waves per SIMD, the wave the early loads (the y tile's partial result, the
+// x and eta tiles' next row) would take; the y tile spills 20 bytes per
+// lane.  The Cartesian and persistent forms keep their occupancy uncapped.
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 256 && !PERSIST && !CART ? 5 : 1))) k_flux_tile(const EvDev* __restrict__ tab, Sched s,
                                                   const Clock* __restrict__ clk, int stage, int cur)
 {
   extern __shared__ double lds[];
