@@ -2346,6 +2346,18 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
         cV[t][o] = gV[o];
       }
     }
+    // the own cell's adjoints this row adds to (li of pi^{d+1,l} and T^{tau l}
+    // for l < 4, of field l otherwise), loaded before the barrier so that the
+    // load waits behind it (only this thread writes them)
+    double lL1 = 0.0, lL2 = 0.0;
+    if (o_ok) {
+      if (l < 4) {
+        if (!CART) lL1 = li[(VIRAL_F_PI0 + pidx(row, l)) * n + xw];
+        lL2 = li[(VIRAL_F_T00 + l) * n + xw];
+      } else {
+        lL1 = li[(long)l * n + xw];
+      }
+    }
     __syncthreads();
     if (o_ok) {
       double sf = 0.0, sv = 0.0;
@@ -2362,9 +2374,9 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
           else if (l == 2) pTb[2] += vb;
           else pTb[3] += vb;
         } else {
-          li[(VIRAL_F_PI0 + pidx(row, l)) * n + xw] += vb;
+          li[(VIRAL_F_PI0 + pidx(row, l)) * n + xw] = lL1 + vb;
         }
-        li[(VIRAL_F_T00 + l) * n + xw] += tau * sv;
+        li[(VIRAL_F_T00 + l) * n + xw] = lL2 + tau * sv;
         if (l == row) ppb += vb;
         udb += vb * um * ww;
         const double ubl = vb * ud * ww;
@@ -2375,7 +2387,7 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
         wb += vb * ud * um;
       } else {
         // F_l = f u^{d+1}/u^0, V_l = f for the state field f = pi[l-4] or PI
-        li[(long)l * n + xw] += sf * ud * iu0 + sv;
+        li[(long)l * n + xw] = lL1 + (sf * ud * iu0 + sv);
         udb += sf * fv * iu0;
         ub0 -= sf * fv * ud * iu0 * iu0;
       }
