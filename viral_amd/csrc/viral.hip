@@ -2398,12 +2398,19 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
   // interior centre, the adjoint dubuf of k_adj_src turned into contributions
   // to the five stencil cells, summed per own cell as above
   const bool cin = rc && line_ok && c >= lo && c < hi;
+  // u^l and the gradient adjoint of row l are loaded one row ahead
+  double nU = w_ok ? si[VIRAL_F_U0 * n + xw] : 0.0, nW = cin ? dubuf[(long)D * n + xw] : 0.0;
   for (int l = 0; l < 4; l++) {
-    sF[t] = w_ok ? si[(VIRAL_F_U0 + l) * n + xw] : 0.0;
+    const double cW = nW;
+    sF[t] = nU;
+    if (l < 3) {
+      nU = w_ok ? si[(VIRAL_F_U0 + l + 1) * n + xw] : 0.0;
+      nW = cin ? dubuf[(long)((l + 1) * 3 + D) * n + xw] : 0.0;
+    }
     __syncthreads();
     double gq[5] = {0, 0, 0, 0, 0};
     if (cin) {
-      const double w = dubuf[(long)(l * 3 + D) * n + xw] * ih;
+      const double w = cW * ih;
       if (w != 0.0) {
         double q[5], dl, dr;
         for (int o = 0; o < 5; o++) q[o] = sF[t + (o - 2) * st1];
