@@ -2425,6 +2425,15 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
       li[(VIRAL_F_U0 + l) * n + xw] += acc;
     }
   }
+  // the own cell's adjoints of En, P, PI and u, which the end adds to,
+  // loaded ahead of the face-speed phase's barriers
+  double eL[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (o_ok) {
+    eL[0] = li[VIRAL_F_EN * n + xw];
+    eL[1] = li[VIRAL_F_P * n + xw];
+    eL[2] = li[VIRAL_F_BULK * n + xw];
+    for (int m = 0; m < 4; m++) eL[3 + m] = li[(VIRAL_F_U0 + m) * n + xw];
+  }
   // the face speeds in reverse: a = max(|A_L|, |A_R|) on the branch taken;
   // abar(c+1/2) = abR(c) + abL(c+1)
   __syncthreads();
@@ -2469,13 +2478,13 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
     for (int m = 0; m < 5; m++) li[(VIRAL_F_PI0 + m) * n + xw] += peb[m];
   }
   const double k = (D == 2 && !CART) ? 1.0 / (tau * tau) : 1.0;
-  li[VIRAL_F_EN * n + xw] += wb;
-  li[VIRAL_F_P * n + xw] += ppb * k + wb;
-  li[VIRAL_F_BULK * n + xw] += -(ppb * k) - wb;
-  li[VIRAL_F_U0 * n + xw] += ub0;
-  li[(VIRAL_F_U0 + 1) * n + xw] += ub1;
-  li[(VIRAL_F_U0 + 2) * n + xw] += ub2;
-  li[(VIRAL_F_U0 + 3) * n + xw] += ub3;
+  li[VIRAL_F_EN * n + xw] = eL[0] + wb;
+  li[VIRAL_F_P * n + xw] = eL[1] + (ppb * k + wb);
+  li[VIRAL_F_BULK * n + xw] = eL[2] + (-(ppb * k) - wb);
+  li[VIRAL_F_U0 * n + xw] = eL[3] + ub0;
+  li[(VIRAL_F_U0 + 1) * n + xw] = eL[4] + ub1;
+  li[(VIRAL_F_U0 + 2) * n + xw] = eL[5] + ub2;
+  li[(VIRAL_F_U0 + 3) * n + xw] = eL[6] + ub3;
 }
 
 #pragma clang fp contract(off)
