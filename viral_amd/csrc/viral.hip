@@ -68,14 +68,28 @@ __device__ __forceinline__ bool interior(const Geom& g, int i, int j, int k)
 struct Box {
   int i0, j0, k0, ni, nj, nk;
 };
+
+// Index decodes run in 32-bit unsigned arithmetic (a 64-bit division is a
+// long software sequence; the light cell kernels spent a large part of their
+// instructions on it).  Every count decoded this way -- the cells of one
+// event, its ghost cells, lines and tiles -- is below 2^32: an event's state
+// takes 29 doubles per ghosted cell in each of its two copies, so 2^32 cells
+// would need 2 TB of HBM.
+__device__ __forceinline__ unsigned udivmod(unsigned a, unsigned b, unsigned& rem)
+{
+  const unsigned q = a / b;
+  rem = a - q * b;
+  return q;
+}
 __device__ __forceinline__ bool box_cell(const Box& b, long t, int& i, int& j, int& k)
 {
-  long nk = b.nk, nj = b.nj;
-  if (t >= (long)b.ni * nj * nk) return false;
-  k = b.k0 + (int)(t % nk);
-  t /= nk;
-  j = b.j0 + (int)(t % nj);
-  i = b.i0 + (int)(t / nj);
+  if (t >= (long)b.ni * b.nj * b.nk) return false;
+  unsigned rk, rj;
+  const unsigned q = udivmod((unsigned)t, (unsigned)b.nk, rk);
+  const unsigned qi = udivmod(q, (unsigned)b.nj, rj);
+  k = b.k0 + (int)rk;
+  j = b.j0 + (int)rj;
+  i = b.i0 + (int)qi;
   return true;
 }
 
@@ -89,15 +103,18 @@ __device__ __forceinline__ bool box_cell(const Box& b, long t, int& i, int& j, i
 constexpr int kSlabRows = 16;
 __device__ __forceinline__ bool slab_cell(const Box& b, long t, int& i, int& j, int& k)
 {
-  const long nk = b.nk, per = (long)b.ni * kSlabRows * nk;
-  if (t >= (long)b.ni * b.nj * nk) return false;
-  const int s = (int)(t / per);
-  const long r = t - s * per;
+  if (t >= (long)b.ni * b.nj * b.nk) return false;
+  const unsigned nk = (unsigned)b.nk;
+  const unsigned long long per64 = (unsigned long long)b.ni * kSlabRows * nk;  // above the box when nj < kSlabRows
+  const unsigned per = per64 > 0xffffffffull ? 0xffffffffu : (unsigned)per64;
+  unsigned r, rk, rj;
+  const int s = (int)udivmod((unsigned)t, per, r);
   const int rows = min(kSlabRows, b.nj - s * kSlabRows);
-  k = b.k0 + (int)(r % nk);
-  const long q = r / nk;
-  j = b.j0 + s * kSlabRows + (int)(q % rows);
-  i = b.i0 + (int)(q / rows);
+  const unsigned q = udivmod(r, nk, rk);
+  const unsigned qi = udivmod(q, (unsigned)rows, rj);
+  k = b.k0 + (int)rk;
+  j = b.j0 + s * kSlabRows + (int)rj;
+  i = b.i0 + (int)qi;
   return true;
 }
 
@@ -386,8 +403,9 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
   const int fpt = TP - 5;            // faces per tile
   const int cpt = fpt - 1;           // cells per tile: both of their faces are in the tile
   const int nseg = (hi - lo + cpt - 1) / cpt;
-  const long lgrp = tile / nseg;
-  const int sg = (int)(tile % nseg);
+  unsigned sgu;
+  const long lgrp = udivmod((unsigned)tile, (unsigned)nseg, sgu);
+  const int sg = (int)sgu;
   const int c0 = lo - 1 + sg * cpt;  // first face cell of this tile
   const int p0 = c0 - 2;             // cell at tile position 0
   const long n = g.n;
@@ -419,7 +437,8 @@ __device__ __forceinline__ void flux_tile_body(const Geom& g, const Params& p, d
     pos[r] = ps;
     const int cd = p0 + ps;
     if (line >= nlines || cd < 0 || cd >= GD) continue;
-    const int a = (int)(line / nb), b = (int)(line % nb);
+    unsigned bu;
+    const int a = (int)udivmod((unsigned)line, (unsigned)nb, bu), b = (int)bu;
     int i, j, k;
     if (D == 0) { i = cd; j = g.jl + a; k = g.kl + b; }
     else if (D == 1) { i = g.il + a; j = cd; k = g.kl + b; }
@@ -625,8 +644,9 @@ __device__ __forceinline__ void shasta_tile_body(const Geom& g, const Params& p,
   const int GD = D == 0 ? g.gx : D == 1 ? g.gy : g.gz;
   const int cpt = TP - 6;  // owned cells per tile
   const int nseg = (hi - lo + cpt - 1) / cpt;
-  const long lgrp = tile / nseg;
-  const int sg = (int)(tile % nseg);
+  unsigned sgu;
+  const long lgrp = udivmod((unsigned)tile, (unsigned)nseg, sgu);
+  const int sg = (int)sgu;
   const int p0 = lo - 3 + sg * cpt;  // cell at tile position 0
   const long n = g.n;
   const int milne = p.milne;
@@ -651,7 +671,8 @@ __device__ __forceinline__ void shasta_tile_body(const Geom& g, const Params& p,
     pos[r] = ps;
     const int cd = p0 + ps;
     if (line >= nlines || cd < 0 || cd >= GD) continue;
-    const int a = (int)(line / nb), b = (int)(line % nb);
+    unsigned bu;
+    const int a = (int)udivmod((unsigned)line, (unsigned)nb, bu), b = (int)bu;
     int i, j, k;
     if (D == 0) { i = cd; j = g.jl + a; k = g.kl + b; }
     else if (D == 1) { i = g.il + a; j = cd; k = g.kl + b; }
@@ -876,7 +897,7 @@ __device__ __forceinline__ void source_terms(const Geom& g, const Params& p, con
     }
     if (CART) {
       // TXYZ5 evaluates the bulk row in the same slice loop (Source.h:272,327)
-      const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)(x % g.gz) < g.kr - g.kl;
+      const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)((unsigned)x % (unsigned)g.gz) < g.kr - g.kl;
 #pragma unroll
       for (int l = 0; l < 5; l++) S[4 + l] = live ? out[cart5(l)] : 0.0;
 #pragma unroll
@@ -1344,10 +1365,11 @@ __device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, doubl
   const bool mirror = (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && p.milne && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
   long dst, src;
   if (t < cx) {
-    int k = g.kl + (int)(t % nz);
-    long r = t / nz;
-    int j = g.jl + (int)(r % ny);
-    int gg = (int)(r / ny);  // 0..2B-1
+    unsigned rk, rj;
+    const unsigned r = udivmod((unsigned)t, (unsigned)nz, rk);
+    int k = g.kl + (int)rk;
+    int gg = (int)udivmod(r, (unsigned)ny, rj);  // 0..2B-1
+    int j = g.jl + (int)rj;
     int side = gg / VIRAL_BORDER, h = gg % VIRAL_BORDER;
     int i = side == 0 ? h : g.ir + h;
     int is = side == 0 ? g.il : g.ir - 1;
@@ -1356,10 +1378,11 @@ __device__ __forceinline__ void ghost_cell(const Geom& g, const Params& p, doubl
     src = cidx(g, is, j, k);
   } else if (t < cx + cy) {
     t -= cx;
-    int k = g.kl + (int)(t % nz);
-    long r = t / nz;
-    int i = g.il + (int)(r % nx);
-    int gg = (int)(r / nx);
+    unsigned rk, ri;
+    const unsigned r = udivmod((unsigned)t, (unsigned)nz, rk);
+    int k = g.kl + (int)rk;
+    int gg = (int)udivmod(r, (unsigned)nx, ri);
+    int i = g.il + (int)ri;
     int side = gg / VIRAL_BORDER, h = gg % VIRAL_BORDER;
     int j = side == 0 ? h : g.jr + h;
     int js = side == 0 ? g.jl : g.jr - 1;
@@ -1381,8 +1404,9 @@ __global__ void __launch_bounds__(BLK) k_ghost_fill(const EvDev* __restrict__ ta
     long lb;
     const EvDev& E = tab[sched_find(s, w_, lb)];
     const long nb = (ghost_count(E.g, E.p.lbi) + BLK - 1) / BLK;
-    const int q = (int)(lb / nb);
-    ghost_cell(E.g, E.p, E.st[cur], q, (lb % nb) * BLK + threadIdx.x);
+    unsigned rb;
+    const int q = (int)udivmod((unsigned)lb, (unsigned)nb, rb);
+    ghost_cell(E.g, E.p, E.st[cur], q, (long)rb * BLK + threadIdx.x);
   }
 }
 
@@ -1907,10 +1931,11 @@ __global__ void __launch_bounds__(BLK) k_adj_ghost(Geom g, Params p, double* __r
   const int nb = D == 2 ? g.jr - g.jl : g.kr - g.kl;
   const long per = (long)na * nb;
   if (t >= per * 2 * 28) return;
-  const int b = (int)(t % nb);
-  const int a = (int)((t / nb) % na);
-  const int side = (int)((t / per) % 2);
-  const int q = (int)(t / (per * 2));
+  unsigned bu, au, su;
+  const unsigned tl = udivmod((unsigned)t, (unsigned)nb, bu);  // line index a + na * (side + 2 q)
+  const unsigned ts = udivmod(tl, (unsigned)na, au);
+  const int q = (int)udivmod(ts, 2u, su);
+  const int b = (int)bu, a = (int)au, side = (int)su;
   const int f = c_bfields[q];
   const bool mirror = D < 2 && (p.quirks & VIRAL_Q_GUBSER_MIRROR) != 0 && p.milne && (f == VIRAL_F_U1 || f == VIRAL_F_U2);
   const int lo = D == 0 ? g.il : D == 1 ? g.jl : g.kl, hi = D == 0 ? g.ir : D == 1 ? g.jr : g.kr;
@@ -1943,7 +1968,9 @@ __global__ void __launch_bounds__(BLK) k_adj_pass(Geom g, const double* __restri
 {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.n) return;
-  const int k = (int)(t % g.gz), j = (int)((t / g.gz) % g.gy), i = (int)(t / ((long)g.gz * g.gy));
+  unsigned ku, ju;
+  const int i = (int)udivmod(udivmod((unsigned)t, (unsigned)g.gz, ku), (unsigned)g.gy, ju);
+  const int k = (int)ku, j = (int)ju;
   const int out = (i < g.il || i >= g.ir) + (j < g.jl || j >= g.jr) + (k < g.kl || k >= g.kr);
   const long n = g.n;
   if (out == 1) {
@@ -1963,7 +1990,8 @@ __global__ void __launch_bounds__(BLK) k_adj_k0(Geom g, Params p, double tn, con
   const int lane = threadIdx.x & 63;
   const int ni = g.ir - g.il, nj = g.jr - g.jl;
   if (wv >= (long)ni * nj) return;  // whole wavefronts leave together
-  const int i = g.il + (int)(wv / nj), j = g.jl + (int)(wv % nj);
+  unsigned ju;
+  const int i = g.il + (int)udivmod((unsigned)wv, (unsigned)nj, ju), j = g.jl + (int)ju;
   const long n = g.n, x0 = cidx(g, i, j, g.k0);
   const double v0[3] = {so[VIRAL_F_VX * n + x0], so[VIRAL_F_VY * n + x0], so[VIRAL_F_VE * n + x0]};
   const double t2 = tn * tn;
@@ -2116,7 +2144,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2))) k
     for (int q = 0; q < 16; q++) dub[q / 4][q % 4] = 0;
     if (CART) {
       // TXYZ5's shear-source loop stops short of the top interior slices
-      const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)(x % g.gz) < g.kr - g.kl;
+      const bool live = !(p.quirks & VIRAL_Q_SHEAR_SLICES) || (int)((unsigned)x % (unsigned)g.gz) < g.kr - g.kl;
       source_vjp_cart(p, e, P, u, pi, live, du, Lb, eb, Pb, ub, pib, dub, ceb, ctb);
       if (BULK && live) bulk_vjp(p, 0, 1.0, e, P, u, PI, du, Lb[14], eb, Pb, PIb, ub, dub);
     } else {
@@ -2211,7 +2239,8 @@ k_adj_tile(Geom g, Params p, double tau, const double* __restrict__ si, const do
   const int na = D == 0 ? g.jr - g.jl : g.ir - g.il;
   const long ln = lgrp * TK + kk;
   const bool line_ok = act && ln < (long)na * nb;
-  const int la = line_ok ? (int)(ln / nb) : 0, lb = line_ok ? (int)(ln % nb) : 0;
+  unsigned lbu = 0;
+  const int la = line_ok ? (int)udivmod((unsigned)ln, (unsigned)nb, lbu) : 0, lb = (int)lbu;
   auto at = [&](int cd) -> long {
     if (D == 0) return cidx(g, cd, g.jl + la, g.kl + lb);
     if (D == 1) return cidx(g, g.il + la, cd, g.kl + lb);
